@@ -1,0 +1,511 @@
+// hfh_core.hip -- context, errors, vectors and BLAS-1 of the gfx950 platform.
+//
+// BLAS-1 kernels are HBM streams: 16 B per lane per load where alignment
+// allows, grid-stride over a device-independent partition so that the
+// reductions are deterministic (the same partials in the same order on every
+// run and every device).  Reductions finish in the last workgroup to arrive
+// (self-resetting counter), so one launch produces the final scalar.
+#include "hfh_internal.h"
+
+#include <cmath>
+#include <cstdarg>
+#include <cstring>
+
+namespace hfh {
+
+static thread_local char g_err[512] = "";
+
+void set_error(const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+}
+
+int dev_alloc(void **p, size_t bytes) {
+    *p = nullptr;
+    if (bytes == 0) bytes = 256;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        set_error("hipMalloc(%zu bytes): %s", bytes, hipGetErrorString(e));
+        *p = nullptr;
+        return HFH_ERR_NOMEM;
+    }
+    return HFH_OK;
+}
+
+int launch_check(const char *what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, what);
+    return HFH_OK;
+}
+
+ScopedKernelTimer::ScopedKernelTimer(hfh_ctx *c, int k) : ctx(c), kclass(k), on(c->timing.on) {
+    if (on) (void)hipEventRecord(ctx->ev0, ctx->stream);
+}
+
+ScopedKernelTimer::~ScopedKernelTimer() {
+    if (!on) return;
+    (void)hipEventRecord(ctx->ev1, ctx->stream);
+    (void)hipEventSynchronize(ctx->ev1);
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1) == hipSuccess) {
+        ctx->timing.ms[kclass] += ms;
+        ctx->timing.launches[kclass] += 1;
+    }
+}
+
+// ----------------------------------------------------------------------------
+// reduction helpers
+
+// Number of partials for a stream of n elements (device independent).
+static inline int red_blocks(int n) {
+    long long nb = (static_cast<long long>(n) + kBlock * 4 - 1) / (kBlock * 4);
+    if (nb < 1) nb = 1;
+    if (nb > kRedBlocks) nb = kRedBlocks;
+    return static_cast<int>(nb);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_dot(int n, const T *__restrict__ x,
+                                                const T *__restrict__ y, double *part,
+                                                unsigned int *counter, double *dst) {
+    __shared__ double sh[kBlock / 64];
+    const int nb = gridDim.x;
+    double s = 0;
+    for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < n;
+         i += (long long)nb * kBlock)
+        s += static_cast<double>(x[i]) * static_cast<double>(y[i]);
+    double v[1] = {block_sum(s, sh)};
+    finish_reduce<1>(v, part, kRedBlocks, blockIdx.x, counter, nb, dst);
+}
+
+// FP32 vectors reduce in FP32 products summed in FP64: the reference sums in
+// the vector's own type, so FP32 results agree to FP32 rounding.
+int dot_async(hfh_ctx *ctx, int dtype, int n, const void *x, const void *y, double *dst) {
+    const int nb = red_blocks(n);
+    if (dtype == HFH_F64)
+        hipLaunchKernelGGL(k_dot<double>, dim3(nb), dim3(kBlock), 0, ctx->stream, n,
+                           (const double *)x, (const double *)y, ctx->red, ctx->counter, dst);
+    else
+        hipLaunchKernelGGL(k_dot<float>, dim3(nb), dim3(kBlock), 0, ctx->stream, n,
+                           (const float *)x, (const float *)y, ctx->red, ctx->counter, dst);
+    return launch_check("k_dot");
+}
+
+int fetch(hfh_ctx *ctx, const double *dsrc, double *host, int count) {
+    HFH_HIP(hipMemcpyAsync(ctx->host_res, dsrc, sizeof(double) * count, hipMemcpyDeviceToHost,
+                           ctx->stream));
+    HFH_HIP(hipStreamSynchronize(ctx->stream));
+    for (int i = 0; i < count; ++i) host[i] = ctx->host_res[i];
+    return HFH_OK;
+}
+
+// ----------------------------------------------------------------------------
+// element-wise kernels
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_axpy(int n, T *__restrict__ y, const T *__restrict__ x, T a) {
+    for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock)
+        y[i] += a * x[i];
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_scaleadd(int n, T *__restrict__ y, T a, const T *__restrict__ x) {
+    for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock)
+        y[i] = a * y[i] + x[i];
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_scale(int n, T *__restrict__ y, T a) {
+    for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock)
+        y[i] *= a;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_ewmult(int n, T *__restrict__ y, const T *__restrict__ x) {
+    for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock)
+        y[i] *= x[i];
+}
+
+template <typename TO, typename TI>
+__global__ __launch_bounds__(kBlock) void k_cast(int n, TO *__restrict__ y, const TI *__restrict__ x) {
+    for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock)
+        y[i] = static_cast<TO>(x[i]);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_gather(int n, T *__restrict__ y, const T *__restrict__ x,
+                                                   const int *__restrict__ idx) {
+    for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock)
+        y[i] = x[idx[i]];
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_scatter(int n, T *__restrict__ y, const T *__restrict__ x,
+                                                    const int *__restrict__ idx) {
+    for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock)
+        y[idx[i]] = x[i];
+}
+
+static inline int ew_blocks(hfh_ctx *ctx, int n) {
+    long long nb = (static_cast<long long>(n) + kBlock - 1) / kBlock;
+    const long long cap = static_cast<long long>(ctx->num_cu > 0 ? ctx->num_cu : 256) * 16;
+    if (nb > cap) nb = cap;
+    if (nb < 1) nb = 1;
+    return static_cast<int>(nb);
+}
+
+}  // namespace hfh
+
+using namespace hfh;
+
+// ============================================================================
+// C ABI
+
+extern "C" const char *hfh_last_error(void) { return hfh::g_err; }
+extern "C" const char *hfh_version(void) { return "hfh-gfx950 0.1"; }
+
+extern "C" int hfh_device_count(int *count) {
+    HFH_CHECK(count, HFH_ERR_INVALID, "hfh_device_count: null pointer");
+    HFH_HIP(hipGetDeviceCount(count));
+    return HFH_OK;
+}
+
+extern "C" int hfh_ctx_create(int device, hfh_ctx **out) {
+    HFH_CHECK(out, HFH_ERR_INVALID, "hfh_ctx_create: null pointer");
+    *out = nullptr;
+    int count = 0;
+    HFH_HIP(hipGetDeviceCount(&count));
+    HFH_CHECK(device >= 0 && device < count, HFH_ERR_INVALID,
+              "hfh_ctx_create: device %d out of range (%d devices)", device, count);
+    HFH_HIP(hipSetDevice(device));
+    hfh_ctx *c = new hfh_ctx();
+    c->device = device;
+    hipDeviceProp_t prop;
+    HFH_HIP(hipGetDeviceProperties(&prop, device));
+    c->num_cu = prop.multiProcessorCount;
+    c->hbm = static_cast<long long>(prop.totalGlobalMem);
+    snprintf(c->name, sizeof c->name, "%s", prop.gcnArchName);
+    HFH_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    int rc = dev_alloc(reinterpret_cast<void **>(&c->red), sizeof(double) * (kRedBlocks * 8 + 256));
+    if (rc) { delete c; return rc; }
+    rc = dev_alloc(reinterpret_cast<void **>(&c->counter), sizeof(unsigned) * 64);
+    if (rc) { delete c; return rc; }
+    HFH_HIP(hipMemset(c->counter, 0, sizeof(unsigned) * 64));
+    HFH_HIP(hipMemset(c->red, 0, sizeof(double) * (kRedBlocks * 8 + 256)));
+    HFH_HIP(hipHostMalloc(reinterpret_cast<void **>(&c->host_res), sizeof(double) * 256));
+    HFH_HIP(hipEventCreate(&c->ev0));
+    HFH_HIP(hipEventCreate(&c->ev1));
+    *out = c;
+    return HFH_OK;
+}
+
+extern "C" int hfh_ctx_destroy(hfh_ctx *c) {
+    if (!c) return HFH_OK;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->red) (void)hipFree(c->red);
+    if (c->counter) (void)hipFree(c->counter);
+    if (c->host_res) (void)hipHostFree(c->host_res);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return HFH_OK;
+}
+
+extern "C" int hfh_ctx_sync(hfh_ctx *c) {
+    HFH_CHECK(c, HFH_ERR_INVALID, "hfh_ctx_sync: null context");
+    HFH_HIP(hipStreamSynchronize(c->stream));
+    return HFH_OK;
+}
+
+extern "C" int hfh_ctx_info(hfh_ctx *c, int *num_cu, long long *hbm, char *name, int len) {
+    HFH_CHECK(c, HFH_ERR_INVALID, "hfh_ctx_info: null context");
+    if (num_cu) *num_cu = c->num_cu;
+    if (hbm) *hbm = c->hbm;
+    if (name && len > 0) snprintf(name, len, "%s", c->name);
+    return HFH_OK;
+}
+
+extern "C" int hfh_ctx_set_timing(hfh_ctx *c, int on) {
+    HFH_CHECK(c, HFH_ERR_INVALID, "hfh_ctx_set_timing: null context");
+    c->timing.on = on != 0;
+    return HFH_OK;
+}
+
+extern "C" int hfh_ctx_timing(hfh_ctx *c, int k, long long *launches, double *ms) {
+    HFH_CHECK(c && k >= 0 && k < 4, HFH_ERR_INVALID, "hfh_ctx_timing: bad argument");
+    if (launches) *launches = c->timing.launches[k];
+    if (ms) *ms = c->timing.ms[k];
+    return HFH_OK;
+}
+
+extern "C" int hfh_ctx_timing_reset(hfh_ctx *c) {
+    HFH_CHECK(c, HFH_ERR_INVALID, "hfh_ctx_timing_reset: null context");
+    for (int k = 0; k < 4; ++k) {
+        c->timing.launches[k] = 0;
+        c->timing.ms[k] = 0;
+    }
+    return HFH_OK;
+}
+
+// ---------------------------------------------------------------- vectors
+
+extern "C" int hfh_vec_create(hfh_ctx *ctx, int n, int dtype, hfh_vec **out) {
+    HFH_CHECK(ctx && out, HFH_ERR_INVALID, "hfh_vec_create: null pointer");
+    HFH_CHECK(n >= 0, HFH_ERR_INVALID, "hfh_vec_create: negative size %d", n);
+    HFH_CHECK(dtype == HFH_F64 || dtype == HFH_F32, HFH_ERR_INVALID, "hfh_vec_create: bad dtype %d", dtype);
+    hfh_vec *v = new hfh_vec();
+    v->ctx = ctx;
+    v->n = n;
+    v->dtype = dtype;
+    int rc = dev_alloc(&v->d, v->elem() * static_cast<size_t>(n));
+    if (rc) { delete v; return rc; }
+    if (n > 0) {
+        hipError_t e = hipMemsetAsync(v->d, 0, v->elem() * n, ctx->stream);
+        if (e != hipSuccess) { (void)hipFree(v->d); delete v; return hip_fail(e, "hipMemsetAsync"); }
+    }
+    *out = v;
+    return HFH_OK;
+}
+
+extern "C" int hfh_vec_destroy(hfh_vec *v) {
+    if (!v) return HFH_OK;
+    (void)hipStreamSynchronize(v->ctx->stream);
+    if (v->d) (void)hipFree(v->d);
+    delete v;
+    return HFH_OK;
+}
+
+extern "C" int hfh_vec_size(const hfh_vec *v, int *n) {
+    HFH_CHECK(v && n, HFH_ERR_INVALID, "hfh_vec_size: null pointer");
+    *n = v->n;
+    return HFH_OK;
+}
+
+extern "C" int hfh_vec_dtype(const hfh_vec *v, int *dtype) {
+    HFH_CHECK(v && dtype, HFH_ERR_INVALID, "hfh_vec_dtype: null pointer");
+    *dtype = v->dtype;
+    return HFH_OK;
+}
+
+extern "C" int hfh_vec_upload(hfh_vec *v, const void *host, int off, int cnt) {
+    HFH_CHECK(v && (host || cnt == 0), HFH_ERR_INVALID, "hfh_vec_upload: null pointer");
+    HFH_CHECK(off >= 0 && cnt >= 0 && static_cast<long long>(off) + cnt <= v->n, HFH_ERR_INVALID,
+              "hfh_vec_upload: range [%d, %d) outside vector of size %d", off, off + cnt, v->n);
+    if (cnt == 0) return HFH_OK;
+    HFH_HIP(hipMemcpyAsync(static_cast<char *>(v->d) + v->elem() * off, host, v->elem() * cnt,
+                           hipMemcpyHostToDevice, v->ctx->stream));
+    HFH_HIP(hipStreamSynchronize(v->ctx->stream));
+    return HFH_OK;
+}
+
+extern "C" int hfh_vec_download(const hfh_vec *v, void *host, int off, int cnt) {
+    HFH_CHECK(v && (host || cnt == 0), HFH_ERR_INVALID, "hfh_vec_download: null pointer");
+    HFH_CHECK(off >= 0 && cnt >= 0 && static_cast<long long>(off) + cnt <= v->n, HFH_ERR_INVALID,
+              "hfh_vec_download: range [%d, %d) outside vector of size %d", off, off + cnt, v->n);
+    if (cnt == 0) return HFH_OK;
+    HFH_HIP(hipMemcpyAsync(host, static_cast<const char *>(v->d) + v->elem() * off, v->elem() * cnt,
+                           hipMemcpyDeviceToHost, v->ctx->stream));
+    HFH_HIP(hipStreamSynchronize(v->ctx->stream));
+    return HFH_OK;
+}
+
+static int check_index(const hfh_vec *v, const int *index, int count, const char *what) {
+    for (int i = 0; i < count; ++i)
+        HFH_CHECK(index[i] >= 0 && index[i] < v->n, HFH_ERR_INVALID, "%s: index[%d]=%d outside [0,%d)",
+                  what, i, index[i], v->n);
+    return HFH_OK;
+}
+
+extern "C" int hfh_vec_set_values(hfh_vec *v, const int *index, int count, const void *values) {
+    HFH_CHECK(v && count >= 0 && (count == 0 || (index && values)), HFH_ERR_INVALID,
+              "hfh_vec_set_values: bad argument");
+    if (count == 0) return HFH_OK;
+    int rc = check_index(v, index, count, "hfh_vec_set_values");
+    if (rc) return rc;
+    void *dv = nullptr, *di = nullptr;
+    if ((rc = dev_alloc(&dv, v->elem() * count))) return rc;
+    if ((rc = dev_alloc(&di, sizeof(int) * count))) { (void)hipFree(dv); return rc; }
+    hipStream_t s = v->ctx->stream;
+    HFH_HIP(hipMemcpyAsync(dv, values, v->elem() * count, hipMemcpyHostToDevice, s));
+    HFH_HIP(hipMemcpyAsync(di, index, sizeof(int) * count, hipMemcpyHostToDevice, s));
+    const int nb = ew_blocks(v->ctx, count);
+    if (v->dtype == HFH_F64)
+        hipLaunchKernelGGL(k_scatter<double>, dim3(nb), dim3(kBlock), 0, s, count, (double *)v->d, (const double *)dv, (const int *)di);
+    else
+        hipLaunchKernelGGL(k_scatter<float>, dim3(nb), dim3(kBlock), 0, s, count, (float *)v->d, (const float *)dv, (const int *)di);
+    rc = launch_check("k_scatter");
+    (void)hipStreamSynchronize(s);
+    (void)hipFree(dv);
+    (void)hipFree(di);
+    return rc;
+}
+
+extern "C" int hfh_vec_get_values(const hfh_vec *v, const int *index, int count, void *values) {
+    HFH_CHECK(v && count >= 0 && (count == 0 || (index && values)), HFH_ERR_INVALID,
+              "hfh_vec_get_values: bad argument");
+    if (count == 0) return HFH_OK;
+    int rc = check_index(v, index, count, "hfh_vec_get_values");
+    if (rc) return rc;
+    void *dv = nullptr, *di = nullptr;
+    if ((rc = dev_alloc(&dv, v->elem() * count))) return rc;
+    if ((rc = dev_alloc(&di, sizeof(int) * count))) { (void)hipFree(dv); return rc; }
+    hipStream_t s = v->ctx->stream;
+    HFH_HIP(hipMemcpyAsync(di, index, sizeof(int) * count, hipMemcpyHostToDevice, s));
+    const int nb = ew_blocks(v->ctx, count);
+    if (v->dtype == HFH_F64)
+        hipLaunchKernelGGL(k_gather<double>, dim3(nb), dim3(kBlock), 0, s, count, (double *)dv, (const double *)v->d, (const int *)di);
+    else
+        hipLaunchKernelGGL(k_gather<float>, dim3(nb), dim3(kBlock), 0, s, count, (float *)dv, (const float *)v->d, (const int *)di);
+    rc = launch_check("k_gather");
+    if (rc == HFH_OK) {
+        HFH_HIP(hipMemcpyAsync(values, dv, v->elem() * count, hipMemcpyDeviceToHost, s));
+    }
+    (void)hipStreamSynchronize(s);
+    (void)hipFree(dv);
+    (void)hipFree(di);
+    return rc;
+}
+
+extern "C" int hfh_vec_zeros(hfh_vec *v) {
+    HFH_CHECK(v, HFH_ERR_INVALID, "hfh_vec_zeros: null vector");
+    if (v->n == 0) return HFH_OK;
+    HFH_HIP(hipMemsetAsync(v->d, 0, v->elem() * v->n, v->ctx->stream));
+    return HFH_OK;
+}
+
+extern "C" int hfh_vec_copy(hfh_vec *dst, const hfh_vec *src) {
+    HFH_CHECK(dst && src, HFH_ERR_INVALID, "hfh_vec_copy: null vector");
+    HFH_CHECK(dst->n == src->n && dst->dtype == src->dtype, HFH_ERR_INVALID,
+              "hfh_vec_copy: size/dtype mismatch (%d vs %d)", dst->n, src->n);
+    if (dst == src || dst->n == 0) return HFH_OK;
+    HFH_HIP(hipMemcpyAsync(dst->d, src->d, dst->elem() * dst->n, hipMemcpyDeviceToDevice, dst->ctx->stream));
+    return HFH_OK;
+}
+
+extern "C" int hfh_vec_copy_range(hfh_vec *dst, int doff, const hfh_vec *src, int soff, int cnt) {
+    HFH_CHECK(dst && src, HFH_ERR_INVALID, "hfh_vec_copy_range: null vector");
+    HFH_CHECK(dst->dtype == src->dtype, HFH_ERR_INVALID, "hfh_vec_copy_range: dtype mismatch");
+    HFH_CHECK(doff >= 0 && soff >= 0 && cnt >= 0 && (long long)doff + cnt <= dst->n &&
+                  (long long)soff + cnt <= src->n,
+              HFH_ERR_INVALID, "hfh_vec_copy_range: range out of bounds");
+    if (cnt == 0) return HFH_OK;
+    HFH_HIP(hipMemcpyAsync(static_cast<char *>(dst->d) + dst->elem() * doff,
+                           static_cast<const char *>(src->d) + src->elem() * soff, dst->elem() * cnt,
+                           hipMemcpyDeviceToDevice, dst->ctx->stream));
+    return HFH_OK;
+}
+
+extern "C" int hfh_vec_cast(hfh_vec *dst, const hfh_vec *src) {
+    HFH_CHECK(dst && src, HFH_ERR_INVALID, "hfh_vec_cast: null vector");
+    HFH_CHECK(dst->n == src->n, HFH_ERR_INVALID, "hfh_vec_cast: size mismatch (%d vs %d)", dst->n, src->n);
+    if (dst->n == 0) return HFH_OK;
+    if (dst->dtype == src->dtype) return hfh_vec_copy(dst, src);
+    const int nb = ew_blocks(dst->ctx, dst->n);
+    hipStream_t s = dst->ctx->stream;
+    if (dst->dtype == HFH_F32)
+        hipLaunchKernelGGL((k_cast<float, double>), dim3(nb), dim3(kBlock), 0, s, dst->n, (float *)dst->d, (const double *)src->d);
+    else
+        hipLaunchKernelGGL((k_cast<double, float>), dim3(nb), dim3(kBlock), 0, s, dst->n, (double *)dst->d, (const float *)src->d);
+    return launch_check("k_cast");
+}
+
+extern "C" int hfh_vec_reorder(hfh_vec *dst, const hfh_vec *src, const int *perm) {
+    HFH_CHECK(dst && src && (perm || dst->n == 0), HFH_ERR_INVALID, "hfh_vec_reorder: null pointer");
+    HFH_CHECK(dst != src, HFH_ERR_INVALID, "hfh_vec_reorder: in-place reorder not supported");
+    HFH_CHECK(dst->n == src->n && dst->dtype == src->dtype, HFH_ERR_INVALID, "hfh_vec_reorder: mismatch");
+    if (dst->n == 0) return HFH_OK;
+    for (int i = 0; i < dst->n; ++i)
+        HFH_CHECK(perm[i] >= 0 && perm[i] < src->n, HFH_ERR_INVALID, "hfh_vec_reorder: perm[%d]=%d out of range", i, perm[i]);
+    void *di = nullptr;
+    int rc = dev_alloc(&di, sizeof(int) * dst->n);
+    if (rc) return rc;
+    hipStream_t s = dst->ctx->stream;
+    HFH_HIP(hipMemcpyAsync(di, perm, sizeof(int) * dst->n, hipMemcpyHostToDevice, s));
+    const int nb = ew_blocks(dst->ctx, dst->n);
+    if (dst->dtype == HFH_F64)
+        hipLaunchKernelGGL(k_gather<double>, dim3(nb), dim3(kBlock), 0, s, dst->n, (double *)dst->d, (const double *)src->d, (const int *)di);
+    else
+        hipLaunchKernelGGL(k_gather<float>, dim3(nb), dim3(kBlock), 0, s, dst->n, (float *)dst->d, (const float *)src->d, (const int *)di);
+    rc = launch_check("k_gather");
+    (void)hipStreamSynchronize(s);
+    (void)hipFree(di);
+    return rc;
+}
+
+// ---------------------------------------------------------------- BLAS-1
+
+static int same_shape(const hfh_vec *a, const hfh_vec *b, const char *what) {
+    HFH_CHECK(a && b, HFH_ERR_INVALID, "%s: null vector", what);
+    HFH_CHECK(a->n == b->n && a->dtype == b->dtype && a->ctx == b->ctx, HFH_ERR_INVALID,
+              "%s: vectors differ in size/dtype/context (%d vs %d)", what, a->n, b->n);
+    return HFH_OK;
+}
+
+extern "C" int hfh_dot(const hfh_vec *x, const hfh_vec *y, double *out) {
+    int rc = same_shape(x, y, "hfh_dot");
+    if (rc) return rc;
+    HFH_CHECK(out, HFH_ERR_INVALID, "hfh_dot: null output");
+    hfh_ctx *c = x->ctx;
+    if (x->n == 0) { *out = 0; return HFH_OK; }
+    double *dst = c->red + kRedBlocks * 8;
+    if ((rc = dot_async(c, x->dtype, x->n, x->d, y->d, dst))) return rc;
+    if ((rc = fetch(c, dst, out, 1))) return rc;
+    if (x->dtype == HFH_F32) *out = static_cast<float>(*out);
+    return HFH_OK;
+}
+
+extern "C" int hfh_nrm2(const hfh_vec *x, double *out) {
+    int rc = hfh_dot(x, x, out);
+    if (rc) return rc;
+    *out = x->dtype == HFH_F32 ? static_cast<double>(sqrtf(static_cast<float>(*out))) : sqrt(*out);
+    return HFH_OK;
+}
+
+extern "C" int hfh_axpy(hfh_vec *y, const hfh_vec *x, double a) {
+    int rc = same_shape(y, x, "hfh_axpy");
+    if (rc || y->n == 0) return rc;
+    const int nb = ew_blocks(y->ctx, y->n);
+    if (y->dtype == HFH_F64)
+        hipLaunchKernelGGL(k_axpy<double>, dim3(nb), dim3(kBlock), 0, y->ctx->stream, y->n, (double *)y->d, (const double *)x->d, a);
+    else
+        hipLaunchKernelGGL(k_axpy<float>, dim3(nb), dim3(kBlock), 0, y->ctx->stream, y->n, (float *)y->d, (const float *)x->d, (float)a);
+    return launch_check("k_axpy");
+}
+
+extern "C" int hfh_scaleadd(hfh_vec *y, double a, const hfh_vec *x) {
+    int rc = same_shape(y, x, "hfh_scaleadd");
+    if (rc || y->n == 0) return rc;
+    const int nb = ew_blocks(y->ctx, y->n);
+    if (y->dtype == HFH_F64)
+        hipLaunchKernelGGL(k_scaleadd<double>, dim3(nb), dim3(kBlock), 0, y->ctx->stream, y->n, (double *)y->d, a, (const double *)x->d);
+    else
+        hipLaunchKernelGGL(k_scaleadd<float>, dim3(nb), dim3(kBlock), 0, y->ctx->stream, y->n, (float *)y->d, (float)a, (const float *)x->d);
+    return launch_check("k_scaleadd");
+}
+
+extern "C" int hfh_scale(hfh_vec *y, double a) {
+    HFH_CHECK(y, HFH_ERR_INVALID, "hfh_scale: null vector");
+    if (y->n == 0) return HFH_OK;
+    const int nb = ew_blocks(y->ctx, y->n);
+    if (y->dtype == HFH_F64)
+        hipLaunchKernelGGL(k_scale<double>, dim3(nb), dim3(kBlock), 0, y->ctx->stream, y->n, (double *)y->d, a);
+    else
+        hipLaunchKernelGGL(k_scale<float>, dim3(nb), dim3(kBlock), 0, y->ctx->stream, y->n, (float *)y->d, (float)a);
+    return launch_check("k_scale");
+}
+
+extern "C" int hfh_ewmult(hfh_vec *y, const hfh_vec *x) {
+    int rc = same_shape(y, x, "hfh_ewmult");
+    if (rc || y->n == 0) return rc;
+    const int nb = ew_blocks(y->ctx, y->n);
+    if (y->dtype == HFH_F64)
+        hipLaunchKernelGGL(k_ewmult<double>, dim3(nb), dim3(kBlock), 0, y->ctx->stream, y->n, (double *)y->d, (const double *)x->d);
+    else
+        hipLaunchKernelGGL(k_ewmult<float>, dim3(nb), dim3(kBlock), 0, y->ctx->stream, y->n, (float *)y->d, (const float *)x->d);
+    return launch_check("k_ewmult");
+}

@@ -1,0 +1,208 @@
+// hfh_internal.h -- private definitions of the gfx950 HIP platform library.
+//
+// Layout in HBM (all arrays 256-byte aligned by hipMalloc):
+//   vector  : T[n] contiguous
+//   CSR     : int32 row[nrow+1], int32 col[nnz] (ascending per row), T val[nnz],
+//             int32 rblk[nrblk+1] row-block boundaries for the streaming SpMV,
+//             int32 diag[nrow] position of the diagonal entry (-1 if absent)
+//   reductions: per-context scratch of FP64 block partials + a completion counter
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/hfh.h"
+
+namespace hfh {
+
+void set_error(const char *fmt, ...);
+
+// Kernel launch geometry shared by the library.
+constexpr int kBlock = 256;          // threads per workgroup (4 wave64)
+constexpr int kSpmvTile = 1024;      // nnz staged per workgroup by the streaming SpMV
+constexpr int kRedBlocks = 1024;     // max partials per reduction
+constexpr int kNumXcd = 8;           // MI355X: 8 accelerator complex dies
+
+struct Timing {
+    bool on = false;
+    long long launches[4] = {0, 0, 0, 0};
+    double ms[4] = {0, 0, 0, 0};
+};
+
+}  // namespace hfh
+
+struct hfh_ctx {
+    int device = 0;
+    int num_cu = 0;
+    long long hbm = 0;
+    char name[128] = {0};
+    hipStream_t stream = nullptr;
+    // reduction scratch: partials[kRedBlocks * 4] doubles, counters, results
+    double *red = nullptr;            // device
+    unsigned int *counter = nullptr;  // device, self-resetting completion counters
+    double *host_res = nullptr;       // pinned host mirror for results
+    hfh::Timing timing;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+struct hfh_vec {
+    hfh_ctx *ctx = nullptr;
+    int n = 0;
+    int dtype = HFH_F64;
+    void *d = nullptr;
+    size_t elem() const { return dtype == HFH_F64 ? 8 : 4; }
+};
+
+struct hfh_csr {
+    hfh_ctx *ctx = nullptr;
+    int nrow = 0, ncol = 0, nnz = 0, dtype = HFH_F64;
+    int *row = nullptr, *col = nullptr;
+    void *val = nullptr;
+    int *rblk = nullptr;   // streaming row blocks
+    int nrblk = 0;
+    int *diag = nullptr;   // diagonal position per row
+    bool has_structure = false;
+    std::vector<int> h_row;  // host copy of the row pointer (for analysis)
+    size_t elem() const { return dtype == HFH_F64 ? 8 : 4; }
+};
+
+// Level schedule on the device: rows in level order, level boundaries on host.
+struct hfh_levels {
+    int nlev = 0;
+    std::vector<int> ptr;      // host, nlev+1
+    int *rows = nullptr;       // device, nrow
+};
+
+struct hfh_prec {
+    int kind = HFH_PREC_NONE;
+    hfh_ctx *ctx = nullptr;
+    const hfh_csr *A = nullptr;     // operator (SGS, MC-SGS) or LU factors (ILU0)
+    int n = 0;
+    int dtype = HFH_F64;
+    void *invd = nullptr;           // Jacobi / MC-SGS: 1/a_ii
+    void *dv = nullptr;             // MC-SGS: -a_ii / (omega (2 - omega))
+    void *sval = nullptr;           // MC-SGS: omega-scaled values (nullptr when omega == 1)
+    double omega = 1.0;
+    int ncolors = 0;
+    std::vector<int> color_off;     // host, ncolors+1
+    hfh_levels lo, up;              // SGS / ILU0 level schedules
+};
+
+namespace hfh {
+
+inline int hip_fail(hipError_t e, const char *what) {
+    set_error("%s: %s", what, hipGetErrorString(e));
+    return HFH_ERR_HIP;
+}
+
+#define HFH_HIP(call)                                          \
+    do {                                                       \
+        hipError_t e_ = (call);                                \
+        if (e_ != hipSuccess) return hfh::hip_fail(e_, #call); \
+    } while (0)
+
+#define HFH_CHECK(cond, code, ...)           \
+    do {                                     \
+        if (!(cond)) {                       \
+            hfh::set_error(__VA_ARGS__);     \
+            return (code);                   \
+        }                                    \
+    } while (0)
+
+int dev_alloc(void **p, size_t bytes);
+int launch_check(const char *what);
+
+// Reductions (hfh_blas1.hip).  Results land in ctx->red + slot (device).
+int dot_async(hfh_ctx *ctx, int dtype, int n, const void *x, const void *y, double *dst);
+int fetch(hfh_ctx *ctx, const double *dsrc, double *host, int count);
+
+// Timing helpers.
+struct ScopedKernelTimer {
+    hfh_ctx *ctx;
+    int kclass;
+    bool on;
+    ScopedKernelTimer(hfh_ctx *c, int k);
+    ~ScopedKernelTimer();
+};
+
+// SpMV launch used by the drivers (hfh_spmv.hip).  mode 0: y = A x;
+// mode 1: y += A x.  When pdot is non-null the kernel also accumulates
+// sum_i y_i * w_i into block partials and the last workgroup writes the total
+// to pdot (device).
+int spmv_launch(const hfh_csr *A, const void *x, void *y, int mode);
+int spmv_launch_ex(const hfh_csr *A, const void *x, void *y, int mode, const void *w, double *part,
+                   unsigned int *counter, double *dst, const int *done);
+
+// Preconditioner apply on raw device pointers (hfh_prec.hip).  `done`
+// (device, nullable): every kernel returns at once when *done != 0.
+int prec_apply_raw(hfh_prec *P, const void *in, void *out, const int *done = nullptr);
+
+// XCD-aware workgroup remap: consecutive logical tiles land on one XCD so its
+// L2 keeps the shared gather window.
+__device__ __forceinline__ int xcd_remap(int b, int nb) {
+    const int per = nb / kNumXcd, rem = nb % kNumXcd;
+    const int xcd = b % kNumXcd, idx = b / kNumXcd;
+    // XCDs [0, rem) own per+1 tiles, the rest own per tiles
+    const int base = xcd < rem ? xcd * (per + 1) : rem * (per + 1) + (xcd - rem) * per;
+    return base + idx;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Deterministic block sum of kBlock threads; result valid in thread 0.
+template <typename T>
+__device__ __forceinline__ T block_sum(T v, T *sh /* >= kBlock/64 */) {
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    T s = 0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < kBlock / 64; ++i) s += sh[i];
+    return s;
+}
+
+// Last-arriving workgroup sums the partials part[q*stride + b], b < nb, in a
+// fixed order (lane-strided sums, then the fixed block tree) and writes
+// dst[q]; the counter is reset for the next launch.  `v` holds this
+// workgroup's NQ sums in thread 0.  `slot` is this workgroup's partial index.
+template <int NQ>
+__device__ __forceinline__ bool finish_reduce(const double (&v)[NQ], double *part, int stride, int slot,
+                                              unsigned int *counter, int nb, double *dst) {
+    __shared__ bool last;
+    __shared__ double sh[kBlock / 64];
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) part[q * stride + slot] = v[q];
+        __threadfence();
+        const unsigned prev = atomicAdd(counter, 1u);
+        last = (prev == static_cast<unsigned>(nb - 1));
+    }
+    __syncthreads();
+    if (!last) return false;
+    __threadfence();
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        double s = 0;
+        for (int b = threadIdx.x; b < nb; b += kBlock) s += __hip_atomic_load(&part[q * stride + b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        s = block_sum(s, sh);
+        if (threadIdx.x == 0) dst[q] = s;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        __threadfence();
+        atomicExch(counter, 0u);
+    }
+    return true;
+}
+
+}  // namespace hfh

@@ -1,0 +1,792 @@
+// hfh_krylov.hip -- device-resident Krylov drivers.
+//
+// CG (linear_solver/cg.cc:163-248) is restated with its scalars kept in HBM so
+// the host never waits on a Dot.  One iteration is three (Jacobi / none) or
+// 3 + precond launches:
+//   K1  Ap = A p, fused partial sums of Ap.p; last workgroup stores pAp
+//   K2  alpha = beta/pAp; x += alpha p; r -= alpha Ap; sums of r.r (and r.z
+//       with z = r.*invD for Jacobi); last workgroup: iter++, res = sqrt(r.r),
+//       IterateControl::Check, beta update
+//   [M] z = M^-1 r, then K2b: r.z and the beta update (non-Jacobi)
+//   K3  p = beta_ratio p + z
+// Every kernel returns at once when the done flag is set, so the host
+// enqueues chunks of iterations (a captured hipGraph when possible) and reads
+// the state once per chunk.  Compulsory traffic per Jacobi iteration:
+// 12 nnz + 108 n bytes (FP64).
+#include "hfh_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace hfh {
+
+struct KState {
+    double pap, rr, rz;     // reduction results (written by last workgroups)
+    double beta, ratio;     // beta = r.z (or r.r), ratio = new/old
+    double res, res0;
+    double atol, rtol, dtol;
+    int iter, done, state, maxits;
+};
+
+__device__ __forceinline__ int check_state(int step, double v, double first, const KState *s) {
+    if (v < s->atol) return HFH_SUCCESS_ABS;
+    if (v < s->rtol * first) return HFH_SUCCESS_REL;
+    if (v > s->dtol * first) return HFH_FAIL_DIV;
+    if (step >= s->maxits) return HFH_FAIL_MAXITS;
+    return HFH_ITERATE;
+}
+
+static inline int host_check(int step, double v, double first, const hfh_control &c) {
+    if (v < c.atol) return HFH_SUCCESS_ABS;
+    if (v < c.rtol * first) return HFH_SUCCESS_REL;
+    if (v > c.dtol * first) return HFH_FAIL_DIV;
+    if (step >= c.maxits) return HFH_FAIL_MAXITS;
+    return HFH_ITERATE;
+}
+
+// MODE 0: no preconditioner (z = r), 1: Jacobi (z = r.*invd), 2: general (z
+// computed by a separate apply; K2 reduces r.r only).
+template <typename T, int MODE>
+__global__ __launch_bounds__(kBlock) void k_pcg_update(int n, T *__restrict__ x, const T *__restrict__ p,
+                                                       T *__restrict__ r, const T *__restrict__ Ap,
+                                                       const T *__restrict__ invd, double *part, int stride,
+                                                       unsigned int *counter, KState *st, double *hist) {
+    if (st->done) return;
+    __shared__ double sh[kBlock / 64];
+    const T alpha = static_cast<T>(static_cast<T>(st->beta) / static_cast<T>(st->pap));
+    double rr = 0, rz = 0;
+    for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock) {
+        x[i] += alpha * p[i];
+        const T ri = r[i] + (-alpha) * Ap[i];
+        r[i] = ri;
+        rr += static_cast<double>(ri) * static_cast<double>(ri);
+        if (MODE == 1) {
+            const T zi = ri * invd[i];
+            rz += static_cast<double>(ri) * static_cast<double>(zi);
+        }
+    }
+    double v[2];
+    v[0] = block_sum(rr, sh);
+    __syncthreads();
+    v[1] = MODE == 1 ? block_sum(rz, sh) : 0.0;
+    double tot[2];
+    __shared__ double res2[2];
+    if (!finish_reduce<2>(v, part, stride, blockIdx.x, counter, gridDim.x, res2)) return;
+    if (threadIdx.x == 0) {
+        tot[0] = res2[0];
+        tot[1] = res2[1];
+        const int it = st->iter + 1;
+        st->iter = it;
+        const T rrT = static_cast<T>(tot[0]);
+        const double res = static_cast<double>(static_cast<T>(sqrt(static_cast<double>(rrT))));
+        st->rr = tot[0];
+        st->res = res;
+        if (hist) hist[it] = res;
+        const int state = check_state(it, res, st->res0, st);
+        st->state = state;
+        if (state != HFH_ITERATE) {
+            st->done = 1;
+        } else if (MODE == 0) {
+            const T beta = static_cast<T>(st->beta);
+            st->ratio = static_cast<double>(rrT / beta);
+            st->beta = static_cast<double>(rrT);
+        } else if (MODE == 1) {
+            const T gamma = static_cast<T>(tot[1]);
+            const T beta = static_cast<T>(st->beta);
+            st->ratio = static_cast<double>(gamma / beta);
+            st->beta = static_cast<double>(gamma);
+        }
+    }
+}
+
+// r.z after a general preconditioner apply; last workgroup updates beta.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_pcg_rz(int n, const T *__restrict__ r, const T *__restrict__ z,
+                                                   double *part, int stride, unsigned int *counter, KState *st) {
+    if (st->done) return;
+    __shared__ double sh[kBlock / 64];
+    double rz = 0;
+    for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock)
+        rz += static_cast<double>(r[i]) * static_cast<double>(z[i]);
+    const double v[1] = {block_sum(rz, sh)};
+    __shared__ double res1[1];
+    if (!finish_reduce<1>(v, part, stride, blockIdx.x, counter, gridDim.x, res1)) return;
+    if (threadIdx.x == 0) {
+        const T gamma = static_cast<T>(res1[0]);
+        const T beta = static_cast<T>(st->beta);
+        st->ratio = static_cast<double>(gamma / beta);
+        st->beta = static_cast<double>(gamma);
+    }
+}
+
+// p = ratio * p + z  (ScaleAdd, cg.cc:233).  MODE as in k_pcg_update.
+template <typename T, int MODE>
+__global__ __launch_bounds__(kBlock) void k_pcg_p(int n, T *__restrict__ p, const T *__restrict__ r,
+                                                  const T *__restrict__ z, const T *__restrict__ invd,
+                                                  const KState *st) {
+    if (st->done) return;
+    const T ratio = static_cast<T>(st->ratio);
+    for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock) {
+        T zi;
+        if (MODE == 0) zi = r[i];
+        else if (MODE == 1) zi = r[i] * invd[i];
+        else zi = z[i];
+        p[i] = ratio * p[i] + zi;
+    }
+}
+
+// Workspace of one solve.
+struct Work {
+    hfh_ctx *ctx = nullptr;
+    std::vector<void *> bufs;
+    ~Work() {
+        for (void *b : bufs)
+            if (b) (void)hipFree(b);
+    }
+    int alloc(void **p, size_t bytes) {
+        int rc = dev_alloc(p, bytes);
+        if (!rc) bufs.push_back(*p);
+        return rc;
+    }
+};
+
+static int stream_blocks(hfh_ctx *ctx, int n) {
+    long long nb = (static_cast<long long>(n) + kBlock * 4 - 1) / (kBlock * 4);
+    const long long cap = std::min<long long>(kRedBlocks, std::max(ctx->num_cu, 1) * 4);
+    nb = std::max<long long>(1, std::min(nb, cap));
+    return static_cast<int>(nb);
+}
+
+template <typename T>
+struct PcgRun {
+    const hfh_csr *A = nullptr;
+    hfh_prec *P = nullptr;
+    int mode = 0;  // 0 none, 1 jacobi, 2 general
+    int n = 0;
+    T *x = nullptr, *r = nullptr, *p = nullptr, *z = nullptr, *Ap = nullptr;
+    const T *invd = nullptr;
+    double *part = nullptr;
+    int stride = 0;
+    unsigned int *cnt = nullptr;  // 3 counters
+    KState *st = nullptr;
+    double *hist = nullptr;
+    int nb = 1;
+
+    int spmv() { return spmv_launch_ex(A, p, Ap, 0, p, part, cnt + 0, &st->pap, &st->done); }
+
+    int tail() {
+        hipStream_t s = A->ctx->stream;
+        int rc;
+        if (mode == 0)
+            hipLaunchKernelGGL((k_pcg_update<T, 0>), dim3(nb), dim3(kBlock), 0, s, n, x, p, r, Ap, invd, part, stride, cnt + 1, st, hist);
+        else if (mode == 1)
+            hipLaunchKernelGGL((k_pcg_update<T, 1>), dim3(nb), dim3(kBlock), 0, s, n, x, p, r, Ap, invd, part, stride, cnt + 1, st, hist);
+        else
+            hipLaunchKernelGGL((k_pcg_update<T, 2>), dim3(nb), dim3(kBlock), 0, s, n, x, p, r, Ap, invd, part, stride, cnt + 1, st, hist);
+        if ((rc = launch_check("k_pcg_update"))) return rc;
+        if (mode == 2) {
+            if ((rc = prec_apply_raw(P, r, z, &st->done))) return rc;
+            hipLaunchKernelGGL(k_pcg_rz<T>, dim3(nb), dim3(kBlock), 0, s, n, r, z, part, stride, cnt + 2, st);
+            if ((rc = launch_check("k_pcg_rz"))) return rc;
+        }
+        if (mode == 0)
+            hipLaunchKernelGGL((k_pcg_p<T, 0>), dim3(nb), dim3(kBlock), 0, s, n, p, r, z, invd, st);
+        else if (mode == 1)
+            hipLaunchKernelGGL((k_pcg_p<T, 1>), dim3(nb), dim3(kBlock), 0, s, n, p, r, z, invd, st);
+        else
+            hipLaunchKernelGGL((k_pcg_p<T, 2>), dim3(nb), dim3(kBlock), 0, s, n, p, r, z, invd, st);
+        return launch_check("k_pcg_p");
+    }
+
+    int iteration() {
+        int rc = spmv();
+        return rc ? rc : tail();
+    }
+};
+
+// Runs the chunked iteration loop until the device state reports done.  With
+// timing on, each SpMV is bracketed by HIP events on the context stream and
+// the launches that did work are added to timing class 0.
+template <typename T>
+static int pcg_loop(PcgRun<T> &R, int maxits, bool use_graph) {
+    hfh_ctx *c = R.A->ctx;
+    hipStream_t s = c->stream;
+    const bool timing = c->timing.on;
+    const int chunk = std::max(1, std::min(16, maxits));
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr;
+    int rc = HFH_OK;
+    if (use_graph && !timing) {
+        if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+            for (int k = 0; k < chunk && !rc; ++k) rc = R.iteration();
+            const hipError_t e2 = hipStreamEndCapture(s, &graph);
+            if (rc || e2 != hipSuccess || hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+                (void)hipGetLastError();
+                if (exec) (void)hipGraphExecDestroy(exec);
+                if (graph) (void)hipGraphDestroy(graph);
+                exec = nullptr;
+                graph = nullptr;
+                rc = HFH_OK;  // fall back to direct launches
+            }
+        } else {
+            (void)hipGetLastError();
+        }
+    }
+    int *hflag = reinterpret_cast<int *>(c->host_res);
+    std::vector<hipEvent_t> evs;
+    long long launched = 0;
+    for (;;) {
+        if (exec) {
+            HFH_HIP(hipGraphLaunch(exec, s));
+        } else {
+            for (int k = 0; k < chunk && !rc; ++k) {
+                if (timing) {
+                    hipEvent_t a, b;
+                    HFH_HIP(hipEventCreate(&a));
+                    HFH_HIP(hipEventCreate(&b));
+                    evs.push_back(a);
+                    evs.push_back(b);
+                    HFH_HIP(hipEventRecord(a, s));
+                    rc = R.spmv();
+                    HFH_HIP(hipEventRecord(b, s));
+                    if (!rc) rc = R.tail();
+                } else {
+                    rc = R.iteration();
+                }
+            }
+        }
+        if (rc) break;
+        launched += chunk;
+        HFH_HIP(hipMemcpyAsync(hflag, &R.st->done, sizeof(int), hipMemcpyDeviceToHost, s));
+        HFH_HIP(hipStreamSynchronize(s));
+        if (*hflag) break;
+        if (launched > static_cast<long long>(maxits) + chunk) break;  // the device test stops at maxits
+    }
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (!evs.empty()) {
+        int it = 0;
+        (void)hipStreamSynchronize(s);
+        (void)hipMemcpy(&it, &R.st->iter, sizeof(int), hipMemcpyDeviceToHost);
+        for (size_t k = 0; k + 1 < evs.size(); k += 2) {
+            float ms = 0;
+            if (static_cast<long long>(k / 2) < it && hipEventElapsedTime(&ms, evs[k], evs[k + 1]) == hipSuccess) {
+                c->timing.ms[0] += ms;
+                c->timing.launches[0] += 1;
+            }
+            (void)hipEventDestroy(evs[k]);
+            (void)hipEventDestroy(evs[k + 1]);
+        }
+    }
+    return rc;
+}
+
+// Workspace + state of a PCG solve of size n in type T.
+template <typename T>
+struct PcgWork {
+    Work w;
+    PcgRun<T> R;
+    KState *st = nullptr;
+    double *hist = nullptr;
+    int maxits = 0;
+
+    int init(const hfh_csr *A, hfh_prec *P, int maxits_) {
+        hfh_ctx *c = A->ctx;
+        const int n = A->nrow;
+        maxits = maxits_;
+        R.A = A;
+        R.P = P;
+        R.n = n;
+        R.mode = (!P || P->kind == HFH_PREC_NONE) ? 0 : (P->kind == HFH_PREC_JACOBI ? 1 : 2);
+        R.invd = (P && P->kind == HFH_PREC_JACOBI) ? static_cast<const T *>(P->invd) : nullptr;
+        R.nb = stream_blocks(c, n);
+        R.stride = std::max(A->nrblk, kRedBlocks);
+        int rc;
+        const size_t vb = sizeof(T) * static_cast<size_t>(std::max(n, 1));
+        if ((rc = w.alloc(reinterpret_cast<void **>(&R.r), vb)) || (rc = w.alloc(reinterpret_cast<void **>(&R.p), vb)) ||
+            (rc = w.alloc(reinterpret_cast<void **>(&R.z), vb)) || (rc = w.alloc(reinterpret_cast<void **>(&R.Ap), vb)) ||
+            (rc = w.alloc(reinterpret_cast<void **>(&R.part), sizeof(double) * 2 * R.stride)) ||
+            (rc = w.alloc(reinterpret_cast<void **>(&R.cnt), sizeof(unsigned) * 4)) ||
+            (rc = w.alloc(reinterpret_cast<void **>(&st), sizeof(KState))) ||
+            (rc = w.alloc(reinterpret_cast<void **>(&hist), sizeof(double) * (static_cast<size_t>(maxits) + 2))))
+            return rc;
+        HFH_HIP(hipMemsetAsync(R.cnt, 0, sizeof(unsigned) * 4, c->stream));
+        R.st = st;
+        R.hist = hist;
+        return HFH_OK;
+    }
+
+    // Solves A x = b from the given x (device pointers); fills info.
+    int solve(const T *b, T *x, const hfh_control &ctl, hfh_solve_info *info, double *hhist, bool use_graph) {
+        const hfh_csr *A = R.A;
+        hfh_ctx *c = A->ctx;
+        hipStream_t s = c->stream;
+        const int n = R.n;
+        R.x = x;
+        hfh_vec vr{c, n, A->dtype, R.r}, vb{c, n, A->dtype, const_cast<T *>(b)}, vz{c, n, A->dtype, R.z},
+            vp{c, n, A->dtype, R.p};
+        hipEvent_t e0, e1;
+        HFH_HIP(hipEventCreate(&e0));
+        HFH_HIP(hipEventCreate(&e1));
+        HFH_HIP(hipEventRecord(e0, s));
+        int rc = spmv_launch_ex(A, x, R.r, 0, nullptr, nullptr, nullptr, nullptr, nullptr);
+        if (!rc) rc = hfh_scaleadd(&vr, -1.0, &vb);
+        double rr = 0;
+        if (!rc) rc = hfh_dot(&vr, &vr, &rr);
+        if (rc) return rc;
+        const T rrT = static_cast<T>(rr);
+        const double res0 = static_cast<double>(static_cast<T>(sqrt(static_cast<double>(rrT))));
+        const int state0 = host_check(0, res0, res0, ctl);
+        double beta = rr;
+        if (R.mode == 0) {
+            HFH_HIP(hipMemcpyAsync(R.p, R.r, sizeof(T) * n, hipMemcpyDeviceToDevice, s));
+        } else {
+            HFH_HIP(hipMemsetAsync(R.z, 0, sizeof(T) * n, s));
+            if ((rc = prec_apply_raw(R.P, R.r, R.z))) return rc;
+            HFH_HIP(hipMemcpyAsync(R.p, R.z, sizeof(T) * n, hipMemcpyDeviceToDevice, s));
+            if ((rc = hfh_dot(&vr, &vz, &beta))) return rc;
+        }
+        (void)vp;
+        KState h{};
+        h.beta = static_cast<double>(static_cast<T>(beta));
+        h.res = res0;
+        h.res0 = res0;
+        h.atol = ctl.atol;
+        h.rtol = ctl.rtol;
+        h.dtol = ctl.dtol;
+        h.maxits = ctl.maxits;
+        h.state = state0;
+        h.done = state0 != HFH_ITERATE;
+        HFH_HIP(hipMemcpyAsync(st, &h, sizeof(KState), hipMemcpyHostToDevice, s));
+        HFH_HIP(hipMemcpyAsync(hist, &res0, sizeof(double), hipMemcpyHostToDevice, s));
+        if (!h.done && (rc = pcg_loop(R, ctl.maxits, use_graph))) return rc;
+        HFH_HIP(hipEventRecord(e1, s));
+        HFH_HIP(hipMemcpyAsync(&h, st, sizeof(KState), hipMemcpyDeviceToHost, s));
+        HFH_HIP(hipStreamSynchronize(s));
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (hhist && h.iter >= 0)
+            HFH_HIP(hipMemcpy(hhist, hist, sizeof(double) * (h.iter + 1), hipMemcpyDeviceToHost));
+        if (info) {
+            info->iters = h.iter;
+            info->state = h.state;
+            info->res = h.res;
+            info->res0 = h.res0;
+            info->time_ms = ms;
+        }
+        return HFH_OK;
+    }
+};
+
+static int solve_status(int state) {
+    if (state == HFH_SUCCESS_ABS || state == HFH_SUCCESS_REL) return HFH_OK;
+    set_error("solver stopped without convergence (state %d)", state);
+    return HFH_ERR_NOT_CONVERGED;
+}
+
+static int check_solve_args(const hfh_csr *A, const hfh_prec *P, const hfh_vec *b, const hfh_vec *x,
+                            const hfh_control *ctl, const char *what) {
+    HFH_CHECK(A && b && x && ctl, HFH_ERR_INVALID, "%s: null argument", what);
+    HFH_CHECK(A->has_structure && A->nrow == A->ncol, HFH_ERR_INVALID, "%s: matrix must be square with a structure", what);
+    HFH_CHECK(b->n == A->nrow && x->n == A->nrow && b->dtype == A->dtype && x->dtype == A->dtype, HFH_ERR_INVALID,
+              "%s: vector size/dtype mismatch", what);
+    HFH_CHECK(!P || (P->n == A->nrow && P->dtype == A->dtype), HFH_ERR_INVALID, "%s: preconditioner mismatch", what);
+    HFH_CHECK(ctl->maxits > 0 && ctl->atol >= 0 && ctl->rtol >= 0 && ctl->dtol > 1, HFH_ERR_INVALID,
+              "%s: control needs maxits > 0, atol >= 0, rtol >= 0, dtol > 1", what);
+    HFH_CHECK(x->d != b->d, HFH_ERR_INVALID, "%s: x and b must not alias", what);
+    return HFH_OK;
+}
+
+}  // namespace hfh
+
+using namespace hfh;
+
+extern "C" int hfh_pcg_solve(const hfh_csr *A, hfh_prec *P, const hfh_vec *b, hfh_vec *x, const hfh_control *ctl,
+                             hfh_solve_info *info, double *hist) {
+    int rc = check_solve_args(A, P, b, x, ctl, "hfh_pcg_solve");
+    if (rc) return rc;
+    hfh_solve_info inf{};
+    if (A->dtype == HFH_F64) {
+        PcgWork<double> W;
+        if ((rc = W.init(A, P, ctl->maxits))) return rc;
+        if ((rc = W.solve((const double *)b->d, (double *)x->d, *ctl, &inf, hist, true))) return rc;
+    } else {
+        PcgWork<float> W;
+        if ((rc = W.init(A, P, ctl->maxits))) return rc;
+        if ((rc = W.solve((const float *)b->d, (float *)x->d, *ctl, &inf, hist, true))) return rc;
+    }
+    if (info) *info = inf;
+    return solve_status(inf.state);
+}
+
+// ============================================================================
+// GMRES / FGMRES (gmres.cc:285-510, fgmres.cc:75-262)
+//
+// Device: the Krylov basis, the preconditioned vectors, A z and the modified
+// Gram-Schmidt sweep.  MGS step i fuses the previous axpy with the next dot
+// (w -= h_{i-1} v_{i-1}; h_i = w.v_i), reading h from HBM, so the host issues
+// j+2 launches per step and reads back one Hessenberg column.  Host: the
+// Givens rotations, the residual estimate |g_{j+1}| and the back
+// substitution, in the reference's order.
+
+namespace hfh {
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_mgs(int n, T *__restrict__ w, const T *__restrict__ vprev,
+                                                const T *__restrict__ vcur, const double *hprev, double *part,
+                                                int stride, unsigned int *counter, double *hout) {
+    __shared__ double sh[kBlock / 64];
+    const T h = vprev ? static_cast<T>(-1) * static_cast<T>(*hprev) : static_cast<T>(0);
+    double s = 0;
+    for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock) {
+        T wi = w[i];
+        if (vprev) {
+            wi += h * vprev[i];
+            w[i] = wi;
+        }
+        const T other = vcur ? vcur[i] : wi;
+        s += static_cast<double>(wi) * static_cast<double>(other);
+    }
+    const double v[1] = {block_sum(s, sh)};
+    __shared__ double r1[1];
+    if (!finish_reduce<1>(v, part, stride, blockIdx.x, counter, gridDim.x, r1)) return;
+    if (threadIdx.x == 0) {
+        const T d = static_cast<T>(r1[0]);
+        *hout = vcur ? static_cast<double>(d) : static_cast<double>(static_cast<T>(sqrt(static_cast<double>(d))));
+    }
+}
+
+// v = w * (1 / h)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_scale_into(int n, T *__restrict__ v, const T *__restrict__ w,
+                                                       const double *h) {
+    const T s = static_cast<T>(1) / static_cast<T>(*h);
+    for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock)
+        v[i] = w[i] * s;
+}
+
+constexpr int kMaxBasis = 256;
+
+struct Coeffs {
+    double y[kMaxBasis];
+};
+
+// x += sum_{j<=k} y_j V_j, added one basis vector after the other
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_multi_axpy(int n, T *__restrict__ x, T *const *__restrict__ V, int k1,
+                                                       Coeffs c) {
+    for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock) {
+        T xi = x[i];
+        for (int j = 0; j < k1; ++j) xi += static_cast<T>(c.y[j]) * V[j][i];
+        x[i] = xi;
+    }
+}
+
+// x += 1 * w  (Axpy(w, 1), gmres.cc:398)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_add_into(int n, T *__restrict__ x, const T *__restrict__ w) {
+    for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock)
+        x[i] += static_cast<T>(1) * w[i];
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_neg_add(int n, T *__restrict__ y, const T *__restrict__ b) {
+    // y = -1 * y + b  (ScaleAdd(b, -1))
+    for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock)
+        y[i] = static_cast<T>(-1) * y[i] + b[i];
+}
+
+template <typename T>
+static int gmres_impl(const hfh_csr *A, hfh_prec *P, int size_basis, int flexible, const T *b, T *x,
+                      const hfh_control &ctl, hfh_solve_info *info, double *hhist) {
+    hfh_ctx *c = A->ctx;
+    hipStream_t s = c->stream;
+    const int n = A->nrow;
+    const int m = std::min(size_basis, ctl.maxits);
+    const int m1 = m + 1;
+    const bool precond = P && P->kind != HFH_PREC_NONE;
+    Work w;
+    int rc;
+    const size_t vb = sizeof(T) * static_cast<size_t>(std::max(n, 1));
+    std::vector<T *> V(m1, nullptr), Z(m1, nullptr);
+    for (int i = 0; i < m1; ++i)
+        if ((rc = w.alloc(reinterpret_cast<void **>(&V[i]), vb))) return rc;
+    if (flexible && precond)
+        for (int i = 0; i < m1; ++i)
+            if ((rc = w.alloc(reinterpret_cast<void **>(&Z[i]), vb))) return rc;
+    T *wv = nullptr, *z = nullptr, **Vd = nullptr, **Zd = nullptr;
+    double *part = nullptr, *Hd = nullptr;
+    unsigned int *cnt = nullptr;
+    const int nb = stream_blocks(c, n);
+    const int stride = std::max(A->nrblk, kRedBlocks);
+    if ((rc = w.alloc(reinterpret_cast<void **>(&wv), vb)) || (rc = w.alloc(reinterpret_cast<void **>(&z), vb)) ||
+        (rc = w.alloc(reinterpret_cast<void **>(&Vd), sizeof(T *) * m1)) ||
+        (rc = w.alloc(reinterpret_cast<void **>(&Zd), sizeof(T *) * m1)) ||
+        (rc = w.alloc(reinterpret_cast<void **>(&part), sizeof(double) * stride)) ||
+        (rc = w.alloc(reinterpret_cast<void **>(&Hd), sizeof(double) * (m1 + 2))) ||
+        (rc = w.alloc(reinterpret_cast<void **>(&cnt), sizeof(unsigned) * 4)))
+        return rc;
+    HFH_HIP(hipMemsetAsync(cnt, 0, sizeof(unsigned) * 4, s));
+    HFH_HIP(hipMemcpyAsync(Vd, V.data(), sizeof(T *) * m1, hipMemcpyHostToDevice, s));
+    if (flexible && precond) HFH_HIP(hipMemcpyAsync(Zd, Z.data(), sizeof(T *) * m1, hipMemcpyHostToDevice, s));
+
+    hipEvent_t e0, e1;
+    HFH_HIP(hipEventCreate(&e0));
+    HFH_HIP(hipEventCreate(&e1));
+    HFH_HIP(hipEventRecord(e0, s));
+
+    auto residual = [&](T *dst) -> int {
+        int r2 = spmv_launch_ex(A, x, dst, 0, nullptr, nullptr, nullptr, nullptr, nullptr);
+        if (r2) return r2;
+        hipLaunchKernelGGL(k_neg_add<T>, dim3(nb), dim3(kBlock), 0, s, n, dst, b);
+        return launch_check("k_neg_add");
+    };
+    auto norm = [&](T *v, double *out) -> int {
+        hipLaunchKernelGGL(k_mgs<T>, dim3(nb), dim3(kBlock), 0, s, n, v, (const T *)nullptr, (const T *)nullptr,
+                           (const double *)nullptr, part, stride, cnt, Hd + m1);
+        int r2 = launch_check("k_mgs");
+        if (r2) return r2;
+        return fetch(c, Hd + m1, out, 1);
+    };
+    // apply the preconditioner (identity without one) into dst
+    auto precond_into = [&](const T *in, T *dst) -> int {
+        if (!precond) {
+            HFH_HIP(hipMemcpyAsync(dst, in, vb, hipMemcpyDeviceToDevice, s));
+            return HFH_OK;
+        }
+        HFH_HIP(hipMemsetAsync(dst, 0, vb, s));
+        return prec_apply_raw(P, in, dst);
+    };
+
+    std::vector<double> H(static_cast<size_t>(m) * m1, 0.0), g(m1), cs(m1), sn(m1), col(m1 + 1);
+    auto Hc = [&](int j, int i) -> double & { return H[static_cast<size_t>(j) * m1 + i]; };
+    auto rot = [](double csv, double snv, double &dx, double &dy) {
+        const T t = static_cast<T>(csv) * static_cast<T>(dx) + static_cast<T>(snv) * static_cast<T>(dy);
+        dy = static_cast<T>(-snv) * static_cast<T>(dx) + static_cast<T>(csv) * static_cast<T>(dy);
+        dx = t;
+    };
+    // x += M^-1 (V y) (GMRES) or x += Z y (FGMRES), y from the back substitution
+    auto update = [&](int k) -> int {
+        Coeffs cf;
+        std::vector<double> y(g.begin(), g.end());
+        for (int i = k + 1; i--;) {
+            y[i] = static_cast<T>(y[i]) / static_cast<T>(Hc(i, i));
+            const T t = static_cast<T>(y[i]);
+            for (int j = 0; j < i; ++j) y[j] = static_cast<T>(y[j]) - static_cast<T>(Hc(i, j)) * t;
+        }
+        for (int j = 0; j <= k; ++j) cf.y[j] = y[j];
+        if (flexible && precond) {
+            hipLaunchKernelGGL(k_multi_axpy<T>, dim3(nb), dim3(kBlock), 0, s, n, x, Zd, k + 1, cf);
+            return launch_check("k_multi_axpy");
+        }
+        HFH_HIP(hipMemsetAsync(z, 0, vb, s));
+        hipLaunchKernelGGL(k_multi_axpy<T>, dim3(nb), dim3(kBlock), 0, s, n, z, Vd, k + 1, cf);
+        int r2 = launch_check("k_multi_axpy");
+        if (r2) return r2;
+        if ((r2 = precond_into(z, wv))) return r2;
+        hipLaunchKernelGGL(k_add_into<T>, dim3(nb), dim3(kBlock), 0, s, n, x, (const T *)wv);
+        return launch_check("k_add_into");
+    };
+
+    int iter = 0;
+    if ((rc = residual(V[0]))) return rc;
+    double res = 0;
+    if ((rc = norm(V[0], &res))) return rc;
+    const double first = res;
+    int state = host_check(0, res, first, ctl);
+    std::vector<double> hist;
+    hist.push_back(res);
+    while (state == HFH_ITERATE) {
+        std::fill(g.begin(), g.end(), 0.0);
+        std::fill(H.begin(), H.end(), 0.0);
+        {
+            // V0 *= 1/res
+            HFH_HIP(hipMemcpyAsync(Hd + m1 + 1, &res, sizeof(double), hipMemcpyHostToDevice, s));
+            hipLaunchKernelGGL(k_scale_into<T>, dim3(nb), dim3(kBlock), 0, s, n, V[0], (const T *)V[0], (const double *)(Hd + m1 + 1));
+            if ((rc = launch_check("k_scale_into"))) return rc;
+        }
+        g[0] = res;
+        int j = 0;
+        for (j = 0; j != m; ++j) {
+            ++iter;
+            T *zj = (flexible && precond) ? Z[j] : z;
+            const T *az = zj;
+            if (precond) {
+                if ((rc = precond_into(V[j], zj))) return rc;
+            } else {
+                az = V[j];
+            }
+            if ((rc = spmv_launch_ex(A, az, wv, 0, nullptr, nullptr, nullptr, nullptr, nullptr))) return rc;
+            for (int i = 0; i <= j; ++i) {
+                hipLaunchKernelGGL(k_mgs<T>, dim3(nb), dim3(kBlock), 0, s, n, wv, i > 0 ? (const T *)V[i - 1] : (const T *)nullptr,
+                                   (const T *)V[i], (const double *)(Hd + (i > 0 ? i - 1 : 0)), part, stride, cnt, Hd + i);
+                if ((rc = launch_check("k_mgs"))) return rc;
+            }
+            // w -= h_j v_j ; h_{j+1} = ||w||
+            hipLaunchKernelGGL(k_mgs<T>, dim3(nb), dim3(kBlock), 0, s, n, wv, (const T *)V[j], (const T *)nullptr,
+                               (const double *)(Hd + j), part, stride, cnt, Hd + j + 1);
+            if ((rc = launch_check("k_mgs"))) return rc;
+            hipLaunchKernelGGL(k_scale_into<T>, dim3(nb), dim3(kBlock), 0, s, n, V[j + 1], (const T *)wv, (const double *)(Hd + j + 1));
+            if ((rc = launch_check("k_scale_into"))) return rc;
+            if ((rc = fetch(c, Hd, col.data(), j + 2))) return rc;
+            for (int i = 0; i <= j + 1; ++i) Hc(j, i) = col[i];
+            if (Hc(j, j + 1) == 0.0) {
+                set_error("GMRES: breakdown, H(%d,%d) = 0", j, j + 1);
+                return HFH_ERR_INVALID;
+            }
+            for (int k2 = 0; k2 < j; ++k2) rot(cs[k2], sn[k2], Hc(j, k2), Hc(j, k2 + 1));
+            {
+                const T dx = static_cast<T>(Hc(j, j)), dy = static_cast<T>(Hc(j, j + 1));
+                const T beta = static_cast<T>(sqrt(static_cast<double>(dx * dx + dy * dy)));
+                cs[j] = dx / beta;
+                sn[j] = dy / beta;
+            }
+            rot(cs[j], sn[j], Hc(j, j), Hc(j, j + 1));
+            rot(cs[j], sn[j], g[j], g[j + 1]);
+            res = std::fabs(static_cast<T>(g[j + 1]));
+            hist.push_back(res);
+            state = host_check(iter, res, first, ctl);
+            if (state != HFH_ITERATE) {
+                if ((rc = update(j))) return rc;
+                break;
+            }
+        }
+        if (state == HFH_ITERATE) {
+            if ((rc = update(m - 1))) return rc;
+            if ((rc = residual(V[0]))) return rc;
+        }
+    }
+    HFH_HIP(hipEventRecord(e1, s));
+    HFH_HIP(hipStreamSynchronize(s));
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (hhist) std::copy(hist.begin(), hist.end(), hhist);
+    if (info) {
+        info->iters = iter;
+        info->state = state;
+        info->res = res;
+        info->res0 = first;
+        info->time_ms = ms;
+    }
+    return HFH_OK;
+}
+
+// ============================================================================
+// mixed-precision defect correction (mpir.cc:69-327)
+
+template <typename TO, typename TI>
+__global__ __launch_bounds__(kBlock) void k_cast_n(int n, TO *__restrict__ y, const TI *__restrict__ x) {
+    for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock)
+        y[i] = static_cast<TO>(x[i]);
+}
+
+// x += (double) c  (CastInteriorFrom + Axpy(cor, 1), mpir.cc:223-253)
+__global__ __launch_bounds__(kBlock) void k_add_correction(int n, double *__restrict__ x, const float *__restrict__ c) {
+    for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock)
+        x[i] += 1.0 * static_cast<double>(c[i]);
+}
+
+}  // namespace hfh
+
+extern "C" int hfh_gmres_solve(const hfh_csr *A, hfh_prec *P, int size_basis, int flexible, const hfh_vec *b,
+                               hfh_vec *x, const hfh_control *ctl, hfh_solve_info *info, double *hist) {
+    int rc = check_solve_args(A, P, b, x, ctl, "hfh_gmres_solve");
+    if (rc) return rc;
+    HFH_CHECK(size_basis > 0 && size_basis <= kMaxBasis, HFH_ERR_INVALID,
+              "hfh_gmres_solve: basis size %d outside 1..%d", size_basis, kMaxBasis);
+    hfh_solve_info inf{};
+    if (A->dtype == HFH_F64)
+        rc = gmres_impl<double>(A, P, size_basis, flexible, (const double *)b->d, (double *)x->d, *ctl, &inf, hist);
+    else
+        rc = gmres_impl<float>(A, P, size_basis, flexible, (const float *)b->d, (float *)x->d, *ctl, &inf, hist);
+    if (rc) return rc;
+    if (info) *info = inf;
+    return solve_status(inf.state);
+}
+
+extern "C" int hfh_dc_solve(const hfh_csr *Ah, const hfh_csr *Al, hfh_prec *Pl, const hfh_vec *b, hfh_vec *x,
+                            const hfh_control *outer, const hfh_control *inner, hfh_dc_info *info, double *hhist) {
+    HFH_CHECK(Ah && Al && b && x && outer && inner, HFH_ERR_INVALID, "hfh_dc_solve: null argument");
+    HFH_CHECK(Ah->dtype == HFH_F64 && Al->dtype == HFH_F32, HFH_ERR_INVALID,
+              "hfh_dc_solve: needs an FP64 outer and an FP32 inner matrix");
+    HFH_CHECK(Ah->nrow == Al->nrow && Ah->nnz == Al->nnz && Ah->nrow == Ah->ncol, HFH_ERR_INVALID,
+              "hfh_dc_solve: matrices differ");
+    HFH_CHECK(b->n == Ah->nrow && x->n == Ah->nrow && b->dtype == HFH_F64 && x->dtype == HFH_F64, HFH_ERR_INVALID,
+              "hfh_dc_solve: vectors must be FP64 of size %d", Ah->nrow);
+    HFH_CHECK(!Pl || (Pl->dtype == HFH_F32 && Pl->n == Al->nrow), HFH_ERR_INVALID, "hfh_dc_solve: bad preconditioner");
+    HFH_CHECK(outer->maxits > 0 && inner->maxits > 0, HFH_ERR_INVALID, "hfh_dc_solve: maxits must be positive");
+    hfh_ctx *c = Ah->ctx;
+    hipStream_t s = c->stream;
+    const int n = Ah->nrow;
+    Work w;
+    double *rh = nullptr;
+    float *rl = nullptr, *cl = nullptr;
+    int rc;
+    if ((rc = w.alloc(reinterpret_cast<void **>(&rh), sizeof(double) * std::max(n, 1))) ||
+        (rc = w.alloc(reinterpret_cast<void **>(&rl), sizeof(float) * std::max(n, 1))) ||
+        (rc = w.alloc(reinterpret_cast<void **>(&cl), sizeof(float) * std::max(n, 1))))
+        return rc;
+    PcgWork<float> W;
+    if ((rc = W.init(Al, Pl, inner->maxits))) return rc;
+    const int nb = stream_blocks(c, n);
+    hfh_vec vr{c, n, HFH_F64, rh}, vb{c, n, HFH_F64, b->d};
+    hipEvent_t e0, e1;
+    HFH_HIP(hipEventCreate(&e0));
+    HFH_HIP(hipEventCreate(&e1));
+    HFH_HIP(hipEventRecord(e0, s));
+    auto true_residual = [&](double *res) -> int {
+        int r2 = spmv_launch_ex(Ah, x->d, rh, 0, nullptr, nullptr, nullptr, nullptr, nullptr);
+        if (!r2) r2 = hfh_scaleadd(&vr, -1.0, &vb);
+        if (!r2) r2 = hfh_nrm2(&vr, res);
+        return r2;
+    };
+    double res = 0;
+    if ((rc = true_residual(&res))) return rc;
+    double res0 = res;
+    const double first = res;
+    std::vector<double> hist{res};
+    int n_ecl = 0, inner_iters = 0;
+    auto ecl_ok = [&]() {
+        return n_ecl < outer->maxits && res > inner->atol && res > res0 * inner->rtol && res < inner->dtol;
+    };
+    while (res > outer->atol && res > res0 * outer->rtol && res < outer->dtol && ecl_ok()) {
+        ++n_ecl;
+        hipLaunchKernelGGL((k_cast_n<float, double>), dim3(nb), dim3(kBlock), 0, s, n, rl, (const double *)rh);
+        if ((rc = launch_check("k_cast_n"))) return rc;
+        HFH_HIP(hipMemsetAsync(cl, 0, sizeof(float) * n, s));
+        hfh_solve_info ii{};
+        if ((rc = W.solve(rl, cl, *inner, &ii, nullptr, true))) return rc;
+        inner_iters += ii.iters;
+        hipLaunchKernelGGL(k_add_correction, dim3(nb), dim3(kBlock), 0, s, n, (double *)x->d, (const float *)cl);
+        if ((rc = launch_check("k_add_correction"))) return rc;
+        if ((rc = true_residual(&res))) return rc;
+        res0 = res;
+        hist.push_back(res);
+    }
+    HFH_HIP(hipEventRecord(e1, s));
+    HFH_HIP(hipStreamSynchronize(s));
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    int state;
+    if (res < outer->atol) state = HFH_SUCCESS_ABS;
+    else if (res < res0 * outer->rtol) state = HFH_SUCCESS_REL;
+    else if (res >= outer->dtol) state = HFH_FAIL_DIV;
+    else state = HFH_FAIL_MAXITS;
+    if (hhist) std::copy(hist.begin(), hist.end(), hhist);
+    if (info) {
+        info->outer_iters = n_ecl;
+        info->inner_iters = inner_iters;
+        info->state = state;
+        info->res = res;
+        info->res0 = first;
+        info->time_ms = ms;
+    }
+    return solve_status(state);
+}

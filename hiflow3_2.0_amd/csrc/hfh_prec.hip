@@ -1,0 +1,445 @@
+// hfh_prec.hip -- preconditioner applies on gfx950.
+//
+// Jacobi  : out = in .* invD (lpreconditioner.cc:151-166), one HBM stream.
+// MC-SGS  : on the colour-permuted matrix every colour is a contiguous row
+//           range and rows of one colour do not couple, so each colour's
+//           forward / backward block step (lpreconditioner_mc_gs.cc:91-113,
+//           lpreconditioner_mc_sgs.cc:71-90) is one data-parallel launch over
+//           that colour's rows; the rows read only the off-colour part of A.
+//           Block products are summed per colour block and added in the
+//           reference's block order (ascending forward, descending backward).
+// SGS / ILU(0): sparse triangular sweeps (lmatrix_csr_cpu.cc:542-616 and
+//           2108-2171) scheduled by levels: rows of one level are
+//           independent, one launch per level, with each row's arithmetic in
+//           the reference's order.
+#include "hfh_internal.h"
+
+#include <algorithm>
+
+namespace hfh {
+
+constexpr int kMaxColors = 64;
+
+struct ColorOffsets {
+    int off[kMaxColors + 1];
+};
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_jacobi(int n, const T *__restrict__ in, const T *__restrict__ invd,
+                                                   T *__restrict__ out, const int *__restrict__ done) {
+    if (done && *done) return;
+    for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock)
+        out[i] = in[i] * invd[i];
+}
+
+// forward step of colour 0 (no lower colours): y = (-in) * iDv
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_mc_fwd0(int r0, int r1, const T *__restrict__ in,
+                                                    const T *__restrict__ idv, T *__restrict__ out,
+                                                    const int *__restrict__ done) {
+    if (done && *done) return;
+    const int i = r0 + blockIdx.x * kBlock + threadIdx.x;
+    if (i >= r1) return;
+    out[i] = (in[i] * static_cast<T>(-1)) * idv[i];
+}
+
+// forward step of colour c >= 1.  When `last` the colour's diagonal and
+// backward steps (no upper colours) are applied too: z = ((y * Dv) * iDv).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_mc_fwd(int c, ColorOffsets co, bool last,
+                                                   const int *__restrict__ row, const int *__restrict__ col,
+                                                   const T *__restrict__ val, const T *__restrict__ in,
+                                                   const T *__restrict__ idv, const T *__restrict__ dv,
+                                                   T *__restrict__ out, const int *__restrict__ done) {
+    if (done && *done) return;
+    const int i = co.off[c] + blockIdx.x * kBlock + threadIdx.x;
+    if (i >= co.off[c + 1]) return;
+    T chunk = in[i] * static_cast<T>(-1);
+    int blk = 0;
+    int blk_end = co.off[1];
+    T res = 0;
+    const int lo = co.off[c];
+    for (int j = row[i], je = row[i + 1]; j < je; ++j) {
+        const int cj = col[j];
+        if (cj >= lo) break;
+        while (cj >= blk_end) {  // entries are column sorted: flush finished blocks
+            chunk += res;
+            res = 0;
+            ++blk;
+            blk_end = co.off[blk + 1];
+        }
+        res += val[j] * out[cj];
+    }
+    chunk += res;
+    chunk *= idv[i];
+    if (last) chunk = (chunk * dv[i]) * idv[i];
+    out[i] = chunk;
+}
+
+// backward step of colour c < ncolors-1, with the diagonal step first.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_mc_bwd(int c, int ncolors, ColorOffsets co,
+                                                   const int *__restrict__ row, const int *__restrict__ col,
+                                                   const T *__restrict__ val, const T *__restrict__ idv,
+                                                   const T *__restrict__ dv, T *__restrict__ out,
+                                                   const int *__restrict__ done) {
+    if (done && *done) return;
+    const int i = co.off[c] + blockIdx.x * kBlock + threadIdx.x;
+    if (i >= co.off[c + 1]) return;
+    T chunk = out[i] * dv[i];
+    const int rs = row[i];
+    int e = row[i + 1];
+    for (int b = ncolors - 1; b > c; --b) {
+        const int lo = co.off[b];
+        int s = e;
+        while (s > rs && col[s - 1] >= lo) --s;
+        T res = 0;
+        for (int j = s; j < e; ++j) res += val[j] * out[col[j]];
+        chunk += res;
+        e = s;
+    }
+    out[i] = chunk * idv[i];
+}
+
+// Level-scheduled triangular sweeps.  mode 0: SGS forward (D+L) y = r;
+// mode 1: SGS backward (D+U) z = D y; mode 2: ILU unit-lower forward;
+// mode 3: ILU (D+U) backward.
+template <typename T, int MODE>
+__global__ __launch_bounds__(kBlock) void k_trsv_level(const int *__restrict__ lrows, int cnt,
+                                                       const int *__restrict__ row, const int *__restrict__ col,
+                                                       const T *__restrict__ val, const int *__restrict__ diag,
+                                                       const T *__restrict__ in, T *__restrict__ out,
+                                                       const int *__restrict__ done) {
+    if (done && *done) return;
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= cnt) return;
+    const int i = lrows[k];
+    const int js = row[i], je = row[i + 1];
+    if (MODE == 0 || MODE == 2) {
+        T acc = in[i];
+        for (int j = js; j < je; ++j) {
+            const int cj = col[j];
+            if (cj >= i) break;
+            acc -= out[cj] * val[j];
+        }
+        if (MODE == 0) acc /= val[diag[i]];
+        out[i] = acc;
+    } else {
+        const T d = val[diag[i]];
+        T acc = out[i];
+        if (MODE == 1) acc *= d;
+        for (int j = js; j < je; ++j) {
+            const int cj = col[j];
+            if (cj > i) acc -= out[cj] * val[j];
+        }
+        out[i] = acc / d;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_mc_diag_setup(int n, const int *__restrict__ diag,
+                                                          const T *__restrict__ val, T omega_relax,
+                                                          T *__restrict__ idv, T *__restrict__ dv,
+                                                          int *__restrict__ bad) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const int p = diag[i];
+    const T d = p >= 0 ? val[p] : static_cast<T>(0);
+    if (d == static_cast<T>(0)) atomicAdd(bad, 1);
+    // iDv: 1/d scaled by -1; Dv: d scaled by -1 then by 1/(omega(2-omega))
+    idv[i] = (static_cast<T>(1) / d) * static_cast<T>(-1);
+    dv[i] = (d * static_cast<T>(-1)) * omega_relax;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_scale_offdiag(int n, const int *__restrict__ row, const int *__restrict__ col,
+                                                          const T *__restrict__ val, T omega, T *__restrict__ sval) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    for (int j = row[i]; j < row[i + 1]; ++j) sval[j] = col[j] == i ? val[j] : val[j] * omega;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_check_diag(int n, const int *__restrict__ diag, const T *__restrict__ val,
+                                                       int *__restrict__ bad) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const int p = diag[i];
+    if (p < 0 || val[p] == static_cast<T>(0)) atomicAdd(bad, 1);
+}
+
+static int count_bad_diag(const hfh_csr *A, const char *what, void *idv, void *dv, double relax) {
+    hfh_ctx *c = A->ctx;
+    int *bad = nullptr;
+    int rc = dev_alloc(reinterpret_cast<void **>(&bad), sizeof(int));
+    if (rc) return rc;
+    HFH_HIP(hipMemsetAsync(bad, 0, sizeof(int), c->stream));
+    const int nb = (A->nrow + kBlock - 1) / kBlock;
+    if (A->nrow > 0) {
+        if (idv) {
+            if (A->dtype == HFH_F64)
+                hipLaunchKernelGGL(k_mc_diag_setup<double>, dim3(nb), dim3(kBlock), 0, c->stream, A->nrow, A->diag,
+                                   (const double *)A->val, relax, (double *)idv, (double *)dv, bad);
+            else
+                hipLaunchKernelGGL(k_mc_diag_setup<float>, dim3(nb), dim3(kBlock), 0, c->stream, A->nrow, A->diag,
+                                   (const float *)A->val, (float)relax, (float *)idv, (float *)dv, bad);
+        } else {
+            if (A->dtype == HFH_F64)
+                hipLaunchKernelGGL(k_check_diag<double>, dim3(nb), dim3(kBlock), 0, c->stream, A->nrow, A->diag, (const double *)A->val, bad);
+            else
+                hipLaunchKernelGGL(k_check_diag<float>, dim3(nb), dim3(kBlock), 0, c->stream, A->nrow, A->diag, (const float *)A->val, bad);
+        }
+        if ((rc = launch_check(what))) { (void)hipFree(bad); return rc; }
+    }
+    int hbad = 0;
+    HFH_HIP(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HFH_HIP(hipStreamSynchronize(c->stream));
+    (void)hipFree(bad);
+    HFH_CHECK(hbad == 0, HFH_ERR_ZERO_PIVOT, "%s: %d rows with a zero or missing diagonal entry", what, hbad);
+    return HFH_OK;
+}
+
+static int upload_levels(hfh_levels &L, int n, int nlev, const int *ptr, const int *rows, const char *what) {
+    HFH_CHECK(nlev >= 0 && ptr && rows, HFH_ERR_INVALID, "%s: missing level schedule", what);
+    HFH_CHECK(ptr[0] == 0 && ptr[nlev] == n, HFH_ERR_INVALID, "%s: level pointer must run from 0 to %d", what, n);
+    std::vector<char> seen(n, 0);
+    for (int l = 0; l < nlev; ++l) HFH_CHECK(ptr[l + 1] >= ptr[l], HFH_ERR_INVALID, "%s: level pointer decreases", what);
+    for (int k = 0; k < n; ++k) {
+        HFH_CHECK(rows[k] >= 0 && rows[k] < n && !seen[rows[k]], HFH_ERR_INVALID,
+                  "%s: level rows are not a permutation (entry %d)", what, k);
+        seen[rows[k]] = 1;
+    }
+    L.nlev = nlev;
+    L.ptr.assign(ptr, ptr + nlev + 1);
+    int rc = dev_alloc(reinterpret_cast<void **>(&L.rows), sizeof(int) * static_cast<size_t>(n));
+    if (rc) return rc;
+    HFH_HIP(hipMemcpy(L.rows, rows, sizeof(int) * static_cast<size_t>(n), hipMemcpyHostToDevice));
+    return HFH_OK;
+}
+
+// Checks that every lower (forward) / upper (backward) dependency of a row sits
+// in an earlier level, using the host copy of the structure.
+static int validate_levels(const hfh_csr *A, const hfh_levels &L, bool upper, const char *what) {
+    const int n = A->nrow;
+    std::vector<int> col(A->nnz), lev(n), rows(n);
+    HFH_HIP(hipMemcpy(col.data(), A->col, sizeof(int) * static_cast<size_t>(A->nnz), hipMemcpyDeviceToHost));
+    HFH_HIP(hipMemcpy(rows.data(), L.rows, sizeof(int) * static_cast<size_t>(n), hipMemcpyDeviceToHost));
+    for (int l = 0; l < L.nlev; ++l)
+        for (int k = L.ptr[l]; k < L.ptr[l + 1]; ++k) lev[rows[k]] = l;
+    for (int i = 0; i < n; ++i)
+        for (int j = A->h_row[i]; j < A->h_row[i + 1]; ++j) {
+            const int c = col[j];
+            const bool dep = upper ? c > i : c < i;
+            HFH_CHECK(!dep || lev[c] < lev[i], HFH_ERR_INVALID,
+                      "%s: row %d (level %d) depends on row %d (level %d) in the %s sweep", what, i, lev[i], c,
+                      lev[c], upper ? "backward" : "forward");
+        }
+    return HFH_OK;
+}
+
+template <typename T>
+static int trsv_sweeps(hfh_prec *P, const T *in, T *out, const int *done) {
+    const hfh_csr *A = P->A;
+    hipStream_t s = P->ctx->stream;
+    const bool ilu = P->kind == HFH_PREC_ILU0;
+    for (int l = 0; l < P->lo.nlev; ++l) {
+        const int a = P->lo.ptr[l], cnt = P->lo.ptr[l + 1] - a;
+        if (cnt == 0) continue;
+        const dim3 g((cnt + kBlock - 1) / kBlock);
+        if (ilu)
+            hipLaunchKernelGGL((k_trsv_level<T, 2>), g, dim3(kBlock), 0, s, P->lo.rows + a, cnt, A->row, A->col, (const T *)A->val, A->diag, in, out, done);
+        else
+            hipLaunchKernelGGL((k_trsv_level<T, 0>), g, dim3(kBlock), 0, s, P->lo.rows + a, cnt, A->row, A->col, (const T *)A->val, A->diag, in, out, done);
+    }
+    for (int l = 0; l < P->up.nlev; ++l) {
+        const int a = P->up.ptr[l], cnt = P->up.ptr[l + 1] - a;
+        if (cnt == 0) continue;
+        const dim3 g((cnt + kBlock - 1) / kBlock);
+        if (ilu)
+            hipLaunchKernelGGL((k_trsv_level<T, 3>), g, dim3(kBlock), 0, s, P->up.rows + a, cnt, A->row, A->col, (const T *)A->val, A->diag, in, out, done);
+        else
+            hipLaunchKernelGGL((k_trsv_level<T, 1>), g, dim3(kBlock), 0, s, P->up.rows + a, cnt, A->row, A->col, (const T *)A->val, A->diag, in, out, done);
+    }
+    return launch_check("k_trsv_level");
+}
+
+template <typename T>
+static int mcsgs_apply(hfh_prec *P, const T *in, T *out, const int *done) {
+    const hfh_csr *A = P->A;
+    hipStream_t s = P->ctx->stream;
+    ColorOffsets co;
+    for (int c = 0; c <= P->ncolors; ++c) co.off[c] = P->color_off[c];
+    const T *val = P->sval ? (const T *)P->sval : (const T *)A->val;
+    const T *idv = (const T *)P->invd, *dv = (const T *)P->dv;
+    const int nc = P->ncolors;
+    for (int c = 0; c < nc; ++c) {
+        const int cnt = co.off[c + 1] - co.off[c];
+        if (cnt == 0) continue;
+        const dim3 g((cnt + kBlock - 1) / kBlock);
+        if (c == 0 && nc > 1)
+            hipLaunchKernelGGL(k_mc_fwd0<T>, g, dim3(kBlock), 0, s, co.off[0], co.off[1], in, idv, out, done);
+        else
+            hipLaunchKernelGGL(k_mc_fwd<T>, g, dim3(kBlock), 0, s, c, co, c == nc - 1, A->row, A->col, val, in, idv, dv, out, done);
+    }
+    for (int c = nc - 2; c >= 0; --c) {
+        const int cnt = co.off[c + 1] - co.off[c];
+        if (cnt == 0) continue;
+        const dim3 g((cnt + kBlock - 1) / kBlock);
+        hipLaunchKernelGGL(k_mc_bwd<T>, g, dim3(kBlock), 0, s, c, nc, co, A->row, A->col, val, idv, dv, out, done);
+    }
+    return launch_check("k_mc_sweep");
+}
+
+int prec_apply_raw(hfh_prec *P, const void *in, void *out, const int *done) {
+    hipStream_t s = P->ctx->stream;
+    const int n = P->n;
+    if (n == 0) return HFH_OK;
+    switch (P->kind) {
+    case HFH_PREC_NONE:
+        HFH_CHECK(!done, HFH_ERR_INVALID, "prec_apply: identity preconditioner inside a driver");
+        HFH_HIP(hipMemcpyAsync(out, in, (P->dtype == HFH_F64 ? 8 : 4) * static_cast<size_t>(n), hipMemcpyDeviceToDevice, s));
+        return HFH_OK;
+    case HFH_PREC_JACOBI: {
+        const int nb = std::min((n + kBlock - 1) / kBlock, std::max(P->ctx->num_cu, 1) * 16);
+        if (P->dtype == HFH_F64)
+            hipLaunchKernelGGL(k_jacobi<double>, dim3(nb), dim3(kBlock), 0, s, n, (const double *)in, (const double *)P->invd, (double *)out, done);
+        else
+            hipLaunchKernelGGL(k_jacobi<float>, dim3(nb), dim3(kBlock), 0, s, n, (const float *)in, (const float *)P->invd, (float *)out, done);
+        return launch_check("k_jacobi");
+    }
+    case HFH_PREC_MCSGS:
+        return P->dtype == HFH_F64 ? mcsgs_apply<double>(P, (const double *)in, (double *)out, done)
+                                   : mcsgs_apply<float>(P, (const float *)in, (float *)out, done);
+    case HFH_PREC_SGS:
+    case HFH_PREC_ILU0:
+        return P->dtype == HFH_F64 ? trsv_sweeps<double>(P, (const double *)in, (double *)out, done)
+                                   : trsv_sweeps<float>(P, (const float *)in, (float *)out, done);
+    default:
+        set_error("prec_apply: unknown kind %d", P->kind);
+        return HFH_ERR_INVALID;
+    }
+}
+
+}  // namespace hfh
+
+using namespace hfh;
+
+static hfh_prec *new_prec(const hfh_csr *A, int kind) {
+    hfh_prec *P = new hfh_prec();
+    P->kind = kind;
+    P->ctx = A->ctx;
+    P->A = A;
+    P->n = A->nrow;
+    P->dtype = A->dtype;
+    return P;
+}
+
+extern "C" int hfh_jacobi_create(const hfh_csr *A, hfh_prec **out) {
+    HFH_CHECK(A && out && A->has_structure, HFH_ERR_INVALID, "hfh_jacobi_create: bad argument");
+    HFH_CHECK(A->nrow == A->ncol, HFH_ERR_INVALID, "hfh_jacobi_create: matrix is not square");
+    hfh_prec *P = new_prec(A, HFH_PREC_JACOBI);
+    int rc = dev_alloc(&P->invd, A->elem() * static_cast<size_t>(A->nrow));
+    if (!rc) {
+        hfh_vec tmp;
+        tmp.ctx = A->ctx;
+        tmp.n = A->nrow;
+        tmp.dtype = A->dtype;
+        tmp.d = P->invd;
+        rc = hfh_csr_invdiag(A, &tmp);
+        if (!rc) rc = count_bad_diag(A, "hfh_jacobi_create", nullptr, nullptr, 1.0);
+    }
+    if (rc) { hfh_prec_destroy(P); return rc; }
+    *out = P;
+    return HFH_OK;
+}
+
+extern "C" int hfh_mcsgs_create(const hfh_csr *A, int ncolors, const int *color_sizes, double omega, hfh_prec **out) {
+    HFH_CHECK(A && out && A->has_structure && color_sizes, HFH_ERR_INVALID, "hfh_mcsgs_create: bad argument");
+    HFH_CHECK(A->nrow == A->ncol, HFH_ERR_INVALID, "hfh_mcsgs_create: matrix is not square");
+    HFH_CHECK(ncolors >= 1 && ncolors <= kMaxColors, HFH_ERR_UNSUPPORTED, "hfh_mcsgs_create: %d colours (1..%d supported)",
+              ncolors, kMaxColors);
+    HFH_CHECK(omega > 0.0 && omega < 2.0, HFH_ERR_INVALID, "hfh_mcsgs_create: omega %g outside (0, 2)", omega);
+    hfh_prec *P = new_prec(A, HFH_PREC_MCSGS);
+    P->ncolors = ncolors;
+    P->omega = omega;
+    P->color_off.assign(ncolors + 1, 0);
+    for (int c = 0; c < ncolors; ++c) {
+        if (color_sizes[c] < 0) { delete P; hfh::set_error("hfh_mcsgs_create: negative colour size"); return HFH_ERR_INVALID; }
+        P->color_off[c + 1] = P->color_off[c] + color_sizes[c];
+    }
+    if (P->color_off[ncolors] != A->nrow) {
+        delete P;
+        hfh::set_error("hfh_mcsgs_create: colour sizes sum to %d, matrix has %d rows", P->color_off.back(), A->nrow);
+        return HFH_ERR_INVALID;
+    }
+    int rc = dev_alloc(&P->invd, A->elem() * static_cast<size_t>(A->nrow));
+    if (!rc) rc = dev_alloc(&P->dv, A->elem() * static_cast<size_t>(A->nrow));
+    if (!rc) rc = count_bad_diag(A, "hfh_mcsgs_create", P->invd, P->dv, 1.0 / (omega * (2.0 - omega)));
+    if (!rc && omega != 1.0 && A->nnz > 0) {
+        rc = dev_alloc(&P->sval, A->elem() * static_cast<size_t>(A->nnz));
+        if (!rc) {
+            const int nb = (A->nrow + kBlock - 1) / kBlock;
+            if (A->dtype == HFH_F64)
+                hipLaunchKernelGGL(k_scale_offdiag<double>, dim3(nb), dim3(kBlock), 0, A->ctx->stream, A->nrow, A->row, A->col,
+                                   (const double *)A->val, omega, (double *)P->sval);
+            else
+                hipLaunchKernelGGL(k_scale_offdiag<float>, dim3(nb), dim3(kBlock), 0, A->ctx->stream, A->nrow, A->row, A->col,
+                                   (const float *)A->val, (float)omega, (float *)P->sval);
+            rc = launch_check("k_scale_offdiag");
+        }
+    }
+    if (rc) { hfh_prec_destroy(P); return rc; }
+    *out = P;
+    return HFH_OK;
+}
+
+static int create_trsv(const hfh_csr *A, int kind, int nlev, const int *lp, const int *lr, int nlu, const int *up,
+                       const int *ur, hfh_prec **out, const char *what) {
+    HFH_CHECK(A && out && A->has_structure, HFH_ERR_INVALID, "%s: bad argument", what);
+    HFH_CHECK(A->nrow == A->ncol, HFH_ERR_INVALID, "%s: matrix is not square", what);
+    hfh_prec *P = new_prec(A, kind);
+    int rc = count_bad_diag(A, what, nullptr, nullptr, 1.0);
+    if (!rc) rc = upload_levels(P->lo, A->nrow, nlev, lp, lr, what);
+    if (!rc) rc = upload_levels(P->up, A->nrow, nlu, up, ur, what);
+    if (!rc) rc = validate_levels(A, P->lo, false, what);
+    if (!rc) rc = validate_levels(A, P->up, true, what);
+    if (rc) { hfh_prec_destroy(P); return rc; }
+    *out = P;
+    return HFH_OK;
+}
+
+extern "C" int hfh_sgs_create(const hfh_csr *A, int nlev, const int *lp, const int *lr, int nlu, const int *up,
+                              const int *ur, hfh_prec **out) {
+    return create_trsv(A, HFH_PREC_SGS, nlev, lp, lr, nlu, up, ur, out, "hfh_sgs_create");
+}
+
+extern "C" int hfh_ilu0_create(const hfh_csr *LU, int nlev, const int *lp, const int *lr, int nlu, const int *up,
+                               const int *ur, hfh_prec **out) {
+    return create_trsv(LU, HFH_PREC_ILU0, nlev, lp, lr, nlu, up, ur, out, "hfh_ilu0_create");
+}
+
+extern "C" int hfh_prec_apply(hfh_prec *P, const hfh_vec *in, hfh_vec *out) {
+    HFH_CHECK(P && in && out, HFH_ERR_INVALID, "hfh_prec_apply: null argument");
+    HFH_CHECK(in->n == P->n && out->n == P->n && in->dtype == P->dtype && out->dtype == P->dtype, HFH_ERR_INVALID,
+              "hfh_prec_apply: vector size/dtype mismatch (n %d, in %d, out %d)", P->n, in->n, out->n);
+    HFH_CHECK(in->d != out->d, HFH_ERR_INVALID, "hfh_prec_apply: in and out must not alias");
+    return prec_apply_raw(P, in->d, out->d);
+}
+
+extern "C" int hfh_prec_kind(const hfh_prec *P, int *kind) {
+    HFH_CHECK(P && kind, HFH_ERR_INVALID, "hfh_prec_kind: null argument");
+    *kind = P->kind;
+    return HFH_OK;
+}
+
+extern "C" int hfh_prec_destroy(hfh_prec *P) {
+    if (!P) return HFH_OK;
+    if (P->ctx && P->ctx->stream) (void)hipStreamSynchronize(P->ctx->stream);
+    if (P->invd) (void)hipFree(P->invd);
+    if (P->dv) (void)hipFree(P->dv);
+    if (P->sval) (void)hipFree(P->sval);
+    if (P->lo.rows) (void)hipFree(P->lo.rows);
+    if (P->up.rows) (void)hipFree(P->up.rows);
+    delete P;
+    return HFH_OK;
+}

@@ -1,0 +1,503 @@
+"""Python binding of the gfx950 HIP platform (libhfhip.so) and of the host
+layer (libhflmp.so), over their C ABIs (include/hfh.h, include/hflmp.h).
+
+There is no fallback: if the HIP library is missing or no GPU is present the
+calls raise.  Arrays cross the boundary as numpy buffers (host) or opaque
+device handles; the device never sees a torch tensor.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+HIP_LIB = os.path.join(_HERE, "libhfhip.so")
+LMP_LIB = os.path.join(_HERE, "libhflmp.so")
+
+F64, F32 = 0, 1
+PREC_NONE, PREC_JACOBI, PREC_SGS, PREC_MCSGS, PREC_ILU0 = 0, 1, 2, 3, 4
+ITERATE, SUCCESS_ABS, SUCCESS_REL, FAIL_DIV, FAIL_MAXITS = 0, 1, 2, 3, 4
+OK, ERR_INVALID, ERR_NOMEM, ERR_HIP, ERR_ZERO_PIVOT, ERR_NOT_CONVERGED, ERR_BUDGET, ERR_UNSUPPORTED = \
+    0, -1, -2, -3, -4, -5, -6, -7
+
+Q1_2D, Q1_3D, FV_DIFFUSION, UPWIND_CONVDIFF = 1, 2, 3, 4
+
+
+class HfhError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"hfh error {code}: {msg}")
+        self.code = code
+
+
+class Control(C.Structure):
+    _fields_ = [("maxits", C.c_int), ("atol", C.c_double), ("rtol", C.c_double), ("dtol", C.c_double)]
+
+
+class SolveInfo(C.Structure):
+    _fields_ = [("iters", C.c_int), ("state", C.c_int), ("res", C.c_double), ("res0", C.c_double),
+                ("time_ms", C.c_double)]
+
+
+class DcInfo(C.Structure):
+    _fields_ = [("outer_iters", C.c_int), ("inner_iters", C.c_int), ("state", C.c_int), ("res", C.c_double),
+                ("res0", C.c_double), ("time_ms", C.c_double)]
+
+
+_hip = None
+_lmp = None
+
+# C ABI signatures: name -> (restype, argtypes)
+_vp, _i, _d, _ll = C.c_void_p, C.c_int, C.c_double, C.c_longlong
+_P = C.POINTER
+HIP_API = {
+    "hfh_last_error": (C.c_char_p, []),
+    "hfh_version": (C.c_char_p, []),
+    "hfh_device_count": (_i, [_P(_i)]),
+    "hfh_ctx_create": (_i, [_i, _P(_vp)]),
+    "hfh_ctx_destroy": (_i, [_vp]),
+    "hfh_ctx_sync": (_i, [_vp]),
+    "hfh_ctx_info": (_i, [_vp, _P(_i), _P(_ll), C.c_char_p, _i]),
+    "hfh_ctx_set_timing": (_i, [_vp, _i]),
+    "hfh_ctx_timing": (_i, [_vp, _i, _P(_ll), _P(_d)]),
+    "hfh_ctx_timing_reset": (_i, [_vp]),
+    "hfh_vec_create": (_i, [_vp, _i, _i, _P(_vp)]),
+    "hfh_vec_destroy": (_i, [_vp]),
+    "hfh_vec_size": (_i, [_vp, _P(_i)]),
+    "hfh_vec_dtype": (_i, [_vp, _P(_i)]),
+    "hfh_vec_upload": (_i, [_vp, _vp, _i, _i]),
+    "hfh_vec_download": (_i, [_vp, _vp, _i, _i]),
+    "hfh_vec_set_values": (_i, [_vp, _vp, _i, _vp]),
+    "hfh_vec_get_values": (_i, [_vp, _vp, _i, _vp]),
+    "hfh_vec_zeros": (_i, [_vp]),
+    "hfh_vec_copy": (_i, [_vp, _vp]),
+    "hfh_vec_copy_range": (_i, [_vp, _i, _vp, _i, _i]),
+    "hfh_vec_cast": (_i, [_vp, _vp]),
+    "hfh_vec_reorder": (_i, [_vp, _vp, _vp]),
+    "hfh_dot": (_i, [_vp, _vp, _P(_d)]),
+    "hfh_nrm2": (_i, [_vp, _P(_d)]),
+    "hfh_axpy": (_i, [_vp, _vp, _d]),
+    "hfh_scaleadd": (_i, [_vp, _d, _vp]),
+    "hfh_scale": (_i, [_vp, _d]),
+    "hfh_ewmult": (_i, [_vp, _vp]),
+    "hfh_csr_create": (_i, [_vp, _i, _i, _i, _i, _P(_vp)]),
+    "hfh_csr_destroy": (_i, [_vp]),
+    "hfh_csr_dims": (_i, [_vp, _P(_i), _P(_i), _P(_i), _P(_i)]),
+    "hfh_csr_upload": (_i, [_vp, _vp, _vp, _vp]),
+    "hfh_csr_upload_values": (_i, [_vp, _vp]),
+    "hfh_csr_download": (_i, [_vp, _vp, _vp, _vp]),
+    "hfh_csr_cast": (_i, [_vp, _vp]),
+    "hfh_spmv": (_i, [_vp, _vp, _vp]),
+    "hfh_spmv_add": (_i, [_vp, _vp, _vp]),
+    "hfh_csr_invdiag": (_i, [_vp, _vp]),
+    "hfh_jacobi_create": (_i, [_vp, _P(_vp)]),
+    "hfh_sgs_create": (_i, [_vp, _i, _vp, _vp, _i, _vp, _vp, _P(_vp)]),
+    "hfh_mcsgs_create": (_i, [_vp, _i, _vp, _d, _P(_vp)]),
+    "hfh_ilu0_create": (_i, [_vp, _i, _vp, _vp, _i, _vp, _vp, _P(_vp)]),
+    "hfh_prec_apply": (_i, [_vp, _vp, _vp]),
+    "hfh_prec_kind": (_i, [_vp, _P(_i)]),
+    "hfh_prec_destroy": (_i, [_vp]),
+    "hfh_pcg_solve": (_i, [_vp, _vp, _vp, _vp, _P(Control), _P(SolveInfo), _vp]),
+    "hfh_gmres_solve": (_i, [_vp, _vp, _i, _i, _vp, _vp, _P(Control), _P(SolveInfo), _vp]),
+    "hfh_dc_solve": (_i, [_vp, _vp, _vp, _vp, _vp, _P(Control), _P(Control), _P(DcInfo), _vp]),
+}
+
+LMP_API = {
+    "hfl_problem_size": (_i, [_i, _i, _i, _i, _d, _d, C.c_ulonglong, _P(_i), _P(_ll)]),
+    "hfl_problem_fill": (_i, [_i, _i, _i, _i, _d, _d, C.c_ulonglong, _vp, _vp, _vp, _vp]),
+    "hfl_uniform_pm1": (_d, [C.c_ulonglong, C.c_ulonglong, C.c_ulonglong]),
+    "hfl_normal01": (_d, [C.c_ulonglong, C.c_ulonglong, C.c_ulonglong]),
+    "hfl_multicoloring": (_i, [_i, _vp, _vp, _P(_i), _vp, _vp, _vp]),
+    "hfl_levels_lower": (_i, [_i, _vp, _vp, _P(_i), _vp, _vp]),
+    "hfl_levels_upper": (_i, [_i, _vp, _vp, _P(_i), _vp, _vp]),
+    "hfl_ilu0": (_i, [_i, _vp, _vp, _vp]),
+    "hfl_csr_permute": (_i, [_i, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+}
+
+
+def _declare(lib, api):
+    for name, (res, args) in api.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+
+
+def hip():
+    """The HIP platform library; raises if it has not been built."""
+    global _hip
+    if _hip is None:
+        if not os.path.exists(HIP_LIB):
+            raise ImportError(f"{HIP_LIB} is missing: run hiflow3_2.0_amd/build.py (or __graft_entry__.build())")
+        lib = C.CDLL(HIP_LIB, mode=C.RTLD_GLOBAL)
+        _declare(lib, HIP_API)
+        _hip = lib
+    return _hip
+
+
+def lmp():
+    global _lmp
+    if _lmp is None:
+        hip()
+        if not os.path.exists(LMP_LIB):
+            raise ImportError(f"{LMP_LIB} is missing: run hiflow3_2.0_amd/build.py")
+        lib = C.CDLL(LMP_LIB)
+        _declare(lib, LMP_API)
+        _lmp = lib
+    return _lmp
+
+
+def _ck(rc):
+    if rc != OK:
+        raise HfhError(rc, hip().hfh_last_error().decode())
+    return rc
+
+
+def _p(a):
+    # data_as keeps a reference to the array, so temporaries live through the call
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _dt(dtype):
+    return F32 if np.dtype(dtype) == np.float32 else F64
+
+
+def _np(dt):
+    return np.float32 if dt == F32 else np.float64
+
+
+def device_count():
+    c = C.c_int(0)
+    _ck(hip().hfh_device_count(C.byref(c)))
+    return c.value
+
+
+class Context:
+    """One HIP stream plus reduction scratch on one device."""
+
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        _ck(hip().hfh_ctx_create(device, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            hip().hfh_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sync(self):
+        _ck(hip().hfh_ctx_sync(self.h))
+
+    def info(self):
+        cu, hbm = C.c_int(), C.c_longlong()
+        name = C.create_string_buffer(128)
+        _ck(hip().hfh_ctx_info(self.h, C.byref(cu), C.byref(hbm), name, 128))
+        return {"num_cu": cu.value, "hbm_bytes": hbm.value, "arch": name.value.decode()}
+
+    def set_timing(self, on):
+        _ck(hip().hfh_ctx_set_timing(self.h, 1 if on else 0))
+
+    def timing(self, kclass=0):
+        n, ms = C.c_longlong(), C.c_double()
+        _ck(hip().hfh_ctx_timing(self.h, kclass, C.byref(n), C.byref(ms)))
+        return n.value, ms.value
+
+    def timing_reset(self):
+        _ck(hip().hfh_ctx_timing_reset(self.h))
+
+
+class Vector:
+    def __init__(self, ctx, n_or_array, dtype=np.float64):
+        self.ctx = ctx
+        if isinstance(n_or_array, (int, np.integer)):
+            arr, n = None, int(n_or_array)
+        else:
+            arr = np.ascontiguousarray(n_or_array, dtype=dtype)
+            n = len(arr)
+        self.n, self.dtype = n, np.dtype(dtype)
+        h = C.c_void_p()
+        _ck(hip().hfh_vec_create(ctx.h, n, _dt(dtype), C.byref(h)))
+        self.h = h
+        if arr is not None:
+            self.upload(arr)
+
+    def __del__(self):
+        try:
+            if self.h:
+                hip().hfh_vec_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def upload(self, arr, offset=0):
+        arr = np.ascontiguousarray(arr, dtype=self.dtype)
+        _ck(hip().hfh_vec_upload(self.h, _p(arr), offset, len(arr)))
+
+    def numpy(self, offset=0, count=None):
+        count = self.n - offset if count is None else count
+        out = np.empty(count, dtype=self.dtype)
+        _ck(hip().hfh_vec_download(self.h, _p(out), offset, count))
+        return out
+
+    def set_values(self, index, values):
+        index = np.ascontiguousarray(index, np.int32)
+        values = np.ascontiguousarray(values, self.dtype)
+        _ck(hip().hfh_vec_set_values(self.h, _p(index), len(index), _p(values)))
+
+    def get_values(self, index):
+        index = np.ascontiguousarray(index, np.int32)
+        out = np.empty(len(index), self.dtype)
+        _ck(hip().hfh_vec_get_values(self.h, _p(index), len(index), _p(out)))
+        return out
+
+    def zeros(self):
+        _ck(hip().hfh_vec_zeros(self.h))
+
+    def copy_from(self, other):
+        _ck(hip().hfh_vec_copy(self.h, other.h))
+
+    def copy_range(self, dst_off, src, src_off, count):
+        _ck(hip().hfh_vec_copy_range(self.h, dst_off, src.h, src_off, count))
+
+    def cast_from(self, other):
+        _ck(hip().hfh_vec_cast(self.h, other.h))
+
+    def reorder_from(self, other, perm):
+        perm = np.ascontiguousarray(perm, np.int32)
+        _ck(hip().hfh_vec_reorder(self.h, other.h, _p(perm)))
+
+    def dot(self, other):
+        d = C.c_double()
+        _ck(hip().hfh_dot(self.h, other.h, C.byref(d)))
+        return d.value
+
+    def nrm2(self):
+        d = C.c_double()
+        _ck(hip().hfh_nrm2(self.h, C.byref(d)))
+        return d.value
+
+    def axpy(self, x, a):
+        _ck(hip().hfh_axpy(self.h, x.h, float(a)))
+
+    def scaleadd(self, a, x):
+        _ck(hip().hfh_scaleadd(self.h, float(a), x.h))
+
+    def scale(self, a):
+        _ck(hip().hfh_scale(self.h, float(a)))
+
+    def ewmult(self, x):
+        _ck(hip().hfh_ewmult(self.h, x.h))
+
+
+class CsrMatrix:
+    def __init__(self, ctx, row, col, val, ncol=None, dtype=None):
+        self.ctx = ctx
+        row = np.ascontiguousarray(row, np.int32)
+        col = np.ascontiguousarray(col, np.int32)
+        dtype = np.dtype(dtype or np.asarray(val).dtype)
+        val = np.ascontiguousarray(val, dtype)
+        self.nrow = len(row) - 1
+        self.ncol = self.nrow if ncol is None else ncol
+        self.nnz = len(col)
+        self.dtype = dtype
+        h = C.c_void_p()
+        _ck(hip().hfh_csr_create(ctx.h, self.nrow, self.ncol, self.nnz, _dt(dtype), C.byref(h)))
+        self.h = h
+        _ck(hip().hfh_csr_upload(self.h, _p(row), _p(col), _p(val)))
+
+    @classmethod
+    def cast_of(cls, other, dtype):
+        self = cls.__new__(cls)
+        self.ctx, self.nrow, self.ncol, self.nnz = other.ctx, other.nrow, other.ncol, other.nnz
+        self.dtype = np.dtype(dtype)
+        h = C.c_void_p()
+        _ck(hip().hfh_csr_create(other.ctx.h, self.nrow, self.ncol, self.nnz, _dt(dtype), C.byref(h)))
+        self.h = h
+        _ck(hip().hfh_csr_cast(self.h, other.h))
+        return self
+
+    def __del__(self):
+        try:
+            if self.h:
+                hip().hfh_csr_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def spmv(self, x, y):
+        _ck(hip().hfh_spmv(self.h, x.h, y.h))
+
+    def spmv_add(self, x, y):
+        _ck(hip().hfh_spmv_add(self.h, x.h, y.h))
+
+    def invdiag(self):
+        v = Vector(self.ctx, self.nrow, self.dtype)
+        _ck(hip().hfh_csr_invdiag(self.h, v.h))
+        return v
+
+
+class Preconditioner:
+    def __init__(self, h, keep=()):
+        self.h = h
+        self._keep = keep
+
+    def __del__(self):
+        try:
+            if self.h:
+                hip().hfh_prec_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    @staticmethod
+    def jacobi(A):
+        h = C.c_void_p()
+        _ck(hip().hfh_jacobi_create(A.h, C.byref(h)))
+        return Preconditioner(h, (A,))
+
+    @staticmethod
+    def mcsgs(A, color_sizes, omega=1.0):
+        sizes = np.ascontiguousarray(color_sizes, np.int32)
+        h = C.c_void_p()
+        _ck(hip().hfh_mcsgs_create(A.h, len(sizes), _p(sizes), float(omega), C.byref(h)))
+        return Preconditioner(h, (A,))
+
+    @staticmethod
+    def _trsv(fn, A, row, col):
+        lo_ptr, lo_rows = levels(row, col, upper=False)
+        up_ptr, up_rows = levels(row, col, upper=True)
+        h = C.c_void_p()
+        _ck(fn(A.h, len(lo_ptr) - 1, _p(lo_ptr), _p(lo_rows), len(up_ptr) - 1, _p(up_ptr), _p(up_rows),
+               C.byref(h)))
+        return Preconditioner(h, (A,))
+
+    @staticmethod
+    def sgs(A, row, col):
+        """Natural-order symmetric Gauss-Seidel; row/col: host copy of A's pattern."""
+        return Preconditioner._trsv(hip().hfh_sgs_create, A, row, col)
+
+    @staticmethod
+    def ilu0(LU, row, col):
+        """ILU(0) apply; LU holds the factors (see ilu0_factor)."""
+        return Preconditioner._trsv(hip().hfh_ilu0_create, LU, row, col)
+
+    def apply(self, x, y):
+        _ck(hip().hfh_prec_apply(self.h, x.h, y.h))
+
+
+def _info(inf):
+    return {"iters": inf.iters, "state": inf.state, "res": inf.res, "res0": inf.res0, "time_ms": inf.time_ms}
+
+
+def pcg(A, b, x, P=None, maxits=1000, atol=1e-15, rtol=1e-8, dtol=1e6, history=False, check=True):
+    """CG<LAD>::Solve on the device; returns the info dict (and history)."""
+    ctl = Control(maxits, atol, rtol, dtol)
+    inf = SolveInfo()
+    hist = np.zeros(maxits + 2) if history else None
+    rc = hip().hfh_pcg_solve(A.h, P.h if P else None, b.h, x.h, C.byref(ctl), C.byref(inf), _p(hist))
+    if rc != OK and (check or rc != ERR_NOT_CONVERGED):
+        _ck(rc)
+    out = _info(inf)
+    out["rc"] = rc
+    if history:
+        out["hist"] = hist[: inf.iters + 1].copy()
+    return out
+
+
+def gmres(A, b, x, P=None, m=30, flexible=False, maxits=1000, atol=1e-15, rtol=1e-8, dtol=1e6,
+          history=False, check=True):
+    ctl = Control(maxits, atol, rtol, dtol)
+    inf = SolveInfo()
+    hist = np.zeros(maxits + 2) if history else None
+    rc = hip().hfh_gmres_solve(A.h, P.h if P else None, m, 1 if flexible else 0, b.h, x.h, C.byref(ctl),
+                               C.byref(inf), _p(hist))
+    if rc != OK and (check or rc != ERR_NOT_CONVERGED):
+        _ck(rc)
+    out = _info(inf)
+    out["rc"] = rc
+    if history:
+        out["hist"] = hist[: inf.iters + 1].copy()
+    return out
+
+
+def dc(Ah, Al, b, x, Pl=None, outer=(20, 1e-10, 0.0, 1e6), inner=(1000, 0.0, 1e-4, 1e6), check=True):
+    """FP64 defect correction with FP32 inner PCG (MPIR<D,S>::Solve)."""
+    oc, ic = Control(*outer), Control(*inner)
+    inf = DcInfo()
+    hist = np.zeros(outer[0] + 2)
+    rc = hip().hfh_dc_solve(Ah.h, Al.h, Pl.h if Pl else None, b.h, x.h, C.byref(oc), C.byref(ic), C.byref(inf),
+                            _p(hist))
+    if rc != OK and (check or rc != ERR_NOT_CONVERGED):
+        _ck(rc)
+    return {"outer_iters": inf.outer_iters, "inner_iters": inf.inner_iters, "state": inf.state, "res": inf.res,
+            "res0": inf.res0, "time_ms": inf.time_ms, "rc": rc, "hist": hist[: inf.outer_iters + 1].copy()}
+
+
+# ---------------------------------------------------------------- host layer
+
+def problem(kind, nx, ny=None, nz=None, p0=1.0, p1=0.0, seed=0):
+    """Generate a seeded CSR system: returns (row, col, val, rhs)."""
+    ny = nx if ny is None else ny
+    nz = (1 if kind == Q1_2D else nx) if nz is None else nz
+    n, nnz = C.c_int(), C.c_longlong()
+    rc = lmp().hfl_problem_size(kind, nx, ny, nz, p0, p1, seed, C.byref(n), C.byref(nnz))
+    if rc:
+        raise ValueError("invalid problem specification")
+    row = np.empty(n.value + 1, np.int32)
+    col = np.empty(nnz.value, np.int32)
+    val = np.empty(nnz.value, np.float64)
+    rhs = np.empty(n.value, np.float64)
+    if lmp().hfl_problem_fill(kind, nx, ny, nz, p0, p1, seed, _p(row), _p(col), _p(val), _p(rhs)):
+        raise ValueError("problem generation failed")
+    return row, col, val, rhs
+
+
+def multicoloring(row, col):
+    n = len(row) - 1
+    nc = C.c_int()
+    color = np.empty(n, np.int32)
+    sizes = np.empty(max(n, 1), np.int32)
+    perm = np.empty(n, np.int32)
+    if lmp().hfl_multicoloring(n, _p(row), _p(col), C.byref(nc), _p(color), _p(sizes), _p(perm)):
+        raise ValueError("multicoloring failed")
+    return nc.value, color, sizes[: nc.value].copy(), perm
+
+
+def levels(row, col, upper=False):
+    row = np.ascontiguousarray(row, np.int32)
+    col = np.ascontiguousarray(col, np.int32)
+    n = len(row) - 1
+    nl = C.c_int()
+    ptr = np.empty(n + 1, np.int32)
+    rows = np.empty(n, np.int32)
+    fn = lmp().hfl_levels_upper if upper else lmp().hfl_levels_lower
+    if fn(n, _p(row), _p(col), C.byref(nl), _p(ptr), _p(rows)):
+        raise ValueError("level scheduling failed")
+    return ptr[: nl.value + 1].copy(), rows
+
+
+def ilu0_factor(row, col, val):
+    lu = np.array(val, np.float64)
+    miss = lmp().hfl_ilu0(len(row) - 1, _p(np.ascontiguousarray(row, np.int32)),
+                          _p(np.ascontiguousarray(col, np.int32)), _p(lu))
+    if miss:
+        raise HfhError(ERR_ZERO_PIVOT, f"ILU(0): {miss} missing diagonal entries")
+    return lu
+
+
+def csr_permute(row, col, val, perm):
+    n = len(row) - 1
+    row = np.ascontiguousarray(row, np.int32)
+    col = np.ascontiguousarray(col, np.int32)
+    val = np.ascontiguousarray(val, np.float64)
+    perm = np.ascontiguousarray(perm, np.int32)
+    prow = np.empty(n + 1, np.int32)
+    pcol = np.empty(len(col), np.int32)
+    pval = np.empty(len(col), np.float64)
+    if lmp().hfl_csr_permute(n, _p(row), _p(col), _p(val), _p(perm), _p(prow), _p(pcol), _p(pval)):
+        raise ValueError("invalid permutation")
+    return prow, pcol, pval

@@ -1,0 +1,206 @@
+/*
+ * hfh.h -- C ABI of the MI355X (gfx950) HIP platform for HiFlow3's lmp layer.
+ *
+ * Plain pointers and sizes only.  Every function returns an int status
+ * (HFH_OK == 0, negative codes on failure) and never exits the process;
+ * hfh_last_error() returns the message of the calling thread's last failure.
+ * All device work of a context is ordered on the context's single HIP stream;
+ * calls that return a host value (dot, nrm2, downloads, solves) synchronise.
+ *
+ * Each entry point names the reference interface it replaces
+ * (paths relative to the HiFlow3 2.0 source tree, src/).
+ */
+#ifndef HFH_H
+#define HFH_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct hfh_ctx hfh_ctx;
+typedef struct hfh_vec hfh_vec;
+typedef struct hfh_csr hfh_csr;
+typedef struct hfh_prec hfh_prec;
+
+enum hfh_status {
+    HFH_OK = 0,
+    HFH_ERR_INVALID = -1,       /* bad argument or inconsistent sizes */
+    HFH_ERR_NOMEM = -2,         /* device allocation failed */
+    HFH_ERR_HIP = -3,           /* HIP runtime error */
+    HFH_ERR_ZERO_PIVOT = -4,    /* zero / missing diagonal entry */
+    HFH_ERR_NOT_CONVERGED = -5, /* solver stopped on maxits or divergence */
+    HFH_ERR_BUDGET = -6,        /* serving admission over the HBM budget */
+    HFH_ERR_UNSUPPORTED = -7    /* dtype / option combination not provided */
+};
+
+enum hfh_dtype { HFH_F64 = 0, HFH_F32 = 1 };
+
+/* IterateControl states (common/iterate_control.h) */
+enum hfh_iter_state {
+    HFH_ITERATE = 0, HFH_SUCCESS_ABS = 1, HFH_SUCCESS_REL = 2,
+    HFH_FAIL_DIV = 3, HFH_FAIL_MAXITS = 4
+};
+
+enum hfh_prec_kind {
+    HFH_PREC_NONE = 0, HFH_PREC_JACOBI = 1, HFH_PREC_SGS = 2,
+    HFH_PREC_MCSGS = 3, HFH_PREC_ILU0 = 4
+};
+
+const char *hfh_last_error(void);
+const char *hfh_version(void);
+
+/* ---- context: platform init (lmp/platform_management.cc:135-187) ---- */
+int hfh_device_count(int *count);
+int hfh_ctx_create(int device, hfh_ctx **ctx);
+int hfh_ctx_destroy(hfh_ctx *ctx);
+int hfh_ctx_sync(hfh_ctx *ctx);
+/* device properties used by the launch heuristics */
+int hfh_ctx_info(hfh_ctx *ctx, int *num_cu, long long *hbm_bytes, char *name, int name_len);
+/* per-kernel-class HIP-event timing; class: 0 = SpMV, 1 = PCG step, 2 = precond */
+int hfh_ctx_set_timing(hfh_ctx *ctx, int on);
+int hfh_ctx_timing(hfh_ctx *ctx, int kclass, long long *launches, double *total_ms);
+int hfh_ctx_timing_reset(hfh_ctx *ctx);
+
+/* ---- vectors: lVector<T> (lmp/lvector.h:39-336) ---- */
+/* lVector::Init (lvector.h:199) + CloneWithoutContent (lvector.cc:52-64): zero filled */
+int hfh_vec_create(hfh_ctx *ctx, int n, int dtype, hfh_vec **v);
+int hfh_vec_destroy(hfh_vec *v);
+int hfh_vec_size(const hfh_vec *v, int *n);
+int hfh_vec_dtype(const hfh_vec *v, int *dtype);
+/* host <-> device: SetBlockValues / GetBlockValues (lvector.h:112-126),
+ * CopyFrom / CopyTo across platforms (lvector_gpu.cc:389-435) */
+int hfh_vec_upload(hfh_vec *v, const void *host, int offset, int count);
+int hfh_vec_download(const hfh_vec *v, void *host, int offset, int count);
+/* SetValues / GetValues with a host index set (lvector.h:93-110) */
+int hfh_vec_set_values(hfh_vec *v, const int *index, int count, const void *values);
+int hfh_vec_get_values(const hfh_vec *v, const int *index, int count, void *values);
+/* Zeros (lvector.h:206) */
+int hfh_vec_zeros(hfh_vec *v);
+/* CopyFrom (lvector.h:222), same platform */
+int hfh_vec_copy(hfh_vec *dst, const hfh_vec *src);
+/* partial_replace_subvector (lvector.h:177-181):
+ * dst[dst_off + i] = src[src_off + i], i < count */
+int hfh_vec_copy_range(hfh_vec *dst, int dst_off, const hfh_vec *src, int src_off, int count);
+/* CastFrom (lvector.h:231-232): f64 <-> f32 with round-to-nearest */
+int hfh_vec_cast(hfh_vec *dst, const hfh_vec *src);
+/* Reorder (lvector.h:216): dst[i] = src[perm[i]], perm on the host */
+int hfh_vec_reorder(hfh_vec *dst, const hfh_vec *src, const int *perm);
+
+/* ---- BLAS-1 (CPUsimple_lvector.cc:142-435; lvector.h:262-291) ---- */
+/* Dot: blocking, returns the host scalar (CoupledVector::Dot, coupled_vector.cc:330-344) */
+int hfh_dot(const hfh_vec *x, const hfh_vec *y, double *out);
+/* Norm2 = sqrt(Dot(self)) (CPUsimple_lvector.cc:142-152) */
+int hfh_nrm2(const hfh_vec *x, double *out);
+/* Axpy: y += a*x (CPUsimple_lvector.cc:234-287) */
+int hfh_axpy(hfh_vec *y, const hfh_vec *x, double a);
+/* ScaleAdd: y = a*y + x (CPUsimple_lvector.cc:310-365) */
+int hfh_scaleadd(hfh_vec *y, double a, const hfh_vec *x);
+/* Scale: y *= a (CPUsimple_lvector.cc:388-435) */
+int hfh_scale(hfh_vec *y, double a);
+/* ElementWiseMult: y .*= x (lvector_cpu.cc:571-578) */
+int hfh_ewmult(hfh_vec *y, const hfh_vec *x);
+
+/* ---- CSR matrices: CSR_lMatrixType<T> (lmp/lmatrix_formats.h:40-46) ---- */
+/* lMatrix::Init (lmatrix.h:107) */
+int hfh_csr_create(hfh_ctx *ctx, int nrow, int ncol, int nnz, int dtype, hfh_csr **A);
+int hfh_csr_destroy(hfh_csr *A);
+int hfh_csr_dims(const hfh_csr *A, int *nrow, int *ncol, int *nnz, int *dtype);
+/* CopyStructureFrom + CopyFrom from a host CSR (lmatrix_csr_gpu.cc:140-192).
+ * Validates row[0]==0, monotone rows, row[nrow]==nnz, columns in range and
+ * strictly ascending within each row (lmatrix_csr_cpu.cc:1017-1044). */
+int hfh_csr_upload(hfh_csr *A, const int *row, const int *col, const void *val);
+/* CopyFrom with an unchanged structure: values only */
+int hfh_csr_upload_values(hfh_csr *A, const void *val);
+int hfh_csr_download(const hfh_csr *A, int *row, int *col, void *val);
+/* CastFrom (lmatrix.h:297-298): dst takes src's structure, values converted */
+int hfh_csr_cast(hfh_csr *dst, const hfh_csr *src);
+/* VectorMult: y = A x (lmatrix.h:331; CPUsimple_csr_lmatrix.cc:75-105) */
+int hfh_spmv(const hfh_csr *A, const hfh_vec *x, hfh_vec *y);
+/* VectorMultAdd: y += A x (lmatrix.h:336; lmatrix_csr_cpu.cc:1354-1375) */
+int hfh_spmv_add(const hfh_csr *A, const hfh_vec *x, hfh_vec *y);
+/* extract_invdiagelements over all rows (lmatrix_csr_cpu.cc:2414-2432) */
+int hfh_csr_invdiag(const hfh_csr *A, hfh_vec *invd);
+
+/* ---- preconditioners: lPreconditioner<T> (lmp/lpreconditioner.h:35-125) ---- */
+/* lPreconditioner_Jacobi::Build (lpreconditioner.cc:127-148) */
+int hfh_jacobi_create(const hfh_csr *A, hfh_prec **P);
+/* lPreconditioner_SymmetricGaussSeidel (lpreconditioner.cc:221-230 ->
+ * lmatrix_csr_cpu.cc:542-616), natural order, solved level by level.
+ * level_ptr[nlev+1], level_rows[nrow]: forward levels (Levelscheduling,
+ * lmatrix_csr_cpu.cc:2295-2391); the backward sweep uses its own levels
+ * computed from the upper triangle on the host. */
+int hfh_sgs_create(const hfh_csr *A, int nlev, const int *level_ptr, const int *level_rows,
+                   int nlev_up, const int *up_level_ptr, const int *up_level_rows,
+                   hfh_prec **P);
+/* lPreconditioner_MultiColoring_SymmetricGaussSeidel (lpreconditioner_mc_sgs.cc,
+ * lpreconditioner_mc.cc:157-770) on the colour-permuted matrix A.  Colour c
+ * holds rows [sum(color_sizes[<c]), +color_sizes[c]).  Intra-colour
+ * off-diagonals are ignored (dropoffs). */
+int hfh_mcsgs_create(const hfh_csr *A, int ncolors, const int *color_sizes, double omega,
+                     hfh_prec **P);
+/* lPreconditioner_ILUp with p = 0 (lpreconditioner.cc:388-425): LU holds the
+ * ILU(0) factors (CPU_CSR_lMatrix::ilu0, lmatrix_csr_cpu.cc:1880-1930, computed
+ * on the host); the apply is ilu_solve (lmatrix_csr_cpu.cc:2108-2171) level by
+ * level, with the forward and backward level sets given as for SGS. */
+int hfh_ilu0_create(const hfh_csr *LU, int nlev, const int *level_ptr, const int *level_rows,
+                    int nlev_up, const int *up_level_ptr, const int *up_level_rows,
+                    hfh_prec **P);
+/* ApplylPreconditioner (lpreconditioner.h): out = M^{-1} in */
+int hfh_prec_apply(hfh_prec *P, const hfh_vec *in, hfh_vec *out);
+int hfh_prec_kind(const hfh_prec *P, int *kind);
+int hfh_prec_destroy(hfh_prec *P);
+
+/* ---- device-resident Krylov drivers ---- */
+typedef struct {
+    int maxits;
+    double atol, rtol, dtol;   /* IterateControl (iterate_control.cc:84-110) */
+} hfh_control;
+
+typedef struct {
+    int iters;       /* LinearSolver::iter() */
+    int state;       /* hfh_iter_state at exit */
+    double res;      /* last residual given to IterateControl */
+    double res0;     /* first residual */
+    double time_ms;  /* device time of the solve (HIP events) */
+} hfh_solve_info;
+
+/* CG<LAD>::SolvePrecond / SolveNoPrecond (linear_solver/cg.cc:84-248) with
+ * the scalars kept on the device and the loop launched in chunks; same
+ * iterates and stopping test as the reference.  P may be NULL (no
+ * preconditioning), Jacobi or MC-SGS.  Returns HFH_OK on convergence,
+ * HFH_ERR_NOT_CONVERGED on maxits/divergence (info is filled either way).
+ * hist (optional, host, maxits+1 doubles) receives the residual history. */
+int hfh_pcg_solve(const hfh_csr *A, hfh_prec *P, const hfh_vec *b, hfh_vec *x,
+                  const hfh_control *ctl, hfh_solve_info *info, double *hist);
+
+/* GMRES<LAD>::SolveRight (gmres.cc:285-510) / FGMRES<LAD>::SolveRight
+ * (fgmres.cc:75-262) with right preconditioning, modified Gram-Schmidt and
+ * Givens rotations; the Hessenberg scalars live on the device.  flexible
+ * selects FGMRES.  P may be NULL. */
+int hfh_gmres_solve(const hfh_csr *A, hfh_prec *P, int size_basis, int flexible,
+                    const hfh_vec *b, hfh_vec *x, const hfh_control *ctl,
+                    hfh_solve_info *info, double *hist);
+
+/* Mixed-precision defect correction (MPIR<D,S>::Solve, mpir.cc:69-327):
+ * FP64 true residual, FP64 update and FP64 outer test on the device; FP32
+ * matrix values and FP32 inner PCG with preconditioner Pl (Jacobi or MC-SGS
+ * built on Al).  Outer stop: true residual < outer.atol, or
+ * < outer.rtol * previous outer residual (the reference res0_ reset), or
+ * outer.maxits corrections.  inner: control of each FP32 inner solve. */
+typedef struct {
+    int outer_iters;     /* corrections applied */
+    int inner_iters;     /* accumulated inner iterations */
+    int state;           /* HFH_SUCCESS_ABS / _REL, or HFH_FAIL_* */
+    double res;          /* final true FP64 residual norm */
+    double res0;         /* initial true residual norm */
+    double time_ms;
+} hfh_dc_info;
+
+int hfh_dc_solve(const hfh_csr *Ah, const hfh_csr *Al, hfh_prec *Pl, const hfh_vec *b,
+                 hfh_vec *x, const hfh_control *outer, const hfh_control *inner,
+                 hfh_dc_info *info, double *hist);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

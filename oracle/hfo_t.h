@@ -1,0 +1,526 @@
+/*
+ * hfo_t.h -- type-generic body of the CPU oracle (TEST INFRASTRUCTURE ONLY).
+ * Included by hfo.c once per value type with T (value type) and S (suffix)
+ * defined.  Each routine follows the cited reference loop order.
+ */
+#define CAT_(a, b) a##b
+#define CAT(a, b) CAT_(a, b)
+#define FN(name) CAT(name, S)
+#define PREC CAT(hfo_prec_, S)
+
+/* CPUsimple_CSR_lMatrix::VectorMult (CPUsimple_csr_lmatrix.cc:75-105):
+ * res = 0; res += val[j] * x[col[j]] over j ascending; y[i] = res. */
+void FN(hfo_spmv_)(int n, const int *row, const int *col, const T *val, const T *x, T *y)
+{
+    for (int i = 0; i < n; ++i) {
+        T res = 0;
+        for (int j = row[i]; j < row[i + 1]; ++j)
+            res += val[j] * x[col[j]];
+        y[i] = res;
+    }
+}
+
+/* CPUopenmp_CSR_lMatrix::VectorMult (openmp/CPUopenmp_csr_lmatrix.cc:144-165):
+ * the same per-row arithmetic, rows split by `omp parallel for`. */
+void FN(hfo_spmv_omp_)(int n, const int *row, const int *col, const T *val, const T *x, T *y)
+{
+#pragma omp parallel for
+    for (int i = 0; i < n; ++i) {
+        T res = 0;
+        for (int j = row[i]; j < row[i + 1]; ++j)
+            res += val[j] * x[col[j]];
+        y[i] = res;
+    }
+}
+
+/* CPU_CSR_lMatrix::VectorMultAdd (lmatrix_csr_cpu.cc:1354-1375): y[i] += row sum */
+void FN(hfo_spmv_add_)(int n, const int *row, const int *col, const T *val, const T *x, T *y)
+{
+    for (int i = 0; i < n; ++i) {
+        T res = 0;
+        for (int j = row[i]; j < row[i + 1]; ++j)
+            res += val[j] * x[col[j]];
+        y[i] += res;
+    }
+}
+
+/* CPUsimple_lVector::Dot (CPUsimple_lvector.cc:160-209): unroll by 5, the
+ * N mod 5 remainder is summed first, then each group of five products is
+ * summed left to right and added to the running total. */
+T FN(hfo_dot_)(int n, const T *x, const T *y)
+{
+    T d = 0;
+    const int m = n % 5;
+    if (m == 0) {
+        for (int i = 0; i < n; i += 5)
+            d += x[i] * y[i] + x[i + 1] * y[i + 1] + x[i + 2] * y[i + 2] +
+                 x[i + 3] * y[i + 3] + x[i + 4] * y[i + 4];
+    } else {
+        for (int i = 0; i != m; ++i)
+            d += x[i] * y[i];
+        if (n > 5)
+            for (int i = m; i < n; i += 5)
+                d += x[i] * y[i] + x[i + 1] * y[i + 1] + x[i + 2] * y[i + 2] +
+                     x[i + 3] * y[i + 3] + x[i + 4] * y[i + 4];
+    }
+    return d;
+}
+
+/* CPUsimple_lVector::Norm2 (CPUsimple_lvector.cc:142-152): sqrt(Dot(self)) */
+T FN(hfo_nrm2_)(int n, const T *x)
+{
+    return (T)sqrt(FN(hfo_dot_)(n, x, x));
+}
+
+/* CPUsimple_lVector::Axpy (CPUsimple_lvector.cc:234-287): y += a*x */
+void FN(hfo_axpy_)(int n, T *y, const T *x, T a)
+{
+    for (int i = 0; i < n; ++i)
+        y[i] += a * x[i];
+}
+
+/* CPUsimple_lVector::ScaleAdd (CPUsimple_lvector.cc:310-365): y = a*y + x */
+void FN(hfo_scaleadd_)(int n, T *y, T a, const T *x)
+{
+    for (int i = 0; i < n; ++i)
+        y[i] = a * y[i] + x[i];
+}
+
+/* CPUsimple_lVector::Scale (CPUsimple_lvector.cc:388-435): y *= a */
+void FN(hfo_scale_)(int n, T *y, T a)
+{
+    for (int i = 0; i < n; ++i)
+        y[i] *= a;
+}
+
+/* CPU_lVector::ElementWiseMult (lvector_cpu.cc:571-578): y .*= x */
+void FN(hfo_ewmult_)(int n, T *y, const T *x)
+{
+    for (int i = 0; i < n; ++i)
+        y[i] *= x[i];
+}
+
+/* CPU_CSR_lMatrix::extract_invdiagelements (lmatrix_csr_cpu.cc:2414-2432) */
+void FN(hfo_invdiag_)(int n, const int *row, const int *col, const T *val, T *invd)
+{
+    for (int i = 0; i < n; ++i)
+        for (int j = row[i]; j < row[i + 1]; ++j)
+            if (col[j] == i)
+                invd[i] = 1 / val[j];
+}
+
+/* CPU_CSR_lMatrix::Psgauss_seidel (lmatrix_csr_cpu.cc:542-616):
+ * (D+L) y = r sequential forward (stop at the diagonal), y <- D y, then
+ * (D+U) z = D y sequential backward. */
+void FN(hfo_sgs_)(int n, const int *row, const int *col, const T *val, const T *in, T *out)
+{
+    int last_j = 0;
+    for (int i = 0; i < n; ++i) {
+        out[i] = in[i];
+        for (int j = row[i]; j < row[i + 1]; ++j) {
+            if (i > col[j])
+                out[i] -= out[col[j]] * val[j];
+            if (i == col[j]) {
+                last_j = j;
+                break;
+            }
+        }
+        out[i] /= val[last_j];
+    }
+    for (int i = 0; i < n; ++i)
+        for (int j = row[i]; j < row[i + 1]; ++j)
+            if (i == col[j]) {
+                out[i] *= val[j];
+                break;
+            }
+    for (int i = n - 1; i >= 0; --i) {
+        for (int j = row[i]; j < row[i + 1]; ++j) {
+            if (i == col[j])
+                last_j = j;
+            if (i < col[j])
+                out[i] -= out[col[j]] * val[j];
+        }
+        out[i] /= val[last_j];
+    }
+}
+
+/* CPU_CSR_lMatrix::ilu0 (lmatrix_csr_cpu.cc:1880-1930): IKJ ILU(0) in place
+ * on a column-sorted CSR pattern.  Returns the number of rows k whose
+ * diagonal entry was not found (the reference only logs them).  The
+ * reference's progress print divides by (n-1-(n-1)%20)/20, which is zero for
+ * n < 21; that logging expression is not restated. */
+int FN(hfo_ilu0_)(int n, const int *row, const int *col, T *val)
+{
+    int diag_k = 0, missing = 0;
+    for (int i = 1; i < n; ++i) {
+        for (int ik = row[i]; col[ik] < i; ++ik) {
+            const int k = col[ik];
+            for (int p = row[k]; p < row[k + 1]; ++p)
+                if (col[p] == k)
+                    diag_k = p;
+            if (col[diag_k] != k)
+                ++missing;
+            val[ik] = val[ik] / (1.0 * val[diag_k]);
+            for (int ij = ik + 1; ij < row[i + 1]; ++ij) {
+                T a_kj = 0;
+                for (int p = row[k]; p < row[k + 1]; ++p)
+                    if (col[p] == col[ij])
+                        a_kj = val[p];
+                val[ij] = val[ij] - val[ik] * a_kj;
+            }
+        }
+    }
+    return missing;
+}
+
+/* CPU_CSR_lMatrix::ilu_solve (lmatrix_csr_cpu.cc:2108-2171): unit lower
+ * forward sweep, then (D+U) backward sweep divided by the diagonal. */
+void FN(hfo_ilu_solve_)(int n, const int *row, const int *col, const T *lu, const T *in, T *out)
+{
+    int last_j = 0;
+    for (int i = 0; i < n; ++i) {
+        out[i] = in[i];
+        for (int j = row[i]; j < row[i + 1]; ++j) {
+            if (i > col[j])
+                out[i] -= out[col[j]] * lu[j];
+            if (i == col[j]) {
+                last_j = j;
+                break;
+            }
+        }
+    }
+    for (int i = n - 1; i >= 0; --i) {
+        for (int j = row[i]; j < row[i + 1]; ++j) {
+            if (i == col[j])
+                last_j = j;
+            if (i < col[j])
+                out[i] -= out[col[j]] * lu[j];
+        }
+        out[i] /= lu[last_j];
+    }
+    (void)last_j;
+}
+
+/* lPreconditioner_MultiColoring_SymmetricGaussSeidel apply on the colour
+ * permuted matrix (lpreconditioner_mc.cc:716-770, lpreconditioner_mc_gs.cc:
+ * 91-113, lpreconditioner_mc_sgs.cc:46-90).  The L/R colour blocks carry the
+ * off-diagonals scaled by omega (ScaleOffdiag, lpreconditioner_mc.cc:486-488);
+ * iDv = -1/D, Dv = -D/(omega(2-omega)).  Each block product is one
+ * VectorMultAdd: the row sum over that block's columns is formed first and
+ * then added to the chunk.  Intra-colour off-diagonals are ignored (the
+ * reference drops them). */
+void FN(hfo_mcsgs_)(int n, const int *row, const int *col, const T *val, int ncolors,
+                    const int *color_sizes, T omega, const T *in, T *out)
+{
+    int *off = (int *)malloc(sizeof(int) * (ncolors + 1));
+    off[0] = 0;
+    for (int c = 0; c < ncolors; ++c)
+        off[c + 1] = off[c] + color_sizes[c];
+    T *d = (T *)malloc(sizeof(T) * (n > 0 ? n : 1));
+    for (int i = 0; i < n; ++i) {
+        d[i] = 0;
+        for (int j = row[i]; j < row[i + 1]; ++j)
+            if (col[j] == i)
+                d[i] = val[j];
+    }
+    const T relax = (T)(1.0 / (omega * (2.0 - omega)));
+    /* forward: chunk_i = -r_i; chunk_i += L_ij chunk_j (j<i); chunk_i *= -1/D */
+    for (int c = 0; c < ncolors; ++c) {
+        for (int i = off[c]; i < off[c + 1]; ++i)
+            out[i] = in[i] * (T)(-1.0);
+        for (int cj = 0; cj < c; ++cj)
+            for (int i = off[c]; i < off[c + 1]; ++i) {
+                T res = 0;
+                for (int j = row[i]; j < row[i + 1]; ++j)
+                    if (col[j] >= off[cj] && col[j] < off[cj + 1])
+                        res += (val[j] * omega) * out[col[j]];
+                out[i] += res;
+            }
+        for (int i = off[c]; i < off[c + 1]; ++i)
+            out[i] *= (T)1 / d[i] * (T)(-1.0);
+    }
+    /* diagonal: chunk_i *= -D_i * relax */
+    for (int i = 0; i < n; ++i)
+        out[i] *= (d[i] * (T)(-1.0)) * relax;
+    /* backward: chunk_i += R_ij chunk_j (j>i, descending); chunk_i *= -1/D */
+    for (int c = ncolors - 1; c >= 0; --c) {
+        for (int cj = ncolors - 1; cj > c; --cj)
+            for (int i = off[c]; i < off[c + 1]; ++i) {
+                T res = 0;
+                for (int j = row[i]; j < row[i + 1]; ++j)
+                    if (col[j] >= off[cj] && col[j] < off[cj + 1])
+                        res += (val[j] * omega) * out[col[j]];
+                out[i] += res;
+            }
+        for (int i = off[c]; i < off[c + 1]; ++i)
+            out[i] *= (T)1 / d[i] * (T)(-1.0);
+    }
+    free(d);
+    free(off);
+}
+
+/* Preconditioner dispatch (PreconditionerBlockJacobiStand::ApplyPreconditioner,
+ * preconditioner_bjacobi_standard.cc:138-150; PreconditionerMultiColoring::
+ * ApplyPreconditioner, preconditioner_mc.cc:141-153).  Jacobi is
+ * CopyFrom + ElementWiseMult (lpreconditioner.cc:151-166). */
+void FN(hfo_prec_apply_)(int n, const PREC *P, const T *in, T *out)
+{
+    switch (P ? P->kind : HFO_PREC_NONE) {
+    case HFO_PREC_JACOBI:
+        memcpy(out, in, sizeof(T) * n);
+        FN(hfo_ewmult_)(n, out, P->invd);
+        break;
+    case HFO_PREC_SGS:
+        FN(hfo_sgs_)(n, P->row, P->col, P->val, in, out);
+        break;
+    case HFO_PREC_MCSGS:
+        FN(hfo_mcsgs_)(n, P->row, P->col, P->val, P->ncolors, P->color_sizes, P->omega, in, out);
+        break;
+    case HFO_PREC_ILU0:
+        FN(hfo_ilu_solve_)(n, P->row, P->col, P->val, in, out);
+        break;
+    default:
+        memcpy(out, in, sizeof(T) * n);
+        break;
+    }
+}
+
+/* IterateControl::Check (iterate_control.cc:84-110) */
+static int FN(check_)(const hfo_ctrl *c, int step, double value, double *first)
+{
+    if (step == 0)
+        *first = value;
+    if (value < c->atol)
+        return HFO_SUCCESS_ABS;
+    if (value < c->rtol * *first)
+        return HFO_SUCCESS_REL;
+    if (value > c->dtol * *first)
+        return HFO_FAIL_DIV;
+    if (step >= c->maxits)
+        return HFO_FAIL_MAXITS;
+    return HFO_ITERATE;
+}
+
+static int FN(retcode_)(int st)
+{
+    return (st == HFO_FAIL_DIV || st == HFO_FAIL_MAXITS) ? HFO_SOLVER_EXCEEDED
+                                                         : HFO_SOLVER_SUCCESS;
+}
+
+/* CG<LAD>::SolvePrecond / SolveNoPrecond (cg.cc:84-248).  `hist` (optional)
+ * receives the residual passed to IterateControl at each step. */
+int FN(hfo_cg_)(int n, const int *row, const int *col, const T *val, const T *b, T *x,
+                const PREC *P, const hfo_ctrl *ctrl, hfo_result *out, double *hist)
+{
+    const int precond = P && P->kind != HFO_PREC_NONE;
+    T *r = (T *)calloc(n, sizeof(T)), *p = (T *)calloc(n, sizeof(T));
+    T *z = (T *)calloc(n, sizeof(T)), *Ap = (T *)calloc(n, sizeof(T));
+    T alpha, beta, gamma, rr;
+    double first = 0, res;
+    int iter = 0, st;
+
+    FN(hfo_spmv_)(n, row, col, val, x, r);
+    FN(hfo_scaleadd_)(n, r, (T)-1., b);
+    rr = FN(hfo_dot_)(n, r, r);
+    res = (T)sqrt(rr);
+    st = FN(check_)(ctrl, iter, res, &first);
+    if (hist)
+        hist[0] = res;
+    if (precond) {
+        memset(z, 0, sizeof(T) * n);
+        FN(hfo_prec_apply_)(n, P, r, z);
+        memcpy(p, z, sizeof(T) * n);
+        beta = FN(hfo_dot_)(n, r, z);
+    } else {
+        memcpy(p, r, sizeof(T) * n);
+        beta = rr;
+    }
+    while (st == HFO_ITERATE) {
+        ++iter;
+        FN(hfo_spmv_)(n, row, col, val, p, Ap);
+        alpha = beta / FN(hfo_dot_)(n, Ap, p);
+        FN(hfo_axpy_)(n, x, p, alpha);
+        FN(hfo_axpy_)(n, r, Ap, -alpha);
+        rr = FN(hfo_dot_)(n, r, r);
+        res = (T)sqrt(rr);
+        if (hist)
+            hist[iter] = res;
+        st = FN(check_)(ctrl, iter, res, &first);
+        if (st != HFO_ITERATE)
+            break;
+        if (precond) {
+            memset(z, 0, sizeof(T) * n);
+            FN(hfo_prec_apply_)(n, P, r, z);
+            gamma = FN(hfo_dot_)(n, r, z);
+            beta = gamma / beta;
+            FN(hfo_scaleadd_)(n, p, beta, z);
+            beta = gamma;
+        } else {
+            beta = rr / beta;
+            FN(hfo_scaleadd_)(n, p, beta, r);
+            beta = rr;
+        }
+    }
+    out->iters = iter;
+    out->status = st;
+    out->res = res;
+    out->res0 = first;
+    out->ret = FN(retcode_)(st);
+    free(r); free(p); free(z); free(Ap);
+    return out->ret;
+}
+
+/* GMRES<LAD>::ApplyPlaneRotation / GeneratePlaneRotation (gmres.h:80-97) */
+static void FN(rot_apply_)(T cs, T sn, T *dx, T *dy)
+{
+    const T t = cs * (*dx) + sn * (*dy);
+    *dy = -sn * (*dx) + cs * (*dy);
+    *dx = t;
+}
+static void FN(rot_gen_)(T dx, T dy, T *cs, T *sn)
+{
+    const T beta = (T)sqrt(dx * dx + dy * dy);
+    *cs = dx / beta;
+    *sn = dy / beta;
+}
+
+/* GMRES<LAD>::UpdateSolution (gmres.cc:487-510): back substitution on the
+ * rotated Hessenberg H (column j stored at H[j*(m+1)]), then x += sum y_j V_j. */
+static void FN(update_)(int n, T **V, const T *H, int m1, const T *g, int k, T *x)
+{
+    T *y = (T *)malloc(sizeof(T) * m1);
+    for (int i = 0; i < m1; ++i)
+        y[i] = g[i];
+    for (int i = k + 1; i--;) {
+        y[i] /= H[i * m1 + i];
+        const T t = y[i];
+        for (int j = 0; j < i; ++j)
+            y[j] -= H[i * m1 + j] * t;
+    }
+    for (int j = 0; j <= k; ++j)
+        FN(hfo_axpy_)(n, x, V[j], y[j]);
+    free(y);
+}
+
+/* GMRES<LAD>::SolveRight (gmres.cc:285-477) when flexible == 0, and
+ * FGMRES<LAD>::SolveRight (fgmres.cc:75-262) when flexible == 1. */
+static int FN(gmres_core_)(int n, const int *row, const int *col, const T *val, const T *b,
+                           T *x, const PREC *P, int size_basis, const hfo_ctrl *ctrl,
+                           hfo_result *out, double *hist, int flexible)
+{
+    if (size_basis <= 0) {
+        out->ret = HFO_SOLVER_ERROR;
+        return out->ret;
+    }
+    const int m = size_basis < ctrl->maxits ? size_basis : ctrl->maxits;
+    const int m1 = m + 1;
+    T *H = (T *)calloc((size_t)m * m1, sizeof(T));
+    T **V = (T **)malloc(sizeof(T *) * m1);
+    T **Z = (T **)malloc(sizeof(T *) * m1);
+    for (int i = 0; i < m1; ++i) {
+        V[i] = (T *)calloc(n, sizeof(T));
+        Z[i] = flexible ? (T *)calloc(n, sizeof(T)) : NULL;
+    }
+    T *w = (T *)calloc(n, sizeof(T)), *z = (T *)calloc(n, sizeof(T));
+    T *g = (T *)calloc(m1, sizeof(T)), *cs = (T *)calloc(m1, sizeof(T));
+    T *sn = (T *)calloc(m1, sizeof(T));
+    double first = 0;
+    T res;
+    int iter = 0, st;
+
+    FN(hfo_spmv_)(n, row, col, val, x, V[0]);
+    FN(hfo_scaleadd_)(n, V[0], (T)-1., b);
+    res = FN(hfo_nrm2_)(n, V[0]);
+    st = FN(check_)(ctrl, iter, res, &first);
+    if (hist)
+        hist[0] = res;
+    while (st == HFO_ITERATE) {
+        for (int i = 0; i < m1; ++i)
+            g[i] = 0;
+        memset(H, 0, sizeof(T) * (size_t)m * m1);
+        FN(hfo_scale_)(n, V[0], (T)1. / res);
+        g[0] = res;
+        for (int j = 0; j != m; ++j) {
+            ++iter;
+            T *zj = flexible ? Z[j] : z;
+            memset(zj, 0, sizeof(T) * n);
+            FN(hfo_prec_apply_)(n, P, V[j], zj);
+            FN(hfo_spmv_)(n, row, col, val, zj, w);
+            for (int i = 0; i <= j; ++i) {
+                H[j * m1 + i] = FN(hfo_dot_)(n, w, V[i]);
+                FN(hfo_axpy_)(n, w, V[i], (T)-1. * H[j * m1 + i]);
+            }
+            H[j * m1 + j + 1] = FN(hfo_nrm2_)(n, w);
+            FN(hfo_scale_)(n, w, (T)1. / H[j * m1 + j + 1]);
+            memcpy(V[j + 1], w, sizeof(T) * n);
+            for (int k = 0; k < j; ++k)
+                FN(rot_apply_)(cs[k], sn[k], &H[j * m1 + k], &H[j * m1 + k + 1]);
+            FN(rot_gen_)(H[j * m1 + j], H[j * m1 + j + 1], &cs[j], &sn[j]);
+            FN(rot_apply_)(cs[j], sn[j], &H[j * m1 + j], &H[j * m1 + j + 1]);
+            FN(rot_apply_)(cs[j], sn[j], &g[j], &g[j + 1]);
+            res = (T)fabs((double)g[j + 1]);
+            if (hist)
+                hist[iter] = res;
+            st = FN(check_)(ctrl, iter, res, &first);
+            if (st != HFO_ITERATE) {
+                if (flexible) {
+                    FN(update_)(n, Z, H, m1, g, j, x);
+                } else {
+                    memset(z, 0, sizeof(T) * n);
+                    FN(update_)(n, V, H, m1, g, j, z);
+                    memset(w, 0, sizeof(T) * n);
+                    FN(hfo_prec_apply_)(n, P, z, w);
+                    FN(hfo_axpy_)(n, x, w, (T)1.);
+                }
+                break;
+            }
+        }
+        if (st == HFO_ITERATE) {
+            if (flexible) {
+                FN(update_)(n, Z, H, m1, g, m - 1, x);
+                FN(hfo_spmv_)(n, row, col, val, x, V[0]);
+                FN(hfo_scale_)(n, V[0], (T)-1.);
+                FN(hfo_axpy_)(n, V[0], b, (T)1.);
+            } else {
+                memset(z, 0, sizeof(T) * n);
+                FN(update_)(n, V, H, m1, g, m - 1, z);
+                memset(w, 0, sizeof(T) * n);
+                FN(hfo_prec_apply_)(n, P, z, w);
+                FN(hfo_axpy_)(n, x, w, (T)1.);
+                FN(hfo_spmv_)(n, row, col, val, x, V[0]);
+                FN(hfo_scaleadd_)(n, V[0], (T)-1., b);
+            }
+            /* the restart keeps res = |g[m]| for the normalisation (gmres.cc:356-358) */
+        }
+    }
+    out->iters = iter;
+    out->status = st;
+    out->res = res;
+    out->res0 = first;
+    out->ret = FN(retcode_)(st);
+    for (int i = 0; i < m1; ++i) {
+        free(V[i]);
+        free(Z[i]);
+    }
+    free(V); free(Z); free(H); free(w); free(z); free(g); free(cs); free(sn);
+    return out->ret;
+}
+
+int FN(hfo_gmres_)(int n, const int *row, const int *col, const T *val, const T *b, T *x,
+                   const PREC *P, int size_basis, const hfo_ctrl *ctrl, hfo_result *out,
+                   double *hist)
+{
+    return FN(gmres_core_)(n, row, col, val, b, x, P, size_basis, ctrl, out, hist, 0);
+}
+
+int FN(hfo_fgmres_)(int n, const int *row, const int *col, const T *val, const T *b, T *x,
+                    const PREC *P, int size_basis, const hfo_ctrl *ctrl, hfo_result *out,
+                    double *hist)
+{
+    return FN(gmres_core_)(n, row, col, val, b, x, P, size_basis, ctrl, out, hist, 1);
+}
+
+#undef PREC
+#undef FN
+#undef CAT
+#undef CAT_

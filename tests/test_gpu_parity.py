@@ -1,0 +1,365 @@
+"""GPU parity of the HIP platform against the CPU oracle, through the C ABI.
+
+Tolerances: SpMV norm-wise relative 1e-13 (FP64); element-wise BLAS-1
+within 2 ulp-scale relative error (the GPU contracts a*x+y to FMA, the
+reference CPU build does not); Krylov solvers: iteration count within +-2
+of the oracle and the true residual at the requested tolerance.
+"""
+import numpy as np
+import pytest
+
+from conftest import relnorm
+
+pytestmark = pytest.mark.gpu
+
+SPMV_TOL = 1e-13
+
+
+def rand_csr(rng, n, ncol, lens):
+    row = np.zeros(n + 1, np.int32)
+    row[1:] = np.cumsum(lens)
+    col = np.concatenate([np.sort(rng.choice(ncol, size=l, replace=False)) for l in lens]).astype(np.int32) \
+        if row[-1] else np.zeros(0, np.int32)
+    val = rng.uniform(-1, 1, row[-1])
+    return row, col, val
+
+
+def spmv_case(pkg, ctx, oracle, row, col, val, ncol=None, dtype=np.float64, seed=0):
+    ncol = len(row) - 1 if ncol is None else ncol
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(-1, 1, ncol).astype(dtype)
+    A = pkg.CsrMatrix(ctx, row, col, val.astype(dtype), ncol=ncol)
+    X = pkg.Vector(ctx, x, dtype)
+    Y = pkg.Vector(ctx, len(row) - 1, dtype)
+    A.spmv(X, Y)
+    ref = oracle.spmv(oracle.Csr(row, col, val.astype(dtype), ncol), x)
+    return Y.numpy(), ref, A, X, Y
+
+
+def test_spmv_reference_known_answers(pkg, ctx, oracle):
+    # test-legacy/lmp_test.cc:80-82 and test-legacy/cuda_test.cc:75-77
+    row = np.array([0, 2, 5, 7], np.int32)
+    col = np.array([0, 1, 0, 1, 2, 1, 2], np.int32)
+    val = np.array([2.0, -1.0, -1.0, 2.0, -1.0, -1.0, 2.0])
+    A = pkg.CsrMatrix(ctx, row, col, val)
+    Y = pkg.Vector(ctx, 3)
+    A.spmv(pkg.Vector(ctx, [1.0, -2.0, 0.0]), Y)
+    assert list(Y.numpy()) == [4.0, -5.0, 2.0]
+    A.spmv(pkg.Vector(ctx, [1.2, -2.3, 0.7]), Y)
+    assert np.allclose(Y.numpy(), [4.7, -6.5, 3.7], atol=1e-12)
+
+
+@pytest.mark.parametrize("case", ["q1_2d_256", "q1_3d_32", "fv_24", "upwind_20"])
+def test_spmv_bitwise_on_configs(pkg, ctx, oracle, case):
+    if case == "q1_2d_256":
+        row, col, val, _ = pkg.problem(pkg.Q1_2D, 255)             # config 1 at full size
+    elif case == "q1_3d_32":
+        row, col, val, _ = pkg.problem(pkg.Q1_3D, 31)
+    elif case == "fv_24":
+        row, col, val, _ = pkg.problem(pkg.FV_DIFFUSION, 24, p0=1.0, seed=1)
+    else:
+        row, col, val, _ = pkg.problem(pkg.UPWIND_CONVDIFF, 20, p0=1e-2, p1=5.0)
+    y, ref, *_ = spmv_case(pkg, ctx, oracle, row, col, val)
+    # products rounded then summed in the reference order: bit-identical rows
+    assert np.array_equal(y, ref)
+    y32, ref32, *_ = spmv_case(pkg, ctx, oracle, row, col, val, dtype=np.float32)
+    assert np.array_equal(y32, ref32)
+
+
+def test_spmv_ragged_and_long_rows(pkg, ctx, oracle):
+    rng = np.random.default_rng(3)
+    n, ncol = 600, 5000
+    lens = rng.integers(0, 40, n)
+    lens[[0, 17, 300]] = 0                      # empty rows
+    lens[[5, 299, 599]] = [1500, 4000, 1025]    # rows longer than one workgroup tile
+    row, col, val = rand_csr(rng, n, ncol, lens)
+    y, ref, A, X, Y = spmv_case(pkg, ctx, oracle, row, col, val, ncol=ncol)
+    assert relnorm(y, ref) < SPMV_TOL
+    short = np.diff(row) <= 1024
+    assert np.array_equal(y[short], ref[short])
+    # y += A x
+    y0 = rng.uniform(-1, 1, n)
+    Y.upload(y0)
+    A.spmv_add(X, Y)
+    assert relnorm(Y.numpy(), oracle.spmv_add(oracle.Csr(row, col, val, ncol), X.numpy(), y0)) < SPMV_TOL
+
+
+def test_spmv_single_row_and_rect(pkg, ctx, oracle):
+    rng = np.random.default_rng(4)
+    row, col, val = rand_csr(rng, 1, 7, [5])
+    y, ref, *_ = spmv_case(pkg, ctx, oracle, row, col, val, ncol=7)
+    assert np.array_equal(y, ref)
+    row, col, val = rand_csr(rng, 50, 3, rng.integers(0, 4, 50))
+    y, ref, *_ = spmv_case(pkg, ctx, oracle, row, col, val, ncol=3)
+    assert np.array_equal(y, ref)
+
+
+def test_csr_upload_validation(pkg, ctx):
+    with pytest.raises(pkg.HfhError) as e:
+        pkg.CsrMatrix(ctx, [0, 2], [1, 0], [1.0, 2.0])       # unsorted columns
+    assert e.value.code == pkg.ERR_INVALID
+    with pytest.raises(pkg.HfhError):
+        pkg.CsrMatrix(ctx, [0, 1], [3], [1.0])               # column out of range
+    with pytest.raises(pkg.HfhError):
+        pkg.CsrMatrix(ctx, [0, 2, 1], [0, 1], [1.0, 2.0])    # decreasing row pointer
+
+
+@pytest.mark.parametrize("n", [1, 7, 1000, 1 << 20])
+def test_blas1(pkg, ctx, oracle, n):
+    rng = np.random.default_rng(n)
+    x = rng.uniform(-1, 1, n)
+    y = rng.uniform(-1, 1, n)
+    X, Y = pkg.Vector(ctx, x), pkg.Vector(ctx, y)
+    assert abs(X.dot(Y) - oracle.dot(x, y)) <= 1e-14 * np.dot(np.abs(x), np.abs(y))
+    assert X.nrm2() == pytest.approx(oracle.nrm2(x), rel=1e-14)
+    a = 0.37
+    Y.axpy(X, a)
+    assert np.allclose(Y.numpy(), oracle.axpy(y, x, a), rtol=2e-16 * 4, atol=1e-16 * 4)
+    y2 = Y.numpy()
+    Y.scaleadd(-1.7, X)
+    assert np.allclose(Y.numpy(), oracle.scaleadd(y2, -1.7, x), rtol=1e-15, atol=1e-15)
+    y3 = Y.numpy()
+    Y.scale(0.5)
+    assert np.array_equal(Y.numpy(), oracle.scale(y3, 0.5))
+    y4 = Y.numpy()
+    Y.ewmult(X)
+    assert np.array_equal(Y.numpy(), oracle.ewmult(y4, x))
+    # FP32 vectors
+    Xf, Yf = pkg.Vector(ctx, x, np.float32), pkg.Vector(ctx, y, np.float32)
+    ref = float(np.dot(x.astype(np.float32).astype(np.float64), y.astype(np.float32).astype(np.float64)))
+    assert abs(Xf.dot(Yf) - ref) <= 1e-6 * np.dot(np.abs(x), np.abs(y)) + 1e-30
+
+
+def test_vector_ops(pkg, ctx):
+    rng = np.random.default_rng(9)
+    x = rng.uniform(-1, 1, 1000)
+    X = pkg.Vector(ctx, x)
+    assert np.array_equal(X.numpy(), x)
+    assert np.array_equal(X.numpy(10, 5), x[10:15])
+    idx = np.array([5, 999, 0, 17])
+    assert np.array_equal(X.get_values(idx), x[idx])
+    X.set_values(idx, [1.0, 2.0, 3.0, 4.0])
+    x[idx] = [1.0, 2.0, 3.0, 4.0]
+    assert np.array_equal(X.numpy(), x)
+    Y = pkg.Vector(ctx, 1000)
+    Y.copy_range(100, X, 200, 50)
+    ref = np.zeros(1000)
+    ref[100:150] = x[200:250]
+    assert np.array_equal(Y.numpy(), ref)
+    F = pkg.Vector(ctx, 1000, np.float32)
+    F.cast_from(X)
+    assert np.array_equal(F.numpy(), x.astype(np.float32))
+    D = pkg.Vector(ctx, 1000)
+    D.cast_from(F)
+    assert np.array_equal(D.numpy(), x.astype(np.float32).astype(np.float64))
+    perm = rng.permutation(1000)
+    D.reorder_from(X, perm)
+    assert np.array_equal(D.numpy(), x[perm])
+    Y.copy_from(X)
+    Y.zeros()
+    assert not Y.numpy().any()
+    with pytest.raises(pkg.HfhError):
+        X.get_values([1000])
+
+
+# ---------------------------------------------------------------- preconditioners
+
+def fv3(pkg, n=16, seed=1, sigma=1.0):
+    return pkg.problem(pkg.FV_DIFFUSION, n, p0=sigma, seed=seed)
+
+
+def test_jacobi_apply(pkg, ctx, oracle):
+    row, col, val, rhs = fv3(pkg)
+    A = pkg.CsrMatrix(ctx, row, col, val)
+    P = pkg.Preconditioner.jacobi(A)
+    Z = pkg.Vector(ctx, len(rhs))
+    P.apply(pkg.Vector(ctx, rhs), Z)
+    ref = rhs * oracle.invdiag(oracle.Csr(row, col, val))
+    assert np.array_equal(Z.numpy(), ref)
+
+
+def test_jacobi_zero_diagonal(pkg, ctx):
+    A = pkg.CsrMatrix(ctx, [0, 1, 2], [0, 1], [1.0, 0.0])
+    with pytest.raises(pkg.HfhError) as e:
+        pkg.Preconditioner.jacobi(A)
+    assert e.value.code == pkg.ERR_ZERO_PIVOT
+
+
+@pytest.mark.parametrize("omega", [1.0, 1.3])
+@pytest.mark.parametrize("kind", ["fv3", "q1_3d", "q1_2d"])
+def test_mcsgs_apply(pkg, ctx, oracle, omega, kind):
+    if kind == "fv3":
+        row, col, val, rhs = fv3(pkg)
+    elif kind == "q1_3d":
+        row, col, val, rhs = pkg.problem(pkg.Q1_3D, 15)
+    else:
+        row, col, val, rhs = pkg.problem(pkg.Q1_2D, 40, 33)
+    nc, color, sizes, perm = pkg.multicoloring(row, col)
+    prow, pcol, pval = pkg.csr_permute(row, col, val, perm)
+    A = pkg.CsrMatrix(ctx, prow, pcol, pval)
+    P = pkg.Preconditioner.mcsgs(A, sizes, omega)
+    r = rhs[perm]
+    Z = pkg.Vector(ctx, len(r))
+    P.apply(pkg.Vector(ctx, r), Z)
+    ref = oracle.mcsgs(oracle.Csr(prow, pcol, pval), sizes, r, omega=omega)
+    assert relnorm(Z.numpy(), ref) < 1e-13
+
+
+def test_sgs_natural_apply(pkg, ctx, oracle):
+    row, col, val, rhs = fv3(pkg, 12)
+    A = pkg.CsrMatrix(ctx, row, col, val)
+    P = pkg.Preconditioner.sgs(A, row, col)
+    Z = pkg.Vector(ctx, len(rhs))
+    P.apply(pkg.Vector(ctx, rhs), Z)
+    assert relnorm(Z.numpy(), oracle.sgs(oracle.Csr(row, col, val), rhs)) < 1e-13
+
+
+@pytest.mark.parametrize("case", ["upwind", "fv3"])
+def test_ilu0_apply(pkg, ctx, oracle, case):
+    if case == "upwind":
+        row, col, val, rhs = pkg.problem(pkg.UPWIND_CONVDIFF, 16, p0=1e-2, p1=5.0, seed=4)
+    else:
+        row, col, val, rhs = fv3(pkg, 14)
+    lu = pkg.ilu0_factor(row, col, val)
+    LU = pkg.CsrMatrix(ctx, row, col, lu)
+    P = pkg.Preconditioner.ilu0(LU, row, col)
+    Z = pkg.Vector(ctx, len(rhs))
+    P.apply(pkg.Vector(ctx, rhs), Z)
+    assert relnorm(Z.numpy(), oracle.ilu_solve(oracle.Csr(row, col, lu), rhs)) < 1e-13
+
+
+# ---------------------------------------------------------------- Krylov drivers
+
+def true_res(row, col, val, x, b):
+    import scipy.sparse as sp
+    return np.linalg.norm(b - sp.csr_matrix((val, col, row)) @ x) / np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("case,prec", [("q1_2d", "jacobi"), ("fv3", "jacobi"), ("fv3", "none"),
+                                       ("q1_3d", "jacobi")])
+def test_pcg_matches_oracle(pkg, ctx, oracle, case, prec):
+    if case == "q1_2d":
+        row, col, val, rhs = pkg.problem(pkg.Q1_2D, 63)
+    elif case == "q1_3d":
+        row, col, val, rhs = pkg.problem(pkg.Q1_3D, 23)
+    else:
+        row, col, val, rhs = fv3(pkg, 20)
+    A = pkg.CsrMatrix(ctx, row, col, val)
+    P = pkg.Preconditioner.jacobi(A) if prec == "jacobi" else None
+    X = pkg.Vector(ctx, len(rhs))
+    info = pkg.pcg(A, pkg.Vector(ctx, rhs), X, P, maxits=3000, atol=0.0, rtol=1e-8, history=True)
+    OA = oracle.Csr(row, col, val)
+    xo, oi = oracle.cg(OA, rhs, P=oracle.Prec.jacobi(OA) if prec == "jacobi" else None,
+                       ctrl=(3000, 0.0, 1e-8, 1e6))
+    assert info["state"] == pkg.SUCCESS_REL and oi["status"] == oracle.SUCCESS_REL
+    assert abs(info["iters"] - oi["iters"]) <= 2
+    k = min(20, len(oi["hist"]), len(info["hist"]))
+    assert np.allclose(info["hist"][:k], oi["hist"][:k], rtol=1e-9)
+    x = X.numpy()
+    assert true_res(row, col, val, x, rhs) < 2e-8
+    assert relnorm(x, xo) < 1e-6
+
+
+def test_pcg_mcsgs_matches_oracle(pkg, ctx, oracle):
+    row, col, val, rhs = fv3(pkg, 18, sigma=1.0, seed=3)
+    nc, color, sizes, perm = pkg.multicoloring(row, col)
+    prow, pcol, pval = pkg.csr_permute(row, col, val, perm)
+    b = rhs[perm]
+    A = pkg.CsrMatrix(ctx, prow, pcol, pval)
+    P = pkg.Preconditioner.mcsgs(A, sizes)
+    X = pkg.Vector(ctx, len(b))
+    info = pkg.pcg(A, pkg.Vector(ctx, b), X, P, maxits=2000, atol=0.0, rtol=1e-10)
+    OA = oracle.Csr(prow, pcol, pval)
+    xo, oi = oracle.cg(OA, b, P=oracle.Prec(oracle.PREC_MCSGS, OA, sizes=sizes), ctrl=(2000, 0.0, 1e-10, 1e6))
+    assert abs(info["iters"] - oi["iters"]) <= 2
+    assert true_res(prow, pcol, pval, X.numpy(), b) < 3e-10
+    assert relnorm(X.numpy(), xo) < 1e-8
+
+
+@pytest.mark.parametrize("flexible", [False, True])
+def test_gmres_ilu0_matches_oracle(pkg, ctx, oracle, flexible):
+    row, col, val, rhs = pkg.problem(pkg.UPWIND_CONVDIFF, 16, p0=1e-2, p1=5.0, seed=7)
+    lu = pkg.ilu0_factor(row, col, val)
+    A = pkg.CsrMatrix(ctx, row, col, val)
+    LU = pkg.CsrMatrix(ctx, row, col, lu)
+    P = pkg.Preconditioner.ilu0(LU, row, col)
+    X = pkg.Vector(ctx, len(rhs))
+    info = pkg.gmres(A, pkg.Vector(ctx, rhs), X, P, m=30, flexible=flexible, maxits=1000, atol=0.0, rtol=1e-8,
+                     history=True)
+    OA = oracle.Csr(row, col, val)
+    fn = oracle.fgmres if flexible else oracle.gmres
+    xo, oi = fn(OA, rhs, P=oracle.Prec(oracle.PREC_ILU0, oracle.Csr(row, col, lu)), m=30,
+                ctrl=(1000, 0.0, 1e-8, 1e6))
+    assert abs(info["iters"] - oi["iters"]) <= 2
+    k = min(10, len(oi["hist"]), len(info["hist"]))
+    assert np.allclose(info["hist"][:k], oi["hist"][:k], rtol=1e-8)
+    assert true_res(row, col, val, X.numpy(), rhs) < 2e-8
+
+
+def test_gmres_restarts(pkg, ctx, oracle):
+    row, col, val, rhs = pkg.problem(pkg.UPWIND_CONVDIFF, 12, p0=1e-2, p1=5.0, seed=8)
+    A = pkg.CsrMatrix(ctx, row, col, val)
+    P = pkg.Preconditioner.jacobi(A)
+    X = pkg.Vector(ctx, len(rhs))
+    info = pkg.gmres(A, pkg.Vector(ctx, rhs), X, P, m=5, maxits=3000, atol=0.0, rtol=1e-8)
+    OA = oracle.Csr(row, col, val)
+    xo, oi = oracle.gmres(OA, rhs, P=oracle.Prec.jacobi(OA), m=5, ctrl=(3000, 0.0, 1e-8, 1e6))
+    assert info["iters"] > 5 and abs(info["iters"] - oi["iters"]) <= 2
+    assert true_res(row, col, val, X.numpy(), rhs) < 3e-8
+
+
+def test_defect_correction_matches_fp64(pkg, ctx, oracle):
+    row, col, val, rhs = fv3(pkg, 20, sigma=1.0, seed=5)
+    nc, color, sizes, perm = pkg.multicoloring(row, col)
+    prow, pcol, pval = pkg.csr_permute(row, col, val, perm)
+    b = rhs[perm]
+    Ah = pkg.CsrMatrix(ctx, prow, pcol, pval)
+    Al = pkg.CsrMatrix.cast_of(Ah, np.float32)
+    Pl = pkg.Preconditioner.mcsgs(Al, sizes)
+    X = pkg.Vector(ctx, len(b))
+    nb = np.linalg.norm(b)
+    info = pkg.dc(Ah, Al, pkg.Vector(ctx, b), X, Pl, outer=(30, 1e-10 * nb, 0.0, 1e30),
+                  inner=(1000, 0.0, 1e-4, 1e30))
+    assert info["state"] == pkg.SUCCESS_ABS
+    assert true_res(prow, pcol, pval, X.numpy(), b) < 1e-10
+    OA = oracle.Csr(prow, pcol, pval)
+    xo, oi = oracle.mpir(OA, b, Pl=oracle.Prec(oracle.PREC_MCSGS, OA, sizes=sizes), max_ecl=30,
+                         abs_tol=1e-10 * nb, rel_tol=0.0, div_tol=1e30, ecl=(1000, 0.0, 1e-4, 1e30))
+    assert abs(info["outer_iters"] - oi["n_ecl"]) <= 1
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    xd = spla.spsolve(sp.csr_matrix((pval, pcol, prow)).tocsc(), b)
+    assert relnorm(X.numpy(), xd) < 1e-8
+
+
+def test_solver_failure_codes(pkg, ctx):
+    row, col, val, rhs = fv3(pkg, 10)
+    A = pkg.CsrMatrix(ctx, row, col, val)
+    X = pkg.Vector(ctx, len(rhs))
+    info = pkg.pcg(A, pkg.Vector(ctx, rhs), X, None, maxits=1, atol=0.0, rtol=1e-12, check=False)
+    assert info["rc"] == pkg.ERR_NOT_CONVERGED and info["state"] == pkg.FAIL_MAXITS and info["iters"] == 1
+    bn = rhs.copy()
+    bn[3] = np.nan
+    X.zeros()
+    info = pkg.pcg(A, pkg.Vector(ctx, bn), X, pkg.Preconditioner.jacobi(A), maxits=7, atol=0.0, rtol=1e-8,
+                   check=False)
+    assert info["state"] == pkg.FAIL_MAXITS and info["iters"] == 7
+    with pytest.raises(pkg.HfhError) as e:
+        pkg.pcg(A, pkg.Vector(ctx, rhs), X, None, maxits=0)
+    assert e.value.code == pkg.ERR_INVALID
+
+
+def test_solver_3x3_known_answer(pkg, ctx):
+    # test/LinearAlgebra/cg.cc:89-91 through the device drivers
+    row = np.array([0, 3, 6, 9], np.int32)
+    col = np.array([0, 1, 2] * 3, np.int32)
+    val = np.array([2.0, -1.0, 0.0, -1.0, 2.0, -1.0, 0.0, -1.0, 2.0])
+    A = pkg.CsrMatrix(ctx, row, col, val)
+    B = pkg.Vector(ctx, [1.0, -2.0, 0.0])
+    for solve in (lambda X: pkg.pcg(A, B, X, None, maxits=100, atol=1e-15, rtol=1e-12),
+                  lambda X: pkg.pcg(A, B, X, pkg.Preconditioner.jacobi(A), maxits=100, atol=1e-15, rtol=1e-12),
+                  lambda X: pkg.gmres(A, B, X, pkg.Preconditioner.jacobi(A), m=3, maxits=100, atol=1e-15,
+                                      rtol=1e-12)):
+        X = pkg.Vector(ctx, 3)
+        solve(X)
+        assert np.allclose(X.numpy(), [-0.25, -1.5, -0.75], atol=1e-12)

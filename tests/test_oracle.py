@@ -1,0 +1,388 @@
+"""Pins the CPU oracle against the reference's own known answers, golden
+fixtures and what the mathematics fixes (closed forms, invariants, dense
+brute force).  CPU only."""
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+from conftest import relnorm
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def tridiag_coo(O, n, diag=2.0, off=-1.0, full=False):
+    rows, cols, vals = [], [], []
+    for i in range(n):
+        for j in range(n) if full else (i - 1, i, i + 1):
+            if 0 <= j < n:
+                rows.append(i)
+                cols.append(j)
+                vals.append(diag if i == j else (off if abs(i - j) == 1 else 0.0))
+    return O.csr_from_coo(n, rows, cols, vals)
+
+
+def fv_matrix(pkg, O, nx, ny=None, nz=None, sigma=1.0, seed=7):
+    row, col, val, rhs = pkg.problem(pkg.FV_DIFFUSION, nx, ny, nz, p0=sigma, seed=seed)
+    return O.Csr(row, col, val), rhs
+
+
+def to_scipy(A):
+    return sp.csr_matrix((A.val.astype(np.float64), A.col, A.row), shape=(A.n, A.ncol))
+
+
+# --------------------------------------------------------------- reference known answers
+
+def test_spmv_known_answers(oracle):
+    # test-legacy/lmp_test.cc:53-82: TransformFromCOO on tridiag(-1,2,-1)
+    rows = [0, 0, 1, 1, 1, 2, 2]
+    cols = [0, 1, 0, 1, 2, 1, 2]
+    vals = [2.0, -1.0, -1.0, 2.0, -1.0, -1.0, 2.0]
+    A = oracle.csr_from_coo(3, rows, cols, vals)
+    assert np.array_equal(oracle.spmv(A, np.array([1.0, -2.0, 0.0])), [4.0, -5.0, 2.0])
+    # test-legacy/cuda_test.cc:75-77
+    y = oracle.spmv(A, np.array([1.2, -2.3, 0.7]))
+    assert np.allclose(y, [4.7, -6.5, 3.7], atol=1e-12)
+    assert np.array_equal(oracle.spmv(A, np.array([1.0, -2.0, 0.0]), omp=True), [4.0, -5.0, 2.0])
+
+
+def test_csr_unordered_structure(oracle):
+    # test/LinearAlgebra/CPUsimple_CSR_lMatrix.cc:159-200
+    A = oracle.csr_from_coo(3, [1, 2, 0, 0, 1], [2, 1, 0, 1, 1])
+    assert list(A.row) == [0, 2, 4, 5]
+    assert list(A.col) == [0, 1, 1, 2, 1]
+    B = oracle.csr_from_coo(3, [2, 0, 0, 1], [1, 0, 1, 1])
+    assert list(B.row) == [0, 2, 3, 4]
+    assert list(B.col) == [0, 1, 1, 1]
+
+
+def test_csr_add_values(oracle):
+    # test/LinearAlgebra/CPUsimple_CSR_lMatrix.cc:79-101
+    A = oracle.csr_from_coo(3, [0, 0, 1, 1, 1, 2, 2], [0, 1, 0, 1, 2, 1, 2])
+    A.val[:] = 0
+    oracle.csr_add_values(A, [0, 1, 2], [0, 1, 2], np.arange(1.0, 10.0))
+    assert list(A.val) == [1.0, 2.0, 4.0, 5.0, 6.0, 8.0, 9.0]
+
+
+@pytest.mark.parametrize("prec", ["none", "jacobi"])
+def test_cg_3x3(oracle, prec):
+    # test/LinearAlgebra/cg.cc:89-91 (and test-legacy/cg_test.cc:87-89)
+    A = tridiag_coo(oracle, 3, full=True)
+    P = oracle.Prec.jacobi(A) if prec == "jacobi" else None
+    x, info = oracle.cg(A, np.array([1.0, -2.0, 0.0]), P=P, ctrl=(100, 1e-15, 1e-12, 1e6))
+    assert info["ret"] == 0 and info["iters"] <= 3
+    assert np.allclose(x, [-0.25, -1.5, -0.75], atol=1e-12)
+
+
+@pytest.mark.parametrize("flex", [False, True])
+def test_gmres_3x3(oracle, flex):
+    # test/LinearAlgebra/gmres.cc (same system, basis 3)
+    A = tridiag_coo(oracle, 3, full=True)
+    fn = oracle.fgmres if flex else oracle.gmres
+    x, info = fn(A, np.array([1.0, -2.0, 0.0]), P=oracle.Prec.jacobi(A), m=3, ctrl=(100, 1e-15, 1e-12, 1e6))
+    assert info["ret"] == 0
+    assert np.allclose(x, [-0.25, -1.5, -0.75], atol=1e-12)
+
+
+def test_equation1_fixture(oracle):
+    # test/LinearAlgebra/solver.cc:106-156 with data/Equation1.*
+    d = np.load(os.path.join(GOLD, "equation1.npz"))
+    A = oracle.csr_from_coo(int(d["n"]), d["rows"], d["cols"], d["vals"])
+    x, info = oracle.cg(A, d["rhs"], x0=d["x0"], ctrl=(100, 1e-15, 1e-12, 1e6))
+    assert np.allclose(x, d["sol"], atol=1e-12)
+
+
+def test_set_C_fixture(oracle):
+    # test/PETSc/PETScLinearSolverReadData.cc:71-215 on data/set_C (tolerance 1e-2 there)
+    d = np.load(os.path.join(GOLD, "set_C.npz"))
+    A = oracle.csr_from_coo(int(d["n"]), d["rows"], d["cols"], d["vals"])
+    for solve in (lambda: oracle.gmres(A, d["rhs"], x0=d["x0"], P=oracle.Prec.jacobi(A), m=30,
+                                       ctrl=(5000, 1e-15, 1e-12, 1e6)),
+                  lambda: oracle.cg(A, d["rhs"], x0=d["x0"], P=oracle.Prec.jacobi(A),
+                                    ctrl=(5000, 1e-15, 1e-12, 1e6))):
+        x, info = solve()
+        assert info["ret"] == 0
+        # sol.txt holds 7 significant digits
+        assert relnorm(x, d["sol"]) < 2e-6
+        assert np.allclose(x, d["sol"], rtol=1e-5, atol=1e-6)
+
+
+# --------------------------------------------------------------- BLAS-1 orders and closed forms
+
+def test_dot_unroll5_order(oracle):
+    # CPUsimple_lvector.cc:160-209: the N mod 5 head first, then groups of five
+    # summed left to right.  With x = [1, 1e16, 1, -1e16, 1, 1], y = 1:
+    # head = 1; group = ((((1e16 + 1) - 1e16) + 1) + 1) = 2 -> 3.  A plain
+    # left-to-right sum gives 2, a pairwise sum gives 2 as well.
+    x = np.array([1.0, 1e16, 1.0, -1e16, 1.0, 1.0])
+    assert oracle.dot(x, np.ones(6)) == 3.0
+    # N = 10, no head: ((1e16+1)-1e16+1+1) + (1+...) = 2 + 5
+    x = np.array([1e16, 1.0, -1e16, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0])
+    assert oracle.dot(x, np.ones(10)) == 7.0
+    # float variant keeps the same order in float arithmetic
+    xf = np.array([1.0, 2.0**25, 1.0, -(2.0**25), 1.0, 1.0], np.float32)
+    assert oracle.dot(xf, np.ones(6, np.float32)) == np.float32(3.0)
+
+
+def test_blas1_closed_forms(oracle):
+    assert oracle.nrm2(np.array([3.0, 4.0])) == 5.0
+    # ScaleAdd is y = a*y + x (not y + a*x)
+    assert list(oracle.scaleadd(np.array([1.0, 2.0]), 3.0, np.array([10.0, 20.0]))) == [13.0, 26.0]
+    assert list(oracle.axpy(np.array([1.0, 2.0]), np.array([10.0, 20.0]), 3.0)) == [31.0, 62.0]
+    assert list(oracle.scale(np.array([1.0, -2.0]), -0.5)) == [-0.5, 1.0]
+    assert list(oracle.ewmult(np.array([2.0, 3.0]), np.array([4.0, -1.0]))) == [8.0, -3.0]
+
+
+def test_spmv_matches_dense(oracle, pkg):
+    rng = np.random.default_rng(1)
+    A, _ = fv_matrix(pkg, oracle, 9, 7, 5)
+    x = rng.uniform(-1, 1, A.n)
+    assert relnorm(oracle.spmv(A, x), A.to_dense() @ x) < 1e-15
+    y0 = rng.uniform(-1, 1, A.n)
+    assert relnorm(oracle.spmv_add(A, x, y0), y0 + A.to_dense() @ x) < 1e-15
+    # linearity
+    z = rng.uniform(-1, 1, A.n)
+    assert relnorm(oracle.spmv(A, 2.0 * x + z), 2.0 * oracle.spmv(A, x) + oracle.spmv(A, z)) < 1e-14
+
+
+# --------------------------------------------------------------- preconditioner applies
+
+def test_sgs_is_inverse_of_sgs_operator(oracle, pkg):
+    A, _ = fv_matrix(pkg, oracle, 6, 5, 4)
+    D = np.diag(np.diag(A.to_dense()))
+    L = np.tril(A.to_dense(), -1)
+    U = np.triu(A.to_dense(), 1)
+    M = (D + L) @ np.linalg.inv(D) @ (D + U)
+    r = np.random.default_rng(2).uniform(-1, 1, A.n)
+    z = oracle.sgs(A, r)
+    assert relnorm(M @ z, r) < 1e-13
+
+
+@pytest.mark.parametrize("omega", [1.0, 1.3])
+def test_mcsgs_operator(oracle, pkg, omega):
+    A, _ = fv_matrix(pkg, oracle, 6, 5, 4)
+    nc, color, sizes, perm = oracle.multicoloring(A)
+    Ap = oracle.csr_reorder(A, perm)
+    Dn = Ap.to_dense()
+    D = np.diag(np.diag(Dn))
+    L = np.tril(Dn, -1)
+    U = np.triu(Dn, 1)
+    r = np.random.default_rng(3).uniform(-1, 1, A.n)
+    z = oracle.mcsgs(Ap, sizes, r, omega=omega)
+    # forward (D + wL) y = r, t = D y / (w(2-w)), backward (D + wU) z = t
+    M = omega * (2 - omega) * (D + omega * L) @ np.linalg.inv(D) @ (D + omega * U)
+    assert relnorm(M @ z, r) < 1e-13
+    if omega == 1.0:
+        # a proper colouring makes MC-SGS natural-order SGS on the permuted matrix
+        assert relnorm(z, oracle.sgs(Ap, r)) < 1e-13
+
+
+def test_ilu0_tridiagonal_is_exact_lu(oracle):
+    n = 40
+    A = tridiag_coo(oracle, n, diag=3.0, off=-1.0)
+    A.val[A.col > np.repeat(np.arange(n), np.diff(A.row))] = -0.5  # nonsymmetric
+    LU, miss = oracle.ilu0(A)
+    assert miss == 0
+    r = np.random.default_rng(4).uniform(-1, 1, n)
+    assert relnorm(oracle.ilu_solve(LU, r), np.linalg.solve(A.to_dense(), r)) < 1e-14
+
+
+def test_ilu0_matches_pattern(oracle, pkg):
+    # defining property of ILU(0): (L U)_ij = a_ij for every (i, j) in the pattern
+    row, col, val, _ = pkg.problem(pkg.UPWIND_CONVDIFF, 5, p0=1e-2, p1=5.0, seed=1)
+    A = oracle.Csr(row, col, val)
+    LU, miss = oracle.ilu0(A)
+    assert miss == 0
+    F = LU.to_dense()
+    L = np.tril(F, -1) + np.eye(A.n)
+    U = np.triu(F)
+    P = L @ U
+    Ad = A.to_dense()
+    mask = np.zeros_like(Ad, dtype=bool)
+    for i in range(A.n):
+        mask[i, A.col[A.row[i]:A.row[i + 1]]] = True
+    assert np.max(np.abs(P[mask] - Ad[mask])) < 1e-14 * np.max(np.abs(Ad))
+    r = np.random.default_rng(5).uniform(-1, 1, A.n)
+    assert relnorm(P @ oracle.ilu_solve(LU, r), r) < 1e-13
+
+
+# --------------------------------------------------------------- colouring and level schedules
+
+def first_fit(A):
+    colors = np.zeros(A.n, np.int64)
+    for j in range(A.n):
+        taken = {colors[c] for c in A.col[A.row[j]:A.row[j + 1]] if c != j}
+        c = 1
+        while c in taken:
+            c += 1
+        colors[j] = c
+    return colors
+
+
+@pytest.mark.parametrize("kind,dims,ncol", [("fv3", (6, 5, 4), 2), ("q1_2d", (9, 7), 4), ("q1_3d", (5, 4, 6), 8),
+                                            ("upwind", (5,), 2)])
+def test_multicoloring(oracle, pkg, kind, dims, ncol):
+    if kind == "fv3":
+        row, col, val, _ = pkg.problem(pkg.FV_DIFFUSION, *dims)
+    elif kind == "q1_2d":
+        row, col, val, _ = pkg.problem(pkg.Q1_2D, dims[0], dims[1])
+    elif kind == "q1_3d":
+        row, col, val, _ = pkg.problem(pkg.Q1_3D, *dims)
+    else:
+        row, col, val, _ = pkg.problem(pkg.UPWIND_CONVDIFF, dims[0], p0=1e-2, p1=5.0)
+    A = oracle.Csr(row, col, val)
+    nc, color, sizes, perm = oracle.multicoloring(A)
+    assert nc == ncol
+    # the reference's rescans reach the textbook first-fit colouring
+    assert np.array_equal(color, first_fit(A))
+    # proper colouring
+    for i in range(A.n):
+        nb = A.col[A.row[i]:A.row[i + 1]]
+        assert np.all(color[nb[nb != i]] != color[i])
+    # permutation: colour by colour, increasing rows within a colour
+    assert np.array_equal(perm, np.lexsort((np.arange(A.n), color)))
+    assert list(sizes) == [int(np.sum(color == c)) for c in range(1, nc + 1)]
+    if kind == "fv3":  # red-black: colour = parity of i+j+k
+        nx, ny, nz = dims
+        k, j, i = np.meshgrid(range(nz), range(ny), range(nx), indexing="ij")
+        assert np.array_equal(color, ((i + j + k) % 2 + 1).ravel())
+
+
+def test_levelscheduling_grid(oracle, pkg):
+    nx, ny, nz = 6, 5, 4
+    row, col, val, _ = pkg.problem(pkg.FV_DIFFUSION, nx, ny, nz)
+    A = oracle.Csr(row, col, val)
+    nl, rep, sizes, perm = oracle.levelscheduling(A)
+    # level of cell (i, j, k) is i + j + k; rows ascending within a level
+    k, j, i = np.meshgrid(range(nz), range(ny), range(nx), indexing="ij")
+    lev = (i + j + k).ravel()
+    assert nl == rep == (nx - 1) + (ny - 1) + (nz - 1) + 1
+    assert np.array_equal(perm, np.lexsort((np.arange(A.n), lev)))
+    assert list(sizes) == [int(np.sum(lev == l)) for l in range(nl)]
+
+
+def test_levelscheduling_chain_quirk(oracle):
+    # a chain needs n levels; the reference then never records the count
+    # (lmatrix_csr_cpu.cc:2324-2329) and reports 0 levels
+    A = tridiag_coo(oracle, 6)
+    nl, rep, sizes, perm = oracle.levelscheduling(A)
+    assert nl == 6 and rep == 0 and list(perm) == list(range(6))
+
+
+def test_reorder_is_symmetric_permutation(oracle, pkg):
+    A, _ = fv_matrix(pkg, oracle, 5, 4, 3)
+    perm = np.random.default_rng(6).permutation(A.n).astype(np.int32)
+    B = oracle.csr_reorder(A, perm)
+    Ad = A.to_dense()
+    assert np.array_equal(B.to_dense(), Ad[np.ix_(perm, perm)])
+    for i in range(B.n):
+        assert np.all(np.diff(B.col[B.row[i]:B.row[i + 1]]) > 0)
+    x = np.arange(A.n, dtype=np.float64)
+    assert np.array_equal(oracle.vec_reorder(x, perm), x[perm])
+
+
+# --------------------------------------------------------------- Krylov drivers
+
+def test_cg_matches_scipy_and_bound(oracle, pkg):
+    A, b = fv_matrix(pkg, oracle, 24, 24, 1, sigma=0.5, seed=3)
+    S = to_scipy(A)
+    invd = oracle.invdiag(A)
+    x, info = oracle.cg(A, b, P=oracle.Prec.jacobi(A), ctrl=(2000, 0.0, 1e-8, 1e6))
+    count = [0]
+    xs, _ = spla.cg(S, b, rtol=1e-8, atol=0.0, maxiter=2000, M=sp.diags(invd),
+                    callback=lambda xk: count.__setitem__(0, count[0] + 1))
+    assert abs(info["iters"] - count[0]) <= 1
+    xd = np.linalg.solve(S.toarray(), b)
+    assert relnorm(x, xd) < 1e-6
+    # residual history: recursive residual of the last step is the reported one
+    assert info["hist"][-1] == info["res"] and info["res"] < 1e-8 * info["res0"]
+    # classical bound with the Jacobi-scaled condition number
+    Ds = np.sqrt(invd)
+    ev = np.linalg.eigvalsh((Ds[:, None] * S.toarray()) * Ds[None, :])
+    kappa = ev[-1] / ev[0]
+    assert info["iters"] <= np.ceil(0.5 * np.sqrt(kappa) * np.log(2 / 1e-8)) + 2
+
+
+def test_cg_diagonal_one_iteration(oracle):
+    d = np.array([1.0, 2.0, 5.0, 7.0])
+    A = oracle.csr_from_coo(4, [0, 1, 2, 3], [0, 1, 2, 3], d)
+    x, info = oracle.cg(A, np.ones(4), P=oracle.Prec.jacobi(A), ctrl=(10, 1e-15, 1e-12, 1e6))
+    assert info["iters"] == 1 and np.allclose(x, 1 / d)
+
+
+def _krylov_min_residuals(Ad, Minv, r0, kmax):
+    AM = Ad @ Minv
+    K = [r0 / np.linalg.norm(r0)]
+    out = []
+    for j in range(1, kmax + 1):
+        v = AM @ K[-1]
+        for q in K:  # keep the explicit basis well conditioned
+            v = v - (q @ v) * q
+        K.append(v / np.linalg.norm(v))
+        Q = np.array(K[:j]).T
+        y, *_ = np.linalg.lstsq(AM @ Q, r0, rcond=None)
+        out.append(np.linalg.norm(r0 - AM @ Q @ y))
+    return np.array(out)
+
+
+def test_gmres_minimises_residual(oracle, pkg):
+    row, col, val, rhs = pkg.problem(pkg.UPWIND_CONVDIFF, 5, p0=1e-2, p1=5.0, seed=2)
+    A = oracle.Csr(row, col, val)
+    LU, _ = oracle.ilu0(A)
+    P = oracle.Prec(oracle.PREC_ILU0, LU)
+    x, info = oracle.gmres(A, rhs, P=P, m=30, ctrl=(30, 0.0, 1e-12, 1e6))
+    F = LU.to_dense()
+    Minv = np.linalg.inv((np.tril(F, -1) + np.eye(A.n)) @ np.triu(F))
+    k = min(12, len(info["hist"]) - 1)
+    ref = _krylov_min_residuals(A.to_dense(), Minv, rhs, k)
+    got = info["hist"][1:k + 1]
+    # rounding limits the agreement once the residual is tiny
+    assert np.all(np.abs(got - ref) <= 1e-6 * ref + 1e-12 * np.linalg.norm(rhs))
+    # FGMRES with a fixed preconditioner is GMRES
+    xf, inf = oracle.fgmres(A, rhs, P=P, m=30, ctrl=(30, 0.0, 1e-12, 1e6))
+    assert np.max(np.abs(inf["hist"] - info["hist"]) / info["hist"]) < 1e-8
+    assert relnorm(xf, x) < 1e-8
+
+
+def test_gmres_restart(oracle, pkg):
+    row, col, val, rhs = pkg.problem(pkg.UPWIND_CONVDIFF, 6, p0=1e-2, p1=5.0, seed=2)
+    A = oracle.Csr(row, col, val)
+    x, info = oracle.gmres(A, rhs, P=oracle.Prec.jacobi(A), m=5, ctrl=(2000, 0.0, 1e-8, 1e6))
+    assert info["ret"] == 0 and info["iters"] > 5
+    true = np.linalg.norm(rhs - A.to_dense() @ x)
+    assert true < 2e-8 * np.linalg.norm(rhs)
+
+
+def test_control_failures(oracle, pkg):
+    A, b = fv_matrix(pkg, oracle, 8, 8, 1)
+    x, info = oracle.cg(A, b, ctrl=(1, 0.0, 1e-12, 1e6))
+    assert info["status"] == oracle.FAIL_MAXITS and info["ret"] == 1 and info["iters"] == 1
+    # residual growth beyond dtol * res0 stops with divergence
+    rng = np.random.default_rng(0)
+    Q, _ = np.linalg.qr(rng.normal(size=(30, 30)))
+    M = Q @ np.diag(np.logspace(0, 6, 30)) @ Q.T
+    r_, c_ = np.nonzero(np.ones((30, 30)))
+    D = oracle.csr_from_coo(30, r_, c_, M.ravel())
+    x, info = oracle.cg(D, rng.uniform(-1, 1, 30), ctrl=(200, 0.0, 1e-14, 1.0 + 1e-9))
+    assert info["status"] == oracle.FAIL_DIV and info["ret"] == 1
+    # NaN in b: no test passes, the loop runs to maxits
+    bn = b.copy()
+    bn[3] = np.nan
+    x, info = oracle.cg(A, bn, ctrl=(7, 0.0, 1e-8, 1e6))
+    assert info["status"] == oracle.FAIL_MAXITS and info["iters"] == 7
+
+
+def test_mpir(oracle, pkg):
+    A, b = fv_matrix(pkg, oracle, 10, 10, 10, sigma=1.0, seed=5)
+    nrm = np.linalg.norm(b)
+    x, info = oracle.mpir(A, b, Pl=oracle.Prec.jacobi(A), max_ecl=30, abs_tol=1e-10 * nrm, rel_tol=1e-12,
+                          ecl=(1000, 0.0, 1e-3, 1e6))
+    assert info["ret"] == 0 and info["n_ech"] == 0
+    true = np.linalg.norm(b - to_scipy(A) @ x)
+    assert true < 1e-10 * nrm and abs(true - info["res"]) < 1e-6 * true
+    # every loop reduces the true residual by about the inner tolerance
+    h = info["hist"]
+    assert np.all(h[1:] < h[:-1]) and np.all(h[1:] / h[:-1] < 1e-2)
+    assert relnorm(x, spla.spsolve(to_scipy(A).tocsc(), b)) < 1e-8
