@@ -3,8 +3,7 @@
 // BLAS-1 kernels are HBM streams: 16 B per lane per load where alignment
 // allows, grid-stride over a device-independent partition so that the
 // reductions are deterministic (the same partials in the same order on every
-// run and every device).  Reductions finish in the last workgroup to arrive
-// (self-resetting counter), so one launch produces the final scalar.
+// run and every device).  A one-workgroup finisher kernel sums the partials.
 #include "hfh_internal.h"
 
 #include <cmath>
@@ -68,30 +67,36 @@ static inline int red_blocks(int n) {
 }
 
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_dot(int n, const T *__restrict__ x,
-                                                const T *__restrict__ y, double *part,
-                                                unsigned int *counter, double *dst) {
+__global__ __launch_bounds__(kBlock) void k_dot(int n, const T *__restrict__ x, const T *__restrict__ y,
+                                                double *part) {
     __shared__ double sh[kBlock / 64];
     const int nb = gridDim.x;
     double s = 0;
-    for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < n;
-         i += (long long)nb * kBlock)
+    for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < n; i += (long long)nb * kBlock)
         s += static_cast<double>(x[i]) * static_cast<double>(y[i]);
-    double v[1] = {block_sum(s, sh)};
-    finish_reduce<1>(v, part, kRedBlocks, blockIdx.x, counter, nb, dst);
+    const double v[1] = {block_sum(s, sh)};
+    store_partials<1>(v, part, kRedBlocks, blockIdx.x);
 }
 
-// FP32 vectors reduce in FP32 products summed in FP64: the reference sums in
-// the vector's own type, so FP32 results agree to FP32 rounding.
+__global__ __launch_bounds__(kFinBlock) void k_fin_store(const double *part, int nb, double *dst) {
+    double out[1];
+    reduce_partials<1>(part, kRedBlocks, nb, out);
+    if (threadIdx.x == 0) dst[0] = out[0];
+}
+
+// FP32 vectors: products formed exactly in FP64 and summed in FP64.
 int dot_async(hfh_ctx *ctx, int dtype, int n, const void *x, const void *y, double *dst) {
     const int nb = red_blocks(n);
     if (dtype == HFH_F64)
-        hipLaunchKernelGGL(k_dot<double>, dim3(nb), dim3(kBlock), 0, ctx->stream, n,
-                           (const double *)x, (const double *)y, ctx->red, ctx->counter, dst);
+        hipLaunchKernelGGL(k_dot<double>, dim3(nb), dim3(kBlock), 0, ctx->stream, n, (const double *)x,
+                           (const double *)y, ctx->red);
     else
-        hipLaunchKernelGGL(k_dot<float>, dim3(nb), dim3(kBlock), 0, ctx->stream, n,
-                           (const float *)x, (const float *)y, ctx->red, ctx->counter, dst);
-    return launch_check("k_dot");
+        hipLaunchKernelGGL(k_dot<float>, dim3(nb), dim3(kBlock), 0, ctx->stream, n, (const float *)x,
+                           (const float *)y, ctx->red);
+    int rc = launch_check("k_dot");
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_fin_store, dim3(1), dim3(kFinBlock), 0, ctx->stream, (const double *)ctx->red, nb, dst);
+    return launch_check("k_fin_store");
 }
 
 int fetch(hfh_ctx *ctx, const double *dsrc, double *host, int count) {
@@ -191,9 +196,6 @@ extern "C" int hfh_ctx_create(int device, hfh_ctx **out) {
     HFH_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     int rc = dev_alloc(reinterpret_cast<void **>(&c->red), sizeof(double) * (kRedBlocks * 8 + 256));
     if (rc) { delete c; return rc; }
-    rc = dev_alloc(reinterpret_cast<void **>(&c->counter), sizeof(unsigned) * 64);
-    if (rc) { delete c; return rc; }
-    HFH_HIP(hipMemset(c->counter, 0, sizeof(unsigned) * 64));
     HFH_HIP(hipMemset(c->red, 0, sizeof(double) * (kRedBlocks * 8 + 256)));
     HFH_HIP(hipHostMalloc(reinterpret_cast<void **>(&c->host_res), sizeof(double) * 256));
     HFH_HIP(hipEventCreate(&c->ev0));
@@ -207,7 +209,6 @@ extern "C" int hfh_ctx_destroy(hfh_ctx *c) {
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->red) (void)hipFree(c->red);
-    if (c->counter) (void)hipFree(c->counter);
     if (c->host_res) (void)hipHostFree(c->host_res);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);

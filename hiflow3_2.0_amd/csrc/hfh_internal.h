@@ -5,7 +5,7 @@
 //   CSR     : int32 row[nrow+1], int32 col[nnz] (ascending per row), T val[nnz],
 //             int32 rblk[nrblk+1] row-block boundaries for the streaming SpMV,
 //             int32 diag[nrow] position of the diagonal entry (-1 if absent)
-//   reductions: per-context scratch of FP64 block partials + a completion counter
+//   reductions: per-context scratch of FP64 workgroup partials
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -41,9 +41,8 @@ struct hfh_ctx {
     long long hbm = 0;
     char name[128] = {0};
     hipStream_t stream = nullptr;
-    // reduction scratch: partials[kRedBlocks * 4] doubles, counters, results
+    // reduction scratch: partials[kRedBlocks * 8] doubles, then result slots
     double *red = nullptr;            // device
-    unsigned int *counter = nullptr;  // device, self-resetting completion counters
     double *host_res = nullptr;       // pinned host mirror for results
     hfh::Timing timing;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -64,6 +63,9 @@ struct hfh_csr {
     void *val = nullptr;
     int *rblk = nullptr;   // streaming row blocks
     int nrblk = 0;
+    int spmv_algo = HFH_SPMV_STREAM, spmv_p1 = 2048;   // configured kernel
+    int spmv_algo_req = 0, spmv_p1_req = 0, spmv_p2_req = 0;  // requested (0 = automatic)
+    int nwg = 1;           // SpMV workgroups = partial sums of a fused dot
     int *diag = nullptr;   // diagonal position per row
     bool has_structure = false;
     std::vector<int> h_row;  // host copy of the row pointer (for analysis)
@@ -130,12 +132,13 @@ struct ScopedKernelTimer {
 };
 
 // SpMV launch used by the drivers (hfh_spmv.hip).  mode 0: y = A x;
-// mode 1: y += A x.  When pdot is non-null the kernel also accumulates
-// sum_i y_i * w_i into block partials and the last workgroup writes the total
-// to pdot (device).
+// mode 1: y += A x.  When w is non-null each workgroup also stores its
+// partial sum of y_i * w_i at part[row block] (A->nrblk partials), to be
+// summed by a finisher kernel (A->nwg partials).
 int spmv_launch(const hfh_csr *A, const void *x, void *y, int mode);
+int spmv_configure(hfh_csr *A, int algo, int p1, int p2);
 int spmv_launch_ex(const hfh_csr *A, const void *x, void *y, int mode, const void *w, double *part,
-                   unsigned int *counter, double *dst, const int *done);
+                   const int *done);
 
 // Preconditioner apply on raw device pointers (hfh_prec.hip).  `done`
 // (device, nullable): every kernel returns at once when *done != 0.
@@ -171,38 +174,41 @@ __device__ __forceinline__ T block_sum(T v, T *sh /* >= kBlock/64 */) {
     return s;
 }
 
-// Last-arriving workgroup sums the partials part[q*stride + b], b < nb, in a
-// fixed order (lane-strided sums, then the fixed block tree) and writes
-// dst[q]; the counter is reset for the next launch.  `v` holds this
-// workgroup's NQ sums in thread 0.  `slot` is this workgroup's partial index.
+// Cross-workgroup reductions never synchronise inside a kernel: on gfx950
+// an agent-scope release fence writes back the XCD's L2, so a per-workgroup
+// fence + atomic costs far more than the reduction itself.  Each workgroup
+// stores its partial sums with plain stores; a finisher kernel launched
+// next on the same stream (the kernel boundary makes the partials visible)
+// sums them in a fixed order, so results are bit-reproducible.
 template <int NQ>
-__device__ __forceinline__ bool finish_reduce(const double (&v)[NQ], double *part, int stride, int slot,
-                                              unsigned int *counter, int nb, double *dst) {
-    __shared__ bool last;
-    __shared__ double sh[kBlock / 64];
+__device__ __forceinline__ void store_partials(const double (&v)[NQ], double *part, int stride, int slot) {
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int q = 0; q < NQ; ++q) part[q * stride + slot] = v[q];
-        __threadfence();
-        const unsigned prev = atomicAdd(counter, 1u);
-        last = (prev == static_cast<unsigned>(nb - 1));
     }
-    __syncthreads();
-    if (!last) return false;
-    __threadfence();
+}
+
+constexpr int kFinBlock = 1024;  // threads of a finisher workgroup
+
+// Sums part[q*stride + b], b < nb, with one workgroup of kFinBlock threads:
+// lane-strided sums in ascending b, then a fixed tree.  Valid in thread 0.
+template <int NQ>
+__device__ __forceinline__ void reduce_partials(const double *part, int stride, int nb, double (&out)[NQ]) {
+    __shared__ double sh[kFinBlock / 64];
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
         double s = 0;
-        for (int b = threadIdx.x; b < nb; b += kBlock) s += __hip_atomic_load(&part[q * stride + b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        s = block_sum(s, sh);
-        if (threadIdx.x == 0) dst[q] = s;
+        for (int b = threadIdx.x; b < nb; b += kFinBlock) s += part[q * stride + b];
+        s = wave_sum(s);
+        const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+        if (lane == 0) sh[w] = s;
+        __syncthreads();
+        double t = 0;
+        if (threadIdx.x == 0)
+            for (int i = 0; i < kFinBlock / 64; ++i) t += sh[i];
+        out[q] = t;
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        __threadfence();
-        atomicExch(counter, 0u);
-    }
-    return true;
 }
 
 }  // namespace hfh

@@ -1,14 +1,15 @@
 // hfh_krylov.hip -- device-resident Krylov drivers.
 //
 // CG (linear_solver/cg.cc:163-248) is restated with its scalars kept in HBM so
-// the host never waits on a Dot.  One iteration is three (Jacobi / none) or
-// 3 + precond launches:
-//   K1  Ap = A p, fused partial sums of Ap.p; last workgroup stores pAp
-//   K2  alpha = beta/pAp; x += alpha p; r -= alpha Ap; sums of r.r (and r.z
-//       with z = r.*invD for Jacobi); last workgroup: iter++, res = sqrt(r.r),
+// the host never waits on a Dot.  One Jacobi iteration is five launches:
+//   K1  Ap = A p with per-workgroup partial sums of Ap.p; F1 stores pAp
+//   K2  alpha = beta/pAp; x += alpha p; r -= alpha Ap; partials of r.r (and
+//       r.z with z = r.*invD for Jacobi); F2: iter++, res = sqrt(r.r),
 //       IterateControl::Check, beta update
-//   [M] z = M^-1 r, then K2b: r.z and the beta update (non-Jacobi)
+//   [M] z = M^-1 r, then K2b partials of r.z and F3: beta update (non-Jacobi)
 //   K3  p = beta_ratio p + z
+// (F* are one-workgroup finisher kernels that sum the partials in a fixed
+// order.)
 // Every kernel returns at once when the done flag is set, so the host
 // enqueues chunks of iterations (a captured hipGraph when possible) and reads
 // the state once per chunk.  Compulsory traffic per Jacobi iteration:
@@ -51,7 +52,7 @@ template <typename T, int MODE>
 __global__ __launch_bounds__(kBlock) void k_pcg_update(int n, T *__restrict__ x, const T *__restrict__ p,
                                                        T *__restrict__ r, const T *__restrict__ Ap,
                                                        const T *__restrict__ invd, double *part, int stride,
-                                                       unsigned int *counter, KState *st, double *hist) {
+                                                       const KState *st) {
     if (st->done) return;
     __shared__ double sh[kBlock / 64];
     const T alpha = static_cast<T>(static_cast<T>(st->beta) / static_cast<T>(st->pap));
@@ -70,12 +71,25 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int n, T *__restrict__ x,
     v[0] = block_sum(rr, sh);
     __syncthreads();
     v[1] = MODE == 1 ? block_sum(rz, sh) : 0.0;
+    store_partials<2>(v, part, stride, blockIdx.x);
+}
+
+// stores pAp = sum of the SpMV partials
+__global__ __launch_bounds__(kFinBlock) void k_fin_pap(const double *part, int nb, KState *st) {
+    if (st->done) return;
+    double out[1];
+    reduce_partials<1>(part, 0, nb, out);
+    if (threadIdx.x == 0) st->pap = out[0];
+}
+
+// iteration bookkeeping after K2 (MODE as in k_pcg_update)
+template <typename T, int MODE>
+__global__ __launch_bounds__(kFinBlock) void k_fin_update(const double *part, int stride, int nb, KState *st,
+                                                          double *hist) {
+    if (st->done) return;
     double tot[2];
-    __shared__ double res2[2];
-    if (!finish_reduce<2>(v, part, stride, blockIdx.x, counter, gridDim.x, res2)) return;
+    reduce_partials<2>(part, stride, nb, tot);
     if (threadIdx.x == 0) {
-        tot[0] = res2[0];
-        tot[1] = res2[1];
         const int it = st->iter + 1;
         st->iter = it;
         const T rrT = static_cast<T>(tot[0]);
@@ -100,20 +114,26 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int n, T *__restrict__ x,
     }
 }
 
-// r.z after a general preconditioner apply; last workgroup updates beta.
+// partials of r.z after a general preconditioner apply
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_pcg_rz(int n, const T *__restrict__ r, const T *__restrict__ z,
-                                                   double *part, int stride, unsigned int *counter, KState *st) {
+                                                   double *part, int stride, const KState *st) {
     if (st->done) return;
     __shared__ double sh[kBlock / 64];
     double rz = 0;
     for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock)
         rz += static_cast<double>(r[i]) * static_cast<double>(z[i]);
     const double v[1] = {block_sum(rz, sh)};
-    __shared__ double res1[1];
-    if (!finish_reduce<1>(v, part, stride, blockIdx.x, counter, gridDim.x, res1)) return;
+    store_partials<1>(v, part, stride, blockIdx.x);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kFinBlock) void k_fin_rz(const double *part, int stride, int nb, KState *st) {
+    if (st->done) return;
+    double out[1];
+    reduce_partials<1>(part, stride, nb, out);
     if (threadIdx.x == 0) {
-        const T gamma = static_cast<T>(res1[0]);
+        const T gamma = static_cast<T>(out[0]);
         const T beta = static_cast<T>(st->beta);
         st->ratio = static_cast<double>(gamma / beta);
         st->beta = static_cast<double>(gamma);
@@ -168,27 +188,41 @@ struct PcgRun {
     const T *invd = nullptr;
     double *part = nullptr;
     int stride = 0;
-    unsigned int *cnt = nullptr;  // 3 counters
+    double *spart = nullptr;      // SpMV partials (A->nwg)
     KState *st = nullptr;
     double *hist = nullptr;
     int nb = 1;
 
-    int spmv() { return spmv_launch_ex(A, p, Ap, 0, p, part, cnt + 0, &st->pap, &st->done); }
+    int spmv() {
+        int rc = spmv_launch_ex(A, p, Ap, 0, p, spart, &st->done);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_fin_pap, dim3(1), dim3(kFinBlock), 0, A->ctx->stream, (const double *)spart, A->nwg, st);
+        return launch_check("k_fin_pap");
+    }
 
     int tail() {
         hipStream_t s = A->ctx->stream;
         int rc;
         if (mode == 0)
-            hipLaunchKernelGGL((k_pcg_update<T, 0>), dim3(nb), dim3(kBlock), 0, s, n, x, p, r, Ap, invd, part, stride, cnt + 1, st, hist);
+            hipLaunchKernelGGL((k_pcg_update<T, 0>), dim3(nb), dim3(kBlock), 0, s, n, x, p, r, Ap, invd, part, stride, (const KState *)st);
         else if (mode == 1)
-            hipLaunchKernelGGL((k_pcg_update<T, 1>), dim3(nb), dim3(kBlock), 0, s, n, x, p, r, Ap, invd, part, stride, cnt + 1, st, hist);
+            hipLaunchKernelGGL((k_pcg_update<T, 1>), dim3(nb), dim3(kBlock), 0, s, n, x, p, r, Ap, invd, part, stride, (const KState *)st);
         else
-            hipLaunchKernelGGL((k_pcg_update<T, 2>), dim3(nb), dim3(kBlock), 0, s, n, x, p, r, Ap, invd, part, stride, cnt + 1, st, hist);
+            hipLaunchKernelGGL((k_pcg_update<T, 2>), dim3(nb), dim3(kBlock), 0, s, n, x, p, r, Ap, invd, part, stride, (const KState *)st);
         if ((rc = launch_check("k_pcg_update"))) return rc;
+        if (mode == 0)
+            hipLaunchKernelGGL((k_fin_update<T, 0>), dim3(1), dim3(kFinBlock), 0, s, (const double *)part, stride, nb, st, hist);
+        else if (mode == 1)
+            hipLaunchKernelGGL((k_fin_update<T, 1>), dim3(1), dim3(kFinBlock), 0, s, (const double *)part, stride, nb, st, hist);
+        else
+            hipLaunchKernelGGL((k_fin_update<T, 2>), dim3(1), dim3(kFinBlock), 0, s, (const double *)part, stride, nb, st, hist);
+        if ((rc = launch_check("k_fin_update"))) return rc;
         if (mode == 2) {
             if ((rc = prec_apply_raw(P, r, z, &st->done))) return rc;
-            hipLaunchKernelGGL(k_pcg_rz<T>, dim3(nb), dim3(kBlock), 0, s, n, r, z, part, stride, cnt + 2, st);
+            hipLaunchKernelGGL(k_pcg_rz<T>, dim3(nb), dim3(kBlock), 0, s, n, r, z, part, stride, (const KState *)st);
             if ((rc = launch_check("k_pcg_rz"))) return rc;
+            hipLaunchKernelGGL(k_fin_rz<T>, dim3(1), dim3(kFinBlock), 0, s, (const double *)part, stride, nb, st);
+            if ((rc = launch_check("k_fin_rz"))) return rc;
         }
         if (mode == 0)
             hipLaunchKernelGGL((k_pcg_p<T, 0>), dim3(nb), dim3(kBlock), 0, s, n, p, r, z, invd, st);
@@ -301,17 +335,16 @@ struct PcgWork {
         R.mode = (!P || P->kind == HFH_PREC_NONE) ? 0 : (P->kind == HFH_PREC_JACOBI ? 1 : 2);
         R.invd = (P && P->kind == HFH_PREC_JACOBI) ? static_cast<const T *>(P->invd) : nullptr;
         R.nb = stream_blocks(c, n);
-        R.stride = std::max(A->nrblk, kRedBlocks);
+        R.stride = kRedBlocks;
         int rc;
         const size_t vb = sizeof(T) * static_cast<size_t>(std::max(n, 1));
         if ((rc = w.alloc(reinterpret_cast<void **>(&R.r), vb)) || (rc = w.alloc(reinterpret_cast<void **>(&R.p), vb)) ||
             (rc = w.alloc(reinterpret_cast<void **>(&R.z), vb)) || (rc = w.alloc(reinterpret_cast<void **>(&R.Ap), vb)) ||
             (rc = w.alloc(reinterpret_cast<void **>(&R.part), sizeof(double) * 2 * R.stride)) ||
-            (rc = w.alloc(reinterpret_cast<void **>(&R.cnt), sizeof(unsigned) * 4)) ||
+            (rc = w.alloc(reinterpret_cast<void **>(&R.spart), sizeof(double) * std::max(A->nwg, 1))) ||
             (rc = w.alloc(reinterpret_cast<void **>(&st), sizeof(KState))) ||
             (rc = w.alloc(reinterpret_cast<void **>(&hist), sizeof(double) * (static_cast<size_t>(maxits) + 2))))
             return rc;
-        HFH_HIP(hipMemsetAsync(R.cnt, 0, sizeof(unsigned) * 4, c->stream));
         R.st = st;
         R.hist = hist;
         return HFH_OK;
@@ -330,7 +363,7 @@ struct PcgWork {
         HFH_HIP(hipEventCreate(&e0));
         HFH_HIP(hipEventCreate(&e1));
         HFH_HIP(hipEventRecord(e0, s));
-        int rc = spmv_launch_ex(A, x, R.r, 0, nullptr, nullptr, nullptr, nullptr, nullptr);
+        int rc = spmv_launch_ex(A, x, R.r, 0, nullptr, nullptr, nullptr);
         if (!rc) rc = hfh_scaleadd(&vr, -1.0, &vb);
         double rr = 0;
         if (!rc) rc = hfh_dot(&vr, &vr, &rr);
@@ -436,8 +469,7 @@ namespace hfh {
 
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_mgs(int n, T *__restrict__ w, const T *__restrict__ vprev,
-                                                const T *__restrict__ vcur, const double *hprev, double *part,
-                                                int stride, unsigned int *counter, double *hout) {
+                                                const T *__restrict__ vcur, const double *hprev, double *part) {
     __shared__ double sh[kBlock / 64];
     const T h = vprev ? static_cast<T>(-1) * static_cast<T>(*hprev) : static_cast<T>(0);
     double s = 0;
@@ -451,11 +483,17 @@ __global__ __launch_bounds__(kBlock) void k_mgs(int n, T *__restrict__ w, const 
         s += static_cast<double>(wi) * static_cast<double>(other);
     }
     const double v[1] = {block_sum(s, sh)};
-    __shared__ double r1[1];
-    if (!finish_reduce<1>(v, part, stride, blockIdx.x, counter, gridDim.x, r1)) return;
+    store_partials<1>(v, part, kRedBlocks, blockIdx.x);
+}
+
+// h = w.v (norm == false) or ||w|| (norm == true), rounded to T
+template <typename T>
+__global__ __launch_bounds__(kFinBlock) void k_fin_mgs(const double *part, int nb, bool norm, double *hout) {
+    double out[1];
+    reduce_partials<1>(part, kRedBlocks, nb, out);
     if (threadIdx.x == 0) {
-        const T d = static_cast<T>(r1[0]);
-        *hout = vcur ? static_cast<double>(d) : static_cast<double>(static_cast<T>(sqrt(static_cast<double>(d))));
+        const T d = static_cast<T>(out[0]);
+        *hout = norm ? static_cast<double>(static_cast<T>(sqrt(static_cast<double>(d)))) : static_cast<double>(d);
     }
 }
 
@@ -519,17 +557,13 @@ static int gmres_impl(const hfh_csr *A, hfh_prec *P, int size_basis, int flexibl
             if ((rc = w.alloc(reinterpret_cast<void **>(&Z[i]), vb))) return rc;
     T *wv = nullptr, *z = nullptr, **Vd = nullptr, **Zd = nullptr;
     double *part = nullptr, *Hd = nullptr;
-    unsigned int *cnt = nullptr;
     const int nb = stream_blocks(c, n);
-    const int stride = std::max(A->nrblk, kRedBlocks);
     if ((rc = w.alloc(reinterpret_cast<void **>(&wv), vb)) || (rc = w.alloc(reinterpret_cast<void **>(&z), vb)) ||
         (rc = w.alloc(reinterpret_cast<void **>(&Vd), sizeof(T *) * m1)) ||
         (rc = w.alloc(reinterpret_cast<void **>(&Zd), sizeof(T *) * m1)) ||
-        (rc = w.alloc(reinterpret_cast<void **>(&part), sizeof(double) * stride)) ||
-        (rc = w.alloc(reinterpret_cast<void **>(&Hd), sizeof(double) * (m1 + 2))) ||
-        (rc = w.alloc(reinterpret_cast<void **>(&cnt), sizeof(unsigned) * 4)))
+        (rc = w.alloc(reinterpret_cast<void **>(&part), sizeof(double) * kRedBlocks)) ||
+        (rc = w.alloc(reinterpret_cast<void **>(&Hd), sizeof(double) * (m1 + 2))))
         return rc;
-    HFH_HIP(hipMemsetAsync(cnt, 0, sizeof(unsigned) * 4, s));
     HFH_HIP(hipMemcpyAsync(Vd, V.data(), sizeof(T *) * m1, hipMemcpyHostToDevice, s));
     if (flexible && precond) HFH_HIP(hipMemcpyAsync(Zd, Z.data(), sizeof(T *) * m1, hipMemcpyHostToDevice, s));
 
@@ -539,15 +573,22 @@ static int gmres_impl(const hfh_csr *A, hfh_prec *P, int size_basis, int flexibl
     HFH_HIP(hipEventRecord(e0, s));
 
     auto residual = [&](T *dst) -> int {
-        int r2 = spmv_launch_ex(A, x, dst, 0, nullptr, nullptr, nullptr, nullptr, nullptr);
+        int r2 = spmv_launch_ex(A, x, dst, 0, nullptr, nullptr, nullptr);
         if (r2) return r2;
         hipLaunchKernelGGL(k_neg_add<T>, dim3(nb), dim3(kBlock), 0, s, n, dst, b);
         return launch_check("k_neg_add");
     };
-    auto norm = [&](T *v, double *out) -> int {
-        hipLaunchKernelGGL(k_mgs<T>, dim3(nb), dim3(kBlock), 0, s, n, v, (const T *)nullptr, (const T *)nullptr,
-                           (const double *)nullptr, part, stride, cnt, Hd + m1);
+    // one MGS launch + its finisher: w -= h_prev v_prev (if vprev); h_out = w.vcur or ||w||
+    auto mgs = [&](T *wv_, const T *vprev, const T *vcur, const double *hprev, double *hout) -> int {
+        hipLaunchKernelGGL(k_mgs<T>, dim3(nb), dim3(kBlock), 0, s, n, wv_, vprev, vcur, hprev, part);
         int r2 = launch_check("k_mgs");
+        if (r2) return r2;
+        hipLaunchKernelGGL(k_fin_mgs<T>, dim3(1), dim3(kFinBlock), 0, s, (const double *)part, nb, vcur == nullptr,
+                           hout);
+        return launch_check("k_fin_mgs");
+    };
+    auto norm = [&](T *v, double *out) -> int {
+        int r2 = mgs(v, nullptr, nullptr, nullptr, Hd + m1);
         if (r2) return r2;
         return fetch(c, Hd + m1, out, 1);
     };
@@ -619,16 +660,11 @@ static int gmres_impl(const hfh_csr *A, hfh_prec *P, int size_basis, int flexibl
             } else {
                 az = V[j];
             }
-            if ((rc = spmv_launch_ex(A, az, wv, 0, nullptr, nullptr, nullptr, nullptr, nullptr))) return rc;
-            for (int i = 0; i <= j; ++i) {
-                hipLaunchKernelGGL(k_mgs<T>, dim3(nb), dim3(kBlock), 0, s, n, wv, i > 0 ? (const T *)V[i - 1] : (const T *)nullptr,
-                                   (const T *)V[i], (const double *)(Hd + (i > 0 ? i - 1 : 0)), part, stride, cnt, Hd + i);
-                if ((rc = launch_check("k_mgs"))) return rc;
-            }
+            if ((rc = spmv_launch_ex(A, az, wv, 0, nullptr, nullptr, nullptr))) return rc;
+            for (int i = 0; i <= j; ++i)
+                if ((rc = mgs(wv, i > 0 ? V[i - 1] : nullptr, V[i], i > 0 ? Hd + i - 1 : nullptr, Hd + i))) return rc;
             // w -= h_j v_j ; h_{j+1} = ||w||
-            hipLaunchKernelGGL(k_mgs<T>, dim3(nb), dim3(kBlock), 0, s, n, wv, (const T *)V[j], (const T *)nullptr,
-                               (const double *)(Hd + j), part, stride, cnt, Hd + j + 1);
-            if ((rc = launch_check("k_mgs"))) return rc;
+            if ((rc = mgs(wv, V[j], nullptr, Hd + j, Hd + j + 1))) return rc;
             hipLaunchKernelGGL(k_scale_into<T>, dim3(nb), dim3(kBlock), 0, s, n, V[j + 1], (const T *)wv, (const double *)(Hd + j + 1));
             if ((rc = launch_check("k_scale_into"))) return rc;
             if ((rc = fetch(c, Hd, col.data(), j + 2))) return rc;
@@ -740,7 +776,7 @@ extern "C" int hfh_dc_solve(const hfh_csr *Ah, const hfh_csr *Al, hfh_prec *Pl, 
     HFH_HIP(hipEventCreate(&e1));
     HFH_HIP(hipEventRecord(e0, s));
     auto true_residual = [&](double *res) -> int {
-        int r2 = spmv_launch_ex(Ah, x->d, rh, 0, nullptr, nullptr, nullptr, nullptr, nullptr);
+        int r2 = spmv_launch_ex(Ah, x->d, rh, 0, nullptr, nullptr, nullptr);
         if (!r2) r2 = hfh_scaleadd(&vr, -1.0, &vb);
         if (!r2) r2 = hfh_nrm2(&vr, res);
         return r2;

@@ -2,71 +2,255 @@
 //
 // SpMV (lMatrix::VectorMult, CPUsimple_csr_lmatrix.cc:75-105) is HBM-bound:
 // per call it must stream val (8 B/nnz) + col (4 B/nnz) + row (4 B/row) and
-// write y (8 B/row); x is gathered through L2 / Infinity Cache.  The rows are
-// partitioned into row blocks holding at most kSpmvTile nonzeros.  A
-// workgroup streams its block's val/col with coalesced, non-temporal loads
-// (so the streams do not evict the x window from L2), forms the products
-// val[j]*x[col[j]] in LDS, and then one lane per row sums its products in
-// ascending j.  The products are rounded before the sum exactly like the
-// reference's `res += val[j]*x[col[j]]`, so rows that fit in one block agree
-// bit for bit with the CPU path.  Rows longer than a block get a workgroup of
-// their own and a tree sum.  Workgroups are remapped so each XCD streams one
-// contiguous eighth of the matrix (its L2 then holds one x window).
+// write y (8 B/row); x is gathered through L2 / Infinity Cache.
+//
+// HFH_SPMV_STREAM: the rows are partitioned into row blocks of at most `tile`
+// nonzeros.  A workgroup streams a block's val/col with coalesced,
+// non-temporal loads (so the streams do not evict the x window from L2),
+// forms the products val[j]*x[col[j]] in LDS, and then one lane per row sums
+// its products in ascending j.  The products are rounded before the sum
+// exactly like the reference's `res += val[j]*x[col[j]]`, so rows that fit in
+// one block agree bit for bit with the CPU path.  Rows longer than a block
+// get a block of their own and a tree sum.
+// HFH_SPMV_VECTOR: GL lanes per row, shuffle-tree row sums (not bitwise).
+// Both are persistent over an XCD-contiguous split: each XCD streams one
+// contiguous eighth of the matrix, so its L2 holds one x window.
 #include "hfh_internal.h"
 
 #include <algorithm>
 
 namespace hfh {
 
-template <typename T, bool ADD, bool DOT>
+// Work split shared by both SpMV kernels: with G >= 8 workgroups (G a
+// multiple of 8, consecutive workgroup ids land on consecutive XCDs) XCD x
+// owns the contiguous item range [x*items/8, (x+1)*items/8) and its G/8
+// workgroups stride through it; below 8 workgroups the items are strided.
+struct Split {
+    int lo, hi, first, step;
+};
+
+__device__ __forceinline__ Split xcd_split(int items, int per) {
+    const int G = gridDim.x, g = blockIdx.x;
+    Split sp;
+    if (G >= kNumXcd) {
+        const int x = g % kNumXcd, slot = g / kNumXcd, nslot = G / kNumXcd;
+        sp.lo = static_cast<int>(static_cast<long long>(items) * x / kNumXcd);
+        sp.hi = static_cast<int>(static_cast<long long>(items) * (x + 1) / kNumXcd);
+        sp.first = sp.lo + slot * per;
+        sp.step = nslot * per;
+    } else {
+        sp.lo = 0;
+        sp.hi = items;
+        sp.first = g * per;
+        sp.step = G * per;
+    }
+    return sp;
+}
+
+// Row-block streaming kernel.  Each workgroup walks its row blocks; a block's
+// val/col stream is loaded coalesced (non-temporal), multiplied by the
+// gathered x into LDS, and each row is then summed by one lane in ascending j.
+template <typename T, bool ADD, bool DOT, int TILE>
 __global__ __launch_bounds__(kBlock) void k_spmv_stream(
     const int *__restrict__ row, const int *__restrict__ col, const T *__restrict__ val,
     const T *__restrict__ x, T *__restrict__ y, const int *__restrict__ rblk, int nblk,
-    const T *__restrict__ w, double *part, unsigned int *counter, double *dst,
-    const int *__restrict__ done) {
+    const T *__restrict__ w, double *part, const int *__restrict__ done) {
     if (done && *done) return;
-    __shared__ T prod[kSpmvTile];
+    __shared__ T prod[TILE];
     __shared__ T sht[kBlock / 64];
     __shared__ double shd[kBlock / 64];
-    const int b = xcd_remap(blockIdx.x, nblk);
-    const int r0 = rblk[b], r1 = rblk[b + 1];
-    const int j0 = row[r0], j1 = row[r1];
-    const int cnt = j1 - j0;
+    const Split sp = xcd_split(nblk, 1);
     double dsum = 0;
-    if (cnt <= kSpmvTile) {
+    for (int b = sp.first; b < sp.hi; b += sp.step) {
+        const int r0 = rblk[b], r1 = rblk[b + 1];
+        const int j0 = row[r0], j1 = row[r1];
+        const int cnt = j1 - j0;
+        if (cnt <= TILE) {
 #pragma unroll
-        for (int k = 0; k < kSpmvTile / kBlock; ++k) {
-            const int e = threadIdx.x + k * kBlock;
+            for (int k = 0; k < TILE / kBlock; ++k) {
+                const int e = threadIdx.x + k * kBlock;
+                if (e < cnt) {
+                    const int c = __builtin_nontemporal_load(&col[j0 + e]);
+                    const T v = __builtin_nontemporal_load(&val[j0 + e]);
+                    prod[e] = v * x[c];
+                }
+            }
+            __syncthreads();
+            for (int r = r0 + threadIdx.x; r < r1; r += kBlock) {
+                const int a = row[r] - j0, e = row[r + 1] - j0;
+                T s = 0;
+                for (int j = a; j < e; ++j) s += prod[j];
+                if (ADD) s = y[r] + s;
+                y[r] = s;
+                if (DOT) dsum += static_cast<double>(s) * static_cast<double>(w[r]);
+            }
+            __syncthreads();
+        } else {
+            // one long row (r1 == r0 + 1): strided partial sums, fixed tree
+            T s = 0;
+            for (int j = j0 + threadIdx.x; j < j1; j += kBlock) s += val[j] * x[col[j]];
+            s = block_sum(s, sht);
+            if (threadIdx.x == 0) {
+                if (ADD) s = y[r0] + s;
+                y[r0] = s;
+                if (DOT) dsum += static_cast<double>(s) * static_cast<double>(w[r0]);
+            }
+            __syncthreads();
+        }
+    }
+    if (DOT) {
+        const double v[1] = {block_sum(dsum, shd)};
+        store_partials<1>(v, part, 0, blockIdx.x);
+    }
+}
+
+// Wave-tile kernel: each wavefront owns its tiles (at most WT nonzeros and
+// 63 rows, so the 64 lanes hold the row pointers) and a private LDS slice, so waves never wait for each other.  The
+// tile's row pointers come in with one coalesced load; the val/col loads of
+// the wave's next tile are issued before the current tile's row sums, so a
+// wave keeps a tile of HBM reads in flight while it reduces.  Row sums are
+// sequential in ascending j over the rounded products (bit-exact).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <typename T, bool ADD, bool DOT, int WT>
+__global__ __launch_bounds__(kBlock) void k_spmv_wave(
+    const int *__restrict__ row, const int *__restrict__ col, const T *__restrict__ val,
+    const T *__restrict__ x, T *__restrict__ y, const int *__restrict__ rblk, int nblk,
+    const T *__restrict__ w, double *part, const int *__restrict__ done) {
+    if (done && *done) return;
+    constexpr int WPB = kBlock / 64, PER = WT / 64;
+    __shared__ T prod_all[WPB][WT];
+    __shared__ double shd[kBlock / 64];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    T *prod = prod_all[wave];
+    const Split sp = xcd_split(nblk, WPB);
+    double dsum = 0;
+    int b = sp.first + wave;
+    // registers of the tile being loaded
+    T v[PER];
+    int c[PER];
+    int j0 = 0, cnt = 0, r0 = 0, r1 = 0;
+    auto issue = [&](int bb) {
+        r0 = rblk[bb];
+        r1 = rblk[bb + 1];
+        j0 = row[r0];
+        cnt = row[r1] - j0;
+        if (cnt > WT) return;  // a long row is streamed when it is processed
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const int e = lane + 64 * k;
             if (e < cnt) {
-                const int c = __builtin_nontemporal_load(&col[j0 + e]);
-                const T v = __builtin_nontemporal_load(&val[j0 + e]);
-                prod[e] = v * x[c];
+                c[k] = __builtin_nontemporal_load(&col[j0 + e]);
+                v[k] = __builtin_nontemporal_load(&val[j0 + e]);
             }
         }
-        __syncthreads();
-        for (int r = r0 + threadIdx.x; r < r1; r += kBlock) {
-            const int a = row[r] - j0, e = row[r + 1] - j0;
+    };
+    if (b < sp.hi) issue(b);
+    while (b < sp.hi) {
+        const int cr0 = r0, cr1 = r1, cj0 = j0, ccnt = cnt;
+        const int bn = b + sp.step;
+        if (ccnt > WT) {
+            // one long row (cr1 == cr0 + 1): lane-strided sums, fixed shuffle tree
+            T s = 0;
+            for (int j = cj0 + lane; j < cj0 + ccnt; j += 64) s += val[j] * x[col[j]];
+            s = wave_sum(s);
+            if (lane == 0) {
+                if (ADD) s = y[cr0] + s;
+                y[cr0] = s;
+                if (DOT) dsum += static_cast<double>(s) * static_cast<double>(w[cr0]);
+            }
+            if (bn < sp.hi) issue(bn);
+            b = bn;
+            continue;
+        }
+        // the row pointers of this tile (<= 63 rows + 1)
+        const int rp = (cr0 + lane <= cr1) ? row[cr0 + lane] - cj0 : 0;
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const int e = lane + 64 * k;
+            if (e < ccnt) prod[e] = v[k] * x[c[k]];
+        }
+        if (bn < sp.hi) issue(bn);
+        wave_sync();
+        const int nr = cr1 - cr0;
+        const int a = rp;
+        const int e = __shfl_down(rp, 1, 64);
+        if (lane < nr) {
             T s = 0;
             for (int j = a; j < e; ++j) s += prod[j];
+            const int r = cr0 + lane;
             if (ADD) s = y[r] + s;
             y[r] = s;
             if (DOT) dsum += static_cast<double>(s) * static_cast<double>(w[r]);
         }
-    } else {
-        // one long row (r1 == r0 + 1): strided partial sums, fixed tree
-        T s = 0;
-        for (int j = j0 + threadIdx.x; j < j1; j += kBlock) s += val[j] * x[col[j]];
-        s = block_sum(s, sht);
-        if (threadIdx.x == 0) {
-            if (ADD) s = y[r0] + s;
-            y[r0] = s;
-            if (DOT) dsum = static_cast<double>(s) * static_cast<double>(w[r0]);
-        }
+        wave_sync();
+        b = bn;
     }
     if (DOT) {
         __syncthreads();
+        const double vv[1] = {block_sum(dsum, shd)};
+        store_partials<1>(vv, part, 0, blockIdx.x);
+    }
+}
+
+// Sub-wavefront "vector" kernel: GL lanes per row, U rows per group in
+// flight; lanes stride the row and the GL partial sums are tree-reduced with
+// cross-lane shuffles.
+template <typename T, bool ADD, bool DOT, int GL, int U>
+__global__ __launch_bounds__(kBlock) void k_spmv_vector(
+    int nrow, const int *__restrict__ row, const int *__restrict__ col, const T *__restrict__ val,
+    const T *__restrict__ x, T *__restrict__ y, const T *__restrict__ w, double *part,
+    const int *__restrict__ done) {
+    if (done && *done) return;
+    __shared__ double shd[kBlock / 64];
+    constexpr int GPW = kBlock / GL;  // row groups per workgroup
+    const int lane = threadIdx.x % GL, grp = threadIdx.x / GL;
+    const Split sp = xcd_split(nrow, GPW * U);
+    double dsum = 0;
+    for (int base = sp.first; base < sp.hi; base += sp.step) {
+        int rs[U], js[U], je[U];
+        int maxlen = 0;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            rs[u] = base + u * GPW + grp;
+            const bool ok = rs[u] < sp.hi;
+            js[u] = ok ? row[rs[u]] : 0;
+            je[u] = ok ? row[rs[u] + 1] : 0;
+            maxlen = max(maxlen, je[u] - js[u]);
+        }
+        T s[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) s[u] = 0;
+        for (int k = lane; k < maxlen; k += GL) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int j = js[u] + k;
+                if (j < je[u]) {
+                    const int c = __builtin_nontemporal_load(&col[j]);
+                    const T v = __builtin_nontemporal_load(&val[j]);
+                    s[u] += v * x[c];
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+#pragma unroll
+            for (int off = GL / 2; off > 0; off >>= 1) s[u] += __shfl_xor(s[u], off, GL);
+            if (lane == 0 && rs[u] < sp.hi) {
+                T r = s[u];
+                if (ADD) r = y[rs[u]] + r;
+                y[rs[u]] = r;
+                if (DOT) dsum += static_cast<double>(r) * static_cast<double>(w[rs[u]]);
+            }
+        }
+    }
+    if (DOT) {
         const double v[1] = {block_sum(dsum, shd)};
-        finish_reduce<1>(v, part, 0, b, counter, nblk, dst);
+        store_partials<1>(v, part, 0, blockIdx.x);
     }
 }
 
@@ -98,9 +282,9 @@ __global__ __launch_bounds__(kBlock) void k_cast_vals(long long n, TO *__restric
         y[i] = static_cast<TO>(x[i]);
 }
 
-// Row blocks: consecutive rows while the block holds <= kSpmvTile nonzeros and
-// <= kBlock rows; a longer row forms a block alone.
-static std::vector<int> make_row_blocks(const std::vector<int> &row) {
+// Row blocks: consecutive rows while the block holds <= tile nonzeros and
+// <= max_rows rows; a longer row forms a block alone.
+static std::vector<int> make_row_blocks(const std::vector<int> &row, int tile, int max_rows = kBlock) {
     const int n = static_cast<int>(row.size()) - 1;
     std::vector<int> rb;
     rb.reserve(n / 16 + 2);
@@ -108,43 +292,131 @@ static std::vector<int> make_row_blocks(const std::vector<int> &row) {
     int r = 0;
     while (r < n) {
         const int start = r;
-        if (row[r + 1] - row[r] > kSpmvTile) {
+        if (row[r + 1] - row[r] > tile) {
             r += 1;
         } else {
-            while (r < n && r - start < kBlock && row[r + 1] - row[start] <= kSpmvTile) ++r;
+            while (r < n && r - start < max_rows && row[r + 1] - row[start] <= tile) ++r;
         }
         rb.push_back(r);
     }
     return rb;
 }
 
-int spmv_launch_ex(const hfh_csr *A, const void *x, void *y, int mode, const void *w, double *part,
-                   unsigned int *counter, double *dst, const int *done) {
-    if (A->nrow == 0) return HFH_OK;
-    hipStream_t s = A->ctx->stream;
-    const dim3 g(A->nrblk), bl(kBlock);
-    const bool dot = w != nullptr;
-#define HFH_SPMV_LAUNCH(T, ADD, DOT)                                                                 \
-    hipLaunchKernelGGL((k_spmv_stream<T, ADD, DOT>), g, bl, 0, s, A->row, A->col, (const T *)A->val, \
-                       (const T *)x, (T *)y, A->rblk, A->nrblk, (const T *)w, part, counter, dst, done)
-    if (A->dtype == HFH_F64) {
-        if (mode == 0 && !dot) HFH_SPMV_LAUNCH(double, false, false);
-        else if (mode == 0) HFH_SPMV_LAUNCH(double, false, true);
-        else if (!dot) HFH_SPMV_LAUNCH(double, true, false);
-        else HFH_SPMV_LAUNCH(double, true, true);
-    } else {
-        if (mode == 0 && !dot) HFH_SPMV_LAUNCH(float, false, false);
-        else if (mode == 0) HFH_SPMV_LAUNCH(float, false, true);
-        else if (!dot) HFH_SPMV_LAUNCH(float, true, false);
-        else HFH_SPMV_LAUNCH(float, true, true);
+static int round8(long long g) { return static_cast<int>(g >= kNumXcd ? (g + kNumXcd - 1) / kNumXcd * kNumXcd : std::max(g, 1LL)); }
+
+// Configures the SpMV kernel of A (host row pointer in A->h_row).
+int spmv_configure(hfh_csr *A, int algo, int p1, int p2) {
+    const int n = A->nrow;
+    const double mean = n ? static_cast<double>(A->nnz) / n : 0.0;
+    if (algo == 0) {  // automatic choice
+        algo = HFH_SPMV_STREAM;
+        p1 = 2048;
+        p2 = 0;
     }
-#undef HFH_SPMV_LAUNCH
-    return launch_check("k_spmv_stream");
+    const int cus = std::max(A->ctx->num_cu, 1);
+    if (algo == HFH_SPMV_STREAM) {
+        const int tile = p1 ? p1 : 2048;
+        HFH_CHECK(tile == 512 || tile == 1024 || tile == 2048 || tile == 4096, HFH_ERR_INVALID,
+                  "spmv: stream tile %d not in {512, 1024, 2048, 4096}", tile);
+        std::vector<int> rb = make_row_blocks(A->h_row, tile);
+        if (A->rblk) (void)hipFree(A->rblk);
+        A->rblk = nullptr;
+        int rc = dev_alloc(reinterpret_cast<void **>(&A->rblk), sizeof(int) * rb.size());
+        if (rc) return rc;
+        A->nrblk = static_cast<int>(rb.size()) - 1;
+        HFH_HIP(hipMemcpy(A->rblk, rb.data(), sizeof(int) * rb.size(), hipMemcpyHostToDevice));
+        const long long g = p2 > 0 ? std::min<long long>(p2, A->nrblk) : std::min<long long>(A->nrblk, static_cast<long long>(cus) * 16);
+        A->spmv_algo = HFH_SPMV_STREAM;
+        A->spmv_p1 = tile;
+        A->nwg = round8(std::max<long long>(g, 1));
+    } else if (algo == HFH_SPMV_WAVE) {
+        const int tile = p1 ? p1 : 512;
+        HFH_CHECK(tile == 256 || tile == 512 || tile == 1024, HFH_ERR_INVALID,
+                  "spmv: wave tile %d not in {256, 512, 1024}", tile);
+        std::vector<int> rb = make_row_blocks(A->h_row, tile, 63);  // 63 rows: 64 row pointers, one per lane
+        if (A->rblk) (void)hipFree(A->rblk);
+        A->rblk = nullptr;
+        int rc = dev_alloc(reinterpret_cast<void **>(&A->rblk), sizeof(int) * rb.size());
+        if (rc) return rc;
+        A->nrblk = static_cast<int>(rb.size()) - 1;
+        HFH_HIP(hipMemcpy(A->rblk, rb.data(), sizeof(int) * rb.size(), hipMemcpyHostToDevice));
+        const long long need = (A->nrblk + kBlock / 64 - 1) / (kBlock / 64);
+        const long long g = p2 > 0 ? std::min<long long>(p2, need) : std::min<long long>(need, static_cast<long long>(cus) * 8);
+        A->spmv_algo = HFH_SPMV_WAVE;
+        A->spmv_p1 = tile;
+        A->nwg = round8(std::max<long long>(g, 1));
+    } else if (algo == HFH_SPMV_VECTOR) {
+        int gl = p1;
+        if (gl == 0) {
+            gl = 4;
+            while (gl < 64 && gl < mean) gl *= 2;
+        }
+        HFH_CHECK(gl == 4 || gl == 8 || gl == 16 || gl == 32 || gl == 64, HFH_ERR_INVALID,
+                  "spmv: vector group %d not in {4, 8, 16, 32, 64}", gl);
+        const long long rows_per_wg = (kBlock / gl) * 2;
+        const long long need = (n + rows_per_wg - 1) / rows_per_wg;
+        const long long g = p2 > 0 ? p2 : std::min<long long>(need, static_cast<long long>(cus) * 16);
+        A->spmv_algo = HFH_SPMV_VECTOR;
+        A->spmv_p1 = gl;
+        A->nwg = round8(std::max<long long>(g, 1));
+    } else {
+        set_error("spmv: unknown algorithm %d", algo);
+        return HFH_ERR_INVALID;
+    }
+    return HFH_OK;
+}
+
+template <typename T, bool ADD, bool DOT>
+static void launch_t(const hfh_csr *A, const void *x, void *y, const void *w, double *part, const int *done) {
+    hipStream_t s = A->ctx->stream;
+    const dim3 g(A->nwg), bl(kBlock);
+    const T *xv = (const T *)x, *wv = (const T *)w, *val = (const T *)A->val;
+    T *yv = (T *)y;
+    if (A->spmv_algo == HFH_SPMV_WAVE) {
+        switch (A->spmv_p1) {
+        case 256: hipLaunchKernelGGL((k_spmv_wave<T, ADD, DOT, 256>), g, bl, 0, s, A->row, A->col, val, xv, yv, A->rblk, A->nrblk, wv, part, done); break;
+        case 512: hipLaunchKernelGGL((k_spmv_wave<T, ADD, DOT, 512>), g, bl, 0, s, A->row, A->col, val, xv, yv, A->rblk, A->nrblk, wv, part, done); break;
+        default: hipLaunchKernelGGL((k_spmv_wave<T, ADD, DOT, 1024>), g, bl, 0, s, A->row, A->col, val, xv, yv, A->rblk, A->nrblk, wv, part, done); break;
+        }
+    } else if (A->spmv_algo == HFH_SPMV_STREAM) {
+        switch (A->spmv_p1) {
+        case 512: hipLaunchKernelGGL((k_spmv_stream<T, ADD, DOT, 512>), g, bl, 0, s, A->row, A->col, val, xv, yv, A->rblk, A->nrblk, wv, part, done); break;
+        case 1024: hipLaunchKernelGGL((k_spmv_stream<T, ADD, DOT, 1024>), g, bl, 0, s, A->row, A->col, val, xv, yv, A->rblk, A->nrblk, wv, part, done); break;
+        case 2048: hipLaunchKernelGGL((k_spmv_stream<T, ADD, DOT, 2048>), g, bl, 0, s, A->row, A->col, val, xv, yv, A->rblk, A->nrblk, wv, part, done); break;
+        default: hipLaunchKernelGGL((k_spmv_stream<T, ADD, DOT, 4096>), g, bl, 0, s, A->row, A->col, val, xv, yv, A->rblk, A->nrblk, wv, part, done); break;
+        }
+    } else {
+        switch (A->spmv_p1) {
+        case 4: hipLaunchKernelGGL((k_spmv_vector<T, ADD, DOT, 4, 2>), g, bl, 0, s, A->nrow, A->row, A->col, val, xv, yv, wv, part, done); break;
+        case 8: hipLaunchKernelGGL((k_spmv_vector<T, ADD, DOT, 8, 2>), g, bl, 0, s, A->nrow, A->row, A->col, val, xv, yv, wv, part, done); break;
+        case 16: hipLaunchKernelGGL((k_spmv_vector<T, ADD, DOT, 16, 2>), g, bl, 0, s, A->nrow, A->row, A->col, val, xv, yv, wv, part, done); break;
+        case 32: hipLaunchKernelGGL((k_spmv_vector<T, ADD, DOT, 32, 2>), g, bl, 0, s, A->nrow, A->row, A->col, val, xv, yv, wv, part, done); break;
+        default: hipLaunchKernelGGL((k_spmv_vector<T, ADD, DOT, 64, 2>), g, bl, 0, s, A->nrow, A->row, A->col, val, xv, yv, wv, part, done); break;
+        }
+    }
+}
+
+int spmv_launch_ex(const hfh_csr *A, const void *x, void *y, int mode, const void *w, double *part,
+                   const int *done) {
+    if (A->nrow == 0) return HFH_OK;
+    const bool dot = w != nullptr;
+    if (A->dtype == HFH_F64) {
+        if (mode == 0 && !dot) launch_t<double, false, false>(A, x, y, w, part, done);
+        else if (mode == 0) launch_t<double, false, true>(A, x, y, w, part, done);
+        else if (!dot) launch_t<double, true, false>(A, x, y, w, part, done);
+        else launch_t<double, true, true>(A, x, y, w, part, done);
+    } else {
+        if (mode == 0 && !dot) launch_t<float, false, false>(A, x, y, w, part, done);
+        else if (mode == 0) launch_t<float, false, true>(A, x, y, w, part, done);
+        else if (!dot) launch_t<float, true, false>(A, x, y, w, part, done);
+        else launch_t<float, true, true>(A, x, y, w, part, done);
+    }
+    return launch_check("k_spmv");
 }
 
 int spmv_launch(const hfh_csr *A, const void *x, void *y, int mode) {
     ScopedKernelTimer t(A->ctx, 0);
-    return spmv_launch_ex(A, x, y, mode, nullptr, nullptr, nullptr, nullptr, nullptr);
+    return spmv_launch_ex(A, x, y, mode, nullptr, nullptr, nullptr);
 }
 
 }  // namespace hfh
@@ -210,18 +482,13 @@ extern "C" int hfh_csr_upload(hfh_csr *A, const int *row, const int *col, const 
     }
     hipStream_t s = A->ctx->stream;
     A->h_row.assign(row, row + n + 1);
-    std::vector<int> rb = make_row_blocks(A->h_row);
-    if (A->rblk) (void)hipFree(A->rblk);
-    A->rblk = nullptr;
-    int rc = dev_alloc(reinterpret_cast<void **>(&A->rblk), sizeof(int) * rb.size());
+    int rc = spmv_configure(A, A->spmv_algo_req, A->spmv_p1_req, A->spmv_p2_req);
     if (rc) return rc;
-    A->nrblk = static_cast<int>(rb.size()) - 1;
     HFH_HIP(hipMemcpyAsync(A->row, row, sizeof(int) * (n + 1), hipMemcpyHostToDevice, s));
     if (A->nnz) {
         HFH_HIP(hipMemcpyAsync(A->col, col, sizeof(int) * static_cast<size_t>(A->nnz), hipMemcpyHostToDevice, s));
         HFH_HIP(hipMemcpyAsync(A->val, val, A->elem() * static_cast<size_t>(A->nnz), hipMemcpyHostToDevice, s));
     }
-    HFH_HIP(hipMemcpyAsync(A->rblk, rb.data(), sizeof(int) * rb.size(), hipMemcpyHostToDevice, s));
     if (n > 0) {
         hipLaunchKernelGGL(k_find_diag<int>, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, n, A->row, A->col, A->diag);
         if ((rc = launch_check("k_find_diag"))) return rc;
@@ -259,12 +526,10 @@ extern "C" int hfh_csr_cast(hfh_csr *dst, const hfh_csr *src) {
     if (src->nnz) HFH_HIP(hipMemcpyAsync(dst->col, src->col, sizeof(int) * static_cast<size_t>(src->nnz), hipMemcpyDeviceToDevice, s));
     if (src->nrow) HFH_HIP(hipMemcpyAsync(dst->diag, src->diag, sizeof(int) * static_cast<size_t>(src->nrow), hipMemcpyDeviceToDevice, s));
     dst->h_row = src->h_row;
-    if (dst->rblk) (void)hipFree(dst->rblk);
-    dst->rblk = nullptr;
-    int rc = dev_alloc(reinterpret_cast<void **>(&dst->rblk), sizeof(int) * (src->nrblk + 1));
+    int rc = spmv_configure(dst, dst->spmv_algo_req ? dst->spmv_algo_req : src->spmv_algo,
+                            dst->spmv_algo_req ? dst->spmv_p1_req : src->spmv_p1,
+                            dst->spmv_algo_req ? dst->spmv_p2_req : 0);
     if (rc) return rc;
-    dst->nrblk = src->nrblk;
-    HFH_HIP(hipMemcpyAsync(dst->rblk, src->rblk, sizeof(int) * (src->nrblk + 1), hipMemcpyDeviceToDevice, s));
     if (src->nnz) {
         const long long nnz = src->nnz;
         const int nb = static_cast<int>(std::min<long long>((nnz + kBlock - 1) / kBlock, 4096));
@@ -313,4 +578,15 @@ extern "C" int hfh_csr_invdiag(const hfh_csr *A, hfh_vec *invd) {
     else
         hipLaunchKernelGGL(k_invdiag<float>, dim3(nb), dim3(kBlock), 0, A->ctx->stream, A->nrow, A->diag, (const float *)A->val, (float *)invd->d);
     return launch_check("k_invdiag");
+}
+
+extern "C" int hfh_csr_set_spmv(hfh_csr *A, int algo, int p1, int p2) {
+    HFH_CHECK(A, HFH_ERR_INVALID, "hfh_csr_set_spmv: null matrix");
+    HFH_CHECK(algo >= 0 && algo <= HFH_SPMV_WAVE && p2 >= 0, HFH_ERR_INVALID, "hfh_csr_set_spmv: bad argument");
+    A->spmv_algo_req = algo;
+    A->spmv_p1_req = p1;
+    A->spmv_p2_req = p2;
+    if (!A->has_structure) return HFH_OK;
+    HFH_HIP(hipStreamSynchronize(A->ctx->stream));
+    return spmv_configure(A, algo, p1, p2);
 }

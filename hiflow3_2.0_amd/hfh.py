@@ -21,6 +21,7 @@ OK, ERR_INVALID, ERR_NOMEM, ERR_HIP, ERR_ZERO_PIVOT, ERR_NOT_CONVERGED, ERR_BUDG
     0, -1, -2, -3, -4, -5, -6, -7
 
 Q1_2D, Q1_3D, FV_DIFFUSION, UPWIND_CONVDIFF = 1, 2, 3, 4
+SPMV_AUTO, SPMV_STREAM, SPMV_VECTOR, SPMV_WAVE = 0, 1, 2, 3
 
 
 class HfhError(RuntimeError):
@@ -89,6 +90,7 @@ HIP_API = {
     "hfh_spmv": (_i, [_vp, _vp, _vp]),
     "hfh_spmv_add": (_i, [_vp, _vp, _vp]),
     "hfh_csr_invdiag": (_i, [_vp, _vp]),
+    "hfh_csr_set_spmv": (_i, [_vp, _i, _i, _i]),
     "hfh_jacobi_create": (_i, [_vp, _P(_vp)]),
     "hfh_sgs_create": (_i, [_vp, _i, _vp, _vp, _i, _vp, _vp, _P(_vp)]),
     "hfh_mcsgs_create": (_i, [_vp, _i, _vp, _d, _P(_vp)]),
@@ -101,6 +103,15 @@ HIP_API = {
     "hfh_dc_solve": (_i, [_vp, _vp, _vp, _vp, _vp, _P(Control), _P(Control), _P(DcInfo), _vp]),
 }
 
+class LmpParams(C.Structure):
+    _fields_ = [("maxits", C.c_int), ("atol", C.c_double), ("rtol", C.c_double), ("dtol", C.c_double)]
+
+
+class LmpResult(C.Structure):
+    _fields_ = [("iters", C.c_int), ("ret", C.c_int), ("state", C.c_int), ("res", C.c_double),
+                ("hist", C.POINTER(C.c_double)), ("hist_cap", C.c_int), ("nhist", C.c_int)]
+
+
 LMP_API = {
     "hfl_problem_size": (_i, [_i, _i, _i, _i, _d, _d, C.c_ulonglong, _P(_i), _P(_ll)]),
     "hfl_problem_fill": (_i, [_i, _i, _i, _i, _d, _d, C.c_ulonglong, _vp, _vp, _vp, _vp]),
@@ -111,7 +122,12 @@ LMP_API = {
     "hfl_levels_upper": (_i, [_i, _vp, _vp, _P(_i), _vp, _vp]),
     "hfl_ilu0": (_i, [_i, _vp, _vp, _vp]),
     "hfl_csr_permute": (_i, [_i, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "hfl_last_error": (C.c_char_p, []),
+    "hfl_lmp_solve": (_i, [_i, _i, _i, _d, _i, _i, _vp, _vp, _vp, _vp, _vp, _P(LmpParams), _P(LmpResult)]),
+    "hfl_lmp_mpir": (_i, [_i, _i, _vp, _vp, _vp, _vp, _vp, _i, _d, _d, _d, _P(LmpParams), _P(LmpResult)]),
 }
+
+SOLVER_CG, SOLVER_GMRES, SOLVER_FGMRES = 0, 1, 2
 
 
 def _declare(lib, api):
@@ -334,6 +350,10 @@ class CsrMatrix:
     def spmv_add(self, x, y):
         _ck(hip().hfh_spmv_add(self.h, x.h, y.h))
 
+    def set_spmv(self, algo, param1=0, param2=0):
+        """Select the SpMV kernel (SPMV_AUTO / SPMV_STREAM / SPMV_VECTOR)."""
+        _ck(hip().hfh_csr_set_spmv(self.h, algo, param1, param2))
+
     def invdiag(self):
         v = Vector(self.ctx, self.nrow, self.dtype)
         _ck(hip().hfh_csr_invdiag(self.h, v.h))
@@ -501,3 +521,46 @@ def csr_permute(row, col, val, perm):
     if lmp().hfl_csr_permute(n, _p(row), _p(col), _p(val), _p(perm), _p(prow), _p(pcol), _p(pval)):
         raise ValueError("invalid permutation")
     return prow, pcol, pval
+
+
+def _lmp_ck(rc):
+    if rc != OK:
+        raise HfhError(rc, lmp().hfl_last_error().decode())
+
+
+def lmp_solve(row, col, val, b, x0=None, solver=SOLVER_CG, precond=PREC_NONE, m=30, omega=1.0, single=False,
+              maxits=1000, atol=1e-15, rtol=1e-8, dtol=1e6):
+    """Runs the C++ LinearSolver templates (CG / GMRES / FGMRES) on HIP vectors."""
+    n = len(row) - 1
+    row = np.ascontiguousarray(row, np.int32)
+    col = np.ascontiguousarray(col, np.int32)
+    val = np.ascontiguousarray(val, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    x = np.array(np.zeros(n) if x0 is None else x0, np.float64)
+    hist = np.zeros(maxits + 2)
+    res = LmpResult()
+    res.hist = hist.ctypes.data_as(C.POINTER(C.c_double))
+    res.hist_cap = len(hist)
+    prm = LmpParams(maxits, atol, rtol, dtol)
+    _lmp_ck(lmp().hfl_lmp_solve(solver, precond, m, omega, 1 if single else 0, n, _p(row), _p(col), _p(val),
+                                _p(b), _p(x), C.byref(prm), C.byref(res)))
+    return x, {"iters": res.iters, "ret": res.ret, "state": res.state, "res": res.res,
+               "hist": hist[: res.nhist].copy()}
+
+
+def lmp_mpir(row, col, val, b, x0=None, precond_low=PREC_JACOBI, max_ecl=20, abs_tol=1e-12, rel_tol=1e-12,
+             div_tol=1e6, ecl=(1000, 0.0, 1e-3, 1e6)):
+    n = len(row) - 1
+    row = np.ascontiguousarray(row, np.int32)
+    col = np.ascontiguousarray(col, np.int32)
+    val = np.ascontiguousarray(val, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    x = np.array(np.zeros(n) if x0 is None else x0, np.float64)
+    hist = np.zeros(max_ecl + 2)
+    res = LmpResult()
+    res.hist = hist.ctypes.data_as(C.POINTER(C.c_double))
+    res.hist_cap = len(hist)
+    _lmp_ck(lmp().hfl_lmp_mpir(precond_low, n, _p(row), _p(col), _p(val), _p(b), _p(x), max_ecl, abs_tol, rel_tol,
+                               div_tol, C.byref(LmpParams(*ecl)), C.byref(res)))
+    return x, {"iters": res.iters, "ret": res.ret, "n_ecl": res.state, "res": res.res,
+               "hist": hist[: res.nhist].copy()}

@@ -3,12 +3,19 @@
 #include "../../include/hflmp.h"
 
 #include <cstddef>
+#include <cstdio>
 #include <vector>
 
 #include "analysis.h"
 #include "problems.h"
 
 using namespace hflmp;
+
+static thread_local char g_err[512] = "";
+
+void hflmp_set_error(const char *msg) { snprintf(g_err, sizeof g_err, "%s", msg); }
+
+extern "C" const char *hfl_last_error(void) { return g_err; }
 
 static ProblemSpec spec(int kind, int nx, int ny, int nz, double p0, double p1, unsigned long long seed) {
     return ProblemSpec{kind, nx, ny, nz, p0, p1, static_cast<uint64_t>(seed)};

@@ -118,6 +118,17 @@ int hfh_csr_cast(hfh_csr *dst, const hfh_csr *src);
 int hfh_spmv(const hfh_csr *A, const hfh_vec *x, hfh_vec *y);
 /* VectorMultAdd: y += A x (lmatrix.h:336; lmatrix_csr_cpu.cc:1354-1375) */
 int hfh_spmv_add(const hfh_csr *A, const hfh_vec *x, hfh_vec *y);
+/* SpMV kernel choice (no reference counterpart; the reference selects its
+ * SpMV by platform/implementation enums, la_global.h:40-50).
+ * HFH_SPMV_STREAM: row-block LDS streaming, param1 = nonzeros per block
+ * (512/1024/2048/4096, 0 = 2048), bit-identical to the CPU row sums;
+ * HFH_SPMV_VECTOR: param1 lanes per row (4..64, 0 = from the mean row
+ * length); HFH_SPMV_WAVE: per-wavefront tiles of param1 nonzeros (256/512/
+ * 1024) and at most 64 rows, bit-identical like STREAM.
+ * param2: workgroups (0 = automatic).  algo 0 = automatic.
+ * Takes effect at once on an uploaded matrix and on later uploads. */
+enum hfh_spmv_algo { HFH_SPMV_AUTO = 0, HFH_SPMV_STREAM = 1, HFH_SPMV_VECTOR = 2, HFH_SPMV_WAVE = 3 };
+int hfh_csr_set_spmv(hfh_csr *A, int algo, int param1, int param2);
 /* extract_invdiagelements over all rows (lmatrix_csr_cpu.cc:2414-2432) */
 int hfh_csr_invdiag(const hfh_csr *A, hfh_vec *invd);
 

@@ -41,7 +41,47 @@ int hfl_ilu0(int n, const int *row, const int *col, double *val);
 int hfl_csr_permute(int n, const int *row, const int *col, const double *val, const int *perm,
                     int *prow, int *pcol, double *pval);
 
+/* ---- the LinearSolver templates on the HIP platform ----
+ * CG<LAD> (linear_solver/cg.cc:84-248), GMRES<LAD> (gmres.cc:285-510) and
+ * FGMRES<LAD> (fgmres.cc:75-262) with PreconditionerBlockJacobiStand
+ * (Jacobi / SGS / ILU(0), preconditioner_bjacobi_standard.cc:45-150) or
+ * PreconditionerMultiColoring (SGS, preconditioner_mc.cc:49-194), over
+ * single-rank CoupledVector / CoupledMatrix of HIP lVectors / lMatrices.
+ * Every Dot returns a host scalar, as through the reference interface. */
+enum hfl_solver { HFL_SOLVER_CG = 0, HFL_SOLVER_GMRES = 1, HFL_SOLVER_FGMRES = 2 };
+enum hfl_prec { HFL_PREC_NONE = 0, HFL_PREC_JACOBI = 1, HFL_PREC_SGS = 2, HFL_PREC_MCSGS = 3,
+                HFL_PREC_ILU0 = 4 };
+
+typedef struct {
+    int maxits;
+    double atol, rtol, dtol;
+} hfl_solve_params;
+
+typedef struct {
+    int iters;       /* LinearSolver::iter() */
+    int ret;         /* LinearSolverState: 0 success, 1 exceeded, 2 error */
+    int state;       /* IterateControl state (MPIR: low-precision loops taken) */
+    double res;      /* LinearSolver::res() */
+    double *hist;    /* optional: residual history (caller buffer) */
+    int hist_cap;    /* capacity of hist */
+    int nhist;       /* entries written */
+} hfl_solve_result;
+
+const char *hfl_last_error(void);
+/* Solves A x = b (x: start vector in, solution out) on device 0.  single != 0
+ * runs the FP32 descriptor.  For HFL_PREC_MCSGS the system is coloured and
+ * renumbered first and x is returned in the original numbering. */
+int hfl_lmp_solve(int solver, int precond, int size_basis, double omega, int single, int n,
+                  const int *row, const int *col, const double *val, const double *b, double *x,
+                  const hfl_solve_params *params, hfl_solve_result *result);
+/* MPIR<LADescriptorHipD, LADescriptorHipS> (mpir.cc:69-327) with a CG<float>
+ * error-correction solver (Jacobi or no preconditioner). */
+int hfl_lmp_mpir(int precond_low, int n, const int *row, const int *col, const double *val,
+                 const double *b, double *x, int max_ecl, double abs_tol, double rel_tol, double div_tol,
+                 const hfl_solve_params *ecl, hfl_solve_result *result);
+
 #ifdef __cplusplus
 }
+void hflmp_set_error(const char *msg);
 #endif
 #endif

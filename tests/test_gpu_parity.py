@@ -24,11 +24,16 @@ def rand_csr(rng, n, ncol, lens):
     return row, col, val
 
 
-def spmv_case(pkg, ctx, oracle, row, col, val, ncol=None, dtype=np.float64, seed=0):
+ALGOS = [(1, 2048), (1, 512), (3, 512), (3, 1024), (3, 256), (2, 0)]  # (algorithm, param)
+BITWISE = {1, 3}  # row-block and wave-tile kernels sum rows in the reference order
+
+
+def spmv_case(pkg, ctx, oracle, row, col, val, ncol=None, dtype=np.float64, seed=0, algo=(1, 2048)):
     ncol = len(row) - 1 if ncol is None else ncol
     rng = np.random.default_rng(seed)
     x = rng.uniform(-1, 1, ncol).astype(dtype)
     A = pkg.CsrMatrix(ctx, row, col, val.astype(dtype), ncol=ncol)
+    A.set_spmv(*algo)
     X = pkg.Vector(ctx, x, dtype)
     Y = pkg.Vector(ctx, len(row) - 1, dtype)
     A.spmv(X, Y)
@@ -49,8 +54,9 @@ def test_spmv_reference_known_answers(pkg, ctx, oracle):
     assert np.allclose(Y.numpy(), [4.7, -6.5, 3.7], atol=1e-12)
 
 
+@pytest.mark.parametrize("algo", ALGOS)
 @pytest.mark.parametrize("case", ["q1_2d_256", "q1_3d_32", "fv_24", "upwind_20"])
-def test_spmv_bitwise_on_configs(pkg, ctx, oracle, case):
+def test_spmv_bitwise_on_configs(pkg, ctx, oracle, case, algo):
     if case == "q1_2d_256":
         row, col, val, _ = pkg.problem(pkg.Q1_2D, 255)             # config 1 at full size
     elif case == "q1_3d_32":
@@ -59,24 +65,31 @@ def test_spmv_bitwise_on_configs(pkg, ctx, oracle, case):
         row, col, val, _ = pkg.problem(pkg.FV_DIFFUSION, 24, p0=1.0, seed=1)
     else:
         row, col, val, _ = pkg.problem(pkg.UPWIND_CONVDIFF, 20, p0=1e-2, p1=5.0)
-    y, ref, *_ = spmv_case(pkg, ctx, oracle, row, col, val)
-    # products rounded then summed in the reference order: bit-identical rows
-    assert np.array_equal(y, ref)
-    y32, ref32, *_ = spmv_case(pkg, ctx, oracle, row, col, val, dtype=np.float32)
-    assert np.array_equal(y32, ref32)
+    y, ref, *_ = spmv_case(pkg, ctx, oracle, row, col, val, algo=algo)
+    y32, ref32, *_ = spmv_case(pkg, ctx, oracle, row, col, val, dtype=np.float32, algo=algo)
+    if algo[0] in BITWISE:
+        # products rounded then summed in the reference order: bit-identical rows
+        assert np.array_equal(y, ref)
+        assert np.array_equal(y32, ref32)
+    else:
+        assert relnorm(y, ref) < SPMV_TOL
+        assert relnorm(y32, ref32) < 1e-6
 
 
-def test_spmv_ragged_and_long_rows(pkg, ctx, oracle):
+@pytest.mark.parametrize("algo", ALGOS)
+def test_spmv_ragged_and_long_rows(pkg, ctx, oracle, algo):
     rng = np.random.default_rng(3)
     n, ncol = 600, 5000
     lens = rng.integers(0, 40, n)
-    lens[[0, 17, 300]] = 0                      # empty rows
-    lens[[5, 299, 599]] = [1500, 4000, 1025]    # rows longer than one workgroup tile
+    lens[[0, 17, 300]] = 0                          # empty rows
+    lens[[5, 299, 599, 400]] = [1500, 4000, 1025, 300]  # rows longer than a tile
+    lens[100:180] = 1                               # many short rows (more than 63 per tile)
     row, col, val = rand_csr(rng, n, ncol, lens)
-    y, ref, A, X, Y = spmv_case(pkg, ctx, oracle, row, col, val, ncol=ncol)
+    y, ref, A, X, Y = spmv_case(pkg, ctx, oracle, row, col, val, ncol=ncol, algo=algo)
     assert relnorm(y, ref) < SPMV_TOL
-    short = np.diff(row) <= 1024
-    assert np.array_equal(y[short], ref[short])
+    if algo[0] in BITWISE:
+        short = np.diff(row) <= min(algo[1], 256)
+        assert np.array_equal(y[short], ref[short])
     # y += A x
     y0 = rng.uniform(-1, 1, n)
     Y.upload(y0)
