@@ -206,6 +206,8 @@ extern "C" int hfh_ctx_create(int device, hfh_ctx **out) {
 
 extern "C" int hfh_ctx_destroy(hfh_ctx *c) {
     if (!c) return HFH_OK;
+    HFH_CHECK(c->live == 0, HFH_ERR_INVALID, "hfh_ctx_destroy: %d objects still use the context; destroy them first",
+              c->live);
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->red) (void)hipFree(c->red);
@@ -269,6 +271,7 @@ extern "C" int hfh_vec_create(hfh_ctx *ctx, int n, int dtype, hfh_vec **out) {
         hipError_t e = hipMemsetAsync(v->d, 0, v->elem() * n, ctx->stream);
         if (e != hipSuccess) { (void)hipFree(v->d); delete v; return hip_fail(e, "hipMemsetAsync"); }
     }
+    ++ctx->live;
     *out = v;
     return HFH_OK;
 }
@@ -277,6 +280,7 @@ extern "C" int hfh_vec_destroy(hfh_vec *v) {
     if (!v) return HFH_OK;
     (void)hipStreamSynchronize(v->ctx->stream);
     if (v->d) (void)hipFree(v->d);
+    --v->ctx->live;
     delete v;
     return HFH_OK;
 }

@@ -46,6 +46,7 @@ struct hfh_ctx {
     double *host_res = nullptr;       // pinned host mirror for results
     hfh::Timing timing;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int live = 0;                     // vectors, matrices and preconditioners created on the context
 };
 
 struct hfh_vec {
@@ -66,6 +67,14 @@ struct hfh_csr {
     int spmv_algo = HFH_SPMV_STREAM, spmv_p1 = 2048;   // configured kernel
     int spmv_algo_req = 0, spmv_p1_req = 0, spmv_p2_req = 0;  // requested (0 = automatic)
     int nwg = 1;           // SpMV workgroups = partial sums of a fused dot
+    std::vector<int> h_col;    // host copy of the columns, kept for the analyses
+    // SELL-64 copy for HFH_SPMV_SELL: slices of 64 rows stored column-major
+    int nslice = 0;
+    int *sl_ptr = nullptr;     // [nslice+1] element offset of each slice
+    int *sl_len = nullptr;     // [nrow] row lengths
+    int *sl_col = nullptr;     // padded columns (padding points at the row itself)
+    void *sl_val = nullptr;    // padded values (padding 0)
+    bool sell_dirty = false;   // SELL copy must be rebuilt from the CSR arrays
     int *diag = nullptr;   // diagonal position per row
     bool has_structure = false;
     std::vector<int> h_row;  // host copy of the row pointer (for analysis)
@@ -137,6 +146,7 @@ struct ScopedKernelTimer {
 // summed by a finisher kernel (A->nwg partials).
 int spmv_launch(const hfh_csr *A, const void *x, void *y, int mode);
 int spmv_configure(hfh_csr *A, int algo, int p1, int p2);
+int spmv_sell_fill(hfh_csr *A);
 int spmv_launch_ex(const hfh_csr *A, const void *x, void *y, int mode, const void *w, double *part,
                    const int *done);
 

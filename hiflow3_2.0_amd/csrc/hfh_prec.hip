@@ -328,6 +328,7 @@ static hfh_prec *new_prec(const hfh_csr *A, int kind) {
     hfh_prec *P = new hfh_prec();
     P->kind = kind;
     P->ctx = A->ctx;
+    ++P->ctx->live;
     P->A = A;
     P->n = A->nrow;
     P->dtype = A->dtype;
@@ -364,11 +365,11 @@ extern "C" int hfh_mcsgs_create(const hfh_csr *A, int ncolors, const int *color_
     P->omega = omega;
     P->color_off.assign(ncolors + 1, 0);
     for (int c = 0; c < ncolors; ++c) {
-        if (color_sizes[c] < 0) { delete P; hfh::set_error("hfh_mcsgs_create: negative colour size"); return HFH_ERR_INVALID; }
+        if (color_sizes[c] < 0) { hfh_prec_destroy(P); hfh::set_error("hfh_mcsgs_create: negative colour size"); return HFH_ERR_INVALID; }
         P->color_off[c + 1] = P->color_off[c] + color_sizes[c];
     }
     if (P->color_off[ncolors] != A->nrow) {
-        delete P;
+        hfh_prec_destroy(P);
         hfh::set_error("hfh_mcsgs_create: colour sizes sum to %d, matrix has %d rows", P->color_off.back(), A->nrow);
         return HFH_ERR_INVALID;
     }
@@ -435,6 +436,7 @@ extern "C" int hfh_prec_kind(const hfh_prec *P, int *kind) {
 extern "C" int hfh_prec_destroy(hfh_prec *P) {
     if (!P) return HFH_OK;
     if (P->ctx && P->ctx->stream) (void)hipStreamSynchronize(P->ctx->stream);
+    if (P->ctx) --P->ctx->live;
     if (P->invd) (void)hipFree(P->invd);
     if (P->dv) (void)hipFree(P->dv);
     if (P->sval) (void)hipFree(P->sval);

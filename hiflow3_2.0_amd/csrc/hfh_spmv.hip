@@ -66,14 +66,24 @@ __global__ __launch_bounds__(kBlock) void k_spmv_stream(
         const int j0 = row[r0], j1 = row[r1];
         const int cnt = j1 - j0;
         if (cnt <= TILE) {
+            // unconditional loads at clamped positions, so every load of the
+            // tile is in flight before the first gather result is needed
+            constexpr int K = TILE / kBlock;
+            int c[K];
+            T v[K], xv[K];
+            const int last = cnt > 0 ? cnt - 1 : 0;
 #pragma unroll
-            for (int k = 0; k < TILE / kBlock; ++k) {
+            for (int k = 0; k < K; ++k) {
+                const int e = min(static_cast<int>(threadIdx.x) + k * kBlock, last);
+                c[k] = __builtin_nontemporal_load(&col[j0 + e]);
+                v[k] = __builtin_nontemporal_load(&val[j0 + e]);
+            }
+#pragma unroll
+            for (int k = 0; k < K; ++k) xv[k] = x[c[k]];
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
                 const int e = threadIdx.x + k * kBlock;
-                if (e < cnt) {
-                    const int c = __builtin_nontemporal_load(&col[j0 + e]);
-                    const T v = __builtin_nontemporal_load(&val[j0 + e]);
-                    prod[e] = v * x[c];
-                }
+                if (e < cnt) prod[e] = v[k] * xv[k];
             }
             __syncthreads();
             for (int r = r0 + threadIdx.x; r < r1; r += kBlock) {
@@ -197,6 +207,100 @@ __global__ __launch_bounds__(kBlock) void k_spmv_wave(
     }
 }
 
+// SELL-64 kernel: the matrix is held a second time in slices of 64 rows (one
+// wavefront), each slice column-major and padded to its longest row.  Lane =
+// row: the val/col streams of a slice are contiguous for the whole wave, and
+// the x gathers of one column slot come from neighbouring rows, hence from
+// neighbouring addresses.  No LDS, no barriers.  Each row is summed in
+// ascending column order with separately rounded products and sums
+// (mul_rn / add_rn are compiled without contraction), so rows agree bit for
+// bit with the CPU path.
+// The operations carry no contraction flag, so the backend cannot fuse them.
+template <typename T>
+__device__ __forceinline__ T mul_rn(T a, T b) {
+#pragma clang fp contract(off)
+    return a * b;
+}
+template <typename T>
+__device__ __forceinline__ T add_rn(T a, T b) {
+#pragma clang fp contract(off)
+    return a + b;
+}
+
+template <typename T, bool ADD, bool DOT, int U>
+__global__ __launch_bounds__(kBlock) void k_spmv_sell(
+    int nrow, int nslice, const int *__restrict__ sptr, const int *__restrict__ rlen, const int *__restrict__ scol,
+    const T *__restrict__ sval, const T *__restrict__ x, T *__restrict__ y, const T *__restrict__ w, double *part,
+    const int *__restrict__ done) {
+    if (done && *done) return;
+    __shared__ double shd[kBlock / 64];
+    constexpr int WPB = kBlock / 64;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const Split sp = xcd_split(nslice, WPB);
+    double dsum = 0;
+    for (int s = sp.first + wave; s < sp.hi; s += sp.step) {
+        const int base = sptr[s];
+        const int L = (sptr[s + 1] - base) >> 6;
+        const int r = s * 64 + lane;
+        const int len = r < nrow ? rlen[r] : 0;
+        T acc = 0;
+        int k = 0;
+        for (; k + U <= L; k += U) {
+            int c[U];
+            T v[U], xv[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int idx = base + (k + u) * 64 + lane;
+                c[u] = __builtin_nontemporal_load(&scol[idx]);
+                v[u] = __builtin_nontemporal_load(&sval[idx]);
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) xv[u] = x[c[u]];
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (k + u < len) acc = add_rn(acc, mul_rn(v[u], xv[u]));
+        }
+        for (; k < L; ++k) {
+            const int idx = base + k * 64 + lane;
+            const int c = __builtin_nontemporal_load(&scol[idx]);
+            const T v = __builtin_nontemporal_load(&sval[idx]);
+            const T xv = x[c];
+            if (k < len) acc = add_rn(acc, mul_rn(v, xv));
+        }
+        if (r < nrow) {
+            if (ADD) acc = y[r] + acc;
+            y[r] = acc;
+            if (DOT) dsum += static_cast<double>(acc) * static_cast<double>(w[r]);
+        }
+    }
+    if (DOT) {
+        __syncthreads();
+        const double vv[1] = {block_sum(dsum, shd)};
+        store_partials<1>(vv, part, 0, blockIdx.x);
+    }
+}
+
+// One thread per SELL row slot (nslice*64 of them, including the padding rows
+// of the last slice).  Padding entries get value 0 and a valid column
+// (min(row, ncol-1)), so the kernel can load them without a bounds test.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_csr_to_sell(int nrow, int ncol, int nslice, const int *__restrict__ row,
+                                                        const int *__restrict__ col, const T *__restrict__ val,
+                                                        const int *__restrict__ sptr, int *__restrict__ scol,
+                                                        T *__restrict__ sval) {
+    const int r = blockIdx.x * kBlock + threadIdx.x;
+    if (r >= nslice * 64) return;
+    const int s = r >> 6, lane = r & 63;
+    const int base = sptr[s], L = (sptr[s + 1] - base) >> 6;
+    const int j0 = r < nrow ? row[r] : 0, len = r < nrow ? row[r + 1] - j0 : 0;
+    const int pad = min(r, ncol - 1);
+    for (int k = 0; k < L; ++k) {
+        const int idx = base + k * 64 + lane;
+        scol[idx] = k < len ? col[j0 + k] : pad;
+        sval[idx] = k < len ? val[j0 + k] : static_cast<T>(0);
+    }
+}
+
 // Sub-wavefront "vector" kernel: GL lanes per row, U rows per group in
 // flight; lanes stride the row and the GL partial sums are tree-reduced with
 // cross-lane shuffles.
@@ -307,6 +411,9 @@ static int round8(long long g) { return static_cast<int>(g >= kNumXcd ? (g + kNu
 // Configures the SpMV kernel of A (host row pointer in A->h_row).
 int spmv_configure(hfh_csr *A, int algo, int p1, int p2) {
     const int n = A->nrow;
+    // buffers of the previous configuration may still be read by queued work
+    HFH_HIP(hipStreamSynchronize(A->ctx->stream));
+    if (algo == HFH_SPMV_SELL && A->ncol == 0) algo = HFH_SPMV_STREAM;
     const double mean = n ? static_cast<double>(A->nnz) / n : 0.0;
     if (algo == 0) {  // automatic choice
         algo = HFH_SPMV_STREAM;
@@ -329,6 +436,42 @@ int spmv_configure(hfh_csr *A, int algo, int p1, int p2) {
         A->spmv_algo = HFH_SPMV_STREAM;
         A->spmv_p1 = tile;
         A->nwg = round8(std::max<long long>(g, 1));
+    } else if (algo == HFH_SPMV_SELL) {
+        const int nsl = (n + 63) / 64;
+        std::vector<int> ptr(nsl + 1, 0), len(std::max(n, 1), 0);
+        long long off = 0;
+        for (int sl = 0; sl < nsl; ++sl) {
+            int L = 0;
+            for (int r = sl * 64; r < std::min(n, sl * 64 + 64); ++r) {
+                len[r] = A->h_row[r + 1] - A->h_row[r];
+                L = std::max(L, len[r]);
+            }
+            ptr[sl] = static_cast<int>(off);
+            off += 64LL * L;
+            HFH_CHECK(off < (1LL << 31), HFH_ERR_UNSUPPORTED, "spmv: SELL copy exceeds 2^31 entries");
+        }
+        ptr[nsl] = static_cast<int>(off);
+        if (A->sl_ptr) (void)hipFree(A->sl_ptr);
+        if (A->sl_len) (void)hipFree(A->sl_len);
+        if (A->sl_col) (void)hipFree(A->sl_col);
+        if (A->sl_val) (void)hipFree(A->sl_val);
+        A->sl_ptr = A->sl_len = A->sl_col = nullptr;
+        A->sl_val = nullptr;
+        int rc;
+        if ((rc = dev_alloc(reinterpret_cast<void **>(&A->sl_ptr), sizeof(int) * ptr.size())) ||
+            (rc = dev_alloc(reinterpret_cast<void **>(&A->sl_len), sizeof(int) * len.size())) ||
+            (rc = dev_alloc(reinterpret_cast<void **>(&A->sl_col), sizeof(int) * static_cast<size_t>(std::max(off, 1LL)))) ||
+            (rc = dev_alloc(&A->sl_val, A->elem() * static_cast<size_t>(std::max(off, 1LL)))))
+            return rc;
+        HFH_HIP(hipMemcpy(A->sl_ptr, ptr.data(), sizeof(int) * ptr.size(), hipMemcpyHostToDevice));
+        HFH_HIP(hipMemcpy(A->sl_len, len.data(), sizeof(int) * len.size(), hipMemcpyHostToDevice));
+        A->nslice = nsl;
+        A->spmv_algo = HFH_SPMV_SELL;
+        A->spmv_p1 = p1 == 2 || p1 == 8 ? p1 : 4;
+        const long long need = (nsl + kBlock / 64 - 1) / (kBlock / 64);
+        const long long g = p2 > 0 ? std::min<long long>(p2, need) : need;
+        A->nwg = round8(std::max<long long>(g, 1));
+        A->sell_dirty = true;  // filled from the device CSR by spmv_sell_fill
     } else if (algo == HFH_SPMV_WAVE) {
         const int tile = p1 ? p1 : 512;
         HFH_CHECK(tile == 256 || tile == 512 || tile == 1024, HFH_ERR_INVALID,
@@ -372,7 +515,13 @@ static void launch_t(const hfh_csr *A, const void *x, void *y, const void *w, do
     const dim3 g(A->nwg), bl(kBlock);
     const T *xv = (const T *)x, *wv = (const T *)w, *val = (const T *)A->val;
     T *yv = (T *)y;
-    if (A->spmv_algo == HFH_SPMV_WAVE) {
+    if (A->spmv_algo == HFH_SPMV_SELL) {
+        switch (A->spmv_p1) {
+        case 2: hipLaunchKernelGGL((k_spmv_sell<T, ADD, DOT, 2>), g, bl, 0, s, A->nrow, A->nslice, A->sl_ptr, A->sl_len, A->sl_col, (const T *)A->sl_val, xv, yv, wv, part, done); break;
+        case 8: hipLaunchKernelGGL((k_spmv_sell<T, ADD, DOT, 8>), g, bl, 0, s, A->nrow, A->nslice, A->sl_ptr, A->sl_len, A->sl_col, (const T *)A->sl_val, xv, yv, wv, part, done); break;
+        default: hipLaunchKernelGGL((k_spmv_sell<T, ADD, DOT, 4>), g, bl, 0, s, A->nrow, A->nslice, A->sl_ptr, A->sl_len, A->sl_col, (const T *)A->sl_val, xv, yv, wv, part, done); break;
+        }
+    } else if (A->spmv_algo == HFH_SPMV_WAVE) {
         switch (A->spmv_p1) {
         case 256: hipLaunchKernelGGL((k_spmv_wave<T, ADD, DOT, 256>), g, bl, 0, s, A->row, A->col, val, xv, yv, A->rblk, A->nrblk, wv, part, done); break;
         case 512: hipLaunchKernelGGL((k_spmv_wave<T, ADD, DOT, 512>), g, bl, 0, s, A->row, A->col, val, xv, yv, A->rblk, A->nrblk, wv, part, done); break;
@@ -394,6 +543,23 @@ static void launch_t(const hfh_csr *A, const void *x, void *y, const void *w, do
         default: hipLaunchKernelGGL((k_spmv_vector<T, ADD, DOT, 64, 2>), g, bl, 0, s, A->nrow, A->row, A->col, val, xv, yv, wv, part, done); break;
         }
     }
+}
+
+// Builds the SELL-64 values and columns from the device CSR (after an
+// upload, a cast or a change of algorithm).
+int spmv_sell_fill(hfh_csr *A) {
+    if (A->spmv_algo != HFH_SPMV_SELL || !A->sell_dirty || A->nrow == 0) return HFH_OK;
+    const dim3 g((A->nslice * 64 + kBlock - 1) / kBlock);
+    if (A->dtype == HFH_F64)
+        hipLaunchKernelGGL(k_csr_to_sell<double>, g, dim3(kBlock), 0, A->ctx->stream, A->nrow, A->ncol, A->nslice,
+                           A->row, A->col, (const double *)A->val, A->sl_ptr, A->sl_col, (double *)A->sl_val);
+    else
+        hipLaunchKernelGGL(k_csr_to_sell<float>, g, dim3(kBlock), 0, A->ctx->stream, A->nrow, A->ncol, A->nslice,
+                           A->row, A->col, (const float *)A->val, A->sl_ptr, A->sl_col, (float *)A->sl_val);
+    int rc = launch_check("k_csr_to_sell");
+    if (rc) return rc;
+    A->sell_dirty = false;
+    return HFH_OK;
 }
 
 int spmv_launch_ex(const hfh_csr *A, const void *x, void *y, int mode, const void *w, double *part,
@@ -429,6 +595,7 @@ extern "C" int hfh_csr_create(hfh_ctx *ctx, int nrow, int ncol, int nnz, int dty
     HFH_CHECK(dtype == HFH_F64 || dtype == HFH_F32, HFH_ERR_INVALID, "hfh_csr_create: bad dtype %d", dtype);
     hfh_csr *A = new hfh_csr();
     A->ctx = ctx;
+    ++ctx->live;
     A->nrow = nrow;
     A->ncol = ncol;
     A->nnz = nnz;
@@ -448,11 +615,16 @@ extern "C" int hfh_csr_create(hfh_ctx *ctx, int nrow, int ncol, int nnz, int dty
 extern "C" int hfh_csr_destroy(hfh_csr *A) {
     if (!A) return HFH_OK;
     if (A->ctx && A->ctx->stream) (void)hipStreamSynchronize(A->ctx->stream);
+    if (A->ctx) --A->ctx->live;
     if (A->row) (void)hipFree(A->row);
     if (A->col) (void)hipFree(A->col);
     if (A->val) (void)hipFree(A->val);
     if (A->rblk) (void)hipFree(A->rblk);
     if (A->diag) (void)hipFree(A->diag);
+    if (A->sl_ptr) (void)hipFree(A->sl_ptr);
+    if (A->sl_len) (void)hipFree(A->sl_len);
+    if (A->sl_col) (void)hipFree(A->sl_col);
+    if (A->sl_val) (void)hipFree(A->sl_val);
     delete A;
     return HFH_OK;
 }
@@ -482,6 +654,7 @@ extern "C" int hfh_csr_upload(hfh_csr *A, const int *row, const int *col, const 
     }
     hipStream_t s = A->ctx->stream;
     A->h_row.assign(row, row + n + 1);
+    if (A->nnz) A->h_col.assign(col, col + A->nnz);
     int rc = spmv_configure(A, A->spmv_algo_req, A->spmv_p1_req, A->spmv_p2_req);
     if (rc) return rc;
     HFH_HIP(hipMemcpyAsync(A->row, row, sizeof(int) * (n + 1), hipMemcpyHostToDevice, s));
@@ -493,8 +666,10 @@ extern "C" int hfh_csr_upload(hfh_csr *A, const int *row, const int *col, const 
         hipLaunchKernelGGL(k_find_diag<int>, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, n, A->row, A->col, A->diag);
         if ((rc = launch_check("k_find_diag"))) return rc;
     }
-    HFH_HIP(hipStreamSynchronize(s));
     A->has_structure = true;
+    A->sell_dirty = true;
+    if ((rc = spmv_sell_fill(A))) return rc;
+    HFH_HIP(hipStreamSynchronize(s));
     return HFH_OK;
 }
 
@@ -503,6 +678,9 @@ extern "C" int hfh_csr_upload_values(hfh_csr *A, const void *val) {
     HFH_CHECK(A->has_structure, HFH_ERR_INVALID, "hfh_csr_upload_values: structure not set");
     if (A->nnz == 0) return HFH_OK;
     HFH_HIP(hipMemcpyAsync(A->val, val, A->elem() * static_cast<size_t>(A->nnz), hipMemcpyHostToDevice, A->ctx->stream));
+    A->sell_dirty = true;
+    int rc = spmv_sell_fill(A);
+    if (rc) return rc;
     HFH_HIP(hipStreamSynchronize(A->ctx->stream));
     return HFH_OK;
 }
@@ -526,6 +704,7 @@ extern "C" int hfh_csr_cast(hfh_csr *dst, const hfh_csr *src) {
     if (src->nnz) HFH_HIP(hipMemcpyAsync(dst->col, src->col, sizeof(int) * static_cast<size_t>(src->nnz), hipMemcpyDeviceToDevice, s));
     if (src->nrow) HFH_HIP(hipMemcpyAsync(dst->diag, src->diag, sizeof(int) * static_cast<size_t>(src->nrow), hipMemcpyDeviceToDevice, s));
     dst->h_row = src->h_row;
+    dst->h_col = src->h_col;
     int rc = spmv_configure(dst, dst->spmv_algo_req ? dst->spmv_algo_req : src->spmv_algo,
                             dst->spmv_algo_req ? dst->spmv_p1_req : src->spmv_p1,
                             dst->spmv_algo_req ? dst->spmv_p2_req : 0);
@@ -541,8 +720,10 @@ extern "C" int hfh_csr_cast(hfh_csr *dst, const hfh_csr *src) {
             hipLaunchKernelGGL((k_cast_vals<double, float>), dim3(nb), dim3(kBlock), 0, s, nnz, (double *)dst->val, (const float *)src->val);
         if ((rc = launch_check("k_cast_vals"))) return rc;
     }
-    HFH_HIP(hipStreamSynchronize(s));
     dst->has_structure = true;
+    dst->sell_dirty = true;
+    if ((rc = spmv_sell_fill(dst))) return rc;
+    HFH_HIP(hipStreamSynchronize(s));
     return HFH_OK;
 }
 
@@ -582,11 +763,15 @@ extern "C" int hfh_csr_invdiag(const hfh_csr *A, hfh_vec *invd) {
 
 extern "C" int hfh_csr_set_spmv(hfh_csr *A, int algo, int p1, int p2) {
     HFH_CHECK(A, HFH_ERR_INVALID, "hfh_csr_set_spmv: null matrix");
-    HFH_CHECK(algo >= 0 && algo <= HFH_SPMV_WAVE && p2 >= 0, HFH_ERR_INVALID, "hfh_csr_set_spmv: bad argument");
+    HFH_CHECK(algo >= 0 && algo <= HFH_SPMV_SELL && p2 >= 0, HFH_ERR_INVALID, "hfh_csr_set_spmv: bad argument");
     A->spmv_algo_req = algo;
     A->spmv_p1_req = p1;
     A->spmv_p2_req = p2;
     if (!A->has_structure) return HFH_OK;
     HFH_HIP(hipStreamSynchronize(A->ctx->stream));
-    return spmv_configure(A, algo, p1, p2);
+    int rc = spmv_configure(A, algo, p1, p2);
+    if (rc) return rc;
+    if ((rc = spmv_sell_fill(A))) return rc;
+    HFH_HIP(hipStreamSynchronize(A->ctx->stream));
+    return HFH_OK;
 }

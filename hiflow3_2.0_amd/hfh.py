@@ -21,7 +21,7 @@ OK, ERR_INVALID, ERR_NOMEM, ERR_HIP, ERR_ZERO_PIVOT, ERR_NOT_CONVERGED, ERR_BUDG
     0, -1, -2, -3, -4, -5, -6, -7
 
 Q1_2D, Q1_3D, FV_DIFFUSION, UPWIND_CONVDIFF = 1, 2, 3, 4
-SPMV_AUTO, SPMV_STREAM, SPMV_VECTOR, SPMV_WAVE = 0, 1, 2, 3
+SPMV_AUTO, SPMV_STREAM, SPMV_VECTOR, SPMV_WAVE, SPMV_SELL = 0, 1, 2, 3, 4
 
 
 class HfhError(RuntimeError):
@@ -195,6 +195,8 @@ class Context:
         self.h = h
 
     def close(self):
+        """Destroys the context once no vector, matrix or preconditioner uses it
+        (the library refuses earlier; the context then lives to process exit)."""
         if self.h:
             hip().hfh_ctx_destroy(self.h)
             self.h = None

@@ -124,10 +124,14 @@ int hfh_spmv_add(const hfh_csr *A, const hfh_vec *x, hfh_vec *y);
  * (512/1024/2048/4096, 0 = 2048), bit-identical to the CPU row sums;
  * HFH_SPMV_VECTOR: param1 lanes per row (4..64, 0 = from the mean row
  * length); HFH_SPMV_WAVE: per-wavefront tiles of param1 nonzeros (256/512/
- * 1024) and at most 64 rows, bit-identical like STREAM.
+ * 1024) and at most 63 rows, bit-identical like STREAM; HFH_SPMV_SELL: a
+ * SELL-64 copy (slices of 64 rows, column-major, built on the device from
+ * the CSR arrays), lane = row, param1 column slots in flight (2/4/8),
+ * bit-identical like STREAM.
  * param2: workgroups (0 = automatic).  algo 0 = automatic.
  * Takes effect at once on an uploaded matrix and on later uploads. */
-enum hfh_spmv_algo { HFH_SPMV_AUTO = 0, HFH_SPMV_STREAM = 1, HFH_SPMV_VECTOR = 2, HFH_SPMV_WAVE = 3 };
+enum hfh_spmv_algo { HFH_SPMV_AUTO = 0, HFH_SPMV_STREAM = 1, HFH_SPMV_VECTOR = 2, HFH_SPMV_WAVE = 3,
+                     HFH_SPMV_SELL = 4 };
 int hfh_csr_set_spmv(hfh_csr *A, int algo, int param1, int param2);
 /* extract_invdiagelements over all rows (lmatrix_csr_cpu.cc:2414-2432) */
 int hfh_csr_invdiag(const hfh_csr *A, hfh_vec *invd);

@@ -24,8 +24,8 @@ def rand_csr(rng, n, ncol, lens):
     return row, col, val
 
 
-ALGOS = [(1, 2048), (1, 512), (3, 512), (3, 1024), (3, 256), (2, 0)]  # (algorithm, param)
-BITWISE = {1, 3}  # row-block and wave-tile kernels sum rows in the reference order
+ALGOS = [(1, 2048), (1, 1024), (1, 512), (1, 4096), (3, 512), (3, 1024), (3, 256), (2, 0), (4, 4), (4, 2), (4, 8)]
+BITWISE = {1, 3, 4}  # row-block, wave-tile and SELL kernels sum rows in the reference order
 
 
 def spmv_case(pkg, ctx, oracle, row, col, val, ncol=None, dtype=np.float64, seed=0, algo=(1, 2048)):
@@ -87,7 +87,9 @@ def test_spmv_ragged_and_long_rows(pkg, ctx, oracle, algo):
     row, col, val = rand_csr(rng, n, ncol, lens)
     y, ref, A, X, Y = spmv_case(pkg, ctx, oracle, row, col, val, ncol=ncol, algo=algo)
     assert relnorm(y, ref) < SPMV_TOL
-    if algo[0] in BITWISE:
+    if algo[0] == 4:
+        assert np.array_equal(y, ref)   # SELL sums every row sequentially, long rows included
+    elif algo[0] in BITWISE:
         short = np.diff(row) <= min(algo[1], 256)
         assert np.array_equal(y[short], ref[short])
     # y += A x
