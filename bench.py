@@ -156,7 +156,7 @@ def bench_c2(P, args):
                    "n": n, "nnz": nnz, "index": "int32"},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": "k_spmv_stream<double>", "bytes_per_launch": b_spmv,
+                     "kernel": "k_spmv_sell<double>", "bytes_per_launch": b_spmv,
                      "avg_launch_us": round(avg_spmv_ms * 1e3, 2), "launches": launches},
         "cpu_baseline": cpu,
         "cg": {"iters": iters, "ms_per_iter": round(ms_iter, 4), "solve_ms": round(ms_step, 3),

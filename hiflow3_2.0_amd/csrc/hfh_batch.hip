@@ -1,0 +1,933 @@
+// hfh_batch.hip -- serving engine: many independent SPD systems solved
+// together by FP32-inner / FP64-outer CG with continuous admission.
+//
+// Each admitted system is solved by defect correction in the form of
+// MPIR<D,S>::Solve (linear_solver/mpir.cc:69-327): FP64 true residual
+// r = b - A x, FP32 inner Jacobi-PCG on A c = r (CG::SolvePrecond,
+// linear_solver/cg.cc:84-248, preconditioner lpreconditioner.cc:104-166),
+// FP64 update x += c, until the true residual is at most
+// max(atol, rtol * initial residual).  The engine has no reference
+// counterpart; it runs the same algorithm for many systems at once.
+//
+// Device layout.  One arena of `hbm_budget` bytes holds every admitted
+// system in a contiguous region (first-fit, 256-byte aligned sections):
+//   sl_ptr[nslice+1] sl_len[n] sl_col[E] v64[E] v32[E] invd[n] b[n] x[n]   (uploaded / assembled)
+//   r64[n] r[n] c[n] z[n] p[n] q[n] part[2*ntiles]                          (work space)
+// with the matrix in SELL-64 form (slices of 64 rows, column-major, E
+// padded entries; FP64 values for the true residual, FP32 values for the
+// inner solve, one shared column array).  A slot table holds each system's
+// descriptor (pointers, sizes) and its solver state.  A tile table lists
+// every active system's tiles (16 slices = 1024 rows each).
+//
+// One CG iteration of all systems is five launches:
+//   K1 tiles : inner  q = A32 p, partial p.q      | outer  r64 = b - A64 x, partial r64.r64
+//   K2 slots : inner  alpha = rz / p.q            | outer  stop test, else start a correction
+//   K3 tiles : inner  c += alpha p, r -= alpha q  | start  r = fl(r64), c = 0
+//              z = invd r, partials r.r, r.z
+//   K4 slots : inner  stop test or beta           | start  rr0, rz
+//   K5 tiles : p = z + beta p | p = z | x += c (correction finished)
+// The tile kernels loop over the tile table with a fixed grid, and the slot
+// kernels over the slot table, so `chunk` iterations are captured once in a
+// hipGraph and replayed while systems come and go: admission and retirement
+// only rewrite the tables between replays.  A system's arithmetic depends
+// only on its own data and its own fixed tiling (partials summed per system
+// in tile order), so its result is bit-identical alone and in any mix.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <vector>
+
+#include "hfh_internal.h"
+
+namespace hfh {
+namespace {
+
+constexpr int kTileSlices = 16;     // 1024 rows per tile
+constexpr int kTileRows = kTileSlices * 64;
+constexpr int kTileGrid = 2048;     // workgroups of the tile kernels
+constexpr int kSellU = 8;           // column slots per SELL step
+constexpr size_t kAlign = 256;
+
+// solver phases (SysState::phase)
+enum : int {
+    P_IDLE = 0,     // free slot
+    P_FIRST = 1,    // next K1: initial true residual
+    P_CORRECT = 2,  // next K5: x += c; next K1: true residual
+    P_SETUP = 3,    // K3/K4: start an inner solve from r64
+    P_START = 4,    // next K5: p = z; next K1: inner SpMV
+    P_GO = 5,       // next K5: p = z + beta p; next K1: inner SpMV
+    P_UPD = 6,      // K3/K4: inner update with alpha
+    P_DONE = 7      // finished (state holds the outcome)
+};
+
+struct SysDesc {
+    int n, nslice, ntiles, pad;
+    const int *sl_ptr;
+    const int *sl_len;
+    const int *sl_col;
+    const double *v64;
+    const float *v32;
+    const float *invd;
+    const double *b;
+    double *x;
+    double *r64;
+    float *r, *c, *z, *p, *q;
+    double *part;
+};
+
+struct SysState {
+    int phase, status, outer, inner, inner_total, pad;
+    double res0, res, tol, rz, rr0, alpha, beta;
+};
+
+struct Tile {
+    int sys, s0, ns, lt;   // slot, first slice, slices, tile index within the system
+};
+
+struct KParams {
+    double rtol, atol, inner_rtol;
+    int outer_max, inner_max;
+};
+
+// Products and sums of the SELL row sums are rounded separately (no
+// contraction), in ascending column order, as the CPU row loop does.
+template <typename T>
+__device__ __forceinline__ T bmul(T a, T b) {
+#pragma clang fp contract(off)
+    return a * b;
+}
+template <typename T>
+__device__ __forceinline__ T badd(T a, T b) {
+#pragma clang fp contract(off)
+    return a + b;
+}
+
+// Row sum of SELL row `lane` of slice `s` against vector v (wave-uniform slice).
+template <typename T, typename V>
+__device__ __forceinline__ T sell_row(const SysDesc &D, const T *__restrict__ val, const V *__restrict__ v, int s,
+                                      int lane, int len) {
+    const int base = D.sl_ptr[s];
+    const int L = (D.sl_ptr[s + 1] - base) >> 6;
+    T acc = 0;
+    for (int k = 0; k < L; k += kSellU) {
+        int c[kSellU];
+        T a[kSellU];
+        V xv[kSellU];
+#pragma unroll
+        for (int u = 0; u < kSellU; ++u) {
+            const int idx = base + min(k + u, L - 1) * 64 + lane;
+            c[u] = __builtin_nontemporal_load(&D.sl_col[idx]);
+            a[u] = __builtin_nontemporal_load(&val[idx]);
+        }
+#pragma unroll
+        for (int u = 0; u < kSellU; ++u) xv[u] = v[c[u]];
+#pragma unroll
+        for (int u = 0; u < kSellU; ++u)
+            if (k + u < len) acc = badd(acc, bmul(a[u], static_cast<T>(xv[u])));
+    }
+    return acc;
+}
+
+__device__ __forceinline__ bool is_inner_spmv(int ph) { return ph == P_START || ph == P_GO; }
+__device__ __forceinline__ bool is_outer_spmv(int ph) { return ph == P_FIRST || ph == P_CORRECT; }
+
+// K1: inner SpMV q = A32 p with p.q partials, or the FP64 true residual.
+__global__ __launch_bounds__(kBlock) void kb_spmv(const SysDesc *__restrict__ sd, const SysState *__restrict__ st,
+                                                  const Tile *__restrict__ tiles, const int *__restrict__ ntiles) {
+    __shared__ double sh[kBlock / 64];
+    const int nt = *ntiles;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int t = xcd_remap(blockIdx.x, gridDim.x); t < nt; t += gridDim.x) {
+        const Tile T = tiles[t];
+        const int ph = st[T.sys].phase;
+        const bool inner = is_inner_spmv(ph);
+        if (!inner && !is_outer_spmv(ph)) continue;   // uniform over the workgroup
+        const SysDesc D = sd[T.sys];
+        double dsum = 0;
+        for (int s = T.s0 + wave; s < T.s0 + T.ns; s += kBlock / 64) {
+            const int r = s * 64 + lane;
+            const int len = r < D.n ? D.sl_len[r] : 0;
+            if (inner) {
+                const float acc = sell_row<float, float>(D, D.v32, D.p, s, lane, len);
+                if (r < D.n) {
+                    D.q[r] = acc;
+                    dsum += static_cast<double>(D.p[r]) * static_cast<double>(acc);
+                }
+            } else {
+                const double acc = sell_row<double, double>(D, D.v64, D.x, s, lane, len);
+                if (r < D.n) {
+                    const double rr = D.b[r] - acc;
+                    D.r64[r] = rr;
+                    dsum += rr * rr;
+                }
+            }
+        }
+        const double tot = block_sum(dsum, sh);
+        if (threadIdx.x == 0) D.part[2 * T.lt] = tot;
+        __syncthreads();   // sh is reused by the next tile
+    }
+}
+
+// Deterministic sum of the system's partials q (0 or 1) over its tiles.
+__device__ __forceinline__ double sum_parts(const SysDesc &D, int q, double *sh) {
+    double v = 0;
+    for (int i = threadIdx.x; i < D.ntiles; i += kBlock) v += D.part[2 * i + q];
+    const double tot = block_sum(v, sh);
+    __syncthreads();
+    return tot;
+}
+
+// K2: per system, after K1.
+__global__ __launch_bounds__(kBlock) void kb_fin1(const SysDesc *__restrict__ sd, SysState *__restrict__ st,
+                                                  int nslots, KParams prm) {
+    __shared__ double sh[kBlock / 64];
+    const int s = blockIdx.x;
+    if (s >= nslots) return;
+    const int ph = st[s].phase;
+    const bool inner = is_inner_spmv(ph);
+    if (!inner && !is_outer_spmv(ph)) return;
+    const SysDesc D = sd[s];
+    const double tot = sum_parts(D, 0, sh);
+    if (threadIdx.x != 0) return;
+    SysState &S = st[s];
+    if (inner) {
+        if (tot > 0 && isfinite(tot)) {
+            S.alpha = S.rz / tot;
+            S.phase = P_UPD;
+        } else {
+            S.phase = P_CORRECT;   // breakdown: end this inner solve with the correction so far
+        }
+        return;
+    }
+    const double res = std::sqrt(tot);
+    if (ph == P_FIRST) {
+        S.res0 = res;
+        S.tol = fmax(prm.atol, prm.rtol * res);
+    }
+    S.res = res;
+    if (res <= S.tol) {
+        S.status = res <= prm.atol ? HFH_SUCCESS_ABS : HFH_SUCCESS_REL;
+        S.phase = P_DONE;
+    } else if (!isfinite(res)) {
+        S.status = HFH_FAIL_DIV;
+        S.phase = P_DONE;
+    } else if (S.outer >= prm.outer_max) {
+        S.status = HFH_FAIL_MAXITS;
+        S.phase = P_DONE;
+    } else {
+        ++S.outer;
+        S.phase = P_SETUP;
+    }
+}
+
+// K3: inner update, or the start of an inner solve; partials r.r and r.z.
+__global__ __launch_bounds__(kBlock) void kb_update(const SysDesc *__restrict__ sd, const SysState *__restrict__ st,
+                                                    const Tile *__restrict__ tiles, const int *__restrict__ ntiles) {
+    __shared__ double sh[kBlock / 64];
+    const int nt = *ntiles;
+    for (int t = xcd_remap(blockIdx.x, gridDim.x); t < nt; t += gridDim.x) {
+        const Tile T = tiles[t];
+        const int ph = st[T.sys].phase;
+        if (ph != P_UPD && ph != P_SETUP) continue;
+        const SysDesc D = sd[T.sys];
+        const float a = static_cast<float>(st[T.sys].alpha);
+        const int r0 = T.s0 * 64, r1 = min(D.n, (T.s0 + T.ns) * 64);
+        double rr = 0, rz = 0;
+        for (int i = r0 + threadIdx.x; i < r1; i += kBlock) {
+            float ri;
+            if (ph == P_UPD) {
+                D.c[i] = D.c[i] + a * D.p[i];
+                ri = D.r[i] - a * D.q[i];
+            } else {
+                ri = static_cast<float>(D.r64[i]);
+                D.c[i] = 0.0f;
+            }
+            D.r[i] = ri;
+            const float zi = D.invd[i] * ri;
+            D.z[i] = zi;
+            rr += static_cast<double>(ri) * static_cast<double>(ri);
+            rz += static_cast<double>(ri) * static_cast<double>(zi);
+        }
+        const double trr = block_sum(rr, sh);
+        __syncthreads();
+        const double trz = block_sum(rz, sh);
+        if (threadIdx.x == 0) {
+            D.part[2 * T.lt] = trr;
+            D.part[2 * T.lt + 1] = trz;
+        }
+        __syncthreads();
+    }
+}
+
+// K4: per system, after K3.
+__global__ __launch_bounds__(kBlock) void kb_fin2(const SysDesc *__restrict__ sd, SysState *__restrict__ st,
+                                                  int nslots, KParams prm) {
+    __shared__ double sh[kBlock / 64];
+    const int s = blockIdx.x;
+    if (s >= nslots) return;
+    const int ph = st[s].phase;
+    if (ph != P_UPD && ph != P_SETUP) return;
+    const SysDesc D = sd[s];
+    const double rr = sum_parts(D, 0, sh);
+    const double rz = sum_parts(D, 1, sh);
+    if (threadIdx.x != 0) return;
+    SysState &S = st[s];
+    const double rn = std::sqrt(rr);
+    if (ph == P_SETUP) {
+        S.rr0 = rn;
+        S.rz = rz;
+        S.inner = 0;
+        S.phase = rn > 0 && isfinite(rn) ? P_START : P_CORRECT;
+        return;
+    }
+    ++S.inner;
+    ++S.inner_total;
+    if (rn <= prm.inner_rtol * S.rr0 || S.inner >= prm.inner_max || !(S.rz != 0) || !isfinite(rz)) {
+        S.phase = P_CORRECT;
+    } else {
+        S.beta = rz / S.rz;
+        S.rz = rz;
+        S.phase = P_GO;
+    }
+}
+
+// K5: search direction, or the FP64 correction x += c.
+__global__ __launch_bounds__(kBlock) void kb_dir(const SysDesc *__restrict__ sd, const SysState *__restrict__ st,
+                                                 const Tile *__restrict__ tiles, const int *__restrict__ ntiles) {
+    const int nt = *ntiles;
+    for (int t = xcd_remap(blockIdx.x, gridDim.x); t < nt; t += gridDim.x) {
+        const Tile T = tiles[t];
+        const int ph = st[T.sys].phase;
+        if (ph != P_GO && ph != P_START && ph != P_CORRECT) continue;
+        const SysDesc D = sd[T.sys];
+        const int r0 = T.s0 * 64, r1 = min(D.n, (T.s0 + T.ns) * 64);
+        if (ph == P_GO) {
+            const float beta = static_cast<float>(st[T.sys].beta);
+            for (int i = r0 + threadIdx.x; i < r1; i += kBlock) D.p[i] = D.z[i] + beta * D.p[i];
+        } else if (ph == P_START) {
+            for (int i = r0 + threadIdx.x; i < r1; i += kBlock) D.p[i] = D.z[i];
+        } else {
+            for (int i = r0 + threadIdx.x; i < r1; i += kBlock) D.x[i] += static_cast<double>(D.c[i]);
+        }
+    }
+}
+
+// Device assembly of the cell-centred FV diffusion matrix (5-pt when nz == 1,
+// else 7-pt) in SELL-64 form of uniform width W from the cell coefficients:
+// the same entries, in the same order and with the same arithmetic as the
+// host generator (harmonic-mean faces, 2 k on Dirichlet faces).  One thread
+// per row slot; rows beyond n (last slice) are padding.
+__global__ __launch_bounds__(kBlock) void kb_fv_assemble(int nx, int ny, int nz, int n, int nslice, int W,
+                                                         const double *__restrict__ kc, int *__restrict__ sl_ptr,
+                                                         int *__restrict__ sl_len, int *__restrict__ sl_col,
+                                                         double *__restrict__ v64, float *__restrict__ v32,
+                                                         float *__restrict__ invd) {
+#pragma clang fp contract(off)
+    const int r = blockIdx.x * kBlock + threadIdx.x;
+    if (r <= nslice && r >= 0) {
+        // slice pointers (nslice+1 of them) from the first threads
+        sl_ptr[r] = r * 64 * W;
+    }
+    if (r >= nslice * 64) return;
+    const int s = r >> 6, lane = r & 63;
+    const int base = s * 64 * W;
+    if (r >= n) {
+        for (int e = 0; e < W; ++e) {
+            sl_col[base + e * 64 + lane] = n - 1;
+            v64[base + e * 64 + lane] = 0.0;
+            v32[base + e * 64 + lane] = 0.0f;
+        }
+        return;
+    }
+    const int i = r % nx, j = (r / nx) % ny, k = r / (nx * ny);
+    const bool in3 = nz > 1;
+    const long long nxy = static_cast<long long>(nx) * ny;
+    const long long nb[6] = {r - nxy, r - nx, r - 1LL, r + 1LL, r + nx, r + nxy};
+    const bool has[6] = {in3 && k > 0, j > 0, i > 0, i < nx - 1, j < ny - 1, in3 && k < nz - 1};
+    const bool axis[6] = {in3, true, true, true, true, in3};
+    const double ks = kc[r];
+    double off[6] = {0, 0, 0, 0, 0, 0};
+    double diag = 0;
+    for (int q = 0; q < 6; ++q) {
+        if (!axis[q]) continue;
+        if (has[q]) {
+            const double kn = kc[nb[q]];
+            const double t = 2.0 * ks * kn / (ks + kn);
+            off[q] = -t;
+            diag += t;
+        } else {
+            diag += 2.0 * ks;
+        }
+    }
+    int e = 0;
+    auto put = [&](int c, double v) {
+        const int idx = base + e * 64 + lane;
+        sl_col[idx] = c;
+        v64[idx] = v;
+        v32[idx] = static_cast<float>(v);
+        ++e;
+    };
+    for (int q = 0; q < 3; ++q)
+        if (has[q]) put(static_cast<int>(nb[q]), off[q]);
+    put(r, diag);
+    for (int q = 3; q < 6; ++q)
+        if (has[q]) put(static_cast<int>(nb[q]), off[q]);
+    sl_len[r] = e;
+    for (int f = e; f < W; ++f) {
+        const int idx = base + f * 64 + lane;
+        sl_col[idx] = r;
+        v64[idx] = 0.0;
+        v32[idx] = 0.0f;
+    }
+    invd[r] = 1.0f / static_cast<float>(diag);
+}
+
+// host-side first-fit allocator over [0, cap) with coalescing
+struct Arena {
+    size_t cap = 0, used = 0;
+    std::map<size_t, size_t> freel;   // offset -> size
+    void init(size_t c) {
+        cap = c;
+        used = 0;
+        freel.clear();
+        if (c) freel[0] = c;
+    }
+    bool alloc(size_t bytes, size_t *off) {
+        for (auto it = freel.begin(); it != freel.end(); ++it) {
+            if (it->second < bytes) continue;
+            *off = it->first;
+            const size_t rest = it->second - bytes;
+            const size_t at = it->first + bytes;
+            freel.erase(it);
+            if (rest) freel[at] = rest;
+            used += bytes;
+            return true;
+        }
+        return false;
+    }
+    void release(size_t off, size_t bytes) {
+        used -= bytes;
+        auto it = freel.emplace(off, bytes).first;
+        auto nx = std::next(it);
+        if (nx != freel.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            freel.erase(nx);
+        }
+        if (it != freel.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                freel.erase(it);
+            }
+        }
+    }
+};
+
+size_t up(size_t b) { return (b + kAlign - 1) / kAlign * kAlign; }
+
+// byte offsets of a system's sections inside its region
+struct Layout {
+    size_t ptr, len, col, v64, v32, invd, b, x, upload, r64, r, c, z, p, q, part, total;
+    Layout(int n, int nslice, long long E, int ntiles) {
+        size_t o = 0;
+        ptr = o; o += up(sizeof(int) * (nslice + 1));
+        len = o; o += up(sizeof(int) * static_cast<size_t>(n));
+        col = o; o += up(sizeof(int) * static_cast<size_t>(E));
+        v64 = o; o += up(sizeof(double) * static_cast<size_t>(E));
+        v32 = o; o += up(sizeof(float) * static_cast<size_t>(E));
+        invd = o; o += up(sizeof(float) * static_cast<size_t>(n));
+        b = o; o += up(sizeof(double) * static_cast<size_t>(n));
+        x = o; o += up(sizeof(double) * static_cast<size_t>(n));
+        upload = o;
+        r64 = o; o += up(sizeof(double) * static_cast<size_t>(n));
+        r = o; o += up(sizeof(float) * static_cast<size_t>(n));
+        c = o; o += up(sizeof(float) * static_cast<size_t>(n));
+        z = o; o += up(sizeof(float) * static_cast<size_t>(n));
+        p = o; o += up(sizeof(float) * static_cast<size_t>(n));
+        q = o; o += up(sizeof(float) * static_cast<size_t>(n));
+        part = o; o += up(sizeof(double) * 2 * static_cast<size_t>(ntiles));
+        total = o;
+    }
+};
+
+int tiles_of(int nslice) { return (nslice + kTileSlices - 1) / kTileSlices; }
+
+// SELL-64 shape of a host CSR matrix: slice widths and padded entry count
+long long sell_entries(int n, const int *row, std::vector<int> *ptr) {
+    const int nsl = (n + 63) / 64;
+    if (ptr) ptr->assign(nsl + 1, 0);
+    long long off = 0;
+    for (int s = 0; s < nsl; ++s) {
+        int L = 0;
+        for (int r = s * 64; r < std::min(n, s * 64 + 64); ++r) L = std::max(L, row[r + 1] - row[r]);
+        if (ptr) (*ptr)[s] = static_cast<int>(std::min<long long>(off, 0x7fffffffLL));
+        off += 64LL * L;
+    }
+    if (ptr) (*ptr)[nsl] = static_cast<int>(std::min<long long>(off, 0x7fffffffLL));
+    return off;
+}
+
+}  // namespace
+}  // namespace hfh
+
+using namespace hfh;
+
+struct hfh_batch {
+    hfh_ctx *ctx = nullptr;
+    hfh_batch_params prm{};
+    char *arena_d = nullptr;
+    Arena arena;
+    SysDesc *d_sys = nullptr;
+    SysState *d_st = nullptr;
+    Tile *d_tiles = nullptr;
+    int *d_ntiles = nullptr;
+    long long tile_cap = 0;
+    // host mirrors
+    struct Slot {
+        int id = -1;          // request id, -1 when free
+        size_t off = 0, bytes = 0;
+        int n = 0, nslice = 0, ntiles = 0;
+        bool reported = false;
+        double *x = nullptr;  // device solution vector
+    };
+    std::vector<Slot> slots;
+    SysState *h_st = nullptr;        // pinned readback of the state table
+    SysDesc *h_desc = nullptr;       // pinned staging of one descriptor
+    SysState *h_init = nullptr;      // pinned staging of one initial state
+    Tile *h_tiles = nullptr;         // pinned tile table staging
+    int *h_ntiles = nullptr;
+    void *stage = nullptr;           // pinned staging of uploads
+    size_t stage_cap = 0;
+    bool tiles_dirty = true;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<int> finished;       // ids finished and not yet collected
+    std::map<int, int> slot_of;      // id -> slot
+    int next_id = 1;
+    long long steps = 0;
+};
+
+namespace {
+
+int ensure_stage(hfh_batch *B, size_t bytes) {
+    if (bytes <= B->stage_cap) return HFH_OK;
+    if (B->stage) HFH_HIP(hipHostFree(B->stage));
+    B->stage = nullptr;
+    B->stage_cap = 0;
+    size_t cap = std::max(bytes, static_cast<size_t>(64) << 20);
+    HFH_HIP(hipHostMalloc(&B->stage, cap, hipHostMallocDefault));
+    B->stage_cap = cap;
+    return HFH_OK;
+}
+
+int free_slot(const hfh_batch *B) {
+    for (size_t s = 0; s < B->slots.size(); ++s)
+        if (B->slots[s].id < 0) return static_cast<int>(s);
+    return -1;
+}
+
+// Reserve a slot and an arena region; fills *slot and *off.
+int reserve(hfh_batch *B, size_t bytes, int *slot, size_t *off) {
+    HFH_CHECK(bytes <= B->arena.cap, HFH_ERR_INVALID, "hfh_batch: request of %zu bytes exceeds the arena of %zu bytes",
+              bytes, B->arena.cap);
+    const int s = free_slot(B);
+    HFH_CHECK(s >= 0, HFH_ERR_BUDGET, "hfh_batch: all %d slots are busy", static_cast<int>(B->slots.size()));
+    HFH_CHECK(B->arena.alloc(bytes, off), HFH_ERR_BUDGET, "hfh_batch: %zu bytes do not fit the HBM budget now "
+              "(%zu of %zu in use)", bytes, B->arena.used, B->arena.cap);
+    *slot = s;
+    return HFH_OK;
+}
+
+// Writes the slot's descriptor and initial state and marks the tables dirty.
+int activate(hfh_batch *B, int slot, size_t off, const Layout &L, int n, int nslice, size_t bytes, int *id) {
+    hipStream_t s = B->ctx->stream;
+    char *base = B->arena_d + off;
+    SysDesc d{};
+    d.n = n;
+    d.nslice = nslice;
+    d.ntiles = tiles_of(nslice);
+    d.sl_ptr = reinterpret_cast<const int *>(base + L.ptr);
+    d.sl_len = reinterpret_cast<const int *>(base + L.len);
+    d.sl_col = reinterpret_cast<const int *>(base + L.col);
+    d.v64 = reinterpret_cast<const double *>(base + L.v64);
+    d.v32 = reinterpret_cast<const float *>(base + L.v32);
+    d.invd = reinterpret_cast<const float *>(base + L.invd);
+    d.b = reinterpret_cast<const double *>(base + L.b);
+    d.x = reinterpret_cast<double *>(base + L.x);
+    d.r64 = reinterpret_cast<double *>(base + L.r64);
+    d.r = reinterpret_cast<float *>(base + L.r);
+    d.c = reinterpret_cast<float *>(base + L.c);
+    d.z = reinterpret_cast<float *>(base + L.z);
+    d.p = reinterpret_cast<float *>(base + L.p);
+    d.q = reinterpret_cast<float *>(base + L.q);
+    d.part = reinterpret_cast<double *>(base + L.part);
+    SysState st{};
+    st.phase = P_FIRST;
+    // the staging of the previous admission must have left the host buffers
+    HFH_HIP(hipStreamSynchronize(s));
+    *B->h_desc = d;
+    *B->h_init = st;
+    HFH_HIP(hipMemcpyAsync(B->d_sys + slot, B->h_desc, sizeof(SysDesc), hipMemcpyHostToDevice, s));
+    HFH_HIP(hipMemcpyAsync(B->d_st + slot, B->h_init, sizeof(SysState), hipMemcpyHostToDevice, s));
+    hfh_batch::Slot &S = B->slots[slot];
+    S.id = B->next_id++;
+    S.off = off;
+    S.bytes = bytes;
+    S.n = n;
+    S.nslice = nslice;
+    S.ntiles = d.ntiles;
+    S.reported = false;
+    S.x = d.x;
+    B->slot_of[S.id] = slot;
+    B->tiles_dirty = true;
+    *id = S.id;
+    return HFH_OK;
+}
+
+int upload_tiles(hfh_batch *B) {
+    hipStream_t s = B->ctx->stream;
+    long long nt = 0;
+    for (size_t k = 0; k < B->slots.size(); ++k) {
+        const hfh_batch::Slot &S = B->slots[k];
+        if (S.id < 0) continue;
+        for (int t = 0; t < S.ntiles; ++t) {
+            HFH_CHECK(nt < B->tile_cap, HFH_ERR_INVALID, "hfh_batch: tile table full");
+            const int s0 = t * kTileSlices;
+            B->h_tiles[nt++] = Tile{static_cast<int>(k), s0, std::min(kTileSlices, S.nslice - s0), t};
+        }
+    }
+    *B->h_ntiles = static_cast<int>(nt);
+    if (nt) HFH_HIP(hipMemcpyAsync(B->d_tiles, B->h_tiles, sizeof(Tile) * nt, hipMemcpyHostToDevice, s));
+    HFH_HIP(hipMemcpyAsync(B->d_ntiles, B->h_ntiles, sizeof(int), hipMemcpyHostToDevice, s));
+    B->tiles_dirty = false;
+    return HFH_OK;
+}
+
+int enqueue_iterations(hfh_batch *B, int iters) {
+    hipStream_t s = B->ctx->stream;
+    const int nslots = static_cast<int>(B->slots.size());
+    const KParams kp{B->prm.rtol, B->prm.atol, B->prm.inner_rtol, B->prm.outer_maxits, B->prm.inner_maxits};
+    for (int it = 0; it < iters; ++it) {
+        hipLaunchKernelGGL(kb_spmv, dim3(kTileGrid), dim3(kBlock), 0, s, B->d_sys, B->d_st, B->d_tiles, B->d_ntiles);
+        hipLaunchKernelGGL(kb_fin1, dim3(nslots), dim3(kBlock), 0, s, B->d_sys, B->d_st, nslots, kp);
+        hipLaunchKernelGGL(kb_update, dim3(kTileGrid), dim3(kBlock), 0, s, B->d_sys, B->d_st, B->d_tiles,
+                           B->d_ntiles);
+        hipLaunchKernelGGL(kb_fin2, dim3(nslots), dim3(kBlock), 0, s, B->d_sys, B->d_st, nslots, kp);
+        hipLaunchKernelGGL(kb_dir, dim3(kTileGrid), dim3(kBlock), 0, s, B->d_sys, B->d_st, B->d_tiles, B->d_ntiles);
+    }
+    return launch_check("hfh_batch iteration");
+}
+
+void release_all(hfh_batch *B) {
+    if (B->exec) (void)hipGraphExecDestroy(B->exec);
+    if (B->graph) (void)hipGraphDestroy(B->graph);
+    if (B->arena_d) (void)hipFree(B->arena_d);
+    if (B->d_sys) (void)hipFree(B->d_sys);
+    if (B->d_st) (void)hipFree(B->d_st);
+    if (B->d_tiles) (void)hipFree(B->d_tiles);
+    if (B->d_ntiles) (void)hipFree(B->d_ntiles);
+    if (B->h_st) (void)hipHostFree(B->h_st);
+    if (B->h_desc) (void)hipHostFree(B->h_desc);
+    if (B->h_init) (void)hipHostFree(B->h_init);
+    if (B->h_tiles) (void)hipHostFree(B->h_tiles);
+    if (B->h_ntiles) (void)hipHostFree(B->h_ntiles);
+    if (B->stage) (void)hipHostFree(B->stage);
+}
+
+}  // namespace
+
+extern "C" int hfh_batch_create(hfh_ctx *ctx, const hfh_batch_params *p, hfh_batch **out) {
+    HFH_CHECK(ctx && p && out, HFH_ERR_INVALID, "hfh_batch_create: null argument");
+    HFH_CHECK(p->max_systems > 0 && p->max_systems <= 65536, HFH_ERR_INVALID,
+              "hfh_batch_create: max_systems %d not in [1, 65536]", p->max_systems);
+    HFH_CHECK(p->hbm_budget >= (1 << 20), HFH_ERR_INVALID, "hfh_batch_create: budget below 1 MiB");
+    HFH_CHECK(p->chunk >= 1 && p->chunk <= 256, HFH_ERR_INVALID, "hfh_batch_create: chunk %d not in [1, 256]",
+              p->chunk);
+    HFH_CHECK(p->outer_maxits > 0 && p->inner_maxits > 0 && p->rtol >= 0 && p->atol >= 0 && p->inner_rtol >= 0,
+              HFH_ERR_INVALID, "hfh_batch_create: bad tolerances or iteration limits");
+    *out = nullptr;
+    hfh_batch *B = new hfh_batch();
+    B->ctx = ctx;
+    B->prm = *p;
+    B->slots.resize(p->max_systems);
+    // every row costs at least 68 bytes of the region (one entry), so this
+    // bounds the tiles the budget can hold
+    B->tile_cap = p->hbm_budget / (68LL * kTileRows) + p->max_systems + 1;
+    auto fail = [&](int rc) {
+        release_all(B);
+        delete B;
+        return rc;
+    };
+    int rc;
+    if ((rc = dev_alloc(reinterpret_cast<void **>(&B->arena_d), static_cast<size_t>(p->hbm_budget))) ||
+        (rc = dev_alloc(reinterpret_cast<void **>(&B->d_sys), sizeof(SysDesc) * p->max_systems)) ||
+        (rc = dev_alloc(reinterpret_cast<void **>(&B->d_st), sizeof(SysState) * p->max_systems)) ||
+        (rc = dev_alloc(reinterpret_cast<void **>(&B->d_tiles), sizeof(Tile) * B->tile_cap)) ||
+        (rc = dev_alloc(reinterpret_cast<void **>(&B->d_ntiles), sizeof(int))))
+        return fail(rc);
+    B->arena.init(static_cast<size_t>(p->hbm_budget));
+    if (hipHostMalloc(reinterpret_cast<void **>(&B->h_st), sizeof(SysState) * p->max_systems) != hipSuccess ||
+        hipHostMalloc(reinterpret_cast<void **>(&B->h_desc), sizeof(SysDesc)) != hipSuccess ||
+        hipHostMalloc(reinterpret_cast<void **>(&B->h_init), sizeof(SysState)) != hipSuccess ||
+        hipHostMalloc(reinterpret_cast<void **>(&B->h_tiles), sizeof(Tile) * B->tile_cap) != hipSuccess ||
+        hipHostMalloc(reinterpret_cast<void **>(&B->h_ntiles), sizeof(int)) != hipSuccess) {
+        set_error("hfh_batch_create: pinned host allocation failed");
+        return fail(HFH_ERR_NOMEM);
+    }
+    hipStream_t s = ctx->stream;
+    // empty slot table (phase P_IDLE everywhere) and no tiles
+    std::memset(B->h_st, 0, sizeof(SysState) * p->max_systems);
+    *B->h_ntiles = 0;
+    if (hipMemcpyAsync(B->d_st, B->h_st, sizeof(SysState) * p->max_systems, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(B->d_ntiles, B->h_ntiles, sizeof(int), hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+        set_error("hfh_batch_create: initial upload failed");
+        return fail(HFH_ERR_HIP);
+    }
+    // capture `chunk` iterations once; the kernels read the tables at run time
+    if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        set_error("hfh_batch_create: stream capture failed");
+        return fail(HFH_ERR_HIP);
+    }
+    const int erc = enqueue_iterations(B, p->chunk);
+    const hipError_t e2 = hipStreamEndCapture(s, &B->graph);
+    if (erc || e2 != hipSuccess || hipGraphInstantiate(&B->exec, B->graph, nullptr, nullptr, 0) != hipSuccess) {
+        if (!erc) set_error("hfh_batch_create: graph capture failed");
+        return fail(erc ? erc : HFH_ERR_HIP);
+    }
+    ++ctx->live;
+    *out = B;
+    return HFH_OK;
+}
+
+extern "C" int hfh_batch_destroy(hfh_batch *B) {
+    if (!B) return HFH_OK;
+    (void)hipStreamSynchronize(B->ctx->stream);
+    --B->ctx->live;
+    release_all(B);
+    delete B;
+    return HFH_OK;
+}
+
+extern "C" int hfh_batch_bytes_csr(int n, const int *row, long long *bytes) {
+    HFH_CHECK(n > 0 && row && bytes, HFH_ERR_INVALID, "hfh_batch_bytes_csr: bad argument");
+    const long long E = sell_entries(n, row, nullptr);
+    const int nsl = (n + 63) / 64;
+    *bytes = static_cast<long long>(Layout(n, nsl, E, tiles_of(nsl)).total);
+    return HFH_OK;
+}
+
+static int fv_shape(int nx, int ny, int nz, int *n, int *W) {
+    HFH_CHECK(nx >= 1 && ny >= 1 && nz >= 1, HFH_ERR_INVALID, "hfh_batch: bad grid %d x %d x %d", nx, ny, nz);
+    const long long N = static_cast<long long>(nx) * ny * nz;
+    HFH_CHECK(N < (1LL << 30), HFH_ERR_INVALID, "hfh_batch: grid too large");
+    *n = static_cast<int>(N);
+    *W = nz > 1 ? 7 : 5;
+    return HFH_OK;
+}
+
+extern "C" int hfh_batch_bytes_fv(int nx, int ny, int nz, long long *bytes) {
+    HFH_CHECK(bytes, HFH_ERR_INVALID, "hfh_batch_bytes_fv: null argument");
+    int n, W, rc;
+    if ((rc = fv_shape(nx, ny, nz, &n, &W))) return rc;
+    const int nsl = (n + 63) / 64;
+    *bytes = static_cast<long long>(Layout(n, nsl, 64LL * W * nsl, tiles_of(nsl)).total);
+    return HFH_OK;
+}
+
+extern "C" int hfh_batch_submit_csr(hfh_batch *B, int n, const int *row, const int *col, const double *val,
+                                    const double *b, const double *x0, int *id) {
+    HFH_CHECK(B && n > 0 && row && col && val && b && id, HFH_ERR_INVALID, "hfh_batch_submit_csr: bad argument");
+    HFH_CHECK(row[0] == 0, HFH_ERR_INVALID, "hfh_batch_submit_csr: row[0] must be 0");
+    for (int r = 0; r < n; ++r) {
+        HFH_CHECK(row[r + 1] >= row[r], HFH_ERR_INVALID, "hfh_batch_submit_csr: row pointer decreases at %d", r);
+        for (int j = row[r]; j < row[r + 1]; ++j)
+            HFH_CHECK(col[j] >= 0 && col[j] < n && (j == row[r] || col[j] > col[j - 1]), HFH_ERR_INVALID,
+                      "hfh_batch_submit_csr: row %d: columns must be ascending and in [0, %d)", r, n);
+    }
+    std::vector<int> sptr;
+    const long long E = sell_entries(n, row, &sptr);
+    HFH_CHECK(E < (1LL << 31), HFH_ERR_UNSUPPORTED, "hfh_batch_submit_csr: SELL copy exceeds 2^31 entries");
+    const int nsl = (n + 63) / 64;
+    const Layout L(n, nsl, E, tiles_of(nsl));
+    // diagonal of every row (the inner Jacobi preconditioner needs it)
+    std::vector<float> invd(n);
+    for (int r = 0; r < n; ++r) {
+        float d = 0.0f;
+        for (int j = row[r]; j < row[r + 1]; ++j)
+            if (col[j] == r) d = static_cast<float>(val[j]);
+        HFH_CHECK(d != 0.0f, HFH_ERR_ZERO_PIVOT, "hfh_batch_submit_csr: row %d has no nonzero diagonal", r);
+        invd[r] = 1.0f / d;
+    }
+    int slot;
+    size_t off;
+    int rc;
+    if ((rc = reserve(B, L.total, &slot, &off))) return rc;
+    HFH_HIP(hipStreamSynchronize(B->ctx->stream));   // the staging buffer may still feed a copy
+    if ((rc = ensure_stage(B, L.upload))) {
+        B->arena.release(off, L.total);
+        return rc;
+    }
+    char *h = static_cast<char *>(B->stage);
+    std::memcpy(h + L.ptr, sptr.data(), sizeof(int) * sptr.size());
+    int *hl = reinterpret_cast<int *>(h + L.len);
+    int *hc = reinterpret_cast<int *>(h + L.col);
+    double *h64 = reinterpret_cast<double *>(h + L.v64);
+    float *h32 = reinterpret_cast<float *>(h + L.v32);
+    for (int s = 0; s < nsl; ++s) {
+        const int base = sptr[s], W = (sptr[s + 1] - base) >> 6;
+        for (int lane = 0; lane < 64; ++lane) {
+            const int r = s * 64 + lane;
+            const int j0 = r < n ? row[r] : 0, len = r < n ? row[r + 1] - row[r] : 0;
+            if (r < n) hl[r] = len;
+            for (int e = 0; e < W; ++e) {
+                const int idx = base + e * 64 + lane;
+                hc[idx] = e < len ? col[j0 + e] : std::min(r, n - 1);
+                h64[idx] = e < len ? val[j0 + e] : 0.0;
+                h32[idx] = e < len ? static_cast<float>(val[j0 + e]) : 0.0f;
+            }
+        }
+    }
+    std::memcpy(h + L.invd, invd.data(), sizeof(float) * n);
+    std::memcpy(h + L.b, b, sizeof(double) * n);
+    if (x0) std::memcpy(h + L.x, x0, sizeof(double) * n);
+    else std::memset(h + L.x, 0, sizeof(double) * n);
+    hipStream_t s = B->ctx->stream;
+    if (hipMemcpyAsync(B->arena_d + off, h, L.upload, hipMemcpyHostToDevice, s) != hipSuccess) {
+        B->arena.release(off, L.total);
+        set_error("hfh_batch_submit_csr: upload failed");
+        return HFH_ERR_HIP;
+    }
+    return activate(B, slot, off, L, n, nsl, L.total, id);
+}
+
+extern "C" int hfh_batch_submit_fv(hfh_batch *B, int nx, int ny, int nz, const double *kcell, const double *b,
+                                   const double *x0, int *id) {
+    HFH_CHECK(B && kcell && b && id, HFH_ERR_INVALID, "hfh_batch_submit_fv: bad argument");
+    int n, W, rc;
+    if ((rc = fv_shape(nx, ny, nz, &n, &W))) return rc;
+    for (int c = 0; c < n; ++c)
+        HFH_CHECK(kcell[c] > 0 && std::isfinite(kcell[c]), HFH_ERR_INVALID,
+                  "hfh_batch_submit_fv: cell coefficient %d must be positive and finite", c);
+    const int nsl = (n + 63) / 64;
+    const long long E = 64LL * W * nsl;
+    HFH_CHECK(E < (1LL << 31), HFH_ERR_UNSUPPORTED, "hfh_batch_submit_fv: SELL copy exceeds 2^31 entries");
+    const Layout L(n, nsl, E, tiles_of(nsl));
+    int slot;
+    size_t off;
+    if ((rc = reserve(B, L.total, &slot, &off))) return rc;
+    HFH_HIP(hipStreamSynchronize(B->ctx->stream));
+    // staging: k (kept in the r64 section until the assembly has run), b, x
+    const size_t kbytes = sizeof(double) * n;
+    if ((rc = ensure_stage(B, 3 * up(kbytes)))) {
+        B->arena.release(off, L.total);
+        return rc;
+    }
+    char *h = static_cast<char *>(B->stage);
+    std::memcpy(h, kcell, kbytes);
+    std::memcpy(h + up(kbytes), b, kbytes);
+    if (x0) std::memcpy(h + 2 * up(kbytes), x0, kbytes);
+    else std::memset(h + 2 * up(kbytes), 0, kbytes);
+    hipStream_t s = B->ctx->stream;
+    char *base = B->arena_d + off;
+    if (hipMemcpyAsync(base + L.r64, h, kbytes, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(base + L.b, h + up(kbytes), kbytes, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(base + L.x, h + 2 * up(kbytes), kbytes, hipMemcpyHostToDevice, s) != hipSuccess) {
+        B->arena.release(off, L.total);
+        set_error("hfh_batch_submit_fv: upload failed");
+        return HFH_ERR_HIP;
+    }
+    const int threads = std::max(nsl * 64, nsl + 1);
+    hipLaunchKernelGGL(kb_fv_assemble, dim3((threads + kBlock - 1) / kBlock), dim3(kBlock), 0, s, nx, ny, nz, n, nsl,
+                       W, reinterpret_cast<const double *>(base + L.r64), reinterpret_cast<int *>(base + L.ptr),
+                       reinterpret_cast<int *>(base + L.len), reinterpret_cast<int *>(base + L.col),
+                       reinterpret_cast<double *>(base + L.v64), reinterpret_cast<float *>(base + L.v32),
+                       reinterpret_cast<float *>(base + L.invd));
+    if ((rc = launch_check("kb_fv_assemble"))) {
+        B->arena.release(off, L.total);
+        return rc;
+    }
+    return activate(B, slot, off, L, n, nsl, L.total, id);
+}
+
+extern "C" int hfh_batch_step(hfh_batch *B, int *active, int *nfinished) {
+    HFH_CHECK(B, HFH_ERR_INVALID, "hfh_batch_step: null engine");
+    hipStream_t s = B->ctx->stream;
+    int rc;
+    if (B->tiles_dirty && (rc = upload_tiles(B))) return rc;
+    int nact = 0;
+    for (const auto &S : B->slots)
+        if (S.id >= 0 && !S.reported) ++nact;
+    if (nact) {
+        HFH_HIP(hipGraphLaunch(B->exec, s));
+        ++B->steps;
+    }
+    HFH_HIP(hipMemcpyAsync(B->h_st, B->d_st, sizeof(SysState) * B->slots.size(), hipMemcpyDeviceToHost, s));
+    HFH_HIP(hipStreamSynchronize(s));
+    int fin = 0;
+    for (size_t k = 0; k < B->slots.size(); ++k) {
+        hfh_batch::Slot &S = B->slots[k];
+        if (S.id < 0 || S.reported) continue;
+        if (B->h_st[k].phase == P_DONE) {
+            S.reported = true;
+            B->finished.push_back(S.id);
+            ++fin;
+        }
+    }
+    if (active) *active = nact - fin;
+    if (nfinished) *nfinished = fin;
+    return HFH_OK;
+}
+
+extern "C" int hfh_batch_finished(hfh_batch *B, int *ids, int cap, int *count) {
+    HFH_CHECK(B && count && (ids || cap == 0) && cap >= 0, HFH_ERR_INVALID, "hfh_batch_finished: bad argument");
+    const int m = std::min<int>(cap, static_cast<int>(B->finished.size()));
+    for (int i = 0; i < m; ++i) ids[i] = B->finished[i];
+    *count = static_cast<int>(B->finished.size());
+    return HFH_OK;
+}
+
+extern "C" int hfh_batch_collect(hfh_batch *B, int id, double *x, hfh_batch_result *res) {
+    HFH_CHECK(B, HFH_ERR_INVALID, "hfh_batch_collect: null engine");
+    auto it = B->slot_of.find(id);
+    HFH_CHECK(it != B->slot_of.end(), HFH_ERR_INVALID, "hfh_batch_collect: unknown request id %d", id);
+    const int slot = it->second;
+    hfh_batch::Slot &S = B->slots[slot];
+    HFH_CHECK(S.reported, HFH_ERR_INVALID, "hfh_batch_collect: request %d has not finished", id);
+    hipStream_t s = B->ctx->stream;
+    HFH_HIP(hipStreamSynchronize(s));
+    if (x) HFH_HIP(hipMemcpy(x, S.x, sizeof(double) * S.n, hipMemcpyDeviceToHost));
+    const SysState &st = B->h_st[slot];
+    if (res) {
+        res->outer_iters = st.outer;
+        res->inner_iters = st.inner_total;
+        res->state = st.status;
+        res->res = st.res;
+        res->res0 = st.res0;
+    }
+    const int status = st.status;
+    // free the slot: the state table entry goes back to P_IDLE before reuse
+    SysState idle{};
+    HFH_HIP(hipStreamSynchronize(s));
+    *B->h_init = idle;
+    HFH_HIP(hipMemcpyAsync(B->d_st + slot, B->h_init, sizeof(SysState), hipMemcpyHostToDevice, s));
+    HFH_HIP(hipStreamSynchronize(s));
+    B->arena.release(S.off, S.bytes);
+    B->slot_of.erase(it);
+    B->finished.erase(std::remove(B->finished.begin(), B->finished.end(), id), B->finished.end());
+    S = hfh_batch::Slot();
+    B->tiles_dirty = true;
+    return status == HFH_SUCCESS_ABS || status == HFH_SUCCESS_REL ? HFH_OK : HFH_ERR_NOT_CONVERGED;
+}
+
+extern "C" int hfh_batch_usage(const hfh_batch *B, int *active, long long *used, long long *cap) {
+    HFH_CHECK(B, HFH_ERR_INVALID, "hfh_batch_usage: null engine");
+    int a = 0;
+    for (const auto &S : B->slots)
+        if (S.id >= 0) ++a;
+    if (active) *active = a;
+    if (used) *used = static_cast<long long>(B->arena.used);
+    if (cap) *cap = static_cast<long long>(B->arena.cap);
+    return HFH_OK;
+}

@@ -244,13 +244,15 @@ __global__ __launch_bounds__(kBlock) void k_spmv_sell(
         const int r = s * 64 + lane;
         const int len = r < nrow ? rlen[r] : 0;
         T acc = 0;
-        int k = 0;
-        for (; k + U <= L; k += U) {
+        // The last step re-reads column slot L-1 for u beyond the slice width,
+        // so every step issues its U loads together; those slots (k+u >= L >=
+        // len) are masked out of the sum.
+        for (int k = 0; k < L; k += U) {
             int c[U];
             T v[U], xv[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                const int idx = base + (k + u) * 64 + lane;
+                const int idx = base + min(k + u, L - 1) * 64 + lane;
                 c[u] = __builtin_nontemporal_load(&scol[idx]);
                 v[u] = __builtin_nontemporal_load(&sval[idx]);
             }
@@ -259,13 +261,6 @@ __global__ __launch_bounds__(kBlock) void k_spmv_sell(
 #pragma unroll
             for (int u = 0; u < U; ++u)
                 if (k + u < len) acc = add_rn(acc, mul_rn(v[u], xv[u]));
-        }
-        for (; k < L; ++k) {
-            const int idx = base + k * 64 + lane;
-            const int c = __builtin_nontemporal_load(&scol[idx]);
-            const T v = __builtin_nontemporal_load(&sval[idx]);
-            const T xv = x[c];
-            if (k < len) acc = add_rn(acc, mul_rn(v, xv));
         }
         if (r < nrow) {
             if (ADD) acc = y[r] + acc;
@@ -416,8 +411,22 @@ int spmv_configure(hfh_csr *A, int algo, int p1, int p2) {
     if (algo == HFH_SPMV_SELL && A->ncol == 0) algo = HFH_SPMV_STREAM;
     const double mean = n ? static_cast<double>(A->nnz) / n : 0.0;
     if (algo == 0) {  // automatic choice
-        algo = HFH_SPMV_STREAM;
-        p1 = 2048;
+        // SELL-64 unless padding to the longest row of each slice would add
+        // more than a quarter to the stored entries; 4 column slots in flight
+        // for long rows, 8 for short (stencil) rows.
+        long long padded = 0;
+        for (int sl = 0; sl < (n + 63) / 64; ++sl) {
+            int L = 0;
+            for (int r = sl * 64; r < std::min(n, sl * 64 + 64); ++r) L = std::max(L, A->h_row[r + 1] - A->h_row[r]);
+            padded += 64LL * L;
+        }
+        if (A->ncol > 0 && static_cast<double>(padded) <= 1.25 * static_cast<double>(std::max(A->nnz, 1))) {
+            algo = HFH_SPMV_SELL;
+            p1 = mean >= 16.0 ? 4 : 8;
+        } else {
+            algo = HFH_SPMV_STREAM;
+            p1 = 2048;
+        }
         p2 = 0;
     }
     const int cus = std::max(A->ctx->num_cu, 1);
@@ -467,7 +476,7 @@ int spmv_configure(hfh_csr *A, int algo, int p1, int p2) {
         HFH_HIP(hipMemcpy(A->sl_len, len.data(), sizeof(int) * len.size(), hipMemcpyHostToDevice));
         A->nslice = nsl;
         A->spmv_algo = HFH_SPMV_SELL;
-        A->spmv_p1 = p1 == 2 || p1 == 8 ? p1 : 4;
+        A->spmv_p1 = p1 == 2 || p1 == 8 || p1 == 16 ? p1 : 4;
         const long long need = (nsl + kBlock / 64 - 1) / (kBlock / 64);
         const long long g = p2 > 0 ? std::min<long long>(p2, need) : need;
         A->nwg = round8(std::max<long long>(g, 1));
@@ -519,6 +528,7 @@ static void launch_t(const hfh_csr *A, const void *x, void *y, const void *w, do
         switch (A->spmv_p1) {
         case 2: hipLaunchKernelGGL((k_spmv_sell<T, ADD, DOT, 2>), g, bl, 0, s, A->nrow, A->nslice, A->sl_ptr, A->sl_len, A->sl_col, (const T *)A->sl_val, xv, yv, wv, part, done); break;
         case 8: hipLaunchKernelGGL((k_spmv_sell<T, ADD, DOT, 8>), g, bl, 0, s, A->nrow, A->nslice, A->sl_ptr, A->sl_len, A->sl_col, (const T *)A->sl_val, xv, yv, wv, part, done); break;
+        case 16: hipLaunchKernelGGL((k_spmv_sell<T, ADD, DOT, 16>), g, bl, 0, s, A->nrow, A->nslice, A->sl_ptr, A->sl_len, A->sl_col, (const T *)A->sl_val, xv, yv, wv, part, done); break;
         default: hipLaunchKernelGGL((k_spmv_sell<T, ADD, DOT, 4>), g, bl, 0, s, A->nrow, A->nslice, A->sl_ptr, A->sl_len, A->sl_col, (const T *)A->sl_val, xv, yv, wv, part, done); break;
         }
     } else if (A->spmv_algo == HFH_SPMV_WAVE) {

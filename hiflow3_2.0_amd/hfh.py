@@ -44,6 +44,17 @@ class DcInfo(C.Structure):
                 ("res0", C.c_double), ("time_ms", C.c_double)]
 
 
+class BatchParams(C.Structure):
+    _fields_ = [("max_systems", C.c_int), ("hbm_budget", C.c_longlong), ("chunk", C.c_int), ("rtol", C.c_double),
+                ("atol", C.c_double), ("outer_maxits", C.c_int), ("inner_rtol", C.c_double),
+                ("inner_maxits", C.c_int)]
+
+
+class BatchResult(C.Structure):
+    _fields_ = [("outer_iters", C.c_int), ("inner_iters", C.c_int), ("state", C.c_int), ("res", C.c_double),
+                ("res0", C.c_double)]
+
+
 _hip = None
 _lmp = None
 
@@ -101,6 +112,16 @@ HIP_API = {
     "hfh_pcg_solve": (_i, [_vp, _vp, _vp, _vp, _P(Control), _P(SolveInfo), _vp]),
     "hfh_gmres_solve": (_i, [_vp, _vp, _i, _i, _vp, _vp, _P(Control), _P(SolveInfo), _vp]),
     "hfh_dc_solve": (_i, [_vp, _vp, _vp, _vp, _vp, _P(Control), _P(Control), _P(DcInfo), _vp]),
+    "hfh_batch_create": (_i, [_vp, _P(BatchParams), _P(_vp)]),
+    "hfh_batch_destroy": (_i, [_vp]),
+    "hfh_batch_bytes_csr": (_i, [_i, _vp, _P(_ll)]),
+    "hfh_batch_bytes_fv": (_i, [_i, _i, _i, _P(_ll)]),
+    "hfh_batch_submit_csr": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp, _P(_i)]),
+    "hfh_batch_submit_fv": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _P(_i)]),
+    "hfh_batch_step": (_i, [_vp, _P(_i), _P(_i)]),
+    "hfh_batch_finished": (_i, [_vp, _vp, _i, _P(_i)]),
+    "hfh_batch_collect": (_i, [_vp, _i, _vp, _P(BatchResult)]),
+    "hfh_batch_usage": (_i, [_vp, _P(_i), _P(_ll), _P(_ll)]),
 }
 
 class LmpParams(C.Structure):
@@ -115,6 +136,7 @@ class LmpResult(C.Structure):
 LMP_API = {
     "hfl_problem_size": (_i, [_i, _i, _i, _i, _d, _d, C.c_ulonglong, _P(_i), _P(_ll)]),
     "hfl_problem_fill": (_i, [_i, _i, _i, _i, _d, _d, C.c_ulonglong, _vp, _vp, _vp, _vp]),
+    "hfl_fv_cells": (_i, [_i, _i, _i, _d, C.c_ulonglong, _vp, _vp]),
     "hfl_uniform_pm1": (_d, [C.c_ulonglong, C.c_ulonglong, C.c_ulonglong]),
     "hfl_normal01": (_d, [C.c_ulonglong, C.c_ulonglong, C.c_ulonglong]),
     "hfl_multicoloring": (_i, [_i, _vp, _vp, _P(_i), _vp, _vp, _vp]),
@@ -459,6 +481,98 @@ def dc(Ah, Al, b, x, Pl=None, outer=(20, 1e-10, 0.0, 1e6), inner=(1000, 0.0, 1e-
             "res0": inf.res0, "time_ms": inf.time_ms, "rc": rc, "hist": hist[: inf.outer_iters + 1].copy()}
 
 
+class BatchEngine:
+    """Serving engine (hfh_batch_*): continuous batching of SPD solves by
+    FP32-inner / FP64-outer CG.  submit_* return a request id, or None when
+    the request does not fit the free slots / HBM budget now."""
+
+    def __init__(self, ctx, max_systems=256, hbm_budget=1 << 30, chunk=16, rtol=1e-10, atol=0.0,
+                 outer_maxits=30, inner_rtol=1e-4, inner_maxits=5000):
+        self.ctx = ctx
+        self.params = BatchParams(max_systems, int(hbm_budget), chunk, rtol, atol, outer_maxits, inner_rtol,
+                                  inner_maxits)
+        h = C.c_void_p()
+        _ck(hip().hfh_batch_create(ctx.h, C.byref(self.params), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            hip().hfh_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def bytes_csr(row):
+        b = C.c_longlong()
+        row = np.ascontiguousarray(row, np.int32)
+        _ck(hip().hfh_batch_bytes_csr(len(row) - 1, _p(row), C.byref(b)))
+        return b.value
+
+    @staticmethod
+    def bytes_fv(nx, ny, nz):
+        b = C.c_longlong()
+        _ck(hip().hfh_batch_bytes_fv(nx, ny, nz, C.byref(b)))
+        return b.value
+
+    def _admit(self, rc, rid):
+        if rc == ERR_BUDGET:
+            return None
+        _ck(rc)
+        return rid.value
+
+    def submit_csr(self, row, col, val, b, x0=None):
+        row = np.ascontiguousarray(row, np.int32)
+        col = np.ascontiguousarray(col, np.int32)
+        val = np.ascontiguousarray(val, np.float64)
+        b = np.ascontiguousarray(b, np.float64)
+        x0 = None if x0 is None else np.ascontiguousarray(x0, np.float64)
+        rid = C.c_int()
+        rc = hip().hfh_batch_submit_csr(self.h, len(row) - 1, _p(row), _p(col), _p(val), _p(b), _p(x0),
+                                        C.byref(rid))
+        return self._admit(rc, rid)
+
+    def submit_fv(self, nx, ny, nz, kcell, b, x0=None):
+        kcell = np.ascontiguousarray(kcell, np.float64)
+        b = np.ascontiguousarray(b, np.float64)
+        x0 = None if x0 is None else np.ascontiguousarray(x0, np.float64)
+        rid = C.c_int()
+        rc = hip().hfh_batch_submit_fv(self.h, nx, ny, nz, _p(kcell), _p(b), _p(x0), C.byref(rid))
+        return self._admit(rc, rid)
+
+    def step(self):
+        """One chunk of iterations; returns (active, finished_this_step)."""
+        a, f = C.c_int(), C.c_int()
+        _ck(hip().hfh_batch_step(self.h, C.byref(a), C.byref(f)))
+        return a.value, f.value
+
+    def finished(self):
+        cnt = C.c_int()
+        _ck(hip().hfh_batch_finished(self.h, None, 0, C.byref(cnt)))
+        ids = np.zeros(max(cnt.value, 1), np.int32)
+        _ck(hip().hfh_batch_finished(self.h, _p(ids), cnt.value, C.byref(cnt)))
+        return [int(i) for i in ids[: cnt.value]]
+
+    def collect(self, rid, n):
+        """Returns (x, info dict); info['rc'] is the collect status."""
+        x = np.empty(n, np.float64)
+        r = BatchResult()
+        rc = hip().hfh_batch_collect(self.h, rid, _p(x), C.byref(r))
+        if rc not in (OK, ERR_NOT_CONVERGED):
+            _ck(rc)
+        return x, {"outer_iters": r.outer_iters, "inner_iters": r.inner_iters, "state": r.state, "res": r.res,
+                   "res0": r.res0, "rc": rc}
+
+    def usage(self):
+        a, u, c = C.c_int(), C.c_longlong(), C.c_longlong()
+        _ck(hip().hfh_batch_usage(self.h, C.byref(a), C.byref(u), C.byref(c)))
+        return a.value, u.value, c.value
+
+
 # ---------------------------------------------------------------- host layer
 
 def problem(kind, nx, ny=None, nz=None, p0=1.0, p1=0.0, seed=0):
@@ -476,6 +590,16 @@ def problem(kind, nx, ny=None, nz=None, p0=1.0, p1=0.0, seed=0):
     if lmp().hfl_problem_fill(kind, nx, ny, nz, p0, p1, seed, _p(row), _p(col), _p(val), _p(rhs)):
         raise ValueError("problem generation failed")
     return row, col, val, rhs
+
+
+def fv_cells(nx, ny, nz, sigma, seed):
+    """Cell coefficients k = exp(sigma g) and rhs of the FV diffusion problem."""
+    n = nx * ny * nz
+    k = np.empty(n, np.float64)
+    b = np.empty(n, np.float64)
+    if lmp().hfl_fv_cells(nx, ny, nz, sigma, seed, _p(k), _p(b)):
+        raise ValueError("invalid grid")
+    return k, b
 
 
 def multicoloring(row, col):

@@ -33,6 +33,11 @@ extern "C" int hfl_problem_fill(int kind, int nx, int ny, int nz, double p0, dou
     return problem_fill(spec(kind, nx, ny, nz, p0, p1, seed), row, col, val, rhs);
 }
 
+extern "C" int hfl_fv_cells(int nx, int ny, int nz, double sigma, unsigned long long seed, double *kcell,
+                            double *rhs) {
+    return fv_cells(spec(kFvDiffusion, nx, ny, nz, sigma, 0.0, seed), kcell, rhs);
+}
+
 extern "C" double hfl_uniform_pm1(unsigned long long seed, unsigned long long stream, unsigned long long index) {
     return uniform_pm1(seed, stream, index);
 }

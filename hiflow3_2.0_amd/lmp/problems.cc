@@ -124,6 +124,17 @@ int problem_size(const ProblemSpec &p, int *n, long long *nnz) {
     return total > 0x7fffffffLL ? -1 : 0;
 }
 
+int fv_cells(const ProblemSpec &p, double *kc, double *rhs) {
+    const long long n = static_cast<long long>(p.nx) * p.ny * p.nz;
+    if (p.nx < 1 || p.ny < 1 || p.nz < 1 || n > 0x7fffffffLL) return -1;
+#pragma omp parallel for schedule(static)
+    for (long long c = 0; c < n; ++c) {
+        if (kc) kc[c] = std::exp(p.p0 * normal01(p.seed, kStreamCoef, c));
+        if (rhs) rhs[c] = uniform_pm1(p.seed, kStreamRhs, c);
+    }
+    return 0;
+}
+
 int problem_fill(const ProblemSpec &p, int *row, int *col, double *val, double *rhs) {
     int n;
     long long nnz;
@@ -135,8 +146,7 @@ int problem_fill(const ProblemSpec &p, int *row, int *col, double *val, double *
     std::vector<double> kc;
     if (p.kind == kFvDiffusion) {
         kc.resize(n);
-#pragma omp parallel for schedule(static)
-        for (long long c = 0; c < n; ++c) kc[c] = std::exp(p.p0 * normal01(p.seed, kStreamCoef, c));
+        fv_cells(p, kc.data(), nullptr);
     }
     // upwind velocity scale: cell Peclet |b| h / eps = p1 at the largest |b0| = sqrt(0.59)
     const double h = 1.0 / g.nx;

@@ -33,5 +33,8 @@ double normal01(uint64_t seed, uint64_t stream, uint64_t index);      // Box-Mul
 
 int problem_size(const ProblemSpec &p, int *n, long long *nnz);
 int problem_fill(const ProblemSpec &p, int *row, int *col, double *val, double *rhs);
+// Cell data of the FV diffusion problem: k_c = exp(p0 g_c) and the
+// right-hand side, exactly the values problem_fill uses (either may be null).
+int fv_cells(const ProblemSpec &p, double *kc, double *rhs);
 
 }  // namespace hflmp

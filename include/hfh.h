@@ -126,7 +126,7 @@ int hfh_spmv_add(const hfh_csr *A, const hfh_vec *x, hfh_vec *y);
  * length); HFH_SPMV_WAVE: per-wavefront tiles of param1 nonzeros (256/512/
  * 1024) and at most 63 rows, bit-identical like STREAM; HFH_SPMV_SELL: a
  * SELL-64 copy (slices of 64 rows, column-major, built on the device from
- * the CSR arrays), lane = row, param1 column slots in flight (2/4/8),
+ * the CSR arrays), lane = row, param1 column slots in flight (2/4/8/16),
  * bit-identical like STREAM.
  * param2: workgroups (0 = automatic).  algo 0 = automatic.
  * Takes effect at once on an uploaded matrix and on later uploads. */
@@ -214,6 +214,63 @@ typedef struct {
 int hfh_dc_solve(const hfh_csr *Ah, const hfh_csr *Al, hfh_prec *Pl, const hfh_vec *b,
                  hfh_vec *x, const hfh_control *outer, const hfh_control *inner,
                  hfh_dc_info *info, double *hist);
+
+/* ---- serving: many independent SPD systems solved together ----
+ * No reference counterpart: the engine runs, for every admitted system, the
+ * defect correction of MPIR<D,S>::Solve (mpir.cc:69-327) with an FP32 inner
+ * Jacobi-PCG (cg.cc:84-248; lpreconditioner.cc:104-166) and FP64 true
+ * residual, update and stop test, for all systems in the same batched
+ * launches (a hipGraph of `chunk` iterations).  Systems are admitted while
+ * others run, as slots and the HBM budget allow, and leave when collected.
+ * A system's result does not depend on what else is in the batch: its
+ * arithmetic and summation order are fixed by its own size.
+ * Outer stop: true residual <= max(atol, rtol * initial true residual);
+ * inner stop: recursive FP32 residual <= inner_rtol * its start, or
+ * inner_maxits.  Calls on one engine must come from one thread. */
+typedef struct hfh_batch hfh_batch;
+typedef struct {
+    int max_systems;       /* resident systems (slots), 1..65536 */
+    long long hbm_budget;  /* bytes of the device arena holding admitted systems */
+    int chunk;             /* iterations per hfh_batch_step (graph replay), 1..256 */
+    double rtol, atol;     /* outer stop */
+    int outer_maxits;      /* corrections per system */
+    double inner_rtol;     /* inner stop */
+    int inner_maxits;      /* iterations per inner solve */
+} hfh_batch_params;
+
+typedef struct {
+    int outer_iters;       /* corrections applied */
+    int inner_iters;       /* inner iterations, all corrections */
+    int state;             /* HFH_SUCCESS_ABS / _REL or HFH_FAIL_* */
+    double res;            /* final true FP64 residual norm */
+    double res0;           /* initial true residual norm */
+} hfh_batch_result;
+
+int hfh_batch_create(hfh_ctx *ctx, const hfh_batch_params *p, hfh_batch **B);
+int hfh_batch_destroy(hfh_batch *B);
+/* bytes of the budget a request would take */
+int hfh_batch_bytes_csr(int n, const int *row, long long *bytes);
+int hfh_batch_bytes_fv(int nx, int ny, int nz, long long *bytes);
+/* Admit an SPD CSR system (columns ascending, nonzero diagonal; x0 nullable
+ * = zero start).  Returns HFH_ERR_BUDGET when no slot or not enough budget is
+ * free now (retry after hfh_batch_step / hfh_batch_collect), HFH_ERR_INVALID
+ * when the request can never fit.  Host arrays are copied before return. */
+int hfh_batch_submit_csr(hfh_batch *B, int n, const int *row, const int *col, const double *val,
+                         const double *b, const double *x0, int *id);
+/* Admit the cell-centred FV diffusion system of an nx*ny*nz grid (5-pt when
+ * nz == 1, else 7-pt) from its cell coefficients k > 0, assembled on the
+ * device with the entries hfl_problem_fill (kind 3) produces. */
+int hfh_batch_submit_fv(hfh_batch *B, int nx, int ny, int nz, const double *kcell, const double *b,
+                        const double *x0, int *id);
+/* Run `chunk` iterations of every admitted system; *active: systems still
+ * running, *finished: systems that finished in this step. */
+int hfh_batch_step(hfh_batch *B, int *active, int *finished);
+/* ids of finished, not yet collected systems (up to cap); *count = all of them */
+int hfh_batch_finished(hfh_batch *B, int *ids, int cap, int *count);
+/* Copy the solution (x nullable) and outcome of a finished system and free
+ * its slot and memory.  Returns HFH_ERR_NOT_CONVERGED for a failed solve. */
+int hfh_batch_collect(hfh_batch *B, int id, double *x, hfh_batch_result *res);
+int hfh_batch_usage(const hfh_batch *B, int *active, long long *used, long long *cap);
 
 #ifdef __cplusplus
 }

@@ -22,6 +22,9 @@ int hfl_problem_size(int kind, int nx, int ny, int nz, double p0, double p1, uns
                      int *n, long long *nnz);
 int hfl_problem_fill(int kind, int nx, int ny, int nz, double p0, double p1, unsigned long long seed,
                      int *row, int *col, double *val, double *rhs);
+/* cell coefficients k = exp(sigma g) and rhs of the FV diffusion problem
+ * (kind 3), the values hfl_problem_fill uses; either output may be null */
+int hfl_fv_cells(int nx, int ny, int nz, double sigma, unsigned long long seed, double *kcell, double *rhs);
 double hfl_uniform_pm1(unsigned long long seed, unsigned long long stream, unsigned long long index);
 double hfl_normal01(unsigned long long seed, unsigned long long stream, unsigned long long index);
 

@@ -24,7 +24,7 @@ def rand_csr(rng, n, ncol, lens):
     return row, col, val
 
 
-ALGOS = [(1, 2048), (1, 1024), (1, 512), (1, 4096), (3, 512), (3, 1024), (3, 256), (2, 0), (4, 4), (4, 2), (4, 8)]
+ALGOS = [(1, 2048), (1, 1024), (1, 512), (1, 4096), (3, 512), (3, 1024), (3, 256), (2, 0), (4, 4), (4, 2), (4, 8), (4, 16)]
 BITWISE = {1, 3, 4}  # row-block, wave-tile and SELL kernels sum rows in the reference order
 
 
