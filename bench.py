@@ -10,8 +10,24 @@ resident in HBM.  `value` is the CSR SpMV bandwidth inside the timed solves:
 algorithmic bytes per launch (12 nnz + 20 n + 4) over the mean SpMV launch
 time, measured with HIP events on the context stream.
 
+The same line carries the other measured workloads under "workloads":
+  c3  variable-coefficient FV diffusion, 192^3 cells (7-pt, k = exp(g), g ~ N(0,1)):
+      FP64 CG + multicolour SGS against FP32-inner / FP64-outer defect
+      correction (FP32 multicolour SGS inner PCG), both to a true relative
+      residual of 1e-10; the error of each against an FP64 CG solve to 1e-12
+  c4  upwind convection-diffusion, 128^3 cells, cell Peclet 5:
+      FP64 GMRES(30) + level-scheduled ILU(0), rtol 1e-8
+  c5  serving stream: SPD FV diffusion requests of log-uniform size 1e3..1e6
+      (2D 5-pt or 3D 7-pt), k = exp(N(0, 0.5^2)), FP32-inner / FP64-outer CG
+      to 1e-10, Poisson arrivals at 0.8x the measured closed-loop capacity,
+      HBM admission budget 200 GB; solves/s and p50 / p99 latency
+      (arrival to result).
+
 Prints one JSON line.  --gpus N > 1 is not provided: this build runs on one GPU.
 """
+import math
+import threading
+import queue
 import argparse
 import importlib.util
 import json
@@ -91,7 +107,18 @@ def bench_c2(P, args):
     X = P.Vector(ctx, n)
     log(f"[bench] uploaded; device {ctx.info()}")
 
-    # SpMV parity at full size against the oracle (bitwise for rows within one tile)
+    if args.calibrate:
+        # known-size stream for the PMC byte counters: y += x over 2^27 doubles
+        # reads 2 GiB and writes 1 GiB per launch (k_axpy)
+        m = 1 << 27
+        cx, cy = P.Vector(ctx, np.ones(m)), P.Vector(ctx, np.ones(m))
+        for _ in range(3):
+            cy.axpy(cx, 1.0)
+        ctx.sync()
+        del cx, cy
+        log("[bench] calibration axpy done (3 launches, 2 GiB read + 1 GiB written each)")
+
+    # SpMV parity at full size against the oracle (bit-identical rows)
     xr = np.random.default_rng(1).uniform(-1, 1, n)
     Xr, Yr = P.Vector(ctx, xr), P.Vector(ctx, n)
     A.spmv(Xr, Yr)
@@ -166,14 +193,188 @@ def bench_c2(P, args):
     }
 
 
+def true_rel(P, ctx, A, X, b):
+    R = P.Vector(ctx, len(b))
+    A.spmv(X, R)
+    return float(np.linalg.norm(b - R.numpy()) / np.linalg.norm(b))
+
+
+def bench_c3(P, args):
+    t0 = time.perf_counter()
+    row, col, val, b = P.problem(P.FV_DIFFUSION, 192, p0=1.0, seed=args.seed)
+    nc, color, sizes, perm = P.multicoloring(row, col)
+    prow, pcol, pval = P.csr_permute(row, col, val, perm)
+    bp = b[perm].copy()
+    del row, col, val, b
+    n, nnz = len(bp), len(pcol)
+    log(f"[c3] FV 192^3: n={n} nnz={nnz} colours={nc} setup {time.perf_counter() - t0:.1f}s")
+    ctx = P.Context(0)
+    A = P.CsrMatrix(ctx, prow, pcol, pval)
+    A32 = P.CsrMatrix.cast_of(A, np.float32)
+    B = P.Vector(ctx, bp)
+    nb = float(np.linalg.norm(bp))
+    P64 = P.Preconditioner.mcsgs(A, sizes, 1.0)
+    P32 = P.Preconditioner.mcsgs(A32, sizes, 1.0)
+    # reference solution: FP64 CG to 1e-12
+    Xr = P.Vector(ctx, n)
+    P.pcg(A, B, Xr, P64, maxits=20000, atol=0.0, rtol=1e-12)
+    xref = Xr.numpy()
+    out = {"n": n, "nnz": nnz, "colours": nc}
+    reps = max(1, args.c3_reps)
+    t64, tdc = [], []
+    for _ in range(reps + 1):
+        X = P.Vector(ctx, n)
+        inf = P.pcg(A, B, X, P64, maxits=20000, atol=0.0, rtol=1e-10)
+        t64.append(inf["time_ms"])
+    x64 = X.numpy()
+    out["fp64_cg_mcsgs"] = {"ms": round(float(np.median(t64[1:])), 2), "iters": inf["iters"],
+                            "true_rel_residual": true_rel(P, ctx, A, X, bp),
+                            "err_vs_fp64_1e-12": float(np.linalg.norm(x64 - xref) / np.linalg.norm(xref))}
+    for _ in range(reps + 1):
+        X = P.Vector(ctx, n)
+        # inner stop: 1e-4 relative, or the outer target reached (MPIR's ecl abs_tol)
+        d = P.dc(A, A32, B, X, P32, outer=(60, 1e-10 * nb, 0.0, 1e6), inner=(20000, 0.5e-10 * nb, 1e-4, 1e6))
+        tdc.append(d["time_ms"])
+    xdc = X.numpy()
+    out["dc_fp32_mcsgs"] = {"ms": round(float(np.median(tdc[1:])), 2), "outer": d["outer_iters"],
+                            "inner_iters": d["inner_iters"], "inner_rtol": 1e-4, "inner_atol_rel_b": 0.5e-10,
+                            "true_rel_residual": true_rel(P, ctx, A, X, bp),
+                            "err_vs_fp64_1e-12": float(np.linalg.norm(xdc - xref) / np.linalg.norm(xref))}
+    out["speedup_dc_vs_fp64"] = round(out["fp64_cg_mcsgs"]["ms"] / out["dc_fp32_mcsgs"]["ms"], 3)
+    log(f"[c3] {out}")
+    return out
+
+
+def bench_c4(P, args):
+    t0 = time.perf_counter()
+    row, col, val, b = P.problem(P.UPWIND_CONVDIFF, 128, p0=1e-2, p1=5.0, seed=args.seed)
+    lu = P.ilu0_factor(row, col, val)
+    ctx = P.Context(0)
+    A = P.CsrMatrix(ctx, row, col, val)
+    LU = P.CsrMatrix(ctx, row, col, lu)
+    Pi = P.Preconditioner.ilu0(LU, row, col)
+    lo, _ = P.levels(row, col)
+    up, _ = P.levels(row, col, upper=True)
+    B = P.Vector(ctx, b)
+    log(f"[c4] upwind 128^3: n={len(b)} nnz={len(col)} levels {len(lo) - 1}/{len(up) - 1} "
+        f"setup {time.perf_counter() - t0:.1f}s")
+    Z = P.Vector(ctx, len(b))
+    Pi.apply(B, Z)
+    ctx.sync()
+    t = time.perf_counter()
+    for _ in range(5):
+        Pi.apply(B, Z)
+    ctx.sync()
+    apply_ms = (time.perf_counter() - t) / 5 * 1e3
+    ts = []
+    for _ in range(max(1, args.c4_reps) + 1):
+        X = P.Vector(ctx, len(b))
+        inf = P.gmres(A, B, X, Pi, m=30, maxits=5000, atol=0.0, rtol=1e-8)
+        ts.append(inf["time_ms"])
+    out = {"n": len(b), "nnz": len(col), "levels": [len(lo) - 1, len(up) - 1],
+           "ms": round(float(np.median(ts[1:])), 2), "iters": inf["iters"], "ilu_apply_ms": round(apply_ms, 3),
+           "true_rel_residual": true_rel(P, ctx, A, X, b)}
+    log(f"[c4] {out}")
+    return out
+
+
+def c5_shape(P, seed):
+    """Request size log-uniform in [1e3, 1e6]; 2D 5-pt or 3D 7-pt with p = 1/2."""
+    u1 = (P.lmp().hfl_uniform_pm1(seed, 3, 0) + 1.0) / 2.0
+    u2 = (P.lmp().hfl_uniform_pm1(seed, 3, 1) + 1.0) / 2.0
+    n = int(round(math.exp(math.log(1e3) + u1 * (math.log(1e6) - math.log(1e3)))))
+    if u2 < 0.5:
+        s = max(2, int(round(math.sqrt(n))))
+        return (s, s, 1)
+    s = max(2, int(round(n ** (1.0 / 3.0))))
+    return (s, s, s)
+
+
+def c5_stream(P, ctx, nreq, rate, chunk, seed0, budget, max_systems):
+    """Serve nreq requests arriving as a Poisson process of the given rate
+    (None: all at t = 0, closed loop).  Returns (wall s, latencies s, ok)."""
+    shapes = [c5_shape(P, seed0 + i) for i in range(nreq)]
+    arr = (np.cumsum(np.random.default_rng(seed0).exponential(1.0 / rate, nreq)) if rate else np.zeros(nreq))
+    q = queue.Queue(maxsize=32)
+
+    def producer():   # the clients: cell coefficients and right-hand sides
+        for i, (nx, ny, nz) in enumerate(shapes):
+            k, b = P.fv_cells(nx, ny, nz, 0.5, seed0 + i)
+            q.put((i, nx, ny, nz, k, b))
+    threading.Thread(target=producer, daemon=True).start()
+    E = P.BatchEngine(ctx, max_systems=max_systems, hbm_budget=budget, chunk=chunk, rtol=1e-10, atol=0.0,
+                      outer_maxits=30, inner_rtol=1e-4, inner_maxits=20000)
+    lat = np.zeros(nreq)
+    ids, pending = {}, []
+    nxt = done = ok = 0
+    cur = None
+    t0 = time.perf_counter()
+    while done < nreq:
+        now = time.perf_counter() - t0
+        while nxt < nreq and arr[nxt] <= now:
+            pending.append(nxt)
+            nxt += 1
+        while pending:   # admission in arrival order, while slots and budget allow
+            if cur is None:
+                cur = q.get()
+            i, nx, ny, nz, k, b = cur
+            rid = E.submit_fv(nx, ny, nz, k, b)
+            if rid is None:
+                break
+            ids[rid] = (i, nx * ny * nz)
+            pending.pop(0)
+            cur = None
+        active, fin = E.step()
+        if fin:
+            for rid in E.finished():
+                i, n = ids.pop(rid)
+                _, info = E.collect(rid, n)
+                lat[i] = time.perf_counter() - t0 - arr[i]
+                ok += info["rc"] == 0
+                done += 1
+        if active == 0 and not pending and nxt < nreq:
+            time.sleep(max(0.0, min(0.002, arr[nxt] - (time.perf_counter() - t0))))
+    wall = time.perf_counter() - t0
+    E.close()
+    return wall, lat, ok, shapes
+
+
+def bench_c5(P, args):
+    ctx = P.Context(0)
+    budget = 200 << 30
+    cal = min(256, args.c5_requests)
+    wall, lat, ok, _ = c5_stream(P, ctx, cal, None, args.c5_chunk, 50_000, budget, args.c5_slots)
+    cap = cal / wall
+    log(f"[c5] closed loop {cal} requests: {cap:.1f} solves/s")
+    rate = 0.8 * cap
+    wall, lat, ok, shapes = c5_stream(P, ctx, args.c5_requests, rate, args.c5_chunk, 10_000, budget, args.c5_slots)
+    ns = np.array([a * b * c for a, b, c in shapes])
+    out = {"requests": args.c5_requests, "arrival_rate_per_s": round(rate, 2),
+           "closed_loop_capacity_per_s": round(cap, 2), "solves_per_s": round(args.c5_requests / wall, 2),
+           "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 2),
+           "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 2), "converged": int(ok),
+           "mean_n": int(ns.mean()), "chunk": args.c5_chunk, "slots": args.c5_slots, "hbm_budget_GB": 200,
+           "wall_s": round(wall, 2)}
+    log(f"[c5] {out}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="c2", choices=["c2"])
+    ap.add_argument("--workloads", default="c3,c4,c5",
+                    help="secondary workloads measured after the headline (comma list of c3,c4,c5, or none)")
+    ap.add_argument("--c3-reps", type=int, default=2)
+    ap.add_argument("--c4-reps", type=int, default=2)
+    ap.add_argument("--c5-requests", type=int, default=4096)
+    ap.add_argument("--c5-chunk", type=int, default=16)
+    ap.add_argument("--c5-slots", type=int, default=512)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="also run a known-size axpy (for scaling rocprofv3 byte counters)")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -182,6 +383,13 @@ def main():
         sys.exit(2)
     P = load_pkg()
     out = bench_c2(P, args)
+    wl = {}
+    for w in [w for w in args.workloads.split(",") if w and w != "none"]:
+        fn = {"c3": bench_c3, "c4": bench_c4, "c5": bench_c5}[w]
+        t0 = time.perf_counter()
+        wl[w] = fn(P, args)
+        wl[w]["bench_s"] = round(time.perf_counter() - t0, 1)
+    out["workloads"] = wl
     print(json.dumps(out), flush=True)
 
 

@@ -86,6 +86,19 @@ struct hfh_levels {
     int nlev = 0;
     std::vector<int> ptr;      // host, nlev+1
     int *rows = nullptr;       // device, nrow
+    int *ptr_d = nullptr;      // device copy of ptr
+    std::vector<std::pair<int, int>> runs;   // launch plan: level ranges [first, last)
+};
+
+// Rows of a preconditioner sweep in SELL-64 form, slices of 64 consecutive
+// rows of one colour, column-major, padded to the longest row of the slice.
+struct hfh_sweep {
+    int nslice = 0;
+    int *ptr = nullptr;        // [nslice+1] element offset of each slice
+    int *len = nullptr;        // [nrow] entries of each row in this sweep
+    int *col = nullptr;        // padded columns (padding: a row of the same slice)
+    void *val = nullptr;       // padded values (padding 0)
+    std::vector<int> slice0;   // host: first slice of each colour (ncolors+1)
 };
 
 struct hfh_prec {
@@ -101,6 +114,7 @@ struct hfh_prec {
     int ncolors = 0;
     std::vector<int> color_off;     // host, ncolors+1
     hfh_levels lo, up;              // SGS / ILU0 level schedules
+    hfh_sweep fw, bw;               // MC-SGS: lower-colour / upper-colour entries per row
 };
 
 namespace hfh {

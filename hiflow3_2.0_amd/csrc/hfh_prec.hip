@@ -20,6 +20,8 @@ namespace hfh {
 
 constexpr int kMaxColors = 64;
 
+constexpr int kMergeBlock = 1024;   // threads of a merged-level workgroup; levels up to this size merge
+
 struct ColorOffsets {
     int off[kMaxColors + 1];
 };
@@ -101,19 +103,123 @@ __global__ __launch_bounds__(kBlock) void k_mc_bwd(int c, int ncolors, ColorOffs
     out[i] = chunk * idv[i];
 }
 
+// The same colour steps on the SELL-64 copies of the sweeps: one wavefront
+// per slice of 64 rows of colour c, lane = row, so the value/column streams
+// and the gathers of neighbouring rows are contiguous.  Each row's entries
+// and its block-by-block summation are those of k_mc_fwd / k_mc_bwd.
+constexpr int kSweepU = 8;   // column slots in flight
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_mcs_fwd(int c, ColorOffsets co, bool last, int slice0, int nsl,
+                                                    const int *__restrict__ sptr, const int *__restrict__ slen,
+                                                    const int *__restrict__ scol, const T *__restrict__ sval,
+                                                    const T *__restrict__ in, const T *__restrict__ idv,
+                                                    const T *__restrict__ dv, T *out, const int *__restrict__ done) {
+    if (done && *done) return;
+    const int w = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (w >= nsl) return;
+    const int s = slice0 + w;
+    const int i = co.off[c] + w * 64 + lane;
+    const bool valid = i < co.off[c + 1];
+    const int base = sptr[s], L = (sptr[s + 1] - base) >> 6;
+    const int len = valid ? slen[i] : 0;
+    T chunk = valid ? in[i] * static_cast<T>(-1) : static_cast<T>(0);
+    int blk = 0, blk_end = co.off[1];
+    T res = 0;
+    for (int k = 0; k < L; k += kSweepU) {
+        int cj[kSweepU];
+        T v[kSweepU], xv[kSweepU];
+#pragma unroll
+        for (int u = 0; u < kSweepU; ++u) {
+            const int idx = base + min(k + u, L - 1) * 64 + lane;
+            cj[u] = __builtin_nontemporal_load(&scol[idx]);
+            v[u] = __builtin_nontemporal_load(&sval[idx]);
+        }
+#pragma unroll
+        for (int u = 0; u < kSweepU; ++u) xv[u] = out[cj[u]];
+#pragma unroll
+        for (int u = 0; u < kSweepU; ++u) {
+            if (k + u >= len) continue;
+            while (cj[u] >= blk_end) {   // entries are column sorted: flush finished blocks
+                chunk += res;
+                res = 0;
+                ++blk;
+                blk_end = co.off[blk + 1];
+            }
+            res += v[u] * xv[u];
+        }
+    }
+    if (!valid) return;
+    chunk += res;
+    chunk *= idv[i];
+    if (last) chunk = (chunk * dv[i]) * idv[i];
+    out[i] = chunk;
+}
+
+// Backward step of colour c: entries stored block by block from the highest
+// colour down, ascending columns within a block.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_mcs_bwd(int c, int ncolors, ColorOffsets co, int slice0, int nsl,
+                                                    const int *__restrict__ sptr, const int *__restrict__ slen,
+                                                    const int *__restrict__ scol, const T *__restrict__ sval,
+                                                    const T *__restrict__ idv, const T *__restrict__ dv, T *out,
+                                                    const int *__restrict__ done) {
+    if (done && *done) return;
+    const int w = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (w >= nsl) return;
+    const int s = slice0 + w;
+    const int i = co.off[c] + w * 64 + lane;
+    const bool valid = i < co.off[c + 1];
+    const int base = sptr[s], L = (sptr[s + 1] - base) >> 6;
+    const int len = valid ? slen[i] : 0;
+    T chunk = valid ? out[i] * dv[i] : static_cast<T>(0);
+    int blk = ncolors - 1, blk_lo = co.off[ncolors - 1];
+    T res = 0;
+    for (int k = 0; k < L; k += kSweepU) {
+        int cj[kSweepU];
+        T v[kSweepU], xv[kSweepU];
+#pragma unroll
+        for (int u = 0; u < kSweepU; ++u) {
+            const int idx = base + min(k + u, L - 1) * 64 + lane;
+            cj[u] = __builtin_nontemporal_load(&scol[idx]);
+            v[u] = __builtin_nontemporal_load(&sval[idx]);
+        }
+#pragma unroll
+        for (int u = 0; u < kSweepU; ++u) xv[u] = out[cj[u]];
+#pragma unroll
+        for (int u = 0; u < kSweepU; ++u) {
+            if (k + u >= len) continue;
+            while (cj[u] < blk_lo) {   // next lower colour block
+                chunk += res;
+                res = 0;
+                --blk;
+                blk_lo = co.off[blk];
+            }
+            res += v[u] * xv[u];
+        }
+    }
+    if (!valid) return;
+    chunk += res;
+    out[i] = chunk * idv[i];
+}
+
+// sweep values from the CSR values through the position map (-1: padding)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_sweep_fill(long long m, const int *__restrict__ pos,
+                                                       const T *__restrict__ val, T *__restrict__ out) {
+    const long long k = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x;
+    if (k >= m) return;
+    const int j = pos[k];
+    out[k] = j >= 0 ? val[j] : static_cast<T>(0);
+}
+
 // Level-scheduled triangular sweeps.  mode 0: SGS forward (D+L) y = r;
 // mode 1: SGS backward (D+U) z = D y; mode 2: ILU unit-lower forward;
 // mode 3: ILU (D+U) backward.
 template <typename T, int MODE>
-__global__ __launch_bounds__(kBlock) void k_trsv_level(const int *__restrict__ lrows, int cnt,
-                                                       const int *__restrict__ row, const int *__restrict__ col,
-                                                       const T *__restrict__ val, const int *__restrict__ diag,
-                                                       const T *__restrict__ in, T *__restrict__ out,
-                                                       const int *__restrict__ done) {
-    if (done && *done) return;
-    const int k = blockIdx.x * kBlock + threadIdx.x;
-    if (k >= cnt) return;
-    const int i = lrows[k];
+__device__ __forceinline__ void trsv_row(int i, const int *__restrict__ row, const int *__restrict__ col,
+                                         const T *__restrict__ val, const int *__restrict__ diag,
+                                         const T *__restrict__ in, T *out) {
     const int js = row[i], je = row[i + 1];
     if (MODE == 0 || MODE == 2) {
         T acc = in[i];
@@ -133,6 +239,36 @@ __global__ __launch_bounds__(kBlock) void k_trsv_level(const int *__restrict__ l
             if (cj > i) acc -= out[cj] * val[j];
         }
         out[i] = acc / d;
+    }
+}
+
+template <typename T, int MODE>
+__global__ __launch_bounds__(kBlock) void k_trsv_level(const int *__restrict__ lrows, int cnt,
+                                                       const int *__restrict__ row, const int *__restrict__ col,
+                                                       const T *__restrict__ val, const int *__restrict__ diag,
+                                                       const T *__restrict__ in, T *out,
+                                                       const int *__restrict__ done) {
+    if (done && *done) return;
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= cnt) return;
+    trsv_row<T, MODE>(lrows[k], row, col, val, diag, in, out);
+}
+
+// A run of consecutive small levels [l0, l1) in one workgroup: the barrier
+// between levels makes each level's results visible to the next, so the
+// run costs one launch instead of l1 - l0.
+template <typename T, int MODE>
+__global__ __launch_bounds__(kMergeBlock) void k_trsv_run(const int *__restrict__ lrows,
+                                                          const int *__restrict__ lptr, int l0, int l1,
+                                                          const int *__restrict__ row, const int *__restrict__ col,
+                                                          const T *__restrict__ val, const int *__restrict__ diag,
+                                                          const T *__restrict__ in, T *out,
+                                                          const int *__restrict__ done) {
+    if (done && *done) return;
+    for (int l = l0; l < l1; ++l) {
+        const int a = lptr[l], cnt = lptr[l + 1] - a;
+        for (int k = threadIdx.x; k < cnt; k += kMergeBlock) trsv_row<T, MODE>(lrows[a + k], row, col, val, diag, in, out);
+        __syncthreads();
     }
 }
 
@@ -212,8 +348,24 @@ static int upload_levels(hfh_levels &L, int n, int nlev, const int *ptr, const i
     L.nlev = nlev;
     L.ptr.assign(ptr, ptr + nlev + 1);
     int rc = dev_alloc(reinterpret_cast<void **>(&L.rows), sizeof(int) * static_cast<size_t>(n));
+    if (!rc) rc = dev_alloc(reinterpret_cast<void **>(&L.ptr_d), sizeof(int) * static_cast<size_t>(nlev + 1));
     if (rc) return rc;
     HFH_HIP(hipMemcpy(L.rows, rows, sizeof(int) * static_cast<size_t>(n), hipMemcpyHostToDevice));
+    HFH_HIP(hipMemcpy(L.ptr_d, ptr, sizeof(int) * static_cast<size_t>(nlev + 1), hipMemcpyHostToDevice));
+    // launch plan: runs of at least two consecutive levels of at most
+    // kMergeBlock rows share one launch; other levels get one launch each
+    L.runs.clear();
+    for (int l = 0; l < nlev;) {
+        int e = l;
+        while (e < nlev && ptr[e + 1] - ptr[e] <= kMergeBlock) ++e;
+        if (e - l >= 2) {
+            L.runs.emplace_back(l, e);
+            l = e;
+        } else {
+            L.runs.emplace_back(l, l + 1);
+            ++l;
+        }
+    }
     return HFH_OK;
 }
 
@@ -237,57 +389,167 @@ static int validate_levels(const hfh_csr *A, const hfh_levels &L, bool upper, co
     return HFH_OK;
 }
 
+template <typename T, int MODE>
+static void sweep(const hfh_levels &Lv, const hfh_csr *A, const T *in, T *out, const int *done, hipStream_t s) {
+    for (const auto &run : Lv.runs) {
+        if (run.second - run.first > 1) {
+            hipLaunchKernelGGL((k_trsv_run<T, MODE>), dim3(1), dim3(kMergeBlock), 0, s, Lv.rows, Lv.ptr_d, run.first,
+                               run.second, A->row, A->col, (const T *)A->val, A->diag, in, out, done);
+            continue;
+        }
+        const int a = Lv.ptr[run.first], cnt = Lv.ptr[run.first + 1] - a;
+        if (cnt == 0) continue;
+        hipLaunchKernelGGL((k_trsv_level<T, MODE>), dim3((cnt + kBlock - 1) / kBlock), dim3(kBlock), 0, s, Lv.rows + a,
+                           cnt, A->row, A->col, (const T *)A->val, A->diag, in, out, done);
+    }
+}
+
 template <typename T>
 static int trsv_sweeps(hfh_prec *P, const T *in, T *out, const int *done) {
     const hfh_csr *A = P->A;
     hipStream_t s = P->ctx->stream;
-    const bool ilu = P->kind == HFH_PREC_ILU0;
-    for (int l = 0; l < P->lo.nlev; ++l) {
-        const int a = P->lo.ptr[l], cnt = P->lo.ptr[l + 1] - a;
-        if (cnt == 0) continue;
-        const dim3 g((cnt + kBlock - 1) / kBlock);
-        if (ilu)
-            hipLaunchKernelGGL((k_trsv_level<T, 2>), g, dim3(kBlock), 0, s, P->lo.rows + a, cnt, A->row, A->col, (const T *)A->val, A->diag, in, out, done);
-        else
-            hipLaunchKernelGGL((k_trsv_level<T, 0>), g, dim3(kBlock), 0, s, P->lo.rows + a, cnt, A->row, A->col, (const T *)A->val, A->diag, in, out, done);
-    }
-    for (int l = 0; l < P->up.nlev; ++l) {
-        const int a = P->up.ptr[l], cnt = P->up.ptr[l + 1] - a;
-        if (cnt == 0) continue;
-        const dim3 g((cnt + kBlock - 1) / kBlock);
-        if (ilu)
-            hipLaunchKernelGGL((k_trsv_level<T, 3>), g, dim3(kBlock), 0, s, P->up.rows + a, cnt, A->row, A->col, (const T *)A->val, A->diag, in, out, done);
-        else
-            hipLaunchKernelGGL((k_trsv_level<T, 1>), g, dim3(kBlock), 0, s, P->up.rows + a, cnt, A->row, A->col, (const T *)A->val, A->diag, in, out, done);
+    if (P->kind == HFH_PREC_ILU0) {
+        sweep<T, 2>(P->lo, A, in, out, done, s);
+        sweep<T, 3>(P->up, A, in, out, done, s);
+    } else {
+        sweep<T, 0>(P->lo, A, in, out, done, s);
+        sweep<T, 1>(P->up, A, in, out, done, s);
     }
     return launch_check("k_trsv_level");
 }
 
 template <typename T>
 static int mcsgs_apply(hfh_prec *P, const T *in, T *out, const int *done) {
-    const hfh_csr *A = P->A;
     hipStream_t s = P->ctx->stream;
     ColorOffsets co;
     for (int c = 0; c <= P->ncolors; ++c) co.off[c] = P->color_off[c];
-    const T *val = P->sval ? (const T *)P->sval : (const T *)A->val;
     const T *idv = (const T *)P->invd, *dv = (const T *)P->dv;
     const int nc = P->ncolors;
+    constexpr int WPB = kBlock / 64;
     for (int c = 0; c < nc; ++c) {
         const int cnt = co.off[c + 1] - co.off[c];
         if (cnt == 0) continue;
-        const dim3 g((cnt + kBlock - 1) / kBlock);
-        if (c == 0 && nc > 1)
-            hipLaunchKernelGGL(k_mc_fwd0<T>, g, dim3(kBlock), 0, s, co.off[0], co.off[1], in, idv, out, done);
-        else
-            hipLaunchKernelGGL(k_mc_fwd<T>, g, dim3(kBlock), 0, s, c, co, c == nc - 1, A->row, A->col, val, in, idv, dv, out, done);
+        if (c == 0 && nc > 1) {
+            hipLaunchKernelGGL(k_mc_fwd0<T>, dim3((cnt + kBlock - 1) / kBlock), dim3(kBlock), 0, s, co.off[0],
+                               co.off[1], in, idv, out, done);
+        } else {
+            const int s0 = P->fw.slice0[c], nsl = P->fw.slice0[c + 1] - s0;
+            hipLaunchKernelGGL(k_mcs_fwd<T>, dim3((nsl + WPB - 1) / WPB), dim3(kBlock), 0, s, c, co, c == nc - 1, s0,
+                               nsl, P->fw.ptr, P->fw.len, P->fw.col, (const T *)P->fw.val, in, idv, dv, out, done);
+        }
     }
     for (int c = nc - 2; c >= 0; --c) {
         const int cnt = co.off[c + 1] - co.off[c];
         if (cnt == 0) continue;
-        const dim3 g((cnt + kBlock - 1) / kBlock);
-        hipLaunchKernelGGL(k_mc_bwd<T>, g, dim3(kBlock), 0, s, c, nc, co, A->row, A->col, val, idv, dv, out, done);
+        const int s0 = P->bw.slice0[c], nsl = P->bw.slice0[c + 1] - s0;
+        hipLaunchKernelGGL(k_mcs_bwd<T>, dim3((nsl + WPB - 1) / WPB), dim3(kBlock), 0, s, c, nc, co, s0, nsl,
+                           P->bw.ptr, P->bw.len, P->bw.col, (const T *)P->bw.val, idv, dv, out, done);
     }
     return launch_check("k_mc_sweep");
+}
+
+// Builds one sweep copy from the host CSR structure.  forward: entries with a
+// column in a lower colour (CSR order); backward: entries in a higher colour,
+// block by block from the highest colour down.
+static int build_sweep(hfh_prec *P, const hfh_csr *A, bool forward, hfh_sweep &W) {
+    const int nc = P->ncolors, n = A->nrow;
+    const std::vector<int> &off = P->color_off;
+    const std::vector<int> &row = A->h_row, &col = A->h_col;
+    std::vector<int> len(std::max(n, 1), 0);
+    W.slice0.assign(nc + 1, 0);
+    for (int c = 0; c < nc; ++c) W.slice0[c + 1] = W.slice0[c] + (off[c + 1] - off[c] + 63) / 64;
+    const int nsl = W.slice0[nc];
+    // entries of row i in sweep order
+    std::vector<std::pair<int, int>> segs;
+    auto entries = [&](int c, int i, std::vector<int> &e) {
+        e.clear();
+        if (forward) {
+            for (int j = row[i]; j < row[i + 1] && col[j] < off[c]; ++j) e.push_back(j);
+            return;
+        }
+        // columns ascend, so the higher-colour entries split into one
+        // segment per colour block; emit the segments from the highest down
+        segs.clear();
+        int j = row[i];
+        const int je = row[i + 1];
+        while (j < je && col[j] < off[c + 1]) ++j;
+        int b = c + 1;
+        while (j < je) {
+            while (col[j] >= off[b + 1]) ++b;
+            const int st = j;
+            while (j < je && col[j] < off[b + 1]) ++j;
+            segs.emplace_back(st, j);
+        }
+        for (auto it = segs.rbegin(); it != segs.rend(); ++it)
+            for (int k = it->first; k < it->second; ++k) e.push_back(k);
+    };
+    std::vector<int> ptr(nsl + 1, 0), e;
+    long long total = 0;
+    for (int c = 0; c < nc; ++c)
+        for (int w = 0; w < W.slice0[c + 1] - W.slice0[c]; ++w) {
+            int L = 0;
+            for (int lane = 0; lane < 64; ++lane) {
+                const int i = off[c] + w * 64 + lane;
+                if (i >= off[c + 1]) break;
+                entries(c, i, e);
+                len[i] = static_cast<int>(e.size());
+                L = std::max(L, len[i]);
+            }
+            ptr[W.slice0[c] + w] = static_cast<int>(total);
+            total += 64LL * L;
+            HFH_CHECK(total < (1LL << 31), HFH_ERR_UNSUPPORTED, "hfh_mcsgs_create: sweep copy exceeds 2^31 entries");
+        }
+    ptr[nsl] = static_cast<int>(total);
+    std::vector<int> pcol(std::max<long long>(total, 1)), pos(std::max<long long>(total, 1));
+    for (int c = 0; c < nc; ++c)
+        for (int w = 0; w < W.slice0[c + 1] - W.slice0[c]; ++w) {
+            const int base = ptr[W.slice0[c] + w], L = (ptr[W.slice0[c] + w + 1] - base) / 64;
+            for (int lane = 0; lane < 64; ++lane) {
+                const int i = off[c] + w * 64 + lane;
+                const bool valid = i < off[c + 1];
+                if (valid) entries(c, i, e);
+                else e.clear();
+                const int own = std::min(i, off[c + 1] - 1);   // a row of this slice
+                for (int k = 0; k < L; ++k) {
+                    const int idx = base + k * 64 + lane;
+                    pos[idx] = k < static_cast<int>(e.size()) ? e[k] : -1;
+                    pcol[idx] = k < static_cast<int>(e.size()) ? col[e[k]] : own;
+                }
+            }
+        }
+    W.nslice = nsl;
+    int rc;
+    if ((rc = dev_alloc(reinterpret_cast<void **>(&W.ptr), sizeof(int) * ptr.size())) ||
+        (rc = dev_alloc(reinterpret_cast<void **>(&W.len), sizeof(int) * len.size())) ||
+        (rc = dev_alloc(reinterpret_cast<void **>(&W.col), sizeof(int) * pcol.size())) ||
+        (rc = dev_alloc(&W.val, A->elem() * pcol.size())))
+        return rc;
+    HFH_HIP(hipMemcpy(W.ptr, ptr.data(), sizeof(int) * ptr.size(), hipMemcpyHostToDevice));
+    HFH_HIP(hipMemcpy(W.len, len.data(), sizeof(int) * len.size(), hipMemcpyHostToDevice));
+    HFH_HIP(hipMemcpy(W.col, pcol.data(), sizeof(int) * pcol.size(), hipMemcpyHostToDevice));
+    int *dpos = nullptr;
+    if ((rc = dev_alloc(reinterpret_cast<void **>(&dpos), sizeof(int) * pos.size()))) return rc;
+    HFH_HIP(hipMemcpy(dpos, pos.data(), sizeof(int) * pos.size(), hipMemcpyHostToDevice));
+    const long long m = static_cast<long long>(pos.size());
+    const dim3 g(static_cast<unsigned>((m + kBlock - 1) / kBlock));
+    const void *src = P->sval ? P->sval : A->val;
+    hipStream_t s = A->ctx->stream;
+    if (A->dtype == HFH_F64)
+        hipLaunchKernelGGL(k_sweep_fill<double>, g, dim3(kBlock), 0, s, m, dpos, (const double *)src, (double *)W.val);
+    else
+        hipLaunchKernelGGL(k_sweep_fill<float>, g, dim3(kBlock), 0, s, m, dpos, (const float *)src, (float *)W.val);
+    rc = launch_check("k_sweep_fill");
+    (void)hipStreamSynchronize(s);
+    (void)hipFree(dpos);
+    return rc;
+}
+
+static void free_sweep(hfh_sweep &W) {
+    if (W.ptr) (void)hipFree(W.ptr);
+    if (W.len) (void)hipFree(W.len);
+    if (W.col) (void)hipFree(W.col);
+    if (W.val) (void)hipFree(W.val);
+    W = hfh_sweep();
 }
 
 int prec_apply_raw(hfh_prec *P, const void *in, void *out, const int *done) {
@@ -389,6 +651,8 @@ extern "C" int hfh_mcsgs_create(const hfh_csr *A, int ncolors, const int *color_
             rc = launch_check("k_scale_offdiag");
         }
     }
+    if (!rc) rc = build_sweep(P, A, true, P->fw);
+    if (!rc) rc = build_sweep(P, A, false, P->bw);
     if (rc) { hfh_prec_destroy(P); return rc; }
     *out = P;
     return HFH_OK;
@@ -442,6 +706,10 @@ extern "C" int hfh_prec_destroy(hfh_prec *P) {
     if (P->sval) (void)hipFree(P->sval);
     if (P->lo.rows) (void)hipFree(P->lo.rows);
     if (P->up.rows) (void)hipFree(P->up.rows);
+    if (P->lo.ptr_d) (void)hipFree(P->lo.ptr_d);
+    if (P->up.ptr_d) (void)hipFree(P->up.ptr_d);
+    free_sweep(P->fw);
+    free_sweep(P->bw);
     delete P;
     return HFH_OK;
 }

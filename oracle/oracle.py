@@ -220,6 +220,20 @@ def invdiag(A):
     return d
 
 
+def cast_d2f(x):
+    x = np.ascontiguousarray(x, np.float64)
+    out = np.zeros(len(x), np.float32)
+    lib().hfo_cast_d2f(len(x), _p(x), _p(out))
+    return out
+
+
+def cast_f2d(x):
+    x = np.ascontiguousarray(x, np.float32)
+    out = np.zeros(len(x), np.float64)
+    lib().hfo_cast_f2d(len(x), _p(x), _p(out))
+    return out
+
+
 def sgs(A, r):
     out = np.zeros(A.n, A.val.dtype)
     getattr(lib(), f"hfo_sgs_{_sfx(A.val.dtype)}")(A.n, _p(A.row), _p(A.col), _p(A.val),

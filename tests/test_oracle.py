@@ -135,6 +135,22 @@ def test_blas1_closed_forms(oracle):
     assert list(oracle.ewmult(np.array([2.0, 3.0]), np.array([4.0, -1.0]))) == [8.0, -3.0]
 
 
+def test_invdiag_and_casts(oracle, pkg):
+    A, _ = fv_matrix(pkg, oracle, 7, 6, 5)
+    assert np.array_equal(oracle.invdiag(A), 1.0 / np.diag(A.to_dense()))
+    # rows without a stored diagonal: the reference leaves its buffer entry
+    # unset (lmatrix_csr_cpu.cc:2417-2424); the oracle defines it as 0
+    B = oracle.csr_from_coo(3, [0, 1, 2], [1, 1, 0], [1.0, 4.0, 2.0])
+    assert list(oracle.invdiag(B)) == [0.0, 0.25, 0.0]
+    # CastFrom: static_cast, round to nearest even (numpy's astype is IEEE RN)
+    x = np.random.default_rng(9).normal(size=1000) * np.logspace(-30, 30, 1000)
+    x[:4] = [1.0 + 2.0**-24, 1.0 + 3 * 2.0**-24, -(1.0 + 2.0**-24), 0.0]
+    f = oracle.cast_d2f(x)
+    assert np.array_equal(f, x.astype(np.float32))
+    assert f[0] == np.float32(1.0) and f[1] == np.float32(1.0 + 2.0**-22)
+    assert np.array_equal(oracle.cast_f2d(f), f.astype(np.float64))
+
+
 def test_spmv_matches_dense(oracle, pkg):
     rng = np.random.default_rng(1)
     A, _ = fv_matrix(pkg, oracle, 9, 7, 5)
