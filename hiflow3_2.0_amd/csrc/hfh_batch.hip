@@ -77,7 +77,8 @@ struct SysDesc {
 };
 
 struct SysState {
-    int phase, status, outer, inner, inner_total, pad;
+    int phase, status, outer, inner, inner_total;
+    int id;   // request id of the occupant (the kernels never change it)
     double res0, res, tol, rz, rr0, alpha, beta;
 };
 
@@ -313,6 +314,16 @@ __global__ __launch_bounds__(kBlock) void kb_dir(const SysDesc *__restrict__ sd,
     }
 }
 
+// Writes one slot's descriptor (when `desc`) and state from the kernel
+// arguments: stream-ordered behind the running iterations, with no host
+// staging buffer to keep alive.
+__global__ void kb_set_slot(SysDesc *__restrict__ sd, SysState *__restrict__ st, int slot, int desc, SysDesc d,
+                            SysState v) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (desc) sd[slot] = d;
+    st[slot] = v;
+}
+
 // Device assembly of the cell-centred FV diffusion matrix (5-pt when nz == 1,
 // else 7-pt) in SELL-64 form of uniform width W from the cell coefficients:
 // the same entries, in the same order and with the same arithmetic as the
@@ -476,6 +487,7 @@ using namespace hfh;
 struct hfh_batch {
     hfh_ctx *ctx = nullptr;
     hfh_batch_params prm{};
+    hipStream_t cs = nullptr;        // uploads, device assembly and solution copies
     char *arena_d = nullptr;
     Arena arena;
     SysDesc *d_sys = nullptr;
@@ -483,22 +495,28 @@ struct hfh_batch {
     Tile *d_tiles = nullptr;
     int *d_ntiles = nullptr;
     long long tile_cap = 0;
-    // host mirrors
     struct Slot {
         int id = -1;          // request id, -1 when free
         size_t off = 0, bytes = 0;
         int n = 0, nslice = 0, ntiles = 0;
         bool reported = false;
         double *x = nullptr;  // device solution vector
+        SysState final_state{};
     };
     std::vector<Slot> slots;
-    SysState *h_st = nullptr;        // pinned readback of the state table
-    SysDesc *h_desc = nullptr;       // pinned staging of one descriptor
-    SysState *h_init = nullptr;      // pinned staging of one initial state
-    Tile *h_tiles = nullptr;         // pinned tile table staging
-    int *h_ntiles = nullptr;
-    void *stage = nullptr;           // pinned staging of uploads
+    // state readback, double buffered: replay k's copy is read while replay k+1 runs
+    SysState *h_st[2] = {nullptr, nullptr};
+    hipEvent_t ev_rb[2] = {nullptr, nullptr};
+    int rb_next = 0, rb_pending = -1;
+    // tile table staging, double buffered
+    Tile *h_tiles[2] = {nullptr, nullptr};
+    int *h_ntiles[2] = {nullptr, nullptr};
+    hipEvent_t ev_tiles[2] = {nullptr, nullptr};
+    int tb = 0;
+    // upload staging: free again once ev_up has passed
+    void *stage = nullptr;
     size_t stage_cap = 0;
+    hipEvent_t ev_up = nullptr;
     bool tiles_dirty = true;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -539,7 +557,9 @@ int reserve(hfh_batch *B, size_t bytes, int *slot, size_t *off) {
     return HFH_OK;
 }
 
-// Writes the slot's descriptor and initial state and marks the tables dirty.
+// Makes an uploaded system live: the main stream waits for the upload, then
+// writes the slot's descriptor and initial state; the tile table is rebuilt
+// before the next replay.
 int activate(hfh_batch *B, int slot, size_t off, const Layout &L, int n, int nslice, size_t bytes, int *id) {
     hipStream_t s = B->ctx->stream;
     char *base = B->arena_d + off;
@@ -562,16 +582,18 @@ int activate(hfh_batch *B, int slot, size_t off, const Layout &L, int n, int nsl
     d.p = reinterpret_cast<float *>(base + L.p);
     d.q = reinterpret_cast<float *>(base + L.q);
     d.part = reinterpret_cast<double *>(base + L.part);
-    SysState st{};
-    st.phase = P_FIRST;
-    // the staging of the previous admission must have left the host buffers
-    HFH_HIP(hipStreamSynchronize(s));
-    *B->h_desc = d;
-    *B->h_init = st;
-    HFH_HIP(hipMemcpyAsync(B->d_sys + slot, B->h_desc, sizeof(SysDesc), hipMemcpyHostToDevice, s));
-    HFH_HIP(hipMemcpyAsync(B->d_st + slot, B->h_init, sizeof(SysState), hipMemcpyHostToDevice, s));
     hfh_batch::Slot &S = B->slots[slot];
     S.id = B->next_id++;
+    SysState st{};
+    st.phase = P_FIRST;
+    st.id = S.id;
+    HFH_HIP(hipStreamWaitEvent(s, B->ev_up, 0));
+    hipLaunchKernelGGL(kb_set_slot, dim3(1), dim3(64), 0, s, B->d_sys, B->d_st, slot, 1, d, st);
+    int rc = launch_check("kb_set_slot");
+    if (rc) {
+        S = hfh_batch::Slot();
+        return rc;
+    }
     S.off = off;
     S.bytes = bytes;
     S.n = n;
@@ -587,6 +609,10 @@ int activate(hfh_batch *B, int slot, size_t off, const Layout &L, int n, int nsl
 
 int upload_tiles(hfh_batch *B) {
     hipStream_t s = B->ctx->stream;
+    const int buf = B->tb;
+    // the copy from this buffer two uploads ago must have left it
+    HFH_HIP(hipEventSynchronize(B->ev_tiles[buf]));
+    Tile *ht = B->h_tiles[buf];
     long long nt = 0;
     for (size_t k = 0; k < B->slots.size(); ++k) {
         const hfh_batch::Slot &S = B->slots[k];
@@ -594,12 +620,14 @@ int upload_tiles(hfh_batch *B) {
         for (int t = 0; t < S.ntiles; ++t) {
             HFH_CHECK(nt < B->tile_cap, HFH_ERR_INVALID, "hfh_batch: tile table full");
             const int s0 = t * kTileSlices;
-            B->h_tiles[nt++] = Tile{static_cast<int>(k), s0, std::min(kTileSlices, S.nslice - s0), t};
+            ht[nt++] = Tile{static_cast<int>(k), s0, std::min(kTileSlices, S.nslice - s0), t};
         }
     }
-    *B->h_ntiles = static_cast<int>(nt);
-    if (nt) HFH_HIP(hipMemcpyAsync(B->d_tiles, B->h_tiles, sizeof(Tile) * nt, hipMemcpyHostToDevice, s));
-    HFH_HIP(hipMemcpyAsync(B->d_ntiles, B->h_ntiles, sizeof(int), hipMemcpyHostToDevice, s));
+    *B->h_ntiles[buf] = static_cast<int>(nt);
+    if (nt) HFH_HIP(hipMemcpyAsync(B->d_tiles, ht, sizeof(Tile) * nt, hipMemcpyHostToDevice, s));
+    HFH_HIP(hipMemcpyAsync(B->d_ntiles, B->h_ntiles[buf], sizeof(int), hipMemcpyHostToDevice, s));
+    HFH_HIP(hipEventRecord(B->ev_tiles[buf], s));
+    B->tb ^= 1;
     B->tiles_dirty = false;
     return HFH_OK;
 }
@@ -619,6 +647,23 @@ int enqueue_iterations(hfh_batch *B, int iters) {
     return launch_check("hfh_batch iteration");
 }
 
+// Marks the systems a state readback shows finished (the id guards against a
+// slot that was collected and reused after the copy was taken).
+int take_readback(hfh_batch *B, int buf) {
+    HFH_HIP(hipEventSynchronize(B->ev_rb[buf]));
+    const SysState *h = B->h_st[buf];
+    int fin = 0;
+    for (size_t k = 0; k < B->slots.size(); ++k) {
+        hfh_batch::Slot &S = B->slots[k];
+        if (S.id < 0 || S.reported || h[k].id != S.id || h[k].phase != P_DONE) continue;
+        S.reported = true;
+        S.final_state = h[k];
+        B->finished.push_back(S.id);
+        ++fin;
+    }
+    return fin;
+}
+
 void release_all(hfh_batch *B) {
     if (B->exec) (void)hipGraphExecDestroy(B->exec);
     if (B->graph) (void)hipGraphDestroy(B->graph);
@@ -627,12 +672,16 @@ void release_all(hfh_batch *B) {
     if (B->d_st) (void)hipFree(B->d_st);
     if (B->d_tiles) (void)hipFree(B->d_tiles);
     if (B->d_ntiles) (void)hipFree(B->d_ntiles);
-    if (B->h_st) (void)hipHostFree(B->h_st);
-    if (B->h_desc) (void)hipHostFree(B->h_desc);
-    if (B->h_init) (void)hipHostFree(B->h_init);
-    if (B->h_tiles) (void)hipHostFree(B->h_tiles);
-    if (B->h_ntiles) (void)hipHostFree(B->h_ntiles);
+    for (int b = 0; b < 2; ++b) {
+        if (B->h_st[b]) (void)hipHostFree(B->h_st[b]);
+        if (B->h_tiles[b]) (void)hipHostFree(B->h_tiles[b]);
+        if (B->h_ntiles[b]) (void)hipHostFree(B->h_ntiles[b]);
+        if (B->ev_rb[b]) (void)hipEventDestroy(B->ev_rb[b]);
+        if (B->ev_tiles[b]) (void)hipEventDestroy(B->ev_tiles[b]);
+    }
+    if (B->ev_up) (void)hipEventDestroy(B->ev_up);
     if (B->stage) (void)hipHostFree(B->stage);
+    if (B->cs) (void)hipStreamDestroy(B->cs);
 }
 
 }  // namespace
@@ -667,21 +716,26 @@ extern "C" int hfh_batch_create(hfh_ctx *ctx, const hfh_batch_params *p, hfh_bat
         (rc = dev_alloc(reinterpret_cast<void **>(&B->d_ntiles), sizeof(int))))
         return fail(rc);
     B->arena.init(static_cast<size_t>(p->hbm_budget));
-    if (hipHostMalloc(reinterpret_cast<void **>(&B->h_st), sizeof(SysState) * p->max_systems) != hipSuccess ||
-        hipHostMalloc(reinterpret_cast<void **>(&B->h_desc), sizeof(SysDesc)) != hipSuccess ||
-        hipHostMalloc(reinterpret_cast<void **>(&B->h_init), sizeof(SysState)) != hipSuccess ||
-        hipHostMalloc(reinterpret_cast<void **>(&B->h_tiles), sizeof(Tile) * B->tile_cap) != hipSuccess ||
-        hipHostMalloc(reinterpret_cast<void **>(&B->h_ntiles), sizeof(int)) != hipSuccess) {
-        set_error("hfh_batch_create: pinned host allocation failed");
+    bool ok = hipStreamCreateWithFlags(&B->cs, hipStreamNonBlocking) == hipSuccess &&
+              hipEventCreateWithFlags(&B->ev_up, hipEventDisableTiming) == hipSuccess;
+    for (int b = 0; b < 2 && ok; ++b)
+        ok = hipHostMalloc(reinterpret_cast<void **>(&B->h_st[b]), sizeof(SysState) * p->max_systems) == hipSuccess &&
+             hipHostMalloc(reinterpret_cast<void **>(&B->h_tiles[b]), sizeof(Tile) * B->tile_cap) == hipSuccess &&
+             hipHostMalloc(reinterpret_cast<void **>(&B->h_ntiles[b]), sizeof(int)) == hipSuccess &&
+             hipEventCreateWithFlags(&B->ev_rb[b], hipEventDisableTiming) == hipSuccess &&
+             hipEventCreateWithFlags(&B->ev_tiles[b], hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
+        set_error("hfh_batch_create: stream, event or pinned host allocation failed");
         return fail(HFH_ERR_NOMEM);
     }
     hipStream_t s = ctx->stream;
     // empty slot table (phase P_IDLE everywhere) and no tiles
-    std::memset(B->h_st, 0, sizeof(SysState) * p->max_systems);
-    *B->h_ntiles = 0;
-    if (hipMemcpyAsync(B->d_st, B->h_st, sizeof(SysState) * p->max_systems, hipMemcpyHostToDevice, s) != hipSuccess ||
-        hipMemcpyAsync(B->d_ntiles, B->h_ntiles, sizeof(int), hipMemcpyHostToDevice, s) != hipSuccess ||
-        hipStreamSynchronize(s) != hipSuccess) {
+    std::memset(B->h_st[0], 0, sizeof(SysState) * p->max_systems);
+    *B->h_ntiles[0] = 0;
+    if (hipMemcpyAsync(B->d_st, B->h_st[0], sizeof(SysState) * p->max_systems, hipMemcpyHostToDevice, s) !=
+            hipSuccess ||
+        hipMemcpyAsync(B->d_ntiles, B->h_ntiles[0], sizeof(int), hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipEventRecord(B->ev_up, B->cs) != hipSuccess || hipStreamSynchronize(s) != hipSuccess) {
         set_error("hfh_batch_create: initial upload failed");
         return fail(HFH_ERR_HIP);
     }
@@ -704,6 +758,7 @@ extern "C" int hfh_batch_create(hfh_ctx *ctx, const hfh_batch_params *p, hfh_bat
 extern "C" int hfh_batch_destroy(hfh_batch *B) {
     if (!B) return HFH_OK;
     (void)hipStreamSynchronize(B->ctx->stream);
+    (void)hipStreamSynchronize(B->cs);
     --B->ctx->live;
     release_all(B);
     delete B;
@@ -764,11 +819,12 @@ extern "C" int hfh_batch_submit_csr(hfh_batch *B, int n, const int *row, const i
     size_t off;
     int rc;
     if ((rc = reserve(B, L.total, &slot, &off))) return rc;
-    HFH_HIP(hipStreamSynchronize(B->ctx->stream));   // the staging buffer may still feed a copy
-    if ((rc = ensure_stage(B, L.upload))) {
+    auto undo = [&](int code) {
         B->arena.release(off, L.total);
-        return rc;
-    }
+        return code;
+    };
+    if (hipEventSynchronize(B->ev_up) != hipSuccess) return undo(hip_fail(hipErrorUnknown, "hfh_batch_submit_csr"));
+    if ((rc = ensure_stage(B, L.upload))) return undo(rc);
     char *h = static_cast<char *>(B->stage);
     std::memcpy(h + L.ptr, sptr.data(), sizeof(int) * sptr.size());
     int *hl = reinterpret_cast<int *>(h + L.len);
@@ -793,13 +849,13 @@ extern "C" int hfh_batch_submit_csr(hfh_batch *B, int n, const int *row, const i
     std::memcpy(h + L.b, b, sizeof(double) * n);
     if (x0) std::memcpy(h + L.x, x0, sizeof(double) * n);
     else std::memset(h + L.x, 0, sizeof(double) * n);
-    hipStream_t s = B->ctx->stream;
-    if (hipMemcpyAsync(B->arena_d + off, h, L.upload, hipMemcpyHostToDevice, s) != hipSuccess) {
-        B->arena.release(off, L.total);
+    if (hipMemcpyAsync(B->arena_d + off, h, L.upload, hipMemcpyHostToDevice, B->cs) != hipSuccess ||
+        hipEventRecord(B->ev_up, B->cs) != hipSuccess) {
         set_error("hfh_batch_submit_csr: upload failed");
-        return HFH_ERR_HIP;
+        return undo(HFH_ERR_HIP);
     }
-    return activate(B, slot, off, L, n, nsl, L.total, id);
+    if ((rc = activate(B, slot, off, L, n, nsl, L.total, id))) return undo(rc);
+    return HFH_OK;
 }
 
 extern "C" int hfh_batch_submit_fv(hfh_batch *B, int nx, int ny, int nz, const double *kcell, const double *b,
@@ -817,38 +873,37 @@ extern "C" int hfh_batch_submit_fv(hfh_batch *B, int nx, int ny, int nz, const d
     int slot;
     size_t off;
     if ((rc = reserve(B, L.total, &slot, &off))) return rc;
-    HFH_HIP(hipStreamSynchronize(B->ctx->stream));
+    auto undo = [&](int code) {
+        B->arena.release(off, L.total);
+        return code;
+    };
     // staging: k (kept in the r64 section until the assembly has run), b, x
     const size_t kbytes = sizeof(double) * n;
-    if ((rc = ensure_stage(B, 3 * up(kbytes)))) {
-        B->arena.release(off, L.total);
-        return rc;
-    }
+    if (hipEventSynchronize(B->ev_up) != hipSuccess) return undo(hip_fail(hipErrorUnknown, "hfh_batch_submit_fv"));
+    if ((rc = ensure_stage(B, 3 * up(kbytes)))) return undo(rc);
     char *h = static_cast<char *>(B->stage);
     std::memcpy(h, kcell, kbytes);
     std::memcpy(h + up(kbytes), b, kbytes);
     if (x0) std::memcpy(h + 2 * up(kbytes), x0, kbytes);
     else std::memset(h + 2 * up(kbytes), 0, kbytes);
-    hipStream_t s = B->ctx->stream;
+    hipStream_t cs = B->cs;
     char *base = B->arena_d + off;
-    if (hipMemcpyAsync(base + L.r64, h, kbytes, hipMemcpyHostToDevice, s) != hipSuccess ||
-        hipMemcpyAsync(base + L.b, h + up(kbytes), kbytes, hipMemcpyHostToDevice, s) != hipSuccess ||
-        hipMemcpyAsync(base + L.x, h + 2 * up(kbytes), kbytes, hipMemcpyHostToDevice, s) != hipSuccess) {
-        B->arena.release(off, L.total);
+    if (hipMemcpyAsync(base + L.r64, h, kbytes, hipMemcpyHostToDevice, cs) != hipSuccess ||
+        hipMemcpyAsync(base + L.b, h + up(kbytes), kbytes, hipMemcpyHostToDevice, cs) != hipSuccess ||
+        hipMemcpyAsync(base + L.x, h + 2 * up(kbytes), kbytes, hipMemcpyHostToDevice, cs) != hipSuccess) {
         set_error("hfh_batch_submit_fv: upload failed");
-        return HFH_ERR_HIP;
+        return undo(HFH_ERR_HIP);
     }
     const int threads = std::max(nsl * 64, nsl + 1);
-    hipLaunchKernelGGL(kb_fv_assemble, dim3((threads + kBlock - 1) / kBlock), dim3(kBlock), 0, s, nx, ny, nz, n, nsl,
+    hipLaunchKernelGGL(kb_fv_assemble, dim3((threads + kBlock - 1) / kBlock), dim3(kBlock), 0, cs, nx, ny, nz, n, nsl,
                        W, reinterpret_cast<const double *>(base + L.r64), reinterpret_cast<int *>(base + L.ptr),
                        reinterpret_cast<int *>(base + L.len), reinterpret_cast<int *>(base + L.col),
                        reinterpret_cast<double *>(base + L.v64), reinterpret_cast<float *>(base + L.v32),
                        reinterpret_cast<float *>(base + L.invd));
-    if ((rc = launch_check("kb_fv_assemble"))) {
-        B->arena.release(off, L.total);
-        return rc;
-    }
-    return activate(B, slot, off, L, n, nsl, L.total, id);
+    if ((rc = launch_check("kb_fv_assemble"))) return undo(rc);
+    if (hipEventRecord(B->ev_up, cs) != hipSuccess) return undo(hip_fail(hipErrorUnknown, "hfh_batch_submit_fv"));
+    if ((rc = activate(B, slot, off, L, n, nsl, L.total, id))) return undo(rc);
+    return HFH_OK;
 }
 
 extern "C" int hfh_batch_step(hfh_batch *B, int *active, int *nfinished) {
@@ -859,23 +914,30 @@ extern "C" int hfh_batch_step(hfh_batch *B, int *active, int *nfinished) {
     int nact = 0;
     for (const auto &S : B->slots)
         if (S.id >= 0 && !S.reported) ++nact;
+    int launched = -1;
     if (nact) {
         HFH_HIP(hipGraphLaunch(B->exec, s));
         ++B->steps;
+        launched = B->rb_next;
+        B->rb_next ^= 1;
+        HFH_HIP(hipMemcpyAsync(B->h_st[launched], B->d_st, sizeof(SysState) * B->slots.size(),
+                               hipMemcpyDeviceToHost, s));
+        HFH_HIP(hipEventRecord(B->ev_rb[launched], s));
     }
-    HFH_HIP(hipMemcpyAsync(B->h_st, B->d_st, sizeof(SysState) * B->slots.size(), hipMemcpyDeviceToHost, s));
-    HFH_HIP(hipStreamSynchronize(s));
+    // read the previous replay's states while this one runs
     int fin = 0;
-    for (size_t k = 0; k < B->slots.size(); ++k) {
-        hfh_batch::Slot &S = B->slots[k];
-        if (S.id < 0 || S.reported) continue;
-        if (B->h_st[k].phase == P_DONE) {
-            S.reported = true;
-            B->finished.push_back(S.id);
-            ++fin;
-        }
+    if (B->rb_pending >= 0) {
+        fin = take_readback(B, B->rb_pending);
+        if (fin < 0) return fin;
+        B->rb_pending = -1;
     }
-    if (active) *active = nact - fin;
+    if (launched >= 0) B->rb_pending = launched;
+    if (active) {
+        int a = 0;
+        for (const auto &S : B->slots)
+            if (S.id >= 0 && !S.reported) ++a;
+        *active = a;
+    }
     if (nfinished) *nfinished = fin;
     return HFH_OK;
 }
@@ -895,10 +957,13 @@ extern "C" int hfh_batch_collect(hfh_batch *B, int id, double *x, hfh_batch_resu
     const int slot = it->second;
     hfh_batch::Slot &S = B->slots[slot];
     HFH_CHECK(S.reported, HFH_ERR_INVALID, "hfh_batch_collect: request %d has not finished", id);
-    hipStream_t s = B->ctx->stream;
-    HFH_HIP(hipStreamSynchronize(s));
-    if (x) HFH_HIP(hipMemcpy(x, S.x, sizeof(double) * S.n, hipMemcpyDeviceToHost));
-    const SysState &st = B->h_st[slot];
+    // a finished system is never written again, so its solution can be read
+    // on the copy stream while later replays run
+    if (x) {
+        HFH_HIP(hipMemcpyAsync(x, S.x, sizeof(double) * S.n, hipMemcpyDeviceToHost, B->cs));
+        HFH_HIP(hipStreamSynchronize(B->cs));
+    }
+    const SysState &st = S.final_state;
     if (res) {
         res->outer_iters = st.outer;
         res->inner_iters = st.inner_total;
@@ -907,17 +972,17 @@ extern "C" int hfh_batch_collect(hfh_batch *B, int id, double *x, hfh_batch_resu
         res->res0 = st.res0;
     }
     const int status = st.status;
-    // free the slot: the state table entry goes back to P_IDLE before reuse
-    SysState idle{};
-    HFH_HIP(hipStreamSynchronize(s));
-    *B->h_init = idle;
-    HFH_HIP(hipMemcpyAsync(B->d_st + slot, B->h_init, sizeof(SysState), hipMemcpyHostToDevice, s));
-    HFH_HIP(hipStreamSynchronize(s));
+    // back to P_IDLE behind the running replays; until then the slot stays
+    // P_DONE, which every kernel skips, so its region can be reused at once
+    hipLaunchKernelGGL(kb_set_slot, dim3(1), dim3(64), 0, B->ctx->stream, B->d_sys, B->d_st, slot, 0, SysDesc{},
+                       SysState{});
+    const int rc = launch_check("kb_set_slot");
     B->arena.release(S.off, S.bytes);
     B->slot_of.erase(it);
     B->finished.erase(std::remove(B->finished.begin(), B->finished.end(), id), B->finished.end());
     S = hfh_batch::Slot();
     B->tiles_dirty = true;
+    if (rc) return rc;
     return status == HFH_SUCCESS_ABS || status == HFH_SUCCESS_REL ? HFH_OK : HFH_ERR_NOT_CONVERGED;
 }
 

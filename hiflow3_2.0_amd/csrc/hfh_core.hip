@@ -464,6 +464,25 @@ extern "C" int hfh_dot(const hfh_vec *x, const hfh_vec *y, double *out) {
     return HFH_OK;
 }
 
+extern "C" int hfh_vec_device_ptr(const hfh_vec *v, void **ptr) {
+    HFH_CHECK(v && ptr, HFH_ERR_INVALID, "hfh_vec_device_ptr: null argument");
+    *ptr = v->d;
+    return HFH_OK;
+}
+
+extern "C" int hfh_dot_dev(const hfh_vec *x, const hfh_vec *y, double *dev_out) {
+    int rc = same_shape(x, y, "hfh_dot_dev");
+    if (rc) return rc;
+    HFH_CHECK(dev_out, HFH_ERR_INVALID, "hfh_dot_dev: null output");
+    hfh_ctx *c = x->ctx;
+    if (x->n == 0) {
+        // no partials: the finisher stores 0
+        hipLaunchKernelGGL(k_fin_store, dim3(1), dim3(kFinBlock), 0, c->stream, (const double *)c->red, 0, dev_out);
+        return launch_check("k_fin_store");
+    }
+    return dot_async(c, x->dtype, x->n, x->d, y->d, dev_out);
+}
+
 extern "C" int hfh_nrm2(const hfh_vec *x, double *out) {
     int rc = hfh_dot(x, x, out);
     if (rc) return rc;

@@ -86,6 +86,8 @@ HIP_API = {
     "hfh_vec_cast": (_i, [_vp, _vp]),
     "hfh_vec_reorder": (_i, [_vp, _vp, _vp]),
     "hfh_dot": (_i, [_vp, _vp, _P(_d)]),
+    "hfh_dot_dev": (_i, [_vp, _vp, _vp]),
+    "hfh_vec_device_ptr": (_i, [_vp, _P(_vp)]),
     "hfh_nrm2": (_i, [_vp, _P(_d)]),
     "hfh_axpy": (_i, [_vp, _vp, _d]),
     "hfh_scaleadd": (_i, [_vp, _d, _vp]),
@@ -314,6 +316,15 @@ class Vector:
         d = C.c_double()
         _ck(hip().hfh_dot(self.h, other.h, C.byref(d)))
         return d.value
+
+    def device_ptr(self):
+        p = C.c_void_p()
+        _ck(hip().hfh_vec_device_ptr(self.h, C.byref(p)))
+        return p.value
+
+    def dot_dev(self, other, dst, index=0):
+        """x.y stored into the FP64 vector dst at index, without a host wait."""
+        _ck(hip().hfh_dot_dev(self.h, other.h, C.c_void_p(dst.device_ptr() + 8 * index)))
 
     def nrm2(self):
         d = C.c_double()

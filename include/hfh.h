@@ -89,6 +89,13 @@ int hfh_vec_reorder(hfh_vec *dst, const hfh_vec *src, const int *perm);
 /* ---- BLAS-1 (CPUsimple_lvector.cc:142-435; lvector.h:262-291) ---- */
 /* Dot: blocking, returns the host scalar (CoupledVector::Dot, coupled_vector.cc:330-344) */
 int hfh_dot(const hfh_vec *x, const hfh_vec *y, double *out);
+/* device address of a vector's data (for interop with other device code;
+ * the handle keeps ownership) */
+int hfh_vec_device_ptr(const hfh_vec *v, void **ptr);
+/* Dot without a host wait: the FP64 result is stored at dev_out (device
+ * memory) in stream order; a later hfh call or hfh_ctx_sync sees it
+ * (the asynchronous form of lVector::Dot, lvector.h:274) */
+int hfh_dot_dev(const hfh_vec *x, const hfh_vec *y, double *dev_out);
 /* Norm2 = sqrt(Dot(self)) (CPUsimple_lvector.cc:142-152) */
 int hfh_nrm2(const hfh_vec *x, double *out);
 /* Axpy: y += a*x (CPUsimple_lvector.cc:234-287) */

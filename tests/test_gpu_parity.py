@@ -145,6 +145,21 @@ def test_blas1(pkg, ctx, oracle, n):
     assert abs(Xf.dot(Yf) - ref) <= 1e-6 * np.dot(np.abs(x), np.abs(y)) + 1e-30
 
 
+@pytest.mark.parametrize("n", [0, 5, 1 << 16])
+def test_dot_dev(pkg, ctx, n):
+    """The device-result Dot stores the same value hfh_dot returns, in stream order."""
+    rng = np.random.default_rng(3)
+    x, y = rng.uniform(-1, 1, n), rng.uniform(-1, 1, n)
+    X, Y = pkg.Vector(ctx, x), pkg.Vector(ctx, y)
+    D = pkg.Vector(ctx, np.full(3, 7.0))
+    X.dot_dev(Y, D, 1)
+    X.dot_dev(X, D, 2)
+    d = D.numpy()
+    assert d[0] == 7.0
+    assert d[1] == (X.dot(Y) if n else 0.0)
+    assert d[2] == (X.dot(X) if n else 0.0)
+
+
 def test_vector_ops(pkg, ctx):
     rng = np.random.default_rng(9)
     x = rng.uniform(-1, 1, 1000)
@@ -241,6 +256,26 @@ def test_ilu0_apply(pkg, ctx, oracle, case):
     Z = pkg.Vector(ctx, len(rhs))
     P.apply(pkg.Vector(ctx, rhs), Z)
     assert relnorm(Z.numpy(), oracle.ilu_solve(oracle.Csr(row, col, lu), rhs)) < 1e-13
+
+
+@pytest.mark.parametrize("prec", ["ilu0", "sgs"])
+def test_level_sweeps_merged_and_single(pkg, ctx, oracle, prec):
+    """Levels of at most 1024 rows run merged in one workgroup, larger levels
+    one launch each; a 40^3 grid has both (level sizes 1 .. 1141)."""
+    row, col, val, rhs = pkg.problem(pkg.UPWIND_CONVDIFF, 40, p0=1e-2, p1=5.0, seed=6)
+    lo, _ = pkg.levels(row, col)
+    sizes = np.diff(lo)
+    assert sizes.max() > 1024 and (sizes <= 1024).sum() > 2
+    if prec == "ilu0":
+        lu = pkg.ilu0_factor(row, col, val)
+        P = pkg.Preconditioner.ilu0(pkg.CsrMatrix(ctx, row, col, lu), row, col)
+        ref = oracle.ilu_solve(oracle.Csr(row, col, lu), rhs)
+    else:
+        P = pkg.Preconditioner.sgs(pkg.CsrMatrix(ctx, row, col, val), row, col)
+        ref = oracle.sgs(oracle.Csr(row, col, val), rhs)
+    Z = pkg.Vector(ctx, len(rhs))
+    P.apply(pkg.Vector(ctx, rhs), Z)
+    assert relnorm(Z.numpy(), ref) < 1e-13
 
 
 # ---------------------------------------------------------------- Krylov drivers
