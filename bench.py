@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Benchmark of the HiFlow3 sparse hot path on one MI355X.
 
-Default workload (configuration 2): plain FP64 CG with the Jacobi
+Headline workload: plain FP64 CG with the Jacobi
 preconditioner on the 3D Q1 trilinear FE Laplacian of a 128^3 hexahedral mesh
 (127^3 = 2,048,383 interior DOFs, 27-point CSR, 54,439,939 nonzeros, int32
 indices), seeded uniform[-1,1] right-hand side, x0 = 0, relative tolerance
@@ -11,13 +11,13 @@ algorithmic bytes per launch (12 nnz + 20 n + 4) over the mean SpMV launch
 time, measured with HIP events on the context stream.
 
 The same line carries the other measured workloads under "workloads":
-  c3  variable-coefficient FV diffusion, 192^3 cells (7-pt, k = exp(g), g ~ N(0,1)):
+  fv192_dc         variable-coefficient FV diffusion, 192^3 cells (7-pt, k = exp(g), g ~ N(0,1)):
       FP64 CG + multicolour SGS against FP32-inner / FP64-outer defect
       correction (FP32 multicolour SGS inner PCG), both to a true relative
       residual of 1e-10; the error of each against an FP64 CG solve to 1e-12
-  c4  upwind convection-diffusion, 128^3 cells, cell Peclet 5:
+  upwind128_gmres  upwind convection-diffusion, 128^3 cells, cell Peclet 5:
       FP64 GMRES(30) + level-scheduled ILU(0), rtol 1e-8
-  c5  serving stream: SPD FV diffusion requests of log-uniform size 1e3..1e6
+  serving          serving stream: SPD FV diffusion requests of log-uniform size 1e3..1e6
       (2D 5-pt or 3D 7-pt), k = exp(N(0, 0.5^2)), FP32-inner / FP64-outer CG
       to 1e-10, Poisson arrivals at 0.8x the measured closed-loop capacity,
       HBM admission budget 200 GB; solves/s and p50 / p99 latency
@@ -94,7 +94,7 @@ def cpu_spmv_baseline(row, col, val, x, budget_s=12.0):
     return y, med, len(times), threads
 
 
-def bench_c2(P, args):
+def bench_pcg_q1(P, args):
     nx = 127
     t0 = time.perf_counter()
     row, col, val, rhs = P.problem(P.Q1_3D, nx, seed=args.seed)
@@ -179,7 +179,7 @@ def bench_c2(P, args):
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded 3D Q1 FE Laplacian, uniform[-1,1] rhs)",
-        "config": {"workload": "cfg2: Jacobi-PCG FP64, 3D Q1 Laplacian 128^3 cells, rtol 1e-8",
+        "config": {"workload": "Jacobi-PCG FP64, 3D Q1 FE Laplacian, 128^3 cells, rtol 1e-8",
                    "n": n, "nnz": nnz, "index": "int32"},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
@@ -199,7 +199,7 @@ def true_rel(P, ctx, A, X, b):
     return float(np.linalg.norm(b - R.numpy()) / np.linalg.norm(b))
 
 
-def bench_c3(P, args):
+def bench_fv192_dc(P, args):
     t0 = time.perf_counter()
     row, col, val, b = P.problem(P.FV_DIFFUSION, 192, p0=1.0, seed=args.seed)
     nc, color, sizes, perm = P.multicoloring(row, col)
@@ -207,7 +207,7 @@ def bench_c3(P, args):
     bp = b[perm].copy()
     del row, col, val, b
     n, nnz = len(bp), len(pcol)
-    log(f"[c3] FV 192^3: n={n} nnz={nnz} colours={nc} setup {time.perf_counter() - t0:.1f}s")
+    log(f"[fv192_dc] FV 192^3: n={n} nnz={nnz} colours={nc} setup {time.perf_counter() - t0:.1f}s")
     ctx = P.Context(0)
     A = P.CsrMatrix(ctx, prow, pcol, pval)
     A32 = P.CsrMatrix.cast_of(A, np.float32)
@@ -220,7 +220,7 @@ def bench_c3(P, args):
     P.pcg(A, B, Xr, P64, maxits=20000, atol=0.0, rtol=1e-12)
     xref = Xr.numpy()
     out = {"n": n, "nnz": nnz, "colours": nc}
-    reps = max(1, args.c3_reps)
+    reps = max(1, args.dc_reps)
     t64, tdc = [], []
     for _ in range(reps + 1):
         X = P.Vector(ctx, n)
@@ -241,11 +241,11 @@ def bench_c3(P, args):
                             "true_rel_residual": true_rel(P, ctx, A, X, bp),
                             "err_vs_fp64_1e-12": float(np.linalg.norm(xdc - xref) / np.linalg.norm(xref))}
     out["speedup_dc_vs_fp64"] = round(out["fp64_cg_mcsgs"]["ms"] / out["dc_fp32_mcsgs"]["ms"], 3)
-    log(f"[c3] {out}")
+    log(f"[fv192_dc] {out}")
     return out
 
 
-def bench_c4(P, args):
+def bench_upwind128_gmres(P, args):
     t0 = time.perf_counter()
     row, col, val, b = P.problem(P.UPWIND_CONVDIFF, 128, p0=1e-2, p1=5.0, seed=args.seed)
     lu = P.ilu0_factor(row, col, val)
@@ -256,7 +256,7 @@ def bench_c4(P, args):
     lo, _ = P.levels(row, col)
     up, _ = P.levels(row, col, upper=True)
     B = P.Vector(ctx, b)
-    log(f"[c4] upwind 128^3: n={len(b)} nnz={len(col)} levels {len(lo) - 1}/{len(up) - 1} "
+    log(f"[upwind128_gmres] upwind 128^3: n={len(b)} nnz={len(col)} levels {len(lo) - 1}/{len(up) - 1} "
         f"setup {time.perf_counter() - t0:.1f}s")
     Z = P.Vector(ctx, len(b))
     Pi.apply(B, Z)
@@ -267,18 +267,18 @@ def bench_c4(P, args):
     ctx.sync()
     apply_ms = (time.perf_counter() - t) / 5 * 1e3
     ts = []
-    for _ in range(max(1, args.c4_reps) + 1):
+    for _ in range(max(1, args.gmres_reps) + 1):
         X = P.Vector(ctx, len(b))
         inf = P.gmres(A, B, X, Pi, m=30, maxits=5000, atol=0.0, rtol=1e-8)
         ts.append(inf["time_ms"])
     out = {"n": len(b), "nnz": len(col), "levels": [len(lo) - 1, len(up) - 1],
            "ms": round(float(np.median(ts[1:])), 2), "iters": inf["iters"], "ilu_apply_ms": round(apply_ms, 3),
            "true_rel_residual": true_rel(P, ctx, A, X, b)}
-    log(f"[c4] {out}")
+    log(f"[upwind128_gmres] {out}")
     return out
 
 
-def c5_shape(P, seed):
+def serving_shape(P, seed):
     """Request size log-uniform in [1e3, 1e6]; 2D 5-pt or 3D 7-pt with p = 1/2."""
     u1 = (P.lmp().hfl_uniform_pm1(seed, 3, 0) + 1.0) / 2.0
     u2 = (P.lmp().hfl_uniform_pm1(seed, 3, 1) + 1.0) / 2.0
@@ -290,10 +290,10 @@ def c5_shape(P, seed):
     return (s, s, s)
 
 
-def c5_stream(P, ctx, nreq, rate, chunk, seed0, budget, max_systems):
+def serve_stream(P, ctx, nreq, rate, chunk, seed0, budget, max_systems):
     """Serve nreq requests arriving as a Poisson process of the given rate
     (None: all at t = 0, closed loop).  Returns (wall s, latencies s, ok)."""
-    shapes = [c5_shape(P, seed0 + i) for i in range(nreq)]
+    shapes = [serving_shape(P, seed0 + i) for i in range(nreq)]
     arr = (np.cumsum(np.random.default_rng(seed0).exponential(1.0 / rate, nreq)) if rate else np.zeros(nreq))
     q = queue.Queue(maxsize=32)
 
@@ -339,23 +339,23 @@ def c5_stream(P, ctx, nreq, rate, chunk, seed0, budget, max_systems):
     return wall, lat, ok, shapes
 
 
-def bench_c5(P, args):
+def bench_serving(P, args):
     ctx = P.Context(0)
     budget = 200 << 30
-    cal = min(256, args.c5_requests)
-    wall, lat, ok, _ = c5_stream(P, ctx, cal, None, args.c5_chunk, 50_000, budget, args.c5_slots)
+    cal = min(256, args.serve_requests)
+    wall, lat, ok, _ = serve_stream(P, ctx, cal, None, args.serve_chunk, 50_000, budget, args.serve_slots)
     cap = cal / wall
-    log(f"[c5] closed loop {cal} requests: {cap:.1f} solves/s")
+    log(f"[serving] closed loop {cal} requests: {cap:.1f} solves/s")
     rate = 0.8 * cap
-    wall, lat, ok, shapes = c5_stream(P, ctx, args.c5_requests, rate, args.c5_chunk, 10_000, budget, args.c5_slots)
+    wall, lat, ok, shapes = serve_stream(P, ctx, args.serve_requests, rate, args.serve_chunk, 10_000, budget, args.serve_slots)
     ns = np.array([a * b * c for a, b, c in shapes])
-    out = {"requests": args.c5_requests, "arrival_rate_per_s": round(rate, 2),
-           "closed_loop_capacity_per_s": round(cap, 2), "solves_per_s": round(args.c5_requests / wall, 2),
+    out = {"requests": args.serve_requests, "arrival_rate_per_s": round(rate, 2),
+           "closed_loop_capacity_per_s": round(cap, 2), "solves_per_s": round(args.serve_requests / wall, 2),
            "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 2),
            "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 2), "converged": int(ok),
-           "mean_n": int(ns.mean()), "chunk": args.c5_chunk, "slots": args.c5_slots, "hbm_budget_GB": 200,
+           "mean_n": int(ns.mean()), "chunk": args.serve_chunk, "slots": args.serve_slots, "hbm_budget_GB": 200,
            "wall_s": round(wall, 2)}
-    log(f"[c5] {out}")
+    log(f"[serving] {out}")
     return out
 
 
@@ -364,13 +364,14 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workloads", default="c3,c4,c5",
-                    help="secondary workloads measured after the headline (comma list of c3,c4,c5, or none)")
-    ap.add_argument("--c3-reps", type=int, default=2)
-    ap.add_argument("--c4-reps", type=int, default=2)
-    ap.add_argument("--c5-requests", type=int, default=4096)
-    ap.add_argument("--c5-chunk", type=int, default=16)
-    ap.add_argument("--c5-slots", type=int, default=512)
+    ap.add_argument("--workloads", default="fv192_dc,upwind128_gmres,serving",
+                    help="workloads measured after the headline: comma list of fv192_dc, upwind128_gmres, "
+                         "serving, or none")
+    ap.add_argument("--dc-reps", type=int, default=2)
+    ap.add_argument("--gmres-reps", type=int, default=2)
+    ap.add_argument("--serve-requests", type=int, default=4096)
+    ap.add_argument("--serve-chunk", type=int, default=16)
+    ap.add_argument("--serve-slots", type=int, default=512)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate", action="store_true",
@@ -382,10 +383,10 @@ def main():
                           "n_gpus": args.gpus}), flush=True)
         sys.exit(2)
     P = load_pkg()
-    out = bench_c2(P, args)
+    out = bench_pcg_q1(P, args)
     wl = {}
     for w in [w for w in args.workloads.split(",") if w and w != "none"]:
-        fn = {"c3": bench_c3, "c4": bench_c4, "c5": bench_c5}[w]
+        fn = {"fv192_dc": bench_fv192_dc, "upwind128_gmres": bench_upwind128_gmres, "serving": bench_serving}[w]
         t0 = time.perf_counter()
         wl[w] = fn(P, args)
         wl[w]["bench_s"] = round(time.perf_counter() - t0, 1)
