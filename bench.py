@@ -193,6 +193,23 @@ def bench_pcg_q1(P, args):
     }
 
 
+def oracle_mod():
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    return O
+
+
+def cpu_iters_sample(run, k, gpu_iters, what):
+    """Time k iterations of the oracle solver (one host thread) and scale to
+    the GPU's iteration count."""
+    t0 = time.perf_counter()
+    run(k)
+    dt = (time.perf_counter() - t0) / k
+    return {"kind": "port", "cores": 1, "ms_per_iter": round(dt * 1e3, 3),
+            "est_solve_ms": round(dt * gpu_iters * 1e3, 1),
+            "sample": f"{what}: {k} iterations on the full system, scaled to {gpu_iters} iterations"}
+
+
 def true_rel(P, ctx, A, X, b):
     R = P.Vector(ctx, len(b))
     A.spmv(X, R)
@@ -241,6 +258,13 @@ def bench_fv192_dc(P, args):
                             "true_rel_residual": true_rel(P, ctx, A, X, bp),
                             "err_vs_fp64_1e-12": float(np.linalg.norm(xdc - xref) / np.linalg.norm(xref))}
     out["speedup_dc_vs_fp64"] = round(out["fp64_cg_mcsgs"]["ms"] / out["dc_fp32_mcsgs"]["ms"], 3)
+    if not args.no_cpu:
+        O = oracle_mod()
+        OA = O.Csr(prow, pcol, pval)
+        OP = O.Prec(O.PREC_MCSGS, OA, sizes=sizes)
+        out["fp64_cg_mcsgs"]["cpu"] = cpu_iters_sample(
+            lambda k: O.cg(OA, bp, P=OP, ctrl=(k, 0.0, 0.0, 1e300)), 5, out["fp64_cg_mcsgs"]["iters"],
+            "oracle FP64 CG + multicolour SGS (CG::SolvePrecond loop order)")
     log(f"[fv192_dc] {out}")
     return out
 
@@ -274,6 +298,12 @@ def bench_upwind128_gmres(P, args):
     out = {"n": len(b), "nnz": len(col), "levels": [len(lo) - 1, len(up) - 1],
            "ms": round(float(np.median(ts[1:])), 2), "iters": inf["iters"], "ilu_apply_ms": round(apply_ms, 3),
            "true_rel_residual": true_rel(P, ctx, A, X, b)}
+    if not args.no_cpu:
+        O = oracle_mod()
+        OA = O.Csr(row, col, val)
+        OP = O.Prec(O.PREC_ILU0, O.Csr(row, col, lu))
+        out["cpu"] = cpu_iters_sample(lambda k: O.gmres(OA, b, P=OP, m=30, ctrl=(k, 0.0, 0.0, 1e300)), 10,
+                                      inf["iters"], "oracle FP64 GMRES(30) + ILU(0) (GMRES::SolveRight loop order)")
     log(f"[upwind128_gmres] {out}")
     return out
 
@@ -355,6 +385,28 @@ def bench_serving(P, args):
            "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 2), "converged": int(ok),
            "mean_n": int(ns.mean()), "chunk": args.serve_chunk, "slots": args.serve_slots, "hbm_budget_GB": 200,
            "wall_s": round(wall, 2)}
+    if not args.no_cpu:
+        # the oracle's MPIR with an FP32 Jacobi-CG inner solver (the engine's
+        # algorithm) on the first requests of the same stream, within a budget
+        O = oracle_mod()
+        t_end = time.perf_counter() + 20.0
+        ts, ns_cpu = [], []
+        for i in range(args.serve_requests):
+            if time.perf_counter() > t_end:
+                break
+            nx, ny, nz = serving_shape(P, 10_000 + i)
+            row, col, val, b = P.problem(P.FV_DIFFUSION, nx, ny, nz, p0=0.5, seed=10_000 + i)
+            A = O.Csr(row, col, val)
+            nb = float(np.linalg.norm(b))
+            t0 = time.perf_counter()
+            O.mpir(A, b, Pl=O.Prec.jacobi(A), max_ecl=30, abs_tol=1e-10 * nb, rel_tol=0.0,
+                   ecl=(20000, 0.0, 1e-4, 1e6))
+            ts.append(time.perf_counter() - t0)
+            ns_cpu.append(nx * ny * nz)
+        out["cpu"] = {"kind": "port", "cores": 1, "requests": len(ts),
+                      "solves_per_s": round(len(ts) / sum(ts), 3) if ts else None,
+                      "sample": f"oracle MPIR (FP32 Jacobi-CG inner) on the first {len(ts)} requests of the "
+                                f"stream (mean n {int(np.mean(ns_cpu)) if ns_cpu else 0}), one after another"}
     log(f"[serving] {out}")
     return out
 
