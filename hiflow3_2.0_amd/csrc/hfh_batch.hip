@@ -139,7 +139,7 @@ __global__ __launch_bounds__(kBlock) void kb_spmv(const SysDesc *__restrict__ sd
     __shared__ double sh[kBlock / 64];
     const int nt = *ntiles;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    for (int t = xcd_remap(blockIdx.x, gridDim.x); t < nt; t += gridDim.x) {
+    for (int t = blockIdx.x; t < nt; t += gridDim.x) {
         const Tile T = tiles[t];
         const int ph = st[T.sys].phase;
         const bool inner = is_inner_spmv(ph);
@@ -227,7 +227,7 @@ __global__ __launch_bounds__(kBlock) void kb_update(const SysDesc *__restrict__ 
                                                     const Tile *__restrict__ tiles, const int *__restrict__ ntiles) {
     __shared__ double sh[kBlock / 64];
     const int nt = *ntiles;
-    for (int t = xcd_remap(blockIdx.x, gridDim.x); t < nt; t += gridDim.x) {
+    for (int t = blockIdx.x; t < nt; t += gridDim.x) {
         const Tile T = tiles[t];
         const int ph = st[T.sys].phase;
         if (ph != P_UPD && ph != P_SETUP) continue;
@@ -297,7 +297,7 @@ __global__ __launch_bounds__(kBlock) void kb_fin2(const SysDesc *__restrict__ sd
 __global__ __launch_bounds__(kBlock) void kb_dir(const SysDesc *__restrict__ sd, const SysState *__restrict__ st,
                                                  const Tile *__restrict__ tiles, const int *__restrict__ ntiles) {
     const int nt = *ntiles;
-    for (int t = xcd_remap(blockIdx.x, gridDim.x); t < nt; t += gridDim.x) {
+    for (int t = blockIdx.x; t < nt; t += gridDim.x) {
         const Tile T = tiles[t];
         const int ph = st[T.sys].phase;
         if (ph != P_GO && ph != P_START && ph != P_CORRECT) continue;

@@ -324,6 +324,10 @@ struct PcgWork {
     KState *st = nullptr;
     double *hist = nullptr;
     int maxits = 0;
+    // search direction kept for a continued solve: p of the last iteration and
+    // the r.z that formed its step
+    bool have_dir = false;
+    double prev_rz = 0;
 
     int init(const hfh_csr *A, hfh_prec *P, int maxits_) {
         hfh_ctx *c = A->ctx;
@@ -351,7 +355,11 @@ struct PcgWork {
     }
 
     // Solves A x = b from the given x (device pointers); fills info.
-    int solve(const T *b, T *x, const hfh_control &ctl, hfh_solve_info *info, double *hhist, bool use_graph) {
+    // x_zero: x holds zeros, so r = b without the SpMV.  keep_direction:
+    // continue the previous solve's search direction, p = z + (r.z / r_k.z_k) p
+    // (the residual replacement of reliable-update CG), instead of p = z.
+    int solve(const T *b, T *x, const hfh_control &ctl, hfh_solve_info *info, double *hhist, bool use_graph,
+              bool x_zero = false, bool keep_direction = false) {
         const hfh_csr *A = R.A;
         hfh_ctx *c = A->ctx;
         hipStream_t s = c->stream;
@@ -363,8 +371,14 @@ struct PcgWork {
         HFH_HIP(hipEventCreate(&e0));
         HFH_HIP(hipEventCreate(&e1));
         HFH_HIP(hipEventRecord(e0, s));
-        int rc = spmv_launch_ex(A, x, R.r, 0, nullptr, nullptr, nullptr);
-        if (!rc) rc = hfh_scaleadd(&vr, -1.0, &vb);
+        int rc;
+        if (x_zero) {
+            HFH_HIP(hipMemcpyAsync(R.r, b, sizeof(T) * n, hipMemcpyDeviceToDevice, s));
+            rc = HFH_OK;
+        } else {
+            rc = spmv_launch_ex(A, x, R.r, 0, nullptr, nullptr, nullptr);
+            if (!rc) rc = hfh_scaleadd(&vr, -1.0, &vb);
+        }
         double rr = 0;
         if (!rc) rc = hfh_dot(&vr, &vr, &rr);
         if (rc) return rc;
@@ -372,15 +386,18 @@ struct PcgWork {
         const double res0 = static_cast<double>(static_cast<T>(sqrt(static_cast<double>(rrT))));
         const int state0 = host_check(0, res0, res0, ctl);
         double beta = rr;
-        if (R.mode == 0) {
-            HFH_HIP(hipMemcpyAsync(R.p, R.r, sizeof(T) * n, hipMemcpyDeviceToDevice, s));
-        } else {
+        const bool cont = keep_direction && have_dir;
+        if (R.mode != 0) {
             HFH_HIP(hipMemsetAsync(R.z, 0, sizeof(T) * n, s));
             if ((rc = prec_apply_raw(R.P, R.r, R.z))) return rc;
-            HFH_HIP(hipMemcpyAsync(R.p, R.z, sizeof(T) * n, hipMemcpyDeviceToDevice, s));
             if ((rc = hfh_dot(&vr, &vz, &beta))) return rc;
         }
-        (void)vp;
+        if (cont) {
+            const T ratio = static_cast<T>(beta) / static_cast<T>(prev_rz);
+            if ((rc = hfh_scaleadd(&vp, static_cast<double>(ratio), R.mode == 0 ? &vr : &vz))) return rc;
+        } else {
+            HFH_HIP(hipMemcpyAsync(R.p, R.mode == 0 ? R.r : R.z, sizeof(T) * n, hipMemcpyDeviceToDevice, s));
+        }
         KState h{};
         h.beta = static_cast<double>(static_cast<T>(beta));
         h.res = res0;
@@ -401,6 +418,10 @@ struct PcgWork {
         (void)hipEventElapsedTime(&ms, e0, e1);
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
+        if (h.iter > 0) {
+            have_dir = true;
+            prev_rz = h.beta;
+        }
         if (hhist && h.iter >= 0)
             HFH_HIP(hipMemcpy(hhist, hist, sizeof(double) * (h.iter + 1), hipMemcpyDeviceToHost));
         if (info) {
@@ -745,9 +766,11 @@ extern "C" int hfh_gmres_solve(const hfh_csr *A, hfh_prec *P, int size_basis, in
     return solve_status(inf.state);
 }
 
-extern "C" int hfh_dc_solve(const hfh_csr *Ah, const hfh_csr *Al, hfh_prec *Pl, const hfh_vec *b, hfh_vec *x,
-                            const hfh_control *outer, const hfh_control *inner, hfh_dc_info *info, double *hhist) {
+extern "C" int hfh_dc_solve_ex(const hfh_csr *Ah, const hfh_csr *Al, hfh_prec *Pl, const hfh_vec *b, hfh_vec *x,
+                               const hfh_control *outer, const hfh_control *inner, const hfh_dc_options *opt,
+                               hfh_dc_info *info, double *hhist) {
     HFH_CHECK(Ah && Al && b && x && outer && inner, HFH_ERR_INVALID, "hfh_dc_solve: null argument");
+    const bool keep = opt && opt->keep_direction;
     HFH_CHECK(Ah->dtype == HFH_F64 && Al->dtype == HFH_F32, HFH_ERR_INVALID,
               "hfh_dc_solve: needs an FP64 outer and an FP32 inner matrix");
     HFH_CHECK(Ah->nrow == Al->nrow && Ah->nnz == Al->nnz && Ah->nrow == Ah->ncol, HFH_ERR_INVALID,
@@ -796,7 +819,7 @@ extern "C" int hfh_dc_solve(const hfh_csr *Ah, const hfh_csr *Al, hfh_prec *Pl, 
         if ((rc = launch_check("k_cast_n"))) return rc;
         HFH_HIP(hipMemsetAsync(cl, 0, sizeof(float) * n, s));
         hfh_solve_info ii{};
-        if ((rc = W.solve(rl, cl, *inner, &ii, nullptr, true))) return rc;
+        if ((rc = W.solve(rl, cl, *inner, &ii, nullptr, true, true, keep))) return rc;
         inner_iters += ii.iters;
         hipLaunchKernelGGL(k_add_correction, dim3(nb), dim3(kBlock), 0, s, n, (double *)x->d, (const float *)cl);
         if ((rc = launch_check("k_add_correction"))) return rc;
@@ -825,4 +848,9 @@ extern "C" int hfh_dc_solve(const hfh_csr *Ah, const hfh_csr *Al, hfh_prec *Pl, 
         info->time_ms = ms;
     }
     return solve_status(state);
+}
+
+extern "C" int hfh_dc_solve(const hfh_csr *Ah, const hfh_csr *Al, hfh_prec *Pl, const hfh_vec *b, hfh_vec *x,
+                            const hfh_control *outer, const hfh_control *inner, hfh_dc_info *info, double *hhist) {
+    return hfh_dc_solve_ex(Ah, Al, Pl, b, x, outer, inner, nullptr, info, hhist);
 }

@@ -236,9 +236,11 @@ __global__ __launch_bounds__(kBlock) void k_spmv_sell(
     __shared__ double shd[kBlock / 64];
     constexpr int WPB = kBlock / 64;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const Split sp = xcd_split(nslice, WPB);
     double dsum = 0;
-    for (int s = sp.first + wave; s < sp.hi; s += sp.step) {
+    // consecutive workgroups take consecutive slices (measured faster on
+    // gfx950 than one contiguous range per XCD: the streams of all XCDs then
+    // stay within one moving window of HBM)
+    for (int s = blockIdx.x * WPB + wave; s < nslice; s += gridDim.x * WPB) {
         const int base = sptr[s];
         const int L = (sptr[s + 1] - base) >> 6;
         const int r = s * 64 + lane;

@@ -114,6 +114,7 @@ HIP_API = {
     "hfh_pcg_solve": (_i, [_vp, _vp, _vp, _vp, _P(Control), _P(SolveInfo), _vp]),
     "hfh_gmres_solve": (_i, [_vp, _vp, _i, _i, _vp, _vp, _P(Control), _P(SolveInfo), _vp]),
     "hfh_dc_solve": (_i, [_vp, _vp, _vp, _vp, _vp, _P(Control), _P(Control), _P(DcInfo), _vp]),
+    "hfh_dc_solve_ex": (_i, [_vp, _vp, _vp, _vp, _vp, _P(Control), _P(Control), _P(C.c_int), _P(DcInfo), _vp]),
     "hfh_batch_create": (_i, [_vp, _P(BatchParams), _P(_vp)]),
     "hfh_batch_destroy": (_i, [_vp]),
     "hfh_batch_bytes_csr": (_i, [_i, _vp, _P(_ll)]),
@@ -479,13 +480,17 @@ def gmres(A, b, x, P=None, m=30, flexible=False, maxits=1000, atol=1e-15, rtol=1
     return out
 
 
-def dc(Ah, Al, b, x, Pl=None, outer=(20, 1e-10, 0.0, 1e6), inner=(1000, 0.0, 1e-4, 1e6), check=True):
-    """FP64 defect correction with FP32 inner PCG (MPIR<D,S>::Solve)."""
+def dc(Ah, Al, b, x, Pl=None, outer=(20, 1e-10, 0.0, 1e6), inner=(1000, 0.0, 1e-4, 1e6), check=True,
+       keep_direction=False):
+    """FP64 defect correction with FP32 inner PCG (MPIR<D,S>::Solve); with
+    keep_direction the inner CG continues its search direction across
+    corrections (reliable updates)."""
     oc, ic = Control(*outer), Control(*inner)
     inf = DcInfo()
     hist = np.zeros(outer[0] + 2)
-    rc = hip().hfh_dc_solve(Ah.h, Al.h, Pl.h if Pl else None, b.h, x.h, C.byref(oc), C.byref(ic), C.byref(inf),
-                            _p(hist))
+    opt = C.c_int(1 if keep_direction else 0)
+    rc = hip().hfh_dc_solve_ex(Ah.h, Al.h, Pl.h if Pl else None, b.h, x.h, C.byref(oc), C.byref(ic), C.byref(opt),
+                               C.byref(inf), _p(hist))
     if rc != OK and (check or rc != ERR_NOT_CONVERGED):
         _ck(rc)
     return {"outer_iters": inf.outer_iters, "inner_iters": inf.inner_iters, "state": inf.state, "res": inf.res,

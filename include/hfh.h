@@ -222,6 +222,22 @@ int hfh_dc_solve(const hfh_csr *Ah, const hfh_csr *Al, hfh_prec *Pl, const hfh_v
                  hfh_vec *x, const hfh_control *outer, const hfh_control *inner,
                  hfh_dc_info *info, double *hist);
 
+/* Options of the defect correction.  keep_direction = 0 is hfh_dc_solve:
+ * every correction starts a fresh inner CG (MPIR, mpir.cc:160-200).
+ * keep_direction = 1: the inner CG keeps its search direction across
+ * corrections.  At each FP64 correction the FP32 residual is replaced by the
+ * cast true residual, and the next direction is p = z + (r.z / r_k.z_k) p
+ * instead of p = z: the reliable-update CG of Sleijpen & van der Vorst (1996),
+ * as used for mixed-precision Krylov solvers by Clark et al. (2010).  No
+ * reference counterpart; the inner control's rtol then sets how often the
+ * FP64 residual is recomputed. */
+typedef struct {
+    int keep_direction;
+} hfh_dc_options;
+int hfh_dc_solve_ex(const hfh_csr *Ah, const hfh_csr *Al, hfh_prec *Pl, const hfh_vec *b,
+                    hfh_vec *x, const hfh_control *outer, const hfh_control *inner,
+                    const hfh_dc_options *opt, hfh_dc_info *info, double *hist);
+
 /* ---- serving: many independent SPD systems solved together ----
  * No reference counterpart: the engine runs, for every admitted system, the
  * defect correction of MPIR<D,S>::Solve (mpir.cc:69-327) with an FP32 inner

@@ -382,6 +382,36 @@ def test_defect_correction_matches_fp64(pkg, ctx, oracle):
     assert relnorm(X.numpy(), xd) < 1e-8
 
 
+@pytest.mark.parametrize("delta", [1e-1, 1e-3])
+def test_defect_correction_keep_direction(pkg, ctx, delta):
+    """Reliable-update form (no reference counterpart): the inner CG keeps its
+    direction across FP64 corrections.  Pinned by what the mathematics fixes:
+    the FP64 true residual meets the tolerance, the solution matches a sparse
+    direct solve, and keeping the direction needs no more inner iterations
+    than restarting at every correction."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    row, col, val, rhs = fv3(pkg, 20, sigma=1.0, seed=5)
+    nc, color, sizes, perm = pkg.multicoloring(row, col)
+    prow, pcol, pval = pkg.csr_permute(row, col, val, perm)
+    b = rhs[perm]
+    Ah = pkg.CsrMatrix(ctx, prow, pcol, pval)
+    Al = pkg.CsrMatrix.cast_of(Ah, np.float32)
+    Pl = pkg.Preconditioner.mcsgs(Al, sizes)
+    nb = np.linalg.norm(b)
+    runs = {}
+    for keep in (False, True):
+        X = pkg.Vector(ctx, len(b))
+        info = pkg.dc(Ah, Al, pkg.Vector(ctx, b), X, Pl, outer=(200, 1e-10 * nb, 0.0, 1e30),
+                      inner=(1000, 0.0, delta, 1e30), keep_direction=keep)
+        assert info["state"] == pkg.SUCCESS_ABS
+        assert true_res(prow, pcol, pval, X.numpy(), b) < 1e-10
+        runs[keep] = (info, X.numpy())
+    assert runs[True][0]["inner_iters"] <= runs[False][0]["inner_iters"]
+    xd = spla.spsolve(sp.csr_matrix((pval, pcol, prow)).tocsc(), b)
+    assert relnorm(runs[True][1], xd) < 1e-8
+
+
 def test_solver_failure_codes(pkg, ctx):
     row, col, val, rhs = fv3(pkg, 10)
     A = pkg.CsrMatrix(ctx, row, col, val)
