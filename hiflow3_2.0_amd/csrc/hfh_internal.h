@@ -75,6 +75,12 @@ struct hfh_csr {
     int *sl_col = nullptr;     // padded columns (padding points at the row itself)
     void *sl_val = nullptr;    // padded values (padding 0)
     bool sell_dirty = false;   // SELL copy must be rebuilt from the CSR arrays
+    // HFH_SPMV_SELL_DICT: columns as one-byte codes into a table of distinct
+    // column offsets (col - row); four codes of consecutive slots per word
+    int *sd_dict = nullptr;    // [sd_ndict] offsets
+    int sd_ndict = 0;
+    unsigned *sd_code = nullptr;   // code words, column-major per slice
+    int *sd_wptr = nullptr;    // [nslice+1] word offset of each slice
     int *diag = nullptr;   // diagonal position per row
     bool has_structure = false;
     std::vector<int> h_row;  // host copy of the row pointer (for analysis)

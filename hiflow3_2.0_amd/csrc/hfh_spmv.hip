@@ -277,6 +277,62 @@ __global__ __launch_bounds__(kBlock) void k_spmv_sell(
     }
 }
 
+// SELL-64 with dictionary-coded columns: column = row + dict[code].  Value
+// slots are padded to a multiple of 4 per slice; code word w of a lane holds
+// the codes of slots 4w..4w+3 (byte t = slot 4w+t).  Row sums as k_spmv_sell.
+template <typename T, bool ADD, bool DOT, int UW>
+__global__ __launch_bounds__(kBlock) void k_spmv_selld(
+    int nrow, int ncol, int nslice, const int *__restrict__ sptr, const int *__restrict__ wptr,
+    const int *__restrict__ rlen, const unsigned *__restrict__ codes, const int *__restrict__ gdict, int ndict,
+    const T *__restrict__ sval, const T *__restrict__ x, T *__restrict__ y, const T *__restrict__ w, double *part,
+    const int *__restrict__ done) {
+    if (done && *done) return;
+    __shared__ int dict[256];
+    __shared__ double shd[kBlock / 64];
+    for (int i = threadIdx.x; i < ndict; i += kBlock) dict[i] = gdict[i];
+    __syncthreads();
+    constexpr int WPB = kBlock / 64;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    double dsum = 0;
+    for (int s = blockIdx.x * WPB + wave; s < nslice; s += gridDim.x * WPB) {
+        const int vbase = sptr[s], W = (sptr[s + 1] - vbase) >> 8;   // code words = slots / 4
+        const int cbase = wptr[s];
+        const int r = s * 64 + lane;
+        const int len = r < nrow ? rlen[r] : 0;
+        T acc = 0;
+        for (int k = 0; k < W; k += UW) {
+            unsigned cw[UW];
+            T v[UW][4], xv[UW][4];
+#pragma unroll
+            for (int u = 0; u < UW; ++u) {
+                const int ww = min(k + u, W - 1);
+                cw[u] = __builtin_nontemporal_load(&codes[cbase + ww * 64 + lane]);
+#pragma unroll
+                for (int t = 0; t < 4; ++t) v[u][t] = __builtin_nontemporal_load(&sval[vbase + (4 * ww + t) * 64 + lane]);
+            }
+#pragma unroll
+            for (int u = 0; u < UW; ++u)
+#pragma unroll
+                for (int t = 0; t < 4; ++t) xv[u][t] = x[min(r + dict[(cw[u] >> (8 * t)) & 255u], ncol - 1)];
+#pragma unroll
+            for (int u = 0; u < UW; ++u)
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+                    if (4 * (k + u) + t < len) acc = add_rn(acc, mul_rn(v[u][t], xv[u][t]));
+        }
+        if (r < nrow) {
+            if (ADD) acc = y[r] + acc;
+            y[r] = acc;
+            if (DOT) dsum += static_cast<double>(acc) * static_cast<double>(w[r]);
+        }
+    }
+    if (DOT) {
+        __syncthreads();
+        const double vv[1] = {block_sum(dsum, shd)};
+        store_partials<1>(vv, part, 0, blockIdx.x);
+    }
+}
+
 // One thread per SELL row slot (nslice*64 of them, including the padding rows
 // of the last slice).  Padding entries get value 0 and a valid column
 // (min(row, ncol-1)), so the kernel can load them without a bounds test.
@@ -293,7 +349,7 @@ __global__ __launch_bounds__(kBlock) void k_csr_to_sell(int nrow, int ncol, int 
     const int pad = min(r, ncol - 1);
     for (int k = 0; k < L; ++k) {
         const int idx = base + k * 64 + lane;
-        scol[idx] = k < len ? col[j0 + k] : pad;
+        if (scol) scol[idx] = k < len ? col[j0 + k] : pad;   // null for the dictionary-coded form
         sval[idx] = k < len ? val[j0 + k] : static_cast<T>(0);
     }
 }
@@ -410,7 +466,39 @@ int spmv_configure(hfh_csr *A, int algo, int p1, int p2) {
     const int n = A->nrow;
     // buffers of the previous configuration may still be read by queued work
     HFH_HIP(hipStreamSynchronize(A->ctx->stream));
-    if (algo == HFH_SPMV_SELL && A->ncol == 0) algo = HFH_SPMV_STREAM;
+    if ((algo == HFH_SPMV_SELL || algo == HFH_SPMV_SELL_DICT) && A->ncol == 0) algo = HFH_SPMV_STREAM;
+    // the dictionary of distinct column offsets (at most 256 codes)
+    std::vector<int> dict;
+    if (algo == HFH_SPMV_SELL_DICT) {
+        std::vector<long long> offs;
+        offs.reserve(1024);
+        offs.push_back(0);
+        bool fits = true;
+        for (int r = 0; r < n && fits; ++r)
+            for (int j = A->h_row[r]; j < A->h_row[r + 1]; ++j) {
+                const long long o = static_cast<long long>(A->h_col[j]) - r;
+                if (std::find(offs.begin(), offs.end(), o) == offs.end()) {
+                    offs.push_back(o);
+                    if (offs.size() > 256) {
+                        fits = false;
+                        break;
+                    }
+                }
+            }
+        if (fits) {
+            std::sort(offs.begin(), offs.end());
+            for (long long o : offs) dict.push_back(static_cast<int>(o));
+        } else {
+            algo = HFH_SPMV_SELL;
+        }
+    }
+    if (A->sd_dict) (void)hipFree(A->sd_dict);
+    if (A->sd_code) (void)hipFree(A->sd_code);
+    if (A->sd_wptr) (void)hipFree(A->sd_wptr);
+    A->sd_dict = nullptr;
+    A->sd_code = nullptr;
+    A->sd_wptr = nullptr;
+    A->sd_ndict = 0;
     const double mean = n ? static_cast<double>(A->nnz) / n : 0.0;
     if (algo == 0) {  // automatic choice
         // SELL-64 unless padding to the longest row of each slice would add
@@ -483,6 +571,69 @@ int spmv_configure(hfh_csr *A, int algo, int p1, int p2) {
         const long long g = p2 > 0 ? std::min<long long>(p2, need) : need;
         A->nwg = round8(std::max<long long>(g, 1));
         A->sell_dirty = true;  // filled from the device CSR by spmv_sell_fill
+    } else if (algo == HFH_SPMV_SELL_DICT) {
+        const int nsl = (n + 63) / 64;
+        std::vector<int> ptr(nsl + 1, 0), wp(nsl + 1, 0), len(std::max(n, 1), 0);
+        long long off = 0;
+        for (int sl = 0; sl < nsl; ++sl) {
+            int L = 0;
+            for (int r = sl * 64; r < std::min(n, sl * 64 + 64); ++r) {
+                len[r] = A->h_row[r + 1] - A->h_row[r];
+                L = std::max(L, len[r]);
+            }
+            const int L4 = (L + 3) / 4 * 4;
+            ptr[sl] = static_cast<int>(off);
+            wp[sl] = static_cast<int>(off / 4);
+            off += 64LL * L4;
+            HFH_CHECK(off < (1LL << 31), HFH_ERR_UNSUPPORTED, "spmv: SELL copy exceeds 2^31 entries");
+        }
+        ptr[nsl] = static_cast<int>(off);
+        wp[nsl] = static_cast<int>(off / 4);
+        // code words: byte t of word w of a lane = code of slot 4w + t
+        std::vector<unsigned> words(std::max(off / 4, 1LL), 0u);
+        const unsigned zero = static_cast<unsigned>(std::lower_bound(dict.begin(), dict.end(), 0) - dict.begin());
+        for (int sl = 0; sl < nsl; ++sl) {
+            const int L4 = (ptr[sl + 1] - ptr[sl]) / 64;
+            for (int lane = 0; lane < 64; ++lane) {
+                const int r = sl * 64 + lane;
+                const int l = r < n ? len[r] : 0, j0 = r < n ? A->h_row[r] : 0;
+                for (int k = 0; k < L4; ++k) {
+                    unsigned code = zero;
+                    if (k < l) {
+                        const int o = A->h_col[j0 + k] - r;
+                        code = static_cast<unsigned>(std::lower_bound(dict.begin(), dict.end(), o) - dict.begin());
+                    }
+                    words[wp[sl] + (k / 4) * 64 + lane] |= code << (8 * (k % 4));
+                }
+            }
+        }
+        if (A->sl_ptr) (void)hipFree(A->sl_ptr);
+        if (A->sl_len) (void)hipFree(A->sl_len);
+        if (A->sl_col) (void)hipFree(A->sl_col);
+        if (A->sl_val) (void)hipFree(A->sl_val);
+        A->sl_ptr = A->sl_len = A->sl_col = nullptr;
+        A->sl_val = nullptr;
+        int rc;
+        if ((rc = dev_alloc(reinterpret_cast<void **>(&A->sl_ptr), sizeof(int) * ptr.size())) ||
+            (rc = dev_alloc(reinterpret_cast<void **>(&A->sl_len), sizeof(int) * len.size())) ||
+            (rc = dev_alloc(&A->sl_val, A->elem() * static_cast<size_t>(std::max(off, 1LL)))) ||
+            (rc = dev_alloc(reinterpret_cast<void **>(&A->sd_wptr), sizeof(int) * wp.size())) ||
+            (rc = dev_alloc(reinterpret_cast<void **>(&A->sd_code), sizeof(unsigned) * words.size())) ||
+            (rc = dev_alloc(reinterpret_cast<void **>(&A->sd_dict), sizeof(int) * dict.size())))
+            return rc;
+        HFH_HIP(hipMemcpy(A->sl_ptr, ptr.data(), sizeof(int) * ptr.size(), hipMemcpyHostToDevice));
+        HFH_HIP(hipMemcpy(A->sl_len, len.data(), sizeof(int) * len.size(), hipMemcpyHostToDevice));
+        HFH_HIP(hipMemcpy(A->sd_wptr, wp.data(), sizeof(int) * wp.size(), hipMemcpyHostToDevice));
+        HFH_HIP(hipMemcpy(A->sd_code, words.data(), sizeof(unsigned) * words.size(), hipMemcpyHostToDevice));
+        HFH_HIP(hipMemcpy(A->sd_dict, dict.data(), sizeof(int) * dict.size(), hipMemcpyHostToDevice));
+        A->sd_ndict = static_cast<int>(dict.size());
+        A->nslice = nsl;
+        A->spmv_algo = HFH_SPMV_SELL_DICT;
+        A->spmv_p1 = p1 == 2 ? 2 : 1;
+        const long long need = (nsl + kBlock / 64 - 1) / (kBlock / 64);
+        const long long g = p2 > 0 ? std::min<long long>(p2, need) : need;
+        A->nwg = round8(std::max<long long>(g, 1));
+        A->sell_dirty = true;  // values filled from the device CSR by spmv_sell_fill
     } else if (algo == HFH_SPMV_WAVE) {
         const int tile = p1 ? p1 : 512;
         HFH_CHECK(tile == 256 || tile == 512 || tile == 1024, HFH_ERR_INVALID,
@@ -526,7 +677,12 @@ static void launch_t(const hfh_csr *A, const void *x, void *y, const void *w, do
     const dim3 g(A->nwg), bl(kBlock);
     const T *xv = (const T *)x, *wv = (const T *)w, *val = (const T *)A->val;
     T *yv = (T *)y;
-    if (A->spmv_algo == HFH_SPMV_SELL) {
+    if (A->spmv_algo == HFH_SPMV_SELL_DICT) {
+        if (A->spmv_p1 == 2)
+            hipLaunchKernelGGL((k_spmv_selld<T, ADD, DOT, 2>), g, bl, 0, s, A->nrow, A->ncol, A->nslice, A->sl_ptr, A->sd_wptr, A->sl_len, A->sd_code, A->sd_dict, A->sd_ndict, (const T *)A->sl_val, xv, yv, wv, part, done);
+        else
+            hipLaunchKernelGGL((k_spmv_selld<T, ADD, DOT, 1>), g, bl, 0, s, A->nrow, A->ncol, A->nslice, A->sl_ptr, A->sd_wptr, A->sl_len, A->sd_code, A->sd_dict, A->sd_ndict, (const T *)A->sl_val, xv, yv, wv, part, done);
+    } else if (A->spmv_algo == HFH_SPMV_SELL) {
         switch (A->spmv_p1) {
         case 2: hipLaunchKernelGGL((k_spmv_sell<T, ADD, DOT, 2>), g, bl, 0, s, A->nrow, A->nslice, A->sl_ptr, A->sl_len, A->sl_col, (const T *)A->sl_val, xv, yv, wv, part, done); break;
         case 8: hipLaunchKernelGGL((k_spmv_sell<T, ADD, DOT, 8>), g, bl, 0, s, A->nrow, A->nslice, A->sl_ptr, A->sl_len, A->sl_col, (const T *)A->sl_val, xv, yv, wv, part, done); break;
@@ -560,7 +716,8 @@ static void launch_t(const hfh_csr *A, const void *x, void *y, const void *w, do
 // Builds the SELL-64 values and columns from the device CSR (after an
 // upload, a cast or a change of algorithm).
 int spmv_sell_fill(hfh_csr *A) {
-    if (A->spmv_algo != HFH_SPMV_SELL || !A->sell_dirty || A->nrow == 0) return HFH_OK;
+    if ((A->spmv_algo != HFH_SPMV_SELL && A->spmv_algo != HFH_SPMV_SELL_DICT) || !A->sell_dirty || A->nrow == 0)
+        return HFH_OK;
     const dim3 g((A->nslice * 64 + kBlock - 1) / kBlock);
     if (A->dtype == HFH_F64)
         hipLaunchKernelGGL(k_csr_to_sell<double>, g, dim3(kBlock), 0, A->ctx->stream, A->nrow, A->ncol, A->nslice,
@@ -637,6 +794,9 @@ extern "C" int hfh_csr_destroy(hfh_csr *A) {
     if (A->sl_len) (void)hipFree(A->sl_len);
     if (A->sl_col) (void)hipFree(A->sl_col);
     if (A->sl_val) (void)hipFree(A->sl_val);
+    if (A->sd_dict) (void)hipFree(A->sd_dict);
+    if (A->sd_code) (void)hipFree(A->sd_code);
+    if (A->sd_wptr) (void)hipFree(A->sd_wptr);
     delete A;
     return HFH_OK;
 }
@@ -775,7 +935,7 @@ extern "C" int hfh_csr_invdiag(const hfh_csr *A, hfh_vec *invd) {
 
 extern "C" int hfh_csr_set_spmv(hfh_csr *A, int algo, int p1, int p2) {
     HFH_CHECK(A, HFH_ERR_INVALID, "hfh_csr_set_spmv: null matrix");
-    HFH_CHECK(algo >= 0 && algo <= HFH_SPMV_SELL && p2 >= 0, HFH_ERR_INVALID, "hfh_csr_set_spmv: bad argument");
+    HFH_CHECK(algo >= 0 && algo <= HFH_SPMV_SELL_DICT && p2 >= 0, HFH_ERR_INVALID, "hfh_csr_set_spmv: bad argument");
     A->spmv_algo_req = algo;
     A->spmv_p1_req = p1;
     A->spmv_p2_req = p2;

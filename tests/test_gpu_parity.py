@@ -24,8 +24,9 @@ def rand_csr(rng, n, ncol, lens):
     return row, col, val
 
 
-ALGOS = [(1, 2048), (1, 1024), (1, 512), (1, 4096), (3, 512), (3, 1024), (3, 256), (2, 0), (4, 4), (4, 2), (4, 8), (4, 16)]
-BITWISE = {1, 3, 4}  # row-block, wave-tile and SELL kernels sum rows in the reference order
+ALGOS = [(1, 2048), (1, 1024), (1, 512), (1, 4096), (3, 512), (3, 1024), (3, 256), (2, 0), (4, 4), (4, 2), (4, 8), (4, 16),
+         (5, 1), (5, 2)]
+BITWISE = {1, 3, 4, 5}  # row-block, wave-tile and SELL kernels sum rows in the reference order
 
 
 def spmv_case(pkg, ctx, oracle, row, col, val, ncol=None, dtype=np.float64, seed=0, algo=(1, 2048)):
@@ -87,7 +88,7 @@ def test_spmv_ragged_and_long_rows(pkg, ctx, oracle, algo):
     row, col, val = rand_csr(rng, n, ncol, lens)
     y, ref, A, X, Y = spmv_case(pkg, ctx, oracle, row, col, val, ncol=ncol, algo=algo)
     assert relnorm(y, ref) < SPMV_TOL
-    if algo[0] == 4:
+    if algo[0] in (4, 5):
         assert np.array_equal(y, ref)   # SELL sums every row sequentially, long rows included
     elif algo[0] in BITWISE:
         short = np.diff(row) <= min(algo[1], 256)
@@ -97,6 +98,31 @@ def test_spmv_ragged_and_long_rows(pkg, ctx, oracle, algo):
     Y.upload(y0)
     A.spmv_add(X, Y)
     assert relnorm(Y.numpy(), oracle.spmv_add(oracle.Csr(row, col, val, ncol), X.numpy(), y0)) < SPMV_TOL
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("words", [1, 2])
+def test_spmv_dictionary_columns(pkg, ctx, oracle, dtype, words):
+    """Dictionary-coded columns on a ragged, rectangular banded matrix with 20
+    distinct offsets (empty rows, rows of 1..20 entries, partial last slice)."""
+    rng = np.random.default_rng(7)
+    n, ncol = 1000, 1100
+    offs = np.sort(rng.choice(np.arange(-300, 400), 20, replace=False))
+    rows, cols = [0], []
+    for r in range(n):
+        k = rng.integers(0, 21) if r % 97 else 0
+        c = np.unique(np.clip(r + rng.choice(offs, k, replace=False), 0, ncol - 1)) if k else np.array([], int)
+        c = c[(c - r >= offs[0]) & np.isin(c - r, offs)]
+        cols.extend(c.tolist())
+        rows.append(len(cols))
+    row, col = np.array(rows, np.int32), np.array(cols, np.int32)
+    val = rng.uniform(-1, 1, len(col))
+    y, ref, A, X, Y = spmv_case(pkg, ctx, oracle, row, col, val, ncol=ncol, dtype=dtype, algo=(5, words))
+    assert np.array_equal(y, ref)
+    y0 = rng.uniform(-1, 1, n).astype(dtype)
+    Y.upload(y0)
+    A.spmv_add(X, Y)
+    assert np.array_equal(Y.numpy(), oracle.spmv_add(oracle.Csr(row, col, val.astype(dtype), ncol), X.numpy(), y0))
 
 
 def test_spmv_single_row_and_rect(pkg, ctx, oracle):
