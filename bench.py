@@ -94,6 +94,14 @@ def cpu_spmv_baseline(row, col, val, x, budget_s=12.0):
     return y, med, len(times), threads
 
 
+def spmv_kernel_name(A):
+    algo, p1 = A.get_spmv()
+    names = {1: f"k_spmv_stream<double, tile {p1}>", 2: f"k_spmv_vector<double, {p1} lanes>",
+             3: f"k_spmv_wave<double, tile {p1}>", 4: f"k_spmv_sell<double, U={p1}>",
+             5: f"k_spmv_selld<double, UW={p1}>", 6: f"k_spmv_sellp<double, U={p1}>"}
+    return names.get(algo, str(algo))
+
+
 def bench_pcg_q1(P, args):
     nx = 127
     t0 = time.perf_counter()
@@ -183,7 +191,7 @@ def bench_pcg_q1(P, args):
                    "n": n, "nnz": nnz, "index": "int32"},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": "k_spmv_sell<double>", "bytes_per_launch": b_spmv,
+                     "kernel": spmv_kernel_name(A), "bytes_per_launch": b_spmv,
                      "avg_launch_us": round(avg_spmv_ms * 1e3, 2), "launches": launches},
         "cpu_baseline": cpu,
         "cg": {"iters": iters, "ms_per_iter": round(ms_iter, 4), "solve_ms": round(ms_step, 3),
@@ -238,26 +246,46 @@ def bench_fv192_dc(P, args):
     xref = Xr.numpy()
     out = {"n": n, "nnz": nnz, "colours": nc}
     reps = max(1, args.dc_reps)
-    t64, tdc = [], []
-    for _ in range(reps + 1):
-        X = P.Vector(ctx, n)
-        inf = P.pcg(A, B, X, P64, maxits=20000, atol=0.0, rtol=1e-10)
-        t64.append(inf["time_ms"])
-    x64 = X.numpy()
-    out["fp64_cg_mcsgs"] = {"ms": round(float(np.median(t64[1:])), 2), "iters": inf["iters"],
-                            "true_rel_residual": true_rel(P, ctx, A, X, bp),
-                            "err_vs_fp64_1e-12": float(np.linalg.norm(x64 - xref) / np.linalg.norm(xref))}
-    for _ in range(reps + 1):
-        X = P.Vector(ctx, n)
-        # inner stop: 1e-4 relative, or the outer target reached (MPIR's ecl abs_tol)
-        d = P.dc(A, A32, B, X, P32, outer=(60, 1e-10 * nb, 0.0, 1e6), inner=(20000, 0.5e-10 * nb, 1e-4, 1e6))
-        tdc.append(d["time_ms"])
-    xdc = X.numpy()
-    out["dc_fp32_mcsgs"] = {"ms": round(float(np.median(tdc[1:])), 2), "outer": d["outer_iters"],
-                            "inner_iters": d["inner_iters"], "inner_rtol": 1e-4, "inner_atol_rel_b": 0.5e-10,
-                            "true_rel_residual": true_rel(P, ctx, A, X, bp),
-                            "err_vs_fp64_1e-12": float(np.linalg.norm(xdc - xref) / np.linalg.norm(xref))}
+    outer = (400, 1e-10 * nb, 0.0, 1e6)
+
+    def fp64_cg():
+        t = []
+        for _ in range(reps + 1):
+            X = P.Vector(ctx, n)
+            inf = P.pcg(A, B, X, P64, maxits=20000, atol=0.0, rtol=1e-10)
+            t.append(inf["time_ms"])
+        x = X.numpy()
+        return {"ms": round(float(np.median(t[1:])), 2), "iters": inf["iters"],
+                "true_rel_residual": true_rel(P, ctx, A, X, bp),
+                "err_vs_fp64_1e-12": float(np.linalg.norm(x - xref) / np.linalg.norm(xref))}
+
+    def dc(keep, inner):
+        t = []
+        for _ in range(reps + 1):
+            X = P.Vector(ctx, n)
+            d = P.dc(A, A32, B, X, P32, outer=outer, inner=inner, keep_direction=keep)
+            t.append(d["time_ms"])
+        x = X.numpy()
+        return {"ms": round(float(np.median(t[1:])), 2), "outer": d["outer_iters"], "inner_iters": d["inner_iters"],
+                "keep_direction": keep, "inner_rtol": inner[2], "inner_atol_rel_b": inner[1] / nb,
+                "true_rel_residual": true_rel(P, ctx, A, X, bp),
+                "err_vs_fp64_1e-12": float(np.linalg.norm(x - xref) / np.linalg.norm(xref))}
+
+    # default SELL-64 operands for both precisions
+    out["fp64_cg_mcsgs"] = fp64_cg()
+    # FP32 inner CG keeps its direction across the FP64 corrections (reliable updates)
+    out["dc_fp32_mcsgs"] = dc(True, (20000, 0.5e-10 * nb, 1e-2, 1e6))
+    # the reference MPIR loop: a fresh inner CG per correction
+    out["dc_fp32_mcsgs_restart"] = dc(False, (20000, 0.5e-10 * nb, 1e-4, 1e6))
+    # both solves with dictionary-coded column indices
+    A.set_spmv(P.SPMV_SELL_DICT, 2, 0)
+    A32.set_spmv(P.SPMV_SELL_DICT, 2, 0)
+    out["dict_columns"] = {"fp64_cg_mcsgs": fp64_cg(), "dc_fp32_mcsgs": dc(True, (20000, 0.5e-10 * nb, 1e-2, 1e6))}
+    out["dict_columns"]["speedup_dc_vs_fp64"] = round(
+        out["dict_columns"]["fp64_cg_mcsgs"]["ms"] / out["dict_columns"]["dc_fp32_mcsgs"]["ms"], 3)
+    A.set_spmv(P.SPMV_AUTO)
     out["speedup_dc_vs_fp64"] = round(out["fp64_cg_mcsgs"]["ms"] / out["dc_fp32_mcsgs"]["ms"], 3)
+    out["speedup_restart_dc_vs_fp64"] = round(out["fp64_cg_mcsgs"]["ms"] / out["dc_fp32_mcsgs_restart"]["ms"], 3)
     if not args.no_cpu:
         O = oracle_mod()
         OA = O.Csr(prow, pcol, pval)
@@ -362,6 +390,8 @@ def serve_stream(P, ctx, nreq, rate, chunk, seed0, budget, max_systems):
                 lat[i] = time.perf_counter() - t0 - arr[i]
                 ok += info["rc"] == 0
                 done += 1
+                if done % 256 == 0:
+                    log(f"[serving] {done}/{nreq} solved after {time.perf_counter() - t0:.1f} s")
         if active == 0 and not pending and nxt < nreq:
             time.sleep(max(0.0, min(0.002, arr[nxt] - (time.perf_counter() - t0))))
     wall = time.perf_counter() - t0
@@ -372,7 +402,7 @@ def serve_stream(P, ctx, nreq, rate, chunk, seed0, budget, max_systems):
 def bench_serving(P, args):
     ctx = P.Context(0)
     budget = 200 << 30
-    cal = min(256, args.serve_requests)
+    cal = min(1024, args.serve_requests)
     wall, lat, ok, _ = serve_stream(P, ctx, cal, None, args.serve_chunk, 50_000, budget, args.serve_slots)
     cap = cal / wall
     log(f"[serving] closed loop {cal} requests: {cap:.1f} solves/s")

@@ -121,6 +121,12 @@ struct hfh_prec {
     std::vector<int> color_off;     // host, ncolors+1
     hfh_levels lo, up;              // SGS / ILU0 level schedules
     hfh_sweep fw, bw;               // MC-SGS: lower-colour / upper-colour entries per row
+    // SGS / ILU0 outside the fused drivers: both sweeps captured once as a
+    // hipGraph on internal in/out buffers (one launch instead of one per level)
+    void *gin = nullptr, *gout = nullptr;
+    hipGraph_t sweep_graph = nullptr;
+    hipGraphExec_t sweep_exec = nullptr;
+    bool graph_failed = false;
 };
 
 namespace hfh {

@@ -193,11 +193,15 @@ struct PcgRun {
     double *hist = nullptr;
     int nb = 1;
 
-    int spmv() {
-        int rc = spmv_launch_ex(A, p, Ap, 0, p, spart, &st->done);
-        if (rc) return rc;
+    // SpMV with fused p.Ap partials; fin_pap sums them
+    int spmv_kernel() { return spmv_launch_ex(A, p, Ap, 0, p, spart, &st->done); }
+    int fin_pap() {
         hipLaunchKernelGGL(k_fin_pap, dim3(1), dim3(kFinBlock), 0, A->ctx->stream, (const double *)spart, A->nwg, st);
         return launch_check("k_fin_pap");
+    }
+    int spmv() {
+        int rc = spmv_kernel();
+        return rc ? rc : fin_pap();
     }
 
     int tail() {
@@ -282,8 +286,9 @@ static int pcg_loop(PcgRun<T> &R, int maxits, bool use_graph) {
                     evs.push_back(a);
                     evs.push_back(b);
                     HFH_HIP(hipEventRecord(a, s));
-                    rc = R.spmv();
+                    rc = R.spmv_kernel();
                     HFH_HIP(hipEventRecord(b, s));
+                    if (!rc) rc = R.fin_pap();
                     if (!rc) rc = R.tail();
                 } else {
                     rc = R.iteration();

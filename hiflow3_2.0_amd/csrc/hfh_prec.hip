@@ -552,6 +552,42 @@ static void free_sweep(hfh_sweep &W) {
     W = hfh_sweep();
 }
 
+// Applies the level-scheduled sweeps through a captured graph of every level
+// launch (built on first use); the sweeps run on the internal buffers gin /
+// gout, with one copy in and one copy out.  Falls back to direct launches if
+// the stream cannot be captured.
+template <typename T>
+static int trsv_graph(hfh_prec *P, const T *in, T *out) {
+    hipStream_t s = P->ctx->stream;
+    const size_t bytes = sizeof(T) * static_cast<size_t>(P->n);
+    if (!P->sweep_exec && !P->graph_failed) {
+        int rc = HFH_OK;
+        if (!P->gin && (rc = dev_alloc(&P->gin, bytes))) return rc;
+        if (!P->gout && (rc = dev_alloc(&P->gout, bytes))) return rc;
+        hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+        bool ok = hipStreamIsCapturing(s, &cap) == hipSuccess && cap == hipStreamCaptureStatusNone &&
+                  hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess;
+        if (ok) {
+            rc = trsv_sweeps<T>(P, static_cast<const T *>(P->gin), static_cast<T *>(P->gout), nullptr);
+            ok = hipStreamEndCapture(s, &P->sweep_graph) == hipSuccess && rc == HFH_OK &&
+                 hipGraphInstantiate(&P->sweep_exec, P->sweep_graph, nullptr, nullptr, 0) == hipSuccess;
+        }
+        if (!ok) {
+            (void)hipGetLastError();
+            if (P->sweep_exec) (void)hipGraphExecDestroy(P->sweep_exec);
+            if (P->sweep_graph) (void)hipGraphDestroy(P->sweep_graph);
+            P->sweep_exec = nullptr;
+            P->sweep_graph = nullptr;
+            P->graph_failed = true;
+        }
+    }
+    if (!P->sweep_exec) return trsv_sweeps<T>(P, in, out, nullptr);
+    HFH_HIP(hipMemcpyAsync(P->gin, in, bytes, hipMemcpyDeviceToDevice, s));
+    HFH_HIP(hipGraphLaunch(P->sweep_exec, s));
+    HFH_HIP(hipMemcpyAsync(out, P->gout, bytes, hipMemcpyDeviceToDevice, s));
+    return HFH_OK;
+}
+
 int prec_apply_raw(hfh_prec *P, const void *in, void *out, const int *done) {
     hipStream_t s = P->ctx->stream;
     const int n = P->n;
@@ -574,6 +610,9 @@ int prec_apply_raw(hfh_prec *P, const void *in, void *out, const int *done) {
                                    : mcsgs_apply<float>(P, (const float *)in, (float *)out, done);
     case HFH_PREC_SGS:
     case HFH_PREC_ILU0:
+        if (!done)
+            return P->dtype == HFH_F64 ? trsv_graph<double>(P, (const double *)in, (double *)out)
+                                       : trsv_graph<float>(P, (const float *)in, (float *)out);
         return P->dtype == HFH_F64 ? trsv_sweeps<double>(P, (const double *)in, (double *)out, done)
                                    : trsv_sweeps<float>(P, (const float *)in, (float *)out, done);
     default:
@@ -710,6 +749,10 @@ extern "C" int hfh_prec_destroy(hfh_prec *P) {
     if (P->up.ptr_d) (void)hipFree(P->up.ptr_d);
     free_sweep(P->fw);
     free_sweep(P->bw);
+    if (P->sweep_exec) (void)hipGraphExecDestroy(P->sweep_exec);
+    if (P->sweep_graph) (void)hipGraphDestroy(P->sweep_graph);
+    if (P->gin) (void)hipFree(P->gin);
+    if (P->gout) (void)hipFree(P->gout);
     delete P;
     return HFH_OK;
 }

@@ -277,6 +277,65 @@ __global__ __launch_bounds__(kBlock) void k_spmv_sell(
     }
 }
 
+template <typename T> struct Pair;
+template <> struct Pair<double> { typedef double type __attribute__((ext_vector_type(2))); };
+template <> struct Pair<float> { typedef float type __attribute__((ext_vector_type(2))); };
+typedef int int_pair __attribute__((ext_vector_type(2)));
+
+// SELL-64 with slot pairs: slots 2q, 2q+1 of a lane at base + q*128 + 2*lane,
+// so each lane loads its two values (and columns) with one 16-byte (8-byte)
+// load.  Row sums as k_spmv_sell (ascending slots, uncontracted).
+template <typename T, bool ADD, bool DOT, int U>
+__global__ __launch_bounds__(kBlock) void k_spmv_sellp(
+    int nrow, int nslice, const int *__restrict__ sptr, const int *__restrict__ rlen, const int *__restrict__ scol,
+    const T *__restrict__ sval, const T *__restrict__ x, T *__restrict__ y, const T *__restrict__ w, double *part,
+    const int *__restrict__ done) {
+    typedef typename Pair<T>::type T2;
+    if (done && *done) return;
+    __shared__ double shd[kBlock / 64];
+    constexpr int WPB = kBlock / 64;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    double dsum = 0;
+    for (int s = blockIdx.x * WPB + wave; s < nslice; s += gridDim.x * WPB) {
+        const int base = sptr[s], Q = (sptr[s + 1] - base) >> 7;   // slot pairs
+        const int r = s * 64 + lane;
+        const int len = r < nrow ? rlen[r] : 0;
+        T acc = 0;
+        for (int q = 0; q < Q; q += U) {
+            int_pair c[U];
+            T2 v[U];
+            T xa[U], xb[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int idx = base + min(q + u, Q - 1) * 128 + 2 * lane;
+                c[u] = __builtin_nontemporal_load(reinterpret_cast<const int_pair *>(&scol[idx]));
+                v[u] = __builtin_nontemporal_load(reinterpret_cast<const T2 *>(&sval[idx]));
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                xa[u] = x[c[u].x];
+                xb[u] = x[c[u].y];
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int k = 2 * (q + u);
+                if (k < len) acc = add_rn(acc, mul_rn(static_cast<T>(v[u].x), xa[u]));
+                if (k + 1 < len) acc = add_rn(acc, mul_rn(static_cast<T>(v[u].y), xb[u]));
+            }
+        }
+        if (r < nrow) {
+            if (ADD) acc = y[r] + acc;
+            y[r] = acc;
+            if (DOT) dsum += static_cast<double>(acc) * static_cast<double>(w[r]);
+        }
+    }
+    if (DOT) {
+        __syncthreads();
+        const double vv[1] = {block_sum(dsum, shd)};
+        store_partials<1>(vv, part, 0, blockIdx.x);
+    }
+}
+
 // SELL-64 with dictionary-coded columns: column = row + dict[code].  Value
 // slots are padded to a multiple of 4 per slice; code word w of a lane holds
 // the codes of slots 4w..4w+3 (byte t = slot 4w+t).  Row sums as k_spmv_sell.
@@ -340,7 +399,7 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_csr_to_sell(int nrow, int ncol, int nslice, const int *__restrict__ row,
                                                         const int *__restrict__ col, const T *__restrict__ val,
                                                         const int *__restrict__ sptr, int *__restrict__ scol,
-                                                        T *__restrict__ sval) {
+                                                        T *__restrict__ sval, int pairs) {
     const int r = blockIdx.x * kBlock + threadIdx.x;
     if (r >= nslice * 64) return;
     const int s = r >> 6, lane = r & 63;
@@ -348,7 +407,7 @@ __global__ __launch_bounds__(kBlock) void k_csr_to_sell(int nrow, int ncol, int 
     const int j0 = r < nrow ? row[r] : 0, len = r < nrow ? row[r + 1] - j0 : 0;
     const int pad = min(r, ncol - 1);
     for (int k = 0; k < L; ++k) {
-        const int idx = base + k * 64 + lane;
+        const int idx = pairs ? base + (k >> 1) * 128 + 2 * lane + (k & 1) : base + k * 64 + lane;
         if (scol) scol[idx] = k < len ? col[j0 + k] : pad;   // null for the dictionary-coded form
         sval[idx] = k < len ? val[j0 + k] : static_cast<T>(0);
     }
@@ -466,7 +525,8 @@ int spmv_configure(hfh_csr *A, int algo, int p1, int p2) {
     const int n = A->nrow;
     // buffers of the previous configuration may still be read by queued work
     HFH_HIP(hipStreamSynchronize(A->ctx->stream));
-    if ((algo == HFH_SPMV_SELL || algo == HFH_SPMV_SELL_DICT) && A->ncol == 0) algo = HFH_SPMV_STREAM;
+    if ((algo == HFH_SPMV_SELL || algo == HFH_SPMV_SELL_DICT || algo == HFH_SPMV_SELL_PAIRS) && A->ncol == 0)
+        algo = HFH_SPMV_STREAM;
     // the dictionary of distinct column offsets (at most 256 codes)
     std::vector<int> dict;
     if (algo == HFH_SPMV_SELL_DICT) {
@@ -511,8 +571,15 @@ int spmv_configure(hfh_csr *A, int algo, int p1, int p2) {
             padded += 64LL * L;
         }
         if (A->ncol > 0 && static_cast<double>(padded) <= 1.25 * static_cast<double>(std::max(A->nnz, 1))) {
-            algo = HFH_SPMV_SELL;
-            p1 = mean >= 16.0 ? 4 : 8;
+            // measured: long FP64 rows run fastest with slot pairs, long FP32
+            // rows with 4 slots in flight, short (stencil) rows with 8
+            if (mean >= 16.0 && A->dtype == HFH_F64) {
+                algo = HFH_SPMV_SELL_PAIRS;
+                p1 = 1;
+            } else {
+                algo = HFH_SPMV_SELL;
+                p1 = mean >= 16.0 ? 4 : 8;
+            }
         } else {
             algo = HFH_SPMV_STREAM;
             p1 = 2048;
@@ -535,7 +602,8 @@ int spmv_configure(hfh_csr *A, int algo, int p1, int p2) {
         A->spmv_algo = HFH_SPMV_STREAM;
         A->spmv_p1 = tile;
         A->nwg = round8(std::max<long long>(g, 1));
-    } else if (algo == HFH_SPMV_SELL) {
+    } else if (algo == HFH_SPMV_SELL || algo == HFH_SPMV_SELL_PAIRS) {
+        const bool pairs = algo == HFH_SPMV_SELL_PAIRS;
         const int nsl = (n + 63) / 64;
         std::vector<int> ptr(nsl + 1, 0), len(std::max(n, 1), 0);
         long long off = 0;
@@ -545,6 +613,7 @@ int spmv_configure(hfh_csr *A, int algo, int p1, int p2) {
                 len[r] = A->h_row[r + 1] - A->h_row[r];
                 L = std::max(L, len[r]);
             }
+            if (pairs) L = (L + 1) / 2 * 2;
             ptr[sl] = static_cast<int>(off);
             off += 64LL * L;
             HFH_CHECK(off < (1LL << 31), HFH_ERR_UNSUPPORTED, "spmv: SELL copy exceeds 2^31 entries");
@@ -565,8 +634,9 @@ int spmv_configure(hfh_csr *A, int algo, int p1, int p2) {
         HFH_HIP(hipMemcpy(A->sl_ptr, ptr.data(), sizeof(int) * ptr.size(), hipMemcpyHostToDevice));
         HFH_HIP(hipMemcpy(A->sl_len, len.data(), sizeof(int) * len.size(), hipMemcpyHostToDevice));
         A->nslice = nsl;
-        A->spmv_algo = HFH_SPMV_SELL;
-        A->spmv_p1 = p1 == 2 || p1 == 8 || p1 == 16 ? p1 : 4;
+        A->spmv_algo = algo;
+        if (pairs) A->spmv_p1 = p1 == 2 ? 2 : 1;
+        else A->spmv_p1 = p1 == 2 || p1 == 8 || p1 == 16 ? p1 : 4;
         const long long need = (nsl + kBlock / 64 - 1) / (kBlock / 64);
         const long long g = p2 > 0 ? std::min<long long>(p2, need) : need;
         A->nwg = round8(std::max<long long>(g, 1));
@@ -677,7 +747,12 @@ static void launch_t(const hfh_csr *A, const void *x, void *y, const void *w, do
     const dim3 g(A->nwg), bl(kBlock);
     const T *xv = (const T *)x, *wv = (const T *)w, *val = (const T *)A->val;
     T *yv = (T *)y;
-    if (A->spmv_algo == HFH_SPMV_SELL_DICT) {
+    if (A->spmv_algo == HFH_SPMV_SELL_PAIRS) {
+        if (A->spmv_p1 == 2)
+            hipLaunchKernelGGL((k_spmv_sellp<T, ADD, DOT, 2>), g, bl, 0, s, A->nrow, A->nslice, A->sl_ptr, A->sl_len, A->sl_col, (const T *)A->sl_val, xv, yv, wv, part, done);
+        else
+            hipLaunchKernelGGL((k_spmv_sellp<T, ADD, DOT, 1>), g, bl, 0, s, A->nrow, A->nslice, A->sl_ptr, A->sl_len, A->sl_col, (const T *)A->sl_val, xv, yv, wv, part, done);
+    } else if (A->spmv_algo == HFH_SPMV_SELL_DICT) {
         if (A->spmv_p1 == 2)
             hipLaunchKernelGGL((k_spmv_selld<T, ADD, DOT, 2>), g, bl, 0, s, A->nrow, A->ncol, A->nslice, A->sl_ptr, A->sd_wptr, A->sl_len, A->sd_code, A->sd_dict, A->sd_ndict, (const T *)A->sl_val, xv, yv, wv, part, done);
         else
@@ -716,15 +791,18 @@ static void launch_t(const hfh_csr *A, const void *x, void *y, const void *w, do
 // Builds the SELL-64 values and columns from the device CSR (after an
 // upload, a cast or a change of algorithm).
 int spmv_sell_fill(hfh_csr *A) {
-    if ((A->spmv_algo != HFH_SPMV_SELL && A->spmv_algo != HFH_SPMV_SELL_DICT) || !A->sell_dirty || A->nrow == 0)
+    if ((A->spmv_algo != HFH_SPMV_SELL && A->spmv_algo != HFH_SPMV_SELL_DICT && A->spmv_algo != HFH_SPMV_SELL_PAIRS) ||
+        !A->sell_dirty || A->nrow == 0)
         return HFH_OK;
+    const int pairs = A->spmv_algo == HFH_SPMV_SELL_PAIRS;
     const dim3 g((A->nslice * 64 + kBlock - 1) / kBlock);
     if (A->dtype == HFH_F64)
         hipLaunchKernelGGL(k_csr_to_sell<double>, g, dim3(kBlock), 0, A->ctx->stream, A->nrow, A->ncol, A->nslice,
-                           A->row, A->col, (const double *)A->val, A->sl_ptr, A->sl_col, (double *)A->sl_val);
+                           A->row, A->col, (const double *)A->val, A->sl_ptr, A->sl_col, (double *)A->sl_val,
+                           pairs);
     else
         hipLaunchKernelGGL(k_csr_to_sell<float>, g, dim3(kBlock), 0, A->ctx->stream, A->nrow, A->ncol, A->nslice,
-                           A->row, A->col, (const float *)A->val, A->sl_ptr, A->sl_col, (float *)A->sl_val);
+                           A->row, A->col, (const float *)A->val, A->sl_ptr, A->sl_col, (float *)A->sl_val, pairs);
     int rc = launch_check("k_csr_to_sell");
     if (rc) return rc;
     A->sell_dirty = false;
@@ -798,6 +876,13 @@ extern "C" int hfh_csr_destroy(hfh_csr *A) {
     if (A->sd_code) (void)hipFree(A->sd_code);
     if (A->sd_wptr) (void)hipFree(A->sd_wptr);
     delete A;
+    return HFH_OK;
+}
+
+extern "C" int hfh_csr_get_spmv(const hfh_csr *A, int *algo, int *param1) {
+    HFH_CHECK(A, HFH_ERR_INVALID, "hfh_csr_get_spmv: null matrix");
+    if (algo) *algo = A->spmv_algo;
+    if (param1) *param1 = A->spmv_p1;
     return HFH_OK;
 }
 
@@ -935,7 +1020,7 @@ extern "C" int hfh_csr_invdiag(const hfh_csr *A, hfh_vec *invd) {
 
 extern "C" int hfh_csr_set_spmv(hfh_csr *A, int algo, int p1, int p2) {
     HFH_CHECK(A, HFH_ERR_INVALID, "hfh_csr_set_spmv: null matrix");
-    HFH_CHECK(algo >= 0 && algo <= HFH_SPMV_SELL_DICT && p2 >= 0, HFH_ERR_INVALID, "hfh_csr_set_spmv: bad argument");
+    HFH_CHECK(algo >= 0 && algo <= HFH_SPMV_SELL_PAIRS && p2 >= 0, HFH_ERR_INVALID, "hfh_csr_set_spmv: bad argument");
     A->spmv_algo_req = algo;
     A->spmv_p1_req = p1;
     A->spmv_p2_req = p2;

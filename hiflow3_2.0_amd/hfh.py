@@ -21,7 +21,7 @@ OK, ERR_INVALID, ERR_NOMEM, ERR_HIP, ERR_ZERO_PIVOT, ERR_NOT_CONVERGED, ERR_BUDG
     0, -1, -2, -3, -4, -5, -6, -7
 
 Q1_2D, Q1_3D, FV_DIFFUSION, UPWIND_CONVDIFF = 1, 2, 3, 4
-SPMV_AUTO, SPMV_STREAM, SPMV_VECTOR, SPMV_WAVE, SPMV_SELL, SPMV_SELL_DICT = 0, 1, 2, 3, 4, 5
+SPMV_AUTO, SPMV_STREAM, SPMV_VECTOR, SPMV_WAVE, SPMV_SELL, SPMV_SELL_DICT, SPMV_SELL_PAIRS = 0, 1, 2, 3, 4, 5, 6
 
 
 class HfhError(RuntimeError):
@@ -104,6 +104,7 @@ HIP_API = {
     "hfh_spmv_add": (_i, [_vp, _vp, _vp]),
     "hfh_csr_invdiag": (_i, [_vp, _vp]),
     "hfh_csr_set_spmv": (_i, [_vp, _i, _i, _i]),
+    "hfh_csr_get_spmv": (_i, [_vp, _P(_i), _P(_i)]),
     "hfh_jacobi_create": (_i, [_vp, _P(_vp)]),
     "hfh_sgs_create": (_i, [_vp, _i, _vp, _vp, _i, _vp, _vp, _P(_vp)]),
     "hfh_mcsgs_create": (_i, [_vp, _i, _vp, _d, _P(_vp)]),
@@ -385,6 +386,12 @@ class CsrMatrix:
 
     def spmv_add(self, x, y):
         _ck(hip().hfh_spmv_add(self.h, x.h, y.h))
+
+    def get_spmv(self):
+        """(algorithm, param1) of the SpMV layout in use."""
+        a, p1 = C.c_int(), C.c_int()
+        _ck(hip().hfh_csr_get_spmv(self.h, C.byref(a), C.byref(p1)))
+        return a.value, p1.value
 
     def set_spmv(self, algo, param1=0, param2=0):
         """Select the SpMV kernel (SPMV_AUTO / SPMV_STREAM / SPMV_VECTOR)."""

@@ -138,7 +138,11 @@ int hfh_spmv_add(const hfh_csr *A, const hfh_vec *x, hfh_vec *y);
  * param2: workgroups (0 = automatic).  algo 0 = automatic.
  * Takes effect at once on an uploaded matrix and on later uploads. */
 enum hfh_spmv_algo { HFH_SPMV_AUTO = 0, HFH_SPMV_STREAM = 1, HFH_SPMV_VECTOR = 2, HFH_SPMV_WAVE = 3,
-                     HFH_SPMV_SELL = 4, HFH_SPMV_SELL_DICT = 5 };
+                     HFH_SPMV_SELL = 4, HFH_SPMV_SELL_DICT = 5, HFH_SPMV_SELL_PAIRS = 6 };
+/* HFH_SPMV_SELL_PAIRS: the SELL-64 copy with the two consecutive column slots
+ * of a row stored side by side (16-byte value loads per lane for FP64); param1:
+ * slot pairs in flight (1 or 2).  Bit-identical like SELL.  The automatic
+ * choice for FP64 matrices with long rows (mean length >= 16). */
 /* HFH_SPMV_SELL_DICT: the SELL-64 copy with its columns stored as one-byte
  * codes into a table of the matrix's distinct column offsets (col - row),
  * four codes of consecutive slots per 32-bit word: 1 index byte per entry
@@ -146,6 +150,8 @@ enum hfh_spmv_algo { HFH_SPMV_AUTO = 0, HFH_SPMV_STREAM = 1, HFH_SPMV_VECTOR = 2
  * Bit-identical like SELL; a matrix with more than 256 distinct offsets gets
  * HFH_SPMV_SELL instead. */
 int hfh_csr_set_spmv(hfh_csr *A, int algo, int param1, int param2);
+/* the SpMV layout in use (after the automatic choice or a fallback) */
+int hfh_csr_get_spmv(const hfh_csr *A, int *algo, int *param1);
 /* extract_invdiagelements over all rows (lmatrix_csr_cpu.cc:2414-2432) */
 int hfh_csr_invdiag(const hfh_csr *A, hfh_vec *invd);
 

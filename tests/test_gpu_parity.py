@@ -25,8 +25,8 @@ def rand_csr(rng, n, ncol, lens):
 
 
 ALGOS = [(1, 2048), (1, 1024), (1, 512), (1, 4096), (3, 512), (3, 1024), (3, 256), (2, 0), (4, 4), (4, 2), (4, 8), (4, 16),
-         (5, 1), (5, 2)]
-BITWISE = {1, 3, 4, 5}  # row-block, wave-tile and SELL kernels sum rows in the reference order
+         (5, 1), (5, 2), (6, 1), (6, 2)]
+BITWISE = {1, 3, 4, 5, 6}  # row-block, wave-tile and SELL kernels sum rows in the reference order
 
 
 def spmv_case(pkg, ctx, oracle, row, col, val, ncol=None, dtype=np.float64, seed=0, algo=(1, 2048)):
@@ -88,7 +88,7 @@ def test_spmv_ragged_and_long_rows(pkg, ctx, oracle, algo):
     row, col, val = rand_csr(rng, n, ncol, lens)
     y, ref, A, X, Y = spmv_case(pkg, ctx, oracle, row, col, val, ncol=ncol, algo=algo)
     assert relnorm(y, ref) < SPMV_TOL
-    if algo[0] in (4, 5):
+    if algo[0] in (4, 5, 6):
         assert np.array_equal(y, ref)   # SELL sums every row sequentially, long rows included
     elif algo[0] in BITWISE:
         short = np.diff(row) <= min(algo[1], 256)
@@ -123,6 +123,27 @@ def test_spmv_dictionary_columns(pkg, ctx, oracle, dtype, words):
     Y.upload(y0)
     A.spmv_add(X, Y)
     assert np.array_equal(Y.numpy(), oracle.spmv_add(oracle.Csr(row, col, val.astype(dtype), ncol), X.numpy(), y0))
+
+
+def test_spmv_layout_choice(pkg, ctx):
+    """The automatic SpMV layout and the dictionary fallback."""
+    row, col, val, _ = pkg.problem(pkg.Q1_3D, 12)
+    assert pkg.CsrMatrix(ctx, row, col, val).get_spmv() == (pkg.SPMV_SELL_PAIRS, 1)      # long FP64 rows
+    assert pkg.CsrMatrix(ctx, row, col, val, dtype=np.float32).get_spmv() == (pkg.SPMV_SELL, 4)
+    row, col, val, _ = pkg.problem(pkg.FV_DIFFUSION, 12)
+    A = pkg.CsrMatrix(ctx, row, col, val)
+    assert A.get_spmv() == (pkg.SPMV_SELL, 8)                                           # stencil rows
+    A.set_spmv(pkg.SPMV_SELL_DICT, 2)
+    assert A.get_spmv() == (pkg.SPMV_SELL_DICT, 2)                                      # 7 distinct offsets
+    rng = np.random.default_rng(5)
+    row, col, val = rand_csr(rng, 400, 3000, rng.integers(0, 30, 400))
+    B = pkg.CsrMatrix(ctx, row, col, val, ncol=3000)
+    B.set_spmv(pkg.SPMV_SELL_DICT, 1)
+    assert B.get_spmv()[0] == pkg.SPMV_SELL                                             # > 256 offsets
+    lens = np.ones(640, int)
+    lens[::64] = 200                                                                    # one long row per slice
+    row, col, val = rand_csr(rng, 640, 1000, lens)
+    assert pkg.CsrMatrix(ctx, row, col, val, ncol=1000).get_spmv()[0] == pkg.SPMV_STREAM  # padding too large
 
 
 def test_spmv_single_row_and_rect(pkg, ctx, oracle):
