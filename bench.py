@@ -363,6 +363,7 @@ def serve_stream(P, ctx, nreq, rate, chunk, seed0, budget, max_systems):
     E = P.BatchEngine(ctx, max_systems=max_systems, hbm_budget=budget, chunk=chunk, rtol=1e-10, atol=0.0,
                       outer_maxits=30, inner_rtol=1e-4, inner_maxits=20000)
     lat = np.zeros(nreq)
+    inner = np.zeros(nreq)
     ids, pending = {}, []
     nxt = done = ok = 0
     cur = None
@@ -388,6 +389,7 @@ def serve_stream(P, ctx, nreq, rate, chunk, seed0, budget, max_systems):
                 i, n = ids.pop(rid)
                 _, info = E.collect(rid, n)
                 lat[i] = time.perf_counter() - t0 - arr[i]
+                inner[i] = info["inner_iters"]
                 ok += info["rc"] == 0
                 done += 1
                 if done % 256 == 0:
@@ -396,25 +398,32 @@ def serve_stream(P, ctx, nreq, rate, chunk, seed0, budget, max_systems):
             time.sleep(max(0.0, min(0.002, arr[nxt] - (time.perf_counter() - t0))))
     wall = time.perf_counter() - t0
     E.close()
-    return wall, lat, ok, shapes
+    return wall, lat, ok, shapes, inner
 
 
 def bench_serving(P, args):
     ctx = P.Context(0)
     budget = 200 << 30
     cal = min(1024, args.serve_requests)
-    wall, lat, ok, _ = serve_stream(P, ctx, cal, None, args.serve_chunk, 50_000, budget, args.serve_slots)
+    wall, lat, ok, _, _ = serve_stream(P, ctx, cal, None, args.serve_chunk, 50_000, budget, args.serve_slots)
     cap = cal / wall
     log(f"[serving] closed loop {cal} requests: {cap:.1f} solves/s")
     rate = 0.8 * cap
-    wall, lat, ok, shapes = serve_stream(P, ctx, args.serve_requests, rate, args.serve_chunk, 10_000, budget, args.serve_slots)
+    wall, lat, ok, shapes, inner = serve_stream(P, ctx, args.serve_requests, rate, args.serve_chunk, 10_000, budget,
+                                                args.serve_slots)
     ns = np.array([a * b * c for a, b, c in shapes])
+    # algorithmic bytes of one inner FP32 iteration per row of a W-entry
+    # stencil: SpMV 8W + 16 (values, columns, length, p, q), update 32, direction 12
+    widths = np.array([5 if c == 1 else 7 for a, b, c in shapes])
+    inner_bytes = float(np.sum(ns * inner * (8 * widths + 60)))
     out = {"requests": args.serve_requests, "arrival_rate_per_s": round(rate, 2),
            "closed_loop_capacity_per_s": round(cap, 2), "solves_per_s": round(args.serve_requests / wall, 2),
            "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 2),
            "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 2), "converged": int(ok),
            "mean_n": int(ns.mean()), "chunk": args.serve_chunk, "slots": args.serve_slots, "hbm_budget_GB": 200,
-           "wall_s": round(wall, 2)}
+           "wall_s": round(wall, 2), "inner_iters_total": int(inner.sum()),
+           "inner_GBps_algorithmic": round(inner_bytes / wall / 1e9, 1),
+           "inner_frac_of_hbm_peak": round(inner_bytes / wall / 1e9 / HBM_PEAK_GBS, 3)}
     if not args.no_cpu:
         # the oracle's MPIR with an FP32 Jacobi-CG inner solver (the engine's
         # algorithm) on the first requests of the same stream, within a budget

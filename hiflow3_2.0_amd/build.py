@@ -29,7 +29,7 @@ HIP_FLAGS = [
     f"-I{INCLUDE}",
 ]
 
-LMP_SOURCES = ["problems.cc", "analysis.cc", "hflmp_capi.cc", "lmp_host.cc", "solve_capi.cc"]
+LMP_SOURCES = ["problems.cc", "analysis.cc", "hflmp_capi.cc", "lmp_host.cc", "solve_capi.cc", "mmio.cc"]
 CXX = os.environ.get("CXX", "g++")
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-fopenmp", "-Wall", "-Wextra", "-Wno-unused-parameter", f"-I{INCLUDE}",
              f"-I{LMP}"]

@@ -272,6 +272,58 @@ __global__ __launch_bounds__(kMergeBlock) void k_trsv_run(const int *__restrict_
     }
 }
 
+// ILU(0) of row i in place (CPU_CSR_lMatrix::ilu0, lmatrix_csr_cpu.cc:1880-1930,
+// IKJ): for each lower entry a_ik (k < i, ascending) with a stored diagonal
+// a_kk: a_ik /= a_kk, then a_ij -= a_ik a_kj for every later entry j of row i
+// that row k also holds.  Rows k are final (earlier levels), and row i is
+// written by its own thread only.  Products and differences are rounded
+// separately, as in the CPU loop.
+template <typename T>
+__device__ __forceinline__ void ilu0_row(int i, const int *__restrict__ row, const int *__restrict__ col, T *val,
+                                         const int *__restrict__ diag) {
+#pragma clang fp contract(off)
+    const int ie = row[i + 1];
+    for (int ik = row[i]; ik < ie; ++ik) {
+        const int k = col[ik];
+        if (k >= i) break;
+        const int dk = diag[k];
+        if (dk < 0) continue;   // missing diagonal: counted on the host, entry left as is
+        const T lik = val[ik] / val[dk];
+        val[ik] = lik;
+        int q = row[k];
+        const int qe = row[k + 1];
+        for (int ij = ik + 1; ij < ie; ++ij) {
+            const int cj = col[ij];
+            while (q < qe && col[q] < cj) ++q;
+            if (q < qe && col[q] == cj) {
+                const T prod = lik * val[q];
+                val[ij] = val[ij] - prod;
+            }
+        }
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_ilu0_level(const int *__restrict__ lrows, int cnt,
+                                                       const int *__restrict__ row, const int *__restrict__ col,
+                                                       T *val, const int *__restrict__ diag) {
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= cnt) return;
+    ilu0_row<T>(lrows[k], row, col, val, diag);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kMergeBlock) void k_ilu0_run(const int *__restrict__ lrows, const int *__restrict__ lptr,
+                                                          int l0, int l1, const int *__restrict__ row,
+                                                          const int *__restrict__ col, T *val,
+                                                          const int *__restrict__ diag) {
+    for (int l = l0; l < l1; ++l) {
+        const int a = lptr[l], cnt = lptr[l + 1] - a;
+        for (int k = threadIdx.x; k < cnt; k += kMergeBlock) ilu0_row<T>(lrows[a + k], row, col, val, diag);
+        __syncthreads();
+    }
+}
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_mc_diag_setup(int n, const int *__restrict__ diag,
                                                           const T *__restrict__ val, T omega_relax,
@@ -754,5 +806,61 @@ extern "C" int hfh_prec_destroy(hfh_prec *P) {
     if (P->gin) (void)hipFree(P->gin);
     if (P->gout) (void)hipFree(P->gout);
     delete P;
+    return HFH_OK;
+}
+
+// ILU(0) factorisation on the device, in place, level by level along the
+// lower level schedule (the rows of one level do not depend on each other).
+extern "C" int hfh_csr_ilu0(hfh_csr *A, int nlev, const int *level_ptr, const int *level_rows, int *missing) {
+    HFH_CHECK(A && A->has_structure && A->nrow == A->ncol, HFH_ERR_INVALID, "hfh_csr_ilu0: bad matrix");
+    hfh_levels L;
+    int rc = upload_levels(L, A->nrow, nlev, level_ptr, level_rows, "hfh_csr_ilu0");
+    if (!rc) rc = validate_levels(A, L, false, "hfh_csr_ilu0");
+    if (rc) {
+        if (L.rows) (void)hipFree(L.rows);
+        if (L.ptr_d) (void)hipFree(L.ptr_d);
+        return rc;
+    }
+    // lower entries whose row k has no stored diagonal are skipped, as on the host
+    int miss = 0;
+    {
+        std::vector<char> hasd(A->nrow, 0);
+        for (int i = 0; i < A->nrow; ++i)
+            for (int j = A->h_row[i]; j < A->h_row[i + 1]; ++j)
+                if (A->h_col[j] == i) hasd[i] = 1;
+        for (int i = 0; i < A->nrow; ++i)
+            for (int j = A->h_row[i]; j < A->h_row[i + 1] && A->h_col[j] < i; ++j)
+                if (!hasd[A->h_col[j]]) ++miss;
+    }
+    hipStream_t s = A->ctx->stream;
+    for (const auto &run : L.runs) {
+        if (run.second - run.first > 1) {
+            if (A->dtype == HFH_F64)
+                hipLaunchKernelGGL(k_ilu0_run<double>, dim3(1), dim3(kMergeBlock), 0, s, L.rows, L.ptr_d, run.first,
+                                   run.second, A->row, A->col, (double *)A->val, A->diag);
+            else
+                hipLaunchKernelGGL(k_ilu0_run<float>, dim3(1), dim3(kMergeBlock), 0, s, L.rows, L.ptr_d, run.first,
+                                   run.second, A->row, A->col, (float *)A->val, A->diag);
+            continue;
+        }
+        const int a = L.ptr[run.first], cnt = L.ptr[run.first + 1] - a;
+        if (cnt == 0) continue;
+        const dim3 g((cnt + kBlock - 1) / kBlock);
+        if (A->dtype == HFH_F64)
+            hipLaunchKernelGGL(k_ilu0_level<double>, g, dim3(kBlock), 0, s, L.rows + a, cnt, A->row, A->col,
+                               (double *)A->val, A->diag);
+        else
+            hipLaunchKernelGGL(k_ilu0_level<float>, g, dim3(kBlock), 0, s, L.rows + a, cnt, A->row, A->col,
+                               (float *)A->val, A->diag);
+    }
+    rc = launch_check("k_ilu0_level");
+    (void)hipStreamSynchronize(s);
+    (void)hipFree(L.rows);
+    (void)hipFree(L.ptr_d);
+    if (rc) return rc;
+    // the SELL copy holds the old values
+    A->sell_dirty = true;
+    if ((rc = spmv_sell_fill(A))) return rc;
+    if (missing) *missing = miss;
     return HFH_OK;
 }

@@ -109,6 +109,7 @@ HIP_API = {
     "hfh_sgs_create": (_i, [_vp, _i, _vp, _vp, _i, _vp, _vp, _P(_vp)]),
     "hfh_mcsgs_create": (_i, [_vp, _i, _vp, _d, _P(_vp)]),
     "hfh_ilu0_create": (_i, [_vp, _i, _vp, _vp, _i, _vp, _vp, _P(_vp)]),
+    "hfh_csr_ilu0": (_i, [_vp, _i, _vp, _vp, _P(_i)]),
     "hfh_prec_apply": (_i, [_vp, _vp, _vp]),
     "hfh_prec_kind": (_i, [_vp, _P(_i)]),
     "hfh_prec_destroy": (_i, [_vp]),
@@ -141,6 +142,10 @@ LMP_API = {
     "hfl_problem_size": (_i, [_i, _i, _i, _i, _d, _d, C.c_ulonglong, _P(_i), _P(_ll)]),
     "hfl_problem_fill": (_i, [_i, _i, _i, _i, _d, _d, C.c_ulonglong, _vp, _vp, _vp, _vp]),
     "hfl_fv_cells": (_i, [_i, _i, _i, _d, C.c_ulonglong, _vp, _vp]),
+    "hfl_mm_read": (_i, [C.c_char_p, _P(_i), _P(_i), _P(_ll), _vp, _vp, _vp]),
+    "hfl_mm_write": (_i, [C.c_char_p, _i, _i, _vp, _vp, _vp]),
+    "hfl_vec_read": (_i, [C.c_char_p, _P(_i), _vp]),
+    "hfl_vec_write": (_i, [C.c_char_p, _i, _vp]),
     "hfl_uniform_pm1": (_d, [C.c_ulonglong, C.c_ulonglong, C.c_ulonglong]),
     "hfl_normal01": (_d, [C.c_ulonglong, C.c_ulonglong, C.c_ulonglong]),
     "hfl_multicoloring": (_i, [_i, _vp, _vp, _P(_i), _vp, _vp, _vp]),
@@ -387,6 +392,16 @@ class CsrMatrix:
     def spmv_add(self, x, y):
         _ck(hip().hfh_spmv_add(self.h, x.h, y.h))
 
+    def download(self):
+        """(row, col, val) of the device matrix."""
+        nr, nc, nz, dt = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        _ck(hip().hfh_csr_dims(self.h, C.byref(nr), C.byref(nc), C.byref(nz), C.byref(dt)))
+        row = np.empty(nr.value + 1, np.int32)
+        col = np.empty(nz.value, np.int32)
+        val = np.empty(nz.value, np.float64 if dt.value == F64 else np.float32)
+        _ck(hip().hfh_csr_download(self.h, _p(row), _p(col), _p(val)))
+        return row, col, val
+
     def get_spmv(self):
         """(algorithm, param1) of the SpMV layout in use."""
         a, p1 = C.c_int(), C.c_int()
@@ -396,6 +411,15 @@ class CsrMatrix:
     def set_spmv(self, algo, param1=0, param2=0):
         """Select the SpMV kernel (SPMV_AUTO / SPMV_STREAM / SPMV_VECTOR)."""
         _ck(hip().hfh_csr_set_spmv(self.h, algo, param1, param2))
+
+    def ilu0(self, row, col):
+        """Factor this matrix in place into ILU(0) factors on the device
+        (row/col: host copy of the pattern, for the level schedule).  Returns
+        the number of lower entries skipped for lack of a diagonal."""
+        ptr, rows = levels(row, col)
+        miss = C.c_int()
+        _ck(hip().hfh_csr_ilu0(self.h, len(ptr) - 1, _p(ptr), _p(rows), C.byref(miss)))
+        return miss.value
 
     def invdiag(self):
         v = Vector(self.ctx, self.nrow, self.dtype)
@@ -613,6 +637,39 @@ def problem(kind, nx, ny=None, nz=None, p0=1.0, p1=0.0, seed=0):
     if lmp().hfl_problem_fill(kind, nx, ny, nz, p0, p1, seed, _p(row), _p(col), _p(val), _p(rhs)):
         raise ValueError("problem generation failed")
     return row, col, val, rhs
+
+
+def mm_read(path):
+    """MatrixMarket file -> (row, col, val, ncol), CSR with ascending columns."""
+    nr, nc, nz = C.c_int(), C.c_int(), C.c_longlong()
+    bpath = os.fsencode(path)
+    _lmp_ck(lmp().hfl_mm_read(bpath, C.byref(nr), C.byref(nc), C.byref(nz), None, None, None))
+    row = np.empty(nr.value + 1, np.int32)
+    col = np.empty(nz.value, np.int32)
+    val = np.empty(nz.value, np.float64)
+    _lmp_ck(lmp().hfl_mm_read(bpath, C.byref(nr), C.byref(nc), C.byref(nz), _p(row), _p(col), _p(val)))
+    return row, col, val, nc.value
+
+
+def mm_write(path, row, col, val, ncol=None):
+    row = np.ascontiguousarray(row, np.int32)
+    n = len(row) - 1
+    _lmp_ck(lmp().hfl_mm_write(os.fsencode(path), n, n if ncol is None else ncol, _p(row),
+                               _p(np.ascontiguousarray(col, np.int32)), _p(np.ascontiguousarray(val, np.float64))))
+
+
+def vec_read(path):
+    n = C.c_int(0)
+    bpath = os.fsencode(path)
+    _lmp_ck(lmp().hfl_vec_read(bpath, C.byref(n), None))
+    out = np.empty(n.value, np.float64)
+    _lmp_ck(lmp().hfl_vec_read(bpath, C.byref(n), _p(out)))
+    return out[: n.value]
+
+
+def vec_write(path, values):
+    values = np.ascontiguousarray(values, np.float64)
+    _lmp_ck(lmp().hfl_vec_write(os.fsencode(path), len(values), _p(values)))
 
 
 def fv_cells(nx, ny, nz, sigma, seed):

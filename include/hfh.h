@@ -179,6 +179,12 @@ int hfh_mcsgs_create(const hfh_csr *A, int ncolors, const int *color_sizes, doub
 int hfh_ilu0_create(const hfh_csr *LU, int nlev, const int *level_ptr, const int *level_rows,
                     int nlev_up, const int *up_level_ptr, const int *up_level_rows,
                     hfh_prec **P);
+/* CPU_CSR_lMatrix::ilu0 (lmatrix_csr_cpu.cc:1880-1930) on the device, in
+ * place: the rows of each level of the lower level schedule (as for
+ * hfh_sgs_create) are factored in parallel with the same per-row arithmetic,
+ * so the factors are bit-identical to the host loop.  *missing: lower entries
+ * skipped for lack of a stored diagonal in their row. */
+int hfh_csr_ilu0(hfh_csr *A, int nlev, const int *level_ptr, const int *level_rows, int *missing);
 /* ApplylPreconditioner (lpreconditioner.h): out = M^{-1} in */
 int hfh_prec_apply(hfh_prec *P, const hfh_vec *in, hfh_vec *out);
 int hfh_prec_kind(const hfh_prec *P, int *kind);

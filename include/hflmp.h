@@ -44,6 +44,21 @@ int hfl_ilu0(int n, const int *row, const int *col, double *val);
 int hfl_csr_permute(int n, const int *row, const int *col, const double *val, const int *perm,
                     int *prow, int *pcol, double *pval);
 
+/* ---- MatrixMarket files (CPU_COO_lMatrix::ReadFile / WriteFile,
+ * lmatrix_coo_cpu.cc:133-309) and text vectors (CPU_lVector::ReadFile /
+ * WriteFile, lvector_cpu.cc:582-660) ----
+ * hfl_mm_read: coordinate real/integer, general or symmetric (mirrored
+ * entries added) into CSR with ascending columns.  With row == NULL only the
+ * sizes are returned (nnz after mirroring); row needs nrow+1 entries, col/val
+ * nnz.  hfl_mm_write: "row col value" triples, 1-based, %24.14e.
+ * hfl_vec_read: *n in = capacity of values (ignored when values is NULL),
+ * out = values read.  hfl_vec_write: std::scientific, 6 digits, as the
+ * reference writes. */
+int hfl_mm_read(const char *path, int *nrow, int *ncol, long long *nnz, int *row, int *col, double *val);
+int hfl_mm_write(const char *path, int nrow, int ncol, const int *row, const int *col, const double *val);
+int hfl_vec_read(const char *path, int *n, double *values);
+int hfl_vec_write(const char *path, int n, const double *values);
+
 /* ---- the LinearSolver templates on the HIP platform ----
  * CG<LAD> (linear_solver/cg.cc:84-248), GMRES<LAD> (gmres.cc:285-510) and
  * FGMRES<LAD> (fgmres.cc:75-262) with PreconditionerBlockJacobiStand

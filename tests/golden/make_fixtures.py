@@ -5,6 +5,9 @@ reference's tests hold and stores them as compact .npz data:
   set_C.npz      test/PETSc/data/set_C/{matrix,rhs,sol,x0}.txt
                  (65x65-node 2D Q1 Poisson system, 4225 rows, 37249 entries)
   equation1.npz  test/LinearAlgebra/data/Equation1.{A,rhs,x0,sol}.txt
+  mm/            the small MatrixMarket files of the reference's tests, as they
+                 are (test/LinearAlgebra/data/Equation1.A.txt,
+                 test/AMG/data/A3.mtx, test/AMG/data/A4.mtx)
 Run:  python tests/golden/make_fixtures.py /root/reference
 """
 import os
@@ -53,5 +56,13 @@ def main(ref):
                         x0=read_vec(os.path.join(d, "Equation1.x0.txt")))
 
 
+def copy_mm(ref):
+    import shutil
+    os.makedirs(os.path.join(HERE, "mm"), exist_ok=True)
+    for src in ("test/LinearAlgebra/data/Equation1.A.txt", "test/AMG/data/A3.mtx", "test/AMG/data/A4.mtx"):
+        shutil.copyfile(os.path.join(ref, src), os.path.join(HERE, "mm", os.path.basename(src)))
+
+
 if __name__ == "__main__":
     main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")
+    copy_mm(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")

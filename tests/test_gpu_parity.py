@@ -305,6 +305,28 @@ def test_ilu0_apply(pkg, ctx, oracle, case):
     assert relnorm(Z.numpy(), oracle.ilu_solve(oracle.Csr(row, col, lu), rhs)) < 1e-13
 
 
+@pytest.mark.parametrize("case", ["upwind40", "fv14"])
+def test_ilu0_factor_on_device_bitwise(pkg, ctx, oracle, case):
+    """Level-scheduled ILU(0) factorisation on the device: the same factors as
+    the sequential host loop, bit for bit."""
+    if case == "upwind40":
+        row, col, val, _ = pkg.problem(pkg.UPWIND_CONVDIFF, 40, p0=1e-2, p1=5.0, seed=6)
+    else:
+        row, col, val, _ = pkg.problem(pkg.FV_DIFFUSION, 14, p0=1.0, seed=2)
+    A = pkg.CsrMatrix(ctx, row, col, val)
+    assert A.ilu0(row, col) == 0
+    r2, c2, lu_dev = A.download()
+    lu_host = pkg.ilu0_factor(row, col, val)
+    assert np.array_equal(lu_dev, lu_host)
+    LU, miss = oracle.ilu0(oracle.Csr(row, col, val))
+    assert miss == 0 and np.array_equal(lu_dev, LU.val)
+    # the SpMV copy follows the new values
+    x = np.random.default_rng(1).uniform(-1, 1, len(row) - 1)
+    Y = pkg.Vector(ctx, len(x))
+    A.spmv(pkg.Vector(ctx, x), Y)
+    assert np.array_equal(Y.numpy(), oracle.spmv(oracle.Csr(row, col, lu_host), x))
+
+
 @pytest.mark.parametrize("prec", ["ilu0", "sgs"])
 def test_level_sweeps_merged_and_single(pkg, ctx, oracle, prec):
     """Levels of at most 1024 rows run merged in one workgroup, larger levels

@@ -192,3 +192,71 @@ def test_rhs_is_seeded_uniform(pkg):
     assert np.array_equal(b1, b2) and not np.array_equal(b1, b3)
     assert b1.min() >= -1 and b1.max() < 1 and abs(b1.mean()) < 0.1
     assert relnorm(b1.std(), 1 / np.sqrt(3)) < 0.1
+
+
+# --------------------------------------------------------------- MatrixMarket and vector files
+
+MM_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "mm")
+
+
+def _scipy_csr(path):
+    import scipy.io
+    S = scipy.io.mmread(path).tocsr()
+    S.sort_indices()
+    return S
+
+
+@pytest.mark.parametrize("name", ["Equation1.A.txt", "A3.mtx", "A4.mtx"])
+def test_mm_read_reference_files(pkg, name):
+    """The reference's own MatrixMarket test files read into sorted CSR; scipy's
+    reader gives the same matrix."""
+    path = os.path.join(MM_DIR, name)
+    row, col, val, ncol = pkg.mm_read(path)
+    S = _scipy_csr(path)
+    assert ncol == S.shape[1] and len(row) - 1 == S.shape[0]
+    assert np.array_equal(row, S.indptr) and np.array_equal(col, S.indices) and np.array_equal(val, S.data)
+    if name == "Equation1.A.txt":
+        d = np.load(os.path.join(os.path.dirname(MM_DIR), "equation1.npz"))
+        assert np.array_equal(sp.csr_matrix((val, col, row)).toarray(),
+                              sp.coo_matrix((d["vals"], (d["rows"], d["cols"]))).toarray())
+
+
+def test_mm_symmetric_and_roundtrip(pkg, tmp_path):
+    p = tmp_path / "sym.mtx"
+    p.write_text("%%MatrixMarket matrix coordinate real symmetric\n% comment\n4 4 5\n1 1 4.0\n2 1 -1.5\n"
+                 "3 2 -2.0\n3 3 5.0\n4 1 0.25\n")
+    row, col, val, ncol = pkg.mm_read(str(p))
+    S = _scipy_csr(str(p))
+    assert np.array_equal(row, S.indptr) and np.array_equal(col, S.indices) and np.array_equal(val, S.data)
+    assert len(val) == 8                                      # three mirrored entries
+    # write -> read of the set_C system keeps every entry (its values have 15 digits)
+    d = np.load(os.path.join(os.path.dirname(MM_DIR), "set_C.npz"))
+    A = sp.coo_matrix((d["vals"], (d["rows"], d["cols"]))).tocsr()
+    A.sort_indices()
+    q = tmp_path / "setC.mtx"
+    pkg.mm_write(str(q), A.indptr, A.indices, A.data)
+    r2, c2, v2, n2 = pkg.mm_read(str(q))
+    assert np.array_equal(r2, A.indptr) and np.array_equal(c2, A.indices) and np.array_equal(v2, A.data)
+    assert q.read_text().splitlines()[2].split() == ["1", "1", "1.00000000000000e+00"]
+
+
+def test_mm_rejects_unsupported(pkg, tmp_path):
+    for banner in ("%%MatrixMarket matrix array real general", "%%MatrixMarket matrix coordinate complex general",
+                   "%%MatrixMarket matrix coordinate pattern general", "%%MatrixMarket vector coordinate real general"):
+        p = tmp_path / "bad.mtx"
+        p.write_text(banner + "\n2 2 1\n1 1 1.0\n")
+        with pytest.raises(pkg.HfhError):
+            pkg.mm_read(str(p))
+    p = tmp_path / "short.mtx"
+    p.write_text("%%MatrixMarket matrix coordinate real general\n2 2 3\n1 1 1.0\n")
+    with pytest.raises(pkg.HfhError):
+        pkg.mm_read(str(p))
+
+
+def test_vector_files(pkg, tmp_path):
+    """CPU_lVector::WriteFile writes std::scientific with 6 digits; ReadFile reads
+    whitespace-separated values."""
+    p = tmp_path / "v.txt"
+    pkg.vec_write(str(p), [1.0, -2.5e-3, 123456.789, 0.0])
+    assert p.read_text() == "1.000000e+00\n-2.500000e-03\n1.234568e+05\n0.000000e+00\n"
+    assert list(pkg.vec_read(str(p))) == [1.0, -2.5e-3, 123456.8, 0.0]
