@@ -136,7 +136,13 @@ int    hfo_gmres_##S(int n, const int *row, const int *col, const T *val, const 
                      hfo_result *res, double *hist);                                     \
 int    hfo_fgmres_##S(int n, const int *row, const int *col, const T *val, const T *b,   \
                       T *x, const hfo_prec_##S *P, int size_basis, const hfo_ctrl *ctrl, \
-                      hfo_result *res, double *hist);
+                      hfo_result *res, double *hist);                                    \
+void   hfo_swap_diag_front_##S(int n, const int *row, int *col, T *val);                  \
+void   hfo_spmv_nodiag_##S(int n, const int *row, const int *col, const T *val,           \
+                           const T *x, T *y);                                             \
+int    hfo_jacobi_##S(int n, const int *row, const int *col, const T *val, const T *b,    \
+                      T *x, int mode, T w, int inner_iter, int smooth_iter,              \
+                      const hfo_ctrl *ctrl, hfo_result *res, double *hist);
 
 HFO_DECL(double, d)
 HFO_DECL(float, f)

@@ -520,6 +520,102 @@ int FN(hfo_fgmres_)(int n, const int *row, const int *col, const T *val, const T
     return FN(gmres_core_)(n, row, col, val, b, x, P, size_basis, ctrl, out, hist, 1);
 }
 
+/* CPU_CSR_lMatrix::SwapDiagElementsToRowFront (lmatrix_csr_cpu.cc:477-495):
+ * the diagonal entry of each row trades places with the row's first entry. */
+void FN(hfo_swap_diag_front_)(int n, const int *row, int *col, T *val)
+{
+    for (int i = 0; i < n; ++i)
+        for (int j = row[i]; j < row[i + 1]; ++j)
+            if (col[j] == i) {
+                const int j0 = row[i];
+                const T tv = val[j0];
+                const int tc = col[j0];
+                val[j0] = val[j];
+                col[j0] = col[j];
+                val[j] = tv;
+                col[j] = tc;
+                break;
+            }
+}
+
+/* CPU_CSR_lMatrix::VectorMultNoDiag (lmatrix_csr_cpu.cc:525-540): the row sum
+ * without the diagonal entry, in stored order. */
+void FN(hfo_spmv_nodiag_)(int n, const int *row, const int *col, const T *val, const T *x, T *y)
+{
+    for (int i = 0; i < n; ++i) {
+        T res = 0;
+        for (int j = row[i]; j < row[i + 1]; ++j)
+            if (col[j] != i)
+                res += val[j] * x[col[j]];
+        y[i] = res;
+    }
+}
+
+/* Jacobi<LAD> (jacobi.cc:120-366) on one rank.  mode 0 SolveNormal, 1
+ * SolveDamped, 2 SmoothNormal, 3 SmoothDamped.  The matrix is the operator
+ * after Prepare (diagonal swapped to the row front).  One step
+ * (DoNormalIteration / DoDampedIteration): y = VectorMultNoDiag(x);
+ * y = -1 y + b; y .*= inv_diag; x = y, or x *= (1 - w); x += w y.  The
+ * solve modes check the true residual ||b - A x|| after every inner_iter
+ * steps (IterateControl, iteration count = steps). */
+int FN(hfo_jacobi_)(int n, const int *row, const int *col, const T *val, const T *b, T *x, int mode, T w,
+                    int inner_iter, int smooth_iter, const hfo_ctrl *ctrl, hfo_result *out, double *hist)
+{
+    T *invd = (T *)calloc(n, sizeof(T)), *y = (T *)calloc(n, sizeof(T)), *r = (T *)calloc(n, sizeof(T));
+    const int damped = mode == 1 || mode == 3;
+    int iter = 0, st = HFO_ITERATE, nh = 0;
+    double first = 0, res = 0;
+    FN(hfo_invdiag_)(n, row, col, val, invd);
+#define JACOBI_STEP()                                                 \
+    do {                                                              \
+        ++iter;                                                       \
+        FN(hfo_spmv_nodiag_)(n, row, col, val, x, y);                 \
+        FN(hfo_scaleadd_)(n, y, (T)-1.0, b);                          \
+        FN(hfo_ewmult_)(n, y, invd);                                  \
+        if (damped) {                                                 \
+            FN(hfo_scale_)(n, x, (T)(1.0 - w));                       \
+            FN(hfo_axpy_)(n, x, y, w);                                \
+        } else {                                                      \
+            memcpy(x, y, sizeof(T) * n);                              \
+        }                                                             \
+    } while (0)
+    if (mode >= 2) {
+        for (int s = 0; s < smooth_iter; ++s)
+            for (int i = 0; i < inner_iter; ++i)
+                JACOBI_STEP();
+        out->iters = iter;
+        out->status = HFO_ITERATE;
+        out->ret = HFO_SOLVER_SUCCESS;
+        out->res = 0;
+        out->res0 = 0;
+    } else {
+        FN(hfo_spmv_)(n, row, col, val, x, r);
+        FN(hfo_scaleadd_)(n, r, (T)-1.0, b);
+        res = (T)sqrt(FN(hfo_dot_)(n, r, r));
+        st = FN(check_)(ctrl, iter, res, &first);
+        if (hist)
+            hist[nh++] = res;
+        while (st == HFO_ITERATE) {
+            for (int i = 0; i < inner_iter; ++i)
+                JACOBI_STEP();
+            FN(hfo_spmv_)(n, row, col, val, x, r);
+            FN(hfo_scaleadd_)(n, r, (T)-1.0, b);
+            res = (T)sqrt(FN(hfo_dot_)(n, r, r));
+            if (hist)
+                hist[nh++] = res;
+            st = FN(check_)(ctrl, iter, res, &first);
+        }
+        out->iters = iter;
+        out->status = st;
+        out->res = res;
+        out->res0 = first;
+        out->ret = FN(retcode_)(st);
+    }
+#undef JACOBI_STEP
+    free(invd); free(y); free(r);
+    return out->ret;
+}
+
 #undef PREC
 #undef FN
 #undef CAT

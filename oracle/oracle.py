@@ -103,6 +103,9 @@ def _declare(L):
             f"hfo_cg_{s}": (i, [i, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
             f"hfo_gmres_{s}": (i, [i, vp, vp, vp, vp, vp, vp, i, vp, vp, vp]),
             f"hfo_fgmres_{s}": (i, [i, vp, vp, vp, vp, vp, vp, i, vp, vp, vp]),
+            f"hfo_swap_diag_front_{s}": (None, [i, vp, vp, vp]),
+            f"hfo_spmv_nodiag_{s}": (None, [i, vp, vp, vp, vp, vp]),
+            f"hfo_jacobi_{s}": (i, [i, vp, vp, vp, vp, vp, i, t, i, i, vp, vp, vp]),
         })
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -364,6 +367,34 @@ def gmres(A, b, x0=None, P=None, m=30, ctrl=(1000, 1e-15, 1e-8, 1e6)):
 
 def fgmres(A, b, x0=None, P=None, m=30, ctrl=(1000, 1e-15, 1e-8, 1e6)):
     return _solve("hfo_fgmres", A, b, x0, P, ctrl, (m,))
+
+
+def swap_diag_front(A):
+    """CPU_CSR_lMatrix::SwapDiagElementsToRowFront on a copy of A."""
+    B = Csr(A.row.copy(), A.col.copy(), A.val.copy(), A.ncol)
+    getattr(lib(), f"hfo_swap_diag_front_{_sfx(B.val.dtype)}")(B.n, _p(B.row), _p(B.col), _p(B.val))
+    return B
+
+
+def spmv_nodiag(A, x):
+    y = np.zeros(A.n, A.val.dtype)
+    getattr(lib(), f"hfo_spmv_nodiag_{_sfx(A.val.dtype)}")(A.n, _p(A.row), _p(A.col), _p(A.val),
+                                                            _p(np.ascontiguousarray(x, A.val.dtype)), _p(y))
+    return y
+
+
+def jacobi(A, b, x0=None, mode=0, w=1.0, inner_iter=1, smooth_iter=1, ctrl=(1000, 1e-15, 1e-8, 1e6)):
+    """Jacobi<LAD> on the operator A after Prepare (diagonal swapped to the row front)."""
+    x = np.array(x0 if x0 is not None else np.zeros(A.n), dtype=A.val.dtype)
+    b = np.ascontiguousarray(b, A.val.dtype)
+    c = Ctrl(*ctrl)
+    r = Result()
+    hist = np.zeros(ctrl[0] + 2, np.float64)
+    getattr(lib(), f"hfo_jacobi_{_sfx(A.val.dtype)}")(A.n, _p(A.row), _p(A.col), _p(A.val), _p(b), _p(x), mode, w,
+                                                       inner_iter, smooth_iter, C.byref(c), C.byref(r), _p(hist))
+    nh = 0 if mode >= 2 else (r.iters // max(inner_iter, 1)) + 1
+    return x, {"iters": r.iters, "status": r.status, "ret": r.ret, "res": r.res, "res0": r.res0,
+               "hist": hist[:nh].copy()}
 
 
 def mpir(A, b, x0=None, Pl=None, Ph=None, max_ech=0, max_ecl=10, abs_tol=1e-12, rel_tol=1e-12,

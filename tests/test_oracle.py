@@ -402,3 +402,50 @@ def test_mpir(oracle, pkg):
     h = info["hist"]
     assert np.all(h[1:] < h[:-1]) and np.all(h[1:] / h[:-1] < 1e-2)
     assert relnorm(x, spla.spsolve(to_scipy(A).tocsc(), b)) < 1e-8
+
+
+# --------------------------------------------------------------- stationary Jacobi (jacobi.cc)
+
+def test_swap_diag_front_and_nodiag(oracle):
+    # SwapDiagElementsToRowFront trades the diagonal with the row's first entry
+    A = oracle.csr_from_coo(3, [0, 0, 0, 1, 1, 2, 2], [0, 1, 2, 0, 1, 1, 2], [4.0, 1.0, 2.0, 3.0, 5.0, 6.0, 7.0])
+    B = oracle.swap_diag_front(A)
+    assert list(B.col) == [0, 1, 2, 1, 0, 2, 1]
+    assert list(B.val) == [4.0, 1.0, 2.0, 5.0, 3.0, 7.0, 6.0]
+    x = np.array([1.0, -2.0, 0.5])
+    D = np.diag(np.diag(A.to_dense()))
+    assert np.array_equal(oracle.spmv_nodiag(B, x), (A.to_dense() - D) @ x)
+    # the stored order is the summation order: with the diagonal at the row
+    # end, the swap moves the row's first entry last
+    C = oracle.csr_from_coo(4, [0, 1, 2, 3, 3, 3, 3], [0, 1, 2, 0, 1, 2, 3], [1.0, 1.0, 1.0, 1.0, 1e16, -1e16, 2.0])
+    Cs = oracle.swap_diag_front(C)
+    assert list(Cs.col[3:]) == [3, 1, 2, 0]
+    z = np.ones(4)
+    assert oracle.spmv_nodiag(C, z)[3] == 0.0      # (1 + 1e16) - 1e16: the 1 is lost
+    assert oracle.spmv_nodiag(Cs, z)[3] == 1.0     # (1e16 - 1e16) + 1
+
+
+def test_jacobi_iteration(oracle, pkg):
+    A, b = fv_matrix(pkg, oracle, 6, 5, 4, sigma=0.5, seed=4)
+    S = oracle.swap_diag_front(A)
+    Ad = A.to_dense()
+    d = np.diag(Ad)
+    x0 = np.random.default_rng(1).uniform(-1, 1, A.n)
+    # one normal step: x = D^-1 (b - (A - D) x0); one damped step: (1-w) x0 + w (that)
+    x1, _ = oracle.jacobi(S, b, x0=x0, mode=2, inner_iter=1, smooth_iter=1)
+    ref = (b - (Ad - np.diag(d)) @ x0) / d
+    assert relnorm(x1, ref) < 1e-15
+    x1d, _ = oracle.jacobi(S, b, x0=x0, mode=3, w=0.7, inner_iter=1, smooth_iter=1)
+    assert relnorm(x1d, 0.3 * x0 + 0.7 * ref) < 1e-15
+    # solve mode: converges to A x = b at the tolerance, counting steps in inner_iter blocks
+    x, info = oracle.jacobi(S, b, mode=0, inner_iter=2, ctrl=(20000, 0.0, 1e-8, 1e6))
+    assert info["status"] == oracle.SUCCESS_REL and info["iters"] % 2 == 0
+    assert np.linalg.norm(b - Ad @ x) < 1e-8 * np.linalg.norm(b)
+    # the residual contracts at the spectral radius of I - D^-1 A
+    rho = np.max(np.abs(np.linalg.eigvals(np.eye(A.n) - Ad / d[:, None])))
+    h = info["hist"]
+    assert h[-1] / h[0] < 50 * rho ** info["iters"]
+    # a diagonal matrix is solved by one step
+    Dm = oracle.csr_from_coo(4, [0, 1, 2, 3], [0, 1, 2, 3], [2.0, 4.0, 8.0, 0.5])
+    xd, inf = oracle.jacobi(oracle.swap_diag_front(Dm), np.ones(4), mode=0, ctrl=(10, 0.0, 1e-12, 1e6))
+    assert inf["iters"] == 1 and list(xd) == [0.5, 0.25, 0.125, 2.0]
