@@ -17,7 +17,9 @@ The same line carries the other measured workloads under "workloads":
       residual of 1e-10; the error of each against an FP64 CG solve to 1e-12
   upwind128_gmres  upwind convection-diffusion, 128^3 cells, cell Peclet 5:
       FP64 GMRES(30) + level-scheduled ILU(0), rtol 1e-8
-  serving          serving stream: SPD FV diffusion requests of log-uniform size 1e3..1e6
+  jacobi_q1        damped Jacobi smoother (w = 2/3) on the headline 3D Q1 matrix after
+      the diagonal-first swap; us per step and GB/s of one fused step
+  serving         serving stream: SPD FV diffusion requests of log-uniform size 1e3..1e6
       (2D 5-pt or 3D 7-pt), k = exp(N(0, 0.5^2)), FP32-inner / FP64-outer CG
       to 1e-10, Poisson arrivals at 0.8x the measured closed-loop capacity,
       HBM admission budget 200 GB; solves/s and p50 / p99 latency
@@ -174,6 +176,17 @@ def bench_pcg_q1(P, args):
     traffic = read_traffic("spmv_q1_3d_128_traffic.txt")
     log(f"[bench] {args.steps} solves: {ms_step:.2f} ms/solve, {iters} iters, {ms_iter * 1e3:.1f} us/iter, "
         f"spmv {avg_spmv_ms * 1e3:.1f} us ({gbs:.0f} GB/s), true rel residual {true_rel:.2e}")
+    ceiling = stream_ceiling(P, ctx)
+    ceiling["spmv_frac_of_best_stream"] = round(gbs / max(ceiling["copy_GBps"], ceiling["triad_GBps"]), 4)
+    # the opt-in dictionary-coded columns (1-byte codes): the same SpMV in
+    # fewer bytes; GB/s on the CSR byte count, reported beside the headline
+    A.set_spmv(P.SPMV_SELL_DICT, 1, 0)
+    dict_us = timed_spmv_us(P, ctx, A, X, P.Vector(ctx, n))
+    A.set_spmv(P.SPMV_AUTO)
+    dict_cols = {"kernel": spmv_kernel_name_for(P.SPMV_SELL_DICT, 1), "avg_launch_us": round(dict_us, 2),
+                 "GBps_on_csr_bytes": round(b_spmv / (dict_us * 1e-6) / 1e9, 1),
+                 "bytes_moved_per_launch": 9 * nnz + 20 * n + 4}
+    log(f"[bench] stream ceiling {ceiling}; dictionary columns {dict_cols}")
     return {
         "metric": "SpMV GB/s (% HBM peak); CG time to 1e-10 rel; served solves/s + p99",
         "value": round(gbs, 1),
@@ -198,7 +211,49 @@ def bench_pcg_q1(P, args):
                "iter_GBps_compulsory": round(it_bytes / (ms_iter * 1e-3) / 1e9, 1),
                "true_rel_residual": true_rel, "state": infos[-1]["state"]},
         "parity": {"spmv_rel_vs_oracle": rel},
+        "stream_ceiling": ceiling,
+        "spmv_dict_columns": dict_cols,
     }
+
+
+def timed_spmv_us(P, ctx, A, X, Y, reps=50):
+    """Mean SpMV launch time (HIP events on the context stream)."""
+    for _ in range(3):
+        A.spmv(X, Y)
+    ctx.set_timing(True)
+    ctx.timing_reset()
+    for _ in range(reps):
+        A.spmv(X, Y)
+    ctx.sync()
+    launches, ms = ctx.timing(0)
+    ctx.set_timing(False)
+    return ms / max(launches, 1) * 1e3
+
+
+def spmv_kernel_name_for(algo, p1):
+    class _L:
+        def get_spmv(self):
+            return algo, p1
+    return spmv_kernel_name(_L())
+
+
+def stream_ceiling(P, ctx, m=1 << 28, reps=10):
+    """Achievable HBM bandwidth of plain streams over 2 GiB vectors: y *= a
+    (one read, one write: copy-like) and y += a x (two reads, one write:
+    triad-like), each the mean of `reps` launches."""
+    x, y = P.Vector(ctx, m), P.Vector(ctx, m)
+    out = {}
+    for name, fn, nbytes in (("copy", lambda: y.scale(1.0), 16 * m), ("triad", lambda: y.axpy(x, 1.0), 24 * m)):
+        fn()
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        ctx.sync()
+        out[f"{name}_GBps"] = round(nbytes * reps / (time.perf_counter() - t0) / 1e9, 1)
+    out["vector_bytes"] = 8 * m
+    del x, y
+    return out
 
 
 def oracle_mod():
@@ -227,13 +282,33 @@ def true_rel(P, ctx, A, X, b):
 def bench_fv192_dc(P, args):
     t0 = time.perf_counter()
     row, col, val, b = P.problem(P.FV_DIFFUSION, 192, p0=1.0, seed=args.seed)
+    t1 = time.perf_counter()
     nc, color, sizes, perm = P.multicoloring(row, col)
+    t2 = time.perf_counter()
     prow, pcol, pval = P.csr_permute(row, col, val, perm)
+    t3 = time.perf_counter()
     bp = b[perm].copy()
-    del row, col, val, b
     n, nnz = len(bp), len(pcol)
-    log(f"[fv192_dc] FV 192^3: n={n} nnz={nnz} colours={nc} setup {time.perf_counter() - t0:.1f}s")
+    log(f"[fv192_dc] FV 192^3: n={n} nnz={nnz} colours={nc} generated {t1 - t0:.1f}s")
     ctx = P.Context(0)
+    # the same setup on the device: first-fit colouring and the permutation
+    A0 = P.CsrMatrix(ctx, row, col, val)
+    A0.multicoloring()                                   # first call loads the kernels
+    ctx.sync()
+    t4 = time.perf_counter()
+    nc_d, _, sizes_d, perm_d = A0.multicoloring()
+    t5 = time.perf_counter()
+    Ad = A0.permuted(perm_d)
+    ctx.sync()
+    t6 = time.perf_counter()
+    _, c_d, v_d = Ad.download()
+    setup = {"host_multicoloring_ms": round((t2 - t1) * 1e3, 1), "host_permute_ms": round((t3 - t2) * 1e3, 1),
+             "host_permute_threads": os.cpu_count(),
+             "device_multicoloring_ms": round((t5 - t4) * 1e3, 1), "device_permute_ms": round((t6 - t5) * 1e3, 1),
+             "device_equals_host": bool(nc_d == nc and np.array_equal(perm_d, perm) and np.array_equal(sizes_d, sizes)
+                                        and np.array_equal(c_d, pcol) and np.array_equal(v_d, pval))}
+    log(f"[fv192_dc] setup {setup}")
+    del A0, Ad, c_d, v_d, row, col, val, b
     A = P.CsrMatrix(ctx, prow, pcol, pval)
     A32 = P.CsrMatrix.cast_of(A, np.float32)
     B = P.Vector(ctx, bp)
@@ -244,7 +319,7 @@ def bench_fv192_dc(P, args):
     Xr = P.Vector(ctx, n)
     P.pcg(A, B, Xr, P64, maxits=20000, atol=0.0, rtol=1e-12)
     xref = Xr.numpy()
-    out = {"n": n, "nnz": nnz, "colours": nc}
+    out = {"n": n, "nnz": nnz, "colours": nc, "setup": setup}
     reps = max(1, args.dc_reps)
     outer = (400, 1e-10 * nb, 0.0, 1e6)
 
@@ -305,8 +380,19 @@ def bench_upwind128_gmres(P, args):
     A = P.CsrMatrix(ctx, row, col, val)
     LU = P.CsrMatrix(ctx, row, col, lu)
     Pi = P.Preconditioner.ilu0(LU, row, col)
-    lo, _ = P.levels(row, col)
+    t1 = time.perf_counter()
+    lo, lo_rows = P.levels(row, col)
     up, _ = P.levels(row, col, upper=True)
+    t2 = time.perf_counter()
+    A.levels()                                           # first call loads the kernels
+    ctx.sync()
+    t3 = time.perf_counter()
+    lo_d, lo_rows_d = A.levels()
+    up_d, _ = A.levels(upper=True)
+    t4 = time.perf_counter()
+    setup = {"host_levels_ms": round((t2 - t1) * 1e3, 1), "device_levels_ms": round((t4 - t3) * 1e3, 1),
+             "device_equals_host": bool(np.array_equal(lo_d, lo) and np.array_equal(lo_rows_d, lo_rows)
+                                        and np.array_equal(up_d, up))}
     B = P.Vector(ctx, b)
     log(f"[upwind128_gmres] upwind 128^3: n={len(b)} nnz={len(col)} levels {len(lo) - 1}/{len(up) - 1} "
         f"setup {time.perf_counter() - t0:.1f}s")
@@ -323,7 +409,7 @@ def bench_upwind128_gmres(P, args):
         X = P.Vector(ctx, len(b))
         inf = P.gmres(A, B, X, Pi, m=30, maxits=5000, atol=0.0, rtol=1e-8)
         ts.append(inf["time_ms"])
-    out = {"n": len(b), "nnz": len(col), "levels": [len(lo) - 1, len(up) - 1],
+    out = {"n": len(b), "nnz": len(col), "levels": [len(lo) - 1, len(up) - 1], "setup": setup,
            "ms": round(float(np.median(ts[1:])), 2), "iters": inf["iters"], "ilu_apply_ms": round(apply_ms, 3),
            "true_rel_residual": true_rel(P, ctx, A, X, b)}
     if not args.no_cpu:
@@ -333,6 +419,43 @@ def bench_upwind128_gmres(P, args):
         out["cpu"] = cpu_iters_sample(lambda k: O.gmres(OA, b, P=OP, m=30, ctrl=(k, 0.0, 0.0, 1e300)), 10,
                                       inf["iters"], "oracle FP64 GMRES(30) + ILU(0) (GMRES::SolveRight loop order)")
     log(f"[upwind128_gmres] {out}")
+    return out
+
+
+def bench_jacobi_q1(P, args):
+    """Damped Jacobi smoother (Jacobi<LAD> SmoothDamped, w = 2/3) on the
+    headline 3D Q1 matrix after the diagonal-first swap.  One step streams the
+    matrix once and reads x, b, inv_diag and writes the new x: algorithmic
+    bytes 12 nnz + 4 (n + 1) + 32 n per step (CSR-equivalent)."""
+    row, col, val, b = P.problem(P.Q1_3D, 127, seed=args.seed)
+    n, nnz = len(row) - 1, len(col)
+    ctx = P.Context(0)
+    A = P.CsrMatrix(ctx, row, col, val)
+    A.swap_diag_front()
+    B = P.Vector(ctx, b)
+    X = P.Vector(ctx, n)
+    w = 2.0 / 3.0
+    P.jacobi(A, B, X, mode=P.JACOBI_SMOOTH_DAMPED, w=w, inner_iter=2)    # warm-up
+    t_short, t_long = [], []
+    for _ in range(3):
+        t_short.append(P.jacobi(A, B, X, mode=P.JACOBI_SMOOTH_DAMPED, w=w, inner_iter=10)["time_ms"])
+        t_long.append(P.jacobi(A, B, X, mode=P.JACOBI_SMOOTH_DAMPED, w=w, inner_iter=60)["time_ms"])
+    step_ms = (float(np.median(t_long)) - float(np.median(t_short))) / 50
+    byts = 12 * nnz + 4 * (n + 1) + 32 * n
+    gbs = byts / (step_ms * 1e-3) / 1e9
+    out = {"n": n, "nnz": nnz, "layout": spmv_kernel_name(A), "us_per_step": round(step_ms * 1e3, 2),
+           "GBps": round(gbs, 1), "frac_hbm_peak": round(gbs / HBM_PEAK_GBS, 4), "bytes_per_step": byts}
+    if not args.no_cpu:
+        O = oracle_mod()
+        S = O.swap_diag_front(O.Csr(row, col, val))
+        k = 2
+        t0 = time.perf_counter()
+        O.jacobi(S, b, None, mode=3, w=w, inner_iter=k)
+        dt = (time.perf_counter() - t0) / k
+        out["cpu"] = {"kind": "port", "cores": 1, "ms_per_step": round(dt * 1e3, 2),
+                      "GBps": round(byts / dt / 1e9, 2),
+                      "sample": f"oracle damped Jacobi (DoDampedIteration order): {k} steps on the full system"}
+    log(f"[jacobi_q1] {out}")
     return out
 
 
@@ -455,8 +578,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workloads", default="fv192_dc,upwind128_gmres,serving",
-                    help="workloads measured after the headline: comma list of fv192_dc, upwind128_gmres, "
+    ap.add_argument("--workloads", default="fv192_dc,upwind128_gmres,jacobi_q1,serving",
+                    help="workloads measured after the headline: comma list of fv192_dc, upwind128_gmres, jacobi_q1, "
                          "serving, or none")
     ap.add_argument("--dc-reps", type=int, default=2)
     ap.add_argument("--gmres-reps", type=int, default=2)
@@ -477,7 +600,8 @@ def main():
     out = bench_pcg_q1(P, args)
     wl = {}
     for w in [w for w in args.workloads.split(",") if w and w != "none"]:
-        fn = {"fv192_dc": bench_fv192_dc, "upwind128_gmres": bench_upwind128_gmres, "serving": bench_serving}[w]
+        fn = {"fv192_dc": bench_fv192_dc, "upwind128_gmres": bench_upwind128_gmres, "jacobi_q1": bench_jacobi_q1,
+              "serving": bench_serving}[w]
         t0 = time.perf_counter()
         wl[w] = fn(P, args)
         wl[w]["bench_s"] = round(time.perf_counter() - t0, 1)

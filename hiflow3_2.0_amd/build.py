@@ -21,7 +21,8 @@ LMP_LIB = os.path.join(HERE, "libhflmp.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("HFH_ARCH", "gfx950")
 
-HIP_SOURCES = ["hfh_core.hip", "hfh_spmv.hip", "hfh_prec.hip", "hfh_krylov.hip", "hfh_batch.hip"]
+HIP_SOURCES = ["hfh_core.hip", "hfh_spmv.hip", "hfh_prec.hip", "hfh_krylov.hip", "hfh_batch.hip", "hfh_jacobi.hip",
+               "hfh_analysis.hip"]
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-result",
     # packed FP32 instructions are kept out of the device code
@@ -64,7 +65,8 @@ def build_hip(force=True, jobs=4):
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         list(ex.map(_run, todo))
     tmp = HIP_LIB + ".tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs)
+    # every symbol resolved at link time: a missing definition fails the build
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-Wl,--no-undefined", "-o", tmp] + objs)
     os.replace(tmp, HIP_LIB)
     return HIP_LIB
 
@@ -85,7 +87,7 @@ def build_lmp(force=True, jobs=4):
     tmp = LMP_LIB + ".tmp"
     # the host layer reaches the device only through libhfhip.so's C ABI
     _run([CXX, "-shared", "-fPIC", "-fopenmp", "-o", tmp] + objs +
-         [f"-L{HERE}", "-lhfhip", "-Wl,-rpath,$ORIGIN"])
+         [f"-L{HERE}", "-lhfhip", "-Wl,-rpath,$ORIGIN", "-Wl,--no-undefined"])
     os.replace(tmp, LMP_LIB)
     return LMP_LIB
 

@@ -83,6 +83,7 @@ struct hfh_csr {
     int *sd_wptr = nullptr;    // [nslice+1] word offset of each slice
     int *diag = nullptr;   // diagonal position per row
     bool has_structure = false;
+    bool sorted = true;      // columns ascending in every row (false after a diagonal-first swap)
     std::vector<int> h_row;  // host copy of the row pointer (for analysis)
     size_t elem() const { return dtype == HFH_F64 ? 8 : 4; }
 };
@@ -173,6 +174,9 @@ struct ScopedKernelTimer {
 int spmv_launch(const hfh_csr *A, const void *x, void *y, int mode);
 int spmv_configure(hfh_csr *A, int algo, int p1, int p2);
 int spmv_sell_fill(hfh_csr *A);
+// After the device columns changed order: diagonal positions, SpMV layout
+// and SELL copy rebuilt from the CSR arrays (host copies already updated).
+int spmv_refresh_structure(hfh_csr *A);
 int spmv_launch_ex(const hfh_csr *A, const void *x, void *y, int mode, const void *w, double *part,
                    const int *done);
 
@@ -188,6 +192,20 @@ __device__ __forceinline__ int xcd_remap(int b, int nb) {
     // XCDs [0, rem) own per+1 tiles, the rest own per tiles
     const int base = xcd < rem ? xcd * (per + 1) : rem * (per + 1) + (xcd - rem) * per;
     return base + idx;
+}
+
+// Separately rounded product and sum.  The contraction pragma sits inside
+// the helpers so that it also holds where they are inlined into code
+// compiled with contraction on.
+template <typename T>
+__device__ __forceinline__ T mul_rn(T a, T b) {
+#pragma clang fp contract(off)
+    return a * b;
+}
+template <typename T>
+__device__ __forceinline__ T add_rn(T a, T b) {
+#pragma clang fp contract(off)
+    return a + b;
 }
 
 template <typename T>

@@ -440,6 +440,13 @@ struct PcgWork {
     }
 };
 
+// An empty system (n = 0) is solved exactly by the empty vector: no
+// iteration, residual 0, reported as an absolute success.
+static int empty_system(hfh_solve_info *info) {
+    if (info) *info = hfh_solve_info{0, HFH_SUCCESS_ABS, 0.0, 0.0, 0.0};
+    return HFH_OK;
+}
+
 static int solve_status(int state) {
     if (state == HFH_SUCCESS_ABS || state == HFH_SUCCESS_REL) return HFH_OK;
     set_error("solver stopped without convergence (state %d)", state);
@@ -467,6 +474,7 @@ extern "C" int hfh_pcg_solve(const hfh_csr *A, hfh_prec *P, const hfh_vec *b, hf
                              hfh_solve_info *info, double *hist) {
     int rc = check_solve_args(A, P, b, x, ctl, "hfh_pcg_solve");
     if (rc) return rc;
+    if (A->nrow == 0) return empty_system(info);
     hfh_solve_info inf{};
     if (A->dtype == HFH_F64) {
         PcgWork<double> W;
@@ -761,6 +769,7 @@ extern "C" int hfh_gmres_solve(const hfh_csr *A, hfh_prec *P, int size_basis, in
     if (rc) return rc;
     HFH_CHECK(size_basis > 0 && size_basis <= kMaxBasis, HFH_ERR_INVALID,
               "hfh_gmres_solve: basis size %d outside 1..%d", size_basis, kMaxBasis);
+    if (A->nrow == 0) return empty_system(info);
     hfh_solve_info inf{};
     if (A->dtype == HFH_F64)
         rc = gmres_impl<double>(A, P, size_basis, flexible, (const double *)b->d, (double *)x->d, *ctl, &inf, hist);
@@ -784,6 +793,11 @@ extern "C" int hfh_dc_solve_ex(const hfh_csr *Ah, const hfh_csr *Al, hfh_prec *P
               "hfh_dc_solve: vectors must be FP64 of size %d", Ah->nrow);
     HFH_CHECK(!Pl || (Pl->dtype == HFH_F32 && Pl->n == Al->nrow), HFH_ERR_INVALID, "hfh_dc_solve: bad preconditioner");
     HFH_CHECK(outer->maxits > 0 && inner->maxits > 0, HFH_ERR_INVALID, "hfh_dc_solve: maxits must be positive");
+    if (Ah->nrow == 0) {
+        // an empty system is solved exactly by the empty vector
+        if (info) *info = hfh_dc_info{0, 0, HFH_SUCCESS_ABS, 0.0, 0.0, 0.0};
+        return HFH_OK;
+    }
     hfh_ctx *c = Ah->ctx;
     hipStream_t s = c->stream;
     const int n = Ah->nrow;

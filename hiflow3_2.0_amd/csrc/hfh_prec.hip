@@ -710,6 +710,8 @@ extern "C" int hfh_jacobi_create(const hfh_csr *A, hfh_prec **out) {
 extern "C" int hfh_mcsgs_create(const hfh_csr *A, int ncolors, const int *color_sizes, double omega, hfh_prec **out) {
     HFH_CHECK(A && out && A->has_structure && color_sizes, HFH_ERR_INVALID, "hfh_mcsgs_create: bad argument");
     HFH_CHECK(A->nrow == A->ncol, HFH_ERR_INVALID, "hfh_mcsgs_create: matrix is not square");
+    HFH_CHECK(A->sorted, HFH_ERR_INVALID,
+              "hfh_mcsgs_create: columns must be ascending in every row (not a diagonal-first matrix)");
     HFH_CHECK(ncolors >= 1 && ncolors <= kMaxColors, HFH_ERR_UNSUPPORTED, "hfh_mcsgs_create: %d colours (1..%d supported)",
               ncolors, kMaxColors);
     HFH_CHECK(omega > 0.0 && omega < 2.0, HFH_ERR_INVALID, "hfh_mcsgs_create: omega %g outside (0, 2)", omega);
@@ -753,6 +755,8 @@ static int create_trsv(const hfh_csr *A, int kind, int nlev, const int *lp, cons
                        const int *ur, hfh_prec **out, const char *what) {
     HFH_CHECK(A && out && A->has_structure, HFH_ERR_INVALID, "%s: bad argument", what);
     HFH_CHECK(A->nrow == A->ncol, HFH_ERR_INVALID, "%s: matrix is not square", what);
+    HFH_CHECK(A->sorted, HFH_ERR_INVALID, "%s: columns must be ascending in every row (not a diagonal-first matrix)",
+              what);
     hfh_prec *P = new_prec(A, kind);
     int rc = count_bad_diag(A, what, nullptr, nullptr, 1.0);
     if (!rc) rc = upload_levels(P->lo, A->nrow, nlev, lp, lr, what);
@@ -813,6 +817,8 @@ extern "C" int hfh_prec_destroy(hfh_prec *P) {
 // lower level schedule (the rows of one level do not depend on each other).
 extern "C" int hfh_csr_ilu0(hfh_csr *A, int nlev, const int *level_ptr, const int *level_rows, int *missing) {
     HFH_CHECK(A && A->has_structure && A->nrow == A->ncol, HFH_ERR_INVALID, "hfh_csr_ilu0: bad matrix");
+    HFH_CHECK(A->sorted, HFH_ERR_INVALID,
+              "hfh_csr_ilu0: columns must be ascending in every row (not a diagonal-first matrix)");
     hfh_levels L;
     int rc = upload_levels(L, A->nrow, nlev, level_ptr, level_rows, "hfh_csr_ilu0");
     if (!rc) rc = validate_levels(A, L, false, "hfh_csr_ilu0");

@@ -67,23 +67,27 @@ __global__ __launch_bounds__(kBlock) void k_spmv_stream(
         const int cnt = j1 - j0;
         if (cnt <= TILE) {
             // unconditional loads at clamped positions, so every load of the
-            // tile is in flight before the first gather result is needed
-            constexpr int K = TILE / kBlock;
-            int c[K];
-            T v[K], xv[K];
-            const int last = cnt > 0 ? cnt - 1 : 0;
+            // tile is in flight before the first gather result is needed; a
+            // block of empty rows (cnt == 0, uniform over the workgroup) loads
+            // nothing: its clamp target col[j0] may lie past the last entry
+            if (cnt > 0) {
+                constexpr int K = TILE / kBlock;
+                int c[K];
+                T v[K], xv[K];
+                const int last = cnt - 1;
 #pragma unroll
-            for (int k = 0; k < K; ++k) {
-                const int e = min(static_cast<int>(threadIdx.x) + k * kBlock, last);
-                c[k] = __builtin_nontemporal_load(&col[j0 + e]);
-                v[k] = __builtin_nontemporal_load(&val[j0 + e]);
-            }
+                for (int k = 0; k < K; ++k) {
+                    const int e = min(static_cast<int>(threadIdx.x) + k * kBlock, last);
+                    c[k] = __builtin_nontemporal_load(&col[j0 + e]);
+                    v[k] = __builtin_nontemporal_load(&val[j0 + e]);
+                }
 #pragma unroll
-            for (int k = 0; k < K; ++k) xv[k] = x[c[k]];
+                for (int k = 0; k < K; ++k) xv[k] = x[c[k]];
 #pragma unroll
-            for (int k = 0; k < K; ++k) {
-                const int e = threadIdx.x + k * kBlock;
-                if (e < cnt) prod[e] = v[k] * xv[k];
+                for (int k = 0; k < K; ++k) {
+                    const int e = threadIdx.x + k * kBlock;
+                    if (e < cnt) prod[e] = v[k] * xv[k];
+                }
             }
             __syncthreads();
             for (int r = r0 + threadIdx.x; r < r1; r += kBlock) {
@@ -215,17 +219,6 @@ __global__ __launch_bounds__(kBlock) void k_spmv_wave(
 // ascending column order with separately rounded products and sums
 // (mul_rn / add_rn are compiled without contraction), so rows agree bit for
 // bit with the CPU path.
-// The operations carry no contraction flag, so the backend cannot fuse them.
-template <typename T>
-__device__ __forceinline__ T mul_rn(T a, T b) {
-#pragma clang fp contract(off)
-    return a * b;
-}
-template <typename T>
-__device__ __forceinline__ T add_rn(T a, T b) {
-#pragma clang fp contract(off)
-    return a + b;
-}
 
 template <typename T, bool ADD, bool DOT, int U>
 __global__ __launch_bounds__(kBlock) void k_spmv_sell(
@@ -284,7 +277,9 @@ typedef int int_pair __attribute__((ext_vector_type(2)));
 
 // SELL-64 with slot pairs: slots 2q, 2q+1 of a lane at base + q*128 + 2*lane,
 // so each lane loads its two values (and columns) with one 16-byte (8-byte)
-// load.  Row sums as k_spmv_sell (ascending slots, uncontracted).
+// load.  A slice of odd width L keeps its last slot single, at
+// base + (L/2)*128 + lane, so no slice is padded beyond its longest row.
+// Row sums as k_spmv_sell (ascending slots, uncontracted).
 template <typename T, bool ADD, bool DOT, int U>
 __global__ __launch_bounds__(kBlock) void k_spmv_sellp(
     int nrow, int nslice, const int *__restrict__ sptr, const int *__restrict__ rlen, const int *__restrict__ scol,
@@ -297,7 +292,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv_sellp(
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     double dsum = 0;
     for (int s = blockIdx.x * WPB + wave; s < nslice; s += gridDim.x * WPB) {
-        const int base = sptr[s], Q = (sptr[s + 1] - base) >> 7;   // slot pairs
+        const int base = sptr[s], L = (sptr[s + 1] - base) >> 6, Q = L >> 1;   // Q slot pairs
         const int r = s * 64 + lane;
         const int len = r < nrow ? rlen[r] : 0;
         T acc = 0;
@@ -318,10 +313,20 @@ __global__ __launch_bounds__(kBlock) void k_spmv_sellp(
             }
 #pragma unroll
             for (int u = 0; u < U; ++u) {
+                // a step past the last pair re-loaded pair Q-1: masked out
+                // (2Q may be below len when the odd last slot is single)
                 const int k = 2 * (q + u);
-                if (k < len) acc = add_rn(acc, mul_rn(static_cast<T>(v[u].x), xa[u]));
-                if (k + 1 < len) acc = add_rn(acc, mul_rn(static_cast<T>(v[u].y), xb[u]));
+                const bool pair = q + u < Q;
+                if (pair && k < len) acc = add_rn(acc, mul_rn(static_cast<T>(v[u].x), xa[u]));
+                if (pair && k + 1 < len) acc = add_rn(acc, mul_rn(static_cast<T>(v[u].y), xb[u]));
             }
+        }
+        if (L & 1) {   // wave-uniform
+            const int idx = base + Q * 128 + lane;
+            const int c = __builtin_nontemporal_load(&scol[idx]);
+            const T v = __builtin_nontemporal_load(&sval[idx]);
+            const T xv = x[c];
+            if (2 * Q < len) acc = add_rn(acc, mul_rn(v, xv));
         }
         if (r < nrow) {
             if (ADD) acc = y[r] + acc;
@@ -406,8 +411,9 @@ __global__ __launch_bounds__(kBlock) void k_csr_to_sell(int nrow, int ncol, int 
     const int base = sptr[s], L = (sptr[s + 1] - base) >> 6;
     const int j0 = r < nrow ? row[r] : 0, len = r < nrow ? row[r + 1] - j0 : 0;
     const int pad = min(r, ncol - 1);
+    const int P = pairs ? (L & ~1) : 0;   // slots held in pairs; an odd last slot is single
     for (int k = 0; k < L; ++k) {
-        const int idx = pairs ? base + (k >> 1) * 128 + 2 * lane + (k & 1) : base + k * 64 + lane;
+        const int idx = k < P ? base + (k >> 1) * 128 + 2 * lane + (k & 1) : base + k * 64 + lane;
         if (scol) scol[idx] = k < len ? col[j0 + k] : pad;   // null for the dictionary-coded form
         sval[idx] = k < len ? val[j0 + k] : static_cast<T>(0);
     }
@@ -532,7 +538,7 @@ int spmv_configure(hfh_csr *A, int algo, int p1, int p2) {
     if (algo == HFH_SPMV_SELL_DICT) {
         std::vector<long long> offs;
         offs.reserve(1024);
-        offs.push_back(0);
+        offs.push_back(0);   // always present: the code of every padding slot
         bool fits = true;
         for (int r = 0; r < n && fits; ++r)
             for (int j = A->h_row[r]; j < A->h_row[r + 1]; ++j) {
@@ -613,7 +619,6 @@ int spmv_configure(hfh_csr *A, int algo, int p1, int p2) {
                 len[r] = A->h_row[r + 1] - A->h_row[r];
                 L = std::max(L, len[r]);
             }
-            if (pairs) L = (L + 1) / 2 * 2;
             ptr[sl] = static_cast<int>(off);
             off += 64LL * L;
             HFH_CHECK(off < (1LL << 31), HFH_ERR_UNSUPPORTED, "spmv: SELL copy exceeds 2^31 entries");
@@ -661,6 +666,8 @@ int spmv_configure(hfh_csr *A, int algo, int p1, int p2) {
         wp[nsl] = static_cast<int>(off / 4);
         // code words: byte t of word w of a lane = code of slot 4w + t
         std::vector<unsigned> words(std::max(off / 4, 1LL), 0u);
+        // padding slots take the code of offset 0, which the table always
+        // holds: their gather reads x[min(r, ncol - 1)], in range
         const unsigned zero = static_cast<unsigned>(std::lower_bound(dict.begin(), dict.end(), 0) - dict.begin());
         for (int sl = 0; sl < nsl; ++sl) {
             const int L4 = (ptr[sl + 1] - ptr[sl]) / 64;
@@ -924,10 +931,24 @@ extern "C" int hfh_csr_upload(hfh_csr *A, const int *row, const int *col, const 
         if ((rc = launch_check("k_find_diag"))) return rc;
     }
     A->has_structure = true;
+    A->sorted = true;
     A->sell_dirty = true;
     if ((rc = spmv_sell_fill(A))) return rc;
     HFH_HIP(hipStreamSynchronize(s));
     return HFH_OK;
+}
+
+int hfh::spmv_refresh_structure(hfh_csr *A) {
+    hipStream_t s = A->ctx->stream;
+    int rc = spmv_configure(A, A->spmv_algo_req, A->spmv_p1_req, A->spmv_p2_req);
+    if (rc) return rc;
+    if (A->nrow > 0) {
+        hipLaunchKernelGGL(k_find_diag<int>, dim3((A->nrow + kBlock - 1) / kBlock), dim3(kBlock), 0, s, A->nrow,
+                           A->row, A->col, A->diag);
+        if ((rc = launch_check("k_find_diag"))) return rc;
+    }
+    A->sell_dirty = true;
+    return spmv_sell_fill(A);
 }
 
 extern "C" int hfh_csr_upload_values(hfh_csr *A, const void *val) {
@@ -962,6 +983,7 @@ extern "C" int hfh_csr_cast(hfh_csr *dst, const hfh_csr *src) {
     if (src->nrow) HFH_HIP(hipMemcpyAsync(dst->diag, src->diag, sizeof(int) * static_cast<size_t>(src->nrow), hipMemcpyDeviceToDevice, s));
     dst->h_row = src->h_row;
     dst->h_col = src->h_col;
+    dst->sorted = src->sorted;
     int rc = spmv_configure(dst, dst->spmv_algo_req ? dst->spmv_algo_req : src->spmv_algo,
                             dst->spmv_algo_req ? dst->spmv_p1_req : src->spmv_p1,
                             dst->spmv_algo_req ? dst->spmv_p2_req : 0);

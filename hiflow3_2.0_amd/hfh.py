@@ -115,6 +115,13 @@ HIP_API = {
     "hfh_prec_destroy": (_i, [_vp]),
     "hfh_pcg_solve": (_i, [_vp, _vp, _vp, _vp, _P(Control), _P(SolveInfo), _vp]),
     "hfh_gmres_solve": (_i, [_vp, _vp, _i, _i, _vp, _vp, _P(Control), _P(SolveInfo), _vp]),
+    "hfh_csr_levels": (_i, [_vp, _i, _P(_i), _vp, _vp]),
+    "hfh_csr_multicoloring": (_i, [_vp, _P(_i), _vp, _vp, _vp]),
+    "hfh_csr_permute": (_i, [_vp, _vp, _vp]),
+    "hfh_csr_swap_diag_front": (_i, [_vp]),
+    "hfh_csr_is_sorted": (_i, [_vp, _P(_i)]),
+    "hfh_spmv_nodiag": (_i, [_vp, _vp, _vp]),
+    "hfh_jacobi_solve": (_i, [_vp, _vp, _vp, _i, _d, _i, _i, _P(Control), _P(SolveInfo), _vp]),
     "hfh_dc_solve": (_i, [_vp, _vp, _vp, _vp, _vp, _P(Control), _P(Control), _P(DcInfo), _vp]),
     "hfh_dc_solve_ex": (_i, [_vp, _vp, _vp, _vp, _vp, _P(Control), _P(Control), _P(C.c_int), _P(DcInfo), _vp]),
     "hfh_batch_create": (_i, [_vp, _P(BatchParams), _P(_vp)]),
@@ -426,6 +433,50 @@ class CsrMatrix:
         _ck(hip().hfh_csr_invdiag(self.h, v.h))
         return v
 
+    def levels(self, upper=False):
+        """Level schedule computed on the device: (level_ptr, level_rows), the
+        same arrays as levels(row, col, upper)."""
+        nl = C.c_int()
+        ptr = np.zeros(self.nrow + 1, np.int32)
+        rows = np.zeros(self.nrow, np.int32)
+        _ck(hip().hfh_csr_levels(self.h, 1 if upper else 0, C.byref(nl), _p(ptr), _p(rows)))
+        return ptr[: nl.value + 1].copy(), rows
+
+    def multicoloring(self):
+        """First-fit multicolouring on the device: (ncolors, color_of,
+        color_sizes, perm), the same as multicoloring(row, col)."""
+        nc = C.c_int()
+        color = np.zeros(self.nrow, np.int32)
+        sizes = np.zeros(self.nrow, np.int32)
+        perm = np.zeros(self.nrow, np.int32)
+        _ck(hip().hfh_csr_multicoloring(self.h, C.byref(nc), _p(color), _p(sizes), _p(perm)))
+        return nc.value, color, sizes[: nc.value].copy(), perm
+
+    def permuted(self, perm):
+        """A new device matrix P with P[i, j] = A[perm[i], perm[j]] (columns
+        ascending), built on the device."""
+        P = CsrMatrix.__new__(CsrMatrix)
+        P.ctx, P.nrow, P.ncol, P.nnz, P.dtype = self.ctx, self.nrow, self.ncol, self.nnz, self.dtype
+        h = C.c_void_p()
+        _ck(hip().hfh_csr_create(self.ctx.h, self.nrow, self.ncol, self.nnz, _dt(self.dtype), C.byref(h)))
+        P.h = h
+        _ck(hip().hfh_csr_permute(self.h, _p(np.ascontiguousarray(perm, np.int32)), P.h))
+        return P
+
+    def swap_diag_front(self):
+        """SwapDiagElementsToRowFront in place (what Jacobi's Prepare does to
+        the operator); the columns are no longer ascending afterwards."""
+        _ck(hip().hfh_csr_swap_diag_front(self.h))
+
+    def is_sorted(self):
+        v = C.c_int()
+        _ck(hip().hfh_csr_is_sorted(self.h, C.byref(v)))
+        return bool(v.value)
+
+    def spmv_nodiag(self, x, y):
+        """VectorMultNoDiag: y = (A - D) x, stored order."""
+        _ck(hip().hfh_spmv_nodiag(self.h, x.h, y.h))
+
 
 class Preconditioner:
     def __init__(self, h, keep=()):
@@ -508,6 +559,27 @@ def gmres(A, b, x, P=None, m=30, flexible=False, maxits=1000, atol=1e-15, rtol=1
     out["rc"] = rc
     if history:
         out["hist"] = hist[: inf.iters + 1].copy()
+    return out
+
+
+JACOBI_SOLVE_NORMAL, JACOBI_SOLVE_DAMPED, JACOBI_SMOOTH_NORMAL, JACOBI_SMOOTH_DAMPED = 0, 1, 2, 3
+
+
+def jacobi(A, b, x, mode=JACOBI_SOLVE_NORMAL, w=1.0, inner_iter=1, smooth_iter=1, maxits=1000, atol=1e-15,
+           rtol=1e-8, dtol=1e6, history=False, check=True):
+    """Jacobi<LAD>::Solve on the device (A after swap_diag_front, as Prepare
+    leaves it); returns the info dict (and the checked residuals)."""
+    ctl = Control(maxits, atol, rtol, dtol)
+    inf = SolveInfo()
+    hist = np.zeros(maxits // max(inner_iter, 1) + 2) if history else None
+    rc = hip().hfh_jacobi_solve(A.h, b.h, x.h, mode, w, inner_iter, smooth_iter, C.byref(ctl), C.byref(inf),
+                                _p(hist))
+    if rc != OK and (check or rc != ERR_NOT_CONVERGED):
+        _ck(rc)
+    out = _info(inf)
+    out["rc"] = rc
+    if history and mode < 2:
+        out["hist"] = hist[: inf.iters // max(inner_iter, 1) + 1].copy()
     return out
 
 

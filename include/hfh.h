@@ -185,6 +185,24 @@ int hfh_ilu0_create(const hfh_csr *LU, int nlev, const int *level_ptr, const int
  * so the factors are bit-identical to the host loop.  *missing: lower entries
  * skipped for lack of a stored diagonal in their row. */
 int hfh_csr_ilu0(hfh_csr *A, int nlev, const int *level_ptr, const int *level_rows, int *missing);
+/* ---- preconditioner setup on the device ----
+ * The same results as the host analyses, computed from the device pattern
+ * (a fixed-point iteration over all rows, then a stable sort); outputs are
+ * host arrays in the formats the create calls above take.
+ * CPU_CSR_lMatrix::Levelscheduling (lmatrix_csr_cpu.cc:2295-2391): upper = 0
+ * orders the rows for the forward sweep (dependencies on lower columns),
+ * upper = 1 for the backward sweep.  level_ptr: n + 1 entries, level_rows: n. */
+int hfh_csr_levels(const hfh_csr *A, int upper, int *nlev, int *level_ptr, int *level_rows);
+/* CPU_CSR_lMatrix::Multicoloring (lmatrix_csr_cpu.cc:2175-2291): first-fit
+ * colours (1-based, color_of optional, n entries), the count of each colour
+ * (color_sizes: room for n entries) and perm[new] = old, rows stable by
+ * colour.  HFH_ERR_INVALID for a pattern without off-diagonal entries. */
+int hfh_csr_multicoloring(const hfh_csr *A, int *ncolors, int *color_of, int *color_sizes, int *perm);
+/* CPU_CSR_lMatrix::Reorder (lmatrix_csr_cpu.cc:1847-1862): P = A permuted
+ * symmetrically, row i of P = row perm[i] of A with columns renumbered and
+ * ascending.  P: created with A's dims, nnz and dtype; perm (host) is checked
+ * to be a permutation. */
+int hfh_csr_permute(const hfh_csr *A, const int *perm, hfh_csr *P);
 /* ApplylPreconditioner (lpreconditioner.h): out = M^{-1} in */
 int hfh_prec_apply(hfh_prec *P, const hfh_vec *in, hfh_vec *out);
 int hfh_prec_kind(const hfh_prec *P, int *kind);
@@ -209,6 +227,8 @@ typedef struct {
  * iterates and stopping test as the reference.  P may be NULL (no
  * preconditioning), Jacobi or MC-SGS.  Returns HFH_OK on convergence,
  * HFH_ERR_NOT_CONVERGED on maxits/divergence (info is filled either way).
+ * An empty system (n = 0) returns at once: 0 iterations, residual 0,
+ * state HFH_SUCCESS_ABS (the same holds for GMRES and defect correction).
  * hist (optional, host, maxits+1 doubles) receives the residual history. */
 int hfh_pcg_solve(const hfh_csr *A, hfh_prec *P, const hfh_vec *b, hfh_vec *x,
                   const hfh_control *ctl, hfh_solve_info *info, double *hist);
@@ -220,6 +240,33 @@ int hfh_pcg_solve(const hfh_csr *A, hfh_prec *P, const hfh_vec *b, hfh_vec *x,
 int hfh_gmres_solve(const hfh_csr *A, hfh_prec *P, int size_basis, int flexible,
                     const hfh_vec *b, hfh_vec *x, const hfh_control *ctl,
                     hfh_solve_info *info, double *hist);
+
+/* ---- stationary Jacobi solver / smoother (linear_solver/jacobi.cc) ---- */
+/* CSR_lMatrix::SwapDiagElementsToRowFront (lmatrix_csr_cpu.cc:477-495), as
+ * Jacobi<LAD>::Prepare (jacobi.cc:44-67) applies it to the operator: each
+ * row's diagonal entry trades places with the row's first entry, in place on
+ * the device.  The SpMV of the matrix then sums rows in the new stored order.
+ * Afterwards the columns are no longer ascending, and the SGS / MC-SGS /
+ * ILU(0) setups refuse the matrix (HFH_ERR_INVALID). */
+int hfh_csr_swap_diag_front(hfh_csr *A);
+/* 1 when every row's columns are ascending (as uploaded), 0 after the swap. */
+int hfh_csr_is_sorted(const hfh_csr *A, int *sorted);
+/* CPU_CSR_lMatrix::VectorMultNoDiag (lmatrix_csr_cpu.cc:521-538):
+ * y_i = sum over j != i of a_ij x_j, in stored order. */
+int hfh_spmv_nodiag(const hfh_csr *A, const hfh_vec *x, hfh_vec *y);
+/* Jacobi<LAD>::Solve (jacobi.cc:100-285) on one rank.  mode 0 SolveNormal,
+ * 1 SolveDamped, 2 SmoothNormal, 3 SmoothDamped (SetSolveMode); w the
+ * damping parameter (SetDampingParameter, > 0 for the damped modes);
+ * inner_iter steps between residual checks (SetInnerIter); smooth_iter
+ * blocks of inner_iter steps in the smoother modes (SetNumIter).  A is the
+ * operator after hfh_csr_swap_diag_front, as Prepare leaves it.  The solve
+ * modes test ||b - A x|| with ctl (IterateControl, iteration count = steps)
+ * and return HFH_ERR_NOT_CONVERGED on maxits / divergence; the smoother modes
+ * ignore ctl (may be NULL) and return HFH_OK.  hist (optional, solve modes,
+ * maxits/inner_iter + 2 doubles) receives every checked residual. */
+int hfh_jacobi_solve(const hfh_csr *A, const hfh_vec *b, hfh_vec *x, int mode, double w,
+                     int inner_iter, int smooth_iter, const hfh_control *ctl,
+                     hfh_solve_info *info, double *hist);
 
 /* Mixed-precision defect correction (MPIR<D,S>::Solve, mpir.cc:69-327):
  * FP64 true residual, FP64 update and FP64 outer test on the device; FP32

@@ -59,7 +59,9 @@ def main(ref):
 def copy_mm(ref):
     import shutil
     os.makedirs(os.path.join(HERE, "mm"), exist_ok=True)
-    for src in ("test/LinearAlgebra/data/Equation1.A.txt", "test/AMG/data/A3.mtx", "test/AMG/data/A4.mtx"):
+    for src in ("test/LinearAlgebra/data/Equation1.A.txt", "test/LinearAlgebra/data/Equation1.rhs.txt",
+                "test/LinearAlgebra/data/Equation1.sol.txt", "test/LinearAlgebra/data/Equation1.x0.txt",
+                "test/AMG/data/A3.mtx", "test/AMG/data/A4.mtx"):
         shutil.copyfile(os.path.join(ref, src), os.path.join(HERE, "mm", os.path.basename(src)))
 
 
