@@ -22,7 +22,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("HFH_ARCH", "gfx950")
 
 HIP_SOURCES = ["hfh_core.hip", "hfh_spmv.hip", "hfh_prec.hip", "hfh_krylov.hip", "hfh_batch.hip", "hfh_jacobi.hip",
-               "hfh_analysis.hip"]
+               "hfh_analysis.hip", "hfh_assemble.hip"]
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-result",
     # packed FP32 instructions are kept out of the device code

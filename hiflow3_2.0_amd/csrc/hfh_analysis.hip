@@ -8,14 +8,16 @@
 //   level(i)  = 1 + max level(d) over the lower neighbours d < i of row i
 //               (0 without one; the upper schedule mirrors it with d > i);
 //   colour(i) = the smallest colour >= 1 that no lower neighbour holds.
-// Each is the unique fixed point of a map that reads only lower neighbours:
-// rows of level 0 are final after one application of the map, rows of level
-// k after k + 1.  The device applies the map to every row at once (one
-// double-buffered sweep over the CSR arrays) until a sweep changes nothing,
-// then orders the rows by (level, index) or (colour, index) with a stable
-// radix sort.  Levels, colours and orders are bit-identical to the host
-// passes.  Cost: (depth + 1) sweeps of about 4 nnz + 12 n bytes plus the
-// neighbour gathers, then a sort of n keys.
+// Both depend only on lower neighbours, so the device visits the rows in a
+// level-synchronous topological order (Kahn): each row counts its lower
+// neighbours; the rows without one form level 0; processing level L gives
+// each of its rows its colour (its lower neighbours all sit on earlier
+// levels) and decrements the count of every row that depends on it, and a row
+// whose count reaches zero joins level L + 1.  One launch per level, O(nnz)
+// atomics in all.  The level and colour of every row are deterministic (the
+// order inside a frontier is not, and is not used): a stable radix sort by
+// (level or colour, row) then yields the host's order.  Levels, colours and
+// orders are bit-identical to the host passes.
 #include "hfh_internal.h"
 
 #include <hipcub/hipcub.hpp>
@@ -26,35 +28,72 @@
 namespace hfh {
 namespace {
 
-constexpr int kChunk = 16;   // sweeps between convergence checks
+constexpr int kChunk = 32;   // level launches between termination checks
 
-// One sweep of the lower map.  flags[0] is set when a value changed (checked
-// sweeps only), flags[1] when a row holds an off-diagonal entry (first sweep
-// only); one store per workgroup.
-template <bool COLOR>
-__global__ __launch_bounds__(kBlock) void k_relax_lower(int n, const int *__restrict__ row,
-                                                        const int *__restrict__ col, const int *__restrict__ lev_in,
-                                                        int *__restrict__ lev_out, const int *__restrict__ clr_in,
-                                                        int *__restrict__ clr_out, int *flags, int check, int first) {
-    bool changed = false, off = false;
+__device__ __forceinline__ bool depends(int d, int i, int upper) { return upper ? d > i : d < i; }
+
+// indeg[i] = neighbours row i waits for; depcnt[d] = rows that wait for d.
+// flags[0]: some row has an off-diagonal entry (one store per workgroup).
+__global__ __launch_bounds__(kBlock) void k_dep_count(int n, const int *__restrict__ row, const int *__restrict__ col,
+                                                      int upper, int *__restrict__ indeg, int *depcnt, int *flags) {
+    bool off = false;
     for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
-        const int j0 = row[i], j1 = row[i + 1];
-        int l = 0;
-        unsigned long long held = 0;   // bit c-1: colour c held by a lower neighbour (c <= 64)
-        for (int q = j0; q < j1; ++q) {
+        int c = 0;
+        for (int q = row[i]; q < row[i + 1]; ++q) {
             const int d = col[q];
             off |= d != i;
-            if (d < i) {
-                l = max(l, lev_in[d] + 1);
-                if (COLOR) {
-                    const int c = clr_in[d];
+            if (depends(d, i, upper)) {
+                ++c;
+                atomicAdd(&depcnt[d], 1);
+            }
+        }
+        indeg[i] = c;
+    }
+    if (__syncthreads_or(off) && threadIdx.x == 0) flags[0] = 1;
+}
+
+// deps[depptr[d] ...] = the rows that wait for d (in no particular order).
+__global__ __launch_bounds__(kBlock) void k_dep_fill(int n, const int *__restrict__ row, const int *__restrict__ col,
+                                                     int upper, const int *__restrict__ depptr, int *fillpos,
+                                                     int *__restrict__ deps) {
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
+        for (int q = row[i]; q < row[i + 1]; ++q) {
+            const int d = col[q];
+            if (depends(d, i, upper)) deps[depptr[d] + atomicAdd(&fillpos[d], 1)] = i;
+        }
+}
+
+// Level 0: the rows that wait for nobody.
+__global__ __launch_bounds__(kBlock) void k_frontier0(int n, const int *__restrict__ indeg, int *__restrict__ front,
+                                                      int *cnt, int *__restrict__ level) {
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
+        if (indeg[i] == 0) {
+            front[atomicAdd(&cnt[0], 1)] = i;
+            level[i] = 0;
+        }
+}
+
+// Level L: rows front[base[L] .. base[L] + cnt[L]).  COLOR: each takes the
+// smallest colour >= 1 not held by a lower neighbour.  Then every row waiting
+// for it counts down; those reaching zero are appended to level L + 1.
+template <bool COLOR>
+__global__ __launch_bounds__(kBlock) void k_level(int L, const int *__restrict__ row, const int *__restrict__ col,
+                                                  const int *__restrict__ depptr, const int *__restrict__ deps,
+                                                  int *indeg, int *front, int *base, int *cnt, int *level, int *clr) {
+    const int b0 = base[L], sz = cnt[L], nb = b0 + sz;
+    if (blockIdx.x == 0 && threadIdx.x == 0) base[L + 1] = nb;
+    for (int t = blockIdx.x * kBlock + threadIdx.x; t < sz; t += gridDim.x * kBlock) {
+        const int j = front[b0 + t];
+        if (COLOR) {
+            const int j0 = row[j], j1 = row[j + 1];
+            unsigned long long held = 0;   // bit c-1: colour c held by a lower neighbour (c <= 64)
+            for (int q = j0; q < j1; ++q) {
+                const int d = col[q];
+                if (d < j) {
+                    const int c = clr[d];
                     if (c <= 64) held |= 1ull << (c - 1);
                 }
             }
-        }
-        changed |= l != lev_in[i];
-        lev_out[i] = l;
-        if (COLOR) {
             int c;
             if (~held != 0ull) {
                 c = __builtin_ctzll(~held) + 1;
@@ -62,42 +101,21 @@ __global__ __launch_bounds__(kBlock) void k_relax_lower(int n, const int *__rest
                 c = 65;
                 while (true) {
                     bool taken = false;
-                    for (int q = j0; q < j1 && !taken; ++q) taken = col[q] < i && clr_in[col[q]] == c;
+                    for (int q = j0; q < j1 && !taken; ++q) taken = col[q] < j && clr[col[q]] == c;
                     if (!taken) break;
                     ++c;
                 }
             }
-            changed |= c != clr_in[i];
-            clr_out[i] = c;
+            clr[j] = c;
+        }
+        for (int q = depptr[j]; q < depptr[j + 1]; ++q) {
+            const int i = deps[q];
+            if (atomicSub(&indeg[i], 1) == 1) {
+                level[i] = L + 1;
+                front[nb + atomicAdd(&cnt[L + 1], 1)] = i;
+            }
         }
     }
-    const int any_changed = __syncthreads_or(changed);
-    const int any_off = __syncthreads_or(off);
-    if (threadIdx.x == 0) {
-        if (check && any_changed) flags[0] = 1;
-        if (first && any_off) flags[1] = 1;
-    }
-}
-
-__global__ __launch_bounds__(kBlock) void k_relax_upper(int n, const int *__restrict__ row,
-                                                        const int *__restrict__ col, const int *__restrict__ lev_in,
-                                                        int *__restrict__ lev_out, int *flags, int check) {
-    bool changed = false;
-    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
-        int l = 0;
-        for (int q = row[i]; q < row[i + 1]; ++q) {
-            const int d = col[q];
-            if (d > i) l = max(l, lev_in[d] + 1);
-        }
-        changed |= l != lev_in[i];
-        lev_out[i] = l;
-    }
-    const int any_changed = __syncthreads_or(changed);
-    if (threadIdx.x == 0 && check && any_changed) flags[0] = 1;
-}
-
-__global__ __launch_bounds__(kBlock) void k_fill_int(int n, int *__restrict__ a, int v) {
-    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) a[i] = v;
 }
 
 __global__ __launch_bounds__(kBlock) void k_iota(int n, int *__restrict__ a) {
@@ -179,55 +197,79 @@ int grid_for(const hfh_ctx *c, int n) {
     return static_cast<int>(std::max<long long>(1, std::min(nb, cap)));
 }
 
-// Iterates the map to its fixed point.  On return lev (and clr) hold the
-// result; *has_off tells whether any row has an off-diagonal entry.
-int relax(const hfh_csr *A, bool upper, bool color, Bufs &B, int **lev, int **clr, bool *has_off) {
+// Levels (and first-fit colours) of all rows by the level-synchronous pass.
+// On return *lev (and *clr) hold them; *has_off tells whether any row has an
+// off-diagonal entry.
+int topo(const hfh_csr *A, bool upper, bool color, Bufs &B, int **lev, int **clr, bool *has_off) {
     hfh_ctx *c = A->ctx;
     hipStream_t s = c->stream;
     const int n = A->nrow;
-    int *L[2] = {nullptr, nullptr}, *K[2] = {nullptr, nullptr}, *flags = nullptr;
+    int *indeg = nullptr, *depcnt = nullptr, *depptr = nullptr, *fillpos = nullptr, *deps = nullptr;
+    int *front = nullptr, *base = nullptr, *cnt = nullptr, *level = nullptr, *colour = nullptr, *flags = nullptr;
     int rc;
-    if ((rc = B.get(reinterpret_cast<void **>(&L[0]), sizeof(int) * n)) ||
-        (rc = B.get(reinterpret_cast<void **>(&L[1]), sizeof(int) * n)) ||
-        (rc = B.get(reinterpret_cast<void **>(&flags), sizeof(int) * 2)))
+    if ((rc = B.get(reinterpret_cast<void **>(&indeg), sizeof(int) * n)) ||
+        (rc = B.get(reinterpret_cast<void **>(&depcnt), sizeof(int) * (n + 1))) ||
+        (rc = B.get(reinterpret_cast<void **>(&depptr), sizeof(int) * (n + 1))) ||
+        (rc = B.get(reinterpret_cast<void **>(&fillpos), sizeof(int) * n)) ||
+        (rc = B.get(reinterpret_cast<void **>(&front), sizeof(int) * n)) ||
+        (rc = B.get(reinterpret_cast<void **>(&base), sizeof(int) * (n + 2))) ||
+        (rc = B.get(reinterpret_cast<void **>(&cnt), sizeof(int) * (n + 2))) ||
+        (rc = B.get(reinterpret_cast<void **>(&level), sizeof(int) * n)) ||
+        (rc = B.get(reinterpret_cast<void **>(&flags), sizeof(int))))
         return rc;
-    if (color && ((rc = B.get(reinterpret_cast<void **>(&K[0]), sizeof(int) * n)) ||
-                  (rc = B.get(reinterpret_cast<void **>(&K[1]), sizeof(int) * n))))
-        return rc;
+    if (color && (rc = B.get(reinterpret_cast<void **>(&colour), sizeof(int) * n))) return rc;
     const int g = grid_for(c, n);
-    HFH_HIP(hipMemsetAsync(L[0], 0, sizeof(int) * n, s));
-    HFH_HIP(hipMemsetAsync(flags, 0, sizeof(int) * 2, s));
-    if (color) hipLaunchKernelGGL(k_fill_int, dim3(g), dim3(kBlock), 0, s, n, K[0], 1);
-    if ((rc = launch_check("k_fill_int"))) return rc;
-    int cur = 0;
-    int *h = reinterpret_cast<int *>(c->host_res);   // pinned, at least 2 ints
-    // the fixed point is reached after depth + 1 <= n sweeps
-    for (long long done = 0; done <= n + kChunk; done += kChunk) {
+    const int up = upper ? 1 : 0;
+    HFH_HIP(hipMemsetAsync(depcnt, 0, sizeof(int) * (n + 1), s));
+    HFH_HIP(hipMemsetAsync(fillpos, 0, sizeof(int) * n, s));
+    HFH_HIP(hipMemsetAsync(base, 0, sizeof(int) * (n + 2), s));
+    HFH_HIP(hipMemsetAsync(cnt, 0, sizeof(int) * (n + 2), s));
+    HFH_HIP(hipMemsetAsync(flags, 0, sizeof(int), s));
+    hipLaunchKernelGGL(k_dep_count, dim3(g), dim3(kBlock), 0, s, n, A->row, A->col, up, indeg, depcnt, flags);
+    if ((rc = launch_check("k_dep_count"))) return rc;
+    size_t tmp_bytes = 0;
+    HFH_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, depcnt, depptr, n + 1, s));
+    void *tmp = nullptr;
+    if ((rc = B.get(&tmp, std::max<size_t>(tmp_bytes, 1)))) return rc;
+    HFH_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, depcnt, depptr, n + 1, s));
+    int *h = reinterpret_cast<int *>(c->host_res);   // pinned, room for 256 doubles
+    HFH_HIP(hipMemcpyAsync(h, depptr + n, sizeof(int), hipMemcpyDeviceToHost, s));
+    HFH_HIP(hipStreamSynchronize(s));
+    const long long ndeps = h[0];
+    if ((rc = B.get(reinterpret_cast<void **>(&deps), sizeof(int) * static_cast<size_t>(std::max(ndeps, 1LL)))))
+        return rc;
+    hipLaunchKernelGGL(k_dep_fill, dim3(g), dim3(kBlock), 0, s, n, A->row, A->col, up, depptr, fillpos, deps);
+    hipLaunchKernelGGL(k_frontier0, dim3(g), dim3(kBlock), 0, s, n, indeg, front, cnt, level);
+    if ((rc = launch_check("k_frontier0"))) return rc;
+    // every row joins a level after at most n levels (the dependencies
+    // point one way, so they form no cycle); a frontier is at most a few
+    // planes of a grid, so the level launches take a smaller grid
+    const int gl = std::min(g, 512);
+    for (int L = 0; L <= n; L += kChunk) {
         for (int k = 0; k < kChunk; ++k) {
-            const int check = k == kChunk - 1, first = done == 0 && k == 0;
-            if (check) HFH_HIP(hipMemsetAsync(flags, 0, sizeof(int), s));
-            if (upper)
-                hipLaunchKernelGGL(k_relax_upper, dim3(g), dim3(kBlock), 0, s, n, A->row, A->col, L[cur], L[cur ^ 1],
-                                   flags, check);
-            else if (color)
-                hipLaunchKernelGGL(k_relax_lower<true>, dim3(g), dim3(kBlock), 0, s, n, A->row, A->col, L[cur],
-                                   L[cur ^ 1], K[cur], K[cur ^ 1], flags, check, first);
+            if (color)
+                hipLaunchKernelGGL(k_level<true>, dim3(gl), dim3(kBlock), 0, s, L + k, A->row, A->col, depptr, deps,
+                                   indeg, front, base, cnt, level, colour);
             else
-                hipLaunchKernelGGL(k_relax_lower<false>, dim3(g), dim3(kBlock), 0, s, n, A->row, A->col, L[cur],
-                                   L[cur ^ 1], nullptr, nullptr, flags, check, first);
-            if ((rc = launch_check("k_relax"))) return rc;
-            cur ^= 1;
+                hipLaunchKernelGGL(k_level<false>, dim3(gl), dim3(kBlock), 0, s, L + k, A->row, A->col, depptr, deps,
+                                   indeg, front, base, cnt, level, nullptr);
+            if ((rc = launch_check("k_level"))) return rc;
+            if (L + k + 1 > n) break;
         }
-        HFH_HIP(hipMemcpyAsync(h, flags, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
+        const int next = std::min(L + kChunk, n + 1);
+        HFH_HIP(hipMemcpyAsync(h, base + next, sizeof(int), hipMemcpyDeviceToHost, s));
+        HFH_HIP(hipMemcpyAsync(h + 1, cnt + next, sizeof(int), hipMemcpyDeviceToHost, s));
+        HFH_HIP(hipMemcpyAsync(h + 2, flags, sizeof(int), hipMemcpyDeviceToHost, s));
         HFH_HIP(hipStreamSynchronize(s));
-        if (h[0] == 0) {
-            *lev = L[cur];
-            if (clr) *clr = color ? K[cur] : nullptr;
-            if (has_off) *has_off = h[1] != 0;
+        if (h[1] == 0) {
+            HFH_CHECK(h[0] == n, HFH_ERR_INVALID, "device analysis: %d of %d rows placed", h[0], n);
+            *lev = level;
+            if (clr) *clr = colour;
+            if (has_off) *has_off = h[2] != 0;
             return HFH_OK;
         }
     }
-    set_error("device analysis: no fixed point after %d sweeps", n + 2 * kChunk);
+    set_error("device analysis: rows left after %d levels", n + 1);
     return HFH_ERR_INVALID;
 }
 
@@ -287,7 +329,7 @@ extern "C" int hfh_csr_levels(const hfh_csr *A, int upper, int *nlev, int *level
     HFH_CHECK(nlev && level_ptr && level_rows, HFH_ERR_INVALID, "hfh_csr_levels: null output");
     Bufs B(A->ctx->stream);
     int *lev = nullptr, *rows = nullptr;
-    if ((rc = relax(A, upper != 0, false, B, &lev, nullptr, nullptr))) return rc;
+    if ((rc = topo(A, upper != 0, false, B, &lev, nullptr, nullptr))) return rc;
     std::vector<int> ptr;
     if ((rc = bucket(A->ctx, A->nrow, lev, 0, B, &rows, ptr))) return rc;
     HFH_HIP(hipMemcpyAsync(level_rows, rows, sizeof(int) * A->nrow, hipMemcpyDeviceToHost, A->ctx->stream));
@@ -304,7 +346,7 @@ extern "C" int hfh_csr_multicoloring(const hfh_csr *A, int *ncolors, int *color_
     Bufs B(A->ctx->stream);
     int *lev = nullptr, *clr = nullptr, *rows = nullptr;
     bool off = false;
-    if ((rc = relax(A, false, true, B, &lev, &clr, &off))) return rc;
+    if ((rc = topo(A, false, true, B, &lev, &clr, &off))) return rc;
     // the reference counts only colours met at rows with off-diagonal
     // entries; a pattern without one has no colouring
     HFH_CHECK(off, HFH_ERR_INVALID, "hfh_csr_multicoloring: the pattern has no off-diagonal entry");

@@ -66,16 +66,62 @@ static inline int red_blocks(int n) {
     return static_cast<int>(nb);
 }
 
+// 16-byte vectors of a stream: 2 doubles or 4 floats.
+template <typename T> struct V16;
+template <> struct V16<double> {
+    typedef double type __attribute__((ext_vector_type(2)));
+    static constexpr int W = 2;
+};
+template <> struct V16<float> {
+    typedef float type __attribute__((ext_vector_type(4)));
+    static constexpr int W = 4;
+};
+
+// Sum of x_i y_i in FP64 (FP32 products are exact in FP64).  nv: 16-byte
+// vectors read as such (0 when an operand is not 16-byte aligned); the
+// elements from nv*W on are read one by one.  Each thread keeps kU vectors of
+// both operands in flight and kU partial sums.
+constexpr int kU = 4;
+
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_dot(int n, const T *__restrict__ x, const T *__restrict__ y,
-                                                double *part) {
+__global__ __launch_bounds__(kBlock) void k_dot(int n, long long nv, const T *__restrict__ x,
+                                                const T *__restrict__ y, double *part) {
+    typedef typename V16<T>::type V;
+    constexpr int W = V16<T>::W;
     __shared__ double sh[kBlock / 64];
-    const int nb = gridDim.x;
-    double s = 0;
-    for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < n; i += (long long)nb * kBlock)
-        s += static_cast<double>(x[i]) * static_cast<double>(y[i]);
-    const double v[1] = {block_sum(s, sh)};
+    const V *__restrict__ xv = reinterpret_cast<const V *>(x);
+    const V *__restrict__ yv = reinterpret_cast<const V *>(y);
+    const long long stride = static_cast<long long>(gridDim.x) * kBlock;
+    long long i = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x;
+    double acc[kU] = {0, 0, 0, 0};
+    for (; i + (kU - 1) * stride < nv; i += kU * stride) {
+        V a[kU], b[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            a[u] = xv[i + u * stride];
+            b[u] = yv[i + u * stride];
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u)
+#pragma unroll
+            for (int w = 0; w < W; ++w) acc[u] += static_cast<double>(a[u][w]) * static_cast<double>(b[u][w]);
+    }
+    for (; i < nv; i += stride) {
+        const V a = xv[i], b = yv[i];
+#pragma unroll
+        for (int w = 0; w < W; ++w) acc[0] += static_cast<double>(a[w]) * static_cast<double>(b[w]);
+    }
+    for (long long j = nv * W + blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x; j < n; j += stride)
+        acc[1] += static_cast<double>(x[j]) * static_cast<double>(y[j]);
+    const double v[1] = {block_sum((acc[0] + acc[1]) + (acc[2] + acc[3]), sh)};
     store_partials<1>(v, part, kRedBlocks, blockIdx.x);
+}
+
+// 16-byte vectors of both operands when both are aligned, else none.
+template <typename T>
+static inline long long vec_count(int n, const void *a, const void *b) {
+    const bool aligned = ((reinterpret_cast<uintptr_t>(a) | reinterpret_cast<uintptr_t>(b)) & 15) == 0;
+    return aligned ? n / V16<T>::W : 0;
 }
 
 __global__ __launch_bounds__(kFinBlock) void k_fin_store(const double *part, int nb, double *dst) {
@@ -88,11 +134,11 @@ __global__ __launch_bounds__(kFinBlock) void k_fin_store(const double *part, int
 int dot_async(hfh_ctx *ctx, int dtype, int n, const void *x, const void *y, double *dst) {
     const int nb = red_blocks(n);
     if (dtype == HFH_F64)
-        hipLaunchKernelGGL(k_dot<double>, dim3(nb), dim3(kBlock), 0, ctx->stream, n, (const double *)x,
-                           (const double *)y, ctx->red);
+        hipLaunchKernelGGL(k_dot<double>, dim3(nb), dim3(kBlock), 0, ctx->stream, n, vec_count<double>(n, x, y),
+                           (const double *)x, (const double *)y, ctx->red);
     else
-        hipLaunchKernelGGL(k_dot<float>, dim3(nb), dim3(kBlock), 0, ctx->stream, n, (const float *)x,
-                           (const float *)y, ctx->red);
+        hipLaunchKernelGGL(k_dot<float>, dim3(nb), dim3(kBlock), 0, ctx->stream, n, vec_count<float>(n, x, y),
+                           (const float *)x, (const float *)y, ctx->red);
     int rc = launch_check("k_dot");
     if (rc) return rc;
     hipLaunchKernelGGL(k_fin_store, dim3(1), dim3(kFinBlock), 0, ctx->stream, (const double *)ctx->red, nb, dst);
@@ -110,28 +156,68 @@ int fetch(hfh_ctx *ctx, const double *dsrc, double *host, int count) {
 // ----------------------------------------------------------------------------
 // element-wise kernels
 
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_axpy(int n, T *__restrict__ y, const T *__restrict__ x, T a) {
-    for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock)
-        y[i] += a * x[i];
-}
+// Element-wise updates y_i = op(y_i, x_i), every product and sum rounded
+// separately (the reference's CPU loops, bit for bit):
+//   Axpy      y + a x        (CPUsimple_lvector.cc:234-287)
+//   ScaleAdd  a y + x        (CPUsimple_lvector.cc:310-365)
+//   Scale     y a            (CPUsimple_lvector.cc:388-435)
+//   EWMult    y x            (lvector_cpu.cc:571-578)
+// 16-byte vectors, kU per thread in flight; nv as for k_dot.
+template <typename T> struct OpAxpy {
+    T a;
+    __device__ __forceinline__ T operator()(T y, T x) const { return add_rn(y, mul_rn(a, x)); }
+};
+template <typename T> struct OpScaleAdd {
+    T a;
+    __device__ __forceinline__ T operator()(T y, T x) const { return add_rn(mul_rn(a, y), x); }
+};
+template <typename T> struct OpScale {
+    T a;
+    __device__ __forceinline__ T operator()(T y, T) const { return mul_rn(y, a); }
+};
+template <typename T> struct OpMult {
+    __device__ __forceinline__ T operator()(T y, T x) const { return mul_rn(y, x); }
+};
 
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_scaleadd(int n, T *__restrict__ y, T a, const T *__restrict__ x) {
-    for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock)
-        y[i] = a * y[i] + x[i];
-}
+// the scalar of an op, rounded to the vector's type as the reference's
+// lVector<T> interface receives it
+template <typename T> inline void set_scalar(OpAxpy<T> &o, double a) { o.a = static_cast<T>(a); }
+template <typename T> inline void set_scalar(OpScaleAdd<T> &o, double a) { o.a = static_cast<T>(a); }
+template <typename T> inline void set_scalar(OpScale<T> &o, double a) { o.a = static_cast<T>(a); }
+template <typename T> inline void set_scalar(OpMult<T> &, double) {}
 
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_scale(int n, T *__restrict__ y, T a) {
-    for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock)
-        y[i] *= a;
-}
-
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_ewmult(int n, T *__restrict__ y, const T *__restrict__ x) {
-    for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock)
-        y[i] *= x[i];
+template <typename T, typename Op, bool BIN>
+__global__ __launch_bounds__(kBlock) void k_update(int n, long long nv, T *__restrict__ y, const T *__restrict__ x,
+                                                   Op op) {
+    typedef typename V16<T>::type V;
+    constexpr int W = V16<T>::W;
+    V *__restrict__ yv = reinterpret_cast<V *>(y);
+    const V *__restrict__ xv = reinterpret_cast<const V *>(x);
+    const long long stride = static_cast<long long>(gridDim.x) * kBlock;
+    long long i = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x;
+    for (; i + (kU - 1) * stride < nv; i += kU * stride) {
+        V a[kU], b[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            a[u] = yv[i + u * stride];
+            if (BIN) b[u] = xv[i + u * stride];
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u)
+#pragma unroll
+            for (int w = 0; w < W; ++w) a[u][w] = op(a[u][w], BIN ? b[u][w] : T(0));
+#pragma unroll
+        for (int u = 0; u < kU; ++u) yv[i + u * stride] = a[u];
+    }
+    for (; i < nv; i += stride) {
+        V a = yv[i], b;
+        if (BIN) b = xv[i];
+#pragma unroll
+        for (int w = 0; w < W; ++w) a[w] = op(a[w], BIN ? b[w] : T(0));
+        yv[i] = a;
+    }
+    for (long long j = nv * W + blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x; j < n; j += stride)
+        y[j] = op(y[j], BIN ? x[j] : T(0));
 }
 
 template <typename TO, typename TI>
@@ -490,46 +576,46 @@ extern "C" int hfh_nrm2(const hfh_vec *x, double *out) {
     return HFH_OK;
 }
 
+// Launches y = op(y, x) (x may be null for a unary op).
+template <template <typename> class Op, bool BIN>
+static int launch_update(hfh_vec *y, const hfh_vec *x, double a, const char *what) {
+    const int nb = ew_blocks(y->ctx, y->n);
+    hipStream_t s = y->ctx->stream;
+    const void *xd = BIN ? x->d : y->d;
+    if (y->dtype == HFH_F64) {
+        Op<double> op{};
+        set_scalar(op, a);
+        hipLaunchKernelGGL((k_update<double, Op<double>, BIN>), dim3(nb), dim3(kBlock), 0, s, y->n,
+                           vec_count<double>(y->n, y->d, xd), (double *)y->d, (const double *)xd, op);
+    } else {
+        Op<float> op{};
+        set_scalar(op, a);
+        hipLaunchKernelGGL((k_update<float, Op<float>, BIN>), dim3(nb), dim3(kBlock), 0, s, y->n,
+                           vec_count<float>(y->n, y->d, xd), (float *)y->d, (const float *)xd, op);
+    }
+    return launch_check(what);
+}
+
 extern "C" int hfh_axpy(hfh_vec *y, const hfh_vec *x, double a) {
     int rc = same_shape(y, x, "hfh_axpy");
     if (rc || y->n == 0) return rc;
-    const int nb = ew_blocks(y->ctx, y->n);
-    if (y->dtype == HFH_F64)
-        hipLaunchKernelGGL(k_axpy<double>, dim3(nb), dim3(kBlock), 0, y->ctx->stream, y->n, (double *)y->d, (const double *)x->d, a);
-    else
-        hipLaunchKernelGGL(k_axpy<float>, dim3(nb), dim3(kBlock), 0, y->ctx->stream, y->n, (float *)y->d, (const float *)x->d, (float)a);
-    return launch_check("k_axpy");
+    return launch_update<OpAxpy, true>(y, x, a, "hfh_axpy");
 }
 
 extern "C" int hfh_scaleadd(hfh_vec *y, double a, const hfh_vec *x) {
     int rc = same_shape(y, x, "hfh_scaleadd");
     if (rc || y->n == 0) return rc;
-    const int nb = ew_blocks(y->ctx, y->n);
-    if (y->dtype == HFH_F64)
-        hipLaunchKernelGGL(k_scaleadd<double>, dim3(nb), dim3(kBlock), 0, y->ctx->stream, y->n, (double *)y->d, a, (const double *)x->d);
-    else
-        hipLaunchKernelGGL(k_scaleadd<float>, dim3(nb), dim3(kBlock), 0, y->ctx->stream, y->n, (float *)y->d, (float)a, (const float *)x->d);
-    return launch_check("k_scaleadd");
+    return launch_update<OpScaleAdd, true>(y, x, a, "hfh_scaleadd");
 }
 
 extern "C" int hfh_scale(hfh_vec *y, double a) {
     HFH_CHECK(y, HFH_ERR_INVALID, "hfh_scale: null vector");
     if (y->n == 0) return HFH_OK;
-    const int nb = ew_blocks(y->ctx, y->n);
-    if (y->dtype == HFH_F64)
-        hipLaunchKernelGGL(k_scale<double>, dim3(nb), dim3(kBlock), 0, y->ctx->stream, y->n, (double *)y->d, a);
-    else
-        hipLaunchKernelGGL(k_scale<float>, dim3(nb), dim3(kBlock), 0, y->ctx->stream, y->n, (float *)y->d, (float)a);
-    return launch_check("k_scale");
+    return launch_update<OpScale, false>(y, nullptr, a, "hfh_scale");
 }
 
 extern "C" int hfh_ewmult(hfh_vec *y, const hfh_vec *x) {
     int rc = same_shape(y, x, "hfh_ewmult");
     if (rc || y->n == 0) return rc;
-    const int nb = ew_blocks(y->ctx, y->n);
-    if (y->dtype == HFH_F64)
-        hipLaunchKernelGGL(k_ewmult<double>, dim3(nb), dim3(kBlock), 0, y->ctx->stream, y->n, (double *)y->d, (const double *)x->d);
-    else
-        hipLaunchKernelGGL(k_ewmult<float>, dim3(nb), dim3(kBlock), 0, y->ctx->stream, y->n, (float *)y->d, (const float *)x->d);
-    return launch_check("k_ewmult");
+    return launch_update<OpMult, true>(y, x, 0.0, "hfh_ewmult");
 }

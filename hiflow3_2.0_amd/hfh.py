@@ -118,6 +118,7 @@ HIP_API = {
     "hfh_csr_levels": (_i, [_vp, _i, _P(_i), _vp, _vp]),
     "hfh_csr_multicoloring": (_i, [_vp, _P(_i), _vp, _vp, _vp]),
     "hfh_csr_permute": (_i, [_vp, _vp, _vp]),
+    "hfh_problem_assemble": (_i, [_vp, _i, _i, _i, _i, _d, _d, C.c_ulonglong, _vp, _vp]),
     "hfh_csr_swap_diag_front": (_i, [_vp]),
     "hfh_csr_is_sorted": (_i, [_vp, _P(_i)]),
     "hfh_spmv_nodiag": (_i, [_vp, _vp, _vp]),
@@ -709,6 +710,28 @@ def problem(kind, nx, ny=None, nz=None, p0=1.0, p1=0.0, seed=0):
     if lmp().hfl_problem_fill(kind, nx, ny, nz, p0, p1, seed, _p(row), _p(col), _p(val), _p(rhs)):
         raise ValueError("problem generation failed")
     return row, col, val, rhs
+
+
+def assemble(ctx, kind, nx, ny=None, nz=None, p0=1.0, p1=0.0, seed=0, dtype=np.float64):
+    """The system problem() generates on the host, assembled on the device:
+    (CsrMatrix, right-hand side Vector), bit-identical to problem()."""
+    ny = nx if ny is None else ny
+    nz = (1 if kind == Q1_2D else nx) if nz is None else nz
+    n, nnz = C.c_int(), C.c_longlong()
+    if lmp().hfl_problem_size(kind, nx, ny, nz, p0, p1, seed, C.byref(n), C.byref(nnz)):
+        raise ValueError("bad problem size")
+    kc = None
+    if kind == FV_DIFFUSION:
+        kc = np.empty(n.value, np.float64)
+        _lmp_ck(lmp().hfl_fv_cells(nx, ny, nz, p0, seed, _p(kc), None))
+    A = CsrMatrix.__new__(CsrMatrix)
+    A.ctx, A.nrow, A.ncol, A.nnz, A.dtype = ctx, n.value, n.value, nnz.value, np.dtype(dtype)
+    h = C.c_void_p()
+    _ck(hip().hfh_csr_create(ctx.h, n.value, n.value, nnz.value, _dt(dtype), C.byref(h)))
+    A.h = h
+    b = Vector(ctx, n.value, dtype)
+    _ck(hip().hfh_problem_assemble(A.h, kind, nx, ny, nz, p0, p1, seed, _p(kc), b.h))
+    return A, b
 
 
 def mm_read(path):

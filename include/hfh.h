@@ -203,6 +203,18 @@ int hfh_csr_multicoloring(const hfh_csr *A, int *ncolors, int *color_of, int *co
  * ascending.  P: created with A's dims, nnz and dtype; perm (host) is checked
  * to be a permutation. */
 int hfh_csr_permute(const hfh_csr *A, const int *perm, hfh_csr *P);
+
+/* ---- the seeded benchmark systems assembled on the device ----
+ * No reference counterpart: the reference assembles on the host
+ * (CSR_lMatrix::init_structure + add_values, lmatrix_csr_cpu.cc:1101-1184).
+ * The rows of hfl_problem_fill (kinds 1 2D Q1, 2 3D Q1, 3 FV diffusion,
+ * 4 upwind convection-diffusion; same grid and parameters) are built on the
+ * device, bit-identical to the host generator; rhs (optional, device, size n,
+ * A's dtype) receives its seeded right-hand side, also bit-identical.  A is
+ * created with the n and nnz hfl_problem_size reports; kcell: the n FV cell
+ * coefficients from hfl_fv_cells (kind 3 only, host), else NULL. */
+int hfh_problem_assemble(hfh_csr *A, int kind, int nx, int ny, int nz, double p0, double p1,
+                         unsigned long long seed, const double *kcell, hfh_vec *rhs);
 /* ApplylPreconditioner (lpreconditioner.h): out = M^{-1} in */
 int hfh_prec_apply(hfh_prec *P, const hfh_vec *in, hfh_vec *out);
 int hfh_prec_kind(const hfh_prec *P, int *kind);

@@ -166,7 +166,7 @@ def test_csr_upload_validation(pkg, ctx):
         pkg.CsrMatrix(ctx, [0, 2, 1], [0, 1], [1.0, 2.0])    # decreasing row pointer
 
 
-@pytest.mark.parametrize("n", [1, 7, 1000, 1 << 20])
+@pytest.mark.parametrize("n", [1, 7, 1000, 4099, (1 << 20) + 3])
 def test_blas1(pkg, ctx, oracle, n):
     rng = np.random.default_rng(n)
     x = rng.uniform(-1, 1, n)
@@ -175,11 +175,13 @@ def test_blas1(pkg, ctx, oracle, n):
     assert abs(X.dot(Y) - oracle.dot(x, y)) <= 1e-14 * np.dot(np.abs(x), np.abs(y))
     assert X.nrm2() == pytest.approx(oracle.nrm2(x), rel=1e-14)
     a = 0.37
+    # element-wise updates round every product and sum separately, as the
+    # reference's CPU loops do: bit-identical
     Y.axpy(X, a)
-    assert np.allclose(Y.numpy(), oracle.axpy(y, x, a), rtol=2e-16 * 4, atol=1e-16 * 4)
+    assert np.array_equal(Y.numpy(), oracle.axpy(y, x, a))
     y2 = Y.numpy()
     Y.scaleadd(-1.7, X)
-    assert np.allclose(Y.numpy(), oracle.scaleadd(y2, -1.7, x), rtol=1e-15, atol=1e-15)
+    assert np.array_equal(Y.numpy(), oracle.scaleadd(y2, -1.7, x))
     y3 = Y.numpy()
     Y.scale(0.5)
     assert np.array_equal(Y.numpy(), oracle.scale(y3, 0.5))
@@ -190,6 +192,21 @@ def test_blas1(pkg, ctx, oracle, n):
     Xf, Yf = pkg.Vector(ctx, x, np.float32), pkg.Vector(ctx, y, np.float32)
     ref = float(np.dot(x.astype(np.float32).astype(np.float64), y.astype(np.float32).astype(np.float64)))
     assert abs(Xf.dot(Yf) - ref) <= 1e-6 * np.dot(np.abs(x), np.abs(y)) + 1e-30
+    xf, yf = x.astype(np.float32), y.astype(np.float32)
+    Yf.axpy(Xf, a)
+    yf = oracle.axpy(yf, xf, a)
+    assert np.array_equal(Yf.numpy(), yf)
+    Yf.scaleadd(-1.7, Xf)
+    yf = oracle.scaleadd(yf, -1.7, xf)
+    assert np.array_equal(Yf.numpy(), yf)
+    Yf.scale(0.3)
+    yf = oracle.scale(yf, 0.3)
+    assert np.array_equal(Yf.numpy(), yf)
+    Yf.ewmult(Xf)
+    assert np.array_equal(Yf.numpy(), oracle.ewmult(yf, xf))
+    # an operand that is also the target
+    Yf.axpy(Yf, 0.5)
+    assert np.array_equal(Yf.numpy(), oracle.axpy(oracle.ewmult(yf, xf), oracle.ewmult(yf, xf), 0.5))
 
 
 @pytest.mark.parametrize("n", [0, 5, 1 << 16])
