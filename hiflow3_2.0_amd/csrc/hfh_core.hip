@@ -58,11 +58,17 @@ ScopedKernelTimer::~ScopedKernelTimer() {
 // ----------------------------------------------------------------------------
 // reduction helpers
 
-// Number of partials for a stream of n elements (device independent).
+// Workgroups (= partials) of a dot over n elements: as many as the
+// element-wise streams use, up to kDotBlocks.  A single-quantity reduction
+// stores its partials contiguously, and the context's partial region holds
+// kRedBlocks * 8 of them.
+constexpr int kDotBlocks = 4096;
+static_assert(kDotBlocks <= kRedBlocks * 8, "dot partials must fit the context's partial region");
+
 static inline int red_blocks(int n) {
-    long long nb = (static_cast<long long>(n) + kBlock * 4 - 1) / (kBlock * 4);
+    long long nb = (static_cast<long long>(n) + kBlock - 1) / kBlock;
     if (nb < 1) nb = 1;
-    if (nb > kRedBlocks) nb = kRedBlocks;
+    if (nb > kDotBlocks) nb = kDotBlocks;
     return static_cast<int>(nb);
 }
 
