@@ -72,17 +72,6 @@ static inline int red_blocks(int n) {
     return static_cast<int>(nb);
 }
 
-// 16-byte vectors of a stream: 2 doubles or 4 floats.
-template <typename T> struct V16;
-template <> struct V16<double> {
-    typedef double type __attribute__((ext_vector_type(2)));
-    static constexpr int W = 2;
-};
-template <> struct V16<float> {
-    typedef float type __attribute__((ext_vector_type(4)));
-    static constexpr int W = 4;
-};
-
 // Sum of x_i y_i in FP64 (FP32 products are exact in FP64).  nv: 16-byte
 // vectors read as such (0 when an operand is not 16-byte aligned); the
 // elements from nv*W on are read one by one.  Each thread keeps kU vectors of

@@ -194,6 +194,17 @@ __device__ __forceinline__ int xcd_remap(int b, int nb) {
     return base + idx;
 }
 
+// 16-byte vectors of a stream: 2 doubles or 4 floats.
+template <typename T> struct V16;
+template <> struct V16<double> {
+    typedef double type __attribute__((ext_vector_type(2)));
+    static constexpr int W = 2;
+};
+template <> struct V16<float> {
+    typedef float type __attribute__((ext_vector_type(4)));
+    static constexpr int W = 4;
+};
+
 // Separately rounded product and sum.  The contraction pragma sits inside
 // the helpers so that it also holds where they are inlined into code
 // compiled with contraction on.

@@ -48,25 +48,56 @@ static inline int host_check(int step, double v, double first, const hfh_control
 
 // MODE 0: no preconditioner (z = r), 1: Jacobi (z = r.*invd), 2: general (z
 // computed by a separate apply; K2 reduces r.r only).
+// x += alpha p, r -= alpha Ap, partials of r.r and (MODE 1) r.(invD r).
+// The vectors are whole device allocations: nv 16-byte vectors of each are
+// read as such, the n mod W elements after them one by one.
 template <typename T, int MODE>
-__global__ __launch_bounds__(kBlock) void k_pcg_update(int n, T *__restrict__ x, const T *__restrict__ p,
+__device__ __forceinline__ void pcg_update_one(T &x, T p, T &r, T a, T d, T alpha, double &rr, double &rz) {
+    x += alpha * p;
+    const T ri = r + (-alpha) * a;
+    r = ri;
+    rr += static_cast<double>(ri) * static_cast<double>(ri);
+    if (MODE == 1) {
+        const T zi = ri * d;
+        rz += static_cast<double>(ri) * static_cast<double>(zi);
+    }
+}
+
+template <typename T, int MODE>
+__global__ __launch_bounds__(kBlock) void k_pcg_update(int n, long long nv, T *__restrict__ x, const T *__restrict__ p,
                                                        T *__restrict__ r, const T *__restrict__ Ap,
                                                        const T *__restrict__ invd, double *part, int stride,
                                                        const KState *st) {
     if (st->done) return;
+    typedef typename V16<T>::type V;
+    constexpr int W = V16<T>::W;
     __shared__ double sh[kBlock / 64];
     const T alpha = static_cast<T>(static_cast<T>(st->beta) / static_cast<T>(st->pap));
     double rr = 0, rz = 0;
-    for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock) {
-        x[i] += alpha * p[i];
-        const T ri = r[i] + (-alpha) * Ap[i];
-        r[i] = ri;
-        rr += static_cast<double>(ri) * static_cast<double>(ri);
-        if (MODE == 1) {
-            const T zi = ri * invd[i];
-            rz += static_cast<double>(ri) * static_cast<double>(zi);
+    const long long step = static_cast<long long>(gridDim.x) * kBlock;
+    const long long t0 = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x;
+    V *__restrict__ xv = reinterpret_cast<V *>(x);
+    V *__restrict__ rv = reinterpret_cast<V *>(r);
+    const V *__restrict__ pv = reinterpret_cast<const V *>(p);
+    const V *__restrict__ av = reinterpret_cast<const V *>(Ap);
+    const V *__restrict__ dv = reinterpret_cast<const V *>(invd);
+    for (long long i = t0; i < nv; i += step) {
+        V xi = xv[i], ri = rv[i];
+        const V pi = pv[i], ai = av[i];
+        V di{};
+        if (MODE == 1) di = dv[i];
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            T xw = xi[w], rw = ri[w];
+            pcg_update_one<T, MODE>(xw, pi[w], rw, ai[w], MODE == 1 ? di[w] : T(0), alpha, rr, rz);
+            xi[w] = xw;
+            ri[w] = rw;
         }
+        xv[i] = xi;
+        rv[i] = ri;
     }
+    for (long long j = nv * W + t0; j < n; j += step)
+        pcg_update_one<T, MODE>(x[j], p[j], r[j], Ap[j], MODE == 1 ? invd[j] : T(0), alpha, rr, rz);
     double v[2];
     v[0] = block_sum(rr, sh);
     __syncthreads();
@@ -140,20 +171,44 @@ __global__ __launch_bounds__(kFinBlock) void k_fin_rz(const double *part, int st
     }
 }
 
-// p = ratio * p + z  (ScaleAdd, cg.cc:233).  MODE as in k_pcg_update.
+// p = ratio * p + z  (ScaleAdd, cg.cc:233).  MODE as in k_pcg_update; nv
+// as there.
 template <typename T, int MODE>
-__global__ __launch_bounds__(kBlock) void k_pcg_p(int n, T *__restrict__ p, const T *__restrict__ r,
+__device__ __forceinline__ T pcg_dir_one(T p, T r, T z, T d, T ratio) {
+    T zi;
+    if (MODE == 0) zi = r;
+    else if (MODE == 1) zi = r * d;
+    else zi = z;
+    return ratio * p + zi;
+}
+
+template <typename T, int MODE>
+__global__ __launch_bounds__(kBlock) void k_pcg_p(int n, long long nv, T *__restrict__ p, const T *__restrict__ r,
                                                   const T *__restrict__ z, const T *__restrict__ invd,
                                                   const KState *st) {
     if (st->done) return;
+    typedef typename V16<T>::type V;
+    constexpr int W = V16<T>::W;
     const T ratio = static_cast<T>(st->ratio);
-    for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock) {
-        T zi;
-        if (MODE == 0) zi = r[i];
-        else if (MODE == 1) zi = r[i] * invd[i];
-        else zi = z[i];
-        p[i] = ratio * p[i] + zi;
+    const long long step = static_cast<long long>(gridDim.x) * kBlock;
+    const long long t0 = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x;
+    V *__restrict__ pv = reinterpret_cast<V *>(p);
+    const V *__restrict__ rv = reinterpret_cast<const V *>(r);
+    const V *__restrict__ zv = reinterpret_cast<const V *>(z);
+    const V *__restrict__ dv = reinterpret_cast<const V *>(invd);
+    for (long long i = t0; i < nv; i += step) {
+        V pi = pv[i];
+        V ri{}, zi{}, di{};
+        if (MODE != 2) ri = rv[i];
+        if (MODE == 1) di = dv[i];
+        if (MODE == 2) zi = zv[i];
+#pragma unroll
+        for (int w = 0; w < W; ++w) pi[w] = pcg_dir_one<T, MODE>(pi[w], ri[w], zi[w], di[w], ratio);
+        pv[i] = pi;
     }
+    for (long long j = nv * W + t0; j < n; j += step)
+        p[j] = pcg_dir_one<T, MODE>(p[j], MODE != 2 ? r[j] : T(0), MODE == 2 ? z[j] : T(0),
+                                    MODE == 1 ? invd[j] : T(0), ratio);
 }
 
 // Workspace of one solve.
@@ -192,6 +247,7 @@ struct PcgRun {
     KState *st = nullptr;
     double *hist = nullptr;
     int nb = 1;
+    long long nv = 0;             // 16-byte vectors of the element-wise kernels (0: unaligned operand)
 
     // SpMV with fused p.Ap partials; fin_pap sums them
     int spmv_kernel() { return spmv_launch_ex(A, p, Ap, 0, p, spart, &st->done); }
@@ -208,11 +264,11 @@ struct PcgRun {
         hipStream_t s = A->ctx->stream;
         int rc;
         if (mode == 0)
-            hipLaunchKernelGGL((k_pcg_update<T, 0>), dim3(nb), dim3(kBlock), 0, s, n, x, p, r, Ap, invd, part, stride, (const KState *)st);
+            hipLaunchKernelGGL((k_pcg_update<T, 0>), dim3(nb), dim3(kBlock), 0, s, n, nv, x, p, r, Ap, invd, part, stride, (const KState *)st);
         else if (mode == 1)
-            hipLaunchKernelGGL((k_pcg_update<T, 1>), dim3(nb), dim3(kBlock), 0, s, n, x, p, r, Ap, invd, part, stride, (const KState *)st);
+            hipLaunchKernelGGL((k_pcg_update<T, 1>), dim3(nb), dim3(kBlock), 0, s, n, nv, x, p, r, Ap, invd, part, stride, (const KState *)st);
         else
-            hipLaunchKernelGGL((k_pcg_update<T, 2>), dim3(nb), dim3(kBlock), 0, s, n, x, p, r, Ap, invd, part, stride, (const KState *)st);
+            hipLaunchKernelGGL((k_pcg_update<T, 2>), dim3(nb), dim3(kBlock), 0, s, n, nv, x, p, r, Ap, invd, part, stride, (const KState *)st);
         if ((rc = launch_check("k_pcg_update"))) return rc;
         if (mode == 0)
             hipLaunchKernelGGL((k_fin_update<T, 0>), dim3(1), dim3(kFinBlock), 0, s, (const double *)part, stride, nb, st, hist);
@@ -229,11 +285,11 @@ struct PcgRun {
             if ((rc = launch_check("k_fin_rz"))) return rc;
         }
         if (mode == 0)
-            hipLaunchKernelGGL((k_pcg_p<T, 0>), dim3(nb), dim3(kBlock), 0, s, n, p, r, z, invd, st);
+            hipLaunchKernelGGL((k_pcg_p<T, 0>), dim3(nb), dim3(kBlock), 0, s, n, nv, p, r, z, invd, st);
         else if (mode == 1)
-            hipLaunchKernelGGL((k_pcg_p<T, 1>), dim3(nb), dim3(kBlock), 0, s, n, p, r, z, invd, st);
+            hipLaunchKernelGGL((k_pcg_p<T, 1>), dim3(nb), dim3(kBlock), 0, s, n, nv, p, r, z, invd, st);
         else
-            hipLaunchKernelGGL((k_pcg_p<T, 2>), dim3(nb), dim3(kBlock), 0, s, n, p, r, z, invd, st);
+            hipLaunchKernelGGL((k_pcg_p<T, 2>), dim3(nb), dim3(kBlock), 0, s, n, nv, p, r, z, invd, st);
         return launch_check("k_pcg_p");
     }
 
@@ -370,6 +426,12 @@ struct PcgWork {
         hipStream_t s = c->stream;
         const int n = R.n;
         R.x = x;
+        {
+            const uintptr_t al = reinterpret_cast<uintptr_t>(x) | reinterpret_cast<uintptr_t>(R.r) |
+                                 reinterpret_cast<uintptr_t>(R.p) | reinterpret_cast<uintptr_t>(R.z) |
+                                 reinterpret_cast<uintptr_t>(R.Ap) | reinterpret_cast<uintptr_t>(R.invd);
+            R.nv = (al & 15) == 0 ? n / V16<T>::W : 0;
+        }
         hfh_vec vr{c, n, A->dtype, R.r}, vb{c, n, A->dtype, const_cast<T *>(b)}, vz{c, n, A->dtype, R.z},
             vp{c, n, A->dtype, R.p};
         hipEvent_t e0, e1;
