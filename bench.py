@@ -573,25 +573,36 @@ def serve_stream(P, ctx, nreq, rate, chunk, seed0, budget, max_systems):
         if active == 0 and not pending and nxt < nreq:
             time.sleep(max(0.0, min(0.002, arr[nxt] - (time.perf_counter() - t0))))
     wall = time.perf_counter() - t0
+    stats = E.stats()
     E.close()
-    return wall, lat, ok, shapes, inner
+    return wall, lat, ok, shapes, inner, stats
+
+
+def serving_bytes(shapes, inner):
+    """Algorithmic bytes of the inner FP32 iterations of a stream: per row of
+    a W-entry stencil, SpMV 8W + 16 (values, columns, length, p, q), update 32,
+    direction 12."""
+    ns = np.array([a * b * c for a, b, c in shapes])
+    widths = np.array([5 if c == 1 else 7 for a, b, c in shapes])
+    return float(np.sum(ns * inner * (8 * widths + 60))), ns
 
 
 def bench_serving(P, args):
     ctx = P.Context(0)
     budget = 200 << 30
     cal = min(1024, args.serve_requests)
-    wall, lat, ok, _, _ = serve_stream(P, ctx, cal, None, args.serve_chunk, 50_000, budget, args.serve_slots)
+    wall, lat, ok, cshapes, cinner, cstats = serve_stream(P, ctx, cal, None, args.serve_chunk, 50_000, budget,
+                                                          args.serve_slots)
     cap = cal / wall
-    log(f"[serving] closed loop {cal} requests: {cap:.1f} solves/s")
+    cbytes, _ = serving_bytes(cshapes, cinner)
+    closed = {"requests": cal, "solves_per_s": round(cap, 2), "gpu_busy_frac": round(cstats[1] * 1e-3 / wall, 3),
+              "busy_GBps_algorithmic": round(cbytes / (cstats[1] * 1e-3) / 1e9, 1) if cstats[1] else None,
+              "replays": cstats[0], "ms_per_replay": round(cstats[1] / max(cstats[0], 1), 3)}
+    log(f"[serving] closed loop {closed}")
     rate = 0.8 * cap
-    wall, lat, ok, shapes, inner = serve_stream(P, ctx, args.serve_requests, rate, args.serve_chunk, 10_000, budget,
-                                                args.serve_slots)
-    ns = np.array([a * b * c for a, b, c in shapes])
-    # algorithmic bytes of one inner FP32 iteration per row of a W-entry
-    # stencil: SpMV 8W + 16 (values, columns, length, p, q), update 32, direction 12
-    widths = np.array([5 if c == 1 else 7 for a, b, c in shapes])
-    inner_bytes = float(np.sum(ns * inner * (8 * widths + 60)))
+    wall, lat, ok, shapes, inner, stats = serve_stream(P, ctx, args.serve_requests, rate, args.serve_chunk, 10_000,
+                                                       budget, args.serve_slots)
+    inner_bytes, ns = serving_bytes(shapes, inner)
     out = {"requests": args.serve_requests, "arrival_rate_per_s": round(rate, 2),
            "closed_loop_capacity_per_s": round(cap, 2), "solves_per_s": round(args.serve_requests / wall, 2),
            "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 2),
@@ -599,7 +610,10 @@ def bench_serving(P, args):
            "mean_n": int(ns.mean()), "chunk": args.serve_chunk, "slots": args.serve_slots, "hbm_budget_GB": 200,
            "wall_s": round(wall, 2), "inner_iters_total": int(inner.sum()),
            "inner_GBps_algorithmic": round(inner_bytes / wall / 1e9, 1),
-           "inner_frac_of_hbm_peak": round(inner_bytes / wall / 1e9 / HBM_PEAK_GBS, 3)}
+           "inner_frac_of_hbm_peak": round(inner_bytes / wall / 1e9 / HBM_PEAK_GBS, 3),
+           "gpu_busy_frac": round(stats[1] * 1e-3 / wall, 3),
+           "busy_GBps_algorithmic": round(inner_bytes / (stats[1] * 1e-3) / 1e9, 1) if stats[1] else None,
+           "replays": stats[0], "ms_per_replay": round(stats[1] / max(stats[0], 1), 3), "closed_loop": closed}
     if not args.no_cpu:
         # the oracle's MPIR with an FP32 Jacobi-CG inner solver (the engine's
         # algorithm) on the first requests of the same stream, within a budget

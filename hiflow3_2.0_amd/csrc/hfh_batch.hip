@@ -502,6 +502,7 @@ struct hfh_batch {
         bool reported = false;
         double *x = nullptr;  // device solution vector
         SysState final_state{};
+        int seen_inner = 0;   // inner iterations counted into row_iters so far
     };
     std::vector<Slot> slots;
     // state readback, double buffered: replay k's copy is read while replay k+1 runs
@@ -524,6 +525,12 @@ struct hfh_batch {
     std::map<int, int> slot_of;      // id -> slot
     int next_id = 1;
     long long steps = 0;
+    // statistics: GPU time of the replays (events around each graph launch,
+    // read with its state copy) and inner row-iterations (sum of n x inner
+    // iterations) seen in the state copies
+    hipEvent_t ev_g0[2] = {nullptr, nullptr}, ev_g1[2] = {nullptr, nullptr};
+    double busy_ms = 0;
+    long long replays = 0, row_iters = 0;
 };
 
 namespace {
@@ -651,10 +658,17 @@ int enqueue_iterations(hfh_batch *B, int iters) {
 // slot that was collected and reused after the copy was taken).
 int take_readback(hfh_batch *B, int buf) {
     HFH_HIP(hipEventSynchronize(B->ev_rb[buf]));
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, B->ev_g0[buf], B->ev_g1[buf]) == hipSuccess) B->busy_ms += ms;
+    ++B->replays;
     const SysState *h = B->h_st[buf];
     int fin = 0;
     for (size_t k = 0; k < B->slots.size(); ++k) {
         hfh_batch::Slot &S = B->slots[k];
+        if (S.id >= 0 && h[k].id == S.id && h[k].inner_total > S.seen_inner) {
+            B->row_iters += static_cast<long long>(h[k].inner_total - S.seen_inner) * S.n;
+            S.seen_inner = h[k].inner_total;
+        }
         if (S.id < 0 || S.reported || h[k].id != S.id || h[k].phase != P_DONE) continue;
         S.reported = true;
         S.final_state = h[k];
@@ -677,6 +691,8 @@ void release_all(hfh_batch *B) {
         if (B->h_tiles[b]) (void)hipHostFree(B->h_tiles[b]);
         if (B->h_ntiles[b]) (void)hipHostFree(B->h_ntiles[b]);
         if (B->ev_rb[b]) (void)hipEventDestroy(B->ev_rb[b]);
+        if (B->ev_g0[b]) (void)hipEventDestroy(B->ev_g0[b]);
+        if (B->ev_g1[b]) (void)hipEventDestroy(B->ev_g1[b]);
         if (B->ev_tiles[b]) (void)hipEventDestroy(B->ev_tiles[b]);
     }
     if (B->ev_up) (void)hipEventDestroy(B->ev_up);
@@ -723,6 +739,7 @@ extern "C" int hfh_batch_create(hfh_ctx *ctx, const hfh_batch_params *p, hfh_bat
              hipHostMalloc(reinterpret_cast<void **>(&B->h_tiles[b]), sizeof(Tile) * B->tile_cap) == hipSuccess &&
              hipHostMalloc(reinterpret_cast<void **>(&B->h_ntiles[b]), sizeof(int)) == hipSuccess &&
              hipEventCreateWithFlags(&B->ev_rb[b], hipEventDisableTiming) == hipSuccess &&
+             hipEventCreate(&B->ev_g0[b]) == hipSuccess && hipEventCreate(&B->ev_g1[b]) == hipSuccess &&
              hipEventCreateWithFlags(&B->ev_tiles[b], hipEventDisableTiming) == hipSuccess;
     if (!ok) {
         set_error("hfh_batch_create: stream, event or pinned host allocation failed");
@@ -916,10 +933,12 @@ extern "C" int hfh_batch_step(hfh_batch *B, int *active, int *nfinished) {
         if (S.id >= 0 && !S.reported) ++nact;
     int launched = -1;
     if (nact) {
-        HFH_HIP(hipGraphLaunch(B->exec, s));
-        ++B->steps;
         launched = B->rb_next;
         B->rb_next ^= 1;
+        HFH_HIP(hipEventRecord(B->ev_g0[launched], s));
+        HFH_HIP(hipGraphLaunch(B->exec, s));
+        HFH_HIP(hipEventRecord(B->ev_g1[launched], s));
+        ++B->steps;
         HFH_HIP(hipMemcpyAsync(B->h_st[launched], B->d_st, sizeof(SysState) * B->slots.size(),
                                hipMemcpyDeviceToHost, s));
         HFH_HIP(hipEventRecord(B->ev_rb[launched], s));
@@ -984,6 +1003,14 @@ extern "C" int hfh_batch_collect(hfh_batch *B, int id, double *x, hfh_batch_resu
     B->tiles_dirty = true;
     if (rc) return rc;
     return status == HFH_SUCCESS_ABS || status == HFH_SUCCESS_REL ? HFH_OK : HFH_ERR_NOT_CONVERGED;
+}
+
+extern "C" int hfh_batch_stats(const hfh_batch *B, long long *replays, double *busy_ms, long long *row_iters) {
+    HFH_CHECK(B, HFH_ERR_INVALID, "hfh_batch_stats: null engine");
+    if (replays) *replays = B->replays;
+    if (busy_ms) *busy_ms = B->busy_ms;
+    if (row_iters) *row_iters = B->row_iters;
+    return HFH_OK;
 }
 
 extern "C" int hfh_batch_usage(const hfh_batch *B, int *active, long long *used, long long *cap) {

@@ -135,6 +135,7 @@ HIP_API = {
     "hfh_batch_finished": (_i, [_vp, _vp, _i, _P(_i)]),
     "hfh_batch_collect": (_i, [_vp, _i, _vp, _P(BatchResult)]),
     "hfh_batch_usage": (_i, [_vp, _P(_i), _P(_ll), _P(_ll)]),
+    "hfh_batch_stats": (_i, [_vp, _P(_ll), _P(C.c_double), _P(_ll)]),
 }
 
 class LmpParams(C.Structure):
@@ -686,6 +687,12 @@ class BatchEngine:
             _ck(rc)
         return x, {"outer_iters": r.outer_iters, "inner_iters": r.inner_iters, "state": r.state, "res": r.res,
                    "res0": r.res0, "rc": rc}
+
+    def stats(self):
+        """(replays, GPU ms of the replays, inner row-iterations) since creation."""
+        r, ms, ri = _ll(), C.c_double(), _ll()
+        _ck(hip().hfh_batch_stats(self.h, C.byref(r), C.byref(ms), C.byref(ri)))
+        return r.value, ms.value, ri.value
 
     def usage(self):
         a, u, c = C.c_int(), C.c_longlong(), C.c_longlong()

@@ -367,6 +367,11 @@ int hfh_batch_submit_fv(hfh_batch *B, int nx, int ny, int nz, const double *kcel
 int hfh_batch_step(hfh_batch *B, int *active, int *finished);
 /* ids of finished, not yet collected systems (up to cap); *count = all of them */
 int hfh_batch_finished(hfh_batch *B, int *ids, int cap, int *count);
+/* Engine statistics since creation (no reference counterpart): graph
+ * replays, their GPU time in ms (HIP events around each replay) and the inner
+ * row-iterations they advanced (sum over systems of n x inner CG iterations,
+ * as seen in the state copies). */
+int hfh_batch_stats(const hfh_batch *B, long long *replays, double *busy_ms, long long *row_iters);
 /* Copy the solution (x nullable) and outcome of a finished system and free
  * its slot and memory.  Returns HFH_ERR_NOT_CONVERGED for a failed solve. */
 int hfh_batch_collect(hfh_batch *B, int id, double *x, hfh_batch_result *res);

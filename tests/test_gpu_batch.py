@@ -122,6 +122,7 @@ def test_batch_alone_equals_mixed(pkg, ctx):
             s = ids.pop(rid)
             results[s] = E.collect(rid, len(data[s][1]))
         assert steps < 100000
+    replays, busy_ms, row_iters = E.stats()
     E.close()
     for s in specs:
         xa, ia = alone[s]
@@ -129,6 +130,11 @@ def test_batch_alone_equals_mixed(pkg, ctx):
         assert np.array_equal(xa, xm), s
         assert ia == im, s
         assert ia["rc"] == pkg.OK
+    # engine statistics: every replay read back is counted (the last one, run
+    # after the final system finished, is not read back), and the
+    # row-iterations are the sum of n x inner iterations of the systems
+    assert steps - 1 <= replays <= steps and busy_ms > 0
+    assert row_iters == sum(s[0] * s[1] * s[2] * results[s][1]["inner_iters"] for s in specs)
 
 
 def test_batch_budget_admission(pkg, ctx):
