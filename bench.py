@@ -82,21 +82,37 @@ def read_traffic(name):
         return None
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_spmv_baseline(row, col, val, x, budget_s=12.0):
-    """The oracle's OpenMP CSR SpMV (CPUopenmp_CSR_lMatrix::VectorMult) on the host cores."""
+    """The oracle's OpenMP CSR SpMV (CPUopenmp_CSR_lMatrix::VectorMult) on the
+    host cores, and its serial loop (CPUsimple_CSR_lMatrix) on one core: the
+    median of the calls that fit the budget (at least 5, after a warm-up)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     A = O.Csr(row, col, val)
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    y = O.spmv(A, x, omp=True)  # warm-up
-    times = []
-    t_end = time.perf_counter() + budget_s
-    while time.perf_counter() < t_end and len(times) < 50:
-        t0 = time.perf_counter()
-        y = O.spmv(A, x, omp=True)
-        times.append(time.perf_counter() - t0)
-    med = float(np.median(times))
-    return y, med, len(times), threads
+
+    def med(omp, budget):
+        y = O.spmv(A, x, omp=omp)  # warm-up
+        times = []
+        t_end = time.perf_counter() + budget
+        while (time.perf_counter() < t_end or len(times) < 5) and len(times) < 50:
+            t0 = time.perf_counter()
+            y = O.spmv(A, x, omp=omp)
+            times.append(time.perf_counter() - t0)
+        return y, float(np.median(times)), len(times)
+    y, m_omp, reps = med(True, budget_s * 0.6)
+    _, m_one, reps1 = med(False, budget_s * 0.4)
+    return y, m_omp, reps, threads, m_one, reps1
 
 
 def spmv_kernel_name(A):
@@ -138,10 +154,14 @@ def bench_pcg_q1(P, args):
     y_gpu = Yr.numpy()
     cpu = None
     if not args.no_cpu:
-        y_cpu, med, reps, threads = cpu_spmv_baseline(row, col, val, xr, args.cpu_budget)
-        cpu = {"value": spmv_bytes(n, nnz) / med / 1e9, "unit": "GB/s", "cores": threads, "kind": "port",
+        y_cpu, med, reps, threads, med1, reps1 = cpu_spmv_baseline(row, col, val, xr, args.cpu_budget)
+        cpu = {"value": round(spmv_bytes(n, nnz) / med / 1e9, 2), "unit": "GB/s", "cores": threads, "kind": "port",
+               "model": cpu_model(),
                "sample": f"oracle OpenMP CSR SpMV (CPUopenmp_CSR_lMatrix order) on the full 128^3 Q1 matrix, "
-                         f"median of {reps} calls, {med * 1e3:.1f} ms per call"}
+                         f"median of {reps} calls, {med * 1e3:.1f} ms per call",
+               "value_1_core": round(spmv_bytes(n, nnz) / med1 / 1e9, 2),
+               "sample_1_core": f"oracle serial CSR SpMV (CPUsimple_CSR_lMatrix order), median of {reps1} calls, "
+                                f"{med1 * 1e3:.1f} ms per call"}
         rel = float(np.linalg.norm(y_gpu - y_cpu) / np.linalg.norm(y_cpu))
         log(f"[bench] cpu spmv {med * 1e3:.1f} ms ({threads} threads); gpu vs oracle rel {rel:.2e}, "
             f"bitwise {np.array_equal(y_gpu, y_cpu)}")
