@@ -84,6 +84,7 @@ struct hfh_csr {
     int *diag = nullptr;   // diagonal position per row
     bool has_structure = false;
     bool sorted = true;      // columns ascending in every row (false after a diagonal-first swap)
+    unsigned version = 0;    // bumped whenever the values or the structure change
     std::vector<int> h_row;  // host copy of the row pointer (for analysis)
     size_t elem() const { return dtype == HFH_F64 ? 8 : 4; }
 };
@@ -108,6 +109,27 @@ struct hfh_sweep {
     std::vector<int> slice0;   // host: first slice of each colour (ncolors+1)
 };
 
+// Rows of a level-scheduled triangular sweep, sliced in level order: the rows
+// of level l fill slices lslice[l] .. lslice[l+1] (64 rows each, lane = row),
+// each slice column-major and padded to its longest row.  A row's entries are
+// its lower (forward sweep) or upper (backward sweep) ones in stored order;
+// its diagonal value sits beside them.  Values are refilled from the CSR
+// arrays through the position maps when the matrix's values change.
+struct hfh_tsweep {
+    int nslice = 0;
+    long long nent = 0;
+    std::vector<int> lslice;   // host: first slice of each level (nlev + 1)
+    int *lslice_d = nullptr;   // device copy
+    int *ptr = nullptr;        // [nslice + 1] element offset of each slice
+    int *row = nullptr;        // [64 nslice] row of each lane (-1: padding lane)
+    int *len = nullptr;        // [64 nslice] entries of each lane
+    int *col = nullptr;        // [nent] columns (padding: 0)
+    int *pos = nullptr;        // [nent] CSR position of each entry (-1: padding)
+    int *dpos = nullptr;       // [64 nslice] CSR position of each lane's diagonal (-1: padding)
+    void *val = nullptr;       // [nent] values
+    void *dval = nullptr;      // [64 nslice] diagonal values
+};
+
 struct hfh_prec {
     int kind = HFH_PREC_NONE;
     hfh_ctx *ctx = nullptr;
@@ -122,6 +144,8 @@ struct hfh_prec {
     std::vector<int> color_off;     // host, ncolors+1
     hfh_levels lo, up;              // SGS / ILU0 level schedules
     hfh_sweep fw, bw;               // MC-SGS: lower-colour / upper-colour entries per row
+    hfh_tsweep tlo, tup;            // SGS / ILU0: forward / backward sweep rows by level
+    unsigned tversion = 0;          // A->version the sweep values were copied at
     // SGS / ILU0 outside the fused drivers: both sweeps captured once as a
     // hipGraph on internal in/out buffers (one launch instead of one per level)
     void *gin = nullptr, *gout = nullptr;
@@ -183,6 +207,9 @@ int spmv_launch_ex(const hfh_csr *A, const void *x, void *y, int mode, const voi
 // Preconditioner apply on raw device pointers (hfh_prec.hip).  `done`
 // (device, nullable): every kernel returns at once when *done != 0.
 int prec_apply_raw(hfh_prec *P, const void *in, void *out, const int *done = nullptr);
+// Brings a preconditioner's copies of the matrix values up to date (after the
+// matrix's values changed); a no-op while the stream is being captured.
+int prec_sync(hfh_prec *P);
 
 // XCD-aware workgroup remap: consecutive logical tiles land on one XCD so its
 // L2 keeps the shared gather window.

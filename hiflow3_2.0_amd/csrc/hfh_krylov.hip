@@ -537,6 +537,7 @@ extern "C" int hfh_pcg_solve(const hfh_csr *A, hfh_prec *P, const hfh_vec *b, hf
     int rc = check_solve_args(A, P, b, x, ctl, "hfh_pcg_solve");
     if (rc) return rc;
     if (A->nrow == 0) return empty_system(info);
+    if ((rc = prec_sync(P))) return rc;   // before the iterations are captured
     hfh_solve_info inf{};
     if (A->dtype == HFH_F64) {
         PcgWork<double> W;
@@ -855,6 +856,7 @@ extern "C" int hfh_dc_solve_ex(const hfh_csr *Ah, const hfh_csr *Al, hfh_prec *P
               "hfh_dc_solve: vectors must be FP64 of size %d", Ah->nrow);
     HFH_CHECK(!Pl || (Pl->dtype == HFH_F32 && Pl->n == Al->nrow), HFH_ERR_INVALID, "hfh_dc_solve: bad preconditioner");
     HFH_CHECK(outer->maxits > 0 && inner->maxits > 0, HFH_ERR_INVALID, "hfh_dc_solve: maxits must be positive");
+    if (int rs = prec_sync(Pl)) return rs;   // before the inner iterations are captured
     if (Ah->nrow == 0) {
         // an empty system is solved exactly by the empty vector
         if (info) *info = hfh_dc_info{0, 0, HFH_SUCCESS_ABS, 0.0, 0.0, 0.0};

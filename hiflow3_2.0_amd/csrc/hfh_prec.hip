@@ -213,61 +213,80 @@ __global__ __launch_bounds__(kBlock) void k_sweep_fill(long long m, const int *_
     out[k] = j >= 0 ? val[j] : static_cast<T>(0);
 }
 
-// Level-scheduled triangular sweeps.  mode 0: SGS forward (D+L) y = r;
-// mode 1: SGS backward (D+U) z = D y; mode 2: ILU unit-lower forward;
-// mode 3: ILU (D+U) backward.
-template <typename T, int MODE>
-__device__ __forceinline__ void trsv_row(int i, const int *__restrict__ row, const int *__restrict__ col,
-                                         const T *__restrict__ val, const int *__restrict__ diag,
-                                         const T *__restrict__ in, T *out) {
-    const int js = row[i], je = row[i + 1];
-    if (MODE == 0 || MODE == 2) {
-        T acc = in[i];
-        for (int j = js; j < je; ++j) {
-            const int cj = col[j];
-            if (cj >= i) break;
-            acc -= out[cj] * val[j];
-        }
-        if (MODE == 0) acc /= val[diag[i]];
-        out[i] = acc;
-    } else {
-        const T d = val[diag[i]];
-        T acc = out[i];
-        if (MODE == 1) acc *= d;
-        for (int j = js; j < je; ++j) {
-            const int cj = col[j];
-            if (cj > i) acc -= out[cj] * val[j];
-        }
-        out[i] = acc / d;
-    }
-}
+// Level-scheduled triangular sweeps (CPU_CSR_lMatrix::Psgauss_seidel,
+// lmatrix_csr_cpu.cc:542-616; ilu_solve, lmatrix_csr_cpu.cc:2108-2171).
+// mode 0: SGS forward (D+L) y = r; mode 1: SGS backward (D+U) z = D y;
+// mode 2: ILU unit-lower forward; mode 3: ILU (D+U) backward.  One wavefront
+// per slice of a level, lane = row: the lane's row index, entry count,
+// diagonal, columns and values are addressed by (slice, lane) alone, so they
+// arrive together; only the gathers of earlier levels' results wait.  Each
+// product and difference is rounded separately, in stored order, as the CPU
+// loop does.
+constexpr int kTsweepU = 4;
 
 template <typename T, int MODE>
-__global__ __launch_bounds__(kBlock) void k_trsv_level(const int *__restrict__ lrows, int cnt,
-                                                       const int *__restrict__ row, const int *__restrict__ col,
-                                                       const T *__restrict__ val, const int *__restrict__ diag,
-                                                       const T *__restrict__ in, T *out,
-                                                       const int *__restrict__ done) {
+__device__ __forceinline__ void tsweep_slice(int s, int lane, const int *__restrict__ ptr, const int *__restrict__ srow,
+                                             const int *__restrict__ slen, const int *__restrict__ scol,
+                                             const T *__restrict__ sval, const T *__restrict__ dval,
+                                             const T *__restrict__ in, T *out) {
+    const int p = s * 64 + lane;
+    const int i = srow[p];
+    if (i < 0) return;   // padding lane of a level's last slice
+    const int base = ptr[s], L = (ptr[s + 1] - base) >> 6;
+    const int len = slen[p];
+    const T d = dval[p];
+    T acc;
+    if (MODE == 0 || MODE == 2) acc = in[i];
+    else if (MODE == 1) acc = mul_rn(out[i], d);
+    else acc = out[i];
+    for (int k = 0; k < L; k += kTsweepU) {
+        int c[kTsweepU];
+        T v[kTsweepU], xv[kTsweepU];
+#pragma unroll
+        for (int u = 0; u < kTsweepU; ++u) {
+            const int idx = base + min(k + u, L - 1) * 64 + lane;
+            c[u] = scol[idx];
+            v[u] = sval[idx];
+        }
+#pragma unroll
+        for (int u = 0; u < kTsweepU; ++u) xv[u] = out[c[u]];
+#pragma unroll
+        for (int u = 0; u < kTsweepU; ++u)
+            if (k + u < len) acc = add_rn(acc, -mul_rn(xv[u], v[u]));
+    }
+    if (MODE != 2) acc = acc / d;
+    out[i] = acc;
+}
+
+// One level: slices [s0, s1).
+template <typename T, int MODE>
+__global__ __launch_bounds__(kBlock) void k_tsweep(int s0, int s1, const int *__restrict__ ptr,
+                                                   const int *__restrict__ srow, const int *__restrict__ slen,
+                                                   const int *__restrict__ scol, const T *__restrict__ sval,
+                                                   const T *__restrict__ dval, const T *__restrict__ in, T *out,
+                                                   const int *__restrict__ done) {
     if (done && *done) return;
-    const int k = blockIdx.x * kBlock + threadIdx.x;
-    if (k >= cnt) return;
-    trsv_row<T, MODE>(lrows[k], row, col, val, diag, in, out);
+    const int s = s0 + blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+    if (s >= s1) return;
+    tsweep_slice<T, MODE>(s, threadIdx.x & 63, ptr, srow, slen, scol, sval, dval, in, out);
 }
 
 // A run of consecutive small levels [l0, l1) in one workgroup: the barrier
 // between levels makes each level's results visible to the next, so the
 // run costs one launch instead of l1 - l0.
 template <typename T, int MODE>
-__global__ __launch_bounds__(kMergeBlock) void k_trsv_run(const int *__restrict__ lrows,
-                                                          const int *__restrict__ lptr, int l0, int l1,
-                                                          const int *__restrict__ row, const int *__restrict__ col,
-                                                          const T *__restrict__ val, const int *__restrict__ diag,
-                                                          const T *__restrict__ in, T *out,
-                                                          const int *__restrict__ done) {
+__global__ __launch_bounds__(kMergeBlock) void k_tsweep_run(const int *__restrict__ lslice, int l0, int l1,
+                                                            const int *__restrict__ ptr, const int *__restrict__ srow,
+                                                            const int *__restrict__ slen, const int *__restrict__ scol,
+                                                            const T *__restrict__ sval, const T *__restrict__ dval,
+                                                            const T *__restrict__ in, T *out,
+                                                            const int *__restrict__ done) {
     if (done && *done) return;
+    constexpr int W = kMergeBlock / 64;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     for (int l = l0; l < l1; ++l) {
-        const int a = lptr[l], cnt = lptr[l + 1] - a;
-        for (int k = threadIdx.x; k < cnt; k += kMergeBlock) trsv_row<T, MODE>(lrows[a + k], row, col, val, diag, in, out);
+        for (int s = lslice[l] + wave; s < lslice[l + 1]; s += W)
+            tsweep_slice<T, MODE>(s, lane, ptr, srow, slen, scol, sval, dval, in, out);
         __syncthreads();
     }
 }
@@ -442,32 +461,33 @@ static int validate_levels(const hfh_csr *A, const hfh_levels &L, bool upper, co
 }
 
 template <typename T, int MODE>
-static void sweep(const hfh_levels &Lv, const hfh_csr *A, const T *in, T *out, const int *done, hipStream_t s) {
+static void sweep(const hfh_levels &Lv, const hfh_tsweep &W, const T *in, T *out, const int *done, hipStream_t s) {
+    const T *sv = static_cast<const T *>(W.val), *dv = static_cast<const T *>(W.dval);
     for (const auto &run : Lv.runs) {
         if (run.second - run.first > 1) {
-            hipLaunchKernelGGL((k_trsv_run<T, MODE>), dim3(1), dim3(kMergeBlock), 0, s, Lv.rows, Lv.ptr_d, run.first,
-                               run.second, A->row, A->col, (const T *)A->val, A->diag, in, out, done);
+            hipLaunchKernelGGL((k_tsweep_run<T, MODE>), dim3(1), dim3(kMergeBlock), 0, s, W.lslice_d, run.first,
+                               run.second, W.ptr, W.row, W.len, W.col, sv, dv, in, out, done);
             continue;
         }
-        const int a = Lv.ptr[run.first], cnt = Lv.ptr[run.first + 1] - a;
-        if (cnt == 0) continue;
-        hipLaunchKernelGGL((k_trsv_level<T, MODE>), dim3((cnt + kBlock - 1) / kBlock), dim3(kBlock), 0, s, Lv.rows + a,
-                           cnt, A->row, A->col, (const T *)A->val, A->diag, in, out, done);
+        const int s0 = W.lslice[run.first], s1 = W.lslice[run.first + 1];
+        if (s1 == s0) continue;
+        constexpr int WPB = kBlock / 64;
+        hipLaunchKernelGGL((k_tsweep<T, MODE>), dim3((s1 - s0 + WPB - 1) / WPB), dim3(kBlock), 0, s, s0, s1, W.ptr,
+                           W.row, W.len, W.col, sv, dv, in, out, done);
     }
 }
 
 template <typename T>
 static int trsv_sweeps(hfh_prec *P, const T *in, T *out, const int *done) {
-    const hfh_csr *A = P->A;
     hipStream_t s = P->ctx->stream;
     if (P->kind == HFH_PREC_ILU0) {
-        sweep<T, 2>(P->lo, A, in, out, done, s);
-        sweep<T, 3>(P->up, A, in, out, done, s);
+        sweep<T, 2>(P->lo, P->tlo, in, out, done, s);
+        sweep<T, 3>(P->up, P->tup, in, out, done, s);
     } else {
-        sweep<T, 0>(P->lo, A, in, out, done, s);
-        sweep<T, 1>(P->up, A, in, out, done, s);
+        sweep<T, 0>(P->lo, P->tlo, in, out, done, s);
+        sweep<T, 1>(P->up, P->tup, in, out, done, s);
     }
-    return launch_check("k_trsv_level");
+    return launch_check("k_tsweep");
 }
 
 template <typename T>
@@ -596,6 +616,100 @@ static int build_sweep(hfh_prec *P, const hfh_csr *A, bool forward, hfh_sweep &W
     return rc;
 }
 
+// Copies the values of a sliced triangular sweep from the CSR values
+// through its position maps (padding: 0).
+static int tsweep_fill(const hfh_csr *A, hfh_tsweep &W) {
+    hipStream_t s = A->ctx->stream;
+    const long long m = std::max(W.nent, 1LL), md = 64LL * std::max(W.nslice, 1);
+    const dim3 g(static_cast<unsigned>((m + kBlock - 1) / kBlock)), gd(static_cast<unsigned>((md + kBlock - 1) / kBlock));
+    if (A->dtype == HFH_F64) {
+        hipLaunchKernelGGL(k_sweep_fill<double>, g, dim3(kBlock), 0, s, m, W.pos, (const double *)A->val, (double *)W.val);
+        hipLaunchKernelGGL(k_sweep_fill<double>, gd, dim3(kBlock), 0, s, md, W.dpos, (const double *)A->val, (double *)W.dval);
+    } else {
+        hipLaunchKernelGGL(k_sweep_fill<float>, g, dim3(kBlock), 0, s, m, W.pos, (const float *)A->val, (float *)W.val);
+        hipLaunchKernelGGL(k_sweep_fill<float>, gd, dim3(kBlock), 0, s, md, W.dpos, (const float *)A->val, (float *)W.dval);
+    }
+    return launch_check("k_sweep_fill");
+}
+
+// Builds the sliced copy of one sweep from the host structure and the level
+// schedule (ptr, rows: host, level order): forward takes each row's lower
+// entries, backward its upper ones, in stored order.
+static int build_tsweep(const hfh_csr *A, const hfh_levels &Lv, const int *lrows, bool forward, hfh_tsweep &W) {
+    const std::vector<int> &row = A->h_row, &col = A->h_col;
+    const int nlev = Lv.nlev;
+    W.lslice.assign(nlev + 1, 0);
+    for (int l = 0; l < nlev; ++l) W.lslice[l + 1] = W.lslice[l] + (Lv.ptr[l + 1] - Lv.ptr[l] + 63) / 64;
+    const int nsl = W.lslice[nlev];
+    std::vector<int> sptr(nsl + 1, 0), srow(64LL * std::max(nsl, 1), -1), slen(64LL * std::max(nsl, 1), 0),
+        sdpos(64LL * std::max(nsl, 1), -1);
+    auto dep = [&](int c, int i) { return forward ? c < i : c > i; };
+    long long total = 0;
+    for (int l = 0; l < nlev; ++l)
+        for (int sl = W.lslice[l]; sl < W.lslice[l + 1]; ++sl) {
+            int L = 0;
+            for (int lane = 0; lane < 64; ++lane) {
+                const int k = Lv.ptr[l] + (sl - W.lslice[l]) * 64 + lane;
+                if (k >= Lv.ptr[l + 1]) break;
+                const int i = lrows[k], p = sl * 64 + lane;
+                int cnt = 0;
+                for (int j = row[i]; j < row[i + 1]; ++j) {
+                    if (col[j] == i) sdpos[p] = j;
+                    cnt += dep(col[j], i);
+                }
+                srow[p] = i;
+                slen[p] = cnt;
+                L = std::max(L, cnt);
+            }
+            sptr[sl] = static_cast<int>(total);
+            total += 64LL * L;
+            HFH_CHECK(total < (1LL << 31), HFH_ERR_UNSUPPORTED, "sweep copy exceeds 2^31 entries");
+        }
+    sptr[nsl] = static_cast<int>(total);
+    std::vector<int> scol(std::max(total, 1LL), 0), spos(std::max(total, 1LL), -1);
+    for (int sl = 0; sl < nsl; ++sl)
+        for (int lane = 0; lane < 64; ++lane) {
+            const int p = sl * 64 + lane, i = srow[p];
+            if (i < 0) continue;
+            int k = 0;
+            for (int j = row[i]; j < row[i + 1]; ++j)
+                if (dep(col[j], i)) {
+                    const long long idx = sptr[sl] + 64LL * k + lane;
+                    scol[idx] = col[j];
+                    spos[idx] = j;
+                    ++k;
+                }
+        }
+    W.nslice = nsl;
+    W.nent = total;
+    int rc;
+    if ((rc = dev_alloc(reinterpret_cast<void **>(&W.lslice_d), sizeof(int) * W.lslice.size())) ||
+        (rc = dev_alloc(reinterpret_cast<void **>(&W.ptr), sizeof(int) * sptr.size())) ||
+        (rc = dev_alloc(reinterpret_cast<void **>(&W.row), sizeof(int) * srow.size())) ||
+        (rc = dev_alloc(reinterpret_cast<void **>(&W.len), sizeof(int) * slen.size())) ||
+        (rc = dev_alloc(reinterpret_cast<void **>(&W.dpos), sizeof(int) * sdpos.size())) ||
+        (rc = dev_alloc(reinterpret_cast<void **>(&W.col), sizeof(int) * scol.size())) ||
+        (rc = dev_alloc(reinterpret_cast<void **>(&W.pos), sizeof(int) * spos.size())) ||
+        (rc = dev_alloc(&W.val, A->elem() * scol.size())) ||
+        (rc = dev_alloc(&W.dval, A->elem() * sdpos.size())))
+        return rc;
+    HFH_HIP(hipMemcpy(W.lslice_d, W.lslice.data(), sizeof(int) * W.lslice.size(), hipMemcpyHostToDevice));
+    HFH_HIP(hipMemcpy(W.ptr, sptr.data(), sizeof(int) * sptr.size(), hipMemcpyHostToDevice));
+    HFH_HIP(hipMemcpy(W.row, srow.data(), sizeof(int) * srow.size(), hipMemcpyHostToDevice));
+    HFH_HIP(hipMemcpy(W.len, slen.data(), sizeof(int) * slen.size(), hipMemcpyHostToDevice));
+    HFH_HIP(hipMemcpy(W.dpos, sdpos.data(), sizeof(int) * sdpos.size(), hipMemcpyHostToDevice));
+    HFH_HIP(hipMemcpy(W.col, scol.data(), sizeof(int) * scol.size(), hipMemcpyHostToDevice));
+    HFH_HIP(hipMemcpy(W.pos, spos.data(), sizeof(int) * spos.size(), hipMemcpyHostToDevice));
+    return tsweep_fill(A, W);
+}
+
+static void free_tsweep(hfh_tsweep &W) {
+    void *bufs[] = {W.lslice_d, W.ptr, W.row, W.len, W.col, W.pos, W.dpos, W.val, W.dval};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    W = hfh_tsweep();
+}
+
 static void free_sweep(hfh_sweep &W) {
     if (W.ptr) (void)hipFree(W.ptr);
     if (W.len) (void)hipFree(W.len);
@@ -640,10 +754,22 @@ static int trsv_graph(hfh_prec *P, const T *in, T *out) {
     return HFH_OK;
 }
 
+int prec_sync(hfh_prec *P) {
+    if (!P || (P->kind != HFH_PREC_SGS && P->kind != HFH_PREC_ILU0) || P->tversion == P->A->version) return HFH_OK;
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(P->ctx->stream, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone) return HFH_OK;
+    int rc = tsweep_fill(P->A, P->tlo);
+    if (!rc) rc = tsweep_fill(P->A, P->tup);
+    if (!rc) P->tversion = P->A->version;
+    return rc;
+}
+
 int prec_apply_raw(hfh_prec *P, const void *in, void *out, const int *done) {
     hipStream_t s = P->ctx->stream;
     const int n = P->n;
     if (n == 0) return HFH_OK;
+    int rc = prec_sync(P);
+    if (rc) return rc;
     switch (P->kind) {
     case HFH_PREC_NONE:
         HFH_CHECK(!done, HFH_ERR_INVALID, "prec_apply: identity preconditioner inside a driver");
@@ -763,6 +889,13 @@ static int create_trsv(const hfh_csr *A, int kind, int nlev, const int *lp, cons
     if (!rc) rc = upload_levels(P->up, A->nrow, nlu, up, ur, what);
     if (!rc) rc = validate_levels(A, P->lo, false, what);
     if (!rc) rc = validate_levels(A, P->up, true, what);
+    if (!rc) rc = build_tsweep(A, P->lo, lr, true, P->tlo);
+    if (!rc) rc = build_tsweep(A, P->up, ur, false, P->tup);
+    if (!rc) {
+        P->tversion = A->version;
+        const hipError_t e = hipStreamSynchronize(A->ctx->stream);
+        if (e != hipSuccess) rc = hip_fail(e, what);
+    }
     if (rc) { hfh_prec_destroy(P); return rc; }
     *out = P;
     return HFH_OK;
@@ -805,6 +938,8 @@ extern "C" int hfh_prec_destroy(hfh_prec *P) {
     if (P->up.ptr_d) (void)hipFree(P->up.ptr_d);
     free_sweep(P->fw);
     free_sweep(P->bw);
+    free_tsweep(P->tlo);
+    free_tsweep(P->tup);
     if (P->sweep_exec) (void)hipGraphExecDestroy(P->sweep_exec);
     if (P->sweep_graph) (void)hipGraphDestroy(P->sweep_graph);
     if (P->gin) (void)hipFree(P->gin);
@@ -865,6 +1000,7 @@ extern "C" int hfh_csr_ilu0(hfh_csr *A, int nlev, const int *level_ptr, const in
     (void)hipFree(L.ptr_d);
     if (rc) return rc;
     // the SELL copy holds the old values
+    ++A->version;
     A->sell_dirty = true;
     if ((rc = spmv_sell_fill(A))) return rc;
     if (missing) *missing = miss;

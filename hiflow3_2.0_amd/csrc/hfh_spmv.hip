@@ -932,6 +932,7 @@ extern "C" int hfh_csr_upload(hfh_csr *A, const int *row, const int *col, const 
     }
     A->has_structure = true;
     A->sorted = true;
+    ++A->version;
     A->sell_dirty = true;
     if ((rc = spmv_sell_fill(A))) return rc;
     HFH_HIP(hipStreamSynchronize(s));
@@ -947,6 +948,7 @@ int hfh::spmv_refresh_structure(hfh_csr *A) {
                            A->row, A->col, A->diag);
         if ((rc = launch_check("k_find_diag"))) return rc;
     }
+    ++A->version;
     A->sell_dirty = true;
     return spmv_sell_fill(A);
 }
@@ -956,6 +958,7 @@ extern "C" int hfh_csr_upload_values(hfh_csr *A, const void *val) {
     HFH_CHECK(A->has_structure, HFH_ERR_INVALID, "hfh_csr_upload_values: structure not set");
     if (A->nnz == 0) return HFH_OK;
     HFH_HIP(hipMemcpyAsync(A->val, val, A->elem() * static_cast<size_t>(A->nnz), hipMemcpyHostToDevice, A->ctx->stream));
+    ++A->version;
     A->sell_dirty = true;
     int rc = spmv_sell_fill(A);
     if (rc) return rc;
@@ -1000,6 +1003,7 @@ extern "C" int hfh_csr_cast(hfh_csr *dst, const hfh_csr *src) {
         if ((rc = launch_check("k_cast_vals"))) return rc;
     }
     dst->has_structure = true;
+    ++dst->version;
     dst->sell_dirty = true;
     if ((rc = spmv_sell_fill(dst))) return rc;
     HFH_HIP(hipStreamSynchronize(s));

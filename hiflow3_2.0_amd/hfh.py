@@ -401,6 +401,11 @@ class CsrMatrix:
     def spmv_add(self, x, y):
         _ck(hip().hfh_spmv_add(self.h, x.h, y.h))
 
+    def upload_values(self, val):
+        """New values for the same structure (CopyFrom with an unchanged pattern)."""
+        v = np.ascontiguousarray(val, self.dtype)
+        _ck(hip().hfh_csr_upload_values(self.h, _p(v)))
+
     def download(self):
         """(row, col, val) of the device matrix."""
         nr, nc, nz, dt = C.c_int(), C.c_int(), C.c_int(), C.c_int()
