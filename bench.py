@@ -119,7 +119,8 @@ def spmv_kernel_name(A):
     algo, p1 = A.get_spmv()
     names = {1: f"k_spmv_stream<double, tile {p1}>", 2: f"k_spmv_vector<double, {p1} lanes>",
              3: f"k_spmv_wave<double, tile {p1}>", 4: f"k_spmv_sell<double, U={p1}>",
-             5: f"k_spmv_selld<double, UW={p1}>", 6: f"k_spmv_sellp<double, U={p1}>"}
+             5: f"k_spmv_selld<double, UW={p1}>", 6: f"k_spmv_sellp<double, U={p1}>",
+             7: "k_spmv_sellx<double> (x staged in LDS)"}
     return names.get(algo, str(algo))
 
 
@@ -201,15 +202,19 @@ def bench_pcg_q1(P, args):
         f"spmv {avg_spmv_ms * 1e3:.1f} us ({gbs:.0f} GB/s), true rel residual {true_rel:.2e}")
     ceiling = stream_ceiling(P, ctx)
     ceiling["spmv_frac_of_read_stream"] = round(gbs / ceiling["read_GBps"], 4)
-    # the opt-in dictionary-coded columns (1-byte codes): the same SpMV in
-    # fewer bytes; GB/s on the CSR byte count, reported beside the headline
-    A.set_spmv(P.SPMV_SELL_DICT, 1, 0)
-    dict_us = timed_spmv_us(P, ctx, A, X, P.Vector(ctx, n))
+    # the same SpMV over the other bit-identical layouts (plain SpMV
+    # launches, no fused dot); GB/s on the CSR byte count
+    layouts = {}
+    Yl = P.Vector(ctx, n)
+    for name, algo, p1 in (("sell_lds", P.SPMV_SELL_LDS, 0), ("sell_pairs", P.SPMV_SELL_PAIRS, 1),
+                           ("sell_dict", P.SPMV_SELL_DICT, 1)):
+        A.set_spmv(algo, p1, 0)
+        us = timed_spmv_us(P, ctx, A, X, Yl)
+        layouts[name] = {"kernel": spmv_kernel_name(A), "avg_launch_us": round(us, 2),
+                         "GBps_on_csr_bytes": round(b_spmv / (us * 1e-6) / 1e9, 1)}
     A.set_spmv(P.SPMV_AUTO)
-    dict_cols = {"kernel": spmv_kernel_name_for(P.SPMV_SELL_DICT, 1), "avg_launch_us": round(dict_us, 2),
-                 "GBps_on_csr_bytes": round(b_spmv / (dict_us * 1e-6) / 1e9, 1),
-                 "bytes_moved_per_launch": 9 * nnz + 20 * n + 4}
-    log(f"[bench] stream ceiling {ceiling}; dictionary columns {dict_cols}")
+    del Yl
+    log(f"[bench] stream ceiling {ceiling}; layouts {layouts}")
     return {
         "metric": "SpMV GB/s (% HBM peak); CG time to 1e-10 rel; served solves/s + p99",
         "value": round(gbs, 1),
@@ -235,7 +240,7 @@ def bench_pcg_q1(P, args):
                "true_rel_residual": true_rel, "state": infos[-1]["state"]},
         "parity": {"spmv_rel_vs_oracle": rel},
         "stream_ceiling": ceiling,
-        "spmv_dict_columns": dict_cols,
+        "spmv_layouts": layouts,
     }
 
 

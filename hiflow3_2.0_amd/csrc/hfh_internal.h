@@ -81,6 +81,14 @@ struct hfh_csr {
     int sd_ndict = 0;
     unsigned *sd_code = nullptr;   // code words, column-major per slice
     int *sd_wptr = nullptr;    // [nslice+1] word offset of each slice
+    // HFH_SPMV_SELL_LDS: the pair layout's values with each slot's column as a
+    // 16-bit index into its slice's LDS image of x (the slice's distinct
+    // 16-element chunks of x, ascending): chunk position * 16 + column % 16
+    unsigned short *sx_code = nullptr;   // [slots] pair layout, padding 0
+    unsigned short *sx_dcode = nullptr;  // [nrow] code of the diagonal (0xffff: none)
+    int *sx_cptr = nullptr;    // [nslice+1] offset of each slice's chunk list
+    int *sx_cid = nullptr;     // chunk ids (x index / 16)
+    int sx_maxc = 0;           // largest chunk list of a slice
     int *diag = nullptr;   // diagonal position per row
     bool has_structure = false;
     bool sorted = true;      // columns ascending in every row (false after a diagonal-first swap)
@@ -203,6 +211,10 @@ int spmv_sell_fill(hfh_csr *A);
 int spmv_refresh_structure(hfh_csr *A);
 int spmv_launch_ex(const hfh_csr *A, const void *x, void *y, int mode, const void *w, double *part,
                    const int *done);
+// One Jacobi step xn = update(b - sum_{j != r} a_rj x_j) over the LDS-staged
+// SELL copy (A->spmv_algo == HFH_SPMV_SELL_LDS); T is A's value type.
+template <typename T>
+int spmv_lds_jacobi(const hfh_csr *A, const T *x, T *xn, const T *b, const T *invd, bool damped, T w, T omw);
 
 // Preconditioner apply on raw device pointers (hfh_prec.hip).  `done`
 // (device, nullable): every kernel returns at once when *done != 0.
@@ -244,6 +256,16 @@ template <typename T>
 __device__ __forceinline__ T add_rn(T a, T b) {
 #pragma clang fp contract(off)
     return a + b;
+}
+
+// Jacobi step epilogue for row r given s = sum_{j != r} a_rj x_j
+// (DoNormalIteration / DoDampedIteration, linear_solver/jacobi.cc):
+// y = (b_r - s) * inv_diag_r; damped: (1 - w) x_r + w y, each step rounded.
+template <typename T, bool DAMPED>
+__device__ __forceinline__ T jacobi_update(T s, T xr, T br, T dr, T w, T omw) {
+    const T y = mul_rn(add_rn(br, -s), dr);
+    if (!DAMPED) return y;
+    return add_rn(mul_rn(xr, omw), mul_rn(w, y));
 }
 
 template <typename T>

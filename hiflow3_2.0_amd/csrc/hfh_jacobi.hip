@@ -13,8 +13,9 @@
 //   x = y                              normal
 //   x = (1 - w) x + w y                damped (x *= 1-w, then x += w y)
 // is one fused kernel per step: the no-diagonal row sum over the matrix's SELL
-// copy (lane = row) and the update in its epilogue, written to a second
-// buffer (every row reads the old x).  Products and sums are rounded
+// copy (lane = row; with the LDS-staged copy, the SpMV kernel's Jacobi form)
+// and the update in its epilogue, written to a second buffer (every row reads
+// the old x).  Products and sums are rounded
 // separately, so every step is bit-identical to the CPU loop.  Matrices whose
 // SpMV layout is not SELL-64 run a one-thread-per-row CSR step.
 //
@@ -45,14 +46,6 @@ __global__ __launch_bounds__(kBlock) void k_swap_diag_front(int n, const int *__
             col[j] = tc;
             break;
         }
-}
-
-// Epilogue of a step for row r given s = sum_{j != r} a_rj x_j.
-template <typename T, bool DAMPED>
-__device__ __forceinline__ T jacobi_update(T s, T xr, T br, T dr, T w, T omw) {
-    const T y = mul_rn(add_rn(br, -s), dr);
-    if (!DAMPED) return y;
-    return add_rn(mul_rn(xr, omw), mul_rn(w, y));
 }
 
 template <typename T> struct JPair;
@@ -161,6 +154,7 @@ int jacobi_step(const hfh_csr *A, const T *x, T *xn, const T *b, const T *invd, 
     hfh_ctx *c = A->ctx;
     hipStream_t s = c->stream;
     const int n = A->nrow;
+    if (A->spmv_algo == HFH_SPMV_SELL_LDS) return spmv_lds_jacobi<T>(A, x, xn, b, invd, damped, w, omw);
     const bool sell = A->spmv_algo == HFH_SPMV_SELL || A->spmv_algo == HFH_SPMV_SELL_PAIRS;
     if (sell) {
         const bool pairs = A->spmv_algo == HFH_SPMV_SELL_PAIRS;

@@ -18,6 +18,8 @@
 #include "hfh_internal.h"
 
 #include <algorithm>
+#include <cstdint>
+#include <thread>
 
 namespace hfh {
 
@@ -289,7 +291,8 @@ __global__ __launch_bounds__(kBlock) void k_spmv_sellp(
     if (done && *done) return;
     __shared__ double shd[kBlock / 64];
     constexpr int WPB = kBlock / 64;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    // the wave index in a scalar register: slice quantities are then scalar
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     double dsum = 0;
     for (int s = blockIdx.x * WPB + wave; s < nslice; s += gridDim.x * WPB) {
         const int base = sptr[s], L = (sptr[s + 1] - base) >> 6, Q = L >> 1;   // Q slot pairs
@@ -333,6 +336,125 @@ __global__ __launch_bounds__(kBlock) void k_spmv_sellp(
             y[r] = acc;
             if (DOT) dsum += static_cast<double>(acc) * static_cast<double>(w[r]);
         }
+    }
+    if (DOT) {
+        __syncthreads();
+        const double vv[1] = {block_sum(dsum, shd)};
+        store_partials<1>(vv, part, 0, blockIdx.x);
+    }
+}
+
+// SELL pair layout with x staged in LDS.  A wave takes a slice, loads the
+// slice's chunks of x (16 elements each, listed ascending at cid[cptr[s]..])
+// into its own LDS image with 16-byte loads (one piece of 16 bytes per lane
+// and step), and reads x_col from the image at the slot's 16-bit code.  The
+// first slot pair and the single slot of an odd width are loaded before the
+// staging, the next pair while the current one is summed (clamped to the
+// last pair, so every load stays in the slice).  Row sums as k_spmv_sellp
+// (ascending slots, uncontracted), so rows agree bit for bit with the CPU
+// loop.  JAC 1 / 2: a Jacobi step (normal / damped) -- entries whose code is
+// the row's diagonal code are skipped and xn_r = jacobi_update(...) is
+// stored instead of the row sum.  xal: x is 16-byte aligned.
+template <typename T, bool ADD, bool DOT, int JAC, int MAXC>
+__global__ __launch_bounds__(kBlock) void k_spmv_sellx(
+    int nrow, int ncol, int nslice, const int *__restrict__ sptr, const int *__restrict__ rlen,
+    const unsigned short *__restrict__ code, const T *__restrict__ sval, const int *__restrict__ cptr,
+    const int *__restrict__ cid, int xal, const T *__restrict__ x, T *__restrict__ y, const T *__restrict__ w,
+    double *part, const int *__restrict__ done, const unsigned short *__restrict__ dcode, const T *__restrict__ b,
+    const T *__restrict__ invd, T jw, T jomw) {
+    typedef typename Pair<T>::type T2;
+    typedef typename V16<T>::type TV;
+    constexpr int WV = V16<T>::W;              // elements of a 16-byte piece
+    constexpr int PPC = 16 / WV;               // pieces of a chunk
+    constexpr int NP = (MAXC * PPC + 63) / 64; // pieces per lane
+    constexpr int WPB = kBlock / 64;
+    if (done && *done) return;
+    __shared__ T xs_all[WPB][MAXC * 16];
+    __shared__ double shd[WPB];
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    T *xs = xs_all[wave];
+    const unsigned *code2 = reinterpret_cast<const unsigned *>(code);
+    double dsum = 0;
+    for (int s = blockIdx.x * WPB + wave; s < nslice; s += gridDim.x * WPB) {
+        const int base = sptr[s], L = (sptr[s + 1] - base) >> 6, Q = L >> 1, qmax = Q > 0 ? Q - 1 : 0;
+        const int c0 = cptr[s], npc = (cptr[s + 1] - c0) * PPC;
+        const int r = s * 64 + lane;
+        const int len = r < nrow ? rlen[r] : 0;
+        const unsigned dc = JAC && r < nrow ? dcode[r] : 0xffffu;
+        unsigned cw = 0, ct = 0;
+        T2 v = T2(0);
+        T vt = 0;
+        if (Q > 0) {   // wave-uniform
+            cw = __builtin_nontemporal_load(&code2[(base >> 1) + lane]);
+            v = __builtin_nontemporal_load(reinterpret_cast<const T2 *>(&sval[base + 2 * lane]));
+        }
+        if (L & 1) {   // wave-uniform
+            const int idx = base + Q * 128 + lane;
+            ct = __builtin_nontemporal_load(&code[idx]);
+            vt = __builtin_nontemporal_load(&sval[idx]);
+        }
+        int id[NP];
+        TV xv[NP];
+#pragma unroll
+        for (int i = 0; i < NP; ++i) {
+            const int p = lane + 64 * i;
+            id[i] = p < npc ? cid[c0 + p / PPC] : 0;
+        }
+#pragma unroll
+        for (int i = 0; i < NP; ++i) {
+            const int p = lane + 64 * i;
+            const int xi = id[i] * 16 + (p % PPC) * WV;
+            xv[i] = TV(0);
+            if (p < npc && xal && xi + WV <= ncol) xv[i] = *reinterpret_cast<const TV *>(&x[xi]);
+        }
+#pragma unroll
+        for (int i = 0; i < NP; ++i) {
+            const int p = lane + 64 * i;
+            if (p < npc) *reinterpret_cast<TV *>(&xs[p * WV]) = xv[i];
+        }
+        // pieces read element by element: all of them when x is not 16-byte
+        // aligned, else only the one holding x's last entries (a separate
+        // loop keeps the 16-byte path above free of its registers)
+#pragma unroll 1
+        for (int i = 0; i < NP; ++i) {
+            const int p = lane + 64 * i;
+            const int xi = id[i] * 16 + (p % PPC) * WV;
+            if (p < npc && !(xal && xi + WV <= ncol))
+                for (int t = 0; t < WV; ++t)
+                    if (xi + t < ncol) xs[p * WV + t] = x[xi + t];
+        }
+        // the image is private to this wave, whose LDS accesses execute in
+        // order: only the compiler has to keep them in program order
+        __builtin_amdgcn_wave_barrier();
+        asm volatile("" ::: "memory");
+        T acc = 0;
+        for (int q = 0; q < Q; ++q) {
+            const unsigned cq = cw;
+            const T2 vq = v;
+            const int qn = min(q + 1, qmax);
+            cw = __builtin_nontemporal_load(&code2[(base >> 1) + qn * 64 + lane]);
+            v = __builtin_nontemporal_load(reinterpret_cast<const T2 *>(&sval[base + qn * 128 + 2 * lane]));
+            const unsigned ka = cq & 0xffffu, kb = cq >> 16;
+            const T xa = xs[ka], xb = xs[kb];
+            const int k = 2 * q;
+            if (k < len && (!JAC || ka != dc)) acc = add_rn(acc, mul_rn(static_cast<T>(vq.x), xa));
+            if (k + 1 < len && (!JAC || kb != dc)) acc = add_rn(acc, mul_rn(static_cast<T>(vq.y), xb));
+        }
+        if (L & 1) {
+            const T xt = xs[ct];
+            if (2 * Q < len && (!JAC || ct != dc)) acc = add_rn(acc, mul_rn(vt, xt));
+        }
+        if (r < nrow) {
+            if (JAC) {
+                y[r] = jacobi_update<T, JAC == 2>(acc, x[r], b[r], invd[r], jw, jomw);
+            } else {
+                if (ADD) acc = y[r] + acc;
+                y[r] = acc;
+                if (DOT) dsum += static_cast<double>(acc) * static_cast<double>(w[r]);
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        asm volatile("" ::: "memory");
     }
     if (DOT) {
         __syncthreads();
@@ -524,6 +646,90 @@ static std::vector<int> make_row_blocks(const std::vector<int> &row, int tile, i
     return rb;
 }
 
+// Runs f(lo, hi) over the parts of [0, n) on up to 16 host threads.
+template <typename F>
+static void host_parallel(int n, F f) {
+    const unsigned hc = std::thread::hardware_concurrency();
+    const int nt = n < 4096 ? 1 : static_cast<int>(std::min(16u, std::max(1u, hc)));
+    if (nt == 1) {
+        f(0, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    for (int t = 0; t < nt; ++t)
+        th.emplace_back(f, static_cast<int>(static_cast<long long>(n) * t / nt),
+                        static_cast<int>(static_cast<long long>(n) * (t + 1) / nt));
+    for (auto &t : th) t.join();
+}
+
+// Largest chunk list of a slice in the LDS-staged layout (64 chunks of 16
+// elements: 8 KiB of FP64 x per wave).
+constexpr int kLdsMaxChunks = 64;
+
+struct LdsPlan {
+    std::vector<int> cptr, cid;
+    std::vector<unsigned short> code, dcode;
+    int maxc = 0;
+};
+
+// Chunk lists and 16-bit codes of the LDS-staged copy over the slot-pair
+// layout with slice offsets ptr.  False when a slice touches more than
+// kLdsMaxChunks chunks.
+static bool lds_plan(const hfh_csr *A, const std::vector<int> &ptr, LdsPlan &P) {
+    const int n = A->nrow, nsl = (n + 63) / 64;
+    const int *row = A->h_row.data(), *col = A->h_col.data();
+    std::vector<std::vector<int>> lists(nsl);
+    host_parallel(nsl, [&](int lo, int hi) {
+        std::vector<int> ids;
+        for (int s = lo; s < hi; ++s) {
+            ids.clear();
+            for (int j = row[s * 64]; j < row[std::min(n, s * 64 + 64)]; ++j) ids.push_back(col[j] >> 4);
+            std::sort(ids.begin(), ids.end());
+            ids.erase(std::unique(ids.begin(), ids.end()), ids.end());
+            lists[s].assign(ids.begin(), ids.end());
+        }
+    });
+    P.maxc = 0;
+    P.cptr.assign(nsl + 1, 0);
+    for (int s = 0; s < nsl; ++s) {
+        P.maxc = std::max(P.maxc, static_cast<int>(lists[s].size()));
+        P.cptr[s + 1] = P.cptr[s] + static_cast<int>(lists[s].size());
+    }
+    if (P.maxc > kLdsMaxChunks) return false;
+    P.cid.resize(std::max(P.cptr[nsl], 1));
+    P.code.assign(std::max(ptr[nsl], 1), 0);
+    P.dcode.assign(std::max(n, 1), 0xffff);
+    host_parallel(nsl, [&](int lo, int hi) {
+        for (int s = lo; s < hi; ++s) {
+            const std::vector<int> &ids = lists[s];
+            std::copy(ids.begin(), ids.end(), P.cid.begin() + P.cptr[s]);
+            const int base = ptr[s], L = (ptr[s + 1] - base) / 64, Pn = L & ~1;
+            for (int lane = 0; lane < 64 && s * 64 + lane < n; ++lane) {
+                const int r = s * 64 + lane, j0 = row[r], len = row[r + 1] - j0;
+                for (int k = 0; k < len; ++k) {
+                    const int c = col[j0 + k];
+                    const int pos = static_cast<int>(std::lower_bound(ids.begin(), ids.end(), c >> 4) - ids.begin());
+                    const unsigned short cd = static_cast<unsigned short>(pos * 16 + (c & 15));
+                    const int idx = k < Pn ? base + (k >> 1) * 128 + 2 * lane + (k & 1) : base + k * 64 + lane;
+                    P.code[idx] = cd;
+                    if (c == r) P.dcode[r] = cd;
+                }
+            }
+        }
+    });
+    return true;
+}
+
+static void free_lds_copy(hfh_csr *A) {
+    if (A->sx_code) (void)hipFree(A->sx_code);
+    if (A->sx_dcode) (void)hipFree(A->sx_dcode);
+    if (A->sx_cptr) (void)hipFree(A->sx_cptr);
+    if (A->sx_cid) (void)hipFree(A->sx_cid);
+    A->sx_code = A->sx_dcode = nullptr;
+    A->sx_cptr = A->sx_cid = nullptr;
+    A->sx_maxc = 0;
+}
+
 static int round8(long long g) { return static_cast<int>(g >= kNumXcd ? (g + kNumXcd - 1) / kNumXcd * kNumXcd : std::max(g, 1LL)); }
 
 // Configures the SpMV kernel of A (host row pointer in A->h_row).
@@ -531,7 +737,8 @@ int spmv_configure(hfh_csr *A, int algo, int p1, int p2) {
     const int n = A->nrow;
     // buffers of the previous configuration may still be read by queued work
     HFH_HIP(hipStreamSynchronize(A->ctx->stream));
-    if ((algo == HFH_SPMV_SELL || algo == HFH_SPMV_SELL_DICT || algo == HFH_SPMV_SELL_PAIRS) && A->ncol == 0)
+    if ((algo == HFH_SPMV_SELL || algo == HFH_SPMV_SELL_DICT || algo == HFH_SPMV_SELL_PAIRS ||
+         algo == HFH_SPMV_SELL_LDS) && A->ncol == 0)
         algo = HFH_SPMV_STREAM;
     // the dictionary of distinct column offsets (at most 256 codes)
     std::vector<int> dict;
@@ -565,7 +772,10 @@ int spmv_configure(hfh_csr *A, int algo, int p1, int p2) {
     A->sd_code = nullptr;
     A->sd_wptr = nullptr;
     A->sd_ndict = 0;
+    free_lds_copy(A);
     const double mean = n ? static_cast<double>(A->nnz) / n : 0.0;
+    // layout taken when the LDS-staged copy does not fit
+    int fb_algo = HFH_SPMV_SELL_PAIRS, fb_p1 = 1;
     if (algo == 0) {  // automatic choice
         // SELL-64 unless padding to the longest row of each slice would add
         // more than a quarter to the stored entries; 4 column slots in flight
@@ -577,14 +787,20 @@ int spmv_configure(hfh_csr *A, int algo, int p1, int p2) {
             padded += 64LL * L;
         }
         if (A->ncol > 0 && static_cast<double>(padded) <= 1.25 * static_cast<double>(std::max(A->nnz, 1))) {
-            // measured: long FP64 rows run fastest with slot pairs, long FP32
-            // rows with 4 slots in flight, short (stencil) rows with 8
-            if (mean >= 16.0 && A->dtype == HFH_F64) {
-                algo = HFH_SPMV_SELL_PAIRS;
-                p1 = 1;
+            // measured on gfx950: long rows run fastest with x staged in
+            // LDS where the slices' x windows fit (3D Q1, 27 entries per row:
+            // FP64 105 us against 125 us with slot pairs, FP32 64 us against
+            // 83 us), else FP64 with slot pairs and FP32 with 4 slots in
+            // flight; short (stencil) rows share few x entries within a
+            // slice and run as fast with plain SELL-64, 8 slots in flight
+            if (mean >= 16.0) {
+                algo = HFH_SPMV_SELL_LDS;
+                p1 = 0;
+                fb_algo = A->dtype == HFH_F64 ? HFH_SPMV_SELL_PAIRS : HFH_SPMV_SELL;
+                fb_p1 = A->dtype == HFH_F64 ? 1 : 4;
             } else {
                 algo = HFH_SPMV_SELL;
-                p1 = mean >= 16.0 ? 4 : 8;
+                p1 = 8;
             }
         } else {
             algo = HFH_SPMV_STREAM;
@@ -608,8 +824,7 @@ int spmv_configure(hfh_csr *A, int algo, int p1, int p2) {
         A->spmv_algo = HFH_SPMV_STREAM;
         A->spmv_p1 = tile;
         A->nwg = round8(std::max<long long>(g, 1));
-    } else if (algo == HFH_SPMV_SELL || algo == HFH_SPMV_SELL_PAIRS) {
-        const bool pairs = algo == HFH_SPMV_SELL_PAIRS;
+    } else if (algo == HFH_SPMV_SELL || algo == HFH_SPMV_SELL_PAIRS || algo == HFH_SPMV_SELL_LDS) {
         const int nsl = (n + 63) / 64;
         std::vector<int> ptr(nsl + 1, 0), len(std::max(n, 1), 0);
         long long off = 0;
@@ -624,6 +839,12 @@ int spmv_configure(hfh_csr *A, int algo, int p1, int p2) {
             HFH_CHECK(off < (1LL << 31), HFH_ERR_UNSUPPORTED, "spmv: SELL copy exceeds 2^31 entries");
         }
         ptr[nsl] = static_cast<int>(off);
+        LdsPlan plan;
+        if (algo == HFH_SPMV_SELL_LDS && !lds_plan(A, ptr, plan)) {
+            algo = fb_algo;   // a slice's x window does not fit the LDS image
+            p1 = fb_p1;
+        }
+        const bool lds = algo == HFH_SPMV_SELL_LDS, pairs = algo == HFH_SPMV_SELL_PAIRS;
         if (A->sl_ptr) (void)hipFree(A->sl_ptr);
         if (A->sl_len) (void)hipFree(A->sl_len);
         if (A->sl_col) (void)hipFree(A->sl_col);
@@ -633,14 +854,27 @@ int spmv_configure(hfh_csr *A, int algo, int p1, int p2) {
         int rc;
         if ((rc = dev_alloc(reinterpret_cast<void **>(&A->sl_ptr), sizeof(int) * ptr.size())) ||
             (rc = dev_alloc(reinterpret_cast<void **>(&A->sl_len), sizeof(int) * len.size())) ||
-            (rc = dev_alloc(reinterpret_cast<void **>(&A->sl_col), sizeof(int) * static_cast<size_t>(std::max(off, 1LL)))) ||
+            (!lds && (rc = dev_alloc(reinterpret_cast<void **>(&A->sl_col), sizeof(int) * static_cast<size_t>(std::max(off, 1LL))))) ||
             (rc = dev_alloc(&A->sl_val, A->elem() * static_cast<size_t>(std::max(off, 1LL)))))
             return rc;
         HFH_HIP(hipMemcpy(A->sl_ptr, ptr.data(), sizeof(int) * ptr.size(), hipMemcpyHostToDevice));
         HFH_HIP(hipMemcpy(A->sl_len, len.data(), sizeof(int) * len.size(), hipMemcpyHostToDevice));
+        if (lds) {
+            if ((rc = dev_alloc(reinterpret_cast<void **>(&A->sx_code), sizeof(unsigned short) * plan.code.size())) ||
+                (rc = dev_alloc(reinterpret_cast<void **>(&A->sx_dcode), sizeof(unsigned short) * plan.dcode.size())) ||
+                (rc = dev_alloc(reinterpret_cast<void **>(&A->sx_cptr), sizeof(int) * plan.cptr.size())) ||
+                (rc = dev_alloc(reinterpret_cast<void **>(&A->sx_cid), sizeof(int) * plan.cid.size())))
+                return rc;
+            HFH_HIP(hipMemcpy(A->sx_code, plan.code.data(), sizeof(unsigned short) * plan.code.size(), hipMemcpyHostToDevice));
+            HFH_HIP(hipMemcpy(A->sx_dcode, plan.dcode.data(), sizeof(unsigned short) * plan.dcode.size(), hipMemcpyHostToDevice));
+            HFH_HIP(hipMemcpy(A->sx_cptr, plan.cptr.data(), sizeof(int) * plan.cptr.size(), hipMemcpyHostToDevice));
+            HFH_HIP(hipMemcpy(A->sx_cid, plan.cid.data(), sizeof(int) * plan.cid.size(), hipMemcpyHostToDevice));
+            A->sx_maxc = plan.maxc;
+        }
         A->nslice = nsl;
         A->spmv_algo = algo;
-        if (pairs) A->spmv_p1 = p1 == 2 ? 2 : 1;
+        if (lds) A->spmv_p1 = 0;
+        else if (pairs) A->spmv_p1 = p1 == 2 ? 2 : 1;
         else A->spmv_p1 = p1 == 2 || p1 == 8 || p1 == 16 ? p1 : 4;
         const long long need = (nsl + kBlock / 64 - 1) / (kBlock / 64);
         const long long g = p2 > 0 ? std::min<long long>(p2, need) : need;
@@ -748,13 +982,49 @@ int spmv_configure(hfh_csr *A, int algo, int p1, int p2) {
     return HFH_OK;
 }
 
+// The LDS-staged kernel with the image sized to the matrix's largest chunk list.
+template <typename T, bool ADD, bool DOT, int JAC>
+static void lds_launch(const hfh_csr *A, int nwg, const T *x, T *y, const T *w, double *part, const int *done,
+                       const T *b, const T *invd, T jw, T jomw) {
+    hipStream_t s = A->ctx->stream;
+    const dim3 g(nwg), bl(kBlock);
+    const int xal = (reinterpret_cast<uintptr_t>(x) & 15) == 0;
+    const T *sv = static_cast<const T *>(A->sl_val);
+    if (A->sx_maxc <= 16)
+        hipLaunchKernelGGL((k_spmv_sellx<T, ADD, DOT, JAC, 16>), g, bl, 0, s, A->nrow, A->ncol, A->nslice, A->sl_ptr, A->sl_len, A->sx_code, sv, A->sx_cptr, A->sx_cid, xal, x, y, w, part, done, A->sx_dcode, b, invd, jw, jomw);
+    else if (A->sx_maxc <= 32)
+        hipLaunchKernelGGL((k_spmv_sellx<T, ADD, DOT, JAC, 32>), g, bl, 0, s, A->nrow, A->ncol, A->nslice, A->sl_ptr, A->sl_len, A->sx_code, sv, A->sx_cptr, A->sx_cid, xal, x, y, w, part, done, A->sx_dcode, b, invd, jw, jomw);
+    else if (A->sx_maxc <= 48)
+        hipLaunchKernelGGL((k_spmv_sellx<T, ADD, DOT, JAC, 48>), g, bl, 0, s, A->nrow, A->ncol, A->nslice, A->sl_ptr, A->sl_len, A->sx_code, sv, A->sx_cptr, A->sx_cid, xal, x, y, w, part, done, A->sx_dcode, b, invd, jw, jomw);
+    else
+        hipLaunchKernelGGL((k_spmv_sellx<T, ADD, DOT, JAC, kLdsMaxChunks>), g, bl, 0, s, A->nrow, A->ncol, A->nslice, A->sl_ptr, A->sl_len, A->sx_code, sv, A->sx_cptr, A->sx_cid, xal, x, y, w, part, done, A->sx_dcode, b, invd, jw, jomw);
+}
+
+template <typename T>
+int spmv_lds_jacobi(const hfh_csr *A, const T *x, T *xn, const T *b, const T *invd, bool damped, T w, T omw) {
+    HFH_CHECK(A->spmv_algo == HFH_SPMV_SELL_LDS, HFH_ERR_INVALID, "spmv_lds_jacobi: matrix has no LDS-staged copy");
+    if (A->nrow == 0) return HFH_OK;
+    const int need = std::max((A->nslice + kBlock / 64 - 1) / (kBlock / 64), 1);
+    if (damped)
+        lds_launch<T, false, false, 2>(A, need, x, xn, nullptr, nullptr, nullptr, b, invd, w, omw);
+    else
+        lds_launch<T, false, false, 1>(A, need, x, xn, nullptr, nullptr, nullptr, b, invd, w, omw);
+    return launch_check("k_spmv_sellx (Jacobi)");
+}
+template int spmv_lds_jacobi<double>(const hfh_csr *, const double *, double *, const double *, const double *, bool,
+                                     double, double);
+template int spmv_lds_jacobi<float>(const hfh_csr *, const float *, float *, const float *, const float *, bool, float,
+                                    float);
+
 template <typename T, bool ADD, bool DOT>
 static void launch_t(const hfh_csr *A, const void *x, void *y, const void *w, double *part, const int *done) {
     hipStream_t s = A->ctx->stream;
     const dim3 g(A->nwg), bl(kBlock);
     const T *xv = (const T *)x, *wv = (const T *)w, *val = (const T *)A->val;
     T *yv = (T *)y;
-    if (A->spmv_algo == HFH_SPMV_SELL_PAIRS) {
+    if (A->spmv_algo == HFH_SPMV_SELL_LDS) {
+        lds_launch<T, ADD, DOT, 0>(A, A->nwg, xv, yv, wv, part, done, nullptr, nullptr, T(0), T(0));
+    } else if (A->spmv_algo == HFH_SPMV_SELL_PAIRS) {
         if (A->spmv_p1 == 2)
             hipLaunchKernelGGL((k_spmv_sellp<T, ADD, DOT, 2>), g, bl, 0, s, A->nrow, A->nslice, A->sl_ptr, A->sl_len, A->sl_col, (const T *)A->sl_val, xv, yv, wv, part, done);
         else
@@ -798,10 +1068,11 @@ static void launch_t(const hfh_csr *A, const void *x, void *y, const void *w, do
 // Builds the SELL-64 values and columns from the device CSR (after an
 // upload, a cast or a change of algorithm).
 int spmv_sell_fill(hfh_csr *A) {
-    if ((A->spmv_algo != HFH_SPMV_SELL && A->spmv_algo != HFH_SPMV_SELL_DICT && A->spmv_algo != HFH_SPMV_SELL_PAIRS) ||
+    if ((A->spmv_algo != HFH_SPMV_SELL && A->spmv_algo != HFH_SPMV_SELL_DICT && A->spmv_algo != HFH_SPMV_SELL_PAIRS &&
+         A->spmv_algo != HFH_SPMV_SELL_LDS) ||
         !A->sell_dirty || A->nrow == 0)
         return HFH_OK;
-    const int pairs = A->spmv_algo == HFH_SPMV_SELL_PAIRS;
+    const int pairs = A->spmv_algo == HFH_SPMV_SELL_PAIRS || A->spmv_algo == HFH_SPMV_SELL_LDS;
     const dim3 g((A->nslice * 64 + kBlock - 1) / kBlock);
     if (A->dtype == HFH_F64)
         hipLaunchKernelGGL(k_csr_to_sell<double>, g, dim3(kBlock), 0, A->ctx->stream, A->nrow, A->ncol, A->nslice,
@@ -882,6 +1153,7 @@ extern "C" int hfh_csr_destroy(hfh_csr *A) {
     if (A->sd_dict) (void)hipFree(A->sd_dict);
     if (A->sd_code) (void)hipFree(A->sd_code);
     if (A->sd_wptr) (void)hipFree(A->sd_wptr);
+    free_lds_copy(A);
     delete A;
     return HFH_OK;
 }
@@ -1046,7 +1318,7 @@ extern "C" int hfh_csr_invdiag(const hfh_csr *A, hfh_vec *invd) {
 
 extern "C" int hfh_csr_set_spmv(hfh_csr *A, int algo, int p1, int p2) {
     HFH_CHECK(A, HFH_ERR_INVALID, "hfh_csr_set_spmv: null matrix");
-    HFH_CHECK(algo >= 0 && algo <= HFH_SPMV_SELL_PAIRS && p2 >= 0, HFH_ERR_INVALID, "hfh_csr_set_spmv: bad argument");
+    HFH_CHECK(algo >= 0 && algo <= HFH_SPMV_SELL_LDS && p2 >= 0, HFH_ERR_INVALID, "hfh_csr_set_spmv: bad argument");
     A->spmv_algo_req = algo;
     A->spmv_p1_req = p1;
     A->spmv_p2_req = p2;

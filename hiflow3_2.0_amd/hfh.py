@@ -21,7 +21,8 @@ OK, ERR_INVALID, ERR_NOMEM, ERR_HIP, ERR_ZERO_PIVOT, ERR_NOT_CONVERGED, ERR_BUDG
     0, -1, -2, -3, -4, -5, -6, -7
 
 Q1_2D, Q1_3D, FV_DIFFUSION, UPWIND_CONVDIFF = 1, 2, 3, 4
-SPMV_AUTO, SPMV_STREAM, SPMV_VECTOR, SPMV_WAVE, SPMV_SELL, SPMV_SELL_DICT, SPMV_SELL_PAIRS = 0, 1, 2, 3, 4, 5, 6
+SPMV_AUTO, SPMV_STREAM, SPMV_VECTOR, SPMV_WAVE, SPMV_SELL, SPMV_SELL_DICT, SPMV_SELL_PAIRS, SPMV_SELL_LDS = \
+    0, 1, 2, 3, 4, 5, 6, 7
 
 
 class HfhError(RuntimeError):

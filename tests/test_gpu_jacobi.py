@@ -13,7 +13,7 @@ from conftest import relnorm
 
 pytestmark = pytest.mark.gpu
 
-LAYOUTS = [(4, 4), (4, 8), (6, 1), (1, 2048)]   # SELL, SELL, SELL pairs, row blocks (CSR step)
+LAYOUTS = [(4, 4), (4, 8), (6, 1), (7, 0), (1, 2048)]   # SELL, SELL, pairs, LDS-staged, row blocks (CSR step)
 
 
 def matrices(pkg, case):
@@ -126,14 +126,17 @@ def test_solve_matches_oracle(pkg, ctx, oracle, case, dtype, mode, w, inner):
     assert relnorm(info["hist"], oi["hist"]) < tol
 
 
-def test_solve_rows_without_diagonal_and_maxits(pkg, ctx, oracle):
+@pytest.mark.parametrize("algo", [(0, 0), (7, 0)])
+def test_solve_rows_without_diagonal_and_maxits(pkg, ctx, oracle, algo):
     """Rows without a diagonal keep inv_diag 0 (their iterate is 0 after a
     step); the control stops on maxits, as the oracle's does."""
     rng = np.random.default_rng(12)
     row, col, val = no_diag_rows(rng)
     b = rng.uniform(-1, 1, len(row) - 1)
     A = pkg.CsrMatrix(ctx, row, col, val)
+    A.set_spmv(*algo)
     A.swap_diag_front()
+    assert algo[0] == 0 or A.get_spmv()[0] == algo[0]
     S = oracle.swap_diag_front(oracle.Csr(row, col, val))
     X = pkg.Vector(ctx, len(b))
     info = pkg.jacobi(A, pkg.Vector(ctx, b), X, mode=1, w=0.9, inner_iter=2, maxits=9, atol=0.0, rtol=1e-30,

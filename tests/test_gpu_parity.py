@@ -29,8 +29,8 @@ def rand_csr(rng, n, ncol, lens):
 
 
 ALGOS = [(1, 2048), (1, 1024), (1, 512), (1, 4096), (3, 512), (3, 1024), (3, 256), (2, 0), (4, 4), (4, 2), (4, 8), (4, 16),
-         (5, 1), (5, 2), (6, 1), (6, 2)]
-BITWISE = {1, 3, 4, 5, 6}  # row-block, wave-tile and SELL kernels sum rows in the reference order
+         (5, 1), (5, 2), (6, 1), (6, 2), (7, 0)]
+BITWISE = {1, 3, 4, 5, 6, 7}  # row-block, wave-tile and SELL kernels sum rows in the reference order
 
 
 def spmv_case(pkg, ctx, oracle, row, col, val, ncol=None, dtype=np.float64, seed=0, algo=(1, 2048)):
@@ -92,7 +92,7 @@ def test_spmv_ragged_and_long_rows(pkg, ctx, oracle, algo):
     row, col, val = rand_csr(rng, n, ncol, lens)
     y, ref, A, X, Y = spmv_case(pkg, ctx, oracle, row, col, val, ncol=ncol, algo=algo)
     assert relnorm(y, ref) < SPMV_TOL
-    if algo[0] in (4, 5, 6):
+    if algo[0] in (4, 5, 6, 7):
         assert np.array_equal(y, ref)   # SELL sums every row sequentially, long rows included
     elif algo[0] in BITWISE:
         short = np.diff(row) <= min(algo[1], 256)
@@ -129,14 +129,97 @@ def test_spmv_dictionary_columns(pkg, ctx, oracle, dtype, words):
     assert np.array_equal(Y.numpy(), oracle.spmv_add(oracle.Csr(row, col, val.astype(dtype), ncol), X.numpy(), y0))
 
 
+def banded_ragged(rng, n, ncol, halfwidth, maxlen):
+    """Rows of 0..maxlen random columns within +-halfwidth of the row (every
+    97th row empty), clipped to [0, ncol)."""
+    rows, cols = [0], []
+    for r in range(n):
+        k = int(rng.integers(0, maxlen + 1)) if r % 97 else 0
+        lo, hi = max(0, r - halfwidth), min(ncol, r + halfwidth + 1)
+        c = np.sort(rng.choice(np.arange(lo, hi), min(k, hi - lo), replace=False)) if k and hi > lo else []
+        cols.extend(int(v) for v in c)
+        rows.append(len(cols))
+    return np.array(rows, np.int32), np.array(cols, np.int32)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("n,ncol,halfwidth,maxlen", [(1000, 1037, 300, 25), (131, 9, 5, 6), (4099, 4099, 40, 31),
+                                                     (70, 2000, 480, 3)])
+def test_spmv_lds_staged_x(pkg, ctx, oracle, dtype, n, ncol, halfwidth, maxlen):
+    """x staged in LDS per slice: ragged banded rows (empty rows, odd and even
+    slice widths, a partial last slice, ncol not a multiple of the 16-element
+    chunk, fewer columns than one chunk, windows up to 61 chunks)."""
+    rng = np.random.default_rng(n + ncol)
+    row, col = banded_ragged(rng, n, ncol, halfwidth, maxlen)
+    val = rng.uniform(-1, 1, len(col))
+    y, ref, A, X, Y = spmv_case(pkg, ctx, oracle, row, col, val, ncol=ncol, dtype=dtype, algo=(7, 0))
+    assert A.get_spmv()[0] == pkg.SPMV_SELL_LDS
+    assert np.array_equal(y, ref)
+    y0 = rng.uniform(-1, 1, n).astype(dtype)
+    Y.upload(y0)
+    A.spmv_add(X, Y)
+    assert np.array_equal(Y.numpy(), oracle.spmv_add(oracle.Csr(row, col, val.astype(dtype), ncol), X.numpy(), y0))
+
+
+def test_spmv_lds_falls_back_when_a_window_is_too_wide(pkg, ctx, oracle):
+    # one slice touching 65 chunks of x: the pair layout instead
+    rng = np.random.default_rng(11)
+    n, ncol = 64, 65 * 16
+    lens = np.full(n, 2)
+    lens[0] = 65
+    row = np.zeros(n + 1, np.int32)
+    row[1:] = np.cumsum(lens)
+    col = np.concatenate([np.arange(65) * 16] + [np.sort(rng.choice(ncol, 2, replace=False)) for _ in range(n - 1)])
+    val = rng.uniform(-1, 1, len(col))
+    y, ref, A, *_ = spmv_case(pkg, ctx, oracle, row, col.astype(np.int32), val, ncol=ncol, algo=(7, 0))
+    assert A.get_spmv() == (pkg.SPMV_SELL_PAIRS, 1)
+    assert np.array_equal(y, ref)
+
+
+@pytest.mark.parametrize("solver", ["pcg", "gmres"])
+def test_spmv_lds_solvers_identical_to_pairs(pkg, ctx, solver):
+    """Every SpMV of the LDS-staged layout equals the pair layout's bit for
+    bit, so whole solves do too: PCG (the SpMV with its fused dot) and GMRES
+    (an odd n puts every other basis vector at an 8-byte offset, which the
+    staging reads element by element)."""
+    if solver == "pcg":
+        row, col, val, rhs = pkg.problem(pkg.Q1_3D, 17)
+    else:
+        row, col, val, rhs = pkg.problem(pkg.UPWIND_CONVDIFF, 15, p0=1e-2, p1=5.0, seed=2)
+    assert len(rhs) % 2 == 1
+    out = []
+    for algo in ((6, 1), (7, 0)):
+        A = pkg.CsrMatrix(ctx, row, col, val)
+        A.set_spmv(*algo)
+        assert A.get_spmv()[0] == algo[0]
+        P = pkg.Preconditioner.jacobi(A)
+        X = pkg.Vector(ctx, len(rhs))
+        if solver == "pcg":
+            info = pkg.pcg(A, pkg.Vector(ctx, rhs), X, P, maxits=2000, atol=0.0, rtol=1e-10)
+        else:
+            info = pkg.gmres(A, pkg.Vector(ctx, rhs), X, P, m=30, maxits=2000, atol=0.0, rtol=1e-10)
+        out.append((info["iters"], X.numpy()))
+    assert out[0][0] == out[1][0]
+    assert np.array_equal(out[0][1], out[1][1])
+
+
 def test_spmv_layout_choice(pkg, ctx):
-    """The automatic SpMV layout and the dictionary fallback."""
+    """The automatic SpMV layout and the LDS-window and dictionary fallbacks."""
     row, col, val, _ = pkg.problem(pkg.Q1_3D, 12)
-    assert pkg.CsrMatrix(ctx, row, col, val).get_spmv() == (pkg.SPMV_SELL_PAIRS, 1)      # long FP64 rows
-    assert pkg.CsrMatrix(ctx, row, col, val, dtype=np.float32).get_spmv() == (pkg.SPMV_SELL, 4)
+    assert pkg.CsrMatrix(ctx, row, col, val).get_spmv() == (pkg.SPMV_SELL_LDS, 0)        # x windows fit
+    assert pkg.CsrMatrix(ctx, row, col, val, dtype=np.float32).get_spmv() == (pkg.SPMV_SELL_LDS, 0)
     row, col, val, _ = pkg.problem(pkg.FV_DIFFUSION, 12)
     A = pkg.CsrMatrix(ctx, row, col, val)
     assert A.get_spmv() == (pkg.SPMV_SELL, 8)                                           # stencil rows
+    # wide x windows (random columns): pairs / SELL
+    rng = np.random.default_rng(4)
+    row, col, val = rand_csr(rng, 640, 40000, np.full(640, 20))
+    assert pkg.CsrMatrix(ctx, row, col, val, ncol=40000).get_spmv() == (pkg.SPMV_SELL_PAIRS, 1)  # long FP64 rows
+    assert pkg.CsrMatrix(ctx, row, col, val, ncol=40000, dtype=np.float32).get_spmv() == (pkg.SPMV_SELL, 4)
+    row, col, val = rand_csr(rng, 640, 40000, np.full(640, 6))
+    assert pkg.CsrMatrix(ctx, row, col, val, ncol=40000).get_spmv() == (pkg.SPMV_SELL, 8)  # short rows
+    row, col, val, _ = pkg.problem(pkg.FV_DIFFUSION, 12)
+    A = pkg.CsrMatrix(ctx, row, col, val)
     A.set_spmv(pkg.SPMV_SELL_DICT, 2)
     assert A.get_spmv() == (pkg.SPMV_SELL_DICT, 2)                                      # 7 distinct offsets
     rng = np.random.default_rng(5)
