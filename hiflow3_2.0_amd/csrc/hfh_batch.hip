@@ -76,6 +76,46 @@ struct SysDesc {
     double *part;
 };
 
+// Device-side view of a descriptor with its pointers in the global address
+// space: pointers loaded from a descriptor in memory are otherwise generic,
+// and generic (flat) loads make every wait drain the LDS counter as well.
+template <typename T>
+using gptr = __attribute__((address_space(1))) T *;
+
+struct GDesc {
+    int n, nslice, ntiles;
+    gptr<const int> sl_ptr, sl_len, sl_col;
+    gptr<const double> v64;
+    gptr<const float> v32, invd;
+    gptr<const double> b;
+    gptr<double> x, r64;
+    gptr<float> r, c, z, p, q;
+    gptr<double> part;
+};
+
+__device__ __forceinline__ GDesc gview(const SysDesc &d) {
+    GDesc g;
+    g.n = d.n;
+    g.nslice = d.nslice;
+    g.ntiles = d.ntiles;
+    g.sl_ptr = (gptr<const int>)d.sl_ptr;
+    g.sl_len = (gptr<const int>)d.sl_len;
+    g.sl_col = (gptr<const int>)d.sl_col;
+    g.v64 = (gptr<const double>)d.v64;
+    g.v32 = (gptr<const float>)d.v32;
+    g.invd = (gptr<const float>)d.invd;
+    g.b = (gptr<const double>)d.b;
+    g.x = (gptr<double>)d.x;
+    g.r64 = (gptr<double>)d.r64;
+    g.r = (gptr<float>)d.r;
+    g.c = (gptr<float>)d.c;
+    g.z = (gptr<float>)d.z;
+    g.p = (gptr<float>)d.p;
+    g.q = (gptr<float>)d.q;
+    g.part = (gptr<double>)d.part;
+    return g;
+}
+
 struct SysState {
     int phase, status, outer, inner, inner_total;
     int id;   // request id of the occupant (the kernels never change it)
@@ -106,8 +146,7 @@ __device__ __forceinline__ T badd(T a, T b) {
 
 // Row sum of SELL row `lane` of slice `s` against vector v (wave-uniform slice).
 template <typename T, typename V>
-__device__ __forceinline__ T sell_row(const SysDesc &D, const T *__restrict__ val, const V *__restrict__ v, int s,
-                                      int lane, int len) {
+__device__ __forceinline__ T sell_row(const GDesc &D, gptr<const T> val, gptr<const V> v, int s, int lane, int len) {
     const int base = D.sl_ptr[s];
     const int L = (D.sl_ptr[s + 1] - base) >> 6;
     T acc = 0;
@@ -138,25 +177,26 @@ __global__ __launch_bounds__(kBlock) void kb_spmv(const SysDesc *__restrict__ sd
                                                   const Tile *__restrict__ tiles, const int *__restrict__ ntiles) {
     __shared__ double sh[kBlock / 64];
     const int nt = *ntiles;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    // the wave index in a scalar register: slice quantities are then scalar
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     for (int t = blockIdx.x; t < nt; t += gridDim.x) {
         const Tile T = tiles[t];
         const int ph = st[T.sys].phase;
         const bool inner = is_inner_spmv(ph);
         if (!inner && !is_outer_spmv(ph)) continue;   // uniform over the workgroup
-        const SysDesc D = sd[T.sys];
+        const GDesc D = gview(sd[T.sys]);
         double dsum = 0;
         for (int s = T.s0 + wave; s < T.s0 + T.ns; s += kBlock / 64) {
             const int r = s * 64 + lane;
             const int len = r < D.n ? D.sl_len[r] : 0;
             if (inner) {
-                const float acc = sell_row<float, float>(D, D.v32, D.p, s, lane, len);
+                const float acc = sell_row<float, float>(D, D.v32, gptr<const float>(D.p), s, lane, len);
                 if (r < D.n) {
                     D.q[r] = acc;
                     dsum += static_cast<double>(D.p[r]) * static_cast<double>(acc);
                 }
             } else {
-                const double acc = sell_row<double, double>(D, D.v64, D.x, s, lane, len);
+                const double acc = sell_row<double, double>(D, D.v64, gptr<const double>(D.x), s, lane, len);
                 if (r < D.n) {
                     const double rr = D.b[r] - acc;
                     D.r64[r] = rr;
@@ -171,7 +211,7 @@ __global__ __launch_bounds__(kBlock) void kb_spmv(const SysDesc *__restrict__ sd
 }
 
 // Deterministic sum of the system's partials q (0 or 1) over its tiles.
-__device__ __forceinline__ double sum_parts(const SysDesc &D, int q, double *sh) {
+__device__ __forceinline__ double sum_parts(const GDesc &D, int q, double *sh) {
     double v = 0;
     for (int i = threadIdx.x; i < D.ntiles; i += kBlock) v += D.part[2 * i + q];
     const double tot = block_sum(v, sh);
@@ -188,7 +228,7 @@ __global__ __launch_bounds__(kBlock) void kb_fin1(const SysDesc *__restrict__ sd
     const int ph = st[s].phase;
     const bool inner = is_inner_spmv(ph);
     if (!inner && !is_outer_spmv(ph)) return;
-    const SysDesc D = sd[s];
+    const GDesc D = gview(sd[s]);
     const double tot = sum_parts(D, 0, sh);
     if (threadIdx.x != 0) return;
     SysState &S = st[s];
@@ -231,7 +271,7 @@ __global__ __launch_bounds__(kBlock) void kb_update(const SysDesc *__restrict__ 
         const Tile T = tiles[t];
         const int ph = st[T.sys].phase;
         if (ph != P_UPD && ph != P_SETUP) continue;
-        const SysDesc D = sd[T.sys];
+        const GDesc D = gview(sd[T.sys]);
         const float a = static_cast<float>(st[T.sys].alpha);
         const int r0 = T.s0 * 64, r1 = min(D.n, (T.s0 + T.ns) * 64);
         double rr = 0, rz = 0;
@@ -269,7 +309,7 @@ __global__ __launch_bounds__(kBlock) void kb_fin2(const SysDesc *__restrict__ sd
     if (s >= nslots) return;
     const int ph = st[s].phase;
     if (ph != P_UPD && ph != P_SETUP) return;
-    const SysDesc D = sd[s];
+    const GDesc D = gview(sd[s]);
     const double rr = sum_parts(D, 0, sh);
     const double rz = sum_parts(D, 1, sh);
     if (threadIdx.x != 0) return;
@@ -301,7 +341,7 @@ __global__ __launch_bounds__(kBlock) void kb_dir(const SysDesc *__restrict__ sd,
         const Tile T = tiles[t];
         const int ph = st[T.sys].phase;
         if (ph != P_GO && ph != P_START && ph != P_CORRECT) continue;
-        const SysDesc D = sd[T.sys];
+        const GDesc D = gview(sd[T.sys]);
         const int r0 = T.s0 * 64, r1 = min(D.n, (T.s0 + T.ns) * 64);
         if (ph == P_GO) {
             const float beta = static_cast<float>(st[T.sys].beta);

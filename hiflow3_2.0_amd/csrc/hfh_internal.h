@@ -81,14 +81,16 @@ struct hfh_csr {
     int sd_ndict = 0;
     unsigned *sd_code = nullptr;   // code words, column-major per slice
     int *sd_wptr = nullptr;    // [nslice+1] word offset of each slice
+    int sd_code0 = 0;          // code of offset 0 (the diagonal)
     // HFH_SPMV_SELL_LDS: the pair layout's values with each slot's column as a
     // 16-bit index into its slice's LDS image of x (the slice's distinct
-    // 16-element chunks of x, ascending): chunk position * 16 + column % 16
+    // chunks of x, ascending): chunk position * sx_chunk + column % sx_chunk
     unsigned short *sx_code = nullptr;   // [slots] pair layout, padding 0
     unsigned short *sx_dcode = nullptr;  // [nrow] code of the diagonal (0xffff: none)
     int *sx_cptr = nullptr;    // [nslice+1] offset of each slice's chunk list
-    int *sx_cid = nullptr;     // chunk ids (x index / 16)
-    int sx_maxc = 0;           // largest chunk list of a slice
+    int *sx_cid = nullptr;     // chunk ids (x index / sx_chunk)
+    int sx_chunk = 0;          // elements per chunk (16)
+    int sx_img = 0;            // largest image of a slice (chunks * sx_chunk)
     int *diag = nullptr;   // diagonal position per row
     bool has_structure = false;
     bool sorted = true;      // columns ascending in every row (false after a diagonal-first swap)
@@ -115,6 +117,13 @@ struct hfh_sweep {
     int *col = nullptr;        // padded columns (padding: a row of the same slice)
     void *val = nullptr;       // padded values (padding 0)
     std::vector<int> slice0;   // host: first slice of each colour (ncolors+1)
+    // with at most 256 distinct offsets col - row over the sweep's entries,
+    // the columns are one-byte codes into `dict` instead (col is null):
+    // code word w of a lane holds slots 4w..4w+3, slice widths padded to 4
+    unsigned *code = nullptr;  // code words
+    int *wptr = nullptr;       // [nslice+1] word offset of each slice
+    int *dict = nullptr;       // [ndict] offsets, ascending (0 included: padding)
+    int ndict = 0;
 };
 
 // Rows of a level-scheduled triangular sweep, sliced in level order: the rows
@@ -212,9 +221,10 @@ int spmv_refresh_structure(hfh_csr *A);
 int spmv_launch_ex(const hfh_csr *A, const void *x, void *y, int mode, const void *w, double *part,
                    const int *done);
 // One Jacobi step xn = update(b - sum_{j != r} a_rj x_j) over the LDS-staged
-// SELL copy (A->spmv_algo == HFH_SPMV_SELL_LDS); T is A's value type.
+// or the dictionary-coded SELL copy (A->spmv_algo HFH_SPMV_SELL_LDS or
+// HFH_SPMV_SELL_DICT); T is A's value type.
 template <typename T>
-int spmv_lds_jacobi(const hfh_csr *A, const T *x, T *xn, const T *b, const T *invd, bool damped, T w, T omw);
+int spmv_coded_jacobi(const hfh_csr *A, const T *x, T *xn, const T *b, const T *invd, bool damped, T w, T omw);
 
 // Preconditioner apply on raw device pointers (hfh_prec.hip).  `done`
 // (device, nullable): every kernel returns at once when *done != 0.

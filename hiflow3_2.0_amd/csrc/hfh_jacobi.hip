@@ -13,7 +13,8 @@
 //   x = y                              normal
 //   x = (1 - w) x + w y                damped (x *= 1-w, then x += w y)
 // is one fused kernel per step: the no-diagonal row sum over the matrix's SELL
-// copy (lane = row; with the LDS-staged copy, the SpMV kernel's Jacobi form)
+// copy (lane = row; with the LDS-staged or dictionary-coded copy, the SpMV
+// kernel's Jacobi form)
 // and the update in its epilogue, written to a second buffer (every row reads
 // the old x).  Products and sums are rounded
 // separately, so every step is bit-identical to the CPU loop.  Matrices whose
@@ -64,7 +65,7 @@ __global__ __launch_bounds__(kBlock) void k_jacobi_sell(int nrow, int nslice, co
                                                         const T *__restrict__ invd, T w, T omw) {
     constexpr int WPB = kBlock / 64;
     constexpr int U = 4;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     for (int s = blockIdx.x * WPB + wave; s < nslice; s += gridDim.x * WPB) {
         const int base = sptr[s], L = (sptr[s + 1] - base) >> 6;
         const int r = s * 64 + lane;
@@ -154,7 +155,8 @@ int jacobi_step(const hfh_csr *A, const T *x, T *xn, const T *b, const T *invd, 
     hfh_ctx *c = A->ctx;
     hipStream_t s = c->stream;
     const int n = A->nrow;
-    if (A->spmv_algo == HFH_SPMV_SELL_LDS) return spmv_lds_jacobi<T>(A, x, xn, b, invd, damped, w, omw);
+    if (A->spmv_algo == HFH_SPMV_SELL_LDS || A->spmv_algo == HFH_SPMV_SELL_DICT)
+        return spmv_coded_jacobi<T>(A, x, xn, b, invd, damped, w, omw);
     const bool sell = A->spmv_algo == HFH_SPMV_SELL || A->spmv_algo == HFH_SPMV_SELL_PAIRS;
     if (sell) {
         const bool pairs = A->spmv_algo == HFH_SPMV_SELL_PAIRS;

@@ -615,7 +615,11 @@ __global__ __launch_bounds__(kBlock) void k_multi_axpy(int n, T *__restrict__ x,
                                                        Coeffs c) {
     for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock) {
         T xi = x[i];
-        for (int j = 0; j < k1; ++j) xi += static_cast<T>(c.y[j]) * V[j][i];
+        for (int j = 0; j < k1; ++j) {
+            // the basis pointers are device allocations: global loads, not flat
+            const __attribute__((address_space(1))) T *vj = (const __attribute__((address_space(1))) T *)V[j];
+            xi += static_cast<T>(c.y[j]) * vj[i];
+        }
         x[i] = xi;
     }
 }

@@ -107,21 +107,52 @@ __global__ __launch_bounds__(kBlock) void k_mc_bwd(int c, int ncolors, ColorOffs
 // per slice of 64 rows of colour c, lane = row, so the value/column streams
 // and the gathers of neighbouring rows are contiguous.  Each row's entries
 // and its block-by-block summation are those of k_mc_fwd / k_mc_bwd.
-constexpr int kSweepU = 8;   // column slots in flight
+constexpr int kSweepU = 8;   // column slots in flight (two code words when coded)
 
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_mcs_fwd(int c, ColorOffsets co, bool last, int slice0, int nsl,
+// Columns of the kSweepU slots k.. of row i: stored (scol), or row + the
+// offset coded in the slot's byte of its code word (codes: kSweepU / 4 words
+// per step, the step's word index clamped to the slice's last word, the
+// column clamped to [0, n): padding slots and rows past the colour read a
+// valid entry, and their products are not summed).
+template <bool CODED>
+__device__ __forceinline__ void sweep_cols(int (&cj)[kSweepU], int k, int L, int base, int wbase, int lane, int i, int n,
+                                           const int *__restrict__ scol, const unsigned *__restrict__ code,
+                                           const int *dict) {
+    if (!CODED) {
+#pragma unroll
+        for (int u = 0; u < kSweepU; ++u) cj[u] = __builtin_nontemporal_load(&scol[base + min(k + u, L - 1) * 64 + lane]);
+        return;
+    }
+    const int W = L >> 2;
+#pragma unroll
+    for (int q = 0; q < kSweepU / 4; ++q) {
+        const unsigned cw = __builtin_nontemporal_load(&code[wbase + min((k >> 2) + q, W - 1) * 64 + lane]);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) cj[4 * q + t] = min(i + dict[(cw >> (8 * t)) & 255u], n - 1);
+    }
+}
+
+template <typename T, bool CODED>
+__global__ __launch_bounds__(kBlock) void k_mcs_fwd(int c, ColorOffsets co, bool last, int slice0, int nsl, int n,
                                                     const int *__restrict__ sptr, const int *__restrict__ slen,
-                                                    const int *__restrict__ scol, const T *__restrict__ sval,
-                                                    const T *__restrict__ in, const T *__restrict__ idv,
-                                                    const T *__restrict__ dv, T *out, const int *__restrict__ done) {
+                                                    const int *__restrict__ scol, const unsigned *__restrict__ code,
+                                                    const int *__restrict__ wptr, const int *__restrict__ gdict,
+                                                    int ndict, const T *__restrict__ sval, const T *__restrict__ in,
+                                                    const T *__restrict__ idv, const T *__restrict__ dv, T *out,
+                                                    const int *__restrict__ done) {
     if (done && *done) return;
-    const int w = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    __shared__ int dict[CODED ? 256 : 1];
+    if (CODED) {
+        for (int t = threadIdx.x; t < ndict; t += kBlock) dict[t] = gdict[t];
+        __syncthreads();
+    }
+    const int w = blockIdx.x * (kBlock / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (w >= nsl) return;
     const int s = slice0 + w;
     const int i = co.off[c] + w * 64 + lane;
     const bool valid = i < co.off[c + 1];
     const int base = sptr[s], L = (sptr[s + 1] - base) >> 6;
+    const int wbase = CODED ? wptr[s] : 0;
     const int len = valid ? slen[i] : 0;
     T chunk = valid ? in[i] * static_cast<T>(-1) : static_cast<T>(0);
     int blk = 0, blk_end = co.off[1];
@@ -129,12 +160,9 @@ __global__ __launch_bounds__(kBlock) void k_mcs_fwd(int c, ColorOffsets co, bool
     for (int k = 0; k < L; k += kSweepU) {
         int cj[kSweepU];
         T v[kSweepU], xv[kSweepU];
+        sweep_cols<CODED>(cj, k, L, base, wbase, lane, i, n, scol, code, dict);
 #pragma unroll
-        for (int u = 0; u < kSweepU; ++u) {
-            const int idx = base + min(k + u, L - 1) * 64 + lane;
-            cj[u] = __builtin_nontemporal_load(&scol[idx]);
-            v[u] = __builtin_nontemporal_load(&sval[idx]);
-        }
+        for (int u = 0; u < kSweepU; ++u) v[u] = __builtin_nontemporal_load(&sval[base + min(k + u, L - 1) * 64 + lane]);
 #pragma unroll
         for (int u = 0; u < kSweepU; ++u) xv[u] = out[cj[u]];
 #pragma unroll
@@ -158,19 +186,26 @@ __global__ __launch_bounds__(kBlock) void k_mcs_fwd(int c, ColorOffsets co, bool
 
 // Backward step of colour c: entries stored block by block from the highest
 // colour down, ascending columns within a block.
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_mcs_bwd(int c, int ncolors, ColorOffsets co, int slice0, int nsl,
+template <typename T, bool CODED>
+__global__ __launch_bounds__(kBlock) void k_mcs_bwd(int c, int ncolors, ColorOffsets co, int slice0, int nsl, int n,
                                                     const int *__restrict__ sptr, const int *__restrict__ slen,
-                                                    const int *__restrict__ scol, const T *__restrict__ sval,
-                                                    const T *__restrict__ idv, const T *__restrict__ dv, T *out,
-                                                    const int *__restrict__ done) {
+                                                    const int *__restrict__ scol, const unsigned *__restrict__ code,
+                                                    const int *__restrict__ wptr, const int *__restrict__ gdict,
+                                                    int ndict, const T *__restrict__ sval, const T *__restrict__ idv,
+                                                    const T *__restrict__ dv, T *out, const int *__restrict__ done) {
     if (done && *done) return;
-    const int w = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    __shared__ int dict[CODED ? 256 : 1];
+    if (CODED) {
+        for (int t = threadIdx.x; t < ndict; t += kBlock) dict[t] = gdict[t];
+        __syncthreads();
+    }
+    const int w = blockIdx.x * (kBlock / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (w >= nsl) return;
     const int s = slice0 + w;
     const int i = co.off[c] + w * 64 + lane;
     const bool valid = i < co.off[c + 1];
     const int base = sptr[s], L = (sptr[s + 1] - base) >> 6;
+    const int wbase = CODED ? wptr[s] : 0;
     const int len = valid ? slen[i] : 0;
     T chunk = valid ? out[i] * dv[i] : static_cast<T>(0);
     int blk = ncolors - 1, blk_lo = co.off[ncolors - 1];
@@ -178,12 +213,9 @@ __global__ __launch_bounds__(kBlock) void k_mcs_bwd(int c, int ncolors, ColorOff
     for (int k = 0; k < L; k += kSweepU) {
         int cj[kSweepU];
         T v[kSweepU], xv[kSweepU];
+        sweep_cols<CODED>(cj, k, L, base, wbase, lane, i, n, scol, code, dict);
 #pragma unroll
-        for (int u = 0; u < kSweepU; ++u) {
-            const int idx = base + min(k + u, L - 1) * 64 + lane;
-            cj[u] = __builtin_nontemporal_load(&scol[idx]);
-            v[u] = __builtin_nontemporal_load(&sval[idx]);
-        }
+        for (int u = 0; u < kSweepU; ++u) v[u] = __builtin_nontemporal_load(&sval[base + min(k + u, L - 1) * 64 + lane]);
 #pragma unroll
         for (int u = 0; u < kSweepU; ++u) xv[u] = out[cj[u]];
 #pragma unroll
@@ -283,7 +315,7 @@ __global__ __launch_bounds__(kMergeBlock) void k_tsweep_run(const int *__restric
                                                             const int *__restrict__ done) {
     if (done && *done) return;
     constexpr int W = kMergeBlock / 64;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     for (int l = l0; l < l1; ++l) {
         for (int s = lslice[l] + wave; s < lslice[l + 1]; s += W)
             tsweep_slice<T, MODE>(s, lane, ptr, srow, slen, scol, sval, dval, in, out);
@@ -506,16 +538,30 @@ static int mcsgs_apply(hfh_prec *P, const T *in, T *out, const int *done) {
                                co.off[1], in, idv, out, done);
         } else {
             const int s0 = P->fw.slice0[c], nsl = P->fw.slice0[c + 1] - s0;
-            hipLaunchKernelGGL(k_mcs_fwd<T>, dim3((nsl + WPB - 1) / WPB), dim3(kBlock), 0, s, c, co, c == nc - 1, s0,
-                               nsl, P->fw.ptr, P->fw.len, P->fw.col, (const T *)P->fw.val, in, idv, dv, out, done);
+            const hfh_sweep &W = P->fw;
+            if (W.code)
+                hipLaunchKernelGGL((k_mcs_fwd<T, true>), dim3((nsl + WPB - 1) / WPB), dim3(kBlock), 0, s, c, co, c == nc - 1,
+                                   s0, nsl, P->n, W.ptr, W.len, W.col, W.code, W.wptr, W.dict, W.ndict, (const T *)W.val, in,
+                                   idv, dv, out, done);
+            else
+                hipLaunchKernelGGL((k_mcs_fwd<T, false>), dim3((nsl + WPB - 1) / WPB), dim3(kBlock), 0, s, c, co, c == nc - 1,
+                                   s0, nsl, P->n, W.ptr, W.len, W.col, W.code, W.wptr, W.dict, W.ndict, (const T *)W.val, in,
+                                   idv, dv, out, done);
         }
     }
     for (int c = nc - 2; c >= 0; --c) {
         const int cnt = co.off[c + 1] - co.off[c];
         if (cnt == 0) continue;
         const int s0 = P->bw.slice0[c], nsl = P->bw.slice0[c + 1] - s0;
-        hipLaunchKernelGGL(k_mcs_bwd<T>, dim3((nsl + WPB - 1) / WPB), dim3(kBlock), 0, s, c, nc, co, s0, nsl,
-                           P->bw.ptr, P->bw.len, P->bw.col, (const T *)P->bw.val, idv, dv, out, done);
+        const hfh_sweep &W = P->bw;
+        if (W.code)
+            hipLaunchKernelGGL((k_mcs_bwd<T, true>), dim3((nsl + WPB - 1) / WPB), dim3(kBlock), 0, s, c, nc, co, s0, nsl,
+                               P->n, W.ptr, W.len, W.col, W.code, W.wptr, W.dict, W.ndict, (const T *)W.val, idv, dv, out,
+                               done);
+        else
+            hipLaunchKernelGGL((k_mcs_bwd<T, false>), dim3((nsl + WPB - 1) / WPB), dim3(kBlock), 0, s, c, nc, co, s0, nsl,
+                               P->n, W.ptr, W.len, W.col, W.code, W.wptr, W.dict, W.ndict, (const T *)W.val, idv, dv, out,
+                               done);
     }
     return launch_check("k_mc_sweep");
 }
@@ -555,8 +601,10 @@ static int build_sweep(hfh_prec *P, const hfh_csr *A, bool forward, hfh_sweep &W
         for (auto it = segs.rbegin(); it != segs.rend(); ++it)
             for (int k = it->first; k < it->second; ++k) e.push_back(k);
     };
-    std::vector<int> ptr(nsl + 1, 0), e;
-    long long total = 0;
+    std::vector<int> ptr(nsl + 1, 0), e, width(nsl, 0);
+    // the distinct offsets col - row of the entries (0 included: padding)
+    std::vector<long long> offs(1, 0);
+    bool coded = true;
     for (int c = 0; c < nc; ++c)
         for (int w = 0; w < W.slice0[c + 1] - W.slice0[c]; ++w) {
             int L = 0;
@@ -566,13 +614,28 @@ static int build_sweep(hfh_prec *P, const hfh_csr *A, bool forward, hfh_sweep &W
                 entries(c, i, e);
                 len[i] = static_cast<int>(e.size());
                 L = std::max(L, len[i]);
+                for (int k = 0; coded && k < len[i]; ++k) {
+                    const long long o = static_cast<long long>(col[e[k]]) - i;
+                    auto it = std::lower_bound(offs.begin(), offs.end(), o);
+                    if (it == offs.end() || *it != o) {
+                        offs.insert(it, o);
+                        coded = offs.size() <= 256;
+                    }
+                }
             }
-            ptr[W.slice0[c] + w] = static_cast<int>(total);
-            total += 64LL * L;
-            HFH_CHECK(total < (1LL << 31), HFH_ERR_UNSUPPORTED, "hfh_mcsgs_create: sweep copy exceeds 2^31 entries");
+            width[W.slice0[c] + w] = L;
         }
+    // coded: slice widths padded to whole code words of 4 slots
+    long long total = 0;
+    for (int sl = 0; sl < nsl; ++sl) {
+        ptr[sl] = static_cast<int>(total);
+        total += 64LL * (coded ? (width[sl] + 3) / 4 * 4 : width[sl]);
+        HFH_CHECK(total < (1LL << 31), HFH_ERR_UNSUPPORTED, "hfh_mcsgs_create: sweep copy exceeds 2^31 entries");
+    }
     ptr[nsl] = static_cast<int>(total);
-    std::vector<int> pcol(std::max<long long>(total, 1)), pos(std::max<long long>(total, 1));
+    std::vector<int> pcol(coded ? 1 : std::max<long long>(total, 1)), pos(std::max<long long>(total, 1));
+    std::vector<unsigned> words(coded ? std::max<long long>(total / 4, 1) : 1, 0u);
+    const unsigned zero = static_cast<unsigned>(std::lower_bound(offs.begin(), offs.end(), 0LL) - offs.begin());
     for (int c = 0; c < nc; ++c)
         for (int w = 0; w < W.slice0[c + 1] - W.slice0[c]; ++w) {
             const int base = ptr[W.slice0[c] + w], L = (ptr[W.slice0[c] + w + 1] - base) / 64;
@@ -584,8 +647,18 @@ static int build_sweep(hfh_prec *P, const hfh_csr *A, bool forward, hfh_sweep &W
                 const int own = std::min(i, off[c + 1] - 1);   // a row of this slice
                 for (int k = 0; k < L; ++k) {
                     const int idx = base + k * 64 + lane;
-                    pos[idx] = k < static_cast<int>(e.size()) ? e[k] : -1;
-                    pcol[idx] = k < static_cast<int>(e.size()) ? col[e[k]] : own;
+                    const bool real = k < static_cast<int>(e.size());
+                    pos[idx] = real ? e[k] : -1;
+                    if (!coded) {
+                        pcol[idx] = real ? col[e[k]] : own;
+                        continue;
+                    }
+                    const unsigned code =
+                        real ? static_cast<unsigned>(std::lower_bound(offs.begin(), offs.end(),
+                                                                      static_cast<long long>(col[e[k]]) - i) -
+                                                     offs.begin())
+                             : zero;
+                    words[base / 4 + (k / 4) * 64 + lane] |= code << (8 * (k % 4));
                 }
             }
         }
@@ -593,12 +666,25 @@ static int build_sweep(hfh_prec *P, const hfh_csr *A, bool forward, hfh_sweep &W
     int rc;
     if ((rc = dev_alloc(reinterpret_cast<void **>(&W.ptr), sizeof(int) * ptr.size())) ||
         (rc = dev_alloc(reinterpret_cast<void **>(&W.len), sizeof(int) * len.size())) ||
-        (rc = dev_alloc(reinterpret_cast<void **>(&W.col), sizeof(int) * pcol.size())) ||
-        (rc = dev_alloc(&W.val, A->elem() * pcol.size())))
+        (rc = dev_alloc(&W.val, A->elem() * pos.size())))
         return rc;
     HFH_HIP(hipMemcpy(W.ptr, ptr.data(), sizeof(int) * ptr.size(), hipMemcpyHostToDevice));
     HFH_HIP(hipMemcpy(W.len, len.data(), sizeof(int) * len.size(), hipMemcpyHostToDevice));
-    HFH_HIP(hipMemcpy(W.col, pcol.data(), sizeof(int) * pcol.size(), hipMemcpyHostToDevice));
+    if (coded) {
+        std::vector<int> wptr(nsl + 1), dict(offs.begin(), offs.end());
+        for (int sl = 0; sl <= nsl; ++sl) wptr[sl] = ptr[sl] / 4;
+        if ((rc = dev_alloc(reinterpret_cast<void **>(&W.code), sizeof(unsigned) * words.size())) ||
+            (rc = dev_alloc(reinterpret_cast<void **>(&W.wptr), sizeof(int) * wptr.size())) ||
+            (rc = dev_alloc(reinterpret_cast<void **>(&W.dict), sizeof(int) * dict.size())))
+            return rc;
+        HFH_HIP(hipMemcpy(W.code, words.data(), sizeof(unsigned) * words.size(), hipMemcpyHostToDevice));
+        HFH_HIP(hipMemcpy(W.wptr, wptr.data(), sizeof(int) * wptr.size(), hipMemcpyHostToDevice));
+        HFH_HIP(hipMemcpy(W.dict, dict.data(), sizeof(int) * dict.size(), hipMemcpyHostToDevice));
+        W.ndict = static_cast<int>(dict.size());
+    } else {
+        if ((rc = dev_alloc(reinterpret_cast<void **>(&W.col), sizeof(int) * pcol.size()))) return rc;
+        HFH_HIP(hipMemcpy(W.col, pcol.data(), sizeof(int) * pcol.size(), hipMemcpyHostToDevice));
+    }
     int *dpos = nullptr;
     if ((rc = dev_alloc(reinterpret_cast<void **>(&dpos), sizeof(int) * pos.size()))) return rc;
     HFH_HIP(hipMemcpy(dpos, pos.data(), sizeof(int) * pos.size(), hipMemcpyHostToDevice));
@@ -715,6 +801,9 @@ static void free_sweep(hfh_sweep &W) {
     if (W.len) (void)hipFree(W.len);
     if (W.col) (void)hipFree(W.col);
     if (W.val) (void)hipFree(W.val);
+    if (W.code) (void)hipFree(W.code);
+    if (W.wptr) (void)hipFree(W.wptr);
+    if (W.dict) (void)hipFree(W.dict);
     W = hfh_sweep();
 }
 

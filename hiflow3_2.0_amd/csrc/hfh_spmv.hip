@@ -141,7 +141,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv_wave(
     constexpr int WPB = kBlock / 64, PER = WT / 64;
     __shared__ T prod_all[WPB][WT];
     __shared__ double shd[kBlock / 64];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     T *prod = prod_all[wave];
     const Split sp = xcd_split(nblk, WPB);
     double dsum = 0;
@@ -230,7 +230,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv_sell(
     if (done && *done) return;
     __shared__ double shd[kBlock / 64];
     constexpr int WPB = kBlock / 64;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     double dsum = 0;
     // consecutive workgroups take consecutive slices (measured faster on
     // gfx950 than one contiguous range per XCD: the streams of all XCDs then
@@ -345,9 +345,12 @@ __global__ __launch_bounds__(kBlock) void k_spmv_sellp(
 }
 
 // SELL pair layout with x staged in LDS.  A wave takes a slice, loads the
-// slice's chunks of x (16 elements each, listed ascending at cid[cptr[s]..])
-// into its own LDS image with 16-byte loads (one piece of 16 bytes per lane
-// and step), and reads x_col from the image at the slot's 16-bit code.  The
+// slice's chunks of x (CH elements each, listed ascending at cid[cptr[s]..])
+// into its own LDS image (img elements per wave, dynamic shared memory sized
+// to the matrix's largest image; IMG >= img bounds the staging registers)
+// with 16-byte loads (one piece of 16 bytes per lane and step), and reads
+// x_col from the image at the slot's 16-bit code (chunk position * CH +
+// col % CH).  The
 // first slot pair and the single slot of an odd width are loaded before the
 // staging, the next pair while the current one is summed (clamped to the
 // last pair, so every load stays in the slice).  Row sums as k_spmv_sellp
@@ -355,24 +358,25 @@ __global__ __launch_bounds__(kBlock) void k_spmv_sellp(
 // loop.  JAC 1 / 2: a Jacobi step (normal / damped) -- entries whose code is
 // the row's diagonal code are skipped and xn_r = jacobi_update(...) is
 // stored instead of the row sum.  xal: x is 16-byte aligned.
-template <typename T, bool ADD, bool DOT, int JAC, int MAXC>
+template <typename T, bool ADD, bool DOT, int JAC, int CH, int IMG>
 __global__ __launch_bounds__(kBlock) void k_spmv_sellx(
     int nrow, int ncol, int nslice, const int *__restrict__ sptr, const int *__restrict__ rlen,
     const unsigned short *__restrict__ code, const T *__restrict__ sval, const int *__restrict__ cptr,
-    const int *__restrict__ cid, int xal, const T *__restrict__ x, T *__restrict__ y, const T *__restrict__ w,
-    double *part, const int *__restrict__ done, const unsigned short *__restrict__ dcode, const T *__restrict__ b,
-    const T *__restrict__ invd, T jw, T jomw) {
+    const int *__restrict__ cid, int img, int xal, const T *__restrict__ x, T *__restrict__ y,
+    const T *__restrict__ w, double *part, const int *__restrict__ done, const unsigned short *__restrict__ dcode,
+    const T *__restrict__ b, const T *__restrict__ invd, T jw, T jomw) {
     typedef typename Pair<T>::type T2;
     typedef typename V16<T>::type TV;
     constexpr int WV = V16<T>::W;              // elements of a 16-byte piece
-    constexpr int PPC = 16 / WV;               // pieces of a chunk
-    constexpr int NP = (MAXC * PPC + 63) / 64; // pieces per lane
+    constexpr int PPC = CH / WV;               // pieces of a chunk
+    constexpr int NP = (IMG / WV + 63) / 64;   // pieces per lane
+    static_assert(CH % WV == 0 && IMG % CH == 0, "chunk of whole pieces, image of whole chunks");
     constexpr int WPB = kBlock / 64;
     if (done && *done) return;
-    __shared__ T xs_all[WPB][MAXC * 16];
+    extern __shared__ __align__(16) unsigned char xs_raw[];   // WPB images of img elements
     __shared__ double shd[WPB];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-    T *xs = xs_all[wave];
+    T *xs = reinterpret_cast<T *>(xs_raw) + wave * img;
     const unsigned *code2 = reinterpret_cast<const unsigned *>(code);
     double dsum = 0;
     for (int s = blockIdx.x * WPB + wave; s < nslice; s += gridDim.x * WPB) {
@@ -403,7 +407,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv_sellx(
 #pragma unroll
         for (int i = 0; i < NP; ++i) {
             const int p = lane + 64 * i;
-            const int xi = id[i] * 16 + (p % PPC) * WV;
+            const int xi = id[i] * CH + (p % PPC) * WV;
             xv[i] = TV(0);
             if (p < npc && xal && xi + WV <= ncol) xv[i] = *reinterpret_cast<const TV *>(&x[xi]);
         }
@@ -418,7 +422,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv_sellx(
 #pragma unroll 1
         for (int i = 0; i < NP; ++i) {
             const int p = lane + 64 * i;
-            const int xi = id[i] * 16 + (p % PPC) * WV;
+            const int xi = id[i] * CH + (p % PPC) * WV;
             if (p < npc && !(xal && xi + WV <= ncol))
                 for (int t = 0; t < WV; ++t)
                     if (xi + t < ncol) xs[p * WV + t] = x[xi + t];
@@ -466,19 +470,23 @@ __global__ __launch_bounds__(kBlock) void k_spmv_sellx(
 // SELL-64 with dictionary-coded columns: column = row + dict[code].  Value
 // slots are padded to a multiple of 4 per slice; code word w of a lane holds
 // the codes of slots 4w..4w+3 (byte t = slot 4w+t).  Row sums as k_spmv_sell.
-template <typename T, bool ADD, bool DOT, int UW>
+// JAC 1 / 2: a Jacobi step (normal / damped) -- entries with the code of
+// offset 0 (code0, the diagonal) are skipped and xn_r = jacobi_update(...)
+// is stored instead of the row sum.
+template <typename T, bool ADD, bool DOT, int UW, int JAC>
 __global__ __launch_bounds__(kBlock) void k_spmv_selld(
     int nrow, int ncol, int nslice, const int *__restrict__ sptr, const int *__restrict__ wptr,
     const int *__restrict__ rlen, const unsigned *__restrict__ codes, const int *__restrict__ gdict, int ndict,
     const T *__restrict__ sval, const T *__restrict__ x, T *__restrict__ y, const T *__restrict__ w, double *part,
-    const int *__restrict__ done) {
+    const int *__restrict__ done, unsigned code0, const T *__restrict__ b, const T *__restrict__ invd, T jw,
+    T jomw) {
     if (done && *done) return;
     __shared__ int dict[256];
     __shared__ double shd[kBlock / 64];
     for (int i = threadIdx.x; i < ndict; i += kBlock) dict[i] = gdict[i];
     __syncthreads();
     constexpr int WPB = kBlock / 64;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     double dsum = 0;
     for (int s = blockIdx.x * WPB + wave; s < nslice; s += gridDim.x * WPB) {
         const int vbase = sptr[s], W = (sptr[s + 1] - vbase) >> 8;   // code words = slots / 4
@@ -504,12 +512,17 @@ __global__ __launch_bounds__(kBlock) void k_spmv_selld(
             for (int u = 0; u < UW; ++u)
 #pragma unroll
                 for (int t = 0; t < 4; ++t)
-                    if (4 * (k + u) + t < len) acc = add_rn(acc, mul_rn(v[u][t], xv[u][t]));
+                    if (4 * (k + u) + t < len && (!JAC || ((cw[u] >> (8 * t)) & 255u) != code0))
+                        acc = add_rn(acc, mul_rn(v[u][t], xv[u][t]));
         }
         if (r < nrow) {
-            if (ADD) acc = y[r] + acc;
-            y[r] = acc;
-            if (DOT) dsum += static_cast<double>(acc) * static_cast<double>(w[r]);
+            if (JAC) {
+                y[r] = jacobi_update<T, JAC == 2>(acc, x[r], b[r], invd[r], jw, jomw);
+            } else {
+                if (ADD) acc = y[r] + acc;
+                y[r] = acc;
+                if (DOT) dsum += static_cast<double>(acc) * static_cast<double>(w[r]);
+            }
         }
     }
     if (DOT) {
@@ -662,9 +675,10 @@ static void host_parallel(int n, F f) {
     for (auto &t : th) t.join();
 }
 
-// Largest chunk list of a slice in the LDS-staged layout (64 chunks of 16
-// elements: 8 KiB of FP64 x per wave).
-constexpr int kLdsMaxChunks = 64;
+// Largest LDS image of a slice in the LDS-staged layout (1024 elements: 8 KiB
+// of FP64 x per wave), and its chunk size.
+constexpr int kLdsMaxImage = 1024;
+constexpr int kLdsChunk = 16;
 
 struct LdsPlan {
     std::vector<int> cptr, cid;
@@ -672,10 +686,12 @@ struct LdsPlan {
     int maxc = 0;
 };
 
-// Chunk lists and 16-bit codes of the LDS-staged copy over the slot-pair
-// layout with slice offsets ptr.  False when a slice touches more than
-// kLdsMaxChunks chunks.
-static bool lds_plan(const hfh_csr *A, const std::vector<int> &ptr, LdsPlan &P) {
+// Chunk lists (chunks of ch elements, ch a power of two) and 16-bit codes of
+// the LDS-staged copy over the slot-pair layout with slice offsets ptr.  False
+// when a slice's chunks exceed kLdsMaxImage elements.
+static bool lds_plan(const hfh_csr *A, const std::vector<int> &ptr, int ch, LdsPlan &P) {
+    int sh = 0;
+    while ((1 << sh) < ch) ++sh;
     const int n = A->nrow, nsl = (n + 63) / 64;
     const int *row = A->h_row.data(), *col = A->h_col.data();
     std::vector<std::vector<int>> lists(nsl);
@@ -683,7 +699,7 @@ static bool lds_plan(const hfh_csr *A, const std::vector<int> &ptr, LdsPlan &P) 
         std::vector<int> ids;
         for (int s = lo; s < hi; ++s) {
             ids.clear();
-            for (int j = row[s * 64]; j < row[std::min(n, s * 64 + 64)]; ++j) ids.push_back(col[j] >> 4);
+            for (int j = row[s * 64]; j < row[std::min(n, s * 64 + 64)]; ++j) ids.push_back(col[j] >> sh);
             std::sort(ids.begin(), ids.end());
             ids.erase(std::unique(ids.begin(), ids.end()), ids.end());
             lists[s].assign(ids.begin(), ids.end());
@@ -695,7 +711,7 @@ static bool lds_plan(const hfh_csr *A, const std::vector<int> &ptr, LdsPlan &P) 
         P.maxc = std::max(P.maxc, static_cast<int>(lists[s].size()));
         P.cptr[s + 1] = P.cptr[s] + static_cast<int>(lists[s].size());
     }
-    if (P.maxc > kLdsMaxChunks) return false;
+    if (P.maxc * ch > kLdsMaxImage) return false;
     P.cid.resize(std::max(P.cptr[nsl], 1));
     P.code.assign(std::max(ptr[nsl], 1), 0);
     P.dcode.assign(std::max(n, 1), 0xffff);
@@ -708,8 +724,8 @@ static bool lds_plan(const hfh_csr *A, const std::vector<int> &ptr, LdsPlan &P) 
                 const int r = s * 64 + lane, j0 = row[r], len = row[r + 1] - j0;
                 for (int k = 0; k < len; ++k) {
                     const int c = col[j0 + k];
-                    const int pos = static_cast<int>(std::lower_bound(ids.begin(), ids.end(), c >> 4) - ids.begin());
-                    const unsigned short cd = static_cast<unsigned short>(pos * 16 + (c & 15));
+                    const int pos = static_cast<int>(std::lower_bound(ids.begin(), ids.end(), c >> sh) - ids.begin());
+                    const unsigned short cd = static_cast<unsigned short>(pos * ch + (c & (ch - 1)));
                     const int idx = k < Pn ? base + (k >> 1) * 128 + 2 * lane + (k & 1) : base + k * 64 + lane;
                     P.code[idx] = cd;
                     if (c == r) P.dcode[r] = cd;
@@ -720,6 +736,57 @@ static bool lds_plan(const hfh_csr *A, const std::vector<int> &ptr, LdsPlan &P) 
     return true;
 }
 
+// The sorted distinct column offsets col - row of A, offset 0 included (the
+// code of every padding slot).  False when there are more than 256.  Rows
+// whose offsets repeat the previous row's (most rows of a stencil matrix)
+// are skipped; the row ranges run on host threads.
+static bool dict_offsets(const hfh_csr *A, std::vector<int> &dict) {
+    const int n = A->nrow;
+    const int *row = A->h_row.data(), *col = A->h_col.data();
+    constexpr int kMax = 256;
+    const int nt = n < 4096 ? 1 : static_cast<int>(std::min(16u, std::max(1u, std::thread::hardware_concurrency())));
+    std::vector<std::vector<long long>> found(nt);
+    std::vector<char> over(nt, 0);
+    host_parallel(nt, [&](int t0, int t1) {
+        for (int t = t0; t < t1; ++t) {
+            const int lo = static_cast<int>(static_cast<long long>(n) * t / nt);
+            const int hi = static_cast<int>(static_cast<long long>(n) * (t + 1) / nt);
+            std::vector<long long> &f = found[t];
+            f.assign(1, 0);
+            int prev = -1;
+            for (int r = lo; r < hi && !over[t]; ++r) {
+                const int j0 = row[r], len = row[r + 1] - j0;
+                bool same = prev >= 0 && len == row[prev + 1] - row[prev];
+                for (int k = 0; same && k < len; ++k)
+                    same = col[j0 + k] - r == col[row[prev] + k] - prev;
+                prev = r;
+                if (same) continue;
+                for (int k = 0; k < len; ++k) {
+                    const long long o = static_cast<long long>(col[j0 + k]) - r;
+                    auto it = std::lower_bound(f.begin(), f.end(), o);
+                    if (it == f.end() || *it != o) {
+                        f.insert(it, o);
+                        if (static_cast<int>(f.size()) > kMax) {
+                            over[t] = 1;
+                            break;
+                        }
+                    }
+                }
+            }
+        }
+    });
+    std::vector<long long> all;
+    for (int t = 0; t < nt; ++t) {
+        if (over[t]) return false;
+        all.insert(all.end(), found[t].begin(), found[t].end());
+    }
+    std::sort(all.begin(), all.end());
+    all.erase(std::unique(all.begin(), all.end()), all.end());
+    if (static_cast<int>(all.size()) > kMax) return false;
+    dict.assign(all.begin(), all.end());
+    return true;
+}
+
 static void free_lds_copy(hfh_csr *A) {
     if (A->sx_code) (void)hipFree(A->sx_code);
     if (A->sx_dcode) (void)hipFree(A->sx_dcode);
@@ -727,7 +794,8 @@ static void free_lds_copy(hfh_csr *A) {
     if (A->sx_cid) (void)hipFree(A->sx_cid);
     A->sx_code = A->sx_dcode = nullptr;
     A->sx_cptr = A->sx_cid = nullptr;
-    A->sx_maxc = 0;
+    A->sx_img = 0;
+    A->sx_chunk = 0;
 }
 
 static int round8(long long g) { return static_cast<int>(g >= kNumXcd ? (g + kNumXcd - 1) / kNumXcd * kNumXcd : std::max(g, 1LL)); }
@@ -740,31 +808,6 @@ int spmv_configure(hfh_csr *A, int algo, int p1, int p2) {
     if ((algo == HFH_SPMV_SELL || algo == HFH_SPMV_SELL_DICT || algo == HFH_SPMV_SELL_PAIRS ||
          algo == HFH_SPMV_SELL_LDS) && A->ncol == 0)
         algo = HFH_SPMV_STREAM;
-    // the dictionary of distinct column offsets (at most 256 codes)
-    std::vector<int> dict;
-    if (algo == HFH_SPMV_SELL_DICT) {
-        std::vector<long long> offs;
-        offs.reserve(1024);
-        offs.push_back(0);   // always present: the code of every padding slot
-        bool fits = true;
-        for (int r = 0; r < n && fits; ++r)
-            for (int j = A->h_row[r]; j < A->h_row[r + 1]; ++j) {
-                const long long o = static_cast<long long>(A->h_col[j]) - r;
-                if (std::find(offs.begin(), offs.end(), o) == offs.end()) {
-                    offs.push_back(o);
-                    if (offs.size() > 256) {
-                        fits = false;
-                        break;
-                    }
-                }
-            }
-        if (fits) {
-            std::sort(offs.begin(), offs.end());
-            for (long long o : offs) dict.push_back(static_cast<int>(o));
-        } else {
-            algo = HFH_SPMV_SELL;
-        }
-    }
     if (A->sd_dict) (void)hipFree(A->sd_dict);
     if (A->sd_code) (void)hipFree(A->sd_code);
     if (A->sd_wptr) (void)hipFree(A->sd_wptr);
@@ -791,22 +834,31 @@ int spmv_configure(hfh_csr *A, int algo, int p1, int p2) {
             // LDS where the slices' x windows fit (3D Q1, 27 entries per row:
             // FP64 105 us against 125 us with slot pairs, FP32 64 us against
             // 83 us), else FP64 with slot pairs and FP32 with 4 slots in
-            // flight; short (stencil) rows share few x entries within a
-            // slice and run as fast with plain SELL-64, 8 slots in flight
+            // flight.  Short (stencil) rows share few x entries within a
+            // slice; they run fastest with one-byte dictionary codes where
+            // the matrix has at most 256 distinct offsets col - row (7-point
+            // FV 192^3: FP64 115 us against 129 us with plain SELL-64, FP32
+            // 72 us against 79 us), else with SELL-64, 8 slots in flight
             if (mean >= 16.0) {
                 algo = HFH_SPMV_SELL_LDS;
                 p1 = 0;
                 fb_algo = A->dtype == HFH_F64 ? HFH_SPMV_SELL_PAIRS : HFH_SPMV_SELL;
                 fb_p1 = A->dtype == HFH_F64 ? 1 : 4;
             } else {
-                algo = HFH_SPMV_SELL;
-                p1 = 8;
+                algo = HFH_SPMV_SELL_DICT;
+                p1 = A->dtype == HFH_F64 ? 1 : 2;   // FP32: 2 code words in flight (66 us against 71 us)
             }
         } else {
             algo = HFH_SPMV_STREAM;
             p1 = 2048;
         }
         p2 = 0;
+    }
+    // the dictionary of distinct column offsets (at most 256 codes)
+    std::vector<int> dict;
+    if (algo == HFH_SPMV_SELL_DICT && !dict_offsets(A, dict)) {
+        algo = HFH_SPMV_SELL;
+        p1 = 8;
     }
     const int cus = std::max(A->ctx->num_cu, 1);
     if (algo == HFH_SPMV_STREAM) {
@@ -840,7 +892,10 @@ int spmv_configure(hfh_csr *A, int algo, int p1, int p2) {
         }
         ptr[nsl] = static_cast<int>(off);
         LdsPlan plan;
-        if (algo == HFH_SPMV_SELL_LDS && !lds_plan(A, ptr, plan)) {
+        // 16-element chunks: 8-element chunks stage fewer unused entries but
+        // measured no faster (3D Q1 FP64: 105.5 us against 104.7 us)
+        const int ch = kLdsChunk;
+        if (algo == HFH_SPMV_SELL_LDS && !lds_plan(A, ptr, ch, plan)) {
             algo = fb_algo;   // a slice's x window does not fit the LDS image
             p1 = fb_p1;
         }
@@ -869,7 +924,8 @@ int spmv_configure(hfh_csr *A, int algo, int p1, int p2) {
             HFH_HIP(hipMemcpy(A->sx_dcode, plan.dcode.data(), sizeof(unsigned short) * plan.dcode.size(), hipMemcpyHostToDevice));
             HFH_HIP(hipMemcpy(A->sx_cptr, plan.cptr.data(), sizeof(int) * plan.cptr.size(), hipMemcpyHostToDevice));
             HFH_HIP(hipMemcpy(A->sx_cid, plan.cid.data(), sizeof(int) * plan.cid.size(), hipMemcpyHostToDevice));
-            A->sx_maxc = plan.maxc;
+            A->sx_img = plan.maxc * ch;
+            A->sx_chunk = ch;
         }
         A->nslice = nsl;
         A->spmv_algo = algo;
@@ -903,21 +959,32 @@ int spmv_configure(hfh_csr *A, int algo, int p1, int p2) {
         // padding slots take the code of offset 0, which the table always
         // holds: their gather reads x[min(r, ncol - 1)], in range
         const unsigned zero = static_cast<unsigned>(std::lower_bound(dict.begin(), dict.end(), 0) - dict.begin());
-        for (int sl = 0; sl < nsl; ++sl) {
-            const int L4 = (ptr[sl + 1] - ptr[sl]) / 64;
-            for (int lane = 0; lane < 64; ++lane) {
-                const int r = sl * 64 + lane;
-                const int l = r < n ? len[r] : 0, j0 = r < n ? A->h_row[r] : 0;
-                for (int k = 0; k < L4; ++k) {
-                    unsigned code = zero;
-                    if (k < l) {
-                        const int o = A->h_col[j0 + k] - r;
-                        code = static_cast<unsigned>(std::lower_bound(dict.begin(), dict.end(), o) - dict.begin());
+        host_parallel(nsl, [&](int s0, int s1) {
+            std::vector<unsigned> codes;
+            int prev = -1;   // previous row whose codes are in `codes`
+            for (int sl = s0; sl < s1; ++sl) {
+                const int L4 = (ptr[sl + 1] - ptr[sl]) / 64;
+                for (int lane = 0; lane < 64; ++lane) {
+                    const int r = sl * 64 + lane;
+                    const int l = r < n ? len[r] : 0, j0 = r < n ? A->h_row[r] : 0;
+                    // a row with the previous row's offsets has its codes
+                    bool same = prev >= 0 && r < n && l == static_cast<int>(codes.size());
+                    for (int k = 0; same && k < l; ++k)
+                        same = A->h_col[j0 + k] - r == A->h_col[A->h_row[prev] + k] - prev;
+                    if (!same && r < n) {
+                        codes.resize(l);
+                        for (int k = 0; k < l; ++k)
+                            codes[k] = static_cast<unsigned>(
+                                std::lower_bound(dict.begin(), dict.end(), A->h_col[j0 + k] - r) - dict.begin());
                     }
-                    words[wp[sl] + (k / 4) * 64 + lane] |= code << (8 * (k % 4));
+                    if (r < n) prev = r;
+                    for (int k = 0; k < L4; ++k) {
+                        const unsigned code = k < l ? codes[k] : zero;
+                        words[wp[sl] + (k / 4) * 64 + lane] |= code << (8 * (k % 4));
+                    }
                 }
             }
-        }
+        });
         if (A->sl_ptr) (void)hipFree(A->sl_ptr);
         if (A->sl_len) (void)hipFree(A->sl_len);
         if (A->sl_col) (void)hipFree(A->sl_col);
@@ -938,6 +1005,7 @@ int spmv_configure(hfh_csr *A, int algo, int p1, int p2) {
         HFH_HIP(hipMemcpy(A->sd_code, words.data(), sizeof(unsigned) * words.size(), hipMemcpyHostToDevice));
         HFH_HIP(hipMemcpy(A->sd_dict, dict.data(), sizeof(int) * dict.size(), hipMemcpyHostToDevice));
         A->sd_ndict = static_cast<int>(dict.size());
+        A->sd_code0 = static_cast<int>(zero);
         A->nslice = nsl;
         A->spmv_algo = HFH_SPMV_SELL_DICT;
         A->spmv_p1 = p1 == 2 ? 2 : 1;
@@ -983,38 +1051,66 @@ int spmv_configure(hfh_csr *A, int algo, int p1, int p2) {
 }
 
 // The LDS-staged kernel with the image sized to the matrix's largest chunk list.
+template <typename T, bool ADD, bool DOT, int JAC, int CH>
+static void lds_launch_img(const hfh_csr *A, dim3 g, hipStream_t s, int xal, const T *x, T *y, const T *w, double *part,
+                           const int *done, const T *b, const T *invd, T jw, T jomw) {
+    const dim3 bl(kBlock);
+    const T *sv = static_cast<const T *>(A->sl_val);
+    const size_t lds = sizeof(T) * (kBlock / 64) * static_cast<size_t>(A->sx_img);
+    if (A->sx_img <= 256)
+        hipLaunchKernelGGL((k_spmv_sellx<T, ADD, DOT, JAC, CH, 256>), g, bl, lds, s, A->nrow, A->ncol, A->nslice, A->sl_ptr, A->sl_len, A->sx_code, sv, A->sx_cptr, A->sx_cid, A->sx_img, xal, x, y, w, part, done, A->sx_dcode, b, invd, jw, jomw);
+    else if (A->sx_img <= 512)
+        hipLaunchKernelGGL((k_spmv_sellx<T, ADD, DOT, JAC, CH, 512>), g, bl, lds, s, A->nrow, A->ncol, A->nslice, A->sl_ptr, A->sl_len, A->sx_code, sv, A->sx_cptr, A->sx_cid, A->sx_img, xal, x, y, w, part, done, A->sx_dcode, b, invd, jw, jomw);
+    else if (A->sx_img <= 768)
+        hipLaunchKernelGGL((k_spmv_sellx<T, ADD, DOT, JAC, CH, 768>), g, bl, lds, s, A->nrow, A->ncol, A->nslice, A->sl_ptr, A->sl_len, A->sx_code, sv, A->sx_cptr, A->sx_cid, A->sx_img, xal, x, y, w, part, done, A->sx_dcode, b, invd, jw, jomw);
+    else
+        hipLaunchKernelGGL((k_spmv_sellx<T, ADD, DOT, JAC, CH, kLdsMaxImage>), g, bl, lds, s, A->nrow, A->ncol, A->nslice, A->sl_ptr, A->sl_len, A->sx_code, sv, A->sx_cptr, A->sx_cid, A->sx_img, xal, x, y, w, part, done, A->sx_dcode, b, invd, jw, jomw);
+}
+
 template <typename T, bool ADD, bool DOT, int JAC>
 static void lds_launch(const hfh_csr *A, int nwg, const T *x, T *y, const T *w, double *part, const int *done,
                        const T *b, const T *invd, T jw, T jomw) {
     hipStream_t s = A->ctx->stream;
-    const dim3 g(nwg), bl(kBlock);
+    const dim3 g(nwg);
     const int xal = (reinterpret_cast<uintptr_t>(x) & 15) == 0;
+    lds_launch_img<T, ADD, DOT, JAC, kLdsChunk>(A, g, s, xal, x, y, w, part, done, b, invd, jw, jomw);
+}
+
+// The dictionary-coded kernel; code0 is the code of offset 0 (the diagonal).
+template <typename T, bool ADD, bool DOT, int JAC>
+static void dict_launch(const hfh_csr *A, int nwg, const T *x, T *y, const T *w, double *part, const int *done,
+                        const T *b, const T *invd, T jw, T jomw) {
+    hipStream_t s = A->ctx->stream;
+    const dim3 g(nwg), bl(kBlock);
     const T *sv = static_cast<const T *>(A->sl_val);
-    if (A->sx_maxc <= 16)
-        hipLaunchKernelGGL((k_spmv_sellx<T, ADD, DOT, JAC, 16>), g, bl, 0, s, A->nrow, A->ncol, A->nslice, A->sl_ptr, A->sl_len, A->sx_code, sv, A->sx_cptr, A->sx_cid, xal, x, y, w, part, done, A->sx_dcode, b, invd, jw, jomw);
-    else if (A->sx_maxc <= 32)
-        hipLaunchKernelGGL((k_spmv_sellx<T, ADD, DOT, JAC, 32>), g, bl, 0, s, A->nrow, A->ncol, A->nslice, A->sl_ptr, A->sl_len, A->sx_code, sv, A->sx_cptr, A->sx_cid, xal, x, y, w, part, done, A->sx_dcode, b, invd, jw, jomw);
-    else if (A->sx_maxc <= 48)
-        hipLaunchKernelGGL((k_spmv_sellx<T, ADD, DOT, JAC, 48>), g, bl, 0, s, A->nrow, A->ncol, A->nslice, A->sl_ptr, A->sl_len, A->sx_code, sv, A->sx_cptr, A->sx_cid, xal, x, y, w, part, done, A->sx_dcode, b, invd, jw, jomw);
+    const unsigned code0 = static_cast<unsigned>(A->sd_code0);
+    if (A->spmv_p1 == 2)
+        hipLaunchKernelGGL((k_spmv_selld<T, ADD, DOT, 2, JAC>), g, bl, 0, s, A->nrow, A->ncol, A->nslice, A->sl_ptr, A->sd_wptr, A->sl_len, A->sd_code, A->sd_dict, A->sd_ndict, sv, x, y, w, part, done, code0, b, invd, jw, jomw);
     else
-        hipLaunchKernelGGL((k_spmv_sellx<T, ADD, DOT, JAC, kLdsMaxChunks>), g, bl, 0, s, A->nrow, A->ncol, A->nslice, A->sl_ptr, A->sl_len, A->sx_code, sv, A->sx_cptr, A->sx_cid, xal, x, y, w, part, done, A->sx_dcode, b, invd, jw, jomw);
+        hipLaunchKernelGGL((k_spmv_selld<T, ADD, DOT, 1, JAC>), g, bl, 0, s, A->nrow, A->ncol, A->nslice, A->sl_ptr, A->sd_wptr, A->sl_len, A->sd_code, A->sd_dict, A->sd_ndict, sv, x, y, w, part, done, code0, b, invd, jw, jomw);
 }
 
 template <typename T>
-int spmv_lds_jacobi(const hfh_csr *A, const T *x, T *xn, const T *b, const T *invd, bool damped, T w, T omw) {
-    HFH_CHECK(A->spmv_algo == HFH_SPMV_SELL_LDS, HFH_ERR_INVALID, "spmv_lds_jacobi: matrix has no LDS-staged copy");
+int spmv_coded_jacobi(const hfh_csr *A, const T *x, T *xn, const T *b, const T *invd, bool damped, T w, T omw) {
+    const bool lds = A->spmv_algo == HFH_SPMV_SELL_LDS;
+    HFH_CHECK(lds || A->spmv_algo == HFH_SPMV_SELL_DICT, HFH_ERR_INVALID,
+              "spmv_coded_jacobi: matrix has no LDS-staged or dictionary-coded copy");
     if (A->nrow == 0) return HFH_OK;
     const int need = std::max((A->nslice + kBlock / 64 - 1) / (kBlock / 64), 1);
-    if (damped)
+    if (lds && damped)
         lds_launch<T, false, false, 2>(A, need, x, xn, nullptr, nullptr, nullptr, b, invd, w, omw);
-    else
+    else if (lds)
         lds_launch<T, false, false, 1>(A, need, x, xn, nullptr, nullptr, nullptr, b, invd, w, omw);
-    return launch_check("k_spmv_sellx (Jacobi)");
+    else if (damped)
+        dict_launch<T, false, false, 2>(A, need, x, xn, nullptr, nullptr, nullptr, b, invd, w, omw);
+    else
+        dict_launch<T, false, false, 1>(A, need, x, xn, nullptr, nullptr, nullptr, b, invd, w, omw);
+    return launch_check(lds ? "k_spmv_sellx (Jacobi)" : "k_spmv_selld (Jacobi)");
 }
-template int spmv_lds_jacobi<double>(const hfh_csr *, const double *, double *, const double *, const double *, bool,
-                                     double, double);
-template int spmv_lds_jacobi<float>(const hfh_csr *, const float *, float *, const float *, const float *, bool, float,
-                                    float);
+template int spmv_coded_jacobi<double>(const hfh_csr *, const double *, double *, const double *, const double *,
+                                       bool, double, double);
+template int spmv_coded_jacobi<float>(const hfh_csr *, const float *, float *, const float *, const float *, bool,
+                                      float, float);
 
 template <typename T, bool ADD, bool DOT>
 static void launch_t(const hfh_csr *A, const void *x, void *y, const void *w, double *part, const int *done) {
@@ -1030,10 +1126,7 @@ static void launch_t(const hfh_csr *A, const void *x, void *y, const void *w, do
         else
             hipLaunchKernelGGL((k_spmv_sellp<T, ADD, DOT, 1>), g, bl, 0, s, A->nrow, A->nslice, A->sl_ptr, A->sl_len, A->sl_col, (const T *)A->sl_val, xv, yv, wv, part, done);
     } else if (A->spmv_algo == HFH_SPMV_SELL_DICT) {
-        if (A->spmv_p1 == 2)
-            hipLaunchKernelGGL((k_spmv_selld<T, ADD, DOT, 2>), g, bl, 0, s, A->nrow, A->ncol, A->nslice, A->sl_ptr, A->sd_wptr, A->sl_len, A->sd_code, A->sd_dict, A->sd_ndict, (const T *)A->sl_val, xv, yv, wv, part, done);
-        else
-            hipLaunchKernelGGL((k_spmv_selld<T, ADD, DOT, 1>), g, bl, 0, s, A->nrow, A->ncol, A->nslice, A->sl_ptr, A->sd_wptr, A->sl_len, A->sd_code, A->sd_dict, A->sd_ndict, (const T *)A->sl_val, xv, yv, wv, part, done);
+        dict_launch<T, ADD, DOT, 0>(A, A->nwg, xv, yv, wv, part, done, nullptr, nullptr, T(0), T(0));
     } else if (A->spmv_algo == HFH_SPMV_SELL) {
         switch (A->spmv_p1) {
         case 2: hipLaunchKernelGGL((k_spmv_sell<T, ADD, DOT, 2>), g, bl, 0, s, A->nrow, A->nslice, A->sl_ptr, A->sl_len, A->sl_col, (const T *)A->sl_val, xv, yv, wv, part, done); break;

@@ -141,16 +141,14 @@ enum hfh_spmv_algo { HFH_SPMV_AUTO = 0, HFH_SPMV_STREAM = 1, HFH_SPMV_VECTOR = 2
                      HFH_SPMV_SELL = 4, HFH_SPMV_SELL_DICT = 5, HFH_SPMV_SELL_PAIRS = 6,
                      HFH_SPMV_SELL_LDS = 7 };
 /* HFH_SPMV_SELL_LDS: the slot-pair layout with x staged in LDS.  For every
- * slice the distinct 16-element chunks of x its columns touch are listed; a
- * wavefront loads its slice's chunks once with 16-byte loads and each slot
- * holds a 16-bit index into that LDS image (2 index bytes per entry instead
- * of 4, and one coalesced load per chunk instead of one gather per entry).
- * Bit-identical like SELL.  The automatic choice for rows of 16 or more
- * entries on average where SELL-64 padding stays small and every slice's
- * window fits; a requested SELL_LDS on a matrix
- * with a slice touching more than 64 chunks (1024 distinct x entries near
- * its rows) gets HFH_SPMV_SELL_PAIRS instead (the automatic choice falls back
- * to its SELL / pairs rule).  param1 unused. */
+ * slice the distinct 16-element chunks of x its columns touch are listed; a wavefront loads its slice's chunks once with 16-byte loads and
+ * each slot holds a 16-bit index into that LDS image (2 index bytes per entry
+ * instead of 4, and one coalesced load per chunk instead of one gather per
+ * entry).  Bit-identical like SELL.  The automatic choice for rows of 16 or
+ * more entries on average where SELL-64 padding stays small and every
+ * slice's window fits.  A requested SELL_LDS on a matrix with a slice whose
+ * chunks exceed 1024 elements of x gets HFH_SPMV_SELL_PAIRS instead (the
+ * automatic choice falls back to its SELL / pairs rule).  param1 unused. */
 /* HFH_SPMV_SELL_PAIRS: the SELL-64 copy with the two consecutive column slots
  * of a row stored side by side (16-byte value loads per lane for FP64); param1:
  * slot pairs in flight (1 or 2).  Bit-identical like SELL.  The automatic

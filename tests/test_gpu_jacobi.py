@@ -13,7 +13,8 @@ from conftest import relnorm
 
 pytestmark = pytest.mark.gpu
 
-LAYOUTS = [(4, 4), (4, 8), (6, 1), (7, 0), (1, 2048)]   # SELL, SELL, pairs, LDS-staged, row blocks (CSR step)
+LAYOUTS = [(4, 4), (4, 8), (6, 1), (7, 0), (5, 1), (5, 2), (1, 2048)]
+# SELL, SELL, pairs, LDS-staged, dictionary codes, row blocks (CSR step)
 
 
 def matrices(pkg, case):
@@ -26,14 +27,16 @@ def matrices(pkg, case):
     return row, col, val, rhs
 
 
-def no_diag_rows(rng, n=300):
+def no_diag_rows(rng, n=300, band=None):
     """Diagonally dominant random rows; every fifth row has no diagonal entry
-    and row 7 is empty."""
+    and row 7 is empty.  band: columns within +-band of the row (at most
+    2 band + 1 distinct offsets)."""
     row, cols, vals = [0], [], []
     for i in range(n):
         c = set()
         if i != 7:
-            c = set(rng.choice(n, size=int(rng.integers(1, 9)), replace=False).tolist())
+            lo, hi = (0, n) if band is None else (max(0, i - band), min(n, i + band + 1))
+            c = set((lo + rng.choice(hi - lo, size=min(hi - lo, int(rng.integers(1, 9))), replace=False)).tolist())
             if i % 5 == 0:
                 c.discard(i)
             else:
@@ -126,12 +129,12 @@ def test_solve_matches_oracle(pkg, ctx, oracle, case, dtype, mode, w, inner):
     assert relnorm(info["hist"], oi["hist"]) < tol
 
 
-@pytest.mark.parametrize("algo", [(0, 0), (7, 0)])
-def test_solve_rows_without_diagonal_and_maxits(pkg, ctx, oracle, algo):
+@pytest.mark.parametrize("algo,band", [((0, 0), None), ((7, 0), None), ((5, 1), 100)])
+def test_solve_rows_without_diagonal_and_maxits(pkg, ctx, oracle, algo, band):
     """Rows without a diagonal keep inv_diag 0 (their iterate is 0 after a
     step); the control stops on maxits, as the oracle's does."""
     rng = np.random.default_rng(12)
-    row, col, val = no_diag_rows(rng)
+    row, col, val = no_diag_rows(rng, band=band)
     b = rng.uniform(-1, 1, len(row) - 1)
     A = pkg.CsrMatrix(ctx, row, col, val)
     A.set_spmv(*algo)

@@ -153,7 +153,7 @@ def test_spmv_lds_staged_x(pkg, ctx, oracle, dtype, n, ncol, halfwidth, maxlen):
     row, col = banded_ragged(rng, n, ncol, halfwidth, maxlen)
     val = rng.uniform(-1, 1, len(col))
     y, ref, A, X, Y = spmv_case(pkg, ctx, oracle, row, col, val, ncol=ncol, dtype=dtype, algo=(7, 0))
-    assert A.get_spmv()[0] == pkg.SPMV_SELL_LDS
+    assert A.get_spmv() == (pkg.SPMV_SELL_LDS, 0)
     assert np.array_equal(y, ref)
     y0 = rng.uniform(-1, 1, n).astype(dtype)
     Y.upload(y0)
@@ -162,14 +162,17 @@ def test_spmv_lds_staged_x(pkg, ctx, oracle, dtype, n, ncol, halfwidth, maxlen):
 
 
 def test_spmv_lds_falls_back_when_a_window_is_too_wide(pkg, ctx, oracle):
-    # one slice touching 65 chunks of x: the pair layout instead
+    # one slice touching 65 chunks of 16 elements of x (1040 > 1024): the pair
+    # layout instead
+    chunk, count = 16, 65
     rng = np.random.default_rng(11)
-    n, ncol = 64, 65 * 16
+    n, ncol = 64, count * chunk
     lens = np.full(n, 2)
-    lens[0] = 65
+    lens[0] = count
     row = np.zeros(n + 1, np.int32)
     row[1:] = np.cumsum(lens)
-    col = np.concatenate([np.arange(65) * 16] + [np.sort(rng.choice(ncol, 2, replace=False)) for _ in range(n - 1)])
+    col = np.concatenate([np.arange(count) * chunk] +
+                         [np.sort(rng.choice(ncol, 2, replace=False)) for _ in range(n - 1)])
     val = rng.uniform(-1, 1, len(col))
     y, ref, A, *_ = spmv_case(pkg, ctx, oracle, row, col.astype(np.int32), val, ncol=ncol, algo=(7, 0))
     assert A.get_spmv() == (pkg.SPMV_SELL_PAIRS, 1)
@@ -199,8 +202,9 @@ def test_spmv_lds_solvers_identical_to_pairs(pkg, ctx, solver):
         else:
             info = pkg.gmres(A, pkg.Vector(ctx, rhs), X, P, m=30, maxits=2000, atol=0.0, rtol=1e-10)
         out.append((info["iters"], X.numpy()))
-    assert out[0][0] == out[1][0]
-    assert np.array_equal(out[0][1], out[1][1])
+    for it, x in out[1:]:
+        assert it == out[0][0]
+        assert np.array_equal(x, out[0][1])
 
 
 def test_spmv_layout_choice(pkg, ctx):
@@ -210,14 +214,14 @@ def test_spmv_layout_choice(pkg, ctx):
     assert pkg.CsrMatrix(ctx, row, col, val, dtype=np.float32).get_spmv() == (pkg.SPMV_SELL_LDS, 0)
     row, col, val, _ = pkg.problem(pkg.FV_DIFFUSION, 12)
     A = pkg.CsrMatrix(ctx, row, col, val)
-    assert A.get_spmv() == (pkg.SPMV_SELL, 8)                                           # stencil rows
+    assert A.get_spmv() == (pkg.SPMV_SELL_DICT, 1)                                      # stencil rows, 7 offsets
     # wide x windows (random columns): pairs / SELL
     rng = np.random.default_rng(4)
     row, col, val = rand_csr(rng, 640, 40000, np.full(640, 20))
     assert pkg.CsrMatrix(ctx, row, col, val, ncol=40000).get_spmv() == (pkg.SPMV_SELL_PAIRS, 1)  # long FP64 rows
     assert pkg.CsrMatrix(ctx, row, col, val, ncol=40000, dtype=np.float32).get_spmv() == (pkg.SPMV_SELL, 4)
     row, col, val = rand_csr(rng, 640, 40000, np.full(640, 6))
-    assert pkg.CsrMatrix(ctx, row, col, val, ncol=40000).get_spmv() == (pkg.SPMV_SELL, 8)  # short rows
+    assert pkg.CsrMatrix(ctx, row, col, val, ncol=40000).get_spmv() == (pkg.SPMV_SELL, 8)  # > 256 offsets
     row, col, val, _ = pkg.problem(pkg.FV_DIFFUSION, 12)
     A = pkg.CsrMatrix(ctx, row, col, val)
     A.set_spmv(pkg.SPMV_SELL_DICT, 2)
@@ -366,15 +370,35 @@ def test_jacobi_zero_diagonal(pkg, ctx):
     assert e.value.code == pkg.ERR_ZERO_PIVOT
 
 
+def random_spd_pattern(rng, n, m):
+    """Symmetric random pattern (m random pairs), diagonally dominant."""
+    import scipy.sparse as sp
+    i, j = rng.integers(0, n, m), rng.integers(0, n, m)
+    M = sp.coo_matrix((rng.uniform(-1, 0, m), (i, j)), shape=(n, n)).tocsr()
+    M = M + M.T
+    M.setdiag(0)
+    M.eliminate_zeros()
+    M = (M + sp.diags(np.asarray(abs(M).sum(axis=1)).ravel() + 1.0)).tocsr()
+    M.sort_indices()
+    return M.indptr.astype(np.int32), M.indices.astype(np.int32), M.data.copy(), rng.uniform(-1, 1, n)
+
+
 @pytest.mark.parametrize("omega", [1.0, 1.3])
-@pytest.mark.parametrize("kind", ["fv3", "q1_3d", "q1_2d"])
+@pytest.mark.parametrize("kind", ["fv3", "q1_3d", "q1_2d", "random"])
 def test_mcsgs_apply(pkg, ctx, oracle, omega, kind):
+    """Stencil matrices have few distinct offsets col - row, so their sweep
+    copies hold one-byte offset codes; the random pattern has hundreds and
+    keeps 4-byte columns."""
     if kind == "fv3":
         row, col, val, rhs = fv3(pkg)
     elif kind == "q1_3d":
         row, col, val, rhs = pkg.problem(pkg.Q1_3D, 15)
-    else:
+    elif kind == "q1_2d":
         row, col, val, rhs = pkg.problem(pkg.Q1_2D, 40, 33)
+    else:
+        row, col, val, rhs = random_spd_pattern(np.random.default_rng(5), 2000, 6000)
+        r_of = np.repeat(np.arange(len(row) - 1), np.diff(row))
+        assert len(np.unique(col - r_of)) > 256
     nc, color, sizes, perm = pkg.multicoloring(row, col)
     prow, pcol, pval = pkg.csr_permute(row, col, val, perm)
     A = pkg.CsrMatrix(ctx, prow, pcol, pval)
