@@ -228,7 +228,9 @@ int spmv_coded_jacobi(const hfh_csr *A, const T *x, T *xn, const T *b, const T *
 
 // Preconditioner apply on raw device pointers (hfh_prec.hip).  `done`
 // (device, nullable): every kernel returns at once when *done != 0.
-int prec_apply_raw(hfh_prec *P, const void *in, void *out, const int *done = nullptr);
+// mc_first_done: for a multicolour SGS of two or more colours, the first
+// forward step (colour 0) is already in `out` (the CG update computes it).
+int prec_apply_raw(hfh_prec *P, const void *in, void *out, const int *done = nullptr, bool mc_first_done = false);
 // Brings a preconditioner's copies of the matrix values up to date (after the
 // matrix's values changed); a no-op while the stream is being captured.
 int prec_sync(hfh_prec *P);

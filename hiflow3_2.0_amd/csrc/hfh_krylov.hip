@@ -6,8 +6,10 @@
 //   K2  alpha = beta/pAp; x += alpha p; r -= alpha Ap; partials of r.r (and
 //       r.z with z = r.*invD for Jacobi); F2: iter++, res = sqrt(r.r),
 //       IterateControl::Check, beta update
-//   [M] z = M^-1 r, then K2b partials of r.z and F3: beta update (non-Jacobi)
 //   K3  p = beta_ratio p + z
+// With another preconditioner, [M] z = M^-1 r and K2b (partials of r.z) run
+// between K2 and F2, which then reduces r.r and r.z together; for a
+// multicolour SGS, K2 also writes the colour-0 forward step of [M].
 // (F* are one-workgroup finisher kernels that sum the partials in a fixed
 // order.)
 // Every kernel returns at once when the done flag is set, so the host
@@ -63,11 +65,15 @@ __device__ __forceinline__ void pcg_update_one(T &x, T p, T &r, T a, T d, T alph
     }
 }
 
-template <typename T, int MODE>
+// F0 (MODE 2 with a multicolour SGS preconditioner): also the
+// preconditioner's first forward step, z_i = (-r_i) * idv_i for the rows
+// i < n0 of colour 0 (k_mc_fwd0's arithmetic), from the new r in registers.
+template <typename T, int MODE, bool F0>
 __global__ __launch_bounds__(kBlock) void k_pcg_update(int n, long long nv, T *__restrict__ x, const T *__restrict__ p,
                                                        T *__restrict__ r, const T *__restrict__ Ap,
                                                        const T *__restrict__ invd, double *part, int stride,
-                                                       const KState *st) {
+                                                       const KState *st, T *__restrict__ z0,
+                                                       const T *__restrict__ idv0, int n0) {
     if (st->done) return;
     typedef typename V16<T>::type V;
     constexpr int W = V16<T>::W;
@@ -95,9 +101,25 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int n, long long nv, T *_
         }
         xv[i] = xi;
         rv[i] = ri;
+        if (F0) {
+            const long long e0 = i * W;
+            if (e0 + W <= n0) {
+                const V dd = reinterpret_cast<const V *>(idv0)[i];
+                V zz;
+#pragma unroll
+                for (int w = 0; w < W; ++w) zz[w] = (ri[w] * static_cast<T>(-1)) * dd[w];
+                reinterpret_cast<V *>(z0)[i] = zz;
+            } else if (e0 < n0) {
+#pragma unroll
+                for (int w = 0; w < W; ++w)
+                    if (e0 + w < n0) z0[e0 + w] = (ri[w] * static_cast<T>(-1)) * idv0[e0 + w];
+            }
+        }
     }
-    for (long long j = nv * W + t0; j < n; j += step)
+    for (long long j = nv * W + t0; j < n; j += step) {
         pcg_update_one<T, MODE>(x[j], p[j], r[j], Ap[j], MODE == 1 ? invd[j] : T(0), alpha, rr, rz);
+        if (F0 && j < n0) z0[j] = (r[j] * static_cast<T>(-1)) * idv0[j];
+    }
     double v[2];
     v[0] = block_sum(rr, sh);
     __syncthreads();
@@ -158,18 +180,6 @@ __global__ __launch_bounds__(kBlock) void k_pcg_rz(int n, const T *__restrict__ 
     store_partials<1>(v, part, stride, blockIdx.x);
 }
 
-template <typename T>
-__global__ __launch_bounds__(kFinBlock) void k_fin_rz(const double *part, int stride, int nb, KState *st) {
-    if (st->done) return;
-    double out[1];
-    reduce_partials<1>(part, stride, nb, out);
-    if (threadIdx.x == 0) {
-        const T gamma = static_cast<T>(out[0]);
-        const T beta = static_cast<T>(st->beta);
-        st->ratio = static_cast<double>(gamma / beta);
-        st->beta = static_cast<double>(gamma);
-    }
-}
 
 // p = ratio * p + z  (ScaleAdd, cg.cc:233).  MODE as in k_pcg_update; nv
 // as there.
@@ -248,6 +258,11 @@ struct PcgRun {
     double *hist = nullptr;
     int nb = 1;
     long long nv = 0;             // 16-byte vectors of the element-wise kernels (0: unaligned operand)
+    // multicolour SGS with two or more colours: its first forward step (rows
+    // of colour 0, z = -r * idv) runs inside k_pcg_update
+    bool f0 = false;
+    const T *idv0 = nullptr;
+    int n0 = 0;
 
     // SpMV with fused p.Ap partials; fin_pap sums them
     int spmv_kernel() { return spmv_launch_ex(A, p, Ap, 0, p, spart, &st->done); }
@@ -264,26 +279,30 @@ struct PcgRun {
         hipStream_t s = A->ctx->stream;
         int rc;
         if (mode == 0)
-            hipLaunchKernelGGL((k_pcg_update<T, 0>), dim3(nb), dim3(kBlock), 0, s, n, nv, x, p, r, Ap, invd, part, stride, (const KState *)st);
+            hipLaunchKernelGGL((k_pcg_update<T, 0, false>), dim3(nb), dim3(kBlock), 0, s, n, nv, x, p, r, Ap, invd, part, stride, (const KState *)st, z, idv0, n0);
         else if (mode == 1)
-            hipLaunchKernelGGL((k_pcg_update<T, 1>), dim3(nb), dim3(kBlock), 0, s, n, nv, x, p, r, Ap, invd, part, stride, (const KState *)st);
+            hipLaunchKernelGGL((k_pcg_update<T, 1, false>), dim3(nb), dim3(kBlock), 0, s, n, nv, x, p, r, Ap, invd, part, stride, (const KState *)st, z, idv0, n0);
+        else if (f0)
+            hipLaunchKernelGGL((k_pcg_update<T, 2, true>), dim3(nb), dim3(kBlock), 0, s, n, nv, x, p, r, Ap, invd, part, stride, (const KState *)st, z, idv0, n0);
         else
-            hipLaunchKernelGGL((k_pcg_update<T, 2>), dim3(nb), dim3(kBlock), 0, s, n, nv, x, p, r, Ap, invd, part, stride, (const KState *)st);
+            hipLaunchKernelGGL((k_pcg_update<T, 2, false>), dim3(nb), dim3(kBlock), 0, s, n, nv, x, p, r, Ap, invd, part, stride, (const KState *)st, z, idv0, n0);
         if ((rc = launch_check("k_pcg_update"))) return rc;
-        if (mode == 0)
-            hipLaunchKernelGGL((k_fin_update<T, 0>), dim3(1), dim3(kFinBlock), 0, s, (const double *)part, stride, nb, st, hist);
-        else if (mode == 1)
-            hipLaunchKernelGGL((k_fin_update<T, 1>), dim3(1), dim3(kFinBlock), 0, s, (const double *)part, stride, nb, st, hist);
-        else
-            hipLaunchKernelGGL((k_fin_update<T, 2>), dim3(1), dim3(kFinBlock), 0, s, (const double *)part, stride, nb, st, hist);
-        if ((rc = launch_check("k_fin_update"))) return rc;
         if (mode == 2) {
-            if ((rc = prec_apply_raw(P, r, z, &st->done))) return rc;
-            hipLaunchKernelGGL(k_pcg_rz<T>, dim3(nb), dim3(kBlock), 0, s, n, r, z, part, stride, (const KState *)st);
+            // general preconditioner: z = M^-1 r and the partials of r.z
+            // (slot 1, beside r.r in slot 0), then one finisher for both --
+            // the step count, residual and test of k_fin_update, and the
+            // beta update from r.z (MODE 1's arithmetic).  On the step that
+            // converges, the apply runs once more than needed.
+            if ((rc = prec_apply_raw(P, r, z, &st->done, f0))) return rc;
+            hipLaunchKernelGGL(k_pcg_rz<T>, dim3(nb), dim3(kBlock), 0, s, n, r, z, part + stride, stride, (const KState *)st);
             if ((rc = launch_check("k_pcg_rz"))) return rc;
-            hipLaunchKernelGGL(k_fin_rz<T>, dim3(1), dim3(kFinBlock), 0, s, (const double *)part, stride, nb, st);
-            if ((rc = launch_check("k_fin_rz"))) return rc;
+            hipLaunchKernelGGL((k_fin_update<T, 1>), dim3(1), dim3(kFinBlock), 0, s, (const double *)part, stride, nb, st, hist);
+        } else if (mode == 1) {
+            hipLaunchKernelGGL((k_fin_update<T, 1>), dim3(1), dim3(kFinBlock), 0, s, (const double *)part, stride, nb, st, hist);
+        } else {
+            hipLaunchKernelGGL((k_fin_update<T, 0>), dim3(1), dim3(kFinBlock), 0, s, (const double *)part, stride, nb, st, hist);
         }
+        if ((rc = launch_check("k_fin_update"))) return rc;
         if (mode == 0)
             hipLaunchKernelGGL((k_pcg_p<T, 0>), dim3(nb), dim3(kBlock), 0, s, n, nv, p, r, z, invd, st);
         else if (mode == 1)
@@ -399,6 +418,9 @@ struct PcgWork {
         R.n = n;
         R.mode = (!P || P->kind == HFH_PREC_NONE) ? 0 : (P->kind == HFH_PREC_JACOBI ? 1 : 2);
         R.invd = (P && P->kind == HFH_PREC_JACOBI) ? static_cast<const T *>(P->invd) : nullptr;
+        R.f0 = R.mode == 2 && P->kind == HFH_PREC_MCSGS && P->ncolors > 1 && P->color_off[1] > 0;
+        R.idv0 = R.f0 ? static_cast<const T *>(P->invd) : nullptr;
+        R.n0 = R.f0 ? P->color_off[1] : 0;
         R.nb = stream_blocks(c, n);
         R.stride = kRedBlocks;
         int rc;
@@ -429,7 +451,8 @@ struct PcgWork {
         {
             const uintptr_t al = reinterpret_cast<uintptr_t>(x) | reinterpret_cast<uintptr_t>(R.r) |
                                  reinterpret_cast<uintptr_t>(R.p) | reinterpret_cast<uintptr_t>(R.z) |
-                                 reinterpret_cast<uintptr_t>(R.Ap) | reinterpret_cast<uintptr_t>(R.invd);
+                                 reinterpret_cast<uintptr_t>(R.Ap) | reinterpret_cast<uintptr_t>(R.invd) |
+                                 reinterpret_cast<uintptr_t>(R.idv0);
             R.nv = (al & 15) == 0 ? n / V16<T>::W : 0;
         }
         hfh_vec vr{c, n, A->dtype, R.r}, vb{c, n, A->dtype, const_cast<T *>(b)}, vz{c, n, A->dtype, R.z},

@@ -523,7 +523,7 @@ static int trsv_sweeps(hfh_prec *P, const T *in, T *out, const int *done) {
 }
 
 template <typename T>
-static int mcsgs_apply(hfh_prec *P, const T *in, T *out, const int *done) {
+static int mcsgs_apply(hfh_prec *P, const T *in, T *out, const int *done, bool first_done) {
     hipStream_t s = P->ctx->stream;
     ColorOffsets co;
     for (int c = 0; c <= P->ncolors; ++c) co.off[c] = P->color_off[c];
@@ -534,6 +534,7 @@ static int mcsgs_apply(hfh_prec *P, const T *in, T *out, const int *done) {
         const int cnt = co.off[c + 1] - co.off[c];
         if (cnt == 0) continue;
         if (c == 0 && nc > 1) {
+            if (first_done) continue;
             hipLaunchKernelGGL(k_mc_fwd0<T>, dim3((cnt + kBlock - 1) / kBlock), dim3(kBlock), 0, s, co.off[0],
                                co.off[1], in, idv, out, done);
         } else {
@@ -853,7 +854,7 @@ int prec_sync(hfh_prec *P) {
     return rc;
 }
 
-int prec_apply_raw(hfh_prec *P, const void *in, void *out, const int *done) {
+int prec_apply_raw(hfh_prec *P, const void *in, void *out, const int *done, bool mc_first_done) {
     hipStream_t s = P->ctx->stream;
     const int n = P->n;
     if (n == 0) return HFH_OK;
@@ -873,8 +874,8 @@ int prec_apply_raw(hfh_prec *P, const void *in, void *out, const int *done) {
         return launch_check("k_jacobi");
     }
     case HFH_PREC_MCSGS:
-        return P->dtype == HFH_F64 ? mcsgs_apply<double>(P, (const double *)in, (double *)out, done)
-                                   : mcsgs_apply<float>(P, (const float *)in, (float *)out, done);
+        return P->dtype == HFH_F64 ? mcsgs_apply<double>(P, (const double *)in, (double *)out, done, mc_first_done)
+                                   : mcsgs_apply<float>(P, (const float *)in, (float *)out, done, mc_first_done);
     case HFH_PREC_SGS:
     case HFH_PREC_ILU0:
         if (!done)

@@ -1352,9 +1352,11 @@ extern "C" int hfh_csr_cast(hfh_csr *dst, const hfh_csr *src) {
     dst->h_row = src->h_row;
     dst->h_col = src->h_col;
     dst->sorted = src->sorted;
-    int rc = spmv_configure(dst, dst->spmv_algo_req ? dst->spmv_algo_req : src->spmv_algo,
-                            dst->spmv_algo_req ? dst->spmv_p1_req : src->spmv_p1,
-                            dst->spmv_algo_req ? dst->spmv_p2_req : 0);
+    // the destination's requested layout; else the source's requested one;
+    // else the automatic choice for the destination's value type
+    const bool own = dst->spmv_algo_req != 0;
+    int rc = spmv_configure(dst, own ? dst->spmv_algo_req : src->spmv_algo_req,
+                            own ? dst->spmv_p1_req : src->spmv_p1_req, own ? dst->spmv_p2_req : src->spmv_p2_req);
     if (rc) return rc;
     if (src->nnz) {
         const long long nnz = src->nnz;

@@ -119,7 +119,9 @@ int hfh_csr_upload(hfh_csr *A, const int *row, const int *col, const void *val);
 /* CopyFrom with an unchanged structure: values only */
 int hfh_csr_upload_values(hfh_csr *A, const void *val);
 int hfh_csr_download(const hfh_csr *A, int *row, int *col, void *val);
-/* CastFrom (lmatrix.h:297-298): dst takes src's structure, values converted */
+/* CastFrom (lmatrix.h:297-298): dst takes src's structure, values converted.
+ * SpMV layout: dst's requested one (hfh_csr_set_spmv), else src's requested
+ * one, else the automatic choice for dst's value type. */
 int hfh_csr_cast(hfh_csr *dst, const hfh_csr *src);
 /* VectorMult: y = A x (lmatrix.h:331; CPUsimple_csr_lmatrix.cc:75-105) */
 int hfh_spmv(const hfh_csr *A, const hfh_vec *x, hfh_vec *y);

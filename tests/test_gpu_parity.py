@@ -215,6 +215,11 @@ def test_spmv_layout_choice(pkg, ctx):
     row, col, val, _ = pkg.problem(pkg.FV_DIFFUSION, 12)
     A = pkg.CsrMatrix(ctx, row, col, val)
     assert A.get_spmv() == (pkg.SPMV_SELL_DICT, 1)                                      # stencil rows, 7 offsets
+    # a cast takes the automatic choice of its own value type, or the requested layout
+    assert pkg.CsrMatrix.cast_of(A, np.float32).get_spmv() == (pkg.SPMV_SELL_DICT, 2)
+    A.set_spmv(pkg.SPMV_SELL, 4)
+    assert pkg.CsrMatrix.cast_of(A, np.float32).get_spmv() == (pkg.SPMV_SELL, 4)
+    A = pkg.CsrMatrix(ctx, row, col, val)
     # wide x windows (random columns): pairs / SELL
     rng = np.random.default_rng(4)
     row, col, val = rand_csr(rng, 640, 40000, np.full(640, 20))
