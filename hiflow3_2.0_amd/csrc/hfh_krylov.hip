@@ -1,17 +1,19 @@
 // hfh_krylov.hip -- device-resident Krylov drivers.
 //
 // CG (linear_solver/cg.cc:163-248) is restated with its scalars kept in HBM so
-// the host never waits on a Dot.  One Jacobi iteration is five launches:
-//   K1  Ap = A p with per-workgroup partial sums of Ap.p; F1 stores pAp
-//   K2  alpha = beta/pAp; x += alpha p; r -= alpha Ap; partials of r.r (and
-//       r.z with z = r.*invD for Jacobi); F2: iter++, res = sqrt(r.r),
-//       IterateControl::Check, beta update
-//   K3  p = beta_ratio p + z
+// the host never waits on a Dot.  One Jacobi iteration is three or four launches:
+//   K1  Ap = A p with per-workgroup partial sums of Ap.p (summed by a
+//       finisher when the folding below would re-read too many partials)
+//   K2  every workgroup sums the pAp partials (the same fixed order in each);
+//       alpha = beta/pAp; x += alpha p; r -= alpha Ap; partials of r.r (and
+//       r.z with z = r.*invD for Jacobi); workgroup 0 advances the step
+//   K3  every workgroup sums the r.r / r.z partials and applies
+//       IterateControl::Check to res = sqrt(r.r); workgroup 0 records the
+//       step and the new beta; unless finished, p = beta_ratio p + z
 // With another preconditioner, [M] z = M^-1 r and K2b (partials of r.z) run
-// between K2 and F2, which then reduces r.r and r.z together; for a
-// multicolour SGS, K2 also writes the colour-0 forward step of [M].
-// (F* are one-workgroup finisher kernels that sum the partials in a fixed
-// order.)
+// between K2 and K3; for a multicolour SGS, K2 also writes the colour-0
+// forward step of [M].  No kernel reads a state field that another
+// workgroup of the same launch writes (beta_prev keeps K2's beta for K3).
 // Every kernel returns at once when the done flag is set, so the host
 // enqueues chunks of iterations (a captured hipGraph when possible) and reads
 // the state once per chunk.  Compulsory traffic per Jacobi iteration:
@@ -25,8 +27,9 @@
 namespace hfh {
 
 struct KState {
-    double pap, rr, rz;     // reduction results (written by last workgroups)
+    double pap, rr, rz;     // reduction results
     double beta, ratio;     // beta = r.z (or r.r), ratio = new/old
+    double beta_prev;       // beta of the step in progress (read by K3 while workgroup 0 updates beta)
     double res, res0;
     double atol, rtol, dtol;
     int iter, done, state, maxits;
@@ -38,6 +41,20 @@ __device__ __forceinline__ int check_state(int step, double v, double first, con
     if (v > s->dtol * first) return HFH_FAIL_DIV;
     if (step >= s->maxits) return HFH_FAIL_MAXITS;
     return HFH_ITERATE;
+}
+
+// Sum of part[0..nb) by one workgroup in a fixed order (lane-strided sums,
+// then the block tree), returned to every thread.  Every workgroup that calls
+// it on the same partials gets the same value.
+__device__ __forceinline__ double block_sum_all(const double *__restrict__ part, int nb, double *sh, double *bc) {
+    double v = 0;
+    for (int b = threadIdx.x; b < nb; b += kBlock) v += part[b];
+    v = block_sum(v, sh);
+    if (threadIdx.x == 0) *bc = v;
+    __syncthreads();
+    v = *bc;
+    __syncthreads();
+    return v;
 }
 
 static inline int host_check(int step, double v, double first, const hfh_control &c) {
@@ -72,13 +89,26 @@ template <typename T, int MODE, bool F0>
 __global__ __launch_bounds__(kBlock) void k_pcg_update(int n, long long nv, T *__restrict__ x, const T *__restrict__ p,
                                                        T *__restrict__ r, const T *__restrict__ Ap,
                                                        const T *__restrict__ invd, double *part, int stride,
-                                                       const KState *st, T *__restrict__ z0,
-                                                       const T *__restrict__ idv0, int n0) {
+                                                       KState *st, T *__restrict__ z0,
+                                                       const T *__restrict__ idv0, int n0,
+                                                       const double *__restrict__ spart, int nsp) {
     if (st->done) return;
     typedef typename V16<T>::type V;
     constexpr int W = V16<T>::W;
     __shared__ double sh[kBlock / 64];
-    const T alpha = static_cast<T>(static_cast<T>(st->beta) / static_cast<T>(st->pap));
+    __shared__ double bc;
+    // pAp: with nsp > 0 every workgroup sums the SpMV's nsp partials in the
+    // same fixed order (no finisher launch), else k_fin_pap has stored it;
+    // workgroup 0 advances the step count and keeps beta for K3, which reads
+    // beta_prev while it updates beta
+    const double pap = nsp > 0 ? block_sum_all(spart, nsp, sh, &bc) : st->pap;
+    const double beta_d = st->beta;
+    const T alpha = static_cast<T>(static_cast<T>(beta_d) / static_cast<T>(pap));
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (nsp > 0) st->pap = pap;
+        st->beta_prev = beta_d;
+        st->iter = st->iter + 1;
+    }
     double rr = 0, rz = 0;
     const long long step = static_cast<long long>(gridDim.x) * kBlock;
     const long long t0 = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x;
@@ -127,7 +157,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int n, long long nv, T *_
     store_partials<2>(v, part, stride, blockIdx.x);
 }
 
-// stores pAp = sum of the SpMV partials
+// stores pAp = sum of the SpMV partials (when K2 does not sum them itself)
 __global__ __launch_bounds__(kFinBlock) void k_fin_pap(const double *part, int nb, KState *st) {
     if (st->done) return;
     double out[1];
@@ -135,37 +165,12 @@ __global__ __launch_bounds__(kFinBlock) void k_fin_pap(const double *part, int n
     if (threadIdx.x == 0) st->pap = out[0];
 }
 
-// iteration bookkeeping after K2 (MODE as in k_pcg_update)
-template <typename T, int MODE>
-__global__ __launch_bounds__(kFinBlock) void k_fin_update(const double *part, int stride, int nb, KState *st,
-                                                          double *hist) {
-    if (st->done) return;
-    double tot[2];
-    reduce_partials<2>(part, stride, nb, tot);
-    if (threadIdx.x == 0) {
-        const int it = st->iter + 1;
-        st->iter = it;
-        const T rrT = static_cast<T>(tot[0]);
-        const double res = static_cast<double>(static_cast<T>(sqrt(static_cast<double>(rrT))));
-        st->rr = tot[0];
-        st->res = res;
-        if (hist) hist[it] = res;
-        const int state = check_state(it, res, st->res0, st);
-        st->state = state;
-        if (state != HFH_ITERATE) {
-            st->done = 1;
-        } else if (MODE == 0) {
-            const T beta = static_cast<T>(st->beta);
-            st->ratio = static_cast<double>(rrT / beta);
-            st->beta = static_cast<double>(rrT);
-        } else if (MODE == 1) {
-            const T gamma = static_cast<T>(tot[1]);
-            const T beta = static_cast<T>(st->beta);
-            st->ratio = static_cast<double>(gamma / beta);
-            st->beta = static_cast<double>(gamma);
-        }
-    }
-}
+// K2 sums the SpMV partials itself when that re-reads at most this many
+// partials over all its workgroups (cheaper than a finisher launch: the
+// 2D Q1 256^2 CG step 17.2 -> 14.0 us); above it, the finisher k_fin_pap
+// (the 3D Q1 128^3 SpMV's 8,008 partials times 1,024 workgroups cost more
+// than the launch they save).
+constexpr long long kFoldReads = 1 << 20;
 
 // partials of r.z after a general preconditioner apply
 template <typename T>
@@ -192,14 +197,43 @@ __device__ __forceinline__ T pcg_dir_one(T p, T r, T z, T d, T ratio) {
     return ratio * p + zi;
 }
 
+// K3 also does the step's bookkeeping (IterateControl::Check on the
+// recursive residual, cg.cc:200-231): every workgroup sums the partials of
+// r.r (slot 0) and r.z (slot `stride`, MODE != 0) of K2 / K2b in the same
+// fixed order and takes the same decision; workgroup 0 records the step
+// (residual, history, state, done, the new beta).  A finished solve leaves
+// p alone.
 template <typename T, int MODE>
 __global__ __launch_bounds__(kBlock) void k_pcg_p(int n, long long nv, T *__restrict__ p, const T *__restrict__ r,
                                                   const T *__restrict__ z, const T *__restrict__ invd,
-                                                  const KState *st) {
+                                                  KState *st, const double *__restrict__ part, int stride, int nb,
+                                                  double *hist) {
     if (st->done) return;
     typedef typename V16<T>::type V;
     constexpr int W = V16<T>::W;
-    const T ratio = static_cast<T>(st->ratio);
+    __shared__ double sh[kBlock / 64];
+    __shared__ double bc;
+    const double trr = block_sum_all(part, nb, sh, &bc);
+    const double trz = MODE != 0 ? block_sum_all(part + stride, nb, sh, &bc) : 0.0;
+    const int it = st->iter;
+    const T rrT = static_cast<T>(trr);
+    const double res = static_cast<double>(static_cast<T>(sqrt(static_cast<double>(rrT))));
+    const int state = check_state(it, res, st->res0, st);
+    const T gamma = MODE == 0 ? rrT : static_cast<T>(trz);
+    const T ratio = gamma / static_cast<T>(st->beta_prev);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st->rr = trr;
+        st->res = res;
+        if (hist) hist[it] = res;
+        st->state = state;
+        if (state != HFH_ITERATE) {
+            st->done = 1;
+        } else {
+            st->ratio = static_cast<double>(ratio);
+            st->beta = static_cast<double>(gamma);
+        }
+    }
+    if (state != HFH_ITERATE) return;
     const long long step = static_cast<long long>(gridDim.x) * kBlock;
     const long long t0 = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x;
     V *__restrict__ pv = reinterpret_cast<V *>(p);
@@ -264,56 +298,50 @@ struct PcgRun {
     const T *idv0 = nullptr;
     int n0 = 0;
 
-    // SpMV with fused p.Ap partials; fin_pap sums them
+    // SpMV with fused p.Ap partials: summed by every K2 workgroup (fold) or
+    // by the finisher k_fin_pap
+    bool fold() const { return static_cast<long long>(A->nwg) * nb <= kFoldReads; }
     int spmv_kernel() { return spmv_launch_ex(A, p, Ap, 0, p, spart, &st->done); }
     int fin_pap() {
+        if (fold()) return HFH_OK;
         hipLaunchKernelGGL(k_fin_pap, dim3(1), dim3(kFinBlock), 0, A->ctx->stream, (const double *)spart, A->nwg, st);
         return launch_check("k_fin_pap");
-    }
-    int spmv() {
-        int rc = spmv_kernel();
-        return rc ? rc : fin_pap();
     }
 
     int tail() {
         hipStream_t s = A->ctx->stream;
+        const int nsp = fold() ? A->nwg : 0;
         int rc;
         if (mode == 0)
-            hipLaunchKernelGGL((k_pcg_update<T, 0, false>), dim3(nb), dim3(kBlock), 0, s, n, nv, x, p, r, Ap, invd, part, stride, (const KState *)st, z, idv0, n0);
+            hipLaunchKernelGGL((k_pcg_update<T, 0, false>), dim3(nb), dim3(kBlock), 0, s, n, nv, x, p, r, Ap, invd, part, stride, st, z, idv0, n0, (const double *)spart, nsp);
         else if (mode == 1)
-            hipLaunchKernelGGL((k_pcg_update<T, 1, false>), dim3(nb), dim3(kBlock), 0, s, n, nv, x, p, r, Ap, invd, part, stride, (const KState *)st, z, idv0, n0);
+            hipLaunchKernelGGL((k_pcg_update<T, 1, false>), dim3(nb), dim3(kBlock), 0, s, n, nv, x, p, r, Ap, invd, part, stride, st, z, idv0, n0, (const double *)spart, nsp);
         else if (f0)
-            hipLaunchKernelGGL((k_pcg_update<T, 2, true>), dim3(nb), dim3(kBlock), 0, s, n, nv, x, p, r, Ap, invd, part, stride, (const KState *)st, z, idv0, n0);
+            hipLaunchKernelGGL((k_pcg_update<T, 2, true>), dim3(nb), dim3(kBlock), 0, s, n, nv, x, p, r, Ap, invd, part, stride, st, z, idv0, n0, (const double *)spart, nsp);
         else
-            hipLaunchKernelGGL((k_pcg_update<T, 2, false>), dim3(nb), dim3(kBlock), 0, s, n, nv, x, p, r, Ap, invd, part, stride, (const KState *)st, z, idv0, n0);
+            hipLaunchKernelGGL((k_pcg_update<T, 2, false>), dim3(nb), dim3(kBlock), 0, s, n, nv, x, p, r, Ap, invd, part, stride, st, z, idv0, n0, (const double *)spart, nsp);
         if ((rc = launch_check("k_pcg_update"))) return rc;
         if (mode == 2) {
             // general preconditioner: z = M^-1 r and the partials of r.z
-            // (slot 1, beside r.r in slot 0), then one finisher for both --
-            // the step count, residual and test of k_fin_update, and the
-            // beta update from r.z (MODE 1's arithmetic).  On the step that
-            // converges, the apply runs once more than needed.
+            // (slot 1, beside r.r in slot 0), both reduced by K3.  On the
+            // step that converges, the apply runs once more than needed.
             if ((rc = prec_apply_raw(P, r, z, &st->done, f0))) return rc;
             hipLaunchKernelGGL(k_pcg_rz<T>, dim3(nb), dim3(kBlock), 0, s, n, r, z, part + stride, stride, (const KState *)st);
             if ((rc = launch_check("k_pcg_rz"))) return rc;
-            hipLaunchKernelGGL((k_fin_update<T, 1>), dim3(1), dim3(kFinBlock), 0, s, (const double *)part, stride, nb, st, hist);
-        } else if (mode == 1) {
-            hipLaunchKernelGGL((k_fin_update<T, 1>), dim3(1), dim3(kFinBlock), 0, s, (const double *)part, stride, nb, st, hist);
-        } else {
-            hipLaunchKernelGGL((k_fin_update<T, 0>), dim3(1), dim3(kFinBlock), 0, s, (const double *)part, stride, nb, st, hist);
         }
-        if ((rc = launch_check("k_fin_update"))) return rc;
+        const double *pt = part;
         if (mode == 0)
-            hipLaunchKernelGGL((k_pcg_p<T, 0>), dim3(nb), dim3(kBlock), 0, s, n, nv, p, r, z, invd, st);
+            hipLaunchKernelGGL((k_pcg_p<T, 0>), dim3(nb), dim3(kBlock), 0, s, n, nv, p, r, z, invd, st, pt, stride, nb, hist);
         else if (mode == 1)
-            hipLaunchKernelGGL((k_pcg_p<T, 1>), dim3(nb), dim3(kBlock), 0, s, n, nv, p, r, z, invd, st);
+            hipLaunchKernelGGL((k_pcg_p<T, 1>), dim3(nb), dim3(kBlock), 0, s, n, nv, p, r, z, invd, st, pt, stride, nb, hist);
         else
-            hipLaunchKernelGGL((k_pcg_p<T, 2>), dim3(nb), dim3(kBlock), 0, s, n, nv, p, r, z, invd, st);
+            hipLaunchKernelGGL((k_pcg_p<T, 2>), dim3(nb), dim3(kBlock), 0, s, n, nv, p, r, z, invd, st, pt, stride, nb, hist);
         return launch_check("k_pcg_p");
     }
 
     int iteration() {
-        int rc = spmv();
+        int rc = spmv_kernel();
+        if (!rc) rc = fin_pap();
         return rc ? rc : tail();
     }
 };

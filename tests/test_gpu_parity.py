@@ -527,6 +527,24 @@ def test_pcg_matches_oracle(pkg, ctx, oracle, case, prec):
     assert relnorm(x, xo) < 1e-6
 
 
+def test_pcg_large_grid_matches_oracle(pkg, ctx, oracle):
+    """3D Q1, 89^3 interior nodes: the SpMV's 2,760 partials times the update
+    kernel's 689 workgroups exceed what the update folds in, so the pAp
+    finisher runs (small systems fold it into the update kernel)."""
+    row, col, val, rhs = pkg.problem(pkg.Q1_3D, 89)
+    A = pkg.CsrMatrix(ctx, row, col, val)
+    P = pkg.Preconditioner.jacobi(A)
+    X = pkg.Vector(ctx, len(rhs))
+    info = pkg.pcg(A, pkg.Vector(ctx, rhs), X, P, maxits=2000, atol=0.0, rtol=1e-8, history=True)
+    OA = oracle.Csr(row, col, val)
+    xo, oi = oracle.cg(OA, rhs, P=oracle.Prec.jacobi(OA), ctrl=(2000, 0.0, 1e-8, 1e6))
+    assert info["state"] == pkg.SUCCESS_REL and oi["status"] == oracle.SUCCESS_REL
+    assert abs(info["iters"] - oi["iters"]) <= 2
+    k = min(20, len(oi["hist"]), len(info["hist"]))
+    assert np.allclose(info["hist"][:k], oi["hist"][:k], rtol=1e-9)
+    assert true_res(row, col, val, X.numpy(), rhs) < 2e-8
+
+
 def test_pcg_mcsgs_matches_oracle(pkg, ctx, oracle):
     row, col, val, rhs = fv3(pkg, 18, sigma=1.0, seed=3)
     nc, color, sizes, perm = pkg.multicoloring(row, col)
