@@ -11,11 +11,15 @@
 //
 // Device layout.  One arena of `hbm_budget` bytes holds every admitted
 // system in a contiguous region (first-fit, 256-byte aligned sections):
-//   sl_ptr[nslice+1] sl_len[n] sl_col[E] v64[E] v32[E] invd[n] b[n] x[n]   (uploaded / assembled)
-//   r64[n] r[n] c[n] z[n] p[n] q[n] part[2*ntiles]                          (work space)
+//   sl_ptr[nslice+1] sl_len[n] cols v64[E] v32[E] invd[n] b[n] x[n]   (uploaded / assembled)
+//   r64[n] r[n] c[n] z[n] p[n] q[n] part[2*ntiles]                      (work space)
 // with the matrix in SELL-64 form (slices of 64 rows, column-major, E
 // padded entries; FP64 values for the true residual, FP32 values for the
-// inner solve, one shared column array).  A slot table holds each system's
+// inner solve, one shared column array).  `cols` is either sl_col[E] or,
+// when the system has at most 256 distinct offsets col - row (every FV
+// system: 5 or 7), one-byte codes into the system's offset table dict[256]:
+// code word w of a lane holds slots 4w..4w+3, ceil(L/4) words per lane of a
+// slice of width L, so the values keep the slice width.  A slot table holds each system's
 // descriptor (pointers, sizes) and its solver state.  A tile table lists
 // every active system's tiles (16 slices = 1024 rows each).
 //
@@ -62,10 +66,13 @@ enum : int {
 };
 
 struct SysDesc {
-    int n, nslice, ntiles, pad;
+    int n, nslice, ntiles, ndict;   // ndict > 0: coded columns
     const int *sl_ptr;
     const int *sl_len;
     const int *sl_col;
+    const unsigned *sl_code;
+    const int *sl_wptr;
+    const int *dict;
     const double *v64;
     const float *v32;
     const float *invd;
@@ -83,8 +90,10 @@ template <typename T>
 using gptr = __attribute__((address_space(1))) T *;
 
 struct GDesc {
-    int n, nslice, ntiles;
+    int n, nslice, ntiles, ndict;
     gptr<const int> sl_ptr, sl_len, sl_col;
+    gptr<const unsigned> sl_code;
+    gptr<const int> sl_wptr, dict;
     gptr<const double> v64;
     gptr<const float> v32, invd;
     gptr<const double> b;
@@ -98,9 +107,13 @@ __device__ __forceinline__ GDesc gview(const SysDesc &d) {
     g.n = d.n;
     g.nslice = d.nslice;
     g.ntiles = d.ntiles;
+    g.ndict = d.ndict;
     g.sl_ptr = (gptr<const int>)d.sl_ptr;
     g.sl_len = (gptr<const int>)d.sl_len;
     g.sl_col = (gptr<const int>)d.sl_col;
+    g.sl_code = (gptr<const unsigned>)d.sl_code;
+    g.sl_wptr = (gptr<const int>)d.sl_wptr;
+    g.dict = (gptr<const int>)d.dict;
     g.v64 = (gptr<const double>)d.v64;
     g.v32 = (gptr<const float>)d.v32;
     g.invd = (gptr<const float>)d.invd;
@@ -169,6 +182,41 @@ __device__ __forceinline__ T sell_row(const GDesc &D, gptr<const T> val, gptr<co
     return acc;
 }
 
+// The same row sum over coded columns: column = min(r + sdict[code], n-1)
+// (padding slots hold the code of offset 0; rows past n read x[n-1] and are
+// not stored).  Word w of a lane (slots 4w..4w+3) sits at
+// sl_wptr[s] + 64 w + lane; the values keep the slice width L.  Two words
+// (8 slots) per step, the slots past L skipped (wave-uniform).
+template <typename T, typename V>
+__device__ __forceinline__ T sell_row_coded(const GDesc &D, gptr<const T> val, gptr<const V> v, int s, int lane,
+                                            int len, int r, const int *sdict) {
+    const int base = D.sl_ptr[s];
+    const int L = (D.sl_ptr[s + 1] - base) >> 6;
+    const int wb = D.sl_wptr[s];
+    T acc = 0;
+    for (int w = 0; 4 * w < L; w += 2) {
+        unsigned cw[2];
+        T a[8];
+        V xv[8];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) cw[h] = 4 * (w + h) < L ? __builtin_nontemporal_load(&D.sl_code[wb + (w + h) * 64 + lane]) : 0u;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int k = 4 * w + u;
+            a[u] = 0;
+            xv[u] = 0;
+            if (k < L) {   // wave-uniform
+                a[u] = __builtin_nontemporal_load(&val[base + k * 64 + lane]);
+                xv[u] = v[min(r + sdict[(cw[u >> 2] >> (8 * (u & 3))) & 255u], D.n - 1)];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (4 * w + u < len) acc = badd(acc, bmul(a[u], static_cast<T>(xv[u])));
+    }
+    return acc;
+}
+
 __device__ __forceinline__ bool is_inner_spmv(int ph) { return ph == P_START || ph == P_GO; }
 __device__ __forceinline__ bool is_outer_spmv(int ph) { return ph == P_FIRST || ph == P_CORRECT; }
 
@@ -176,6 +224,7 @@ __device__ __forceinline__ bool is_outer_spmv(int ph) { return ph == P_FIRST || 
 __global__ __launch_bounds__(kBlock) void kb_spmv(const SysDesc *__restrict__ sd, const SysState *__restrict__ st,
                                                   const Tile *__restrict__ tiles, const int *__restrict__ ntiles) {
     __shared__ double sh[kBlock / 64];
+    __shared__ int sdict[256];
     const int nt = *ntiles;
     // the wave index in a scalar register: slice quantities are then scalar
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
@@ -185,18 +234,25 @@ __global__ __launch_bounds__(kBlock) void kb_spmv(const SysDesc *__restrict__ sd
         const bool inner = is_inner_spmv(ph);
         if (!inner && !is_outer_spmv(ph)) continue;   // uniform over the workgroup
         const GDesc D = gview(sd[T.sys]);
+        const bool coded = D.ndict > 0;
+        if (coded) {   // the system's offset table (the previous tile's readers passed the barrier below)
+            for (int i = threadIdx.x; i < D.ndict; i += kBlock) sdict[i] = D.dict[i];
+            __syncthreads();
+        }
         double dsum = 0;
         for (int s = T.s0 + wave; s < T.s0 + T.ns; s += kBlock / 64) {
             const int r = s * 64 + lane;
             const int len = r < D.n ? D.sl_len[r] : 0;
             if (inner) {
-                const float acc = sell_row<float, float>(D, D.v32, gptr<const float>(D.p), s, lane, len);
+                const float acc = coded ? sell_row_coded<float, float>(D, D.v32, gptr<const float>(D.p), s, lane, len, r, sdict)
+                                        : sell_row<float, float>(D, D.v32, gptr<const float>(D.p), s, lane, len);
                 if (r < D.n) {
                     D.q[r] = acc;
                     dsum += static_cast<double>(D.p[r]) * static_cast<double>(acc);
                 }
             } else {
-                const double acc = sell_row<double, double>(D, D.v64, gptr<const double>(D.x), s, lane, len);
+                const double acc = coded ? sell_row_coded<double, double>(D, D.v64, gptr<const double>(D.x), s, lane, len, r, sdict)
+                                         : sell_row<double, double>(D, D.v64, gptr<const double>(D.x), s, lane, len);
                 if (r < D.n) {
                     const double rr = D.b[r] - acc;
                     D.r64[r] = rr;
@@ -367,36 +423,53 @@ __global__ void kb_set_slot(SysDesc *__restrict__ sd, SysState *__restrict__ st,
 // Device assembly of the cell-centred FV diffusion matrix (5-pt when nz == 1,
 // else 7-pt) in SELL-64 form of uniform width W from the cell coefficients:
 // the same entries, in the same order and with the same arithmetic as the
-// host generator (harmonic-mean faces, 2 k on Dirichlet faces).  One thread
+// host generator (harmonic-mean faces, 2 k on Dirichlet faces), the columns
+// as one-byte codes into the offset table (written here too).  One thread
 // per row slot; rows beyond n (last slice) are padding.
 __global__ __launch_bounds__(kBlock) void kb_fv_assemble(int nx, int ny, int nz, int n, int nslice, int W,
                                                          const double *__restrict__ kc, int *__restrict__ sl_ptr,
-                                                         int *__restrict__ sl_len, int *__restrict__ sl_col,
+                                                         int *__restrict__ sl_len, unsigned *__restrict__ sl_code,
+                                                         int *__restrict__ sl_wptr, int *__restrict__ dict,
                                                          double *__restrict__ v64, float *__restrict__ v32,
                                                          float *__restrict__ invd) {
 #pragma clang fp contract(off)
+    const bool in3 = nz > 1;
+    const long long nxy = static_cast<long long>(nx) * ny;
+    // offset table, in the order of a row's entries: -nxy, -nx, -1, 0, 1, nx, nxy
+    // (2D: without +-nxy); code0 = the code of offset 0 (padding)
+    const int nd = in3 ? 7 : 5, code0 = in3 ? 3 : 2;
+    const int WW = (W + 3) / 4;   // code words per lane and slice
     const int r = blockIdx.x * kBlock + threadIdx.x;
     if (r <= nslice && r >= 0) {
-        // slice pointers (nslice+1 of them) from the first threads
+        // slice and word pointers (nslice+1 of each) from the first threads
         sl_ptr[r] = r * 64 * W;
+        sl_wptr[r] = r * 64 * WW;
+    }
+    if (r < nd) {
+        const long long tab3[7] = {-nxy, -nx, -1, 0, 1, nx, nxy};
+        const long long tab2[5] = {-static_cast<long long>(nx), -1, 0, 1, nx};
+        dict[r] = static_cast<int>(in3 ? tab3[r] : tab2[r]);
     }
     if (r >= nslice * 64) return;
     const int s = r >> 6, lane = r & 63;
-    const int base = s * 64 * W;
+    const int base = s * 64 * W, wbase = s * 64 * WW;
+    unsigned words[2] = {0u, 0u};   // WW <= 2
+    auto code_at = [&](int e, unsigned c) { words[e >> 2] |= c << (8 * (e & 3)); };
     if (r >= n) {
         for (int e = 0; e < W; ++e) {
-            sl_col[base + e * 64 + lane] = n - 1;
             v64[base + e * 64 + lane] = 0.0;
             v32[base + e * 64 + lane] = 0.0f;
         }
+        for (int e = 0; e < 4 * WW; ++e) code_at(e, code0);
+        for (int w = 0; w < WW; ++w) sl_code[wbase + w * 64 + lane] = words[w];
         return;
     }
     const int i = r % nx, j = (r / nx) % ny, k = r / (nx * ny);
-    const bool in3 = nz > 1;
-    const long long nxy = static_cast<long long>(nx) * ny;
     const long long nb[6] = {r - nxy, r - nx, r - 1LL, r + 1LL, r + nx, r + nxy};
     const bool has[6] = {in3 && k > 0, j > 0, i > 0, i < nx - 1, j < ny - 1, in3 && k < nz - 1};
     const bool axis[6] = {in3, true, true, true, true, in3};
+    // code of neighbour q (the offset table skips +-nxy in 2D)
+    const unsigned qcode[6] = {0u, in3 ? 1u : 0u, in3 ? 2u : 1u, in3 ? 4u : 3u, in3 ? 5u : 4u, 6u};
     const double ks = kc[r];
     double off[6] = {0, 0, 0, 0, 0, 0};
     double diag = 0;
@@ -412,25 +485,26 @@ __global__ __launch_bounds__(kBlock) void kb_fv_assemble(int nx, int ny, int nz,
         }
     }
     int e = 0;
-    auto put = [&](int c, double v) {
+    auto put = [&](unsigned code, double v) {
         const int idx = base + e * 64 + lane;
-        sl_col[idx] = c;
+        code_at(e, code);
         v64[idx] = v;
         v32[idx] = static_cast<float>(v);
         ++e;
     };
     for (int q = 0; q < 3; ++q)
-        if (has[q]) put(static_cast<int>(nb[q]), off[q]);
-    put(r, diag);
+        if (has[q]) put(qcode[q], off[q]);
+    put(static_cast<unsigned>(code0), diag);
     for (int q = 3; q < 6; ++q)
-        if (has[q]) put(static_cast<int>(nb[q]), off[q]);
+        if (has[q]) put(qcode[q], off[q]);
     sl_len[r] = e;
     for (int f = e; f < W; ++f) {
         const int idx = base + f * 64 + lane;
-        sl_col[idx] = r;
         v64[idx] = 0.0;
         v32[idx] = 0.0f;
     }
+    for (int f = e; f < 4 * WW; ++f) code_at(f, code0);
+    for (int w = 0; w < WW; ++w) sl_code[wbase + w * 64 + lane] = words[w];
     invd[r] = 1.0f / static_cast<float>(diag);
 }
 
@@ -478,13 +552,18 @@ struct Arena {
 size_t up(size_t b) { return (b + kAlign - 1) / kAlign * kAlign; }
 
 // byte offsets of a system's sections inside its region
+// words > 0: coded columns (`words` code words, a word pointer per slice and
+// the offset table) instead of E column indices
 struct Layout {
-    size_t ptr, len, col, v64, v32, invd, b, x, upload, r64, r, c, z, p, q, part, total;
-    Layout(int n, int nslice, long long E, int ntiles) {
+    size_t ptr, len, col, wptr, dict, v64, v32, invd, b, x, upload, r64, r, c, z, p, q, part, total;
+    long long words;
+    Layout(int n, int nslice, long long E, int ntiles, long long words_ = 0) : words(words_) {
         size_t o = 0;
         ptr = o; o += up(sizeof(int) * (nslice + 1));
         len = o; o += up(sizeof(int) * static_cast<size_t>(n));
-        col = o; o += up(sizeof(int) * static_cast<size_t>(E));
+        col = o; o += up(words ? sizeof(unsigned) * static_cast<size_t>(words) : sizeof(int) * static_cast<size_t>(E));
+        wptr = o; o += words ? up(sizeof(int) * (nslice + 1)) : 0;
+        dict = o; o += words ? up(sizeof(int) * 256) : 0;
         v64 = o; o += up(sizeof(double) * static_cast<size_t>(E));
         v32 = o; o += up(sizeof(float) * static_cast<size_t>(E));
         invd = o; o += up(sizeof(float) * static_cast<size_t>(n));
@@ -607,7 +686,7 @@ int reserve(hfh_batch *B, size_t bytes, int *slot, size_t *off) {
 // Makes an uploaded system live: the main stream waits for the upload, then
 // writes the slot's descriptor and initial state; the tile table is rebuilt
 // before the next replay.
-int activate(hfh_batch *B, int slot, size_t off, const Layout &L, int n, int nslice, size_t bytes, int *id) {
+int activate(hfh_batch *B, int slot, size_t off, const Layout &L, int n, int nslice, size_t bytes, int ndict, int *id) {
     hipStream_t s = B->ctx->stream;
     char *base = B->arena_d + off;
     SysDesc d{};
@@ -616,7 +695,14 @@ int activate(hfh_batch *B, int slot, size_t off, const Layout &L, int n, int nsl
     d.ntiles = tiles_of(nslice);
     d.sl_ptr = reinterpret_cast<const int *>(base + L.ptr);
     d.sl_len = reinterpret_cast<const int *>(base + L.len);
-    d.sl_col = reinterpret_cast<const int *>(base + L.col);
+    if (L.words) {
+        d.sl_code = reinterpret_cast<const unsigned *>(base + L.col);
+        d.sl_wptr = reinterpret_cast<const int *>(base + L.wptr);
+        d.dict = reinterpret_cast<const int *>(base + L.dict);
+        d.ndict = ndict;
+    } else {
+        d.sl_col = reinterpret_cast<const int *>(base + L.col);
+    }
     d.v64 = reinterpret_cast<const double *>(base + L.v64);
     d.v32 = reinterpret_cast<const float *>(base + L.v32);
     d.invd = reinterpret_cast<const float *>(base + L.invd);
@@ -839,12 +925,16 @@ static int fv_shape(int nx, int ny, int nz, int *n, int *W) {
     return HFH_OK;
 }
 
+// FV systems store coded columns: 5 or 7 offsets, ceil(W / 4) = 2 code words
+// per lane and slice
+static long long fv_words(int W, int nsl) { return 64LL * ((W + 3) / 4) * nsl; }
+
 extern "C" int hfh_batch_bytes_fv(int nx, int ny, int nz, long long *bytes) {
     HFH_CHECK(bytes, HFH_ERR_INVALID, "hfh_batch_bytes_fv: null argument");
     int n, W, rc;
     if ((rc = fv_shape(nx, ny, nz, &n, &W))) return rc;
     const int nsl = (n + 63) / 64;
-    *bytes = static_cast<long long>(Layout(n, nsl, 64LL * W * nsl, tiles_of(nsl)).total);
+    *bytes = static_cast<long long>(Layout(n, nsl, 64LL * W * nsl, tiles_of(nsl), fv_words(W, nsl)).total);
     return HFH_OK;
 }
 
@@ -911,7 +1001,7 @@ extern "C" int hfh_batch_submit_csr(hfh_batch *B, int n, const int *row, const i
         set_error("hfh_batch_submit_csr: upload failed");
         return undo(HFH_ERR_HIP);
     }
-    if ((rc = activate(B, slot, off, L, n, nsl, L.total, id))) return undo(rc);
+    if ((rc = activate(B, slot, off, L, n, nsl, L.total, 0, id))) return undo(rc);
     return HFH_OK;
 }
 
@@ -926,7 +1016,7 @@ extern "C" int hfh_batch_submit_fv(hfh_batch *B, int nx, int ny, int nz, const d
     const int nsl = (n + 63) / 64;
     const long long E = 64LL * W * nsl;
     HFH_CHECK(E < (1LL << 31), HFH_ERR_UNSUPPORTED, "hfh_batch_submit_fv: SELL copy exceeds 2^31 entries");
-    const Layout L(n, nsl, E, tiles_of(nsl));
+    const Layout L(n, nsl, E, tiles_of(nsl), fv_words(W, nsl));
     int slot;
     size_t off;
     if ((rc = reserve(B, L.total, &slot, &off))) return rc;
@@ -954,12 +1044,13 @@ extern "C" int hfh_batch_submit_fv(hfh_batch *B, int nx, int ny, int nz, const d
     const int threads = std::max(nsl * 64, nsl + 1);
     hipLaunchKernelGGL(kb_fv_assemble, dim3((threads + kBlock - 1) / kBlock), dim3(kBlock), 0, cs, nx, ny, nz, n, nsl,
                        W, reinterpret_cast<const double *>(base + L.r64), reinterpret_cast<int *>(base + L.ptr),
-                       reinterpret_cast<int *>(base + L.len), reinterpret_cast<int *>(base + L.col),
+                       reinterpret_cast<int *>(base + L.len), reinterpret_cast<unsigned *>(base + L.col),
+                       reinterpret_cast<int *>(base + L.wptr), reinterpret_cast<int *>(base + L.dict),
                        reinterpret_cast<double *>(base + L.v64), reinterpret_cast<float *>(base + L.v32),
                        reinterpret_cast<float *>(base + L.invd));
     if ((rc = launch_check("kb_fv_assemble"))) return undo(rc);
     if (hipEventRecord(B->ev_up, cs) != hipSuccess) return undo(hip_fail(hipErrorUnknown, "hfh_batch_submit_fv"));
-    if ((rc = activate(B, slot, off, L, n, nsl, L.total, id))) return undo(rc);
+    if ((rc = activate(B, slot, off, L, n, nsl, L.total, nz > 1 ? 7 : 5, id))) return undo(rc);
     return HFH_OK;
 }
 
