@@ -21,7 +21,7 @@
 // code word w of a lane holds slots 4w..4w+3, ceil(L/4) words per lane of a
 // slice of width L, so the values keep the slice width.  A slot table holds each system's
 // descriptor (pointers, sizes) and its solver state.  A tile table lists
-// every active system's tiles (16 slices = 1024 rows each).
+// every active system's tiles (32 slices = 2048 rows each).
 //
 // One CG iteration of all systems is five launches:
 //   K1 tiles : inner  q = A32 p, partial p.q      | outer  r64 = b - A64 x, partial r64.r64
@@ -47,7 +47,7 @@
 namespace hfh {
 namespace {
 
-constexpr int kTileSlices = 16;     // 1024 rows per tile
+constexpr int kTileSlices = 32;     // 2048 rows per tile
 constexpr int kTileRows = kTileSlices * 64;
 constexpr int kTileGrid = 2048;     // workgroups of the tile kernels
 constexpr int kSellU = 8;           // column slots per SELL step
