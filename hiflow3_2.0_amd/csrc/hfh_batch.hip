@@ -88,6 +88,8 @@ struct SysDesc {
 // and generic (flat) loads make every wait drain the LDS counter as well.
 template <typename T>
 using gptr = __attribute__((address_space(1))) T *;
+typedef float bf4 __attribute__((ext_vector_type(4)));
+typedef double bd2 __attribute__((ext_vector_type(2)));
 
 struct GDesc {
     int n, nslice, ntiles, ndict;
@@ -331,20 +333,57 @@ __global__ __launch_bounds__(kBlock) void kb_update(const SysDesc *__restrict__ 
         const float a = static_cast<float>(st[T.sys].alpha);
         const int r0 = T.s0 * 64, r1 = min(D.n, (T.s0 + T.ns) * 64);
         double rr = 0, rz = 0;
-        for (int i = r0 + threadIdx.x; i < r1; i += kBlock) {
+        // 4 rows per thread and step with 16-byte accesses (a tile starts at a
+        // multiple of 2048 rows and every array at a 256-byte boundary), the
+        // rows past the last multiple of 4 one by one
+        auto one = [&](float &ci, float &ri_io, float pi, float qi, double r64i, float di, float &zi) {
             float ri;
             if (ph == P_UPD) {
-                D.c[i] = D.c[i] + a * D.p[i];
-                ri = D.r[i] - a * D.q[i];
+                ci = ci + a * pi;
+                ri = ri_io - a * qi;
             } else {
-                ri = static_cast<float>(D.r64[i]);
-                D.c[i] = 0.0f;
+                ri = static_cast<float>(r64i);
+                ci = 0.0f;
             }
-            D.r[i] = ri;
-            const float zi = D.invd[i] * ri;
-            D.z[i] = zi;
+            ri_io = ri;
+            zi = di * ri;
             rr += static_cast<double>(ri) * static_cast<double>(ri);
             rz += static_cast<double>(ri) * static_cast<double>(zi);
+        };
+        const int rv = r0 + ((r1 - r0) & ~3);
+        for (int i = r0 + 4 * threadIdx.x; i < rv; i += 4 * kBlock) {
+            bf4 c4 = ph == P_UPD ? *reinterpret_cast<gptr<const bf4>>(&D.c[i]) : bf4(0.0f);
+            bf4 r4 = ph == P_UPD ? *reinterpret_cast<gptr<const bf4>>(&D.r[i]) : bf4(0.0f);
+            bf4 p4 = bf4(0.0f), q4 = bf4(0.0f);
+            bd2 ra = bd2(0.0), rb = bd2(0.0);
+            if (ph == P_UPD) {
+                p4 = *reinterpret_cast<gptr<const bf4>>(&D.p[i]);
+                q4 = *reinterpret_cast<gptr<const bf4>>(&D.q[i]);
+            } else {
+                ra = *reinterpret_cast<gptr<const bd2>>(&D.r64[i]);
+                rb = *reinterpret_cast<gptr<const bd2>>(&D.r64[i + 2]);
+            }
+            const bf4 d4 = *reinterpret_cast<gptr<const bf4>>(&D.invd[i]);
+            bf4 z4;
+#pragma unroll
+            for (int w = 0; w < 4; ++w) {
+                float ci = c4[w], ri = r4[w], zi;
+                one(ci, ri, p4[w], q4[w], w < 2 ? ra[w] : rb[w - 2], d4[w], zi);
+                c4[w] = ci;
+                r4[w] = ri;
+                z4[w] = zi;
+            }
+            *reinterpret_cast<gptr<bf4>>(&D.c[i]) = c4;
+            *reinterpret_cast<gptr<bf4>>(&D.r[i]) = r4;
+            *reinterpret_cast<gptr<bf4>>(&D.z[i]) = z4;
+        }
+        for (int i = rv + threadIdx.x; i < r1; i += kBlock) {
+            float ci = ph == P_UPD ? D.c[i] : 0.0f, ri = ph == P_UPD ? D.r[i] : 0.0f, zi;
+            one(ci, ri, ph == P_UPD ? D.p[i] : 0.0f, ph == P_UPD ? D.q[i] : 0.0f, ph == P_UPD ? 0.0 : D.r64[i],
+                D.invd[i], zi);
+            D.c[i] = ci;
+            D.r[i] = ri;
+            D.z[i] = zi;
         }
         const double trr = block_sum(rr, sh);
         __syncthreads();
@@ -399,13 +438,36 @@ __global__ __launch_bounds__(kBlock) void kb_dir(const SysDesc *__restrict__ sd,
         if (ph != P_GO && ph != P_START && ph != P_CORRECT) continue;
         const GDesc D = gview(sd[T.sys]);
         const int r0 = T.s0 * 64, r1 = min(D.n, (T.s0 + T.ns) * 64);
+        const int rv = r0 + ((r1 - r0) & ~3);   // 16-byte accesses up to here, as in kb_update
         if (ph == P_GO) {
             const float beta = static_cast<float>(st[T.sys].beta);
-            for (int i = r0 + threadIdx.x; i < r1; i += kBlock) D.p[i] = D.z[i] + beta * D.p[i];
+            for (int i = r0 + 4 * threadIdx.x; i < rv; i += 4 * kBlock) {
+                const bf4 z4 = *reinterpret_cast<gptr<const bf4>>(&D.z[i]);
+                bf4 p4 = *reinterpret_cast<gptr<const bf4>>(&D.p[i]);
+                p4.x = z4.x + beta * p4.x;
+                p4.y = z4.y + beta * p4.y;
+                p4.z = z4.z + beta * p4.z;
+                p4.w = z4.w + beta * p4.w;
+                *reinterpret_cast<gptr<bf4>>(&D.p[i]) = p4;
+            }
+            for (int i = rv + threadIdx.x; i < r1; i += kBlock) D.p[i] = D.z[i] + beta * D.p[i];
         } else if (ph == P_START) {
-            for (int i = r0 + threadIdx.x; i < r1; i += kBlock) D.p[i] = D.z[i];
+            for (int i = r0 + 4 * threadIdx.x; i < rv; i += 4 * kBlock)
+                *reinterpret_cast<gptr<bf4>>(&D.p[i]) = *reinterpret_cast<gptr<const bf4>>(&D.z[i]);
+            for (int i = rv + threadIdx.x; i < r1; i += kBlock) D.p[i] = D.z[i];
         } else {
-            for (int i = r0 + threadIdx.x; i < r1; i += kBlock) D.x[i] += static_cast<double>(D.c[i]);
+            for (int i = r0 + 4 * threadIdx.x; i < rv; i += 4 * kBlock) {
+                const bf4 c4 = *reinterpret_cast<gptr<const bf4>>(&D.c[i]);
+                bd2 xa = *reinterpret_cast<gptr<const bd2>>(&D.x[i]);
+                bd2 xb = *reinterpret_cast<gptr<const bd2>>(&D.x[i + 2]);
+                xa.x += static_cast<double>(c4.x);
+                xa.y += static_cast<double>(c4.y);
+                xb.x += static_cast<double>(c4.z);
+                xb.y += static_cast<double>(c4.w);
+                *reinterpret_cast<gptr<bd2>>(&D.x[i]) = xa;
+                *reinterpret_cast<gptr<bd2>>(&D.x[i + 2]) = xb;
+            }
+            for (int i = rv + threadIdx.x; i < r1; i += kBlock) D.x[i] += static_cast<double>(D.c[i]);
         }
     }
 }
