@@ -207,6 +207,28 @@ def test_spmv_lds_solvers_identical_to_pairs(pkg, ctx, solver):
         assert np.array_equal(x, out[0][1])
 
 
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("algo", [(4, 8), (5, 1), (6, 1), (7, 0), (1, 2048)])
+def test_spmv_new_values_every_layout(pkg, ctx, oracle, algo, dtype):
+    """New values for the same structure (upload_values) reach each layout's
+    copy (values refilled from the CSR arrays; columns, codes and chunk lists
+    kept): the SpMV stays bit-identical."""
+    row, col, val, _ = pkg.problem(pkg.Q1_3D, 13)
+    n = len(row) - 1
+    rng = np.random.default_rng(8)
+    x = rng.uniform(-1, 1, n).astype(dtype)
+    A = pkg.CsrMatrix(ctx, row, col, val.astype(dtype))
+    A.set_spmv(*algo)
+    assert A.get_spmv()[0] == algo[0]
+    X, Y = pkg.Vector(ctx, x, dtype), pkg.Vector(ctx, n, dtype)
+    A.spmv(X, Y)
+    assert np.array_equal(Y.numpy(), oracle.spmv(oracle.Csr(row, col, val.astype(dtype)), x))
+    val2 = (val * rng.uniform(0.5, 1.5, len(val))).astype(dtype)
+    A.upload_values(val2)
+    A.spmv(X, Y)
+    assert np.array_equal(Y.numpy(), oracle.spmv(oracle.Csr(row, col, val2), x))
+
+
 def test_spmv_layout_choice(pkg, ctx):
     """The automatic SpMV layout and the LDS-window and dictionary fallbacks."""
     row, col, val, _ = pkg.problem(pkg.Q1_3D, 12)
