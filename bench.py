@@ -196,7 +196,7 @@ def bench_pcg_q1(P, args):
     gbs = b_spmv / (avg_spmv_ms * 1e-3) / 1e9
     ms_step = wall * 1e3 / args.steps
     ms_iter = ms_step / max(iters, 1)
-    it_bytes = 12 * nnz + 108 * n
+    it_bytes = 12 * nnz + 100 * n
     traffic = read_traffic("spmv_q1_3d_128_traffic.txt")
     log(f"[bench] {args.steps} solves: {ms_step:.2f} ms/solve, {iters} iters, {ms_iter * 1e3:.1f} us/iter, "
         f"spmv {avg_spmv_ms * 1e3:.1f} us ({gbs:.0f} GB/s), true rel residual {true_rel:.2e}")

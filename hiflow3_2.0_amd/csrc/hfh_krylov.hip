@@ -5,19 +5,20 @@
 //   K1  Ap = A p with per-workgroup partial sums of Ap.p (summed by a
 //       finisher when the folding below would re-read too many partials)
 //   K2  every workgroup sums the pAp partials (the same fixed order in each);
-//       alpha = beta/pAp; x += alpha p; r -= alpha Ap; partials of r.r (and
-//       r.z with z = r.*invD for Jacobi); workgroup 0 advances the step
-//   K3  every workgroup sums the r.r / r.z partials and applies
+//       alpha = beta/pAp; r -= alpha Ap; partials of r.r (and r.z with
+//       z = r.*invD for Jacobi); workgroup 0 advances the step
+//   K3  x += alpha p (one read of p serves this and the direction update);
+//       every workgroup sums the r.r / r.z partials and applies
 //       IterateControl::Check to res = sqrt(r.r); workgroup 0 records the
 //       step and the new beta; unless finished, p = beta_ratio p + z
 // With another preconditioner, [M] z = M^-1 r and K2b (partials of r.z) run
 // between K2 and K3; for a multicolour SGS, K2 also writes the colour-0
 // forward step of [M].  No kernel reads a state field that another
 // workgroup of the same launch writes (beta_prev keeps K2's beta for K3).
-// Every kernel returns at once when the done flag is set, so the host
-// enqueues chunks of iterations (a captured hipGraph when possible) and reads
-// the state once per chunk.  Compulsory traffic per Jacobi iteration:
-// 12 nnz + 108 n bytes (FP64).
+// Every kernel returns at once when the solve is done (K3 on the idle flag
+// that K2 sets then), so the host enqueues chunks of iterations (a captured
+// hipGraph when possible) and reads the state once per chunk.  Compulsory
+// traffic per Jacobi iteration: 12 nnz + 100 n bytes (FP64).
 #include "hfh_internal.h"
 
 #include <algorithm>
@@ -33,6 +34,7 @@ struct KState {
     double res, res0;
     double atol, rtol, dtol;
     int iter, done, state, maxits;
+    int idle;               // set by K2 when it finds the solve done; K3 returns on it
 };
 
 __device__ __forceinline__ int check_state(int step, double v, double first, const KState *s) {
@@ -67,12 +69,12 @@ static inline int host_check(int step, double v, double first, const hfh_control
 
 // MODE 0: no preconditioner (z = r), 1: Jacobi (z = r.*invd), 2: general (z
 // computed by a separate apply; K2 reduces r.r only).
-// x += alpha p, r -= alpha Ap, partials of r.r and (MODE 1) r.(invD r).
+// K2: r -= alpha Ap, partials of r.r and (MODE 1) r.(invD r).  The step's
+// x += alpha p runs in K3, which reads p anyway (one pass over p fewer).
 // The vectors are whole device allocations: nv 16-byte vectors of each are
 // read as such, the n mod W elements after them one by one.
 template <typename T, int MODE>
-__device__ __forceinline__ void pcg_update_one(T &x, T p, T &r, T a, T d, T alpha, double &rr, double &rz) {
-    x += alpha * p;
+__device__ __forceinline__ void pcg_update_one(T &r, T a, T d, T alpha, double &rr, double &rz) {
     const T ri = r + (-alpha) * a;
     r = ri;
     rr += static_cast<double>(ri) * static_cast<double>(ri);
@@ -86,13 +88,18 @@ __device__ __forceinline__ void pcg_update_one(T &x, T p, T &r, T a, T d, T alph
 // preconditioner's first forward step, z_i = (-r_i) * idv_i for the rows
 // i < n0 of colour 0 (k_mc_fwd0's arithmetic), from the new r in registers.
 template <typename T, int MODE, bool F0>
-__global__ __launch_bounds__(kBlock) void k_pcg_update(int n, long long nv, T *__restrict__ x, const T *__restrict__ p,
-                                                       T *__restrict__ r, const T *__restrict__ Ap,
+__global__ __launch_bounds__(kBlock) void k_pcg_update(int n, long long nv, T *__restrict__ r, const T *__restrict__ Ap,
                                                        const T *__restrict__ invd, double *part, int stride,
                                                        KState *st, T *__restrict__ z0,
                                                        const T *__restrict__ idv0, int n0,
                                                        const double *__restrict__ spart, int nsp) {
-    if (st->done) return;
+    // K3 cannot test `done` itself: its workgroup 0 sets it on the step that
+    // ends the solve while later workgroups of that launch still run the
+    // step's x update.  K2 sees `done` only on the launches after that step.
+    if (st->done) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) st->idle = 1;
+        return;
+    }
     typedef typename V16<T>::type V;
     constexpr int W = V16<T>::W;
     __shared__ double sh[kBlock / 64];
@@ -112,24 +119,20 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int n, long long nv, T *_
     double rr = 0, rz = 0;
     const long long step = static_cast<long long>(gridDim.x) * kBlock;
     const long long t0 = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x;
-    V *__restrict__ xv = reinterpret_cast<V *>(x);
     V *__restrict__ rv = reinterpret_cast<V *>(r);
-    const V *__restrict__ pv = reinterpret_cast<const V *>(p);
     const V *__restrict__ av = reinterpret_cast<const V *>(Ap);
     const V *__restrict__ dv = reinterpret_cast<const V *>(invd);
     for (long long i = t0; i < nv; i += step) {
-        V xi = xv[i], ri = rv[i];
-        const V pi = pv[i], ai = av[i];
+        V ri = rv[i];
+        const V ai = av[i];
         V di{};
         if (MODE == 1) di = dv[i];
 #pragma unroll
         for (int w = 0; w < W; ++w) {
-            T xw = xi[w], rw = ri[w];
-            pcg_update_one<T, MODE>(xw, pi[w], rw, ai[w], MODE == 1 ? di[w] : T(0), alpha, rr, rz);
-            xi[w] = xw;
+            T rw = ri[w];
+            pcg_update_one<T, MODE>(rw, ai[w], MODE == 1 ? di[w] : T(0), alpha, rr, rz);
             ri[w] = rw;
         }
-        xv[i] = xi;
         rv[i] = ri;
         if (F0) {
             const long long e0 = i * W;
@@ -147,7 +150,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int n, long long nv, T *_
         }
     }
     for (long long j = nv * W + t0; j < n; j += step) {
-        pcg_update_one<T, MODE>(x[j], p[j], r[j], Ap[j], MODE == 1 ? invd[j] : T(0), alpha, rr, rz);
+        pcg_update_one<T, MODE>(r[j], Ap[j], MODE == 1 ? invd[j] : T(0), alpha, rr, rz);
         if (F0 && j < n0) z0[j] = (r[j] * static_cast<T>(-1)) * idv0[j];
     }
     double v[2];
@@ -197,18 +200,18 @@ __device__ __forceinline__ T pcg_dir_one(T p, T r, T z, T d, T ratio) {
     return ratio * p + zi;
 }
 
-// K3 also does the step's bookkeeping (IterateControl::Check on the
-// recursive residual, cg.cc:200-231): every workgroup sums the partials of
-// r.r (slot 0) and r.z (slot `stride`, MODE != 0) of K2 / K2b in the same
-// fixed order and takes the same decision; workgroup 0 records the step
-// (residual, history, state, done, the new beta).  A finished solve leaves
-// p alone.
+// K3: x += alpha p (cg.cc:213, with K2's alpha = beta_prev / pAp), then the
+// step's bookkeeping (IterateControl::Check on the recursive residual,
+// cg.cc:200-231): every workgroup sums the partials of r.r (slot 0) and r.z
+// (slot `stride`, MODE != 0) of K2 / K2b in the same fixed order and takes
+// the same decision; workgroup 0 records the step (residual, history, state,
+// done, the new beta).  A finished solve updates x and leaves p alone.
 template <typename T, int MODE>
-__global__ __launch_bounds__(kBlock) void k_pcg_p(int n, long long nv, T *__restrict__ p, const T *__restrict__ r,
-                                                  const T *__restrict__ z, const T *__restrict__ invd,
-                                                  KState *st, const double *__restrict__ part, int stride, int nb,
-                                                  double *hist) {
-    if (st->done) return;
+__global__ __launch_bounds__(kBlock) void k_pcg_p(int n, long long nv, T *__restrict__ x, T *__restrict__ p,
+                                                  const T *__restrict__ r, const T *__restrict__ z,
+                                                  const T *__restrict__ invd, KState *st,
+                                                  const double *__restrict__ part, int stride, int nb, double *hist) {
+    if (st->idle) return;
     typedef typename V16<T>::type V;
     constexpr int W = V16<T>::W;
     __shared__ double sh[kBlock / 64];
@@ -221,6 +224,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_p(int n, long long nv, T *__rest
     const int state = check_state(it, res, st->res0, st);
     const T gamma = MODE == 0 ? rrT : static_cast<T>(trz);
     const T ratio = gamma / static_cast<T>(st->beta_prev);
+    const T alpha = static_cast<T>(static_cast<T>(st->beta_prev) / static_cast<T>(st->pap));
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         st->rr = trr;
         st->res = res;
@@ -233,15 +237,20 @@ __global__ __launch_bounds__(kBlock) void k_pcg_p(int n, long long nv, T *__rest
             st->beta = static_cast<double>(gamma);
         }
     }
-    if (state != HFH_ITERATE) return;
+    const bool go = state == HFH_ITERATE;
     const long long step = static_cast<long long>(gridDim.x) * kBlock;
     const long long t0 = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x;
+    V *__restrict__ xv = reinterpret_cast<V *>(x);
     V *__restrict__ pv = reinterpret_cast<V *>(p);
     const V *__restrict__ rv = reinterpret_cast<const V *>(r);
     const V *__restrict__ zv = reinterpret_cast<const V *>(z);
     const V *__restrict__ dv = reinterpret_cast<const V *>(invd);
     for (long long i = t0; i < nv; i += step) {
-        V pi = pv[i];
+        V pi = pv[i], xi = xv[i];
+#pragma unroll
+        for (int w = 0; w < W; ++w) xi[w] += alpha * pi[w];
+        xv[i] = xi;
+        if (!go) continue;
         V ri{}, zi{}, di{};
         if (MODE != 2) ri = rv[i];
         if (MODE == 1) di = dv[i];
@@ -250,9 +259,13 @@ __global__ __launch_bounds__(kBlock) void k_pcg_p(int n, long long nv, T *__rest
         for (int w = 0; w < W; ++w) pi[w] = pcg_dir_one<T, MODE>(pi[w], ri[w], zi[w], di[w], ratio);
         pv[i] = pi;
     }
-    for (long long j = nv * W + t0; j < n; j += step)
-        p[j] = pcg_dir_one<T, MODE>(p[j], MODE != 2 ? r[j] : T(0), MODE == 2 ? z[j] : T(0),
-                                    MODE == 1 ? invd[j] : T(0), ratio);
+    for (long long j = nv * W + t0; j < n; j += step) {
+        const T pj = p[j];
+        x[j] += alpha * pj;
+        if (go)
+            p[j] = pcg_dir_one<T, MODE>(pj, MODE != 2 ? r[j] : T(0), MODE == 2 ? z[j] : T(0),
+                                        MODE == 1 ? invd[j] : T(0), ratio);
+    }
 }
 
 // Workspace of one solve.
@@ -313,13 +326,13 @@ struct PcgRun {
         const int nsp = fold() ? A->nwg : 0;
         int rc;
         if (mode == 0)
-            hipLaunchKernelGGL((k_pcg_update<T, 0, false>), dim3(nb), dim3(kBlock), 0, s, n, nv, x, p, r, Ap, invd, part, stride, st, z, idv0, n0, (const double *)spart, nsp);
+            hipLaunchKernelGGL((k_pcg_update<T, 0, false>), dim3(nb), dim3(kBlock), 0, s, n, nv, r, Ap, invd, part, stride, st, z, idv0, n0, (const double *)spart, nsp);
         else if (mode == 1)
-            hipLaunchKernelGGL((k_pcg_update<T, 1, false>), dim3(nb), dim3(kBlock), 0, s, n, nv, x, p, r, Ap, invd, part, stride, st, z, idv0, n0, (const double *)spart, nsp);
+            hipLaunchKernelGGL((k_pcg_update<T, 1, false>), dim3(nb), dim3(kBlock), 0, s, n, nv, r, Ap, invd, part, stride, st, z, idv0, n0, (const double *)spart, nsp);
         else if (f0)
-            hipLaunchKernelGGL((k_pcg_update<T, 2, true>), dim3(nb), dim3(kBlock), 0, s, n, nv, x, p, r, Ap, invd, part, stride, st, z, idv0, n0, (const double *)spart, nsp);
+            hipLaunchKernelGGL((k_pcg_update<T, 2, true>), dim3(nb), dim3(kBlock), 0, s, n, nv, r, Ap, invd, part, stride, st, z, idv0, n0, (const double *)spart, nsp);
         else
-            hipLaunchKernelGGL((k_pcg_update<T, 2, false>), dim3(nb), dim3(kBlock), 0, s, n, nv, x, p, r, Ap, invd, part, stride, st, z, idv0, n0, (const double *)spart, nsp);
+            hipLaunchKernelGGL((k_pcg_update<T, 2, false>), dim3(nb), dim3(kBlock), 0, s, n, nv, r, Ap, invd, part, stride, st, z, idv0, n0, (const double *)spart, nsp);
         if ((rc = launch_check("k_pcg_update"))) return rc;
         if (mode == 2) {
             // general preconditioner: z = M^-1 r and the partials of r.z
@@ -331,11 +344,11 @@ struct PcgRun {
         }
         const double *pt = part;
         if (mode == 0)
-            hipLaunchKernelGGL((k_pcg_p<T, 0>), dim3(nb), dim3(kBlock), 0, s, n, nv, p, r, z, invd, st, pt, stride, nb, hist);
+            hipLaunchKernelGGL((k_pcg_p<T, 0>), dim3(nb), dim3(kBlock), 0, s, n, nv, x, p, r, z, invd, st, pt, stride, nb, hist);
         else if (mode == 1)
-            hipLaunchKernelGGL((k_pcg_p<T, 1>), dim3(nb), dim3(kBlock), 0, s, n, nv, p, r, z, invd, st, pt, stride, nb, hist);
+            hipLaunchKernelGGL((k_pcg_p<T, 1>), dim3(nb), dim3(kBlock), 0, s, n, nv, x, p, r, z, invd, st, pt, stride, nb, hist);
         else
-            hipLaunchKernelGGL((k_pcg_p<T, 2>), dim3(nb), dim3(kBlock), 0, s, n, nv, p, r, z, invd, st, pt, stride, nb, hist);
+            hipLaunchKernelGGL((k_pcg_p<T, 2>), dim3(nb), dim3(kBlock), 0, s, n, nv, x, p, r, z, invd, st, pt, stride, nb, hist);
         return launch_check("k_pcg_p");
     }
 
@@ -526,6 +539,7 @@ struct PcgWork {
         h.maxits = ctl.maxits;
         h.state = state0;
         h.done = state0 != HFH_ITERATE;
+        h.idle = h.done;
         HFH_HIP(hipMemcpyAsync(st, &h, sizeof(KState), hipMemcpyHostToDevice, s));
         HFH_HIP(hipMemcpyAsync(hist, &res0, sizeof(double), hipMemcpyHostToDevice, s));
         if (!h.done && (rc = pcg_loop(R, ctl.maxits, use_graph))) return rc;
