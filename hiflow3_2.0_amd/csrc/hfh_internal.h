@@ -230,7 +230,11 @@ int spmv_coded_jacobi(const hfh_csr *A, const T *x, T *xn, const T *b, const T *
 // (device, nullable): every kernel returns at once when *done != 0.
 // mc_first_done: for a multicolour SGS of two or more colours, the first
 // forward step (colour 0) is already in `out` (the CG update computes it).
-int prec_apply_raw(hfh_prec *P, const void *in, void *out, const int *done = nullptr, bool mc_first_done = false);
+// rzp / rz_out (multicolour SGS only): the apply also forms in.out (r.z of a
+// CG step) through partials in rzp (mcsgs_rz_slots(P) doubles) into *rz_out.
+int prec_apply_raw(hfh_prec *P, const void *in, void *out, const int *done = nullptr, bool mc_first_done = false,
+                   double *rzp = nullptr, double *rz_out = nullptr);
+int mcsgs_rz_slots(const hfh_prec *P);
 // Brings a preconditioner's copies of the matrix values up to date (after the
 // matrix's values changed); a no-op while the stream is being captured.
 int prec_sync(hfh_prec *P);

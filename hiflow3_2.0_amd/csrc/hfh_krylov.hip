@@ -13,7 +13,8 @@
 //       step and the new beta; unless finished, p = beta_ratio p + z
 // With another preconditioner, [M] z = M^-1 r and K2b (partials of r.z) run
 // between K2 and K3; for a multicolour SGS, K2 also writes the colour-0
-// forward step of [M].  No kernel reads a state field that another
+// forward step of [M], and the sweeps that finalise z store the r.z partials
+// in place of K2b (a one-workgroup finisher sums them).  No kernel reads a state field that another
 // workgroup of the same launch writes (beta_prev keeps K2's beta for K3).
 // Every kernel returns at once when the solve is done (K3 on the idle flag
 // that K2 sets then), so the host enqueues chunks of iterations (a captured
@@ -206,18 +207,20 @@ __device__ __forceinline__ T pcg_dir_one(T p, T r, T z, T d, T ratio) {
 // (slot `stride`, MODE != 0) of K2 / K2b in the same fixed order and takes
 // the same decision; workgroup 0 records the step (residual, history, state,
 // done, the new beta).  A finished solve updates x and leaves p alone.
+// nbz: r.z partials in slot `stride` (1 when a finisher has summed them).
 template <typename T, int MODE>
 __global__ __launch_bounds__(kBlock) void k_pcg_p(int n, long long nv, T *__restrict__ x, T *__restrict__ p,
                                                   const T *__restrict__ r, const T *__restrict__ z,
                                                   const T *__restrict__ invd, KState *st,
-                                                  const double *__restrict__ part, int stride, int nb, double *hist) {
+                                                  const double *__restrict__ part, int stride, int nb, int nbz,
+                                                  double *hist) {
     if (st->idle) return;
     typedef typename V16<T>::type V;
     constexpr int W = V16<T>::W;
     __shared__ double sh[kBlock / 64];
     __shared__ double bc;
     const double trr = block_sum_all(part, nb, sh, &bc);
-    const double trz = MODE != 0 ? block_sum_all(part + stride, nb, sh, &bc) : 0.0;
+    const double trz = MODE != 0 ? block_sum_all(part + stride, nbz, sh, &bc) : 0.0;
     const int it = st->iter;
     const T rrT = static_cast<T>(trr);
     const double res = static_cast<double>(static_cast<T>(sqrt(static_cast<double>(rrT))));
@@ -310,6 +313,15 @@ struct PcgRun {
     bool f0 = false;
     const T *idv0 = nullptr;
     int n0 = 0;
+    // multicolour SGS: the sweeps that finalise z store r.z partials here and
+    // the apply sums them into slot `stride` (no separate r.z pass)
+    double *rzp = nullptr;
+    // the instantiated graph of one chunk of iterations, kept for the next
+    // solve on this workspace (the same buffers; re-captured when x or the
+    // chunk differs); destroyed with the workspace
+    hipGraphExec_t exec = nullptr;
+    const T *exec_x = nullptr;
+    int exec_chunk = 0;
 
     // SpMV with fused p.Ap partials: summed by every K2 workgroup (fold) or
     // by the finisher k_fin_pap
@@ -334,21 +346,27 @@ struct PcgRun {
         else
             hipLaunchKernelGGL((k_pcg_update<T, 2, false>), dim3(nb), dim3(kBlock), 0, s, n, nv, r, Ap, invd, part, stride, st, z, idv0, n0, (const double *)spart, nsp);
         if ((rc = launch_check("k_pcg_update"))) return rc;
+        int nbz = nb;
         if (mode == 2) {
             // general preconditioner: z = M^-1 r and the partials of r.z
             // (slot 1, beside r.r in slot 0), both reduced by K3.  On the
             // step that converges, the apply runs once more than needed.
-            if ((rc = prec_apply_raw(P, r, z, &st->done, f0))) return rc;
-            hipLaunchKernelGGL(k_pcg_rz<T>, dim3(nb), dim3(kBlock), 0, s, n, r, z, part + stride, stride, (const KState *)st);
-            if ((rc = launch_check("k_pcg_rz"))) return rc;
+            if (rzp) {
+                if ((rc = prec_apply_raw(P, r, z, &st->done, f0, rzp, part + stride))) return rc;
+                nbz = 1;
+            } else {
+                if ((rc = prec_apply_raw(P, r, z, &st->done, f0))) return rc;
+                hipLaunchKernelGGL(k_pcg_rz<T>, dim3(nb), dim3(kBlock), 0, s, n, r, z, part + stride, stride, (const KState *)st);
+                if ((rc = launch_check("k_pcg_rz"))) return rc;
+            }
         }
         const double *pt = part;
         if (mode == 0)
-            hipLaunchKernelGGL((k_pcg_p<T, 0>), dim3(nb), dim3(kBlock), 0, s, n, nv, x, p, r, z, invd, st, pt, stride, nb, hist);
+            hipLaunchKernelGGL((k_pcg_p<T, 0>), dim3(nb), dim3(kBlock), 0, s, n, nv, x, p, r, z, invd, st, pt, stride, nb, nbz, hist);
         else if (mode == 1)
-            hipLaunchKernelGGL((k_pcg_p<T, 1>), dim3(nb), dim3(kBlock), 0, s, n, nv, x, p, r, z, invd, st, pt, stride, nb, hist);
+            hipLaunchKernelGGL((k_pcg_p<T, 1>), dim3(nb), dim3(kBlock), 0, s, n, nv, x, p, r, z, invd, st, pt, stride, nb, nbz, hist);
         else
-            hipLaunchKernelGGL((k_pcg_p<T, 2>), dim3(nb), dim3(kBlock), 0, s, n, nv, x, p, r, z, invd, st, pt, stride, nb, hist);
+            hipLaunchKernelGGL((k_pcg_p<T, 2>), dim3(nb), dim3(kBlock), 0, s, n, nv, x, p, r, z, invd, st, pt, stride, nb, nbz, hist);
         return launch_check("k_pcg_p");
     }
 
@@ -369,23 +387,29 @@ static int pcg_loop(PcgRun<T> &R, int maxits, bool use_graph) {
     const bool timing = c->timing.on;
     const int chunk = std::max(1, std::min(16, maxits));
     hipGraphExec_t exec = nullptr;
-    hipGraph_t graph = nullptr;
     int rc = HFH_OK;
-    if (use_graph && !timing) {
+    if (use_graph && !timing && R.exec && R.exec_x == R.x && R.exec_chunk == chunk) {
+        exec = R.exec;
+    } else if (use_graph && !timing) {
+        if (R.exec) (void)hipGraphExecDestroy(R.exec);
+        R.exec = nullptr;
+        hipGraph_t graph = nullptr;
         if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess) {
             for (int k = 0; k < chunk && !rc; ++k) rc = R.iteration();
             const hipError_t e2 = hipStreamEndCapture(s, &graph);
             if (rc || e2 != hipSuccess || hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
                 (void)hipGetLastError();
                 if (exec) (void)hipGraphExecDestroy(exec);
-                if (graph) (void)hipGraphDestroy(graph);
                 exec = nullptr;
-                graph = nullptr;
                 rc = HFH_OK;  // fall back to direct launches
             }
+            if (graph) (void)hipGraphDestroy(graph);
         } else {
             (void)hipGetLastError();
         }
+        R.exec = exec;
+        R.exec_x = R.x;
+        R.exec_chunk = chunk;
     }
     int *hflag = reinterpret_cast<int *>(c->host_res);
     std::vector<hipEvent_t> evs;
@@ -418,8 +442,6 @@ static int pcg_loop(PcgRun<T> &R, int maxits, bool use_graph) {
         if (*hflag) break;
         if (launched > static_cast<long long>(maxits) + chunk) break;  // the device test stops at maxits
     }
-    if (exec) (void)hipGraphExecDestroy(exec);
-    if (graph) (void)hipGraphDestroy(graph);
     if (!evs.empty()) {
         int it = 0;
         (void)hipStreamSynchronize(s);
@@ -449,6 +471,9 @@ struct PcgWork {
     // the r.z that formed its step
     bool have_dir = false;
     double prev_rz = 0;
+    ~PcgWork() {
+        if (R.exec) (void)hipGraphExecDestroy(R.exec);
+    }
 
     int init(const hfh_csr *A, hfh_prec *P, int maxits_) {
         hfh_ctx *c = A->ctx;
@@ -472,6 +497,9 @@ struct PcgWork {
             (rc = w.alloc(reinterpret_cast<void **>(&R.spart), sizeof(double) * std::max(A->nwg, 1))) ||
             (rc = w.alloc(reinterpret_cast<void **>(&st), sizeof(KState))) ||
             (rc = w.alloc(reinterpret_cast<void **>(&hist), sizeof(double) * (static_cast<size_t>(maxits) + 2))))
+            return rc;
+        if (R.mode == 2 && P->kind == HFH_PREC_MCSGS &&
+            (rc = w.alloc(reinterpret_cast<void **>(&R.rzp), sizeof(double) * mcsgs_rz_slots(P))))
             return rc;
         R.st = st;
         R.hist = hist;

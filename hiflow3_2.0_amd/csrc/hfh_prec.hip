@@ -15,6 +15,7 @@
 #include "hfh_internal.h"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace hfh {
 
@@ -132,14 +133,16 @@ __device__ __forceinline__ void sweep_cols(int (&cj)[kSweepU], int k, int L, int
     }
 }
 
-template <typename T, bool CODED>
+// RZ: the step also stores its workgroup's partial of in.out over its rows
+// (r.z of a CG step, for the colour whose z this step finalises) in rzp.
+template <typename T, bool CODED, bool RZ>
 __global__ __launch_bounds__(kBlock) void k_mcs_fwd(int c, ColorOffsets co, bool last, int slice0, int nsl, int n,
                                                     const int *__restrict__ sptr, const int *__restrict__ slen,
                                                     const int *__restrict__ scol, const unsigned *__restrict__ code,
                                                     const int *__restrict__ wptr, const int *__restrict__ gdict,
                                                     int ndict, const T *__restrict__ sval, const T *__restrict__ in,
                                                     const T *__restrict__ idv, const T *__restrict__ dv, T *out,
-                                                    const int *__restrict__ done) {
+                                                    const int *__restrict__ done, double *__restrict__ rzp) {
     if (done && *done) return;
     __shared__ int dict[CODED ? 256 : 1];
     if (CODED) {
@@ -147,52 +150,65 @@ __global__ __launch_bounds__(kBlock) void k_mcs_fwd(int c, ColorOffsets co, bool
         __syncthreads();
     }
     const int w = blockIdx.x * (kBlock / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-    if (w >= nsl) return;
-    const int s = slice0 + w;
-    const int i = co.off[c] + w * 64 + lane;
-    const bool valid = i < co.off[c + 1];
-    const int base = sptr[s], L = (sptr[s + 1] - base) >> 6;
-    const int wbase = CODED ? wptr[s] : 0;
-    const int len = valid ? slen[i] : 0;
-    T chunk = valid ? in[i] * static_cast<T>(-1) : static_cast<T>(0);
-    int blk = 0, blk_end = co.off[1];
-    T res = 0;
-    for (int k = 0; k < L; k += kSweepU) {
-        int cj[kSweepU];
-        T v[kSweepU], xv[kSweepU];
-        sweep_cols<CODED>(cj, k, L, base, wbase, lane, i, n, scol, code, dict);
+    if (!RZ && w >= nsl) return;
+    double rz = 0;
+    if (w < nsl) {
+        const int s = slice0 + w;
+        const int i = co.off[c] + w * 64 + lane;
+        const bool valid = i < co.off[c + 1];
+        const int base = sptr[s], L = (sptr[s + 1] - base) >> 6;
+        const int wbase = CODED ? wptr[s] : 0;
+        const int len = valid ? slen[i] : 0;
+        const T ri = valid ? in[i] : static_cast<T>(0);
+        T chunk = ri * static_cast<T>(-1);
+        int blk = 0, blk_end = co.off[1];
+        T res = 0;
+        for (int k = 0; k < L; k += kSweepU) {
+            int cj[kSweepU];
+            T v[kSweepU], xv[kSweepU];
+            sweep_cols<CODED>(cj, k, L, base, wbase, lane, i, n, scol, code, dict);
 #pragma unroll
-        for (int u = 0; u < kSweepU; ++u) v[u] = __builtin_nontemporal_load(&sval[base + min(k + u, L - 1) * 64 + lane]);
+            for (int u = 0; u < kSweepU; ++u) v[u] = __builtin_nontemporal_load(&sval[base + min(k + u, L - 1) * 64 + lane]);
 #pragma unroll
-        for (int u = 0; u < kSweepU; ++u) xv[u] = out[cj[u]];
+            for (int u = 0; u < kSweepU; ++u) xv[u] = out[cj[u]];
 #pragma unroll
-        for (int u = 0; u < kSweepU; ++u) {
-            if (k + u >= len) continue;
-            while (cj[u] >= blk_end) {   // entries are column sorted: flush finished blocks
-                chunk += res;
-                res = 0;
-                ++blk;
-                blk_end = co.off[blk + 1];
+            for (int u = 0; u < kSweepU; ++u) {
+                if (k + u >= len) continue;
+                while (cj[u] >= blk_end) {   // entries are column sorted: flush finished blocks
+                    chunk += res;
+                    res = 0;
+                    ++blk;
+                    blk_end = co.off[blk + 1];
+                }
+                res += v[u] * xv[u];
             }
-            res += v[u] * xv[u];
+        }
+        if (valid) {
+            chunk += res;
+            chunk *= idv[i];
+            if (last) chunk = (chunk * dv[i]) * idv[i];
+            out[i] = chunk;
+            if (RZ) rz = static_cast<double>(ri) * static_cast<double>(chunk);
         }
     }
-    if (!valid) return;
-    chunk += res;
-    chunk *= idv[i];
-    if (last) chunk = (chunk * dv[i]) * idv[i];
-    out[i] = chunk;
+    if (RZ) {
+        __shared__ double sh[kBlock / 64];
+        rz = block_sum(rz, sh);
+        if (threadIdx.x == 0) rzp[blockIdx.x] = rz;
+    }
 }
 
 // Backward step of colour c: entries stored block by block from the highest
 // colour down, ascending columns within a block.
-template <typename T, bool CODED>
+// RZ as in k_mcs_fwd, with the step's input r in rin.
+template <typename T, bool CODED, bool RZ>
 __global__ __launch_bounds__(kBlock) void k_mcs_bwd(int c, int ncolors, ColorOffsets co, int slice0, int nsl, int n,
                                                     const int *__restrict__ sptr, const int *__restrict__ slen,
                                                     const int *__restrict__ scol, const unsigned *__restrict__ code,
                                                     const int *__restrict__ wptr, const int *__restrict__ gdict,
                                                     int ndict, const T *__restrict__ sval, const T *__restrict__ idv,
-                                                    const T *__restrict__ dv, T *out, const int *__restrict__ done) {
+                                                    const T *__restrict__ dv, T *out, const int *__restrict__ done,
+                                                    const T *__restrict__ rin, double *__restrict__ rzp) {
     if (done && *done) return;
     __shared__ int dict[CODED ? 256 : 1];
     if (CODED) {
@@ -200,39 +216,60 @@ __global__ __launch_bounds__(kBlock) void k_mcs_bwd(int c, int ncolors, ColorOff
         __syncthreads();
     }
     const int w = blockIdx.x * (kBlock / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-    if (w >= nsl) return;
-    const int s = slice0 + w;
-    const int i = co.off[c] + w * 64 + lane;
-    const bool valid = i < co.off[c + 1];
-    const int base = sptr[s], L = (sptr[s + 1] - base) >> 6;
-    const int wbase = CODED ? wptr[s] : 0;
-    const int len = valid ? slen[i] : 0;
-    T chunk = valid ? out[i] * dv[i] : static_cast<T>(0);
-    int blk = ncolors - 1, blk_lo = co.off[ncolors - 1];
-    T res = 0;
-    for (int k = 0; k < L; k += kSweepU) {
-        int cj[kSweepU];
-        T v[kSweepU], xv[kSweepU];
-        sweep_cols<CODED>(cj, k, L, base, wbase, lane, i, n, scol, code, dict);
+    if (!RZ && w >= nsl) return;
+    double rz = 0;
+    if (w < nsl) {
+        const int s = slice0 + w;
+        const int i = co.off[c] + w * 64 + lane;
+        const bool valid = i < co.off[c + 1];
+        const int base = sptr[s], L = (sptr[s + 1] - base) >> 6;
+        const int wbase = CODED ? wptr[s] : 0;
+        const int len = valid ? slen[i] : 0;
+        T chunk = valid ? out[i] * dv[i] : static_cast<T>(0);
+        int blk = ncolors - 1, blk_lo = co.off[ncolors - 1];
+        T res = 0;
+        for (int k = 0; k < L; k += kSweepU) {
+            int cj[kSweepU];
+            T v[kSweepU], xv[kSweepU];
+            sweep_cols<CODED>(cj, k, L, base, wbase, lane, i, n, scol, code, dict);
 #pragma unroll
-        for (int u = 0; u < kSweepU; ++u) v[u] = __builtin_nontemporal_load(&sval[base + min(k + u, L - 1) * 64 + lane]);
+            for (int u = 0; u < kSweepU; ++u) v[u] = __builtin_nontemporal_load(&sval[base + min(k + u, L - 1) * 64 + lane]);
 #pragma unroll
-        for (int u = 0; u < kSweepU; ++u) xv[u] = out[cj[u]];
+            for (int u = 0; u < kSweepU; ++u) xv[u] = out[cj[u]];
 #pragma unroll
-        for (int u = 0; u < kSweepU; ++u) {
-            if (k + u >= len) continue;
-            while (cj[u] < blk_lo) {   // next lower colour block
-                chunk += res;
-                res = 0;
-                --blk;
-                blk_lo = co.off[blk];
+            for (int u = 0; u < kSweepU; ++u) {
+                if (k + u >= len) continue;
+                while (cj[u] < blk_lo) {   // next lower colour block
+                    chunk += res;
+                    res = 0;
+                    --blk;
+                    blk_lo = co.off[blk];
+                }
+                res += v[u] * xv[u];
             }
-            res += v[u] * xv[u];
+        }
+        if (valid) {
+            chunk += res;
+            const T zi = chunk * idv[i];
+            out[i] = zi;
+            if (RZ) rz = static_cast<double>(rin[i]) * static_cast<double>(zi);
         }
     }
-    if (!valid) return;
-    chunk += res;
-    out[i] = chunk * idv[i];
+    if (RZ) {
+        __shared__ double sh[kBlock / 64];
+        rz = block_sum(rz, sh);
+        if (threadIdx.x == 0) rzp[blockIdx.x] = rz;
+    }
+}
+
+// r.z of a CG step from the sweeps' partials: one workgroup sums the m
+// partials in a fixed order into *out.
+__global__ __launch_bounds__(kFinBlock) void k_rz_sum(const double *__restrict__ part, int m, double *out,
+                                                      const int *__restrict__ done) {
+    if (done && *done) return;
+    double v[1];
+    reduce_partials<1>(part, 0, m, v);
+    if (threadIdx.x == 0) *out = v[0];
 }
 
 // sweep values from the CSR values through the position map (-1: padding)
@@ -522,14 +559,60 @@ static int trsv_sweeps(hfh_prec *P, const T *in, T *out, const int *done) {
     return launch_check("k_tsweep");
 }
 
+// r.z partials that a multicolour SGS apply with rzp stores: one per
+// workgroup of the steps that finalise z (an upper bound for any colouring).
+int mcsgs_rz_slots(const hfh_prec *P) {
+    constexpr int WPB = kBlock / 64;
+    int m = 0;
+    for (int c = 0; c < P->ncolors; ++c) {
+        const int nf = P->fw.slice0[c + 1] - P->fw.slice0[c], nb = P->bw.slice0[c + 1] - P->bw.slice0[c];
+        m += (std::max(nf, nb) + WPB - 1) / WPB;
+    }
+    return std::max(m, 1);
+}
+
 template <typename T>
-static int mcsgs_apply(hfh_prec *P, const T *in, T *out, const int *done, bool first_done) {
+static int mcsgs_apply(hfh_prec *P, const T *in, T *out, const int *done, bool first_done, double *rzp,
+                       double *rz_out) {
     hipStream_t s = P->ctx->stream;
     ColorOffsets co;
     for (int c = 0; c <= P->ncolors; ++c) co.off[c] = P->color_off[c];
     const T *idv = (const T *)P->invd, *dv = (const T *)P->dv;
     const int nc = P->ncolors;
     constexpr int WPB = kBlock / 64;
+    // with rzp, the steps that finalise z (the last colour's forward step,
+    // every backward step) store r.z partials there, one per workgroup
+    int nrz = 0;
+    auto fwd = [&](auto coded, auto rz, int c, int s0, int nsl) {
+        const hfh_sweep &W = P->fw;
+        const int g = (nsl + WPB - 1) / WPB;
+        hipLaunchKernelGGL((k_mcs_fwd<T, decltype(coded)::value, decltype(rz)::value>), dim3(g), dim3(kBlock), 0, s, c,
+                           co, c == nc - 1, s0, nsl, P->n, W.ptr, W.len, W.col, W.code, W.wptr, W.dict, W.ndict,
+                           (const T *)W.val, in, idv, dv, out, done, rzp + nrz);
+        if (decltype(rz)::value) nrz += g;
+    };
+    auto bwd = [&](auto coded, auto rz, int c, int s0, int nsl) {
+        const hfh_sweep &W = P->bw;
+        const int g = (nsl + WPB - 1) / WPB;
+        hipLaunchKernelGGL((k_mcs_bwd<T, decltype(coded)::value, decltype(rz)::value>), dim3(g), dim3(kBlock), 0, s, c,
+                           nc, co, s0, nsl, P->n, W.ptr, W.len, W.col, W.code, W.wptr, W.dict, W.ndict,
+                           (const T *)W.val, idv, dv, out, done, in, rzp + nrz);
+        if (decltype(rz)::value) nrz += g;
+    };
+    if (rzp) {
+        int need = 0;
+        for (int c = 0; c < nc; ++c) {
+            if (co.off[c + 1] == co.off[c]) continue;
+            const hfh_sweep &W = c == nc - 1 ? P->fw : P->bw;
+            need += (W.slice0[c + 1] - W.slice0[c] + WPB - 1) / WPB;
+        }
+        if (need > mcsgs_rz_slots(P)) {
+            set_error("mcsgs_apply: %d r.z partials exceed the %d slots", need, mcsgs_rz_slots(P));
+            return HFH_ERR_INVALID;
+        }
+    }
+    using Yes = std::true_type;
+    using No = std::false_type;
     for (int c = 0; c < nc; ++c) {
         const int cnt = co.off[c + 1] - co.off[c];
         if (cnt == 0) continue;
@@ -539,31 +622,24 @@ static int mcsgs_apply(hfh_prec *P, const T *in, T *out, const int *done, bool f
                                co.off[1], in, idv, out, done);
         } else {
             const int s0 = P->fw.slice0[c], nsl = P->fw.slice0[c + 1] - s0;
-            const hfh_sweep &W = P->fw;
-            if (W.code)
-                hipLaunchKernelGGL((k_mcs_fwd<T, true>), dim3((nsl + WPB - 1) / WPB), dim3(kBlock), 0, s, c, co, c == nc - 1,
-                                   s0, nsl, P->n, W.ptr, W.len, W.col, W.code, W.wptr, W.dict, W.ndict, (const T *)W.val, in,
-                                   idv, dv, out, done);
+            const bool rz = rzp && c == nc - 1;
+            if (P->fw.code)
+                rz ? fwd(Yes{}, Yes{}, c, s0, nsl) : fwd(Yes{}, No{}, c, s0, nsl);
             else
-                hipLaunchKernelGGL((k_mcs_fwd<T, false>), dim3((nsl + WPB - 1) / WPB), dim3(kBlock), 0, s, c, co, c == nc - 1,
-                                   s0, nsl, P->n, W.ptr, W.len, W.col, W.code, W.wptr, W.dict, W.ndict, (const T *)W.val, in,
-                                   idv, dv, out, done);
+                rz ? fwd(No{}, Yes{}, c, s0, nsl) : fwd(No{}, No{}, c, s0, nsl);
         }
     }
     for (int c = nc - 2; c >= 0; --c) {
         const int cnt = co.off[c + 1] - co.off[c];
         if (cnt == 0) continue;
         const int s0 = P->bw.slice0[c], nsl = P->bw.slice0[c + 1] - s0;
-        const hfh_sweep &W = P->bw;
-        if (W.code)
-            hipLaunchKernelGGL((k_mcs_bwd<T, true>), dim3((nsl + WPB - 1) / WPB), dim3(kBlock), 0, s, c, nc, co, s0, nsl,
-                               P->n, W.ptr, W.len, W.col, W.code, W.wptr, W.dict, W.ndict, (const T *)W.val, idv, dv, out,
-                               done);
+        if (P->bw.code)
+            rzp ? bwd(Yes{}, Yes{}, c, s0, nsl) : bwd(Yes{}, No{}, c, s0, nsl);
         else
-            hipLaunchKernelGGL((k_mcs_bwd<T, false>), dim3((nsl + WPB - 1) / WPB), dim3(kBlock), 0, s, c, nc, co, s0, nsl,
-                               P->n, W.ptr, W.len, W.col, W.code, W.wptr, W.dict, W.ndict, (const T *)W.val, idv, dv, out,
-                               done);
+            rzp ? bwd(No{}, Yes{}, c, s0, nsl) : bwd(No{}, No{}, c, s0, nsl);
     }
+    if (rzp)
+        hipLaunchKernelGGL(k_rz_sum, dim3(1), dim3(kFinBlock), 0, s, (const double *)rzp, nrz, rz_out, done);
     return launch_check("k_mc_sweep");
 }
 
@@ -854,7 +930,8 @@ int prec_sync(hfh_prec *P) {
     return rc;
 }
 
-int prec_apply_raw(hfh_prec *P, const void *in, void *out, const int *done, bool mc_first_done) {
+int prec_apply_raw(hfh_prec *P, const void *in, void *out, const int *done, bool mc_first_done, double *rzp,
+                   double *rz_out) {
     hipStream_t s = P->ctx->stream;
     const int n = P->n;
     if (n == 0) return HFH_OK;
@@ -874,8 +951,9 @@ int prec_apply_raw(hfh_prec *P, const void *in, void *out, const int *done, bool
         return launch_check("k_jacobi");
     }
     case HFH_PREC_MCSGS:
-        return P->dtype == HFH_F64 ? mcsgs_apply<double>(P, (const double *)in, (double *)out, done, mc_first_done)
-                                   : mcsgs_apply<float>(P, (const float *)in, (float *)out, done, mc_first_done);
+        return P->dtype == HFH_F64
+                   ? mcsgs_apply<double>(P, (const double *)in, (double *)out, done, mc_first_done, rzp, rz_out)
+                   : mcsgs_apply<float>(P, (const float *)in, (float *)out, done, mc_first_done, rzp, rz_out);
     case HFH_PREC_SGS:
     case HFH_PREC_ILU0:
         if (!done)

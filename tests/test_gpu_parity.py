@@ -583,6 +583,33 @@ def test_pcg_mcsgs_matches_oracle(pkg, ctx, oracle):
     assert relnorm(X.numpy(), xo) < 1e-8
 
 
+@pytest.mark.parametrize("case,ncol", [("q1_2d", 4), ("q1_3d", 8)])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_pcg_mcsgs_many_colours_matches_oracle(pkg, ctx, oracle, case, ncol, dtype):
+    """CG + multicolour SGS with 4 and 8 colours: r.z comes from partials of
+    the last colour's forward sweep and of every backward sweep."""
+    if case == "q1_2d":
+        row, col, val, rhs = pkg.problem(pkg.Q1_2D, 200)
+    else:
+        row, col, val, rhs = pkg.problem(pkg.Q1_3D, 40)
+    nc, color, sizes, perm = pkg.multicoloring(row, col)
+    assert nc == ncol
+    prow, pcol, pval = pkg.csr_permute(row, col, val, perm)
+    pval, b = pval.astype(dtype), rhs[perm].astype(dtype)
+    A = pkg.CsrMatrix(ctx, prow, pcol, pval)
+    P = pkg.Preconditioner.mcsgs(A, sizes)
+    X = pkg.Vector(ctx, len(b), dtype)
+    rtol = 1e-10 if dtype == np.float64 else 1e-5
+    info = pkg.pcg(A, pkg.Vector(ctx, b, dtype), X, P, maxits=2000, atol=0.0, rtol=rtol, history=True)
+    OA = oracle.Csr(prow, pcol, pval)
+    xo, oi = oracle.cg(OA, b, P=oracle.Prec(oracle.PREC_MCSGS, OA, sizes=sizes), ctrl=(2000, 0.0, rtol, 1e6))
+    assert info["state"] == oi["status"] == pkg.SUCCESS_REL
+    assert abs(info["iters"] - oi["iters"]) <= 2
+    k = min(10, len(oi["hist"]), len(info["hist"]))
+    assert np.allclose(info["hist"][:k], oi["hist"][:k], rtol=1e-9 if dtype == np.float64 else 1e-4)
+    assert relnorm(X.numpy(), xo) < (1e-8 if dtype == np.float64 else 1e-3)
+
+
 @pytest.mark.parametrize("flexible", [False, True])
 def test_gmres_ilu0_matches_oracle(pkg, ctx, oracle, flexible):
     row, col, val, rhs = pkg.problem(pkg.UPWIND_CONVDIFF, 16, p0=1e-2, p1=5.0, seed=7)
