@@ -328,6 +328,8 @@ __device__ __forceinline__ void reduce_partials(const double *part, int stride, 
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
         double s = 0;
+        // unrolled so that the loads issue together; the adds keep their order
+#pragma unroll 8
         for (int b = threadIdx.x; b < nb; b += kFinBlock) s += part[q * stride + b];
         s = wave_sum(s);
         const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;

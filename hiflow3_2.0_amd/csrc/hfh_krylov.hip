@@ -51,6 +51,8 @@ __device__ __forceinline__ int check_state(int step, double v, double first, con
 // it on the same partials gets the same value.
 __device__ __forceinline__ double block_sum_all(const double *__restrict__ part, int nb, double *sh, double *bc) {
     double v = 0;
+    // unrolled so that the loads issue together; the adds keep their order
+#pragma unroll 4
     for (int b = threadIdx.x; b < nb; b += kBlock) v += part[b];
     v = block_sum(v, sh);
     if (threadIdx.x == 0) *bc = v;
