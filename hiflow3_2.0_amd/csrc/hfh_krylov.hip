@@ -62,6 +62,40 @@ __device__ __forceinline__ double block_sum_all(const double *__restrict__ part,
     return v;
 }
 
+// The sums of pa[0..na) and pb[0..nb) that two block_sum_all calls return
+// (the same order of adds for each), with the loads of both in one pass and
+// one set of barriers.
+__device__ __forceinline__ void block_sum_all2(const double *__restrict__ pa, int na, const double *__restrict__ pb,
+                                               int nb, double *sh /* 2 kBlock/64 */, double *bc /* 2 */, double &a,
+                                               double &b) {
+    double va = 0, vb = 0;
+    const int m = max(na, nb);
+#pragma unroll 4
+    for (int i = threadIdx.x; i < m; i += kBlock) {
+        if (i < na) va += pa[i];
+        if (i < nb) vb += pb[i];
+    }
+    va = wave_sum(va);
+    vb = wave_sum(vb);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) {
+        sh[w] = va;
+        sh[kBlock / 64 + w] = vb;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double ta = 0, tb = 0;
+        for (int k = 0; k < kBlock / 64; ++k) ta += sh[k];
+        for (int k = 0; k < kBlock / 64; ++k) tb += sh[kBlock / 64 + k];
+        bc[0] = ta;
+        bc[1] = tb;
+    }
+    __syncthreads();
+    a = bc[0];
+    b = bc[1];
+    __syncthreads();
+}
+
 static inline int host_check(int step, double v, double first, const hfh_control &c) {
     if (v < c.atol) return HFH_SUCCESS_ABS;
     if (v < c.rtol * first) return HFH_SUCCESS_REL;
@@ -219,10 +253,13 @@ __global__ __launch_bounds__(kBlock) void k_pcg_p(int n, long long nv, T *__rest
     if (st->idle) return;
     typedef typename V16<T>::type V;
     constexpr int W = V16<T>::W;
-    __shared__ double sh[kBlock / 64];
-    __shared__ double bc;
-    const double trr = block_sum_all(part, nb, sh, &bc);
-    const double trz = MODE != 0 ? block_sum_all(part + stride, nbz, sh, &bc) : 0.0;
+    __shared__ double sh[2 * (kBlock / 64)];
+    __shared__ double bc[2];
+    double trr, trz = 0.0;
+    if (MODE != 0)
+        block_sum_all2(part, nb, part + stride, nbz, sh, bc, trr, trz);
+    else
+        trr = block_sum_all(part, nb, sh, bc);
     const int it = st->iter;
     const T rrT = static_cast<T>(trr);
     const double res = static_cast<double>(static_cast<T>(sqrt(static_cast<double>(rrT))));
