@@ -119,7 +119,8 @@ struct hfh_sweep {
     std::vector<int> slice0;   // host: first slice of each colour (ncolors+1)
     // with at most 256 distinct offsets col - row over the sweep's entries,
     // the columns are one-byte codes into `dict` instead (col is null):
-    // code word w of a lane holds slots 4w..4w+3, slice widths padded to 4
+    // code word w of a lane holds slots 4w..4w+3; the values keep the
+    // slice's width, its code words cover it rounded up to 4 slots
     unsigned *code = nullptr;  // code words
     int *wptr = nullptr;       // [nslice+1] word offset of each slice
     int *dict = nullptr;       // [ndict] offsets, ascending (0 included: padding)

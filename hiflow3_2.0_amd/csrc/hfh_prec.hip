@@ -124,7 +124,7 @@ __device__ __forceinline__ void sweep_cols(int (&cj)[kSweepU], int k, int L, int
         for (int u = 0; u < kSweepU; ++u) cj[u] = __builtin_nontemporal_load(&scol[base + min(k + u, L - 1) * 64 + lane]);
         return;
     }
-    const int W = L >> 2;
+    const int W = (L + 3) >> 2;   // code words of the slice (its width rounded up to 4 slots)
 #pragma unroll
     for (int q = 0; q < kSweepU / 4; ++q) {
         const unsigned cw = __builtin_nontemporal_load(&code[wbase + min((k >> 2) + q, W - 1) * 64 + lane]);
@@ -702,30 +702,36 @@ static int build_sweep(hfh_prec *P, const hfh_csr *A, bool forward, hfh_sweep &W
             }
             width[W.slice0[c] + w] = L;
         }
-    // coded: slice widths padded to whole code words of 4 slots
-    long long total = 0;
+    // values keep each slice's width; coded: the code words cover it rounded
+    // up to whole words of 4 slots (wptr: word offset of each slice)
+    long long total = 0, wtotal = 0;
+    std::vector<int> wptr(nsl + 1, 0);
     for (int sl = 0; sl < nsl; ++sl) {
         ptr[sl] = static_cast<int>(total);
-        total += 64LL * (coded ? (width[sl] + 3) / 4 * 4 : width[sl]);
+        wptr[sl] = static_cast<int>(wtotal);
+        total += 64LL * width[sl];
+        wtotal += 64LL * ((width[sl] + 3) / 4);
         HFH_CHECK(total < (1LL << 31), HFH_ERR_UNSUPPORTED, "hfh_mcsgs_create: sweep copy exceeds 2^31 entries");
     }
     ptr[nsl] = static_cast<int>(total);
+    wptr[nsl] = static_cast<int>(wtotal);
     std::vector<int> pcol(coded ? 1 : std::max<long long>(total, 1)), pos(std::max<long long>(total, 1));
-    std::vector<unsigned> words(coded ? std::max<long long>(total / 4, 1) : 1, 0u);
+    std::vector<unsigned> words(coded ? std::max<long long>(wtotal, 1) : 1, 0u);
     const unsigned zero = static_cast<unsigned>(std::lower_bound(offs.begin(), offs.end(), 0LL) - offs.begin());
     for (int c = 0; c < nc; ++c)
         for (int w = 0; w < W.slice0[c + 1] - W.slice0[c]; ++w) {
-            const int base = ptr[W.slice0[c] + w], L = (ptr[W.slice0[c] + w + 1] - base) / 64;
+            const int sl = W.slice0[c] + w;
+            const int base = ptr[sl], L = (ptr[sl + 1] - base) / 64, wbase = wptr[sl];
             for (int lane = 0; lane < 64; ++lane) {
                 const int i = off[c] + w * 64 + lane;
                 const bool valid = i < off[c + 1];
                 if (valid) entries(c, i, e);
                 else e.clear();
                 const int own = std::min(i, off[c + 1] - 1);   // a row of this slice
-                for (int k = 0; k < L; ++k) {
+                for (int k = 0; k < (coded ? (L + 3) / 4 * 4 : L); ++k) {
                     const int idx = base + k * 64 + lane;
                     const bool real = k < static_cast<int>(e.size());
-                    pos[idx] = real ? e[k] : -1;
+                    if (k < L) pos[idx] = real ? e[k] : -1;
                     if (!coded) {
                         pcol[idx] = real ? col[e[k]] : own;
                         continue;
@@ -735,7 +741,7 @@ static int build_sweep(hfh_prec *P, const hfh_csr *A, bool forward, hfh_sweep &W
                                                                       static_cast<long long>(col[e[k]]) - i) -
                                                      offs.begin())
                              : zero;
-                    words[base / 4 + (k / 4) * 64 + lane] |= code << (8 * (k % 4));
+                    words[wbase + (k / 4) * 64 + lane] |= code << (8 * (k % 4));
                 }
             }
         }
@@ -748,8 +754,7 @@ static int build_sweep(hfh_prec *P, const hfh_csr *A, bool forward, hfh_sweep &W
     HFH_HIP(hipMemcpy(W.ptr, ptr.data(), sizeof(int) * ptr.size(), hipMemcpyHostToDevice));
     HFH_HIP(hipMemcpy(W.len, len.data(), sizeof(int) * len.size(), hipMemcpyHostToDevice));
     if (coded) {
-        std::vector<int> wptr(nsl + 1), dict(offs.begin(), offs.end());
-        for (int sl = 0; sl <= nsl; ++sl) wptr[sl] = ptr[sl] / 4;
+        std::vector<int> dict(offs.begin(), offs.end());
         if ((rc = dev_alloc(reinterpret_cast<void **>(&W.code), sizeof(unsigned) * words.size())) ||
             (rc = dev_alloc(reinterpret_cast<void **>(&W.wptr), sizeof(int) * wptr.size())) ||
             (rc = dev_alloc(reinterpret_cast<void **>(&W.dict), sizeof(int) * dict.size())))
