@@ -990,8 +990,17 @@ extern "C" int hfh_dc_solve_ex(const hfh_csr *Ah, const hfh_csr *Al, hfh_prec *P
               "hfh_dc_solve: matrices differ");
     HFH_CHECK(b->n == Ah->nrow && x->n == Ah->nrow && b->dtype == HFH_F64 && x->dtype == HFH_F64, HFH_ERR_INVALID,
               "hfh_dc_solve: vectors must be FP64 of size %d", Ah->nrow);
+    HFH_CHECK(Ah->has_structure && Al->has_structure, HFH_ERR_INVALID, "hfh_dc_solve: matrices need a structure");
     HFH_CHECK(!Pl || (Pl->dtype == HFH_F32 && Pl->n == Al->nrow), HFH_ERR_INVALID, "hfh_dc_solve: bad preconditioner");
-    HFH_CHECK(outer->maxits > 0 && inner->maxits > 0, HFH_ERR_INVALID, "hfh_dc_solve: maxits must be positive");
+    HFH_CHECK(x->d != b->d, HFH_ERR_INVALID, "hfh_dc_solve: x and b must not alias");
+    HFH_CHECK(outer->maxits > 0 && outer->atol >= 0 && outer->rtol >= 0 && outer->dtol > 1, HFH_ERR_INVALID,
+              "hfh_dc_solve: outer control needs maxits > 0, atol >= 0, rtol >= 0, dtol > 1");
+    HFH_CHECK(inner->maxits > 0 && inner->atol >= 0 && inner->rtol >= 0 && inner->dtol > 1, HFH_ERR_INVALID,
+              "hfh_dc_solve: inner control needs maxits > 0, atol >= 0, rtol >= 0, dtol > 1");
+    // the low precision tolerances are DataType_low members of the reference's
+    // MPIR (mpir.h:225-240): its gates and the FP32 solver see them rounded
+    const hfh_control inl{inner->maxits, (double)(float)inner->atol, (double)(float)inner->rtol,
+                          (double)(float)inner->dtol};
     if (int rs = prec_sync(Pl)) return rs;   // before the inner iterations are captured
     if (Ah->nrow == 0) {
         // an empty system is solved exactly by the empty vector
@@ -1030,7 +1039,7 @@ extern "C" int hfh_dc_solve_ex(const hfh_csr *Ah, const hfh_csr *Al, hfh_prec *P
     std::vector<double> hist{res};
     int n_ecl = 0, inner_iters = 0;
     auto ecl_ok = [&]() {
-        return n_ecl < outer->maxits && res > inner->atol && res > res0 * inner->rtol && res < inner->dtol;
+        return n_ecl < outer->maxits && res > inl.atol && res > res0 * inl.rtol && res < inl.dtol;
     };
     while (res > outer->atol && res > res0 * outer->rtol && res < outer->dtol && ecl_ok()) {
         ++n_ecl;
@@ -1038,7 +1047,7 @@ extern "C" int hfh_dc_solve_ex(const hfh_csr *Ah, const hfh_csr *Al, hfh_prec *P
         if ((rc = launch_check("k_cast_n"))) return rc;
         HFH_HIP(hipMemsetAsync(cl, 0, sizeof(float) * n, s));
         hfh_solve_info ii{};
-        if ((rc = W.solve(rl, cl, *inner, &ii, nullptr, true, true, keep))) return rc;
+        if ((rc = W.solve(rl, cl, inl, &ii, nullptr, true, true, keep))) return rc;
         inner_iters += ii.iters;
         hipLaunchKernelGGL(k_add_correction, dim3(nb), dim3(kBlock), 0, s, n, (double *)x->d, (const float *)cl);
         if ((rc = launch_check("k_add_correction"))) return rc;

@@ -463,11 +463,20 @@ class MPIR : public LinearSolver<LADh> {
         rel_ = rel_tol;
         div_ = div_tol;
     }
-    void SetLowPrecErrCorSolverParameters(int maxit, double atol, double rtol, double dtol) {
-        ecl_p_ = {static_cast<double>(maxit), atol, rtol, dtol};
+    // the low precision tolerances are stored in DataType_low, as the
+    // reference's members are (mpir.h:225-240): the gates and the inner
+    // solver's InitControl see them rounded to that type
+    void SetLowPrecErrCorSolverParameters(int maxit, DataL atol, DataL rtol, DataL dtol) {
+        ecl_max_ = maxit;
+        ecl_atol_ = atol;
+        ecl_rtol_ = rtol;
+        ecl_dtol_ = dtol;
     }
-    void SetHighPrecErrCorSolverParameters(int maxit, double atol, double rtol, double dtol) {
-        ech_p_ = {static_cast<double>(maxit), atol, rtol, dtol};
+    void SetHighPrecErrCorSolverParameters(int maxit, DataH atol, DataH rtol, DataH dtol) {
+        ech_max_ = maxit;
+        ech_atol_ = atol;
+        ech_rtol_ = rtol;
+        ech_dtol_ = dtol;
     }
     int n_ecl() const { return n_ecl_; }
     int n_ech() const { return n_ech_; }
@@ -477,10 +486,10 @@ class MPIR : public LinearSolver<LADh> {
         this->iter_ = 0;
         this->hist_.clear();
         ecl_->SetupOperator(*Al_);
-        ecl_->InitControl(static_cast<int>(ecl_p_[0]), ecl_p_[1], ecl_p_[2], ecl_p_[3]);
+        ecl_->InitControl(ecl_max_, ecl_atol_, ecl_rtol_, ecl_dtol_);
         if (ech_) {
             ech_->SetupOperator(*Ah_);
-            ech_->InitControl(static_cast<int>(ech_p_[0]), ech_p_[1], ech_p_[2], ech_p_[3]);
+            ech_->InitControl(ech_max_, ech_atol_, ech_rtol_, ech_dtol_);
         }
         VectorH resh, corh;
         VectorL resl, corl;
@@ -493,12 +502,16 @@ class MPIR : public LinearSolver<LADh> {
         this->res_ = resh.Norm2();
         double res0 = this->res_;
         this->hist_.push_back(this->res_);
+        // res0 is reset to the residual just computed after every correction
+        // (mpir.cc:275): with relative tolerances below 1 the relative tests
+        // compare the residual with itself and never end the loop
         auto ecl_ok = [&]() {
-            return n_ecl_ < max_ecl_ && this->res_ > ecl_p_[1] && this->res_ > res0 * ecl_p_[2] && this->res_ < ecl_p_[3];
+            return n_ecl_ < max_ecl_ && this->res_ > ecl_atol_ && this->res_ > res0 * ecl_rtol_ &&
+                   this->res_ < ecl_dtol_;
         };
         auto ech_ok = [&]() {
-            return ech_ && n_ech_ < max_ech_ && this->res_ > ech_p_[1] && this->res_ > res0 * ech_p_[2] &&
-                   this->res_ < ech_p_[3];
+            return ech_ && n_ech_ < max_ech_ && this->res_ > ech_atol_ && this->res_ > res0 * ech_rtol_ &&
+                   this->res_ < ech_dtol_;
         };
         while (this->res_ > abs_ && this->res_ > res0 * rel_ && this->res_ < div_ && (ech_ok() || ecl_ok())) {
             if (ecl_ok()) {
@@ -531,7 +544,9 @@ class MPIR : public LinearSolver<LADh> {
     LinearSolver<LADh> *ech_ = nullptr;
     int max_ech_ = 0, max_ecl_ = 0, n_ech_ = 0, n_ecl_ = 0;
     double abs_ = 0, rel_ = 0, div_ = 0;
-    std::vector<double> ecl_p_{1000, 1e-6, 1e-1, 1e6}, ech_p_{1000, 1e-12, 1e-12, 1e6};
+    int ecl_max_ = 1000, ech_max_ = 1000;
+    DataL ecl_atol_ = DataL(1e-6), ecl_rtol_ = DataL(1e-1), ecl_dtol_ = DataL(1e6);
+    DataH ech_atol_ = DataH(1e-12), ech_rtol_ = DataH(1e-12), ech_dtol_ = DataH(1e6);
 };
 
 }  // namespace hflmp

@@ -296,9 +296,16 @@ int hfh_jacobi_solve(const hfh_csr *A, const hfh_vec *b, hfh_vec *x, int mode, d
 /* Mixed-precision defect correction (MPIR<D,S>::Solve, mpir.cc:69-327):
  * FP64 true residual, FP64 update and FP64 outer test on the device; FP32
  * matrix values and FP32 inner PCG with preconditioner Pl (Jacobi or MC-SGS
- * built on Al).  Outer stop: true residual < outer.atol, or
- * < outer.rtol * previous outer residual (the reference res0_ reset), or
- * outer.maxits corrections.  inner: control of each FP32 inner solve. */
+ * built on Al).  The outer loop runs while the true residual is > outer.atol,
+ * > outer.rtol * res0 and < outer.dtol, and while a correction is allowed
+ * (fewer than outer.maxits so far, residual > inner.atol, > inner.rtol * res0,
+ * < inner.dtol).  res0 is reset to the residual just computed after every
+ * correction (the reference's res0_ reset, mpir.cc:275), so with rtol < 1
+ * the relative tests compare the residual with itself: only atol, dtol and
+ * maxits end the loop, and HFH_SUCCESS_REL is not returned.  inner: control
+ * of each FP32 inner solve; its tolerances are rounded to float first, as the
+ * reference stores them (DataType_low members, mpir.h:225-240).  x and b must
+ * not alias. */
 typedef struct {
     int outer_iters;     /* corrections applied */
     int inner_iters;     /* accumulated inner iterations */

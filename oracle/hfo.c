@@ -270,8 +270,13 @@ void hfo_cast_f2d(int n, const float *in, double *out)
 }
 
 /* MPIR<LAD_high, LAD_low>::Solve and ContinueIteration (mpir.cc:69-327).
- * res0 is reset to the residual after every correction (mpir.cc:275), so the
- * relative MPIR test compares against the previous outer residual. */
+ * res0 is reset to the residual just computed after every correction
+ * (mpir.cc:275), so the relative tests (MPIR's own and the error-correction
+ * gates) compare the residual with itself: with rel_tol < 1 they never end the
+ * loop, and only abs_tol, div_tol or the loop limits do.  The low precision
+ * tolerances are members of type DataType_low (mpir.h:225-240), so they are
+ * rounded to float before the gates compare them and before they reach the
+ * inner solver's InitControl (mpir.cc:456-472). */
 int hfo_mpir(int n, const int *row, const int *col, const double *val_d,
              const float *val_f, const double *b, double *x,
              const hfo_prec_f *Pl, const hfo_prec_d *Ph,
@@ -283,6 +288,10 @@ int hfo_mpir(int n, const int *row, const int *col, const double *val_d,
     float *corl = (float *)calloc(n, sizeof(float));
     int n_ecl = 0, n_ech = 0, iters = 0, nloop = 0;
     double res, res0;
+    hfo_ctrl ecl = p->ecl;
+    ecl.atol = (double)(float)ecl.atol;
+    ecl.rtol = (double)(float)ecl.rtol;
+    ecl.dtol = (double)(float)ecl.dtol;
 
     hfo_spmv_d(n, row, col, val_d, x, resh);
     hfo_scaleadd_d(n, resh, -1.0, b);
@@ -290,8 +299,8 @@ int hfo_mpir(int n, const int *row, const int *col, const double *val_d,
     res0 = res;
     if (hist)
         hist[nloop] = res;
-#define ECL_OK (n_ecl < p->max_ecl && res > p->ecl.atol && res > res0 * p->ecl.rtol && \
-                res < p->ecl.dtol)
+#define ECL_OK (n_ecl < p->max_ecl && res > ecl.atol && res > res0 * ecl.rtol && \
+                res < ecl.dtol)
 #define ECH_OK (n_ech < p->max_ech && res > p->ech.atol && res > res0 * p->ech.rtol && \
                 res < p->ech.dtol)
     while (res > p->abs_tol && res > res0 * p->rel_tol && res < p->div_tol &&
@@ -301,7 +310,7 @@ int hfo_mpir(int n, const int *row, const int *col, const double *val_d,
             ++n_ecl;
             hfo_cast_d2f(n, resh, resl);
             memset(corl, 0, sizeof(float) * n);
-            hfo_cg_f(n, row, col, val_f, resl, corl, Pl, &p->ecl, &r, NULL);
+            hfo_cg_f(n, row, col, val_f, resl, corl, Pl, &ecl, &r, NULL);
             iters += r.iters;
             hfo_cast_f2d(n, corl, corh);
         } else if (ECH_OK) {

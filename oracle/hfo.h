@@ -14,9 +14,7 @@
  * the HiFlow3 2.0 source tree).
  *
  * Parity pins (what each function is checked against) are listed in
- * oracle/README.md.  Parity unpinned: the loop count of hfo_mpir under the
- * reference's res0_ reset (mpir.cc:275); its result is pinned.  Every other
- * routine has an independent check.
+ * oracle/README.md.  Every routine has an independent check.
  */
 #ifndef HFO_H
 #define HFO_H

@@ -79,6 +79,22 @@ def test_lmp_mpir(pkg, oracle):
     assert relnorm(x, xo) < 1e-8
 
 
+def test_lmp_mpir_res0_reset(pkg, oracle):
+    """The template keeps the reference's res0_ reset (mpir.cc:275): with
+    rel_tol 0.5 and no reachable abs_tol, exactly max_ecl corrections run and
+    Solve returns kSolverExceeded, as the oracle's MPIR."""
+    row, col, val, rhs = pkg.problem(pkg.FV_DIFFUSION, 10, p0=0.5, seed=3)
+    OA = oracle.Csr(row, col, val)
+    for k in (1, 3):
+        x, info = pkg.lmp_mpir(row, col, val, rhs, precond_low=pkg.PREC_JACOBI, max_ecl=k, abs_tol=0.0,
+                               rel_tol=0.5, ecl=(1000, 0.0, 0.125, 1e6))
+        xo, oi = oracle.mpir(OA, rhs, Pl=oracle.Prec.jacobi(OA), max_ecl=k, abs_tol=0.0, rel_tol=0.5,
+                             ecl=(1000, 0.0, 0.125, 1e6))
+        assert info["n_ecl"] == oi["n_ecl"] == k and info["ret"] == oi["ret"] == 1
+        assert info["hist"][1] < 0.5 * info["hist"][0]
+        assert relnorm(info["hist"], oi["hist"]) < 1e-5
+
+
 def test_lmp_errors(pkg):
     row, col, val, rhs = pkg.problem(pkg.FV_DIFFUSION, 4)
     with pytest.raises(pkg.HfhError):

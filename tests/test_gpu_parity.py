@@ -666,6 +666,46 @@ def test_defect_correction_matches_fp64(pkg, ctx, oracle):
     assert relnorm(X.numpy(), xd) < 1e-8
 
 
+def test_defect_correction_res0_reset(pkg, ctx, oracle):
+    """mpir.cc:275 resets res0_ after every correction, so a relative tolerance
+    below 1 never ends the loop: exactly maxits corrections and no success,
+    as the oracle's MPIR, although the first correction already halves the
+    residual.  An inner absolute gate above the residual stops before any."""
+    row, col, val, rhs = fv3(pkg, 12, sigma=0.5, seed=3)
+    A = pkg.CsrMatrix(ctx, row, col, val)
+    Al = pkg.CsrMatrix.cast_of(A, np.float32)
+    Pl = pkg.Preconditioner.jacobi(Al)
+    OA = oracle.Csr(row, col, val)
+    for k in (1, 3):
+        X = pkg.Vector(ctx, len(rhs))
+        info = pkg.dc(A, Al, pkg.Vector(ctx, rhs), X, Pl, outer=(k, 0.0, 0.5, 1e30),
+                      inner=(1000, 0.0, 0.125, 1e30), check=False)
+        xo, oi = oracle.mpir(OA, rhs, Pl=oracle.Prec.jacobi(OA), max_ecl=k, abs_tol=0.0, rel_tol=0.5,
+                             div_tol=1e30, ecl=(1000, 0.0, 0.125, 1e30))
+        assert oi["n_ecl"] == k and oi["ret"] == 1
+        assert info["outer_iters"] == k and info["state"] == pkg.FAIL_MAXITS and info["rc"] == pkg.ERR_NOT_CONVERGED
+        assert info["hist"][1] < 0.5 * info["hist"][0]
+        assert relnorm(info["hist"], oi["hist"]) < 1e-5
+    X = pkg.Vector(ctx, len(rhs))
+    info = pkg.dc(A, Al, pkg.Vector(ctx, rhs), X, Pl, outer=(4, 0.0, 0.0, 1e30),
+                  inner=(1000, 2.0 * np.linalg.norm(rhs), 0.125, 1e30), check=False)
+    assert info["outer_iters"] == 0 and info["state"] == pkg.FAIL_MAXITS and np.all(X.numpy() == 0.0)
+
+
+def test_defect_correction_argument_checks(pkg, ctx):
+    row, col, val, rhs = fv3(pkg, 6)
+    A = pkg.CsrMatrix(ctx, row, col, val)
+    Al = pkg.CsrMatrix.cast_of(A, np.float32)
+    B = pkg.Vector(ctx, rhs)
+    for args in (dict(x=B), dict(outer=(5, -1.0, 0.0, 1e6)), dict(inner=(100, 0.0, 0.1, 0.5)),
+                 dict(outer=(5, 0.0, -0.1, 1e6))):
+        kw = dict(x=pkg.Vector(ctx, len(rhs)), outer=(5, 0.0, 1e-8, 1e6), inner=(100, 0.0, 0.1, 1e6))
+        kw.update(args)
+        with pytest.raises(pkg.HfhError) as e:
+            pkg.dc(A, Al, B, kw["x"], None, outer=kw["outer"], inner=kw["inner"])
+        assert e.value.code == pkg.ERR_INVALID
+
+
 @pytest.mark.parametrize("delta", [1e-1, 1e-3])
 def test_defect_correction_keep_direction(pkg, ctx, delta):
     """Reliable-update form (no reference counterpart): the inner CG keeps its

@@ -404,6 +404,51 @@ def test_mpir(oracle, pkg):
     assert relnorm(x, spla.spsolve(to_scipy(A).tocsc(), b)) < 1e-8
 
 
+def _float_gate(res0):
+    """A double d whose float rounding lies on the other side of res0: the
+    gate 'res0 > d' has one answer in double and the other in float."""
+    lo = np.float32(res0)
+    if float(lo) > res0:
+        lo = np.nextafter(lo, np.float32(0))
+    hi = np.nextafter(lo, np.float32(np.inf))
+    mid = (float(lo) + float(hi)) / 2
+    d = (res0 + mid) / 2      # between res0 and the rounding midpoint
+    assert (res0 > d) != (res0 > float(np.float32(d)))
+    return d, res0 > float(np.float32(d))
+
+
+def test_mpir_res0_reset(oracle, pkg):
+    """mpir.cc:275 resets res0_ to the residual just computed, after every
+    correction, and ContinueIteration (mpir.cc:308-327) tests res_ > res0_*rel.
+    With rel_tol < 1 that test compares the residual with itself, so it never
+    ends the loop: exactly max_ecl corrections run and Solve returns
+    kSolverExceeded (mpir.cc:300-303).  A rule relative to the initial residual
+    would stop after the first correction, which already gains 10x here."""
+    A, b = fv_matrix(pkg, oracle, 8, 8, 8, sigma=0.5, seed=3)
+    for k in (1, 3, 5):
+        x, info = oracle.mpir(A, b, Pl=oracle.Prec.jacobi(A), max_ecl=k, abs_tol=0.0, rel_tol=0.5,
+                              ecl=(1000, 0.0, 0.125, 1e6))
+        assert info["n_ecl"] == k and info["ret"] == 1
+        h = info["hist"]
+        assert len(h) == k + 1 and h[1] < 0.5 * h[0]
+    # the error-correction gates use res0_ too: a relative gate of 0.5 never
+    # closes, an absolute gate above the residual closes before any correction
+    x, info = oracle.mpir(A, b, Pl=oracle.Prec.jacobi(A), max_ecl=4, abs_tol=0.0, rel_tol=0.0,
+                          ecl=(1000, 0.0, 0.5, 1e6))
+    assert info["n_ecl"] == 4
+    nb = np.linalg.norm(b)
+    x, info = oracle.mpir(A, b, Pl=oracle.Prec.jacobi(A), max_ecl=4, abs_tol=0.0, rel_tol=0.0,
+                          ecl=(1000, 2.0 * nb, 0.125, 1e6))
+    assert info["n_ecl"] == 0 and info["ret"] == 1 and np.all(x == 0.0)
+    # abs_tol_ecl_ is a DataType_low member (mpir.h:225): the gate compares the
+    # FP64 residual with the tolerance rounded to float
+    res0 = float(np.linalg.norm(b))
+    d, runs = _float_gate(res0)
+    x, info = oracle.mpir(A, b, Pl=oracle.Prec.jacobi(A), max_ecl=1, abs_tol=0.0, rel_tol=0.0,
+                          ecl=(1000, d, 0.125, 1e6))
+    assert info["n_ecl"] == (1 if runs else 0)
+
+
 # --------------------------------------------------------------- stationary Jacobi (jacobi.cc)
 
 def test_swap_diag_front_and_nodiag(oracle):
