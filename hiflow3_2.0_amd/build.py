@@ -22,7 +22,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("HFH_ARCH", "gfx950")
 
 HIP_SOURCES = ["hfh_core.hip", "hfh_spmv.hip", "hfh_prec.hip", "hfh_krylov.hip", "hfh_batch.hip", "hfh_jacobi.hip",
-               "hfh_analysis.hip", "hfh_assemble.hip"]
+               "hfh_analysis.hip", "hfh_assemble.hip", "hfh_vecops.hip", "hfh_matops.hip"]
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-result",
     # packed FP32 instructions are kept out of the device code
@@ -90,6 +90,23 @@ def build_lmp(force=True, jobs=4):
          [f"-L{HERE}", "-lhfhip", "-Wl,-rpath,$ORIGIN", "-Wl,--no-undefined"])
     os.replace(tmp, LMP_LIB)
     return LMP_LIB
+
+
+ORACLE = os.path.join(ROOT, "oracle")
+TEST_DRIVER = os.path.join(ROOT, "tests", "bin", "lmp_iface")
+
+
+def build_test_driver():
+    """The C++ test of the lmp interface (tests/lmp_iface.cc), linked with
+    both product libraries and the oracle it checks against."""
+    os.makedirs(os.path.dirname(TEST_DRIVER), exist_ok=True)
+    tmp = TEST_DRIVER + ".tmp"
+    _run([CXX, "-O1", "-std=c++17", "-Wall", f"-I{INCLUDE}", f"-I{LMP}", f"-I{ORACLE}",
+          "-o", tmp, os.path.join(ROOT, "tests", "lmp_iface.cc"),
+          f"-L{HERE}", "-lhflmp", "-lhfhip", f"-L{ORACLE}", "-lhfo", "-fopenmp",
+          f"-Wl,-rpath,$ORIGIN/../../hiflow3_2.0_amd", "-Wl,-rpath,$ORIGIN/../../oracle"])
+    os.replace(tmp, TEST_DRIVER)
+    return TEST_DRIVER
 
 
 def build_all(force=True):

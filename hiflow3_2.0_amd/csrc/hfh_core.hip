@@ -158,6 +158,11 @@ int dot_async(hfh_ctx *ctx, int dtype, int n, const void *x, const void *y, doub
     return launch_check("k_fin_store");
 }
 
+int partials_sum(hfh_ctx *ctx, int nb, double *dst) {
+    hipLaunchKernelGGL(k_fin_store, dim3(1), dim3(kFinBlock), 0, ctx->stream, (const double *)ctx->red, nb, dst);
+    return launch_check("k_fin_store");
+}
+
 int fetch(hfh_ctx *ctx, const double *dsrc, double *host, int count) {
     HFH_HIP(hipMemcpyAsync(ctx->host_res, dsrc, sizeof(double) * count, hipMemcpyDeviceToHost,
                            ctx->stream));
@@ -379,6 +384,7 @@ extern "C" int hfh_vec_destroy(hfh_vec *v) {
     if (!v) return HFH_OK;
     (void)hipStreamSynchronize(v->ctx->stream);
     if (v->d) (void)hipFree(v->d);
+    if (v->iset) (void)hipFree(v->iset);
     --v->ctx->live;
     delete v;
     return HFH_OK;
@@ -423,30 +429,6 @@ static int check_index(const hfh_vec *v, const int *index, int count, const char
         HFH_CHECK(index[i] >= 0 && index[i] < v->n, HFH_ERR_INVALID, "%s: index[%d]=%d outside [0,%d)",
                   what, i, index[i], v->n);
     return HFH_OK;
-}
-
-extern "C" int hfh_vec_set_values(hfh_vec *v, const int *index, int count, const void *values) {
-    HFH_CHECK(v && count >= 0 && (count == 0 || (index && values)), HFH_ERR_INVALID,
-              "hfh_vec_set_values: bad argument");
-    if (count == 0) return HFH_OK;
-    int rc = check_index(v, index, count, "hfh_vec_set_values");
-    if (rc) return rc;
-    void *dv = nullptr, *di = nullptr;
-    if ((rc = dev_alloc(&dv, v->elem() * count))) return rc;
-    if ((rc = dev_alloc(&di, sizeof(int) * count))) { (void)hipFree(dv); return rc; }
-    hipStream_t s = v->ctx->stream;
-    HFH_HIP(hipMemcpyAsync(dv, values, v->elem() * count, hipMemcpyHostToDevice, s));
-    HFH_HIP(hipMemcpyAsync(di, index, sizeof(int) * count, hipMemcpyHostToDevice, s));
-    const int nb = ew_blocks(v->ctx, count);
-    if (v->dtype == HFH_F64)
-        hipLaunchKernelGGL(k_scatter<double>, dim3(nb), dim3(kBlock), 0, s, count, (double *)v->d, (const double *)dv, (const int *)di);
-    else
-        hipLaunchKernelGGL(k_scatter<float>, dim3(nb), dim3(kBlock), 0, s, count, (float *)v->d, (const float *)dv, (const int *)di);
-    rc = launch_check("k_scatter");
-    (void)hipStreamSynchronize(s);
-    (void)hipFree(dv);
-    (void)hipFree(di);
-    return rc;
 }
 
 extern "C" int hfh_vec_get_values(const hfh_vec *v, const int *index, int count, void *values) {

@@ -54,6 +54,10 @@ struct hfh_vec {
     int n = 0;
     int dtype = HFH_F64;
     void *d = nullptr;
+    // index set (lVector::set_indexset): device copy for the gather / scatter
+    // of GetIndexedValues / SetIndexedValues, host copy for get_indexset
+    int *iset = nullptr;
+    std::vector<int> h_iset;
     size_t elem() const { return dtype == HFH_F64 ? 8 : 4; }
 };
 
@@ -199,6 +203,9 @@ int launch_check(const char *what);
 // Reductions (hfh_blas1.hip).  Results land in ctx->red + slot (device).
 int dot_async(hfh_ctx *ctx, int dtype, int n, const void *x, const void *y, double *dst);
 int fetch(hfh_ctx *ctx, const double *dsrc, double *host, int count);
+// Sums the first nb single-quantity partials of ctx->red into dst (device) in
+// fixed order (one finisher workgroup).
+int partials_sum(hfh_ctx *ctx, int nb, double *dst);
 
 // Timing helpers.
 struct ScopedKernelTimer {

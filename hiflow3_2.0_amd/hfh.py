@@ -94,6 +94,24 @@ HIP_API = {
     "hfh_scaleadd": (_i, [_vp, _d, _vp]),
     "hfh_scale": (_i, [_vp, _d]),
     "hfh_ewmult": (_i, [_vp, _vp]),
+    "hfh_norm1": (_i, [_vp, _P(_d)]),
+    "hfh_iamax": (_i, [_vp, _P(_i)]),
+    "hfh_iamin": (_i, [_vp, _P(_i)]),
+    "hfh_normmax": (_i, [_vp, _P(_d)]),
+    "hfh_vec_add_values": (_i, [_vp, _vp, _i, _vp]),
+    "hfh_vec_add_range": (_i, [_vp, _i, _vp, _i, _i, _d]),
+    "hfh_vec_set_indexset": (_i, [_vp, _vp, _i]),
+    "hfh_vec_get_indexset": (_i, [_vp, _vp, _P(_i)]),
+    "hfh_vec_get_indexed_values": (_i, [_vp, _vp]),
+    "hfh_vec_set_indexed_values": (_i, [_vp, _vp]),
+    "hfh_rot": (_i, [_vp, _vp, _d, _d]),
+    "hfh_rotm": (_i, [_vp, _vp, _vp]),
+    "hfh_csr_diag": (_i, [_vp, _i, _i, _i, _vp]),
+    "hfh_csr_zeros": (_i, [_vp]),
+    "hfh_csr_scale": (_i, [_vp, _d, _i]),
+    "hfh_csr_zero_rows": (_i, [_vp, _vp, _i, _d]),
+    "hfh_csr_zero_cols": (_i, [_vp, _vp, _i, _d]),
+    "hfh_csr_pjacobi": (_i, [_vp, _vp, _vp]),
     "hfh_csr_create": (_i, [_vp, _i, _i, _i, _i, _P(_vp)]),
     "hfh_csr_destroy": (_i, [_vp]),
     "hfh_csr_dims": (_i, [_vp, _P(_i), _P(_i), _P(_i), _P(_i)]),
@@ -360,6 +378,61 @@ class Vector:
     def ewmult(self, x):
         _ck(hip().hfh_ewmult(self.h, x.h))
 
+    def norm1(self):
+        d = C.c_double()
+        _ck(hip().hfh_norm1(self.h, C.byref(d)))
+        return d.value
+
+    def normmax(self):
+        d = C.c_double()
+        _ck(hip().hfh_normmax(self.h, C.byref(d)))
+        return d.value
+
+    def argmax(self):
+        i = C.c_int()
+        _ck(hip().hfh_iamax(self.h, C.byref(i)))
+        return i.value
+
+    def argmin(self):
+        i = C.c_int()
+        _ck(hip().hfh_iamin(self.h, C.byref(i)))
+        return i.value
+
+    def add_values(self, index, values):
+        idx = np.ascontiguousarray(index, np.int32)
+        v = np.ascontiguousarray(values, self.dtype)
+        _ck(hip().hfh_vec_add_values(self.h, _p(idx), len(idx), _p(v)))
+
+    def add_range(self, dst_off, src, src_off, count, weight):
+        _ck(hip().hfh_vec_add_range(self.h, dst_off, src.h, src_off, count, float(weight)))
+
+    def set_indexset(self, index):
+        idx = np.ascontiguousarray(index, np.int32)
+        _ck(hip().hfh_vec_set_indexset(self.h, _p(idx), len(idx)))
+
+    def get_indexset(self):
+        n = C.c_int()
+        _ck(hip().hfh_vec_get_indexset(self.h, None, C.byref(n)))
+        idx = np.zeros(n.value, np.int32)
+        _ck(hip().hfh_vec_get_indexset(self.h, _p(idx), C.byref(n)))
+        return idx
+
+    def get_indexed_values(self):
+        out = np.zeros(len(self.get_indexset()), self.dtype)
+        _ck(hip().hfh_vec_get_indexed_values(self.h, _p(out)))
+        return out
+
+    def set_indexed_values(self, values):
+        v = np.ascontiguousarray(values, self.dtype)
+        _ck(hip().hfh_vec_set_indexed_values(self.h, _p(v)))
+
+    def rot(self, other, c, s):
+        _ck(hip().hfh_rot(self.h, other.h, float(c), float(s)))
+
+    def rotm(self, other, param):
+        p = np.ascontiguousarray(param, np.float64)
+        _ck(hip().hfh_rotm(self.h, other.h, _p(p)))
+
 
 class CsrMatrix:
     def __init__(self, ctx, row, col, val, ncol=None, dtype=None):
@@ -440,6 +513,29 @@ class CsrMatrix:
         v = Vector(self.ctx, self.nrow, self.dtype)
         _ck(hip().hfh_csr_invdiag(self.h, v.h))
         return v
+
+    def diag(self, start=0, end=None, invert=False):
+        end = self.nrow if end is None else end
+        v = Vector(self.ctx, end - start, self.dtype)
+        _ck(hip().hfh_csr_diag(self.h, start, end, 1 if invert else 0, v.h))
+        return v
+
+    def zeros(self):
+        _ck(hip().hfh_csr_zeros(self.h))
+
+    def scale(self, alpha, offdiag_only=False):
+        _ck(hip().hfh_csr_scale(self.h, float(alpha), 1 if offdiag_only else 0))
+
+    def zero_rows(self, rows, alpha):
+        r = np.ascontiguousarray(rows, np.int32)
+        _ck(hip().hfh_csr_zero_rows(self.h, _p(r), len(r), float(alpha)))
+
+    def zero_cols(self, cols, alpha):
+        c = np.ascontiguousarray(cols, np.int32)
+        _ck(hip().hfh_csr_zero_cols(self.h, _p(c), len(c), float(alpha)))
+
+    def pjacobi(self, x, y):
+        _ck(hip().hfh_csr_pjacobi(self.h, x.h, y.h))
 
     def levels(self, upper=False):
         """Level schedule computed on the device: (level_ptr, level_rows), the

@@ -134,3 +134,36 @@ int csr_permute(int n, const int *row, const int *col, const double *val, const 
 }
 
 }  // namespace hflmp
+
+namespace hflmp {
+
+void levelscheduling(int n, const int *row, const int *col, int *nlevels, int *level_sizes, int *perm) {
+    std::vector<int> depth(n, 0), ends;
+    std::vector<char> placed(n, 0);
+    int done = 0, nlev = 0;
+    for (int L = 0; L < n; ++L) {
+        if (done == n) {
+            nlev = static_cast<int>(ends.size());
+            break;
+        }
+        for (int k = 0; k < n; ++k) {
+            if (placed[k] || depth[k] != L) continue;
+            bool ready = true;
+            for (int j = row[k]; j < row[k + 1] && ready; ++j)
+                if (col[j] < k && !placed[col[j]]) ready = false;
+            if (!ready) continue;
+            perm[done++] = k;
+            placed[k] = 1;
+            for (int j = row[k]; j < row[k + 1]; ++j)
+                if (col[j] > k && depth[col[j]] <= depth[k]) depth[col[j]] = depth[k] + 1;
+        }
+        ends.push_back(done);
+    }
+    *nlevels = nlev;
+    for (int i = 0, prev = 0; i < nlev; ++i) {
+        level_sizes[i] = ends[i] - prev;
+        prev = ends[i];
+    }
+}
+
+}  // namespace hflmp

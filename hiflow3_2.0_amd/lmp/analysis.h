@@ -33,3 +33,16 @@ int csr_permute(int n, const int *row, const int *col, const double *val, const 
                 int *pcol, double *pval);
 
 }  // namespace hflmp
+
+namespace hflmp {
+
+// CPU_CSR_lMatrix::Levelscheduling (lmatrix_csr_cpu.cc:2295-2391) itself:
+// sweeps L = 0, 1, ... over the rows in order; a row of depth L whose lower
+// neighbours are all placed is placed, and raises the depth of its higher
+// neighbours to L + 1.  *nlevels is the count the reference reports: the
+// levels are only counted when a sweep starts with every row placed, so a
+// schedule whose last row is placed in sweep n - 1 reports 0 levels.
+// level_sizes needs n entries (*nlevels are written), perm n.
+void levelscheduling(int n, const int *row, const int *col, int *nlevels, int *level_sizes, int *perm);
+
+}  // namespace hflmp

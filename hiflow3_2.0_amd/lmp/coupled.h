@@ -67,6 +67,14 @@ class CoupledMatrix {
         m->CastFrom(dynamic_cast<const HIP_CSR_lMatrix<U> &>(other.diagonal()));
         diagonal_.reset(m);
     }
+    // CoupledMatrix::Zeros (coupled_matrix.cc:540-547)
+    void Zeros() { diagonal_->Zeros(); }
+    // CoupledMatrix::diagonalize_rows (coupled_matrix.cc:550-575): the listed
+    // (global = local on one rank) rows zeroed, diagonal_value on the diagonal;
+    // the off-diagonal block is empty on one rank
+    void diagonalize_rows(const int *row_indices, int num_rows, T diagonal_value) {
+        diagonal_->ZeroRows(row_indices, num_rows, diagonal_value);
+    }
     // CoupledMatrix::VectorMult (coupled_matrix.cc:433-468): diagonal block only on one rank
     void VectorMult(const CoupledVector<T> &in, CoupledVector<T> *out) const {
         diagonal_->VectorMult(in.interior(), &out->interior());

@@ -106,6 +106,43 @@ int hfh_scaleadd(hfh_vec *y, double a, const hfh_vec *x);
 int hfh_scale(hfh_vec *y, double a);
 /* ElementWiseMult: y .*= x (lvector_cpu.cc:571-578) */
 int hfh_ewmult(hfh_vec *y, const hfh_vec *x);
+/* Norm1: sum |x_i| (CPUsimple_lvector.cc:95-140); FP64 partials per
+ * workgroup summed in a fixed order (the result differs from the reference's
+ * unroll-5 order in the last bits, as Dot does).  HFH_ERR_INVALID when empty. */
+int hfh_norm1(const hfh_vec *x, double *out);
+/* ArgMax / ArgMin (CPUsimple_lvector.cc:51-93): the first index of the
+ * largest / smallest |x_i|, with the reference loop's start at |x_0|: NaN
+ * entries are never chosen, and a NaN x_0 answers 0.  HFH_ERR_INVALID when
+ * empty. */
+int hfh_iamax(const hfh_vec *x, int *index);
+int hfh_iamin(const hfh_vec *x, int *index);
+/* NormMax = |x[ArgMax]| (CPUsimple_lvector.cc:154-157) */
+int hfh_normmax(const hfh_vec *x, double *out);
+/* add_values (lvector_cpu.cc:123-135): v[index[i]] += values[i], a repeated
+ * index adding its values in their order (add_value is count = 1) */
+int hfh_vec_add_values(hfh_vec *v, const int *index, int count, const void *values);
+/* partial_add_subvector (lvector_cpu.cc:504-548):
+ * dst[dst_off + i] += weight * src[src_off + i], i < count */
+int hfh_vec_add_range(hfh_vec *dst, int dst_off, const hfh_vec *src, int src_off, int count, double weight);
+/* set_indexset / get_indexset / get_indexset_size / GetIndexedValues /
+ * SetIndexedValues (lvector_cpu.cc:217-297): the index set lives on the
+ * device with the vector; the values move between the host and the indexed
+ * entries (a gather / scatter kernel); a repeated index takes its last value
+ * on SetIndexedValues, as the sequential loop does.  get_indexset: index may
+ * be NULL to query the count. */
+int hfh_vec_set_indexset(hfh_vec *v, const int *index, int count);
+int hfh_vec_get_indexset(const hfh_vec *v, int *index, int *count);
+int hfh_vec_get_indexed_values(const hfh_vec *v, void *values);
+int hfh_vec_set_indexed_values(hfh_vec *v, const void *values);
+/* Rot (CPUsimple_lvector.cc:448-466): x_i' = c x_i + s y_i, y_i' = c y_i - s x_i
+ * (x is the vector Rot is called on, y its argument) */
+int hfh_rot(hfh_vec *x, hfh_vec *y, double c, double s);
+/* Rotm (CPUsimple_lvector.cc:540-600), param = {flag, h11, h21, h12, h22}:
+ * u is the vector Rotm is called on, v its argument, with the reference's
+ * roles: flag -1: v' = h11 v + h12 u, u' = h21 v + h22 u; flag 0: the same
+ * with h11 = h22 = 1; flag 1: v' = h11 v + u, u' = h22 u - v; any other flag
+ * (-2): no change */
+int hfh_rotm(hfh_vec *u, hfh_vec *v, const double *param);
 
 /* ---- CSR matrices: CSR_lMatrixType<T> (lmp/lmatrix_formats.h:40-46) ---- */
 /* lMatrix::Init (lmatrix.h:107) */
@@ -167,6 +204,24 @@ int hfh_csr_set_spmv(hfh_csr *A, int algo, int param1, int param2);
 int hfh_csr_get_spmv(const hfh_csr *A, int *algo, int *param1);
 /* extract_invdiagelements over all rows (lmatrix_csr_cpu.cc:2414-2432) */
 int hfh_csr_invdiag(const hfh_csr *A, hfh_vec *invd);
+/* extract_diagelements / extract_invdiagelements over rows [start, end)
+ * (lmatrix_csr_cpu.cc:2395-2432): out[i] = a_ii or 1 / a_ii, i = start + k;
+ * 0 for a row without a stored diagonal */
+int hfh_csr_diag(const hfh_csr *A, int start, int end, int invert, hfh_vec *out);
+/* Zeros (lmatrix_csr_cpu.cc:100-104): every value 0, structure kept */
+int hfh_csr_zeros(hfh_csr *A);
+/* Scale / ScaleOffdiag (lmatrix_csr_cpu.cc:206-225): values *= alpha (only
+ * the entries off the diagonal when offdiag_only) */
+int hfh_csr_scale(hfh_csr *A, double alpha, int offdiag_only);
+/* ZeroRows (lmatrix_csr_cpu.cc:1047-1068): the listed rows' entries 0, their
+ * diagonal entries alpha; ZeroCols (lmatrix_csr_cpu.cc:1071-1098): the same
+ * for the listed columns (count > 0).  Indices are checked against the
+ * matrix's rows / columns. */
+int hfh_csr_zero_rows(hfh_csr *A, const int *rows, int count, double alpha);
+int hfh_csr_zero_cols(hfh_csr *A, const int *cols, int count, double alpha);
+/* Pjacobi (lmatrix_csr_cpu.cc:920-948): out_i = in_i / a_ii; rows without a
+ * stored diagonal leave out_i unchanged */
+int hfh_csr_pjacobi(const hfh_csr *A, const hfh_vec *in, hfh_vec *out);
 
 /* ---- preconditioners: lPreconditioner<T> (lmp/lpreconditioner.h:35-125) ---- */
 /* lPreconditioner_Jacobi::Build (lpreconditioner.cc:127-148) */

@@ -14,15 +14,19 @@
 
 #define T double
 #define S d
+#define ABS fabs
 #include "hfo_t.h"
 #undef T
 #undef S
+#undef ABS
 
 #define T float
 #define S f
+#define ABS fabsf
 #include "hfo_t.h"
 #undef T
 #undef S
+#undef ABS
 
 /* CPU_CSR_lMatrix::TransformFromCOO (lmatrix_csr_cpu.cc:951-1044) */
 void hfo_csr_from_coo_d(int nrow, int nnz, const int *rows, const int *cols,

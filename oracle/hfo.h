@@ -136,6 +136,22 @@ int    hfo_fgmres_##S(int n, const int *row, const int *col, const T *val, const
                       T *x, const hfo_prec_##S *P, int size_basis, const hfo_ctrl *ctrl, \
                       hfo_result *res, double *hist);                                    \
 void   hfo_swap_diag_front_##S(int n, const int *row, int *col, T *val);                  \
+T      hfo_norm1_##S(int n, const T *x);                                                  \
+int    hfo_argmax_##S(int n, const T *x);                                                 \
+int    hfo_argmin_##S(int n, const T *x);                                                 \
+void   hfo_add_values_##S(T *y, const int *idx, int m, const T *v);                       \
+void   hfo_add_range_##S(T *y, int s, const T *x, int ss, int m, T w);                    \
+void   hfo_rot_##S(int n, T *a, T *b, T c, T s);                                          \
+void   hfo_rotm_##S(int n, T *u, T *v, const T *p);                                       \
+void   hfo_rotg_##S(T *sa, T *sb, T *sc, T *ss);                                          \
+void   hfo_zero_rows_##S(const int *row, const int *col, T *val, const int *idx, int m,   \
+                         T alpha);                                                        \
+void   hfo_zero_cols_##S(int n, const int *row, const int *col, T *val, const int *idx,   \
+                         int m, T alpha);                                                 \
+void   hfo_mat_scale_##S(int n, const int *row, const int *col, T *val, T a,              \
+                         int offdiag_only);                                               \
+void   hfo_pjacobi_##S(int n, const int *row, const int *col, const T *val, const T *in,  \
+                       T *out);                                                           \
 void   hfo_spmv_nodiag_##S(int n, const int *row, const int *col, const T *val,           \
                            const T *x, T *y);                                             \
 int    hfo_jacobi_##S(int n, const int *row, const int *col, const T *val, const T *b,    \

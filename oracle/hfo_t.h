@@ -100,6 +100,167 @@ void FN(hfo_ewmult_)(int n, T *y, const T *x)
         y[i] *= x[i];
 }
 
+/* CPUsimple_lVector::Norm1 (CPUsimple_lvector.cc:95-140): the unroll-5 order
+ * of Dot with |x_i| in place of the products. */
+T FN(hfo_norm1_)(int n, const T *x)
+{
+    T d = 0;
+    const int m = n % 5;
+    if (m == 0) {
+        for (int i = 0; i < n; i += 5)
+            d += ABS(x[i]) + ABS(x[i + 1]) + ABS(x[i + 2]) + ABS(x[i + 3]) + ABS(x[i + 4]);
+    } else {
+        for (int i = 0; i != m; ++i)
+            d += ABS(x[i]);
+        if (n > 5)
+            for (int i = m; i < n; i += 5)
+                d += ABS(x[i]) + ABS(x[i + 1]) + ABS(x[i + 2]) + ABS(x[i + 3]) + ABS(x[i + 4]);
+    }
+    return d;
+}
+
+/* CPUsimple_lVector::ArgMax / ArgMin (CPUsimple_lvector.cc:51-93): start at
+ * |x_0|, replace on a strictly larger (smaller) |x_i|. */
+int FN(hfo_argmax_)(int n, const T *x)
+{
+    int best = 0;
+    T v = ABS(x[0]);
+    for (int i = 1; i < n; ++i) {
+        const T c = ABS(x[i]);
+        if (c > v) {
+            best = i;
+            v = c;
+        }
+    }
+    return best;
+}
+
+int FN(hfo_argmin_)(int n, const T *x)
+{
+    int best = 0;
+    T v = ABS(x[0]);
+    for (int i = 1; i < n; ++i) {
+        const T c = ABS(x[i]);
+        if (c < v) {
+            best = i;
+            v = c;
+        }
+    }
+    return best;
+}
+
+/* CPU_lVector::add_values (lvector_cpu.cc:123-135) */
+void FN(hfo_add_values_)(T *y, const int *idx, int m, const T *v)
+{
+    for (int i = 0; i != m; ++i)
+        y[idx[i]] += v[i];
+}
+
+/* CPU_lVector::partial_add_subvector (lvector_cpu.cc:532-548):
+ * y[s + i] += w * x[ss + i] */
+void FN(hfo_add_range_)(T *y, int s, const T *x, int ss, int m, T w)
+{
+    for (int i = 0; i != m; ++i)
+        y[s + i] += w * x[ss + i];
+}
+
+/* CPUsimple_lVector::Rot (CPUsimple_lvector.cc:448-466) on this = a, vec = b */
+void FN(hfo_rot_)(int n, T *a, T *b, T c, T s)
+{
+    for (int i = 0; i != n; ++i) {
+        const T t = c * a[i] + s * b[i];
+        b[i] = c * b[i] - s * a[i];
+        a[i] = t;
+    }
+}
+
+/* CPUsimple_lVector::Rotm (CPUsimple_lvector.cc:540-600) on this = u,
+ * vec = v, p = {flag, h11, h21, h12, h22} */
+void FN(hfo_rotm_)(int n, T *u, T *v, const T *p)
+{
+    const T flag = p[0];
+    if (flag == -2)
+        return;
+    for (int i = 0; i != n; ++i) {
+        const T t1 = v[i], t2 = u[i];
+        if (flag == -1) {
+            v[i] = t1 * p[1] + t2 * p[3];
+            u[i] = t1 * p[2] + t2 * p[4];
+        } else if (flag == 0) {
+            v[i] = t1 + t2 * p[3];
+            u[i] = t1 * p[2] + t2;
+        } else if (flag == 1) {
+            v[i] = t1 * p[1] + t2;
+            u[i] = t2 * p[4] - t1;
+        }
+    }
+}
+
+/* CPUsimple_lVector::Rotg (CPUsimple_lvector.cc:492-536) */
+void FN(hfo_rotg_)(T *sa, T *sb, T *sc, T *ss)
+{
+    const T ta = ABS(*sa), tb = ABS(*sb);
+    const T roe = ta > tb ? *sa : *sb;
+    const T scal = ta + tb;
+    if (scal == 0) {
+        *sc = 1;
+        *ss = *sa = *sb = 0;
+    } else {
+        const T t0 = ta / scal, t1 = tb / scal;
+        T r = scal * (T)sqrt(t0 * t0 + t1 * t1), z;
+        if (roe < 0)
+            r = -r;
+        *sc = *sa / r;
+        *ss = *sb / r;
+        if (ta > tb)
+            z = *ss;
+        else if (*sc != 0)
+            z = 1 / *sc;
+        else
+            z = 1;
+        *sa = r;
+        *sb = z;
+    }
+}
+
+/* CPU_CSR_lMatrix::ZeroRows (lmatrix_csr_cpu.cc:1047-1068) */
+void FN(hfo_zero_rows_)(const int *row, const int *col, T *val, const int *idx, int m, T alpha)
+{
+    for (int i = 0; i != m; ++i) {
+        const int r = idx[i];
+        for (int j = row[r]; j != row[r + 1]; ++j)
+            val[j] = col[j] == r ? alpha : 0;
+    }
+}
+
+/* CPU_CSR_lMatrix::ZeroCols (lmatrix_csr_cpu.cc:1071-1098) */
+void FN(hfo_zero_cols_)(int n, const int *row, const int *col, T *val, const int *idx, int m, T alpha)
+{
+    for (int k = 0; k < n; ++k)
+        for (int j = row[k]; j < row[k + 1]; ++j)
+            for (int i = 0; i < m; ++i)
+                if (idx[i] == col[j])
+                    val[j] = k == idx[i] ? alpha : 0;
+}
+
+/* CPU_CSR_lMatrix::Scale / ScaleOffdiag (lmatrix_csr_cpu.cc:206-225) */
+void FN(hfo_mat_scale_)(int n, const int *row, const int *col, T *val, T a, int offdiag_only)
+{
+    for (int i = 0; i < n; ++i)
+        for (int j = row[i]; j < row[i + 1]; ++j)
+            if (!offdiag_only || col[j] != i)
+                val[j] *= a;
+}
+
+/* CPU_CSR_lMatrix::Pjacobi (lmatrix_csr_cpu.cc:920-948): out_i = in_i / a_ii */
+void FN(hfo_pjacobi_)(int n, const int *row, const int *col, const T *val, const T *in, T *out)
+{
+    for (int i = 0; i < n; ++i)
+        for (int j = row[i]; j < row[i + 1]; ++j)
+            if (col[j] == i)
+                out[i] = in[col[j]] / val[j];
+}
+
 /* CPU_CSR_lMatrix::extract_invdiagelements (lmatrix_csr_cpu.cc:2414-2432) */
 void FN(hfo_invdiag_)(int n, const int *row, const int *col, const T *val, T *invd)
 {

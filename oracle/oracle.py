@@ -106,6 +106,18 @@ def _declare(L):
             f"hfo_swap_diag_front_{s}": (None, [i, vp, vp, vp]),
             f"hfo_spmv_nodiag_{s}": (None, [i, vp, vp, vp, vp, vp]),
             f"hfo_jacobi_{s}": (i, [i, vp, vp, vp, vp, vp, i, t, i, i, vp, vp, vp]),
+            f"hfo_norm1_{s}": (t, [i, vp]),
+            f"hfo_argmax_{s}": (i, [i, vp]),
+            f"hfo_argmin_{s}": (i, [i, vp]),
+            f"hfo_add_values_{s}": (None, [vp, vp, i, vp]),
+            f"hfo_add_range_{s}": (None, [vp, i, vp, i, i, t]),
+            f"hfo_rot_{s}": (None, [i, vp, vp, t, t]),
+            f"hfo_rotm_{s}": (None, [i, vp, vp, vp]),
+            f"hfo_rotg_{s}": (None, [vp, vp, vp, vp]),
+            f"hfo_zero_rows_{s}": (None, [vp, vp, vp, vp, i, t]),
+            f"hfo_zero_cols_{s}": (None, [i, vp, vp, vp, vp, i, t]),
+            f"hfo_mat_scale_{s}": (None, [i, vp, vp, vp, t, i]),
+            f"hfo_pjacobi_{s}": (None, [i, vp, vp, vp, vp, vp]),
         })
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -215,6 +227,76 @@ def ewmult(y, x):
     y = np.array(y)
     getattr(lib(), f"hfo_ewmult_{_sfx(y.dtype)}")(len(y), _p(y), _p(np.ascontiguousarray(x, y.dtype)))
     return y
+
+
+def norm1(x):
+    return getattr(lib(), f"hfo_norm1_{_sfx(x.dtype)}")(len(x), _p(x))
+
+
+def argmax(x):
+    return getattr(lib(), f"hfo_argmax_{_sfx(x.dtype)}")(len(x), _p(x))
+
+
+def argmin(x):
+    return getattr(lib(), f"hfo_argmin_{_sfx(x.dtype)}")(len(x), _p(x))
+
+
+def add_values(y, idx, v):
+    y = np.array(y)
+    idx = np.ascontiguousarray(idx, np.int32)
+    getattr(lib(), f"hfo_add_values_{_sfx(y.dtype)}")(_p(y), _p(idx), len(idx), _p(np.ascontiguousarray(v, y.dtype)))
+    return y
+
+
+def add_range(y, s, x, ss, m, w):
+    y = np.array(y)
+    getattr(lib(), f"hfo_add_range_{_sfx(y.dtype)}")(_p(y), s, _p(np.ascontiguousarray(x, y.dtype)), ss, m, w)
+    return y
+
+
+def rot(a, b, c, s):
+    a, b = np.array(a), np.array(b, dtype=np.asarray(a).dtype)
+    getattr(lib(), f"hfo_rot_{_sfx(a.dtype)}")(len(a), _p(a), _p(b), c, s)
+    return a, b
+
+
+def rotm(u, v, p):
+    u, v = np.array(u), np.array(v, dtype=np.asarray(u).dtype)
+    getattr(lib(), f"hfo_rotm_{_sfx(u.dtype)}")(len(u), _p(u), _p(v), _p(np.ascontiguousarray(p, u.dtype)))
+    return u, v
+
+
+def rotg(a, b, dtype=np.float64):
+    v = np.array([a, b, 0, 0], dtype)
+    getattr(lib(), f"hfo_rotg_{_sfx(dtype)}")(_p(v[0:]), _p(v[1:]), _p(v[2:]), _p(v[3:]))
+    return tuple(v)
+
+
+def zero_rows(A, idx, alpha):
+    val = A.val.copy()
+    idx = np.ascontiguousarray(idx, np.int32)
+    getattr(lib(), f"hfo_zero_rows_{_sfx(val.dtype)}")(_p(A.row), _p(A.col), _p(val), _p(idx), len(idx), alpha)
+    return val
+
+
+def zero_cols(A, idx, alpha):
+    val = A.val.copy()
+    idx = np.ascontiguousarray(idx, np.int32)
+    getattr(lib(), f"hfo_zero_cols_{_sfx(val.dtype)}")(A.n, _p(A.row), _p(A.col), _p(val), _p(idx), len(idx), alpha)
+    return val
+
+
+def mat_scale(A, a, offdiag_only=False):
+    val = A.val.copy()
+    getattr(lib(), f"hfo_mat_scale_{_sfx(val.dtype)}")(A.n, _p(A.row), _p(A.col), _p(val), a, int(offdiag_only))
+    return val
+
+
+def pjacobi(A, x, out=None):
+    out = np.zeros(A.n, A.val.dtype) if out is None else np.array(out, A.val.dtype)
+    getattr(lib(), f"hfo_pjacobi_{_sfx(A.val.dtype)}")(A.n, _p(A.row), _p(A.col), _p(A.val),
+                                                       _p(np.ascontiguousarray(x, A.val.dtype)), _p(out))
+    return out
 
 
 def invdiag(A):

@@ -494,3 +494,96 @@ def test_jacobi_iteration(oracle, pkg):
     Dm = oracle.csr_from_coo(4, [0, 1, 2, 3], [0, 1, 2, 3], [2.0, 4.0, 8.0, 0.5])
     xd, inf = oracle.jacobi(oracle.swap_diag_front(Dm), np.ones(4), mode=0, ctrl=(10, 0.0, 1e-12, 1e6))
     assert inf["iters"] == 1 and list(xd) == [0.5, 0.25, 0.125, 2.0]
+
+
+# --------------------------------------------------------------- the rest of lVector / lMatrix
+
+def test_norm1_order_and_closed_form(oracle):
+    # CPUsimple_lvector.cc:95-140: the N mod 5 head first, left to right:
+    # ((2^53 + 1) + 1) loses both ones; any order adding the ones first keeps them
+    assert oracle.norm1(np.array([2.0**53, 1.0, 1.0])) == 2.0**53
+    assert oracle.norm1(np.array([1.0, 1.0, 2.0**53])) == 2.0**53 + 2
+    # groups of five: the group is summed first, then added to the total
+    x = np.array([2.0**53] + [1.0] * 4 + [1.0] * 5 + [1.0])
+    assert oracle.norm1(x) == 2.0**53 + 4 + 4
+    x = np.random.default_rng(3).uniform(-8, 8, 1001).round()
+    assert oracle.norm1(x) == np.abs(x).sum()
+    assert oracle.norm1(x.astype(np.float32)) == np.float32(np.abs(x).sum())
+
+
+def test_argmax_argmin(oracle):
+    rng = np.random.default_rng(4)
+    for n in (1, 2, 7, 1000):
+        x = rng.integers(-5, 6, n).astype(np.float64)
+        # ties keep the first index (numpy's argmax / argmin do as well)
+        assert oracle.argmax(x) == int(np.argmax(np.abs(x)))
+        assert oracle.argmin(x) == int(np.argmin(np.abs(x)))
+    # the loop starts at |x_0|: a NaN there is never replaced; elsewhere it never wins
+    assert oracle.argmax(np.array([np.nan, 5.0, 9.0])) == 0
+    assert oracle.argmax(np.array([1.0, np.nan, -9.0, 9.0])) == 2
+    assert oracle.argmin(np.array([4.0, np.nan, -0.5, 0.5])) == 2
+    assert oracle.argmax(np.array([-0.0, 0.0])) == 0
+
+
+def test_add_values_and_ranges(oracle):
+    y = np.array([1.0, 2.0, 3.0])
+    # a repeated index adds its values in order: (1 + 1e16) - 1e16 = 0, not 1
+    out = oracle.add_values(y, [0, 2, 0, 0], [1e16, 5.0, -1e16, 0.5])
+    assert list(out) == [0.5, 2.0, 8.0]
+    y = np.arange(6.0)
+    out = oracle.add_range(y, 2, np.array([10.0, 20.0, 30.0, 40.0]), 1, 3, 0.5)
+    assert list(out) == [0.0, 1.0, 12.0, 18.0, 24.0, 5.0]
+
+
+def test_rot_rotm_rotg(oracle):
+    rng = np.random.default_rng(5)
+    a, b = rng.normal(size=50), rng.normal(size=50)
+    # a quarter turn: a' = b, b' = -a exactly
+    ra, rb = oracle.rot(a, b, 0.0, 1.0)
+    assert np.array_equal(ra, b) and np.array_equal(rb, -a)
+    th = 0.3
+    ra, rb = oracle.rot(a, b, np.cos(th), np.sin(th))
+    assert np.allclose(np.hypot(ra, rb), np.hypot(a, b), rtol=1e-14)
+    # Rotm: the argument vector v is the first operand (CPUsimple_lvector.cc:540-600)
+    u, v = rng.integers(-9, 9, 20).astype(float), rng.integers(-9, 9, 20).astype(float)
+    H = np.array([[2.0, 3.0], [-1.0, 4.0]])    # [[h11, h12], [h21, h22]]
+    ru, rv = oracle.rotm(u, v, [-1.0, H[0, 0], H[1, 0], H[0, 1], H[1, 1]])
+    assert np.array_equal(rv, H[0, 0] * v + H[0, 1] * u) and np.array_equal(ru, H[1, 0] * v + H[1, 1] * u)
+    ru, rv = oracle.rotm(u, v, [0.0, 99.0, -1.0, 3.0, 99.0])
+    assert np.array_equal(rv, v + 3.0 * u) and np.array_equal(ru, -v + u)
+    ru, rv = oracle.rotm(u, v, [1.0, 2.0, 99.0, 99.0, 5.0])
+    assert np.array_equal(rv, 2.0 * v + u) and np.array_equal(ru, 5.0 * u - v)
+    ru, rv = oracle.rotm(u, v, [-2.0, 2.0, 2.0, 2.0, 2.0])
+    assert np.array_equal(ru, u) and np.array_equal(rv, v)
+    # Rotg: r = +-sqrt(a^2 + b^2) with the sign of the larger entry, c = a/r,
+    # s = b/r; z = s when |a| > |b|, else 1/c (1 when c = 0)
+    r, z, c, s = oracle.rotg(3.0, 4.0)
+    assert abs(r - 5.0) < 1e-15 and abs(c - 0.6) < 1e-15 and abs(s - 0.8) < 1e-15 and abs(z - 1 / c) < 1e-14
+    r, z, c, s = oracle.rotg(-4.0, 3.0)
+    assert abs(r + 5.0) < 1e-15 and abs(c - 0.8) < 1e-15 and abs(s + 0.6) < 1e-15 and z == s
+    assert oracle.rotg(0.0, 0.0) == (0.0, 0.0, 1.0, 0.0)
+    r, z, c, s = oracle.rotg(0.0, -2.0)
+    assert r == -2.0 and c == 0.0 and s == 1.0 and z == 1.0
+
+
+def test_matrix_value_ops(oracle, pkg):
+    A, _ = fv_matrix(pkg, oracle, 5, 4, 3, sigma=0.7, seed=2)
+    D = A.to_dense()
+    rows = [0, 7, 7, 59]
+    V = oracle.zero_rows(A, rows, 2.5)
+    Z = oracle.Csr(A.row, A.col, V).to_dense()
+    ref = D.copy()
+    ref[rows, :] = 0
+    ref[rows, rows] = 2.5
+    assert np.array_equal(Z, ref)
+    V = oracle.zero_cols(A, [3, 11], -1.0)
+    ref = D.copy()
+    ref[:, [3, 11]] = np.where(D[:, [3, 11]] != 0, 0.0, 0.0)
+    ref[3, 3] = ref[11, 11] = -1.0
+    assert np.array_equal(oracle.Csr(A.row, A.col, V).to_dense(), ref)
+    assert np.array_equal(oracle.Csr(A.row, A.col, oracle.mat_scale(A, 0.5)).to_dense(), D * 0.5)
+    off = D * 3.0
+    np.fill_diagonal(off, np.diag(D))
+    assert np.array_equal(oracle.Csr(A.row, A.col, oracle.mat_scale(A, 3.0, True)).to_dense(), off)
+    x = np.random.default_rng(2).normal(size=A.n)
+    assert np.array_equal(oracle.pjacobi(A, x), x / np.diag(D))
