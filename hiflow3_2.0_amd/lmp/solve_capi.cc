@@ -68,16 +68,18 @@ int run(int solver, int prec, int size_basis, double omega, int n, const int *ro
         cg->InitParameter(P ? "Preconditioning" : "NoPreconditioning");
         S.reset(cg);
     } else if (solver == HFL_SOLVER_GMRES || solver == HFL_SOLVER_FGMRES) {
+        // without a preconditioner the reference's NoPreconditioning path
+        // (gmres.cc:103-264) runs
+        const char *method = P ? "RightPreconditioning" : "NoPreconditioning";
         if (solver == HFL_SOLVER_GMRES) {
             GMRES<LAD> *g = new GMRES<LAD>();
-            g->InitParameter(size_basis, "RightPreconditioning");
+            g->InitParameter(size_basis, method);
             S.reset(g);
         } else {
             FGMRES<LAD> *g = new FGMRES<LAD>();
-            g->InitParameter(size_basis, "RightPreconditioning");
+            g->InitParameter(size_basis, method);
             S.reset(g);
         }
-        if (!P) P = &id;
     } else {
         throw Error(HFH_ERR_INVALID, "hfl_lmp_solve: unknown solver");
     }

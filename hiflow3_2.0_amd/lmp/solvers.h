@@ -245,8 +245,12 @@ class GMRESBase : public LinearSolver<LAD> {
         size_basis_ = size_basis;
         this->method_ = method;
     }
+    // GMRES::Solve (gmres.cc:80-98): dispatch on the method; left
+    // preconditioning is not implemented in the reference either
+    // (gmres.cc:267-279, fgmres.cc:59-70) and returns kSolverError
     LinearSolverState Solve(const VectorType &b, VectorType *x) override {
         if (size_basis_ <= 0) return kSolverError;
+        if (this->method_ == "NoPreconditioning") return SolveNoPrecond(b, x);
         if (this->method_ != "RightPreconditioning" || !this->precond_) return kSolverError;
         return SolveRight(b, x);
     }
@@ -261,6 +265,65 @@ class GMRESBase : public LinearSolver<LAD> {
             for (int j = 0; j < i; ++j) y[j] -= H[i][j] * t;
         }
         for (int j = 0; j <= k; ++j) x->Axpy(*V[j], y[j]);
+    }
+
+    // GMRES::SolveNoPrecond (gmres.cc:103-264): the Arnoldi process on A
+    // itself, x += V y at convergence and at every restart (FGMRES uses the
+    // same path without a preconditioner)
+    LinearSolverState SolveNoPrecond(const VectorType &b, VectorType *x) {
+        IterateControl::State conv;
+        const int m = std::min(size_basis_, this->control_.maxits());
+        std::vector<std::vector<DataType>> H(m, std::vector<DataType>(m + 1, DataType(0)));
+        std::vector<std::unique_ptr<VectorType>> V(m + 1);
+        for (int i = 0; i <= m; ++i) V[i].reset(new VectorType());
+        VectorType w;
+        w.CloneFromWithoutContent(b);
+        std::vector<DataType> g(m + 1), cs(m + 1), sn(m + 1);
+        int iter = 0;
+        this->hist_.clear();
+        V[0]->CloneFromWithoutContent(b);
+        this->op_->VectorMult(*x, V[0].get());
+        V[0]->ScaleAdd(b, static_cast<DataType>(-1.));
+        this->res_ = V[0]->Norm2();
+        this->hist_.push_back(this->res_);
+        conv = this->control_.Check(iter, this->res_);
+        while (conv == IterateControl::kIterate) {
+            std::fill(g.begin(), g.end(), DataType(0));
+            for (auto &h : H) std::fill(h.begin(), h.end(), DataType(0));
+            V[0]->Scale(static_cast<DataType>(1.) / static_cast<DataType>(this->res_));
+            g[0] = static_cast<DataType>(this->res_);
+            for (int j = 0; j != m; ++j) {
+                ++iter;
+                this->op_->VectorMult(*V[j], &w);
+                for (int i = 0; i <= j; ++i) {
+                    H[j][i] = w.Dot(*V[i]);
+                    w.Axpy(*V[i], static_cast<DataType>(-1.) * H[j][i]);
+                }
+                H[j][j + 1] = w.Norm2();
+                if (H[j][j + 1] == DataType(0)) return kSolverError;
+                w.Scale(static_cast<DataType>(1.) / H[j][j + 1]);
+                V[j + 1]->CloneFrom(w);
+                for (int k = 0; k < j; ++k) apply_plane_rotation(cs[k], sn[k], &H[j][k], &H[j][k + 1]);
+                generate_plane_rotation(H[j][j], H[j][j + 1], &cs[j], &sn[j]);
+                apply_plane_rotation(cs[j], sn[j], &H[j][j], &H[j][j + 1]);
+                apply_plane_rotation(cs[j], sn[j], &g[j], &g[j + 1]);
+                this->res_ = std::abs(g[j + 1]);
+                this->hist_.push_back(this->res_);
+                conv = this->control_.Check(iter, this->res_);
+                if (conv != IterateControl::kIterate) {
+                    UpdateSolution(V, H, g, j, x);
+                    break;
+                }
+            }
+            if (conv == IterateControl::kIterate) {
+                UpdateSolution(V, H, g, m - 1, x);
+                this->op_->VectorMult(*x, V[0].get());
+                V[0]->ScaleAdd(b, static_cast<DataType>(-1.));
+                // the restart normalises with res = |g[m]| (gmres.cc:163)
+            }
+        }
+        this->iter_ = iter;
+        return this->state_of(conv);
     }
 
     LinearSolverState SolveRight(const VectorType &b, VectorType *x) {

@@ -564,7 +564,9 @@ static void FN(update_)(int n, T **V, const T *H, int m1, const T *g, int k, T *
 }
 
 /* GMRES<LAD>::SolveRight (gmres.cc:285-477) when flexible == 0, and
- * FGMRES<LAD>::SolveRight (fgmres.cc:75-262) when flexible == 1. */
+ * FGMRES<LAD>::SolveRight (fgmres.cc:75-262) when flexible == 1; without a
+ * preconditioner (P NULL or kind NONE) GMRES<LAD>::SolveNoPrecond
+ * (gmres.cc:103-264) for both: w = A v_j, and x += V y directly. */
 static int FN(gmres_core_)(int n, const int *row, const int *col, const T *val, const T *b,
                            T *x, const PREC *P, int size_basis, const hfo_ctrl *ctrl,
                            hfo_result *out, double *hist, int flexible)
@@ -588,6 +590,7 @@ static int FN(gmres_core_)(int n, const int *row, const int *col, const T *val, 
     double first = 0;
     T res;
     int iter = 0, st;
+    const int noprec = !P || P->kind == HFO_PREC_NONE;
 
     FN(hfo_spmv_)(n, row, col, val, x, V[0]);
     FN(hfo_scaleadd_)(n, V[0], (T)-1., b);
@@ -603,10 +606,14 @@ static int FN(gmres_core_)(int n, const int *row, const int *col, const T *val, 
         g[0] = res;
         for (int j = 0; j != m; ++j) {
             ++iter;
-            T *zj = flexible ? Z[j] : z;
-            memset(zj, 0, sizeof(T) * n);
-            FN(hfo_prec_apply_)(n, P, V[j], zj);
-            FN(hfo_spmv_)(n, row, col, val, zj, w);
+            if (noprec) {
+                FN(hfo_spmv_)(n, row, col, val, V[j], w);
+            } else {
+                T *zj = flexible ? Z[j] : z;
+                memset(zj, 0, sizeof(T) * n);
+                FN(hfo_prec_apply_)(n, P, V[j], zj);
+                FN(hfo_spmv_)(n, row, col, val, zj, w);
+            }
             for (int i = 0; i <= j; ++i) {
                 H[j * m1 + i] = FN(hfo_dot_)(n, w, V[i]);
                 FN(hfo_axpy_)(n, w, V[i], (T)-1. * H[j * m1 + i]);
@@ -624,7 +631,9 @@ static int FN(gmres_core_)(int n, const int *row, const int *col, const T *val, 
                 hist[iter] = res;
             st = FN(check_)(ctrl, iter, res, &first);
             if (st != HFO_ITERATE) {
-                if (flexible) {
+                if (noprec) {
+                    FN(update_)(n, V, H, m1, g, j, x);
+                } else if (flexible) {
                     FN(update_)(n, Z, H, m1, g, j, x);
                 } else {
                     memset(z, 0, sizeof(T) * n);
@@ -637,7 +646,11 @@ static int FN(gmres_core_)(int n, const int *row, const int *col, const T *val, 
             }
         }
         if (st == HFO_ITERATE) {
-            if (flexible) {
+            if (noprec) {
+                FN(update_)(n, V, H, m1, g, m - 1, x);
+                FN(hfo_spmv_)(n, row, col, val, x, V[0]);
+                FN(hfo_scaleadd_)(n, V[0], (T)-1., b);
+            } else if (flexible) {
                 FN(update_)(n, Z, H, m1, g, m - 1, x);
                 FN(hfo_spmv_)(n, row, col, val, x, V[0]);
                 FN(hfo_scale_)(n, V[0], (T)-1.);

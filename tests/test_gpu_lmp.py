@@ -66,6 +66,25 @@ def test_lmp_cg_float_descriptor(pkg, oracle):
     assert relnorm(x, xo) < 1e-4
 
 
+@pytest.mark.parametrize("solver", ["gmres", "fgmres"])
+def test_lmp_gmres_no_preconditioning(pkg, oracle, solver):
+    # GMRES::SolveNoPrecond (gmres.cc:103-264), which FGMRES also runs
+    # without a preconditioner; restarts of basis 10
+    row, col, val, rhs = pkg.problem(pkg.UPWIND_CONVDIFF, 8, p0=1e-2, p1=5.0, seed=4)
+    code = pkg.SOLVER_GMRES if solver == "gmres" else pkg.SOLVER_FGMRES
+    x, info = pkg.lmp_solve(row, col, val, rhs, solver=code, precond=pkg.PREC_NONE, m=10, maxits=3000, atol=0.0,
+                            rtol=1e-8)
+    OA = oracle.Csr(row, col, val)
+    fn = oracle.gmres if solver == "gmres" else oracle.fgmres
+    xo, oi = fn(OA, rhs, m=10, ctrl=(3000, 0.0, 1e-8, 1e6))
+    assert info["ret"] == 0 and oi["ret"] == 0 and info["iters"] > 10
+    assert abs(info["iters"] - oi["iters"]) <= 2
+    k = min(10, len(info["hist"]), len(oi["hist"]))
+    assert np.allclose(info["hist"][:k], oi["hist"][:k], rtol=1e-8)
+    assert true_res(row, col, val, x, rhs) < 2e-8
+    assert relnorm(x, xo) < 1e-6
+
+
 def test_lmp_mpir(pkg, oracle):
     row, col, val, rhs = pkg.problem(pkg.FV_DIFFUSION, 12, p0=1.0, seed=5)
     nb = np.linalg.norm(rhs)

@@ -362,6 +362,26 @@ def test_gmres_minimises_residual(oracle, pkg):
     assert relnorm(xf, x) < 1e-8
 
 
+def test_gmres_no_preconditioning(oracle, pkg):
+    # GMRES::SolveNoPrecond (gmres.cc:103-264): the minimal residual over the
+    # Krylov space of A itself; restarts reach the tolerance; the 3x3 answer
+    row, col, val, rhs = pkg.problem(pkg.UPWIND_CONVDIFF, 5, p0=1e-2, p1=5.0, seed=3)
+    A = oracle.Csr(row, col, val)
+    x, info = oracle.gmres(A, rhs, m=30, ctrl=(30, 0.0, 1e-12, 1e6))
+    k = min(12, len(info["hist"]) - 1)
+    ref = _krylov_min_residuals(A.to_dense(), np.eye(A.n), rhs, k)
+    got = info["hist"][1:k + 1]
+    assert np.all(np.abs(got - ref) <= 1e-6 * ref + 1e-12 * np.linalg.norm(rhs))
+    for flex in (False, True):
+        fn = oracle.fgmres if flex else oracle.gmres
+        x, info = fn(A, rhs, m=7, ctrl=(5000, 0.0, 1e-9, 1e6))
+        assert info["ret"] == 0 and info["iters"] > 7
+        assert np.linalg.norm(rhs - A.to_dense() @ x) < 2e-9 * np.linalg.norm(rhs)
+    T = tridiag_coo(oracle, 3, full=True)
+    x, info = oracle.gmres(T, np.array([1.0, -2.0, 0.0]), m=3, ctrl=(100, 1e-15, 1e-12, 1e6))
+    assert np.allclose(x, [-0.25, -1.5, -0.75], atol=1e-12)
+
+
 def test_gmres_restart(oracle, pkg):
     row, col, val, rhs = pkg.problem(pkg.UPWIND_CONVDIFF, 6, p0=1e-2, p1=5.0, seed=2)
     A = oracle.Csr(row, col, val)
