@@ -25,7 +25,18 @@ def test_norm1(pkg, ctx, oracle, dt, n):
     x = np.random.default_rng(n).normal(size=n).astype(dt)
     got = pkg.Vector(ctx, x, dt).norm1()
     ref = oracle.norm1(x)
-    assert abs(got - ref) <= (1e-14 if dt == np.float64 else 1e-6) * abs(ref)
+    if dt == np.float64:
+        assert abs(got - ref) <= 1e-14 * abs(ref)
+    else:
+        # the device sums FP32 magnitudes in FP64 partials and rounds the
+        # total to FP32 once (the reference returns T); the reference sums
+        # them in FP32, one add after another, whose error grows to about
+        # sqrt(n) * 2^-24 (1.6e-6 relative at 2^20 entries): both are
+        # checked against the exact sum, the device within one FP32
+        # rounding, the oracle within the sequential bound n * 2^-24
+        exact = float(np.sum(np.abs(x.astype(np.float64))))
+        assert abs(got - exact) <= (2.0 ** -24 + 1e-12) * exact
+        assert abs(ref - exact) <= max(n, 1) * 2.0 ** -24 * exact
 
 
 @pytest.mark.parametrize("dt", DTYPES)
