@@ -174,6 +174,13 @@ struct hfh_prec {
     hipGraph_t sweep_graph = nullptr;
     hipGraphExec_t sweep_exec = nullptr;
     bool graph_failed = false;
+    // SGS / ILU0 sweep schedule: sync-free (one launch per sweep) or one
+    // launch per level; the sync-free form keeps the forward results in sf_y
+    // and its ticket counters and failure word in sf_ctl[0..2]
+    int sweep_mode = HFH_SWEEP_SYNCFREE;
+    int sf_waves = 0;                  // sync-free: waves per CU (0: from the level widths)
+    void *sf_y = nullptr;
+    int *sf_ctl = nullptr;
 };
 
 namespace hfh {
@@ -199,6 +206,9 @@ inline int hip_fail(hipError_t e, const char *what) {
 
 int dev_alloc(void **p, size_t bytes);
 int launch_check(const char *what);
+// HFH_ERR_HIP (and the failure word cleared) when a sync-free sweep of P gave
+// up waiting since the last check; HFH_OK otherwise (also for P = NULL).
+int sweep_failed(hfh_prec *P);
 
 // Reductions (hfh_blas1.hip).  Results land in ctx->red + slot (device).
 int dot_async(hfh_ctx *ctx, int dtype, int n, const void *x, const void *y, double *dst);
@@ -296,6 +306,12 @@ template <typename T>
 __device__ __forceinline__ T wave_sum(T v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
     return v;
 }
 

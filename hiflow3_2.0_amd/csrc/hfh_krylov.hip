@@ -680,6 +680,7 @@ extern "C" int hfh_pcg_solve(const hfh_csr *A, hfh_prec *P, const hfh_vec *b, hf
         if ((rc = W.init(A, P, ctl->maxits))) return rc;
         if ((rc = W.solve((const float *)b->d, (float *)x->d, *ctl, &inf, hist, true))) return rc;
     }
+    if ((rc = sweep_failed(P))) return rc;
     if (info) *info = inf;
     return solve_status(inf.state);
 }
@@ -975,6 +976,7 @@ extern "C" int hfh_gmres_solve(const hfh_csr *A, hfh_prec *P, int size_basis, in
     else
         rc = gmres_impl<float>(A, P, size_basis, flexible, (const float *)b->d, (float *)x->d, *ctl, &inf, hist);
     if (rc) return rc;
+    if ((rc = sweep_failed(P))) return rc;
     if (info) *info = inf;
     return solve_status(inf.state);
 }

@@ -360,6 +360,159 @@ __global__ __launch_bounds__(kMergeBlock) void k_tsweep_run(const int *__restric
     }
 }
 
+// ---------------------------------------------------------------------------
+// Sync-free triangular sweeps: one launch per sweep instead of one per level.
+// Waves take slices in level order from a ticket counter; a lane needing the
+// result of an earlier row polls that result itself.  Every result starts as
+// a signalling-NaN sentinel (k_sf_fill, before the forward sweep).
+// Arithmetic never yields a signalling NaN, so a value other than the
+// sentinel is final.  Results are stored and polled with agent-scope relaxed
+// atomics: coherent across the XCDs' L2 caches, with no fence.  A wave waits
+// only on slices with smaller tickets, whose waves are already running or
+// done, so the smallest unfinished slice always progresses.  Every wait is
+// bounded: after kSfSpins polls, or once any lane has given up, a lane
+// raises the failure word and goes on with the sentinel (the NaN then spreads
+// through the results), so every wave ends and the drivers report the
+// failure.  Each row's arithmetic is tsweep_slice's: results are
+// bit-identical to the level launches.
+template <typename T> struct Sent;
+template <> struct Sent<double> {
+    typedef unsigned long long U;
+    static constexpr U bits = 0x7FF4DEAD5EEDBEEFull;   // exponent all ones, quiet bit clear
+    static constexpr U quiet = 1ull << 51;
+};
+template <> struct Sent<float> {
+    typedef unsigned int U;
+    static constexpr U bits = 0x7FA5EED5u;
+    static constexpr U quiet = 1u << 22;
+};
+constexpr int kSfSpins = 1 << 20;
+
+template <typename T>
+__device__ __forceinline__ typename Sent<T>::U sf_load(const T *p) {
+    return __hip_atomic_load(reinterpret_cast<const typename Sent<T>::U *>(p), __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Stores a final result.  Only a copied input that holds the sentinel's own
+// bits could equal it (arithmetic quiets every NaN); it is stored quieted.
+template <typename T>
+__device__ __forceinline__ void sf_store(T *p, T v) {
+    typedef typename Sent<T>::U U;
+    U u = __builtin_bit_cast(U, v);
+    if (u == Sent<T>::bits) u |= Sent<T>::quiet;
+    __hip_atomic_store(reinterpret_cast<U *>(p), u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Waits until none of the first m of w[] (results at out + c[]) is the
+// sentinel, polling the pending ones together (bounded, see above); `most`
+// keeps the largest poll count.
+template <typename T, int U>
+__device__ __forceinline__ void sf_wait_all(const T *out, const int *c, typename Sent<T>::U *w, int m, int *fail,
+                                            int &most) {
+    auto pending = [&]() {
+        bool p = false;
+#pragma unroll
+        for (int u = 0; u < U; ++u) p |= u < m && w[u] == Sent<T>::bits;
+        return p;
+    };
+    int spins = 0;
+    while (pending()) {
+        ++spins;
+        if ((spins & 255) == 0 &&
+            (spins >= kSfSpins || __hip_atomic_load(fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+            __hip_atomic_store(fail, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (u < m && w[u] == Sent<T>::bits) w[u] = sf_load(out + c[u]);
+    }
+    most = max(most, spins);
+}
+
+// y and z to the sentinel, the two ticket counters to 0 (ctl[0], ctl[1]).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_sf_fill(int n, T *y, T *z, int *ctl, const int *__restrict__ done) {
+    if (done && *done) return;
+    typedef typename Sent<T>::U U;
+    for (long long i = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x; i < n;
+         i += static_cast<long long>(gridDim.x) * kBlock) {
+        reinterpret_cast<U *>(y)[i] = Sent<T>::bits;
+        reinterpret_cast<U *>(z)[i] = Sent<T>::bits;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ctl[0] = 0;
+        ctl[1] = 0;
+    }
+}
+
+// One slice of a sync-free sweep; MODE as tsweep_slice.  Forward (0, 2):
+// out = y, read from `in`.  Backward (1, 3): out = z, y_i from `yin`.
+template <typename T, int MODE>
+__device__ __forceinline__ void sf_slice(int s, int lane, const int *__restrict__ ptr, const int *__restrict__ srow,
+                                         const int *__restrict__ slen, const int *__restrict__ scol,
+                                         const T *__restrict__ sval, const T *__restrict__ dval,
+                                         const T *__restrict__ in, const T *__restrict__ yin, T *out, int *fail,
+                                         int &most) {
+    typedef typename Sent<T>::U U;
+    const int p = s * 64 + lane;
+    const int base = ptr[s], L = (ptr[s + 1] - base) >> 6;
+    const int i = srow[p];
+    if (i < 0) return;   // padding lane of a level's last slice
+    const int len = slen[p];
+    const T d = dval[p];
+    T acc;
+    if (MODE == 0 || MODE == 2) acc = in[i];
+    else if (MODE == 1) acc = mul_rn(yin[i], d);
+    else acc = yin[i];
+    for (int k = 0; k < L; k += kTsweepU) {
+        int c[kTsweepU];
+        T v[kTsweepU];
+        U w[kTsweepU];
+#pragma unroll
+        for (int u = 0; u < kTsweepU; ++u) {
+            const int idx = base + min(k + u, L - 1) * 64 + lane;
+            c[u] = scol[idx];
+            v[u] = sval[idx];
+        }
+#pragma unroll
+        for (int u = 0; u < kTsweepU; ++u) w[u] = k + u < len ? sf_load(out + c[u]) : U(0);
+        sf_wait_all<T, kTsweepU>(out, c, w, len - k, fail, most);
+#pragma unroll
+        for (int u = 0; u < kTsweepU; ++u)
+            if (k + u < len) acc = add_rn(acc, -mul_rn(__builtin_bit_cast(T, w[u]), v[u]));
+    }
+    if (MODE != 2) acc = acc / d;
+    sf_store(out + i, acc);
+}
+
+// A whole sweep: each wave takes one slice per ticket until every slice is
+// taken.  ctl: [0] forward ticket, [1] backward ticket,
+// [2] failure word, [3] the most polls any lane needed for one result.
+template <typename T, int MODE>
+__global__ __launch_bounds__(kBlock) void k_tsweep_sf(int nslice, const int *__restrict__ ptr,
+                                                      const int *__restrict__ srow, const int *__restrict__ slen,
+                                                      const int *__restrict__ scol, const T *__restrict__ sval,
+                                                      const T *__restrict__ dval, const T *__restrict__ in,
+                                                      const T *__restrict__ yin, T *out, int *ctl,
+                                                      const int *__restrict__ done) {
+    if (done && *done) return;
+    const int lane = threadIdx.x & 63;
+    int *ticket = ctl + ((MODE == 1 || MODE == 3) ? 1 : 0);
+    int most = 0;
+    for (;;) {
+        int t = 0;
+        if (lane == 0) t = __hip_atomic_fetch_add(ticket, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        t = __builtin_amdgcn_readfirstlane(t);
+        if (t >= nslice) break;
+        sf_slice<T, MODE>(t, lane, ptr, srow, slen, scol, sval, dval, in, yin, out, ctl + 2, most);
+    }
+    most = wave_max(most);
+    if (lane == 0 && most > 0) __hip_atomic_fetch_max(ctl + 3, most, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // ILU(0) of row i in place (CPU_CSR_lMatrix::ilu0, lmatrix_csr_cpu.cc:1880-1930,
 // IKJ): for each lower entry a_ik (k < i, ascending) with a stored diagonal
 // a_kk: a_ik /= a_kk, then a_ij -= a_ik a_kj for every later entry j of row i
@@ -546,9 +699,50 @@ static void sweep(const hfh_levels &Lv, const hfh_tsweep &W, const T *in, T *out
     }
 }
 
+// Sync-free form of both sweeps: the fill, the forward sweep into P->sf_y,
+// the backward sweep into out.
+template <typename T, int FW, int BW>
+static void sf_sweeps(hfh_prec *P, const T *in, T *out, const int *done) {
+    hipStream_t s = P->ctx->stream;
+    T *y = static_cast<T *>(P->sf_y);
+    const int nfill = std::max(1, std::min((P->n + kBlock - 1) / kBlock, std::max(P->ctx->num_cu, 1) * 8));
+    hipLaunchKernelGGL(k_sf_fill<T>, dim3(nfill), dim3(kBlock), 0, s, P->n, y, out, P->sf_ctl, done);
+    // Waiting waves poll, and every poll is a memory round trip: more waves
+    // than the levels keep busy slow all of them (upwind 128^3 ILU(0) apply:
+    // 1.40 ms at one wave per CU, 1.85 ms at 4, 2.39 ms at 6, and at 32 the
+    // polls flood the memory system).  Automatic (sf_waves = 0): about three
+    // levels' slices in flight, between 1 and 8 waves per CU.
+    const int ncu = std::max(P->ctx->num_cu, 1);
+    auto grid = [&](const hfh_tsweep &W) {
+        int waves;
+        if (P->sf_waves > 0) {
+            waves = ncu * P->sf_waves;
+        } else {
+            const int nlev = std::max<int>(1, static_cast<int>(W.lslice.size()) - 1);
+            waves = std::min(ncu * 8, std::max(ncu, 3 * ((W.nslice + nlev - 1) / nlev)));
+        }
+        waves = std::min(waves, W.nslice);
+        return std::max(1, (waves + kBlock / 64 - 1) / (kBlock / 64));
+    };
+    const hfh_tsweep &L = P->tlo, &U = P->tup;
+    if (L.nslice > 0)
+        hipLaunchKernelGGL((k_tsweep_sf<T, FW>), dim3(grid(L)), dim3(kBlock), 0, s, L.nslice, L.ptr, L.row,
+                           L.len, L.col, (const T *)L.val, (const T *)L.dval, in, (const T *)nullptr, y, P->sf_ctl,
+                           done);
+    if (U.nslice > 0)
+        hipLaunchKernelGGL((k_tsweep_sf<T, BW>), dim3(grid(U)), dim3(kBlock), 0, s, U.nslice, U.ptr, U.row,
+                           U.len, U.col, (const T *)U.val, (const T *)U.dval, (const T *)nullptr, (const T *)y, out,
+                           P->sf_ctl, done);
+}
+
 template <typename T>
 static int trsv_sweeps(hfh_prec *P, const T *in, T *out, const int *done) {
     hipStream_t s = P->ctx->stream;
+    if (P->sweep_mode == HFH_SWEEP_SYNCFREE && P->sf_y && in != out) {
+        if (P->kind == HFH_PREC_ILU0) sf_sweeps<T, 2, 3>(P, in, out, done);
+        else sf_sweeps<T, 0, 1>(P, in, out, done);
+        return launch_check("k_tsweep_sf");
+    }
     if (P->kind == HFH_PREC_ILU0) {
         sweep<T, 2>(P->lo, P->tlo, in, out, done, s);
         sweep<T, 3>(P->up, P->tup, in, out, done, s);
@@ -896,6 +1090,8 @@ static void free_sweep(hfh_sweep &W) {
 template <typename T>
 static int trsv_graph(hfh_prec *P, const T *in, T *out) {
     hipStream_t s = P->ctx->stream;
+    // three launches already: no graph
+    if (P->sweep_mode == HFH_SWEEP_SYNCFREE && P->sf_y && in != out) return trsv_sweeps<T>(P, in, out, nullptr);
     const size_t bytes = sizeof(T) * static_cast<size_t>(P->n);
     if (!P->sweep_exec && !P->graph_failed) {
         int rc = HFH_OK;
@@ -1064,6 +1260,13 @@ static int create_trsv(const hfh_csr *A, int kind, int nlev, const int *lp, cons
     if (!rc) rc = validate_levels(A, P->up, true, what);
     if (!rc) rc = build_tsweep(A, P->lo, lr, true, P->tlo);
     if (!rc) rc = build_tsweep(A, P->up, ur, false, P->tup);
+    // sync-free sweeps: forward results and the ticket / failure words
+    if (!rc) rc = dev_alloc(&P->sf_y, A->elem() * static_cast<size_t>(std::max(A->nrow, 1)));
+    if (!rc) rc = dev_alloc(reinterpret_cast<void **>(&P->sf_ctl), sizeof(int) * 4);
+    if (!rc) {
+        const hipError_t e = hipMemsetAsync(P->sf_ctl, 0, sizeof(int) * 4, A->ctx->stream);
+        if (e != hipSuccess) rc = hip_fail(e, what);
+    }
     if (!rc) {
         P->tversion = A->version;
         const hipError_t e = hipStreamSynchronize(A->ctx->stream);
@@ -1092,6 +1295,38 @@ extern "C" int hfh_prec_apply(hfh_prec *P, const hfh_vec *in, hfh_vec *out) {
     return prec_apply_raw(P, in->d, out->d);
 }
 
+extern "C" int hfh_prec_set_sweep(hfh_prec *P, int mode, int waves_per_cu) {
+    HFH_CHECK(P, HFH_ERR_INVALID, "hfh_prec_set_sweep: null preconditioner");
+    HFH_CHECK(mode == HFH_SWEEP_SYNCFREE || mode == HFH_SWEEP_LEVELS, HFH_ERR_INVALID,
+              "hfh_prec_set_sweep: mode %d is neither HFH_SWEEP_SYNCFREE nor HFH_SWEEP_LEVELS", mode);
+    HFH_CHECK(waves_per_cu >= 0 && waves_per_cu <= 32, HFH_ERR_INVALID,
+              "hfh_prec_set_sweep: waves_per_cu %d not 0 (default) or in [1, 32]", waves_per_cu);
+    P->sweep_mode = mode;
+    P->sf_waves = waves_per_cu;
+    return HFH_OK;
+}
+
+int hfh::sweep_failed(hfh_prec *P) {
+    if (!P || !P->sf_ctl) return HFH_OK;
+    hipStream_t s = P->ctx->stream;
+    int *h = reinterpret_cast<int *>(P->ctx->host_res);
+    HFH_HIP(hipMemcpyAsync(h, P->sf_ctl + 2, sizeof(int), hipMemcpyDeviceToHost, s));
+    HFH_HIP(hipStreamSynchronize(s));
+    if (*h == 0) return HFH_OK;
+    HFH_HIP(hipMemcpyAsync(h, P->sf_ctl + 3, sizeof(int), hipMemcpyDeviceToHost, s));
+    HFH_HIP(hipStreamSynchronize(s));
+    const int most = *h;
+    HFH_HIP(hipMemsetAsync(P->sf_ctl + 2, 0, 2 * sizeof(int), s));
+    set_error("sync-free triangular sweep: a row waited longer than its bound for an earlier row's result "
+              "(%d polls; results are NaN)", most);
+    return HFH_ERR_HIP;
+}
+
+extern "C" int hfh_prec_sweep_status(hfh_prec *P) {
+    HFH_CHECK(P, HFH_ERR_INVALID, "hfh_prec_sweep_status: null preconditioner");
+    return sweep_failed(P);
+}
+
 extern "C" int hfh_prec_kind(const hfh_prec *P, int *kind) {
     HFH_CHECK(P && kind, HFH_ERR_INVALID, "hfh_prec_kind: null argument");
     *kind = P->kind;
@@ -1117,6 +1352,8 @@ extern "C" int hfh_prec_destroy(hfh_prec *P) {
     if (P->sweep_graph) (void)hipGraphDestroy(P->sweep_graph);
     if (P->gin) (void)hipFree(P->gin);
     if (P->gout) (void)hipFree(P->gout);
+    if (P->sf_y) (void)hipFree(P->sf_y);
+    if (P->sf_ctl) (void)hipFree(P->sf_ctl);
     delete P;
     return HFH_OK;
 }

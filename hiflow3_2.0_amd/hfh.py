@@ -21,6 +21,7 @@ OK, ERR_INVALID, ERR_NOMEM, ERR_HIP, ERR_ZERO_PIVOT, ERR_NOT_CONVERGED, ERR_BUDG
     0, -1, -2, -3, -4, -5, -6, -7
 
 Q1_2D, Q1_3D, FV_DIFFUSION, UPWIND_CONVDIFF = 1, 2, 3, 4
+SWEEP_SYNCFREE, SWEEP_LEVELS = 0, 1
 SPMV_AUTO, SPMV_STREAM, SPMV_VECTOR, SPMV_WAVE, SPMV_SELL, SPMV_SELL_DICT, SPMV_SELL_PAIRS, SPMV_SELL_LDS = \
     0, 1, 2, 3, 4, 5, 6, 7
 
@@ -131,6 +132,8 @@ HIP_API = {
     "hfh_csr_ilu0": (_i, [_vp, _i, _vp, _vp, _P(_i)]),
     "hfh_prec_apply": (_i, [_vp, _vp, _vp]),
     "hfh_prec_kind": (_i, [_vp, _P(_i)]),
+    "hfh_prec_set_sweep": (_i, [_vp, _i, _i]),
+    "hfh_prec_sweep_status": (_i, [_vp]),
     "hfh_prec_destroy": (_i, [_vp]),
     "hfh_pcg_solve": (_i, [_vp, _vp, _vp, _vp, _P(Control), _P(SolveInfo), _vp]),
     "hfh_gmres_solve": (_i, [_vp, _vp, _i, _i, _vp, _vp, _P(Control), _P(SolveInfo), _vp]),
@@ -629,6 +632,15 @@ class Preconditioner:
 
     def apply(self, x, y):
         _ck(hip().hfh_prec_apply(self.h, x.h, y.h))
+
+    def set_sweep(self, mode, waves_per_cu=0):
+        """SGS / ILU(0): SWEEP_SYNCFREE (default) or SWEEP_LEVELS schedule;
+        waves_per_cu of a sync-free sweep (0: default)."""
+        _ck(hip().hfh_prec_set_sweep(self.h, mode, waves_per_cu))
+
+    def sweep_status(self):
+        """Raises if a sync-free sweep gave up waiting since the last check."""
+        _ck(hip().hfh_prec_sweep_status(self.h))
 
 
 def _info(inf):

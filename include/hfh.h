@@ -288,6 +288,21 @@ int hfh_prec_apply(hfh_prec *P, const hfh_vec *in, hfh_vec *out);
 int hfh_prec_kind(const hfh_prec *P, int *kind);
 int hfh_prec_destroy(hfh_prec *P);
 
+/* Schedule of the SGS / ILU(0) triangular sweeps (no reference counterpart;
+ * the reference sweeps row by row on the CPU).  HFH_SWEEP_SYNCFREE (the
+ * default): one launch per sweep; waves take slices of rows in level order
+ * from a counter and each row waits for the earlier results it needs
+ * (bounded waits).  HFH_SWEEP_LEVELS: one launch per level (runs of small
+ * levels merged).  Both give bit-identical results.  waves_per_cu: waves of
+ * a sync-free sweep per compute unit (1..32); 0 (the default) sizes it to
+ * about three levels' slices, between 1 and 8 per compute unit. */
+enum { HFH_SWEEP_SYNCFREE = 0, HFH_SWEEP_LEVELS = 1 };
+int hfh_prec_set_sweep(hfh_prec *P, int mode, int waves_per_cu);
+/* HFH_ERR_HIP when a sync-free sweep of P gave up waiting since the last
+ * call (its results are then NaN); clears the condition.  Synchronises the
+ * context's stream.  The drivers check it at the end of each solve. */
+int hfh_prec_sweep_status(hfh_prec *P);
+
 /* ---- device-resident Krylov drivers ---- */
 typedef struct {
     int maxits;
