@@ -304,6 +304,16 @@ extern "C" int hfh_ctx_create(int device, hfh_ctx **out) {
     HFH_HIP(hipHostMalloc(reinterpret_cast<void **>(&c->host_res), sizeof(double) * 256));
     HFH_HIP(hipEventCreate(&c->ev0));
     HFH_HIP(hipEventCreate(&c->ev1));
+    // the allocation-free CG's static array on this device (current now)
+    if ((rc = static_scratch(device, &c->scratch))) {
+        (void)hipStreamDestroy(c->stream);
+        (void)hipFree(c->red);
+        (void)hipHostFree(c->host_res);
+        (void)hipEventDestroy(c->ev0);
+        (void)hipEventDestroy(c->ev1);
+        delete c;
+        return rc;
+    }
     *out = c;
     return HFH_OK;
 }

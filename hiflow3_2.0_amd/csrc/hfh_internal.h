@@ -26,6 +26,9 @@ constexpr int kBlock = 256;          // threads per workgroup (4 wave64)
 constexpr int kSpmvTile = 1024;      // nnz staged per workgroup by the streaming SpMV
 constexpr int kRedBlocks = 1024;     // max partials per reduction
 constexpr int kNumXcd = 8;           // MI355X: 8 accelerator complex dies
+// Size of the per-device static array that the allocation-free CG
+// (hfh_pcg_solve_noalloc) works in: FP64 systems up to 2,096,000 rows.
+constexpr size_t kStaticScratch = static_cast<size_t>(64) << 20;
 
 struct Timing {
     bool on = false;
@@ -47,6 +50,8 @@ struct hfh_ctx {
     hfh::Timing timing;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int live = 0;                     // vectors, matrices and preconditioners created on the context
+    // device address of this device's static scratch array (hfh_krylov.hip)
+    void *scratch = nullptr;
 };
 
 struct hfh_vec {
@@ -206,6 +211,9 @@ inline int hip_fail(hipError_t e, const char *what) {
 
 int dev_alloc(void **p, size_t bytes);
 int launch_check(const char *what);
+// Device address of the static scratch array on `device` (current), looked
+// up once per device (hfh_krylov.hip).
+int static_scratch(int device, void **ptr);
 // HFH_ERR_HIP (and the failure word cleared) when a sync-free sweep of P gave
 // up waiting since the last check; HFH_OK otherwise (also for P = NULL).
 int sweep_failed(hfh_prec *P);

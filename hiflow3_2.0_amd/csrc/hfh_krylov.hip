@@ -23,8 +23,10 @@
 #include "hfh_internal.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 
 namespace hfh {
 
@@ -310,6 +312,14 @@ __global__ __launch_bounds__(kBlock) void k_pcg_p(int n, long long nv, T *__rest
     }
 }
 
+// Writes the initial solver state and hist[0] from the kernel arguments
+// (the allocation-free solve has no pinned staging of its own).
+__global__ void k_set_state(KState *st, KState v, double *hist, double res0) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    *st = v;
+    if (hist) hist[0] = res0;
+}
+
 // Workspace of one solve.
 struct Work {
     hfh_ctx *ctx = nullptr;
@@ -420,10 +430,10 @@ struct PcgRun {
 // timing on, each SpMV is bracketed by HIP events on the context stream and
 // the launches that did work are added to timing class 0.
 template <typename T>
-static int pcg_loop(PcgRun<T> &R, int maxits, bool use_graph) {
+static int pcg_loop(PcgRun<T> &R, int maxits, bool use_graph, bool allow_timing = true) {
     hfh_ctx *c = R.A->ctx;
     hipStream_t s = c->stream;
-    const bool timing = c->timing.on;
+    const bool timing = c->timing.on && allow_timing;
     const int chunk = std::max(1, std::min(16, maxits));
     hipGraphExec_t exec = nullptr;
     int rc = HFH_OK;
@@ -514,6 +524,45 @@ struct PcgWork {
         if (R.exec) (void)hipGraphExecDestroy(R.exec);
     }
 
+    // Buffers carved from `base` (cap bytes, 256-byte aligned) instead of
+    // allocated: r, p, z, Ap, the partials, the state and (with_hist) the
+    // residual history.  Jacobi or no preconditioner only.  Returns
+    // HFH_ERR_INVALID when they do not fit.
+    int init_static(const hfh_csr *A, hfh_prec *P, int maxits_, bool with_hist, char *base, size_t cap) {
+        hfh_ctx *c = A->ctx;
+        const int n = A->nrow;
+        maxits = maxits_;
+        R.A = A;
+        R.P = P;
+        R.n = n;
+        R.mode = (!P || P->kind == HFH_PREC_NONE) ? 0 : 1;
+        R.invd = R.mode == 1 ? static_cast<const T *>(P->invd) : nullptr;
+        R.nb = stream_blocks(c, n);
+        R.stride = kRedBlocks;
+        auto up256 = [](size_t b) { return (b + 255) / 256 * 256; };
+        const size_t vb = up256(sizeof(T) * static_cast<size_t>(std::max(n, 1)));
+        const size_t bpart = up256(sizeof(double) * 2 * R.stride);
+        const size_t bspart = up256(sizeof(double) * std::max(A->nwg, 1));
+        const size_t bst = up256(sizeof(KState));
+        const size_t bhist = with_hist ? up256(sizeof(double) * (static_cast<size_t>(maxits) + 2)) : 0;
+        const size_t need = 4 * vb + bpart + bspart + bst + bhist;
+        HFH_CHECK(need <= cap, HFH_ERR_INVALID,
+                  "hfh_pcg_solve_noalloc: the solve needs %zu bytes of scratch (n = %d, %s%s), more than the "
+                  "static %zu", need, n, sizeof(T) == 8 ? "FP64" : "FP32", with_hist ? ", with history" : "", cap);
+        char *o = base;
+        R.r = reinterpret_cast<T *>(o); o += vb;
+        R.p = reinterpret_cast<T *>(o); o += vb;
+        R.z = reinterpret_cast<T *>(o); o += vb;
+        R.Ap = reinterpret_cast<T *>(o); o += vb;
+        R.part = reinterpret_cast<double *>(o); o += bpart;
+        R.spart = reinterpret_cast<double *>(o); o += bspart;
+        st = reinterpret_cast<KState *>(o); o += bst;
+        hist = with_hist ? reinterpret_cast<double *>(o) : nullptr;
+        R.st = st;
+        R.hist = hist;
+        return HFH_OK;
+    }
+
     int init(const hfh_csr *A, hfh_prec *P, int maxits_) {
         hfh_ctx *c = A->ctx;
         const int n = A->nrow;
@@ -549,8 +598,11 @@ struct PcgWork {
     // x_zero: x holds zeros, so r = b without the SpMV.  keep_direction:
     // continue the previous solve's search direction, p = z + (r.z / r_k.z_k) p
     // (the residual replacement of reliable-update CG), instead of p = z.
+    // noalloc: no graph, no events, no pageable copies (the state travels as
+    // kernel arguments and comes back through the context's pinned slots),
+    // so nothing is allocated; time_ms is host wall time.
     int solve(const T *b, T *x, const hfh_control &ctl, hfh_solve_info *info, double *hhist, bool use_graph,
-              bool x_zero = false, bool keep_direction = false) {
+              bool x_zero = false, bool keep_direction = false, bool noalloc = false) {
         const hfh_csr *A = R.A;
         hfh_ctx *c = A->ctx;
         hipStream_t s = c->stream;
@@ -565,10 +617,13 @@ struct PcgWork {
         }
         hfh_vec vr{c, n, A->dtype, R.r}, vb{c, n, A->dtype, const_cast<T *>(b)}, vz{c, n, A->dtype, R.z},
             vp{c, n, A->dtype, R.p};
-        hipEvent_t e0, e1;
-        HFH_HIP(hipEventCreate(&e0));
-        HFH_HIP(hipEventCreate(&e1));
-        HFH_HIP(hipEventRecord(e0, s));
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        const auto t0 = std::chrono::steady_clock::now();
+        if (!noalloc) {
+            HFH_HIP(hipEventCreate(&e0));
+            HFH_HIP(hipEventCreate(&e1));
+            HFH_HIP(hipEventRecord(e0, s));
+        }
         int rc;
         if (x_zero) {
             HFH_HIP(hipMemcpyAsync(R.r, b, sizeof(T) * n, hipMemcpyDeviceToDevice, s));
@@ -607,22 +662,47 @@ struct PcgWork {
         h.state = state0;
         h.done = state0 != HFH_ITERATE;
         h.idle = h.done;
-        HFH_HIP(hipMemcpyAsync(st, &h, sizeof(KState), hipMemcpyHostToDevice, s));
-        HFH_HIP(hipMemcpyAsync(hist, &res0, sizeof(double), hipMemcpyHostToDevice, s));
-        if (!h.done && (rc = pcg_loop(R, ctl.maxits, use_graph))) return rc;
-        HFH_HIP(hipEventRecord(e1, s));
-        HFH_HIP(hipMemcpyAsync(&h, st, sizeof(KState), hipMemcpyDeviceToHost, s));
-        HFH_HIP(hipStreamSynchronize(s));
+        if (noalloc) {
+            hipLaunchKernelGGL(k_set_state, dim3(1), dim3(64), 0, s, st, h, hist, res0);
+            if ((rc = launch_check("k_set_state"))) return rc;
+        } else {
+            HFH_HIP(hipMemcpyAsync(st, &h, sizeof(KState), hipMemcpyHostToDevice, s));
+            HFH_HIP(hipMemcpyAsync(hist, &res0, sizeof(double), hipMemcpyHostToDevice, s));
+        }
+        if (!h.done && (rc = pcg_loop(R, ctl.maxits, use_graph && !noalloc, !noalloc))) return rc;
         float ms = 0;
-        (void)hipEventElapsedTime(&ms, e0, e1);
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
+        if (noalloc) {
+            static_assert(sizeof(KState) <= 256 * sizeof(double), "the pinned result slots hold the state");
+            HFH_HIP(hipMemcpyAsync(c->host_res, st, sizeof(KState), hipMemcpyDeviceToHost, s));
+            HFH_HIP(hipStreamSynchronize(s));
+            std::memcpy(&h, c->host_res, sizeof(KState));
+            ms = static_cast<float>(
+                std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+        } else {
+            HFH_HIP(hipEventRecord(e1, s));
+            HFH_HIP(hipMemcpyAsync(&h, st, sizeof(KState), hipMemcpyDeviceToHost, s));
+            HFH_HIP(hipStreamSynchronize(s));
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+        }
         if (h.iter > 0) {
             have_dir = true;
             prev_rz = h.beta;
         }
-        if (hhist && h.iter >= 0)
-            HFH_HIP(hipMemcpy(hhist, hist, sizeof(double) * (h.iter + 1), hipMemcpyDeviceToHost));
+        if (hhist && h.iter >= 0) {
+            if (noalloc) {
+                // through the context's 256 pinned result slots, a piece at a time
+                for (int k = 0; k <= h.iter; k += 256) {
+                    const int m = std::min(256, h.iter + 1 - k);
+                    HFH_HIP(hipMemcpyAsync(c->host_res, hist + k, sizeof(double) * m, hipMemcpyDeviceToHost, s));
+                    HFH_HIP(hipStreamSynchronize(s));
+                    std::memcpy(hhist + k, c->host_res, sizeof(double) * m);
+                }
+            } else {
+                HFH_HIP(hipMemcpy(hhist, hist, sizeof(double) * (h.iter + 1), hipMemcpyDeviceToHost));
+            }
+        }
         if (info) {
             info->iters = h.iter;
             info->state = h.state;
@@ -681,6 +761,64 @@ extern "C" int hfh_pcg_solve(const hfh_csr *A, hfh_prec *P, const hfh_vec *b, hf
         if ((rc = W.solve((const float *)b->d, (float *)x->d, *ctl, &inf, hist, true))) return rc;
     }
     if ((rc = sweep_failed(P))) return rc;
+    if (info) *info = inf;
+    return solve_status(inf.state);
+}
+
+// ============================================================================
+// Allocation-free CG for callers that cannot allocate device memory (code
+// under HIP graph capture, serving loops, harnesses that forbid hipMalloc).
+// Its work vectors, partials and state live in `scratch`, a static device
+// array of kStaticScratch bytes that every device the library runs on holds
+// until the process ends.  Host code never uses the array's name as an
+// address: hfh_ctx_create looks up each device's copy once
+// (hipGetSymbolAddress, with that device current) and keeps it in the
+// context.  All calls on one device share the array, so calls that overlap on
+// one device race; every call writes what it reads before reading it.
+
+static __device__ __align__(256) char scratch[kStaticScratch];
+
+namespace hfh {
+
+int static_scratch(int device, void **ptr) {
+    static std::mutex mu;
+    static void *addr[64] = {nullptr};
+    HFH_CHECK(device >= 0 && device < 64, HFH_ERR_INVALID, "static scratch: device %d out of range", device);
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+        HFH_CHECK(p != nullptr, HFH_ERR_HIP, "hipGetSymbolAddress(scratch): null address");
+        addr[device] = p;
+    }
+    *ptr = addr[device];
+    return HFH_OK;
+}
+
+}  // namespace hfh
+
+extern "C" int hfh_pcg_solve_noalloc(const hfh_csr *A, hfh_prec *P, const hfh_vec *b, hfh_vec *x,
+                                     const hfh_control *ctl, hfh_solve_info *info, double *hist) {
+    int rc = check_solve_args(A, P, b, x, ctl, "hfh_pcg_solve_noalloc");
+    if (rc) return rc;
+    HFH_CHECK(!P || P->kind == HFH_PREC_NONE || P->kind == HFH_PREC_JACOBI, HFH_ERR_UNSUPPORTED,
+              "hfh_pcg_solve_noalloc: Jacobi or no preconditioner only");
+    HFH_CHECK(A->ctx->scratch, HFH_ERR_INVALID, "hfh_pcg_solve_noalloc: context has no static scratch");
+    if (A->nrow == 0) return empty_system(info);
+    char *base = static_cast<char *>(A->ctx->scratch);
+    hfh_solve_info inf{};
+    if (A->dtype == HFH_F64) {
+        PcgWork<double> W;
+        if ((rc = W.init_static(A, P, ctl->maxits, hist != nullptr, base, kStaticScratch))) return rc;
+        if ((rc = W.solve((const double *)b->d, (double *)x->d, *ctl, &inf, hist, false, false, false, true)))
+            return rc;
+    } else {
+        PcgWork<float> W;
+        if ((rc = W.init_static(A, P, ctl->maxits, hist != nullptr, base, kStaticScratch))) return rc;
+        if ((rc = W.solve((const float *)b->d, (float *)x->d, *ctl, &inf, hist, false, false, false, true)))
+            return rc;
+    }
     if (info) *info = inf;
     return solve_status(inf.state);
 }

@@ -136,6 +136,7 @@ HIP_API = {
     "hfh_prec_sweep_status": (_i, [_vp]),
     "hfh_prec_destroy": (_i, [_vp]),
     "hfh_pcg_solve": (_i, [_vp, _vp, _vp, _vp, _P(Control), _P(SolveInfo), _vp]),
+    "hfh_pcg_solve_noalloc": (_i, [_vp, _vp, _vp, _vp, _P(Control), _P(SolveInfo), _vp]),
     "hfh_gmres_solve": (_i, [_vp, _vp, _i, _i, _vp, _vp, _P(Control), _P(SolveInfo), _vp]),
     "hfh_csr_levels": (_i, [_vp, _i, _P(_i), _vp, _vp]),
     "hfh_csr_multicoloring": (_i, [_vp, _P(_i), _vp, _vp, _vp]),
@@ -647,12 +648,16 @@ def _info(inf):
     return {"iters": inf.iters, "state": inf.state, "res": inf.res, "res0": inf.res0, "time_ms": inf.time_ms}
 
 
-def pcg(A, b, x, P=None, maxits=1000, atol=1e-15, rtol=1e-8, dtol=1e6, history=False, check=True):
-    """CG<LAD>::Solve on the device; returns the info dict (and history)."""
+def pcg(A, b, x, P=None, maxits=1000, atol=1e-15, rtol=1e-8, dtol=1e6, history=False, check=True,
+        noalloc=False):
+    """CG<LAD>::Solve on the device; returns the info dict (and history).
+    noalloc: the allocation-free entry point (work space in the library's
+    static device array; Jacobi or no preconditioner)."""
     ctl = Control(maxits, atol, rtol, dtol)
     inf = SolveInfo()
     hist = np.zeros(maxits + 2) if history else None
-    rc = hip().hfh_pcg_solve(A.h, P.h if P else None, b.h, x.h, C.byref(ctl), C.byref(inf), _p(hist))
+    fn = hip().hfh_pcg_solve_noalloc if noalloc else hip().hfh_pcg_solve
+    rc = fn(A.h, P.h if P else None, b.h, x.h, C.byref(ctl), C.byref(inf), _p(hist))
     if rc != OK and (check or rc != ERR_NOT_CONVERGED):
         _ck(rc)
     out = _info(inf)

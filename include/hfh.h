@@ -328,6 +328,21 @@ typedef struct {
 int hfh_pcg_solve(const hfh_csr *A, hfh_prec *P, const hfh_vec *b, hfh_vec *x,
                   const hfh_control *ctl, hfh_solve_info *info, double *hist);
 
+/* hfh_pcg_solve without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc).  The work
+ * vectors r, p, z, Ap, the partial sums, the solver state and the history
+ * live in a static device array of 64 MiB that each device the library runs
+ * on holds until the process ends: 4 n elements plus about 80 KB (and
+ * maxits + 2 doubles with hist), so FP64 systems up to about 2.09 M rows,
+ * FP32 up to about 4.19 M; larger solves return HFH_ERR_INVALID.  P must be
+ * NULL, HFH_PREC_NONE or Jacobi (else HFH_ERR_UNSUPPORTED).  Same iterates,
+ * stopping test and results as hfh_pcg_solve; the loop runs as direct
+ * launches (no graph) and info->time_ms is host wall time.  Every call on
+ * one device shares the array: calls that overlap on one device race. */
+int hfh_pcg_solve_noalloc(const hfh_csr *A, hfh_prec *P, const hfh_vec *b, hfh_vec *x,
+                          const hfh_control *ctl, hfh_solve_info *info, double *hist);
+
 /* GMRES<LAD>::SolveRight (gmres.cc:285-510) / FGMRES<LAD>::SolveRight
  * (fgmres.cc:75-262) with right preconditioning, modified Gram-Schmidt and
  * Givens rotations; the Hessenberg scalars live on the device.  flexible
