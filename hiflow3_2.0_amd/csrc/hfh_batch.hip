@@ -945,15 +945,18 @@ extern "C" int hfh_batch_create(hfh_ctx *ctx, const hfh_batch_params *p, hfh_bat
         return fail(HFH_ERR_HIP);
     }
     // capture `chunk` iterations once; the kernels read the tables at run time
-    if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+    // (without graphs, hfh_batch_step launches them directly)
+    if (graphs_enabled() && hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
         set_error("hfh_batch_create: stream capture failed");
         return fail(HFH_ERR_HIP);
     }
-    const int erc = enqueue_iterations(B, p->chunk);
-    const hipError_t e2 = hipStreamEndCapture(s, &B->graph);
-    if (erc || e2 != hipSuccess || hipGraphInstantiate(&B->exec, B->graph, nullptr, nullptr, 0) != hipSuccess) {
-        if (!erc) set_error("hfh_batch_create: graph capture failed");
-        return fail(erc ? erc : HFH_ERR_HIP);
+    if (graphs_enabled()) {
+        const int erc = enqueue_iterations(B, p->chunk);
+        const hipError_t e2 = hipStreamEndCapture(s, &B->graph);
+        if (erc || e2 != hipSuccess || hipGraphInstantiate(&B->exec, B->graph, nullptr, nullptr, 0) != hipSuccess) {
+            if (!erc) set_error("hfh_batch_create: graph capture failed");
+            return fail(erc ? erc : HFH_ERR_HIP);
+        }
     }
     ++ctx->live;
     *out = B;
@@ -1129,7 +1132,11 @@ extern "C" int hfh_batch_step(hfh_batch *B, int *active, int *nfinished) {
         launched = B->rb_next;
         B->rb_next ^= 1;
         HFH_HIP(hipEventRecord(B->ev_g0[launched], s));
-        HFH_HIP(hipGraphLaunch(B->exec, s));
+        if (B->exec) {
+            HFH_HIP(hipGraphLaunch(B->exec, s));
+        } else if ((rc = enqueue_iterations(B, B->prm.chunk))) {
+            return rc;
+        }
         HFH_HIP(hipEventRecord(B->ev_g1[launched], s));
         ++B->steps;
         HFH_HIP(hipMemcpyAsync(B->h_st[launched], B->d_st, sizeof(SysState) * B->slots.size(),

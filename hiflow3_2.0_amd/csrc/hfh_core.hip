@@ -8,6 +8,7 @@
 
 #include <cmath>
 #include <cstdarg>
+#include <cstdlib>
 #include <cstring>
 
 namespace hfh {
@@ -19,6 +20,14 @@ void set_error(const char *fmt, ...) {
     va_start(ap, fmt);
     vsnprintf(g_err, sizeof g_err, fmt, ap);
     va_end(ap);
+}
+
+bool graphs_enabled() {
+    static const bool on = [] {
+        const char *e = std::getenv("HFH_GRAPHS");
+        return !(e && std::strcmp(e, "0") == 0);
+    }();
+    return on;
 }
 
 int dev_alloc(void **p, size_t bytes) {

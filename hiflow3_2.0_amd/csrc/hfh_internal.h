@@ -211,6 +211,10 @@ inline int hip_fail(hipError_t e, const char *what) {
 
 int dev_alloc(void **p, size_t bytes);
 int launch_check(const char *what);
+// False when the environment sets HFH_GRAPHS=0: the drivers and the serving
+// engine then launch every iteration directly instead of replaying a
+// captured graph (the same kernels in the same order, the same results).
+bool graphs_enabled();
 // Device address of the static scratch array on `device` (current), looked
 // up once per device (hfh_krylov.hip).
 int static_scratch(int device, void **ptr);

@@ -434,6 +434,7 @@ static int pcg_loop(PcgRun<T> &R, int maxits, bool use_graph, bool allow_timing 
     hfh_ctx *c = R.A->ctx;
     hipStream_t s = c->stream;
     const bool timing = c->timing.on && allow_timing;
+    use_graph = use_graph && graphs_enabled();
     const int chunk = std::max(1, std::min(16, maxits));
     hipGraphExec_t exec = nullptr;
     int rc = HFH_OK;

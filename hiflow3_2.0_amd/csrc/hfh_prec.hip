@@ -1093,7 +1093,7 @@ static int trsv_graph(hfh_prec *P, const T *in, T *out) {
     // three launches already: no graph
     if (P->sweep_mode == HFH_SWEEP_SYNCFREE && P->sf_y && in != out) return trsv_sweeps<T>(P, in, out, nullptr);
     const size_t bytes = sizeof(T) * static_cast<size_t>(P->n);
-    if (!P->sweep_exec && !P->graph_failed) {
+    if (!P->sweep_exec && !P->graph_failed && graphs_enabled()) {
         int rc = HFH_OK;
         if (!P->gin && (rc = dev_alloc(&P->gin, bytes))) return rc;
         if (!P->gout && (rc = dev_alloc(&P->gout, bytes))) return rc;

@@ -51,6 +51,9 @@ const char *hfh_version(void);
 
 /* ---- context: platform init (lmp/platform_management.cc:135-187) ---- */
 int hfh_device_count(int *count);
+/* The drivers replay captured hipGraphs of their iteration chunks; with the
+ * environment variable HFH_GRAPHS=0 (read once per process) they launch the
+ * same kernels directly instead (same results), e.g. for tracing tools. */
 int hfh_ctx_create(int device, hfh_ctx **ctx);
 int hfh_ctx_destroy(hfh_ctx *ctx);
 int hfh_ctx_sync(hfh_ctx *ctx);
