@@ -215,6 +215,15 @@ def bench_pcg_q1(P, args):
     A.set_spmv(P.SPMV_AUTO)
     del Yl
     log(f"[bench] stream ceiling {ceiling}; layouts {layouts}")
+    cg_out = {"iters": iters, "ms_per_iter": round(ms_iter, 4), "solve_ms": round(ms_step, 3),
+              "iter_GBps_compulsory": round(it_bytes / (ms_iter * 1e-3) / 1e9, 1),
+              "true_rel_residual": true_rel, "state": infos[-1]["state"]}
+    if not args.no_cpu:
+        O = oracle_mod()
+        OA = O.Csr(row, col, val)
+        cg_out["cpu"] = cpu_full_solve(lambda: O.cg(OA, rhs, P=O.Prec.jacobi(OA), ctrl=(5000, 0.0, 1e-8, 1e6)),
+                                       iters, "oracle FP64 Jacobi-PCG (CG::SolvePrecond loop order)")
+        log(f"[bench] cpu cg {cg_out['cpu']}")
     return {
         "metric": "SpMV GB/s (% HBM peak); CG time to 1e-10 rel; served solves/s + p99",
         "value": round(gbs, 1),
@@ -233,11 +242,13 @@ def bench_pcg_q1(P, args):
         "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": spmv_kernel_name(A), "bytes_per_launch": b_spmv,
-                     "avg_launch_us": round(avg_spmv_ms * 1e3, 2), "launches": launches},
+                     "avg_launch_us": round(avg_spmv_ms * 1e3, 2), "launches": launches,
+                     # the HBM bytes the counters measured per launch over the same launch time
+                     "traffic_GBps": round(traffic / (avg_spmv_ms * 1e-3) / 1e9, 1) if traffic else None,
+                     "traffic_frac": round(traffic / (avg_spmv_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+                     if traffic else None},
         "cpu_baseline": cpu,
-        "cg": {"iters": iters, "ms_per_iter": round(ms_iter, 4), "solve_ms": round(ms_step, 3),
-               "iter_GBps_compulsory": round(it_bytes / (ms_iter * 1e-3) / 1e9, 1),
-               "true_rel_residual": true_rel, "state": infos[-1]["state"]},
+        "cg": cg_out,
         "parity": {"spmv_rel_vs_oracle": rel},
         "stream_ceiling": ceiling,
         "spmv_layouts": layouts,
@@ -293,15 +304,28 @@ def oracle_mod():
     return O
 
 
-def cpu_iters_sample(run, k, gpu_iters, what):
-    """Time k iterations of the oracle solver (one host thread) and scale to
-    the GPU's iteration count."""
-    t0 = time.perf_counter()
-    run(k)
-    dt = (time.perf_counter() - t0) / k
-    return {"kind": "port", "cores": 1, "ms_per_iter": round(dt * 1e3, 3),
-            "est_solve_ms": round(dt * gpu_iters * 1e3, 1),
-            "sample": f"{what}: {k} iterations on the full system, scaled to {gpu_iters} iterations"}
+def host_threads():
+    return int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+
+
+def cpu_full_solve(run, gpu_iters, what):
+    """The oracle's full solve with its row-parallel loops (SpMV, element-wise
+    updates, the rows of one colour) on all host threads; dot products and
+    level-ordered sweeps stay sequential, so the iterates are the
+    single-thread (reference-order) ones."""
+    O = oracle_mod()
+    threads = host_threads()
+    O.set_threads(threads)
+    try:
+        t0 = time.perf_counter()
+        _, info = run()
+        dt = time.perf_counter() - t0
+    finally:
+        O.set_threads(1)
+    return {"kind": "port", "cores": threads, "ms": round(dt * 1e3, 1), "iters": int(info["iters"]),
+            "iters_within_2_of_gpu": abs(int(info["iters"]) - gpu_iters) <= 2, "rel_residual_reported": info["res"] / info["res0"],
+            "sample": f"{what}: the full solve, row-parallel loops on {threads} threads (dot products and "
+                      f"triangular sweeps sequential)"}
 
 
 def true_rel(P, ctx, A, X, b):
@@ -407,12 +431,26 @@ def bench_fv192_dc(P, args):
     A.set_spmv(P.SPMV_AUTO)
     out["speedup_dc_vs_fp64"] = round(out["fp64_cg_mcsgs"]["ms"] / out["dc_fp32_mcsgs"]["ms"], 3)
     out["speedup_restart_dc_vs_fp64"] = round(out["fp64_cg_mcsgs"]["ms"] / out["dc_fp32_mcsgs_restart"]["ms"], 3)
+    # the operator's SpMV on its own (automatic layout), both precisions, GB/s
+    # on the CSR byte count
+    Xs, Ys = P.Vector(ctx, bp), P.Vector(ctx, n)
+    Xs32, Ys32 = P.Vector(ctx, bp, np.float32), P.Vector(ctx, n, np.float32)
+    us64 = timed_spmv_us(P, ctx, A, Xs, Ys)
+    us32 = timed_spmv_us(P, ctx, A32, Xs32, Ys32)
+    b64, b32 = spmv_bytes(n, nnz), spmv_bytes(n, nnz, val_bytes=4)
+    out["spmv"] = {"fp64": {"kernel": spmv_kernel_name(A), "avg_launch_us": round(us64, 2),
+                            "GBps_on_csr_bytes": round(b64 / (us64 * 1e-6) / 1e9, 1),
+                            "frac_hbm_peak": round(b64 / (us64 * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)},
+                   "fp32": {"avg_launch_us": round(us32, 2),
+                            "GBps_on_csr_bytes": round(b32 / (us32 * 1e-6) / 1e9, 1),
+                            "frac_hbm_peak": round(b32 / (us32 * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)}}
+    del Xs, Ys, Xs32, Ys32
     if not args.no_cpu:
         O = oracle_mod()
         OA = O.Csr(prow, pcol, pval)
         OP = O.Prec(O.PREC_MCSGS, OA, sizes=sizes)
-        out["fp64_cg_mcsgs"]["cpu"] = cpu_iters_sample(
-            lambda k: O.cg(OA, bp, P=OP, ctrl=(k, 0.0, 0.0, 1e300)), 5, out["fp64_cg_mcsgs"]["iters"],
+        out["fp64_cg_mcsgs"]["cpu"] = cpu_full_solve(
+            lambda: O.cg(OA, bp, P=OP, ctrl=(20000, 0.0, 1e-10, 1e6)), out["fp64_cg_mcsgs"]["iters"],
             "oracle FP64 CG + multicolour SGS (CG::SolvePrecond loop order)")
     log(f"[fv192_dc] {out}")
     return out
@@ -443,27 +481,46 @@ def bench_upwind128_gmres(P, args):
     log(f"[upwind128_gmres] upwind 128^3: n={len(b)} nnz={len(col)} levels {len(lo) - 1}/{len(up) - 1} "
         f"setup {time.perf_counter() - t0:.1f}s")
     Z = P.Vector(ctx, len(b))
-    Pi.apply(B, Z)
-    ctx.sync()
-    t = time.perf_counter()
-    for _ in range(5):
+
+    def apply_time():
         Pi.apply(B, Z)
-    ctx.sync()
-    apply_ms = (time.perf_counter() - t) / 5 * 1e3
+        ctx.sync()
+        t = time.perf_counter()
+        for _ in range(5):
+            Pi.apply(B, Z)
+        ctx.sync()
+        return (time.perf_counter() - t) / 5 * 1e3
+    # the one-launch-per-level schedule beside the default sync-free sweeps
+    Pi.set_sweep(P.SWEEP_LEVELS)
+    apply_lv_ms = apply_time()
+    Pi.set_sweep(P.SWEEP_SYNCFREE)
+    apply_ms = apply_time()
+    Pi.sweep_status()
     ts = []
     for _ in range(max(1, args.gmres_reps) + 1):
         X = P.Vector(ctx, len(b))
         inf = P.gmres(A, B, X, Pi, m=30, maxits=5000, atol=0.0, rtol=1e-8)
         ts.append(inf["time_ms"])
-    out = {"n": len(b), "nnz": len(col), "levels": [len(lo) - 1, len(up) - 1], "setup": setup,
+    n, nnz = len(b), len(col)
+    Ys = P.Vector(ctx, n)
+    us = timed_spmv_us(P, ctx, A, B, Ys)
+    del Ys
+    # ILU(0) apply: compulsory bytes of both sweeps, 12 nnz + 24 n (factors,
+    # columns, rhs, result), over the apply time
+    out = {"n": n, "nnz": nnz, "levels": [len(lo) - 1, len(up) - 1], "setup": setup,
            "ms": round(float(np.median(ts[1:])), 2), "iters": inf["iters"], "ilu_apply_ms": round(apply_ms, 3),
-           "true_rel_residual": true_rel(P, ctx, A, X, b)}
+           "ilu_apply_GBps": round((12 * nnz + 24 * n) / (apply_ms * 1e-3) / 1e9, 1),
+           "ilu_apply_ms_level_launches": round(apply_lv_ms, 3),
+           "true_rel_residual": true_rel(P, ctx, A, X, b),
+           "spmv": {"kernel": spmv_kernel_name(A), "avg_launch_us": round(us, 2),
+                    "GBps_on_csr_bytes": round(spmv_bytes(n, nnz) / (us * 1e-6) / 1e9, 1),
+                    "frac_hbm_peak": round(spmv_bytes(n, nnz) / (us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)}}
     if not args.no_cpu:
         O = oracle_mod()
         OA = O.Csr(row, col, val)
         OP = O.Prec(O.PREC_ILU0, O.Csr(row, col, lu))
-        out["cpu"] = cpu_iters_sample(lambda k: O.gmres(OA, b, P=OP, m=30, ctrl=(k, 0.0, 0.0, 1e300)), 10,
-                                      inf["iters"], "oracle FP64 GMRES(30) + ILU(0) (GMRES::SolveRight loop order)")
+        out["cpu"] = cpu_full_solve(lambda: O.gmres(OA, b, P=OP, m=30, ctrl=(5000, 0.0, 1e-8, 1e6)), inf["iters"],
+                                    "oracle FP64 GMRES(30) + ILU(0) (GMRES::SolveRight loop order)")
     log(f"[upwind128_gmres] {out}")
     return out
 
@@ -496,6 +553,8 @@ def bench_q1_2d_cg(P, args):
         out["cpu"] = {"kind": "port", "cores": 1, "ms": round(dt * 1e3, 1), "iters": oi["iters"],
                       "sample": "oracle FP64 Jacobi-PCG (CG::SolvePrecond loop order), the full solve"}
         out["iters_within_2_of_cpu"] = abs(oi["iters"] - inf["iters"]) <= 2
+        out["cpu_all_cores"] = cpu_full_solve(lambda: O.cg(OA, b, P=O.Prec.jacobi(OA), ctrl=(5000, 0.0, 1e-8, 1e6)),
+                                              inf["iters"], "oracle FP64 Jacobi-PCG (CG::SolvePrecond loop order)")
     log(f"[q1_2d_cg] {out}")
     return out
 
@@ -643,7 +702,9 @@ def bench_serving(P, args):
         # the oracle's MPIR with an FP32 Jacobi-CG inner solver (the engine's
         # algorithm) on the first requests of the same stream, within a budget
         O = oracle_mod()
-        t_end = time.perf_counter() + 20.0
+        threads = host_threads()
+        O.set_threads(threads)
+        t_end = time.perf_counter() + 15.0
         ts, ns_cpu = [], []
         for i in range(args.serve_requests):
             if time.perf_counter() > t_end:
@@ -657,10 +718,12 @@ def bench_serving(P, args):
                    ecl=(20000, 0.0, 1e-4, 1e6))
             ts.append(time.perf_counter() - t0)
             ns_cpu.append(nx * ny * nz)
-        out["cpu"] = {"kind": "port", "cores": 1, "requests": len(ts),
+        O.set_threads(1)
+        out["cpu"] = {"kind": "port", "cores": threads, "requests": len(ts),
                       "solves_per_s": round(len(ts) / sum(ts), 3) if ts else None,
                       "sample": f"oracle MPIR (FP32 Jacobi-CG inner) on the first {len(ts)} requests of the "
-                                f"stream (mean n {int(np.mean(ns_cpu)) if ns_cpu else 0}), one after another"}
+                                f"stream (mean n {int(np.mean(ns_cpu)) if ns_cpu else 0}), one after another, "
+                                f"row-parallel loops on {threads} threads"}
     log(f"[serving] {out}")
     return out
 

@@ -12,6 +12,22 @@
 #include <stdlib.h>
 #include <string.h>
 
+/* Threads of the row-parallel loops (SpMV, the element-wise updates, the
+ * rows of one colour in the multicolour sweeps): rows are independent there,
+ * so any thread count gives the same bits.  Dot products, level-ordered
+ * sweeps and everything else stay sequential, in the reference's order. */
+static int hfo_nt = 1;
+
+void hfo_set_threads(int t)
+{
+    hfo_nt = t < 1 ? 1 : t;
+}
+
+int hfo_get_threads(void)
+{
+    return hfo_nt;
+}
+
 #define T double
 #define S d
 #define ABS fabs

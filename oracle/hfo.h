@@ -102,6 +102,13 @@ int hfo_levelscheduling(int n, const int *row, const int *col,
 int hfo_mc_dropoffs_d(int n, int *row, int *col, double *val, int ncolors,
                       const int *color_sizes);
 
+/* Threads of the row-parallel loops (SpMV, element-wise updates, the rows of
+ * one colour of the multicolour sweeps); 1 (the default) runs everything in
+ * one thread.  The results do not depend on it: dot products and the
+ * level-ordered sweeps stay sequential in the reference's order. */
+void hfo_set_threads(int t);
+int hfo_get_threads(void);
+
 /* ---------------- type-generic kernels: _d (double) and _f (float) ---------------- */
 #define HFO_DECL(T, S)                                                                  \
 void   hfo_spmv_##S(int n, const int *row, const int *col, const T *val,                 \

@@ -9,9 +9,12 @@
 #define PREC CAT(hfo_prec_, S)
 
 /* CPUsimple_CSR_lMatrix::VectorMult (CPUsimple_csr_lmatrix.cc:75-105):
- * res = 0; res += val[j] * x[col[j]] over j ascending; y[i] = res. */
+ * res = 0; res += val[j] * x[col[j]] over j ascending; y[i] = res.  The
+ * rows run on hfo_nt threads (the same per-row arithmetic, as
+ * CPUopenmp_CSR_lMatrix::VectorMult splits them). */
 void FN(hfo_spmv_)(int n, const int *row, const int *col, const T *val, const T *x, T *y)
 {
+    #pragma omp parallel for if (hfo_nt > 1) num_threads(hfo_nt)
     for (int i = 0; i < n; ++i) {
         T res = 0;
         for (int j = row[i]; j < row[i + 1]; ++j)
@@ -75,6 +78,7 @@ T FN(hfo_nrm2_)(int n, const T *x)
 /* CPUsimple_lVector::Axpy (CPUsimple_lvector.cc:234-287): y += a*x */
 void FN(hfo_axpy_)(int n, T *y, const T *x, T a)
 {
+    #pragma omp parallel for if (hfo_nt > 1) num_threads(hfo_nt)
     for (int i = 0; i < n; ++i)
         y[i] += a * x[i];
 }
@@ -82,6 +86,7 @@ void FN(hfo_axpy_)(int n, T *y, const T *x, T a)
 /* CPUsimple_lVector::ScaleAdd (CPUsimple_lvector.cc:310-365): y = a*y + x */
 void FN(hfo_scaleadd_)(int n, T *y, T a, const T *x)
 {
+    #pragma omp parallel for if (hfo_nt > 1) num_threads(hfo_nt)
     for (int i = 0; i < n; ++i)
         y[i] = a * y[i] + x[i];
 }
@@ -89,6 +94,7 @@ void FN(hfo_scaleadd_)(int n, T *y, T a, const T *x)
 /* CPUsimple_lVector::Scale (CPUsimple_lvector.cc:388-435): y *= a */
 void FN(hfo_scale_)(int n, T *y, T a)
 {
+    #pragma omp parallel for if (hfo_nt > 1) num_threads(hfo_nt)
     for (int i = 0; i < n; ++i)
         y[i] *= a;
 }
@@ -96,6 +102,7 @@ void FN(hfo_scale_)(int n, T *y, T a)
 /* CPU_lVector::ElementWiseMult (lvector_cpu.cc:571-578): y .*= x */
 void FN(hfo_ewmult_)(int n, T *y, const T *x)
 {
+    #pragma omp parallel for if (hfo_nt > 1) num_threads(hfo_nt)
     for (int i = 0; i < n; ++i)
         y[i] *= x[i];
 }
@@ -378,6 +385,7 @@ void FN(hfo_mcsgs_)(int n, const int *row, const int *col, const T *val, int nco
     for (int c = 0; c < ncolors; ++c)
         off[c + 1] = off[c] + color_sizes[c];
     T *d = (T *)malloc(sizeof(T) * (n > 0 ? n : 1));
+    #pragma omp parallel for if (hfo_nt > 1) num_threads(hfo_nt)
     for (int i = 0; i < n; ++i) {
         d[i] = 0;
         for (int j = row[i]; j < row[i + 1]; ++j)
@@ -387,9 +395,11 @@ void FN(hfo_mcsgs_)(int n, const int *row, const int *col, const T *val, int nco
     const T relax = (T)(1.0 / (omega * (2.0 - omega)));
     /* forward: chunk_i = -r_i; chunk_i += L_ij chunk_j (j<i); chunk_i *= -1/D */
     for (int c = 0; c < ncolors; ++c) {
+        #pragma omp parallel for if (hfo_nt > 1) num_threads(hfo_nt)
         for (int i = off[c]; i < off[c + 1]; ++i)
             out[i] = in[i] * (T)(-1.0);
         for (int cj = 0; cj < c; ++cj)
+            #pragma omp parallel for if (hfo_nt > 1) num_threads(hfo_nt)
             for (int i = off[c]; i < off[c + 1]; ++i) {
                 T res = 0;
                 for (int j = row[i]; j < row[i + 1]; ++j)
@@ -397,15 +407,18 @@ void FN(hfo_mcsgs_)(int n, const int *row, const int *col, const T *val, int nco
                         res += (val[j] * omega) * out[col[j]];
                 out[i] += res;
             }
+        #pragma omp parallel for if (hfo_nt > 1) num_threads(hfo_nt)
         for (int i = off[c]; i < off[c + 1]; ++i)
             out[i] *= (T)1 / d[i] * (T)(-1.0);
     }
     /* diagonal: chunk_i *= -D_i * relax */
+    #pragma omp parallel for if (hfo_nt > 1) num_threads(hfo_nt)
     for (int i = 0; i < n; ++i)
         out[i] *= (d[i] * (T)(-1.0)) * relax;
     /* backward: chunk_i += R_ij chunk_j (j>i, descending); chunk_i *= -1/D */
     for (int c = ncolors - 1; c >= 0; --c) {
         for (int cj = ncolors - 1; cj > c; --cj)
+            #pragma omp parallel for if (hfo_nt > 1) num_threads(hfo_nt)
             for (int i = off[c]; i < off[c + 1]; ++i) {
                 T res = 0;
                 for (int j = row[i]; j < row[i + 1]; ++j)
@@ -413,6 +426,7 @@ void FN(hfo_mcsgs_)(int n, const int *row, const int *col, const T *val, int nco
                         res += (val[j] * omega) * out[col[j]];
                 out[i] += res;
             }
+        #pragma omp parallel for if (hfo_nt > 1) num_threads(hfo_nt)
         for (int i = off[c]; i < off[c + 1]; ++i)
             out[i] *= (T)1 / d[i] * (T)(-1.0);
     }

@@ -82,6 +82,8 @@ def _declare(L):
         "hfo_cast_d2f": (None, [i, vp, vp]),
         "hfo_cast_f2d": (None, [i, vp, vp]),
         "hfo_mpir": (i, [i, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
+        "hfo_set_threads": (None, [i]),
+        "hfo_get_threads": (i, []),
     }
     for s, t in (("d", d), ("f", f)):
         sig.update({
@@ -123,6 +125,15 @@ def _declare(L):
         fn = getattr(L, name)
         fn.restype = res
         fn.argtypes = args
+
+
+def set_threads(t):
+    """Threads of the oracle's row-parallel loops (results do not depend on it)."""
+    lib().hfo_set_threads(int(t))
+
+
+def get_threads():
+    return lib().hfo_get_threads()
 
 
 def _p(a):

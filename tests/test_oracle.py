@@ -607,3 +607,36 @@ def test_matrix_value_ops(oracle, pkg):
     assert np.array_equal(oracle.Csr(A.row, A.col, oracle.mat_scale(A, 3.0, True)).to_dense(), off)
     x = np.random.default_rng(2).normal(size=A.n)
     assert np.array_equal(oracle.pjacobi(A, x), x / np.diag(D))
+
+
+def test_threaded_rows_same_bits(oracle, pkg):
+    """The oracle's row-parallel loops (SpMV, element-wise updates, the rows
+    of one colour in the multicolour sweeps) give the single-thread bits: the
+    CG + MC-SGS and GMRES + ILU(0) solves that the benchmark times on all host
+    cores are the reference-order solves."""
+    row, col, val, rhs = pkg.problem(pkg.FV_DIFFUSION, 20, p0=1.0, seed=5)
+    nc, color, sizes, perm = pkg.multicoloring(row, col)
+    prow, pcol, pval = pkg.csr_permute(row, col, val, perm)
+    A = oracle.Csr(prow, pcol, pval)
+    b = rhs[perm]
+    urow, ucol, uval, ub = pkg.problem(pkg.UPWIND_CONVDIFF, 14, p0=1e-2, p1=5.0, seed=2)
+    U = oracle.Csr(urow, ucol, uval)
+    LU, miss = oracle.ilu0(U)
+    assert miss == 0
+    runs = []
+    try:
+        for t in (1, 4):
+            oracle.set_threads(t)
+            assert oracle.get_threads() == t
+            x1, i1 = oracle.cg(A, b, P=oracle.Prec(oracle.PREC_MCSGS, A, sizes=sizes),
+                               ctrl=(2000, 0.0, 1e-10, 1e6))
+            x2, i2 = oracle.gmres(U, ub, P=oracle.Prec(oracle.PREC_ILU0, LU), m=30, ctrl=(2000, 0.0, 1e-8, 1e6))
+            y = oracle.spmv(U, ub)
+            runs.append((x1, i1, x2, i2, y))
+    finally:
+        oracle.set_threads(1)
+    (a1, ai1, a2, ai2, ay), (b1, bi1, b2, bi2, by) = runs
+    assert ai1["iters"] == bi1["iters"] and np.array_equal(ai1["hist"], bi1["hist"]) and np.array_equal(a1, b1)
+    assert ai2["iters"] == bi2["iters"] and np.array_equal(ai2["hist"], bi2["hist"]) and np.array_equal(a2, b2)
+    assert np.array_equal(ay, by)
+    assert ai1["iters"] > 20 and ai2["iters"] > 5
