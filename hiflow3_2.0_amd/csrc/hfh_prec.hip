@@ -112,22 +112,24 @@ constexpr int kSweepU = 8;   // column slots in flight (two code words when code
 
 // Columns of the kSweepU slots k.. of row i: stored (scol), or row + the
 // offset coded in the slot's byte of its code word (codes: kSweepU / 4 words
-// per step, the step's word index clamped to the slice's last word, the
-// column clamped to [0, n): padding slots and rows past the colour read a
-// valid entry, and their products are not summed).
+// per step; the column clamped to [0, n): padding slots and rows past the
+// colour read a valid entry, and their products are not summed).  Slots past
+// the slice's width L are not loaded (a wave-uniform test: L is the slice's),
+// so a slice of 3 slots issues 3 loads, not kSweepU.
 template <bool CODED>
 __device__ __forceinline__ void sweep_cols(int (&cj)[kSweepU], int k, int L, int base, int wbase, int lane, int i, int n,
                                            const int *__restrict__ scol, const unsigned *__restrict__ code,
                                            const int *dict) {
     if (!CODED) {
 #pragma unroll
-        for (int u = 0; u < kSweepU; ++u) cj[u] = __builtin_nontemporal_load(&scol[base + min(k + u, L - 1) * 64 + lane]);
+        for (int u = 0; u < kSweepU; ++u)
+            cj[u] = k + u < L ? __builtin_nontemporal_load(&scol[base + (k + u) * 64 + lane]) : min(i, n - 1);
         return;
     }
     const int W = (L + 3) >> 2;   // code words of the slice (its width rounded up to 4 slots)
 #pragma unroll
     for (int q = 0; q < kSweepU / 4; ++q) {
-        const unsigned cw = __builtin_nontemporal_load(&code[wbase + min((k >> 2) + q, W - 1) * 64 + lane]);
+        const unsigned cw = (k >> 2) + q < W ? __builtin_nontemporal_load(&code[wbase + ((k >> 2) + q) * 64 + lane]) : 0u;
 #pragma unroll
         for (int t = 0; t < 4; ++t) cj[4 * q + t] = min(i + dict[(cw >> (8 * t)) & 255u], n - 1);
     }
@@ -149,10 +151,9 @@ __global__ __launch_bounds__(kBlock) void k_mcs_fwd(int c, ColorOffsets co, bool
         for (int t = threadIdx.x; t < ndict; t += kBlock) dict[t] = gdict[t];
         __syncthreads();
     }
-    const int w = blockIdx.x * (kBlock / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-    if (!RZ && w >= nsl) return;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     double rz = 0;
-    if (w < nsl) {
+    for (int w = blockIdx.x * (kBlock / 64) + wave; w < nsl; w += gridDim.x * (kBlock / 64)) {
         const int s = slice0 + w;
         const int i = co.off[c] + w * 64 + lane;
         const bool valid = i < co.off[c + 1];
@@ -168,9 +169,10 @@ __global__ __launch_bounds__(kBlock) void k_mcs_fwd(int c, ColorOffsets co, bool
             T v[kSweepU], xv[kSweepU];
             sweep_cols<CODED>(cj, k, L, base, wbase, lane, i, n, scol, code, dict);
 #pragma unroll
-            for (int u = 0; u < kSweepU; ++u) v[u] = __builtin_nontemporal_load(&sval[base + min(k + u, L - 1) * 64 + lane]);
-#pragma unroll
-            for (int u = 0; u < kSweepU; ++u) xv[u] = out[cj[u]];
+            for (int u = 0; u < kSweepU; ++u) {   // slots past the slice width: not loaded (wave-uniform)
+                v[u] = k + u < L ? __builtin_nontemporal_load(&sval[base + (k + u) * 64 + lane]) : static_cast<T>(0);
+                xv[u] = k + u < L ? out[cj[u]] : static_cast<T>(0);
+            }
 #pragma unroll
             for (int u = 0; u < kSweepU; ++u) {
                 if (k + u >= len) continue;
@@ -188,7 +190,7 @@ __global__ __launch_bounds__(kBlock) void k_mcs_fwd(int c, ColorOffsets co, bool
             chunk *= idv[i];
             if (last) chunk = (chunk * dv[i]) * idv[i];
             out[i] = chunk;
-            if (RZ) rz = static_cast<double>(ri) * static_cast<double>(chunk);
+            if (RZ) rz += static_cast<double>(ri) * static_cast<double>(chunk);
         }
     }
     if (RZ) {
@@ -215,10 +217,9 @@ __global__ __launch_bounds__(kBlock) void k_mcs_bwd(int c, int ncolors, ColorOff
         for (int t = threadIdx.x; t < ndict; t += kBlock) dict[t] = gdict[t];
         __syncthreads();
     }
-    const int w = blockIdx.x * (kBlock / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-    if (!RZ && w >= nsl) return;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     double rz = 0;
-    if (w < nsl) {
+    for (int w = blockIdx.x * (kBlock / 64) + wave; w < nsl; w += gridDim.x * (kBlock / 64)) {
         const int s = slice0 + w;
         const int i = co.off[c] + w * 64 + lane;
         const bool valid = i < co.off[c + 1];
@@ -233,9 +234,10 @@ __global__ __launch_bounds__(kBlock) void k_mcs_bwd(int c, int ncolors, ColorOff
             T v[kSweepU], xv[kSweepU];
             sweep_cols<CODED>(cj, k, L, base, wbase, lane, i, n, scol, code, dict);
 #pragma unroll
-            for (int u = 0; u < kSweepU; ++u) v[u] = __builtin_nontemporal_load(&sval[base + min(k + u, L - 1) * 64 + lane]);
-#pragma unroll
-            for (int u = 0; u < kSweepU; ++u) xv[u] = out[cj[u]];
+            for (int u = 0; u < kSweepU; ++u) {   // slots past the slice width: not loaded (wave-uniform)
+                v[u] = k + u < L ? __builtin_nontemporal_load(&sval[base + (k + u) * 64 + lane]) : static_cast<T>(0);
+                xv[u] = k + u < L ? out[cj[u]] : static_cast<T>(0);
+            }
 #pragma unroll
             for (int u = 0; u < kSweepU; ++u) {
                 if (k + u >= len) continue;
@@ -252,7 +254,7 @@ __global__ __launch_bounds__(kBlock) void k_mcs_bwd(int c, int ncolors, ColorOff
             chunk += res;
             const T zi = chunk * idv[i];
             out[i] = zi;
-            if (RZ) rz = static_cast<double>(rin[i]) * static_cast<double>(zi);
+            if (RZ) rz += static_cast<double>(rin[i]) * static_cast<double>(zi);
         }
     }
     if (RZ) {
@@ -753,6 +755,15 @@ static int trsv_sweeps(hfh_prec *P, const T *in, T *out, const int *done) {
     return launch_check("k_tsweep");
 }
 
+// Workgroups per CU of a multicolour SGS colour step (0: one slice per
+// wave).  FP32 steps are light: 8 workgroups per CU whose waves take several
+// slices each (and store 8 r.z partials per CU instead of one per 4 slices)
+// beat one slice per wave; FP64 keeps one slice per wave (FV 192^3: the FP32
+// defect correction 96 -> 85 ms with the FP32 SpMV's like grid; FP64 CG +
+// MC-SGS 150 ms either way, 158 ms with FP64 steps capped too).
+template <typename T> constexpr int kSweepGridPerCu = 0;
+template <> constexpr int kSweepGridPerCu<float> = 8;
+
 // r.z partials that a multicolour SGS apply with rzp stores: one per
 // workgroup of the steps that finalise z (an upper bound for any colouring).
 int mcsgs_rz_slots(const hfh_prec *P) {
@@ -777,9 +788,16 @@ static int mcsgs_apply(hfh_prec *P, const T *in, T *out, const int *done, bool f
     // with rzp, the steps that finalise z (the last colour's forward step,
     // every backward step) store r.z partials there, one per workgroup
     int nrz = 0;
+    // workgroups of a colour step: one slice per wave, or at most
+    // kSweepGridPerCu per CU with waves taking several slices
+    const int gcap = std::max(P->ctx->num_cu, 1) * kSweepGridPerCu<T>;
+    auto grid = [&](int nsl) {
+        const int need = (nsl + WPB - 1) / WPB;
+        return std::max(1, gcap > 0 ? std::min(need, gcap) : need);
+    };
     auto fwd = [&](auto coded, auto rz, int c, int s0, int nsl) {
         const hfh_sweep &W = P->fw;
-        const int g = (nsl + WPB - 1) / WPB;
+        const int g = grid(nsl);
         hipLaunchKernelGGL((k_mcs_fwd<T, decltype(coded)::value, decltype(rz)::value>), dim3(g), dim3(kBlock), 0, s, c,
                            co, c == nc - 1, s0, nsl, P->n, W.ptr, W.len, W.col, W.code, W.wptr, W.dict, W.ndict,
                            (const T *)W.val, in, idv, dv, out, done, rzp + nrz);
@@ -787,7 +805,7 @@ static int mcsgs_apply(hfh_prec *P, const T *in, T *out, const int *done, bool f
     };
     auto bwd = [&](auto coded, auto rz, int c, int s0, int nsl) {
         const hfh_sweep &W = P->bw;
-        const int g = (nsl + WPB - 1) / WPB;
+        const int g = grid(nsl);
         hipLaunchKernelGGL((k_mcs_bwd<T, decltype(coded)::value, decltype(rz)::value>), dim3(g), dim3(kBlock), 0, s, c,
                            nc, co, s0, nsl, P->n, W.ptr, W.len, W.col, W.code, W.wptr, W.dict, W.ndict,
                            (const T *)W.val, idv, dv, out, done, in, rzp + nrz);
@@ -798,7 +816,7 @@ static int mcsgs_apply(hfh_prec *P, const T *in, T *out, const int *done, bool f
         for (int c = 0; c < nc; ++c) {
             if (co.off[c + 1] == co.off[c]) continue;
             const hfh_sweep &W = c == nc - 1 ? P->fw : P->bw;
-            need += (W.slice0[c + 1] - W.slice0[c] + WPB - 1) / WPB;
+            need += grid(W.slice0[c + 1] - W.slice0[c]);
         }
         if (need > mcsgs_rz_slots(P)) {
             set_error("mcsgs_apply: %d r.z partials exceed the %d slots", need, mcsgs_rz_slots(P));

@@ -467,9 +467,11 @@ __global__ __launch_bounds__(kBlock) void k_spmv_sellx(
     }
 }
 
-// SELL-64 with dictionary-coded columns: column = row + dict[code].  Value
-// slots are padded to a multiple of 4 per slice; code word w of a lane holds
-// the codes of slots 4w..4w+3 (byte t = slot 4w+t).  Row sums as k_spmv_sell.
+// SELL-64 with dictionary-coded columns: column = row + dict[code].  Each
+// slice keeps its width L of value slots; code word w of a lane holds the
+// codes of slots 4w..4w+3 (byte t = slot 4w+t), ceil(L / 4) words per lane,
+// and the slots of the last word past L are neither stored nor loaded.  Row
+// sums as k_spmv_sell.
 // JAC 1 / 2: a Jacobi step (normal / damped) -- entries with the code of
 // offset 0 (code0, the diagonal) are skipped and xn_r = jacobi_update(...)
 // is stored instead of the row sum.
@@ -489,7 +491,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv_selld(
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     double dsum = 0;
     for (int s = blockIdx.x * WPB + wave; s < nslice; s += gridDim.x * WPB) {
-        const int vbase = sptr[s], W = (sptr[s + 1] - vbase) >> 8;   // code words = slots / 4
+        const int vbase = sptr[s], L = (sptr[s + 1] - vbase) >> 6, W = (L + 3) >> 2;   // slots, code words
         const int cbase = wptr[s];
         const int r = s * 64 + lane;
         const int len = r < nrow ? rlen[r] : 0;
@@ -502,7 +504,9 @@ __global__ __launch_bounds__(kBlock) void k_spmv_selld(
                 const int ww = min(k + u, W - 1);
                 cw[u] = __builtin_nontemporal_load(&codes[cbase + ww * 64 + lane]);
 #pragma unroll
-                for (int t = 0; t < 4; ++t) v[u][t] = __builtin_nontemporal_load(&sval[vbase + (4 * ww + t) * 64 + lane]);
+                for (int t = 0; t < 4; ++t)   // slots past the slice width: not stored (wave-uniform test)
+                    v[u][t] = 4 * ww + t < L ? __builtin_nontemporal_load(&sval[vbase + (4 * ww + t) * 64 + lane])
+                                             : static_cast<T>(0);
             }
 #pragma unroll
             for (int u = 0; u < UW; ++u)
@@ -939,23 +943,24 @@ int spmv_configure(hfh_csr *A, int algo, int p1, int p2) {
     } else if (algo == HFH_SPMV_SELL_DICT) {
         const int nsl = (n + 63) / 64;
         std::vector<int> ptr(nsl + 1, 0), wp(nsl + 1, 0), len(std::max(n, 1), 0);
-        long long off = 0;
+        // values: each slice's width L of slots; code words: ceil(L / 4) per lane
+        long long off = 0, woff = 0;
         for (int sl = 0; sl < nsl; ++sl) {
             int L = 0;
             for (int r = sl * 64; r < std::min(n, sl * 64 + 64); ++r) {
                 len[r] = A->h_row[r + 1] - A->h_row[r];
                 L = std::max(L, len[r]);
             }
-            const int L4 = (L + 3) / 4 * 4;
             ptr[sl] = static_cast<int>(off);
-            wp[sl] = static_cast<int>(off / 4);
-            off += 64LL * L4;
+            wp[sl] = static_cast<int>(woff);
+            off += 64LL * L;
+            woff += 64LL * ((L + 3) / 4);
             HFH_CHECK(off < (1LL << 31), HFH_ERR_UNSUPPORTED, "spmv: SELL copy exceeds 2^31 entries");
         }
         ptr[nsl] = static_cast<int>(off);
-        wp[nsl] = static_cast<int>(off / 4);
+        wp[nsl] = static_cast<int>(woff);
         // code words: byte t of word w of a lane = code of slot 4w + t
-        std::vector<unsigned> words(std::max(off / 4, 1LL), 0u);
+        std::vector<unsigned> words(std::max(woff, 1LL), 0u);
         // padding slots take the code of offset 0, which the table always
         // holds: their gather reads x[min(r, ncol - 1)], in range
         const unsigned zero = static_cast<unsigned>(std::lower_bound(dict.begin(), dict.end(), 0) - dict.begin());
@@ -963,7 +968,7 @@ int spmv_configure(hfh_csr *A, int algo, int p1, int p2) {
             std::vector<unsigned> codes;
             int prev = -1;   // previous row whose codes are in `codes`
             for (int sl = s0; sl < s1; ++sl) {
-                const int L4 = (ptr[sl + 1] - ptr[sl]) / 64;
+                const int L4 = (wp[sl + 1] - wp[sl]) / 64 * 4;   // the slots the code words cover
                 for (int lane = 0; lane < 64; ++lane) {
                     const int r = sl * 64 + lane;
                     const int l = r < n ? len[r] : 0, j0 = r < n ? A->h_row[r] : 0;
@@ -1010,7 +1015,12 @@ int spmv_configure(hfh_csr *A, int algo, int p1, int p2) {
         A->spmv_algo = HFH_SPMV_SELL_DICT;
         A->spmv_p1 = p1 == 2 ? 2 : 1;
         const long long need = (nsl + kBlock / 64 - 1) / (kBlock / 64);
-        const long long g = p2 > 0 ? std::min<long long>(p2, need) : need;
+        // FP32 rows are light: waves that take several slices each (8
+        // workgroups per CU) beat one slice per wave (FV 192^3: 60.6 us
+        // against 70.1 us); FP64 keeps one slice per wave (103.7 us against
+        // 115.8 us at 8 workgroups per CU)
+        const long long dflt = A->dtype == HFH_F32 ? std::min<long long>(need, static_cast<long long>(cus) * 8) : need;
+        const long long g = p2 > 0 ? std::min<long long>(p2, need) : dflt;
         A->nwg = round8(std::max<long long>(g, 1));
         A->sell_dirty = true;  // values filled from the device CSR by spmv_sell_fill
     } else if (algo == HFH_SPMV_WAVE) {

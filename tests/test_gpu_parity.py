@@ -845,3 +845,23 @@ def test_solver_3x3_known_answer(pkg, ctx):
         X = pkg.Vector(ctx, 3)
         solve(X)
         assert np.allclose(X.numpy(), [-0.25, -1.5, -0.75], atol=1e-12)
+
+
+def test_blas1_beyond_cache_bitwise(pkg, ctx):
+    """Axpy / ScaleAdd / Scale on vectors larger than the 256 MB Infinity Cache
+    run with non-temporal streams from more workgroups: the same bits as the
+    separately rounded CPU loops."""
+    n = (1 << 25) + 5                      # 2 x 256 MiB + 80 B of FP64
+    rng = np.random.default_rng(21)
+    x = rng.uniform(-1, 1, n)
+    y = rng.uniform(-1, 1, n)
+    X, Y = pkg.Vector(ctx, x), pkg.Vector(ctx, y)
+    a = 0.37
+    Y.axpy(X, a)
+    ref = y + a * x                         # product rounded, then the sum: no contraction
+    assert np.array_equal(Y.numpy(), ref)
+    Y.scaleadd(-1.7, X)
+    ref = -1.7 * ref + x
+    assert np.array_equal(Y.numpy(), ref)
+    Y.scale(0.5)
+    assert np.array_equal(Y.numpy(), ref * 0.5)
