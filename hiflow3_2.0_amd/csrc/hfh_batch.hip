@@ -67,6 +67,7 @@ enum : int {
 
 struct SysDesc {
     int n, nslice, ntiles, ndict;   // ndict > 0: coded columns
+    int small;                      // solved by kb_small, not by the tile and slot kernels
     const int *sl_ptr;
     const int *sl_len;
     const int *sl_col;
@@ -222,6 +223,37 @@ __device__ __forceinline__ T sell_row_coded(const GDesc &D, gptr<const T> val, g
 __device__ __forceinline__ bool is_inner_spmv(int ph) { return ph == P_START || ph == P_GO; }
 __device__ __forceinline__ bool is_outer_spmv(int ph) { return ph == P_FIRST || ph == P_CORRECT; }
 
+// One tile's share of K1 for a group of kBlock threads (wave 0..3 of the
+// group, wave-uniform): the inner SpMV q = A32 p or the FP64 true residual
+// r64 = b - A64 x over the tile's slices, and this thread's part of the
+// tile's partial p.q or r64.r64.
+__device__ __forceinline__ double spmv_tile(const GDesc &D, const Tile &T, bool inner, int wave, int lane,
+                                            const int *sdict) {
+    const bool coded = D.ndict > 0;
+    double dsum = 0;
+    for (int s = T.s0 + wave; s < T.s0 + T.ns; s += kBlock / 64) {
+        const int r = s * 64 + lane;
+        const int len = r < D.n ? D.sl_len[r] : 0;
+        if (inner) {
+            const float acc = coded ? sell_row_coded<float, float>(D, D.v32, gptr<const float>(D.p), s, lane, len, r, sdict)
+                                    : sell_row<float, float>(D, D.v32, gptr<const float>(D.p), s, lane, len);
+            if (r < D.n) {
+                D.q[r] = acc;
+                dsum += static_cast<double>(D.p[r]) * static_cast<double>(acc);
+            }
+        } else {
+            const double acc = coded ? sell_row_coded<double, double>(D, D.v64, gptr<const double>(D.x), s, lane, len, r, sdict)
+                                     : sell_row<double, double>(D, D.v64, gptr<const double>(D.x), s, lane, len);
+            if (r < D.n) {
+                const double rr = D.b[r] - acc;
+                D.r64[r] = rr;
+                dsum += rr * rr;
+            }
+        }
+    }
+    return dsum;
+}
+
 // K1: inner SpMV q = A32 p with p.q partials, or the FP64 true residual.
 __global__ __launch_bounds__(kBlock) void kb_spmv(const SysDesc *__restrict__ sd, const SysState *__restrict__ st,
                                                   const Tile *__restrict__ tiles, const int *__restrict__ ntiles) {
@@ -236,33 +268,11 @@ __global__ __launch_bounds__(kBlock) void kb_spmv(const SysDesc *__restrict__ sd
         const bool inner = is_inner_spmv(ph);
         if (!inner && !is_outer_spmv(ph)) continue;   // uniform over the workgroup
         const GDesc D = gview(sd[T.sys]);
-        const bool coded = D.ndict > 0;
-        if (coded) {   // the system's offset table (the previous tile's readers passed the barrier below)
+        if (D.ndict > 0) {   // the system's offset table (the previous tile's readers passed the barrier below)
             for (int i = threadIdx.x; i < D.ndict; i += kBlock) sdict[i] = D.dict[i];
             __syncthreads();
         }
-        double dsum = 0;
-        for (int s = T.s0 + wave; s < T.s0 + T.ns; s += kBlock / 64) {
-            const int r = s * 64 + lane;
-            const int len = r < D.n ? D.sl_len[r] : 0;
-            if (inner) {
-                const float acc = coded ? sell_row_coded<float, float>(D, D.v32, gptr<const float>(D.p), s, lane, len, r, sdict)
-                                        : sell_row<float, float>(D, D.v32, gptr<const float>(D.p), s, lane, len);
-                if (r < D.n) {
-                    D.q[r] = acc;
-                    dsum += static_cast<double>(D.p[r]) * static_cast<double>(acc);
-                }
-            } else {
-                const double acc = coded ? sell_row_coded<double, double>(D, D.v64, gptr<const double>(D.x), s, lane, len, r, sdict)
-                                         : sell_row<double, double>(D, D.v64, gptr<const double>(D.x), s, lane, len);
-                if (r < D.n) {
-                    const double rr = D.b[r] - acc;
-                    D.r64[r] = rr;
-                    dsum += rr * rr;
-                }
-            }
-        }
-        const double tot = block_sum(dsum, sh);
+        const double tot = block_sum(spmv_tile(D, T, inner, wave, lane, sdict), sh);
         if (threadIdx.x == 0) D.part[2 * T.lt] = tot;
         __syncthreads();   // sh is reused by the next tile
     }
@@ -277,20 +287,11 @@ __device__ __forceinline__ double sum_parts(const GDesc &D, int q, double *sh) {
     return tot;
 }
 
-// K2: per system, after K1.
-__global__ __launch_bounds__(kBlock) void kb_fin1(const SysDesc *__restrict__ sd, SysState *__restrict__ st,
-                                                  int nslots, KParams prm) {
-    __shared__ double sh[kBlock / 64];
-    const int s = blockIdx.x;
-    if (s >= nslots) return;
-    const int ph = st[s].phase;
-    const bool inner = is_inner_spmv(ph);
-    if (!inner && !is_outer_spmv(ph)) return;
-    const GDesc D = gview(sd[s]);
-    const double tot = sum_parts(D, 0, sh);
-    if (threadIdx.x != 0) return;
-    SysState &S = st[s];
-    if (inner) {
+// The decision after K1 from the sum `tot` of its partials: inner, alpha or
+// a breakdown; outer, the true residual's test.
+__device__ __forceinline__ void decide_after_spmv(SysState &S, double tot, const KParams &prm) {
+    const int ph = S.phase;
+    if (is_inner_spmv(ph)) {
         if (tot > 0 && isfinite(tot)) {
             S.alpha = S.rz / tot;
             S.phase = P_UPD;
@@ -320,6 +321,80 @@ __global__ __launch_bounds__(kBlock) void kb_fin1(const SysDesc *__restrict__ sd
     }
 }
 
+// K2: per system, after K1.
+__global__ __launch_bounds__(kBlock) void kb_fin1(const SysDesc *__restrict__ sd, SysState *__restrict__ st,
+                                                  int nslots, KParams prm) {
+    __shared__ double sh[kBlock / 64];
+    const int s = blockIdx.x;
+    if (s >= nslots) return;
+    const int ph = st[s].phase;
+    if (!is_inner_spmv(ph) && !is_outer_spmv(ph)) return;
+    if (sd[s].small) return;   // kb_small's
+    const GDesc D = gview(sd[s]);
+    const double tot = sum_parts(D, 0, sh);
+    if (threadIdx.x == 0) decide_after_spmv(st[s], tot, prm);
+}
+
+// One tile's share of K3 for a group of kBlock threads (tid 0..kBlock-1):
+// the inner update with alpha, or the start of an inner solve, and this
+// thread's parts of the tile's r.r and r.z.
+__device__ __forceinline__ void update_tile(const GDesc &D, const Tile &T, int ph, float a, int tid, double &rr,
+                                            double &rz) {
+    const int r0 = T.s0 * 64, r1 = min(D.n, (T.s0 + T.ns) * 64);
+    // 4 rows per thread and step with 16-byte accesses (a tile starts at a
+    // multiple of 2048 rows and every array at a 256-byte boundary), the
+    // rows past the last multiple of 4 one by one
+    auto one = [&](float &ci, float &ri_io, float pi, float qi, double r64i, float di, float &zi) {
+        float ri;
+        if (ph == P_UPD) {
+            ci = ci + a * pi;
+            ri = ri_io - a * qi;
+        } else {
+            ri = static_cast<float>(r64i);
+            ci = 0.0f;
+        }
+        ri_io = ri;
+        zi = di * ri;
+        rr += static_cast<double>(ri) * static_cast<double>(ri);
+        rz += static_cast<double>(ri) * static_cast<double>(zi);
+    };
+    const int rv = r0 + ((r1 - r0) & ~3);
+    for (int i = r0 + 4 * tid; i < rv; i += 4 * kBlock) {
+        bf4 c4 = ph == P_UPD ? *reinterpret_cast<gptr<const bf4>>(&D.c[i]) : bf4(0.0f);
+        bf4 r4 = ph == P_UPD ? *reinterpret_cast<gptr<const bf4>>(&D.r[i]) : bf4(0.0f);
+        bf4 p4 = bf4(0.0f), q4 = bf4(0.0f);
+        bd2 ra = bd2(0.0), rb = bd2(0.0);
+        if (ph == P_UPD) {
+            p4 = *reinterpret_cast<gptr<const bf4>>(&D.p[i]);
+            q4 = *reinterpret_cast<gptr<const bf4>>(&D.q[i]);
+        } else {
+            ra = *reinterpret_cast<gptr<const bd2>>(&D.r64[i]);
+            rb = *reinterpret_cast<gptr<const bd2>>(&D.r64[i + 2]);
+        }
+        const bf4 d4 = *reinterpret_cast<gptr<const bf4>>(&D.invd[i]);
+        bf4 z4;
+#pragma unroll
+        for (int w = 0; w < 4; ++w) {
+            float ci = c4[w], ri = r4[w], zi;
+            one(ci, ri, p4[w], q4[w], w < 2 ? ra[w] : rb[w - 2], d4[w], zi);
+            c4[w] = ci;
+            r4[w] = ri;
+            z4[w] = zi;
+        }
+        *reinterpret_cast<gptr<bf4>>(&D.c[i]) = c4;
+        *reinterpret_cast<gptr<bf4>>(&D.r[i]) = r4;
+        *reinterpret_cast<gptr<bf4>>(&D.z[i]) = z4;
+    }
+    for (int i = rv + tid; i < r1; i += kBlock) {
+        float ci = ph == P_UPD ? D.c[i] : 0.0f, ri = ph == P_UPD ? D.r[i] : 0.0f, zi;
+        one(ci, ri, ph == P_UPD ? D.p[i] : 0.0f, ph == P_UPD ? D.q[i] : 0.0f, ph == P_UPD ? 0.0 : D.r64[i],
+            D.invd[i], zi);
+        D.c[i] = ci;
+        D.r[i] = ri;
+        D.z[i] = zi;
+    }
+}
+
 // K3: inner update, or the start of an inner solve; partials r.r and r.z.
 __global__ __launch_bounds__(kBlock) void kb_update(const SysDesc *__restrict__ sd, const SysState *__restrict__ st,
                                                     const Tile *__restrict__ tiles, const int *__restrict__ ntiles) {
@@ -330,61 +405,8 @@ __global__ __launch_bounds__(kBlock) void kb_update(const SysDesc *__restrict__ 
         const int ph = st[T.sys].phase;
         if (ph != P_UPD && ph != P_SETUP) continue;
         const GDesc D = gview(sd[T.sys]);
-        const float a = static_cast<float>(st[T.sys].alpha);
-        const int r0 = T.s0 * 64, r1 = min(D.n, (T.s0 + T.ns) * 64);
         double rr = 0, rz = 0;
-        // 4 rows per thread and step with 16-byte accesses (a tile starts at a
-        // multiple of 2048 rows and every array at a 256-byte boundary), the
-        // rows past the last multiple of 4 one by one
-        auto one = [&](float &ci, float &ri_io, float pi, float qi, double r64i, float di, float &zi) {
-            float ri;
-            if (ph == P_UPD) {
-                ci = ci + a * pi;
-                ri = ri_io - a * qi;
-            } else {
-                ri = static_cast<float>(r64i);
-                ci = 0.0f;
-            }
-            ri_io = ri;
-            zi = di * ri;
-            rr += static_cast<double>(ri) * static_cast<double>(ri);
-            rz += static_cast<double>(ri) * static_cast<double>(zi);
-        };
-        const int rv = r0 + ((r1 - r0) & ~3);
-        for (int i = r0 + 4 * threadIdx.x; i < rv; i += 4 * kBlock) {
-            bf4 c4 = ph == P_UPD ? *reinterpret_cast<gptr<const bf4>>(&D.c[i]) : bf4(0.0f);
-            bf4 r4 = ph == P_UPD ? *reinterpret_cast<gptr<const bf4>>(&D.r[i]) : bf4(0.0f);
-            bf4 p4 = bf4(0.0f), q4 = bf4(0.0f);
-            bd2 ra = bd2(0.0), rb = bd2(0.0);
-            if (ph == P_UPD) {
-                p4 = *reinterpret_cast<gptr<const bf4>>(&D.p[i]);
-                q4 = *reinterpret_cast<gptr<const bf4>>(&D.q[i]);
-            } else {
-                ra = *reinterpret_cast<gptr<const bd2>>(&D.r64[i]);
-                rb = *reinterpret_cast<gptr<const bd2>>(&D.r64[i + 2]);
-            }
-            const bf4 d4 = *reinterpret_cast<gptr<const bf4>>(&D.invd[i]);
-            bf4 z4;
-#pragma unroll
-            for (int w = 0; w < 4; ++w) {
-                float ci = c4[w], ri = r4[w], zi;
-                one(ci, ri, p4[w], q4[w], w < 2 ? ra[w] : rb[w - 2], d4[w], zi);
-                c4[w] = ci;
-                r4[w] = ri;
-                z4[w] = zi;
-            }
-            *reinterpret_cast<gptr<bf4>>(&D.c[i]) = c4;
-            *reinterpret_cast<gptr<bf4>>(&D.r[i]) = r4;
-            *reinterpret_cast<gptr<bf4>>(&D.z[i]) = z4;
-        }
-        for (int i = rv + threadIdx.x; i < r1; i += kBlock) {
-            float ci = ph == P_UPD ? D.c[i] : 0.0f, ri = ph == P_UPD ? D.r[i] : 0.0f, zi;
-            one(ci, ri, ph == P_UPD ? D.p[i] : 0.0f, ph == P_UPD ? D.q[i] : 0.0f, ph == P_UPD ? 0.0 : D.r64[i],
-                D.invd[i], zi);
-            D.c[i] = ci;
-            D.r[i] = ri;
-            D.z[i] = zi;
-        }
+        update_tile(D, T, ph, static_cast<float>(st[T.sys].alpha), threadIdx.x, rr, rz);
         const double trr = block_sum(rr, sh);
         __syncthreads();
         const double trz = block_sum(rz, sh);
@@ -396,21 +418,10 @@ __global__ __launch_bounds__(kBlock) void kb_update(const SysDesc *__restrict__ 
     }
 }
 
-// K4: per system, after K3.
-__global__ __launch_bounds__(kBlock) void kb_fin2(const SysDesc *__restrict__ sd, SysState *__restrict__ st,
-                                                  int nslots, KParams prm) {
-    __shared__ double sh[kBlock / 64];
-    const int s = blockIdx.x;
-    if (s >= nslots) return;
-    const int ph = st[s].phase;
-    if (ph != P_UPD && ph != P_SETUP) return;
-    const GDesc D = gview(sd[s]);
-    const double rr = sum_parts(D, 0, sh);
-    const double rz = sum_parts(D, 1, sh);
-    if (threadIdx.x != 0) return;
-    SysState &S = st[s];
+// The decision after K3 from its r.r and r.z sums.
+__device__ __forceinline__ void decide_after_update(SysState &S, double rr, double rz, const KParams &prm) {
     const double rn = std::sqrt(rr);
-    if (ph == P_SETUP) {
+    if (S.phase == P_SETUP) {
         S.rr0 = rn;
         S.rz = rz;
         S.inner = 0;
@@ -428,6 +439,57 @@ __global__ __launch_bounds__(kBlock) void kb_fin2(const SysDesc *__restrict__ sd
     }
 }
 
+// K4: per system, after K3.
+__global__ __launch_bounds__(kBlock) void kb_fin2(const SysDesc *__restrict__ sd, SysState *__restrict__ st,
+                                                  int nslots, KParams prm) {
+    __shared__ double sh[kBlock / 64];
+    const int s = blockIdx.x;
+    if (s >= nslots) return;
+    const int ph = st[s].phase;
+    if (ph != P_UPD && ph != P_SETUP) return;
+    if (sd[s].small) return;   // kb_small's
+    const GDesc D = gview(sd[s]);
+    const double rr = sum_parts(D, 0, sh);
+    const double rz = sum_parts(D, 1, sh);
+    if (threadIdx.x == 0) decide_after_update(st[s], rr, rz, prm);
+}
+
+// One tile's share of K5 for a group of kBlock threads: the search
+// direction, or the FP64 correction x += c.
+__device__ __forceinline__ void dir_tile(const GDesc &D, const Tile &T, int ph, float beta, int tid) {
+    const int r0 = T.s0 * 64, r1 = min(D.n, (T.s0 + T.ns) * 64);
+    const int rv = r0 + ((r1 - r0) & ~3);   // 16-byte accesses up to here, as in update_tile
+    if (ph == P_GO) {
+        for (int i = r0 + 4 * tid; i < rv; i += 4 * kBlock) {
+            const bf4 z4 = *reinterpret_cast<gptr<const bf4>>(&D.z[i]);
+            bf4 p4 = *reinterpret_cast<gptr<const bf4>>(&D.p[i]);
+            p4.x = z4.x + beta * p4.x;
+            p4.y = z4.y + beta * p4.y;
+            p4.z = z4.z + beta * p4.z;
+            p4.w = z4.w + beta * p4.w;
+            *reinterpret_cast<gptr<bf4>>(&D.p[i]) = p4;
+        }
+        for (int i = rv + tid; i < r1; i += kBlock) D.p[i] = D.z[i] + beta * D.p[i];
+    } else if (ph == P_START) {
+        for (int i = r0 + 4 * tid; i < rv; i += 4 * kBlock)
+            *reinterpret_cast<gptr<bf4>>(&D.p[i]) = *reinterpret_cast<gptr<const bf4>>(&D.z[i]);
+        for (int i = rv + tid; i < r1; i += kBlock) D.p[i] = D.z[i];
+    } else {
+        for (int i = r0 + 4 * tid; i < rv; i += 4 * kBlock) {
+            const bf4 c4 = *reinterpret_cast<gptr<const bf4>>(&D.c[i]);
+            bd2 xa = *reinterpret_cast<gptr<const bd2>>(&D.x[i]);
+            bd2 xb = *reinterpret_cast<gptr<const bd2>>(&D.x[i + 2]);
+            xa.x += static_cast<double>(c4.x);
+            xa.y += static_cast<double>(c4.y);
+            xb.x += static_cast<double>(c4.z);
+            xb.y += static_cast<double>(c4.w);
+            *reinterpret_cast<gptr<bd2>>(&D.x[i]) = xa;
+            *reinterpret_cast<gptr<bd2>>(&D.x[i + 2]) = xb;
+        }
+        for (int i = rv + tid; i < r1; i += kBlock) D.x[i] += static_cast<double>(D.c[i]);
+    }
+}
+
 // K5: search direction, or the FP64 correction x += c.
 __global__ __launch_bounds__(kBlock) void kb_dir(const SysDesc *__restrict__ sd, const SysState *__restrict__ st,
                                                  const Tile *__restrict__ tiles, const int *__restrict__ ntiles) {
@@ -436,39 +498,99 @@ __global__ __launch_bounds__(kBlock) void kb_dir(const SysDesc *__restrict__ sd,
         const Tile T = tiles[t];
         const int ph = st[T.sys].phase;
         if (ph != P_GO && ph != P_START && ph != P_CORRECT) continue;
-        const GDesc D = gview(sd[T.sys]);
-        const int r0 = T.s0 * 64, r1 = min(D.n, (T.s0 + T.ns) * 64);
-        const int rv = r0 + ((r1 - r0) & ~3);   // 16-byte accesses up to here, as in kb_update
-        if (ph == P_GO) {
-            const float beta = static_cast<float>(st[T.sys].beta);
-            for (int i = r0 + 4 * threadIdx.x; i < rv; i += 4 * kBlock) {
-                const bf4 z4 = *reinterpret_cast<gptr<const bf4>>(&D.z[i]);
-                bf4 p4 = *reinterpret_cast<gptr<const bf4>>(&D.p[i]);
-                p4.x = z4.x + beta * p4.x;
-                p4.y = z4.y + beta * p4.y;
-                p4.z = z4.z + beta * p4.z;
-                p4.w = z4.w + beta * p4.w;
-                *reinterpret_cast<gptr<bf4>>(&D.p[i]) = p4;
+        dir_tile(gview(sd[T.sys]), T, ph, static_cast<float>(st[T.sys].beta), threadIdx.x);
+    }
+}
+
+// Sum of a group of kBlock threads (the 4 waves from `gw`), with block_sum's
+// order: wave sums, then 0 + w0 + w1 + w2 + w3; valid in the group's first
+// thread.  Every thread of the workgroup calls it (it holds a barrier).
+__device__ __forceinline__ double group_sum(double v, double *gsh /* kBlock / 64 */, int gwave, int lane) {
+    v = wave_sum(v);
+    if (lane == 0) gsh[gwave] = v;
+    __syncthreads();
+    double s = 0;
+    for (int i = 0; i < kBlock / 64; ++i) s += gsh[i];
+    return s;
+}
+
+// Systems of at most kSmallTiles tiles: one workgroup per system runs all
+// `iters` iterations of a step, K1 to K5 with barriers in place of launches.
+// Group g of kBlock threads does what the tile kernels' workgroup does for
+// the system's tile g, and group 0 what the slot kernels do, in the same
+// order of operations, so the results are those of the tile path bit for bit.
+constexpr int kSmallTiles = 4;
+constexpr int kSmallBlock = kSmallTiles * kBlock;
+// iterations of kb_small per step, in chunks: a small system's iterations are
+// short, and the step lasts as long as the tile launches anyway
+constexpr int kSmallChunks = 4;
+
+__global__ __launch_bounds__(kSmallBlock) void kb_small(const SysDesc *__restrict__ sd, SysState *__restrict__ st,
+                                                        const int *__restrict__ small, int nsmall, int iters,
+                                                        KParams prm) {
+    __shared__ double gsh[kSmallTiles][kBlock / 64];
+    __shared__ int sdict[256];
+    __shared__ SysState S;
+    const int g = __builtin_amdgcn_readfirstlane(threadIdx.x / kBlock);
+    const int tid = threadIdx.x % kBlock;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+    for (int k = blockIdx.x; k < nsmall; k += gridDim.x) {
+        const int slot = small[k];
+        const GDesc D = gview(sd[slot]);
+        for (int i = threadIdx.x; i < D.ndict; i += kSmallBlock) sdict[i] = D.dict[i];
+        if (threadIdx.x == 0) S = st[slot];
+        __syncthreads();
+        const bool mine = g < D.ntiles;   // this group's tile exists
+        const Tile T{slot, g * kTileSlices, min(kTileSlices, D.nslice - g * kTileSlices), g};
+        for (int it = 0; it < iters; ++it) {
+            int ph = S.phase;
+            if (ph == P_DONE || ph == P_IDLE) break;   // uniform
+            if (is_inner_spmv(ph) || is_outer_spmv(ph)) {
+                // K1, then K2 (group 0, the slot kernel's order)
+                const double tot = group_sum(mine ? spmv_tile(D, T, is_inner_spmv(ph), wave, lane, sdict) : 0.0,
+                                             gsh[g], wave, lane);
+                if (tid == 0 && mine) D.part[2 * g] = tot;
+                __syncthreads();
+                double v = 0;
+                if (g == 0)
+                    for (int i = tid; i < D.ntiles; i += kBlock) v += D.part[2 * i];
+                const double sum = group_sum(v, gsh[g], wave, lane);
+                if (threadIdx.x == 0) decide_after_spmv(S, sum, prm);
+                __syncthreads();
+                ph = S.phase;
             }
-            for (int i = rv + threadIdx.x; i < r1; i += kBlock) D.p[i] = D.z[i] + beta * D.p[i];
-        } else if (ph == P_START) {
-            for (int i = r0 + 4 * threadIdx.x; i < rv; i += 4 * kBlock)
-                *reinterpret_cast<gptr<bf4>>(&D.p[i]) = *reinterpret_cast<gptr<const bf4>>(&D.z[i]);
-            for (int i = rv + threadIdx.x; i < r1; i += kBlock) D.p[i] = D.z[i];
-        } else {
-            for (int i = r0 + 4 * threadIdx.x; i < rv; i += 4 * kBlock) {
-                const bf4 c4 = *reinterpret_cast<gptr<const bf4>>(&D.c[i]);
-                bd2 xa = *reinterpret_cast<gptr<const bd2>>(&D.x[i]);
-                bd2 xb = *reinterpret_cast<gptr<const bd2>>(&D.x[i + 2]);
-                xa.x += static_cast<double>(c4.x);
-                xa.y += static_cast<double>(c4.y);
-                xb.x += static_cast<double>(c4.z);
-                xb.y += static_cast<double>(c4.w);
-                *reinterpret_cast<gptr<bd2>>(&D.x[i]) = xa;
-                *reinterpret_cast<gptr<bd2>>(&D.x[i + 2]) = xb;
+            if (ph == P_UPD || ph == P_SETUP) {
+                // K3, then K4 (group 0)
+                double rr = 0, rz = 0;
+                if (mine) update_tile(D, T, ph, static_cast<float>(S.alpha), tid, rr, rz);
+                const double trr = group_sum(rr, gsh[g], wave, lane);
+                __syncthreads();
+                const double trz = group_sum(rz, gsh[g], wave, lane);
+                if (tid == 0 && mine) {
+                    D.part[2 * g] = trr;
+                    D.part[2 * g + 1] = trz;
+                }
+                __syncthreads();
+                double v0 = 0, v1 = 0;
+                if (g == 0)
+                    for (int i = tid; i < D.ntiles; i += kBlock) {
+                        v0 += D.part[2 * i];
+                        v1 += D.part[2 * i + 1];
+                    }
+                const double srr = group_sum(v0, gsh[g], wave, lane);
+                __syncthreads();
+                const double srz = group_sum(v1, gsh[g], wave, lane);
+                if (threadIdx.x == 0) decide_after_update(S, srr, srz, prm);
+                __syncthreads();
+                ph = S.phase;
             }
-            for (int i = rv + threadIdx.x; i < r1; i += kBlock) D.x[i] += static_cast<double>(D.c[i]);
+            if (ph == P_GO || ph == P_START || ph == P_CORRECT) {
+                if (mine) dir_tile(D, T, ph, static_cast<float>(S.beta), tid);
+                __syncthreads();   // the next SpMV gathers p (or x) across the groups
+            }
         }
+        if (threadIdx.x == 0) st[slot] = S;
+        __syncthreads();
     }
 }
 
@@ -669,6 +791,12 @@ struct hfh_batch {
     hfh_ctx *ctx = nullptr;
     hfh_batch_params prm{};
     hipStream_t cs = nullptr;        // uploads, device assembly and solution copies
+    hipStream_t ss = nullptr;        // kb_small, beside the tile launches on the context stream
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    int small_tiles = 0;             // systems of at most this many tiles run in kb_small
+    int *d_small = nullptr;          // slots of the small systems (kb_small's list)
+    int *h_small[2] = {nullptr, nullptr};
+    int nsmall_now = 0, ntiles_now = 0;   // the tables the next replay reads
     char *arena_d = nullptr;
     Arena arena;
     SysDesc *d_sys = nullptr;
@@ -680,6 +808,7 @@ struct hfh_batch {
         int id = -1;          // request id, -1 when free
         size_t off = 0, bytes = 0;
         int n = 0, nslice = 0, ntiles = 0;
+        bool small = false;   // solved by kb_small
         bool reported = false;
         double *x = nullptr;  // device solution vector
         SysState final_state{};
@@ -755,6 +884,7 @@ int activate(hfh_batch *B, int slot, size_t off, const Layout &L, int n, int nsl
     d.n = n;
     d.nslice = nslice;
     d.ntiles = tiles_of(nslice);
+    d.small = d.ntiles <= B->small_tiles;
     d.sl_ptr = reinterpret_cast<const int *>(base + L.ptr);
     d.sl_len = reinterpret_cast<const int *>(base + L.len);
     if (L.words) {
@@ -794,6 +924,7 @@ int activate(hfh_batch *B, int slot, size_t off, const Layout &L, int n, int nsl
     S.n = n;
     S.nslice = nslice;
     S.ntiles = d.ntiles;
+    S.small = d.small != 0;
     S.reported = false;
     S.x = d.x;
     B->slot_of[S.id] = slot;
@@ -808,10 +939,16 @@ int upload_tiles(hfh_batch *B) {
     // the copy from this buffer two uploads ago must have left it
     HFH_HIP(hipEventSynchronize(B->ev_tiles[buf]));
     Tile *ht = B->h_tiles[buf];
+    int *hs = B->h_small[buf];
     long long nt = 0;
+    int ns = 0;
     for (size_t k = 0; k < B->slots.size(); ++k) {
         const hfh_batch::Slot &S = B->slots[k];
         if (S.id < 0) continue;
+        if (S.small) {
+            hs[ns++] = static_cast<int>(k);
+            continue;
+        }
         for (int t = 0; t < S.ntiles; ++t) {
             HFH_CHECK(nt < B->tile_cap, HFH_ERR_INVALID, "hfh_batch: tile table full");
             const int s0 = t * kTileSlices;
@@ -820,6 +957,9 @@ int upload_tiles(hfh_batch *B) {
     }
     *B->h_ntiles[buf] = static_cast<int>(nt);
     if (nt) HFH_HIP(hipMemcpyAsync(B->d_tiles, ht, sizeof(Tile) * nt, hipMemcpyHostToDevice, s));
+    if (ns) HFH_HIP(hipMemcpyAsync(B->d_small, hs, sizeof(int) * ns, hipMemcpyHostToDevice, s));
+    B->nsmall_now = ns;
+    B->ntiles_now = static_cast<int>(nt);
     HFH_HIP(hipMemcpyAsync(B->d_ntiles, B->h_ntiles[buf], sizeof(int), hipMemcpyHostToDevice, s));
     HFH_HIP(hipEventRecord(B->ev_tiles[buf], s));
     B->tb ^= 1;
@@ -874,7 +1014,9 @@ void release_all(hfh_batch *B) {
     if (B->d_st) (void)hipFree(B->d_st);
     if (B->d_tiles) (void)hipFree(B->d_tiles);
     if (B->d_ntiles) (void)hipFree(B->d_ntiles);
+    if (B->d_small) (void)hipFree(B->d_small);
     for (int b = 0; b < 2; ++b) {
+        if (B->h_small[b]) (void)hipHostFree(B->h_small[b]);
         if (B->h_st[b]) (void)hipHostFree(B->h_st[b]);
         if (B->h_tiles[b]) (void)hipHostFree(B->h_tiles[b]);
         if (B->h_ntiles[b]) (void)hipHostFree(B->h_ntiles[b]);
@@ -884,8 +1026,11 @@ void release_all(hfh_batch *B) {
         if (B->ev_tiles[b]) (void)hipEventDestroy(B->ev_tiles[b]);
     }
     if (B->ev_up) (void)hipEventDestroy(B->ev_up);
+    if (B->ev_fork) (void)hipEventDestroy(B->ev_fork);
+    if (B->ev_join) (void)hipEventDestroy(B->ev_join);
     if (B->stage) (void)hipHostFree(B->stage);
     if (B->cs) (void)hipStreamDestroy(B->cs);
+    if (B->ss) (void)hipStreamDestroy(B->ss);
 }
 
 }  // namespace
@@ -899,11 +1044,14 @@ extern "C" int hfh_batch_create(hfh_ctx *ctx, const hfh_batch_params *p, hfh_bat
               p->chunk);
     HFH_CHECK(p->outer_maxits > 0 && p->inner_maxits > 0 && p->rtol >= 0 && p->atol >= 0 && p->inner_rtol >= 0,
               HFH_ERR_INVALID, "hfh_batch_create: bad tolerances or iteration limits");
+    HFH_CHECK(p->small_tiles >= 0 && p->small_tiles <= kSmallTiles, HFH_ERR_INVALID,
+              "hfh_batch_create: small_tiles %d not in [0, %d]", p->small_tiles, kSmallTiles);
     *out = nullptr;
     hfh_batch *B = new hfh_batch();
     B->ctx = ctx;
     B->prm = *p;
     B->slots.resize(p->max_systems);
+    B->small_tiles = p->small_tiles;   // 0: every system in the tile launches
     // every row costs at least 68 bytes of the region (one entry), so this
     // bounds the tiles the budget can hold
     B->tile_cap = p->hbm_budget / (68LL * kTileRows) + p->max_systems + 1;
@@ -917,15 +1065,20 @@ extern "C" int hfh_batch_create(hfh_ctx *ctx, const hfh_batch_params *p, hfh_bat
         (rc = dev_alloc(reinterpret_cast<void **>(&B->d_sys), sizeof(SysDesc) * p->max_systems)) ||
         (rc = dev_alloc(reinterpret_cast<void **>(&B->d_st), sizeof(SysState) * p->max_systems)) ||
         (rc = dev_alloc(reinterpret_cast<void **>(&B->d_tiles), sizeof(Tile) * B->tile_cap)) ||
-        (rc = dev_alloc(reinterpret_cast<void **>(&B->d_ntiles), sizeof(int))))
+        (rc = dev_alloc(reinterpret_cast<void **>(&B->d_ntiles), sizeof(int))) ||
+        (rc = dev_alloc(reinterpret_cast<void **>(&B->d_small), sizeof(int) * p->max_systems)))
         return fail(rc);
     B->arena.init(static_cast<size_t>(p->hbm_budget));
     bool ok = hipStreamCreateWithFlags(&B->cs, hipStreamNonBlocking) == hipSuccess &&
-              hipEventCreateWithFlags(&B->ev_up, hipEventDisableTiming) == hipSuccess;
+              hipStreamCreateWithFlags(&B->ss, hipStreamNonBlocking) == hipSuccess &&
+              hipEventCreateWithFlags(&B->ev_up, hipEventDisableTiming) == hipSuccess &&
+              hipEventCreateWithFlags(&B->ev_fork, hipEventDisableTiming) == hipSuccess &&
+              hipEventCreateWithFlags(&B->ev_join, hipEventDisableTiming) == hipSuccess;
     for (int b = 0; b < 2 && ok; ++b)
         ok = hipHostMalloc(reinterpret_cast<void **>(&B->h_st[b]), sizeof(SysState) * p->max_systems) == hipSuccess &&
              hipHostMalloc(reinterpret_cast<void **>(&B->h_tiles[b]), sizeof(Tile) * B->tile_cap) == hipSuccess &&
              hipHostMalloc(reinterpret_cast<void **>(&B->h_ntiles[b]), sizeof(int)) == hipSuccess &&
+             hipHostMalloc(reinterpret_cast<void **>(&B->h_small[b]), sizeof(int) * p->max_systems) == hipSuccess &&
              hipEventCreateWithFlags(&B->ev_rb[b], hipEventDisableTiming) == hipSuccess &&
              hipEventCreate(&B->ev_g0[b]) == hipSuccess && hipEventCreate(&B->ev_g1[b]) == hipSuccess &&
              hipEventCreateWithFlags(&B->ev_tiles[b], hipEventDisableTiming) == hipSuccess;
@@ -967,6 +1120,7 @@ extern "C" int hfh_batch_destroy(hfh_batch *B) {
     if (!B) return HFH_OK;
     (void)hipStreamSynchronize(B->ctx->stream);
     (void)hipStreamSynchronize(B->cs);
+    (void)hipStreamSynchronize(B->ss);
     --B->ctx->live;
     release_all(B);
     delete B;
@@ -1132,11 +1286,27 @@ extern "C" int hfh_batch_step(hfh_batch *B, int *active, int *nfinished) {
         launched = B->rb_next;
         B->rb_next ^= 1;
         HFH_HIP(hipEventRecord(B->ev_g0[launched], s));
-        if (B->exec) {
-            HFH_HIP(hipGraphLaunch(B->exec, s));
-        } else if ((rc = enqueue_iterations(B, B->prm.chunk))) {
-            return rc;
+        // the small systems' `chunk` iterations on the second stream, after
+        // the table uploads and slot writes on this one; joined below, so
+        // every later slot write on this stream follows them
+        if (B->nsmall_now > 0) {
+            const KParams kp{B->prm.rtol, B->prm.atol, B->prm.inner_rtol, B->prm.outer_maxits,
+                             B->prm.inner_maxits};
+            HFH_HIP(hipEventRecord(B->ev_fork, s));
+            HFH_HIP(hipStreamWaitEvent(B->ss, B->ev_fork, 0));
+            hipLaunchKernelGGL(kb_small, dim3(B->nsmall_now), dim3(kSmallBlock), 0, B->ss, B->d_sys, B->d_st,
+                               B->d_small, B->nsmall_now, B->prm.chunk * kSmallChunks, kp);
+            if ((rc = launch_check("kb_small"))) return rc;
+            HFH_HIP(hipEventRecord(B->ev_join, B->ss));
         }
+        if (B->ntiles_now > 0) {
+            if (B->exec) {
+                HFH_HIP(hipGraphLaunch(B->exec, s));
+            } else if ((rc = enqueue_iterations(B, B->prm.chunk))) {
+                return rc;
+            }
+        }
+        if (B->nsmall_now > 0) HFH_HIP(hipStreamWaitEvent(s, B->ev_join, 0));
         HFH_HIP(hipEventRecord(B->ev_g1[launched], s));
         ++B->steps;
         HFH_HIP(hipMemcpyAsync(B->h_st[launched], B->d_st, sizeof(SysState) * B->slots.size(),

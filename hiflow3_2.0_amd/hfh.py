@@ -49,7 +49,7 @@ class DcInfo(C.Structure):
 class BatchParams(C.Structure):
     _fields_ = [("max_systems", C.c_int), ("hbm_budget", C.c_longlong), ("chunk", C.c_int), ("rtol", C.c_double),
                 ("atol", C.c_double), ("outer_maxits", C.c_int), ("inner_rtol", C.c_double),
-                ("inner_maxits", C.c_int)]
+                ("inner_maxits", C.c_int), ("small_tiles", C.c_int)]
 
 
 class BatchResult(C.Structure):
@@ -727,10 +727,10 @@ class BatchEngine:
     the request does not fit the free slots / HBM budget now."""
 
     def __init__(self, ctx, max_systems=256, hbm_budget=1 << 30, chunk=16, rtol=1e-10, atol=0.0,
-                 outer_maxits=30, inner_rtol=1e-4, inner_maxits=5000):
+                 outer_maxits=30, inner_rtol=1e-4, inner_maxits=5000, small_tiles=0):
         self.ctx = ctx
         self.params = BatchParams(max_systems, int(hbm_budget), chunk, rtol, atol, outer_maxits, inner_rtol,
-                                  inner_maxits)
+                                  inner_maxits, small_tiles)
         h = C.c_void_p()
         _ck(hip().hfh_batch_create(ctx.h, C.byref(self.params), C.byref(h)))
         self.h = h

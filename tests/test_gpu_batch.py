@@ -189,3 +189,31 @@ def test_batch_rejects_bad_requests(pkg, ctx):
     with pytest.raises(pkg.HfhError):
         E.submit_fv(4, 4, 1, -np.ones(16), np.ones(16))
     E.close()
+
+
+def test_small_systems_persistent_equals_tiles(pkg, ctx):
+    """With small_tiles = 4, systems of at most 4 tiles (8,192 rows) run a
+    step's iterations in one persistent workgroup each (kb_small) beside the
+    tile launches of the larger ones.  Group g of the workgroup does what the
+    tile path does for tile g, in the same order: the results equal the tile
+    path's bit for bit, in a mix with a large system."""
+    specs = [(9, 7, 1, 31), (40, 40, 1, 32), (20, 20, 20, 33), (90, 90, 1, 34), (64, 64, 2, 35), (130, 120, 1, 36)]
+    data = {s: fv(pkg, *s) for s in specs}
+    assert (130 * 120 + 2047) // 2048 > 4 and (64 * 64 * 2 + 2047) // 2048 == 4
+    res = {}
+    for small_tiles in (4, 0):
+        E = engine(pkg, ctx, small_tiles=small_tiles)
+        ids = {}
+        for s in specs:
+            k, b = data[s][0], data[s][1]
+            ids[E.submit_fv(s[0], s[1], s[2], k, b)] = s
+        run_to_end(E)
+        res[small_tiles] = {ids[r]: E.collect(r, len(data[ids[r]][1])) for r in list(ids)}
+        E.close()
+    for s in specs:
+        xa, ia = res[4][s]
+        xb, ib = res[0][s]
+        print(f"small path case {s}: outer {ia['outer_iters']} inner {ia['inner_iters']}", flush=True)
+        assert ia["state"] in (pkg.SUCCESS_ABS, pkg.SUCCESS_REL)
+        assert np.array_equal(xa, xb), s
+        assert ia == ib, s
