@@ -281,11 +281,14 @@ def stream_ceiling(P, ctx, m=1 << 28, reps=10):
     beyond the 256 MB Infinity Cache): y *= a (one read, one write:
     copy-like), y += a x (two reads, one write: triad-like) and x . y (two
     reads: read-only, the mix closest to a SpMV, which reads 97 % of its
-    bytes), each the mean of `reps` calls."""
+    bytes), each the mean of `reps` calls; beside them the HIP runtime's own
+    fill (Zeros: write-only) and device-to-device copy (CopyFrom), which
+    bound what any kernel that writes can reach on this part."""
     x, y = P.Vector(ctx, m), P.Vector(ctx, m)
     out = {}
     for name, fn, nbytes in (("copy", lambda: y.scale(1.0), 16 * m), ("triad", lambda: y.axpy(x, 1.0), 24 * m),
-                             ("read", lambda: x.dot(y), 16 * m)):
+                             ("read", lambda: x.dot(y), 16 * m), ("write_runtime_fill", lambda: y.zeros(), 8 * m),
+                             ("copy_runtime", lambda: y.copy_from(x), 16 * m)):
         fn()
         ctx.sync()
         t0 = time.perf_counter()
