@@ -379,4 +379,42 @@ __device__ __forceinline__ void reduce_partials(const double *part, int stride, 
     }
 }
 
+// The sum reduce_partials<1> returns, computed by a kBlock-thread workgroup
+// (every workgroup of a kernel that needs it, in place of a finisher
+// launch).  Thread t stands in for the finisher's threads t + kBlock k,
+// k < kFinBlock / kBlock: those sit in lane t mod 64 of the finisher's waves
+// t / 64 + (kBlock / 64) k, so each stand-in's strided sum and wave tree are
+// the finisher's own and the wave totals meet in the same order: the result
+// has the finisher's bits.  Valid in every thread.
+__device__ __forceinline__ double fin_sum_all(const double *__restrict__ part, int nb,
+                                              double *sh /* kFinBlock/64 */, double *bc) {
+    constexpr int K = kFinBlock / kBlock;
+    double s[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) s[k] = 0;
+    for (int b0 = 0; b0 < nb; b0 += kFinBlock) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const int b = b0 + static_cast<int>(threadIdx.x) + kBlock * k;
+            if (b < nb) s[k] += part[b];
+        }
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const double t = wave_sum(s[k]);
+        if (lane == 0) sh[w + (kBlock / 64) * k] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0;
+        for (int i = 0; i < kFinBlock / 64; ++i) t += sh[i];
+        *bc = t;
+    }
+    __syncthreads();
+    const double v = *bc;
+    __syncthreads();
+    return v;
+}
+
 }  // namespace hfh

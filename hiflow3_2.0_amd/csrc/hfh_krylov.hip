@@ -829,20 +829,61 @@ extern "C" int hfh_pcg_solve_noalloc(const hfh_csr *A, hfh_prec *P, const hfh_ve
 //
 // Device: the Krylov basis, the preconditioned vectors, A z and the modified
 // Gram-Schmidt sweep.  MGS step i fuses the previous axpy with the next dot
-// (w -= h_{i-1} v_{i-1}; h_i = w.v_i), reading h from HBM, so the host issues
-// j+2 launches per step and reads back one Hessenberg column.  Host: the
-// Givens rotations, the residual estimate |g_{j+1}| and the back
-// substitution, in the reference's order.
+// (w -= h_{i-1} v_{i-1}; h_i = w.v_i): every workgroup of step i sums step
+// i-1's partials itself, in the one-workgroup finisher's order (the same
+// bits), while its first loads of w and v are in flight, so the host issues
+// j+3 launches per step (j+2 MGS steps, the normalisation) and reads back one
+// Hessenberg column.  Host: the Givens rotations, the residual estimate
+// |g_{j+1}| and the back substitution, in the reference's order.
 
 namespace hfh {
 
+// Elements per thread loaded before the workgroup knows h: their loads
+// overlap the partial sums' latency.
+constexpr int kMgsPre = 8;
+
+// With vprev: h_{i-1} = the previous step's partials pprev[0..np) summed as
+// k_fin_mgs sums them, rounded to T (workgroup 0 stores it at hout_prev),
+// and w -= h_{i-1} vprev.  Then the partials of w.vcur (w.w without vcur)
+// into part.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_mgs(int n, T *__restrict__ w, const T *__restrict__ vprev,
-                                                const T *__restrict__ vcur, const double *hprev, double *part) {
-    __shared__ double sh[kBlock / 64];
-    const T h = vprev ? static_cast<T>(-1) * static_cast<T>(*hprev) : static_cast<T>(0);
+                                                const T *__restrict__ vcur, const double *__restrict__ pprev, int np,
+                                                double *hout_prev, double *part) {
+    __shared__ double sh[kFinBlock / 64];
+    __shared__ double bc;
+    const long long stride = static_cast<long long>(gridDim.x) * kBlock;
+    const long long i0 = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x;
+    T wp[kMgsPre], vp[kMgsPre], cp[kMgsPre];
+#pragma unroll
+    for (int k = 0; k < kMgsPre; ++k) {
+        const long long i = i0 + k * stride;
+        const bool in = i < n;
+        wp[k] = in ? w[i] : static_cast<T>(0);
+        vp[k] = (vprev && in) ? vprev[i] : static_cast<T>(0);
+        cp[k] = (vcur && in) ? vcur[i] : static_cast<T>(0);
+    }
+    T h = 0;
+    if (vprev) {
+        const T hp = static_cast<T>(fin_sum_all(pprev, np, sh, &bc));
+        if (blockIdx.x == 0 && threadIdx.x == 0) *hout_prev = static_cast<double>(hp);
+        h = static_cast<T>(-1) * hp;
+    }
     double s = 0;
-    for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock) {
+#pragma unroll
+    for (int k = 0; k < kMgsPre; ++k) {
+        const long long i = i0 + k * stride;
+        if (i < n) {
+            T wi = wp[k];
+            if (vprev) {
+                wi += h * vp[k];
+                w[i] = wi;
+            }
+            const T other = vcur ? cp[k] : wi;
+            s += static_cast<double>(wi) * static_cast<double>(other);
+        }
+    }
+    for (long long i = i0 + kMgsPre * stride; i < n; i += stride) {
         T wi = w[i];
         if (vprev) {
             wi += h * vprev[i];
@@ -873,6 +914,29 @@ __global__ __launch_bounds__(kBlock) void k_scale_into(int n, T *__restrict__ v,
     const T s = static_cast<T>(1) / static_cast<T>(*h);
     for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock)
         v[i] = w[i] * s;
+}
+
+// v = w * (1 / h) with h = ||w|| from the last MGS step's partials of w.w
+// (k_fin_mgs's arithmetic, summed in every workgroup while its first loads
+// of w are in flight; workgroup 0 stores h at hout)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_norm_scale_into(int n, T *__restrict__ v, const T *__restrict__ w,
+                                                            const double *__restrict__ pw, int np, double *hout) {
+    __shared__ double sh[kFinBlock / 64];
+    __shared__ double bc;
+    const long long stride = static_cast<long long>(gridDim.x) * kBlock;
+    const long long i0 = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x;
+    T wp[kMgsPre];
+#pragma unroll
+    for (int k = 0; k < kMgsPre; ++k) wp[k] = i0 + k * stride < n ? w[i0 + k * stride] : static_cast<T>(0);
+    const T d = static_cast<T>(fin_sum_all(pw, np, sh, &bc));
+    const T h = static_cast<T>(sqrt(static_cast<double>(d)));
+    if (blockIdx.x == 0 && threadIdx.x == 0) *hout = static_cast<double>(h);
+    const T s = static_cast<T>(1) / h;
+#pragma unroll
+    for (int k = 0; k < kMgsPre; ++k)
+        if (i0 + k * stride < n) v[i0 + k * stride] = wp[k] * s;
+    for (long long i = i0 + kMgsPre * stride; i < n; i += stride) v[i] = w[i] * s;
 }
 
 constexpr int kMaxBasis = 256;
@@ -934,7 +998,7 @@ static int gmres_impl(const hfh_csr *A, hfh_prec *P, int size_basis, int flexibl
     if ((rc = w.alloc(reinterpret_cast<void **>(&wv), vb)) || (rc = w.alloc(reinterpret_cast<void **>(&z), vb)) ||
         (rc = w.alloc(reinterpret_cast<void **>(&Vd), sizeof(T *) * m1)) ||
         (rc = w.alloc(reinterpret_cast<void **>(&Zd), sizeof(T *) * m1)) ||
-        (rc = w.alloc(reinterpret_cast<void **>(&part), sizeof(double) * kRedBlocks)) ||
+        (rc = w.alloc(reinterpret_cast<void **>(&part), sizeof(double) * 2 * kRedBlocks)) ||
         (rc = w.alloc(reinterpret_cast<void **>(&Hd), sizeof(double) * (m1 + 2))))
         return rc;
     HFH_HIP(hipMemcpyAsync(Vd, V.data(), sizeof(T *) * m1, hipMemcpyHostToDevice, s));
@@ -951,27 +1015,32 @@ static int gmres_impl(const hfh_csr *A, hfh_prec *P, int size_basis, int flexibl
         hipLaunchKernelGGL(k_neg_add<T>, dim3(nb), dim3(kBlock), 0, s, n, dst, b);
         return launch_check("k_neg_add");
     };
-    // one MGS launch + its finisher: w -= h_prev v_prev (if vprev); h_out = w.vcur or ||w||
-    auto mgs = [&](T *wv_, const T *vprev, const T *vcur, const double *hprev, double *hout) -> int {
-        hipLaunchKernelGGL(k_mgs<T>, dim3(nb), dim3(kBlock), 0, s, n, wv_, vprev, vcur, hprev, part);
-        int r2 = launch_check("k_mgs");
-        if (r2) return r2;
-        hipLaunchKernelGGL(k_fin_mgs<T>, dim3(1), dim3(kFinBlock), 0, s, (const double *)part, nb, vcur == nullptr,
-                           hout);
-        return launch_check("k_fin_mgs");
+    // the MGS steps' partials alternate between two buffers: a step reads
+    // the previous step's while it writes its own
+    int ph = 0;
+    auto pbuf = [&](int k) { return part + static_cast<size_t>(k & 1) * kRedBlocks; };
+    // one MGS launch: with vprev, h_prev from the previous launch's partials
+    // (stored at hprev_out) and w -= h_prev v_prev; partials of w.vcur or w.w
+    auto mgs = [&](T *wv_, const T *vprev, const T *vcur, double *hprev_out) -> int {
+        const double *pp = pbuf(ph);
+        ++ph;
+        hipLaunchKernelGGL(k_mgs<T>, dim3(nb), dim3(kBlock), 0, s, n, wv_, vprev, vcur, pp, nb, hprev_out, pbuf(ph));
+        return launch_check("k_mgs");
     };
     auto norm = [&](T *v, double *out) -> int {
-        int r2 = mgs(v, nullptr, nullptr, nullptr, Hd + m1);
+        int r2 = mgs(v, nullptr, nullptr, nullptr);
         if (r2) return r2;
+        hipLaunchKernelGGL(k_fin_mgs<T>, dim3(1), dim3(kFinBlock), 0, s, (const double *)pbuf(ph), nb, true, Hd + m1);
+        if ((r2 = launch_check("k_fin_mgs"))) return r2;
         return fetch(c, Hd + m1, out, 1);
     };
-    // apply the preconditioner (identity without one) into dst
+    // apply the preconditioner (identity without one) into dst; every
+    // preconditioner's apply writes every entry of dst, so dst is not cleared
     auto precond_into = [&](const T *in, T *dst) -> int {
         if (!precond) {
             HFH_HIP(hipMemcpyAsync(dst, in, vb, hipMemcpyDeviceToDevice, s));
             return HFH_OK;
         }
-        HFH_HIP(hipMemsetAsync(dst, 0, vb, s));
         return prec_apply_raw(P, in, dst);
     };
 
@@ -1034,12 +1103,14 @@ static int gmres_impl(const hfh_csr *A, hfh_prec *P, int size_basis, int flexibl
                 az = V[j];
             }
             if ((rc = spmv_launch_ex(A, az, wv, 0, nullptr, nullptr, nullptr))) return rc;
+            // step i stores h_{i-1}
             for (int i = 0; i <= j; ++i)
-                if ((rc = mgs(wv, i > 0 ? V[i - 1] : nullptr, V[i], i > 0 ? Hd + i - 1 : nullptr, Hd + i))) return rc;
-            // w -= h_j v_j ; h_{j+1} = ||w||
-            if ((rc = mgs(wv, V[j], nullptr, Hd + j, Hd + j + 1))) return rc;
-            hipLaunchKernelGGL(k_scale_into<T>, dim3(nb), dim3(kBlock), 0, s, n, V[j + 1], (const T *)wv, (const double *)(Hd + j + 1));
-            if ((rc = launch_check("k_scale_into"))) return rc;
+                if ((rc = mgs(wv, i > 0 ? V[i - 1] : nullptr, V[i], i > 0 ? Hd + i - 1 : nullptr))) return rc;
+            // w -= h_j v_j (storing h_j); h_{j+1} = ||w||, v_{j+1} = w / h_{j+1}
+            if ((rc = mgs(wv, V[j], nullptr, Hd + j))) return rc;
+            hipLaunchKernelGGL(k_norm_scale_into<T>, dim3(nb), dim3(kBlock), 0, s, n, V[j + 1], (const T *)wv,
+                               (const double *)pbuf(ph), nb, Hd + j + 1);
+            if ((rc = launch_check("k_norm_scale_into"))) return rc;
             if ((rc = fetch(c, Hd, col.data(), j + 2))) return rc;
             for (int i = 0; i <= j + 1; ++i) Hc(j, i) = col[i];
             if (Hc(j, j + 1) == 0.0) {
