@@ -1004,6 +1004,24 @@ static int gmres_impl(const hfh_csr *A, hfh_prec *P, int size_basis, int flexibl
     HFH_HIP(hipMemcpyAsync(Vd, V.data(), sizeof(T *) * m1, hipMemcpyHostToDevice, s));
     if (flexible && precond) HFH_HIP(hipMemcpyAsync(Zd, Z.data(), sizeof(T *) * m1, hipMemcpyHostToDevice, s));
 
+    // pinned staging of the Hessenberg columns in two slots (the host reads
+    // column j while the copy of column j+1 may be in flight), and their
+    // events; released after the stream has drained
+    struct Staging {
+        hipStream_t s = nullptr;
+        double *col = nullptr;
+        hipEvent_t ev[2] = {nullptr, nullptr};
+        ~Staging() {
+            if (s) (void)hipStreamSynchronize(s);
+            for (hipEvent_t e : ev)
+                if (e) (void)hipEventDestroy(e);
+            if (col) (void)hipHostFree(col);
+        }
+    } stg;
+    stg.s = s;
+    HFH_HIP(hipHostMalloc(reinterpret_cast<void **>(&stg.col), sizeof(double) * 2 * (m1 + 1)));
+    for (hipEvent_t &e : stg.ev) HFH_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+
     hipEvent_t e0, e1;
     HFH_HIP(hipEventCreate(&e0));
     HFH_HIP(hipEventCreate(&e1));
@@ -1092,26 +1110,55 @@ static int gmres_impl(const hfh_csr *A, hfh_prec *P, int size_basis, int flexibl
             if ((rc = launch_check("k_scale_into"))) return rc;
         }
         g[0] = res;
-        int j = 0;
-        for (j = 0; j != m; ++j) {
-            ++iter;
+        // the device work of Arnoldi step j: z_j = M^-1 v_j, w = A z_j, the
+        // MGS sweep and v_{j+1}; then the copy of H's column j to the host
+        auto enqueue_step = [&](int j) -> int {
             T *zj = (flexible && precond) ? Z[j] : z;
             const T *az = zj;
+            int r2;
             if (precond) {
-                if ((rc = precond_into(V[j], zj))) return rc;
+                if ((r2 = precond_into(V[j], zj))) return r2;
             } else {
                 az = V[j];
             }
-            if ((rc = spmv_launch_ex(A, az, wv, 0, nullptr, nullptr, nullptr))) return rc;
+            if ((r2 = spmv_launch_ex(A, az, wv, 0, nullptr, nullptr, nullptr))) return r2;
             // step i stores h_{i-1}
             for (int i = 0; i <= j; ++i)
-                if ((rc = mgs(wv, i > 0 ? V[i - 1] : nullptr, V[i], i > 0 ? Hd + i - 1 : nullptr))) return rc;
+                if ((r2 = mgs(wv, i > 0 ? V[i - 1] : nullptr, V[i], i > 0 ? Hd + i - 1 : nullptr))) return r2;
             // w -= h_j v_j (storing h_j); h_{j+1} = ||w||, v_{j+1} = w / h_{j+1}
-            if ((rc = mgs(wv, V[j], nullptr, Hd + j))) return rc;
+            if ((r2 = mgs(wv, V[j], nullptr, Hd + j))) return r2;
             hipLaunchKernelGGL(k_norm_scale_into<T>, dim3(nb), dim3(kBlock), 0, s, n, V[j + 1], (const T *)wv,
                                (const double *)pbuf(ph), nb, Hd + j + 1);
-            if ((rc = launch_check("k_norm_scale_into"))) return rc;
-            if ((rc = fetch(c, Hd, col.data(), j + 2))) return rc;
+            if ((r2 = launch_check("k_norm_scale_into"))) return r2;
+            HFH_HIP(hipMemcpyAsync(stg.col + (j & 1) * (m1 + 1), Hd, sizeof(double) * (j + 2),
+                                   hipMemcpyDeviceToHost, s));
+            HFH_HIP(hipEventRecord(stg.ev[j & 1], s));
+            return HFH_OK;
+        };
+        int j = 0, queued = 0;   // steps 0 .. queued-1 of this cycle are enqueued
+        for (j = 0; j != m; ++j) {
+            ++iter;
+            if (queued == j) {
+                if ((rc = enqueue_step(j))) return rc;
+                queued = j + 1;
+            }
+            // Step j+1 is enqueued before step j's column is read, so the
+            // device runs it while the host applies the rotations, unless
+            // step j is expected to end the solve (its residual estimate,
+            // extrapolated from the last two, meets the tolerance, or the
+            // iteration limit is reached).  A step enqueued past the end
+            // writes only v_{j+2}, z_{j+1}, w, z and H's next column, which
+            // the update does not read.
+            if (j + 1 < m && iter < ctl.maxits) {
+                const size_t h = hist.size();
+                const double est = h >= 2 && hist[h - 2] > 0 ? hist[h - 1] * (hist[h - 1] / hist[h - 2]) : res;
+                if (!(est < ctl.atol || est < ctl.rtol * first)) {
+                    if ((rc = enqueue_step(j + 1))) return rc;
+                    queued = j + 2;
+                }
+            }
+            HFH_HIP(hipEventSynchronize(stg.ev[j & 1]));
+            for (int i = 0; i <= j + 1; ++i) col[i] = stg.col[(j & 1) * (m1 + 1) + i];
             for (int i = 0; i <= j + 1; ++i) Hc(j, i) = col[i];
             if (Hc(j, j + 1) == 0.0) {
                 set_error("GMRES: breakdown, H(%d,%d) = 0", j, j + 1);
