@@ -667,8 +667,10 @@ def serve_stream(P, ctx, nreq, rate, chunk, seed0, budget, max_systems):
 
 def serving_bytes(shapes, inner):
     """Algorithmic bytes of the inner FP32 iterations of a stream: per row of
-    a W-entry stencil, SpMV 8W + 16 (values, columns, length, p, q), update 32,
-    direction 12."""
+    a W-entry stencil and step, the SpMV's matrix 8W (values and columns, as
+    CSR), row length 4, u gathered and read for w.u 8 and w written 4, the
+    vector step reading r, w, c, s, p and the diagonal's inverse 24 and
+    writing r, u, c, s and p 20."""
     ns = np.array([a * b * c for a, b, c in shapes])
     widths = np.array([5 if c == 1 else 7 for a, b, c in shapes])
     return float(np.sum(ns * inner * (8 * widths + 60))), ns

@@ -3,35 +3,45 @@
 //
 // Each admitted system is solved by defect correction in the form of
 // MPIR<D,S>::Solve (linear_solver/mpir.cc:69-327): FP64 true residual
-// r = b - A x, FP32 inner Jacobi-PCG on A c = r (CG::SolvePrecond,
-// linear_solver/cg.cc:84-248, preconditioner lpreconditioner.cc:104-166),
-// FP64 update x += c, until the true residual is at most
-// max(atol, rtol * initial residual).  The engine has no reference
-// counterpart; it runs the same algorithm for many systems at once.
+// r = b - A x, FP32 inner Jacobi-preconditioned CG on A c = r
+// (preconditioner lpreconditioner.cc:104-166), FP64 update x += c, until the
+// true residual is at most max(atol, rtol * initial residual).  The engine
+// has no reference counterpart; it runs the same algorithm for many systems
+// at once.  The inner CG is the Chronopoulos-Gear form of CG::SolvePrecond
+// (linear_solver/cg.cc:84-248): the same iterates in exact arithmetic, with
+// both inner products of a step formed after its one SpMV, so a step needs
+// one grid-wide reduction instead of two:
+//   p = u + beta p,  s = w + beta s,  c += alpha p,  r -= alpha s,
+//   u = M r,  w = A u,  gamma' = r.u,  delta = w.u,
+//   beta = gamma' / gamma,  alpha = gamma' / (delta - beta gamma' / alpha)
+// (w = A u and s = A p are carried instead of the second product A p).
 //
 // Device layout.  One arena of `hbm_budget` bytes holds every admitted
 // system in a contiguous region (first-fit, 256-byte aligned sections):
 //   sl_ptr[nslice+1] sl_len[n] cols v64[E] v32[E] invd[n] b[n] x[n]   (uploaded / assembled)
-//   r64[n] r[n] c[n] z[n] p[n] q[n] part[2*ntiles]                      (work space)
+//   r64[n] r[n] w[n] s[n] u[n] c[n] p[n] part[3*ntiles]                  (work space)
 // with the matrix in SELL-64 form (slices of 64 rows, column-major, E
 // padded entries; FP64 values for the true residual, FP32 values for the
 // inner solve, one shared column array).  `cols` is either sl_col[E] or,
 // when the system has at most 256 distinct offsets col - row (every FV
 // system: 5 or 7), one-byte codes into the system's offset table dict[256]:
 // code word w of a lane holds slots 4w..4w+3, ceil(L/4) words per lane of a
-// slice of width L, so the values keep the slice width.  A slot table holds each system's
-// descriptor (pointers, sizes) and its solver state.  A tile table lists
-// every active system's tiles (32 slices = 2048 rows each).
+// slice of width L, so the values keep the slice width.  A slot table holds
+// each system's descriptor (pointers, sizes) and its solver state.  A tile
+// table lists every active system's tiles (32 slices = 2048 rows each).
 //
-// One CG iteration of all systems is five launches:
-//   K1 tiles : inner  q = A32 p, partial p.q      | outer  r64 = b - A64 x, partial r64.r64
-//   K2 slots : inner  alpha = rz / p.q            | outer  stop test, else start a correction
-//   K3 tiles : inner  c += alpha p, r -= alpha q  | start  r = fl(r64), c = 0
-//              z = invd r, partials r.r, r.z
-//   K4 slots : inner  stop test or beta           | start  rr0, rz
-//   K5 tiles : p = z + beta p | p = z | x += c (correction finished)
+// One step of all systems is three launches:
+//   K1 tiles : inner  p = u + beta p, s = w + beta s, c += alpha p,
+//                     r -= alpha s, u = invd r (stored for K2's gathers),
+//                     partials r.u and r.r
+//              setup  r = fl(r64), u = invd r and the partials
+//              outer  x += c, c = 0 (after an inner solve; c = 0 first)
+//   K2 tiles : inner / setup  w = A32 u, partial w.u
+//              outer  r64 = b - A64 x, partial r64.r64
+//   K3 slots : the decision from the system's partials: alpha and beta, the
+//              inner stop test, the outer stop test, the next phase
 // The tile kernels loop over the tile table with a fixed grid, and the slot
-// kernels over the slot table, so `chunk` iterations are captured once in a
+// kernel over the slot table, so `chunk` steps are captured once in a
 // hipGraph and replayed while systems come and go: admission and retirement
 // only rewrite the tables between replays.  A system's arithmetic depends
 // only on its own data and its own fixed tiling (partials summed per system
@@ -49,19 +59,19 @@ namespace {
 
 constexpr int kTileSlices = 32;     // 2048 rows per tile
 constexpr int kTileRows = kTileSlices * 64;
-constexpr int kTileGrid = 2048;     // workgroups of the tile kernels
+constexpr int kTileGrid = 2048;     // workgroups of the tile kernel
 constexpr int kSellU = 8;           // column slots per SELL step
 constexpr size_t kAlign = 256;
+constexpr int kParts = 3;           // partials per tile: r.u, w.u, r.r (outer: 0, 0, r64.r64)
 
-// solver phases (SysState::phase)
+// solver phases (SysState::phase): what the next K1 and K2 do
 enum : int {
     P_IDLE = 0,     // free slot
-    P_FIRST = 1,    // next K1: initial true residual
-    P_CORRECT = 2,  // next K5: x += c; next K1: true residual
-    P_SETUP = 3,    // K3/K4: start an inner solve from r64
-    P_START = 4,    // next K5: p = z; next K1: inner SpMV
-    P_GO = 5,       // next K5: p = z + beta p; next K1: inner SpMV
-    P_UPD = 6,      // K3/K4: inner update with alpha
+    P_FIRST = 1,    // c = 0; the initial true residual
+    P_CORRECT = 2,  // x += c, c = 0; the true residual
+    P_SETUP = 3,    // the start of an inner solve from r64: r, u, w = A u
+    P_START = 4,    // first inner step (p = u, s = w)
+    P_GO = 5,       // inner step with beta
     P_DONE = 7      // finished (state holds the outcome)
 };
 
@@ -80,7 +90,7 @@ struct SysDesc {
     const double *b;
     double *x;
     double *r64;
-    float *r, *c, *z, *p, *q;
+    float *r, *w, *s, *u, *c, *p;
     double *part;
 };
 
@@ -89,8 +99,6 @@ struct SysDesc {
 // and generic (flat) loads make every wait drain the LDS counter as well.
 template <typename T>
 using gptr = __attribute__((address_space(1))) T *;
-typedef float bf4 __attribute__((ext_vector_type(4)));
-typedef double bd2 __attribute__((ext_vector_type(2)));
 
 struct GDesc {
     int n, nslice, ntiles, ndict;
@@ -101,7 +109,7 @@ struct GDesc {
     gptr<const float> v32, invd;
     gptr<const double> b;
     gptr<double> x, r64;
-    gptr<float> r, c, z, p, q;
+    gptr<float> r, w, s, u, c, p;
     gptr<double> part;
 };
 
@@ -124,10 +132,11 @@ __device__ __forceinline__ GDesc gview(const SysDesc &d) {
     g.x = (gptr<double>)d.x;
     g.r64 = (gptr<double>)d.r64;
     g.r = (gptr<float>)d.r;
+    g.w = (gptr<float>)d.w;
+    g.s = (gptr<float>)d.s;
+    g.u = (gptr<float>)d.u;
     g.c = (gptr<float>)d.c;
-    g.z = (gptr<float>)d.z;
     g.p = (gptr<float>)d.p;
-    g.q = (gptr<float>)d.q;
     g.part = (gptr<double>)d.part;
     return g;
 }
@@ -135,7 +144,7 @@ __device__ __forceinline__ GDesc gview(const SysDesc &d) {
 struct SysState {
     int phase, status, outer, inner, inner_total;
     int id;   // request id of the occupant (the kernels never change it)
-    double res0, res, tol, rz, rr0, alpha, beta;
+    double res0, res, tol, rr0, alpha, beta, gamma;
 };
 
 struct Tile {
@@ -160,16 +169,26 @@ __device__ __forceinline__ T badd(T a, T b) {
     return a + b;
 }
 
-// Row sum of SELL row `lane` of slice `s` against vector v (wave-uniform slice).
-template <typename T, typename V>
-__device__ __forceinline__ T sell_row(const GDesc &D, gptr<const T> val, gptr<const V> v, int s, int lane, int len) {
+typedef float bf4 __attribute__((ext_vector_type(4)));
+typedef double bd2 __attribute__((ext_vector_type(2)));
+
+// The SpMV operand at row j.
+template <typename V>
+struct VecAt {
+    gptr<const V> v;
+    __device__ __forceinline__ V operator()(int j) const { return v[j]; }
+};
+
+// Row sum of SELL row `lane` of slice `s` against operand f (wave-uniform slice).
+template <typename T, typename F>
+__device__ __forceinline__ T sell_row(const GDesc &D, gptr<const T> val, const F &v, int s, int lane, int len) {
     const int base = D.sl_ptr[s];
     const int L = (D.sl_ptr[s + 1] - base) >> 6;
     T acc = 0;
     for (int k = 0; k < L; k += kSellU) {
         int c[kSellU];
         T a[kSellU];
-        V xv[kSellU];
+        T xv[kSellU];
 #pragma unroll
         for (int u = 0; u < kSellU; ++u) {
             const int idx = base + min(k + u, L - 1) * 64 + lane;
@@ -177,10 +196,10 @@ __device__ __forceinline__ T sell_row(const GDesc &D, gptr<const T> val, gptr<co
             a[u] = __builtin_nontemporal_load(&val[idx]);
         }
 #pragma unroll
-        for (int u = 0; u < kSellU; ++u) xv[u] = v[c[u]];
+        for (int u = 0; u < kSellU; ++u) xv[u] = v(c[u]);
 #pragma unroll
         for (int u = 0; u < kSellU; ++u)
-            if (k + u < len) acc = badd(acc, bmul(a[u], static_cast<T>(xv[u])));
+            if (k + u < len) acc = badd(acc, bmul(a[u], xv[u]));
     }
     return acc;
 }
@@ -190,9 +209,9 @@ __device__ __forceinline__ T sell_row(const GDesc &D, gptr<const T> val, gptr<co
 // not stored).  Word w of a lane (slots 4w..4w+3) sits at
 // sl_wptr[s] + 64 w + lane; the values keep the slice width L.  Two words
 // (8 slots) per step, the slots past L skipped (wave-uniform).
-template <typename T, typename V>
-__device__ __forceinline__ T sell_row_coded(const GDesc &D, gptr<const T> val, gptr<const V> v, int s, int lane,
-                                            int len, int r, const int *sdict) {
+template <typename T, typename F>
+__device__ __forceinline__ T sell_row_coded(const GDesc &D, gptr<const T> val, const F &v, int s, int lane, int len,
+                                            int r, const int *sdict) {
     const int base = D.sl_ptr[s];
     const int L = (D.sl_ptr[s + 1] - base) >> 6;
     const int wb = D.sl_wptr[s];
@@ -200,7 +219,7 @@ __device__ __forceinline__ T sell_row_coded(const GDesc &D, gptr<const T> val, g
     for (int w = 0; 4 * w < L; w += 2) {
         unsigned cw[2];
         T a[8];
-        V xv[8];
+        T xv[8];
 #pragma unroll
         for (int h = 0; h < 2; ++h) cw[h] = 4 * (w + h) < L ? __builtin_nontemporal_load(&D.sl_code[wb + (w + h) * 64 + lane]) : 0u;
 #pragma unroll
@@ -210,40 +229,156 @@ __device__ __forceinline__ T sell_row_coded(const GDesc &D, gptr<const T> val, g
             xv[u] = 0;
             if (k < L) {   // wave-uniform
                 a[u] = __builtin_nontemporal_load(&val[base + k * 64 + lane]);
-                xv[u] = v[min(r + sdict[(cw[u >> 2] >> (8 * (u & 3))) & 255u], D.n - 1)];
+                xv[u] = v(min(r + sdict[(cw[u >> 2] >> (8 * (u & 3))) & 255u], D.n - 1));
             }
         }
 #pragma unroll
         for (int u = 0; u < 8; ++u)
-            if (4 * w + u < len) acc = badd(acc, bmul(a[u], static_cast<T>(xv[u])));
+            if (4 * w + u < len) acc = badd(acc, bmul(a[u], xv[u]));
     }
     return acc;
 }
 
-__device__ __forceinline__ bool is_inner_spmv(int ph) { return ph == P_START || ph == P_GO; }
-__device__ __forceinline__ bool is_outer_spmv(int ph) { return ph == P_FIRST || ph == P_CORRECT; }
+__device__ __forceinline__ bool is_inner(int ph) { return ph == P_START || ph == P_GO; }
+__device__ __forceinline__ bool is_outer(int ph) { return ph == P_FIRST || ph == P_CORRECT; }
+__device__ __forceinline__ bool works(int ph) { return ph >= P_FIRST && ph <= P_GO; }
 
-// One tile's share of K1 for a group of kBlock threads (wave 0..3 of the
-// group, wave-uniform): the inner SpMV q = A32 p or the FP64 true residual
-// r64 = b - A64 x over the tile's slices, and this thread's part of the
-// tile's partial p.q or r64.r64.
-__device__ __forceinline__ double spmv_tile(const GDesc &D, const Tile &T, bool inner, int wave, int lane,
+// The vector step of an inner phase PH over rows [r0, r1) of a tile (rv:
+// the end of the rows taken 4 at a time with 16-byte accesses): P_SETUP
+// r = fl(r64); P_START p = u, s = w; P_GO p = u + b p, s = w + b s; then
+// c += a p, r -= a s (not in P_SETUP) and u = invd r, with this thread's
+// parts of r.u and r.r.
+template <int PH>
+__device__ __forceinline__ void vec_rows(const GDesc &D, int r0, int r1, int rv, float a, float b, int tid, double &ru,
+                                         double &rr) {
+    // the current u = invd r is formed again from r (the same bits as the
+    // stored copy, which only the SpMV reads)
+    auto one = [&](float &ri, float &si, float &pi, float &ci, float &un, float wi, float di, double r64i) {
+        if (PH == P_SETUP) {
+            ri = static_cast<float>(r64i);
+        } else {
+            const float ui = bmul(di, ri);
+            pi = PH == P_START ? ui : ui + b * pi;
+            si = PH == P_START ? wi : wi + b * si;
+            ci = ci + a * pi;
+            ri = ri - a * si;
+        }
+        un = bmul(di, ri);
+        ru += static_cast<double>(ri) * static_cast<double>(un);
+        rr += static_cast<double>(ri) * static_cast<double>(ri);
+    };
+    for (int i = r0 + 4 * tid; i < rv; i += 4 * kBlock) {
+        bf4 r4 = bf4(0.0f), s4 = bf4(0.0f), p4 = bf4(0.0f), c4 = bf4(0.0f), w4 = bf4(0.0f), u4;
+        bd2 ra = bd2(0.0), rb = bd2(0.0);
+        if (PH == P_SETUP) {
+            ra = *reinterpret_cast<gptr<const bd2>>(&D.r64[i]);
+            rb = *reinterpret_cast<gptr<const bd2>>(&D.r64[i + 2]);
+        } else {
+            r4 = *reinterpret_cast<gptr<const bf4>>(&D.r[i]);
+            w4 = *reinterpret_cast<gptr<const bf4>>(&D.w[i]);
+            c4 = *reinterpret_cast<gptr<const bf4>>(&D.c[i]);
+            if (PH == P_GO) {
+                s4 = *reinterpret_cast<gptr<const bf4>>(&D.s[i]);
+                p4 = *reinterpret_cast<gptr<const bf4>>(&D.p[i]);
+            }
+        }
+        const bf4 d4 = *reinterpret_cast<gptr<const bf4>>(&D.invd[i]);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            float ri = r4[k], si = s4[k], pi = p4[k], ci = c4[k], un;
+            one(ri, si, pi, ci, un, w4[k], d4[k], k < 2 ? ra[k] : rb[k - 2]);
+            r4[k] = ri;
+            s4[k] = si;
+            p4[k] = pi;
+            c4[k] = ci;
+            u4[k] = un;
+        }
+        *reinterpret_cast<gptr<bf4>>(&D.r[i]) = r4;
+        *reinterpret_cast<gptr<bf4>>(&D.u[i]) = u4;
+        if (PH != P_SETUP) {
+            *reinterpret_cast<gptr<bf4>>(&D.s[i]) = s4;
+            *reinterpret_cast<gptr<bf4>>(&D.p[i]) = p4;
+            *reinterpret_cast<gptr<bf4>>(&D.c[i]) = c4;
+        }
+    }
+    for (int i = rv + tid; i < r1; i += kBlock) {
+        float ri = PH == P_SETUP ? 0.0f : D.r[i], si = PH == P_GO ? D.s[i] : 0.0f, pi = PH == P_GO ? D.p[i] : 0.0f;
+        float ci = PH == P_SETUP ? 0.0f : D.c[i], un;
+        one(ri, si, pi, ci, un, PH == P_SETUP ? 0.0f : D.w[i], D.invd[i], PH == P_SETUP ? D.r64[i] : 0.0);
+        D.r[i] = ri;
+        D.u[i] = un;
+        if (PH != P_SETUP) {
+            D.s[i] = si;
+            D.p[i] = pi;
+            D.c[i] = ci;
+        }
+    }
+}
+
+// One tile's share of K1 for a group of kBlock threads (tid 0..kBlock-1):
+// the inner step's vector updates, the start of an inner solve, or the
+// correction of x; this thread's parts of the tile's r.u and r.r.
+__device__ __forceinline__ void vec_tile(const GDesc &D, const Tile &T, const SysState &S, int tid, double &ru,
+                                         double &rr) {
+    const int ph = S.phase;
+    const int r0 = T.s0 * 64, r1 = min(D.n, (T.s0 + T.ns) * 64);
+    // 4 rows per thread and step with 16-byte accesses (a tile starts at a
+    // multiple of 2048 rows and every array at a 256-byte boundary), the
+    // rows past the last multiple of 4 one by one
+    const int rv = r0 + ((r1 - r0) & ~3);
+    if (is_outer(ph)) {
+        for (int i = r0 + 4 * tid; i < rv; i += 4 * kBlock) {
+            if (ph == P_CORRECT) {
+                const bf4 c4 = *reinterpret_cast<gptr<const bf4>>(&D.c[i]);
+                bd2 xa = *reinterpret_cast<gptr<const bd2>>(&D.x[i]);
+                bd2 xb = *reinterpret_cast<gptr<const bd2>>(&D.x[i + 2]);
+                xa.x += static_cast<double>(c4.x);
+                xa.y += static_cast<double>(c4.y);
+                xb.x += static_cast<double>(c4.z);
+                xb.y += static_cast<double>(c4.w);
+                *reinterpret_cast<gptr<bd2>>(&D.x[i]) = xa;
+                *reinterpret_cast<gptr<bd2>>(&D.x[i + 2]) = xb;
+            }
+            *reinterpret_cast<gptr<bf4>>(&D.c[i]) = bf4(0.0f);
+        }
+        for (int i = rv + tid; i < r1; i += kBlock) {
+            if (ph == P_CORRECT) D.x[i] += static_cast<double>(D.c[i]);
+            D.c[i] = 0.0f;
+        }
+        return;
+    }
+    const float a = static_cast<float>(S.alpha), b = static_cast<float>(S.beta);
+    if (ph == P_SETUP)
+        vec_rows<P_SETUP>(D, r0, r1, rv, a, b, tid, ru, rr);
+    else if (ph == P_START)
+        vec_rows<P_START>(D, r0, r1, rv, a, b, tid, ru, rr);
+    else
+        vec_rows<P_GO>(D, r0, r1, rv, a, b, tid, ru, rr);
+}
+
+// One tile's share of K2 for a group of kBlock threads (wave 0..3 of the
+// group, wave-uniform): w = A32 u with this thread's part of w.u, or the
+// FP64 true residual r64 = b - A64 x with its part of r64.r64.
+__device__ __forceinline__ double spmv_tile(const GDesc &D, const Tile &T, int ph, int wave, int lane,
                                             const int *sdict) {
     const bool coded = D.ndict > 0;
+    const bool outer = is_outer(ph);
     double dsum = 0;
     for (int s = T.s0 + wave; s < T.s0 + T.ns; s += kBlock / 64) {
         const int r = s * 64 + lane;
         const int len = r < D.n ? D.sl_len[r] : 0;
-        if (inner) {
-            const float acc = coded ? sell_row_coded<float, float>(D, D.v32, gptr<const float>(D.p), s, lane, len, r, sdict)
-                                    : sell_row<float, float>(D, D.v32, gptr<const float>(D.p), s, lane, len);
+        if (!outer) {
+            const VecAt<float> f{gptr<const float>(D.u)};
+            const float acc = coded ? sell_row_coded<float>(D, D.v32, f, s, lane, len, r, sdict)
+                                    : sell_row<float>(D, D.v32, f, s, lane, len);
             if (r < D.n) {
-                D.q[r] = acc;
-                dsum += static_cast<double>(D.p[r]) * static_cast<double>(acc);
+                D.w[r] = acc;
+                dsum += static_cast<double>(acc) * static_cast<double>(f(r));
             }
         } else {
-            const double acc = coded ? sell_row_coded<double, double>(D, D.v64, gptr<const double>(D.x), s, lane, len, r, sdict)
-                                     : sell_row<double, double>(D, D.v64, gptr<const double>(D.x), s, lane, len);
+            const VecAt<double> f{gptr<const double>(D.x)};
+            const double acc = coded ? sell_row_coded<double>(D, D.v64, f, s, lane, len, r, sdict)
+                                     : sell_row<double>(D, D.v64, f, s, lane, len);
             if (r < D.n) {
                 const double rr = D.b[r] - acc;
                 D.r64[r] = rr;
@@ -254,7 +389,33 @@ __device__ __forceinline__ double spmv_tile(const GDesc &D, const Tile &T, bool 
     return dsum;
 }
 
-// K1: inner SpMV q = A32 p with p.q partials, or the FP64 true residual.
+// K1: every active system's tiles: the vector step; partials r.u (slot 0)
+// and r.r (slot 2) of the inner phases.
+__global__ __launch_bounds__(kBlock) void kb_vec(const SysDesc *__restrict__ sd, const SysState *__restrict__ st,
+                                                 const Tile *__restrict__ tiles, const int *__restrict__ ntiles) {
+    __shared__ double sh[kBlock / 64];
+    const int nt = *ntiles;
+    for (int t = blockIdx.x; t < nt; t += gridDim.x) {
+        const Tile T = tiles[t];
+        const SysState S = st[T.sys];
+        if (!works(S.phase)) continue;   // uniform over the workgroup
+        const GDesc D = gview(sd[T.sys]);
+        double ru = 0, rr = 0;
+        vec_tile(D, T, S, threadIdx.x, ru, rr);
+        if (is_outer(S.phase)) continue;
+        const double tru = block_sum(ru, sh);
+        __syncthreads();
+        const double trr = block_sum(rr, sh);
+        if (threadIdx.x == 0) {
+            D.part[kParts * T.lt] = tru;
+            D.part[kParts * T.lt + 2] = trr;
+        }
+        __syncthreads();
+    }
+}
+
+// K2: every active system's tiles: the SpMV; partial w.u (slot 1) or
+// r64.r64 (slot 2).
 __global__ __launch_bounds__(kBlock) void kb_spmv(const SysDesc *__restrict__ sd, const SysState *__restrict__ st,
                                                   const Tile *__restrict__ tiles, const int *__restrict__ ntiles) {
     __shared__ double sh[kBlock / 64];
@@ -265,246 +426,104 @@ __global__ __launch_bounds__(kBlock) void kb_spmv(const SysDesc *__restrict__ sd
     for (int t = blockIdx.x; t < nt; t += gridDim.x) {
         const Tile T = tiles[t];
         const int ph = st[T.sys].phase;
-        const bool inner = is_inner_spmv(ph);
-        if (!inner && !is_outer_spmv(ph)) continue;   // uniform over the workgroup
+        if (!works(ph)) continue;   // uniform over the workgroup
         const GDesc D = gview(sd[T.sys]);
         if (D.ndict > 0) {   // the system's offset table (the previous tile's readers passed the barrier below)
             for (int i = threadIdx.x; i < D.ndict; i += kBlock) sdict[i] = D.dict[i];
             __syncthreads();
         }
-        const double tot = block_sum(spmv_tile(D, T, inner, wave, lane, sdict), sh);
-        if (threadIdx.x == 0) D.part[2 * T.lt] = tot;
-        __syncthreads();   // sh is reused by the next tile
+        const double tot = block_sum(spmv_tile(D, T, ph, wave, lane, sdict), sh);
+        if (threadIdx.x == 0) D.part[kParts * T.lt + (is_outer(ph) ? 2 : 1)] = tot;
+        __syncthreads();   // sh and sdict are reused by the next tile
     }
 }
 
-// Deterministic sum of the system's partials q (0 or 1) over its tiles.
+// Deterministic sum of the system's partials q over its tiles (kBlock
+// threads strided over the tiles, then block_sum's tree); valid in thread 0.
 __device__ __forceinline__ double sum_parts(const GDesc &D, int q, double *sh) {
     double v = 0;
-    for (int i = threadIdx.x; i < D.ntiles; i += kBlock) v += D.part[2 * i + q];
+    for (int i = threadIdx.x; i < D.ntiles; i += kBlock) v += D.part[kParts * i + q];
     const double tot = block_sum(v, sh);
     __syncthreads();
     return tot;
 }
 
-// The decision after K1 from the sum `tot` of its partials: inner, alpha or
-// a breakdown; outer, the true residual's test.
-__device__ __forceinline__ void decide_after_spmv(SysState &S, double tot, const KParams &prm) {
+// The decision after K1 and K2 from the sums of their partials (g = r.u,
+// d = w.u, rho = r.r, or the true residual's r64.r64).
+__device__ __forceinline__ void decide(SysState &S, double g, double d, double rho, const KParams &prm) {
     const int ph = S.phase;
-    if (is_inner_spmv(ph)) {
-        if (tot > 0 && isfinite(tot)) {
-            S.alpha = S.rz / tot;
-            S.phase = P_UPD;
+    if (is_outer(ph)) {
+        const double res = std::sqrt(rho);
+        if (ph == P_FIRST) {
+            S.res0 = res;
+            S.tol = fmax(prm.atol, prm.rtol * res);
+        }
+        S.res = res;
+        if (res <= S.tol) {
+            S.status = res <= prm.atol ? HFH_SUCCESS_ABS : HFH_SUCCESS_REL;
+            S.phase = P_DONE;
+        } else if (!isfinite(res)) {
+            S.status = HFH_FAIL_DIV;
+            S.phase = P_DONE;
+        } else if (S.outer >= prm.outer_max) {
+            S.status = HFH_FAIL_MAXITS;
+            S.phase = P_DONE;
         } else {
-            S.phase = P_CORRECT;   // breakdown: end this inner solve with the correction so far
+            ++S.outer;
+            S.phase = P_SETUP;
         }
         return;
     }
-    const double res = std::sqrt(tot);
-    if (ph == P_FIRST) {
-        S.res0 = res;
-        S.tol = fmax(prm.atol, prm.rtol * res);
-    }
-    S.res = res;
-    if (res <= S.tol) {
-        S.status = res <= prm.atol ? HFH_SUCCESS_ABS : HFH_SUCCESS_REL;
-        S.phase = P_DONE;
-    } else if (!isfinite(res)) {
-        S.status = HFH_FAIL_DIV;
-        S.phase = P_DONE;
-    } else if (S.outer >= prm.outer_max) {
-        S.status = HFH_FAIL_MAXITS;
-        S.phase = P_DONE;
-    } else {
-        ++S.outer;
-        S.phase = P_SETUP;
-    }
-}
-
-// K2: per system, after K1.
-__global__ __launch_bounds__(kBlock) void kb_fin1(const SysDesc *__restrict__ sd, SysState *__restrict__ st,
-                                                  int nslots, KParams prm) {
-    __shared__ double sh[kBlock / 64];
-    const int s = blockIdx.x;
-    if (s >= nslots) return;
-    const int ph = st[s].phase;
-    if (!is_inner_spmv(ph) && !is_outer_spmv(ph)) return;
-    if (sd[s].small) return;   // kb_small's
-    const GDesc D = gview(sd[s]);
-    const double tot = sum_parts(D, 0, sh);
-    if (threadIdx.x == 0) decide_after_spmv(st[s], tot, prm);
-}
-
-// One tile's share of K3 for a group of kBlock threads (tid 0..kBlock-1):
-// the inner update with alpha, or the start of an inner solve, and this
-// thread's parts of the tile's r.r and r.z.
-__device__ __forceinline__ void update_tile(const GDesc &D, const Tile &T, int ph, float a, int tid, double &rr,
-                                            double &rz) {
-    const int r0 = T.s0 * 64, r1 = min(D.n, (T.s0 + T.ns) * 64);
-    // 4 rows per thread and step with 16-byte accesses (a tile starts at a
-    // multiple of 2048 rows and every array at a 256-byte boundary), the
-    // rows past the last multiple of 4 one by one
-    auto one = [&](float &ci, float &ri_io, float pi, float qi, double r64i, float di, float &zi) {
-        float ri;
-        if (ph == P_UPD) {
-            ci = ci + a * pi;
-            ri = ri_io - a * qi;
-        } else {
-            ri = static_cast<float>(r64i);
-            ci = 0.0f;
-        }
-        ri_io = ri;
-        zi = di * ri;
-        rr += static_cast<double>(ri) * static_cast<double>(ri);
-        rz += static_cast<double>(ri) * static_cast<double>(zi);
-    };
-    const int rv = r0 + ((r1 - r0) & ~3);
-    for (int i = r0 + 4 * tid; i < rv; i += 4 * kBlock) {
-        bf4 c4 = ph == P_UPD ? *reinterpret_cast<gptr<const bf4>>(&D.c[i]) : bf4(0.0f);
-        bf4 r4 = ph == P_UPD ? *reinterpret_cast<gptr<const bf4>>(&D.r[i]) : bf4(0.0f);
-        bf4 p4 = bf4(0.0f), q4 = bf4(0.0f);
-        bd2 ra = bd2(0.0), rb = bd2(0.0);
-        if (ph == P_UPD) {
-            p4 = *reinterpret_cast<gptr<const bf4>>(&D.p[i]);
-            q4 = *reinterpret_cast<gptr<const bf4>>(&D.q[i]);
-        } else {
-            ra = *reinterpret_cast<gptr<const bd2>>(&D.r64[i]);
-            rb = *reinterpret_cast<gptr<const bd2>>(&D.r64[i + 2]);
-        }
-        const bf4 d4 = *reinterpret_cast<gptr<const bf4>>(&D.invd[i]);
-        bf4 z4;
-#pragma unroll
-        for (int w = 0; w < 4; ++w) {
-            float ci = c4[w], ri = r4[w], zi;
-            one(ci, ri, p4[w], q4[w], w < 2 ? ra[w] : rb[w - 2], d4[w], zi);
-            c4[w] = ci;
-            r4[w] = ri;
-            z4[w] = zi;
-        }
-        *reinterpret_cast<gptr<bf4>>(&D.c[i]) = c4;
-        *reinterpret_cast<gptr<bf4>>(&D.r[i]) = r4;
-        *reinterpret_cast<gptr<bf4>>(&D.z[i]) = z4;
-    }
-    for (int i = rv + tid; i < r1; i += kBlock) {
-        float ci = ph == P_UPD ? D.c[i] : 0.0f, ri = ph == P_UPD ? D.r[i] : 0.0f, zi;
-        one(ci, ri, ph == P_UPD ? D.p[i] : 0.0f, ph == P_UPD ? D.q[i] : 0.0f, ph == P_UPD ? 0.0 : D.r64[i],
-            D.invd[i], zi);
-        D.c[i] = ci;
-        D.r[i] = ri;
-        D.z[i] = zi;
-    }
-}
-
-// K3: inner update, or the start of an inner solve; partials r.r and r.z.
-__global__ __launch_bounds__(kBlock) void kb_update(const SysDesc *__restrict__ sd, const SysState *__restrict__ st,
-                                                    const Tile *__restrict__ tiles, const int *__restrict__ ntiles) {
-    __shared__ double sh[kBlock / 64];
-    const int nt = *ntiles;
-    for (int t = blockIdx.x; t < nt; t += gridDim.x) {
-        const Tile T = tiles[t];
-        const int ph = st[T.sys].phase;
-        if (ph != P_UPD && ph != P_SETUP) continue;
-        const GDesc D = gview(sd[T.sys]);
-        double rr = 0, rz = 0;
-        update_tile(D, T, ph, static_cast<float>(st[T.sys].alpha), threadIdx.x, rr, rz);
-        const double trr = block_sum(rr, sh);
-        __syncthreads();
-        const double trz = block_sum(rz, sh);
-        if (threadIdx.x == 0) {
-            D.part[2 * T.lt] = trr;
-            D.part[2 * T.lt + 1] = trz;
-        }
-        __syncthreads();
-    }
-}
-
-// The decision after K3 from its r.r and r.z sums.
-__device__ __forceinline__ void decide_after_update(SysState &S, double rr, double rz, const KParams &prm) {
-    const double rn = std::sqrt(rr);
-    if (S.phase == P_SETUP) {
+    const double rn = std::sqrt(rho);
+    if (ph == P_SETUP) {
         S.rr0 = rn;
-        S.rz = rz;
         S.inner = 0;
-        S.phase = rn > 0 && isfinite(rn) ? P_START : P_CORRECT;
+        if (rn > 0 && isfinite(rn) && d > 0 && isfinite(d) && isfinite(g)) {
+            S.alpha = g / d;
+            S.beta = 0;
+            S.gamma = g;
+            S.phase = P_START;
+        } else {
+            S.phase = P_CORRECT;   // c = 0: the true residual of x again
+        }
         return;
     }
     ++S.inner;
     ++S.inner_total;
-    if (rn <= prm.inner_rtol * S.rr0 || S.inner >= prm.inner_max || !(S.rz != 0) || !isfinite(rz)) {
+    if (rn <= prm.inner_rtol * S.rr0 || S.inner >= prm.inner_max || !(g != 0) || !isfinite(g)) {
         S.phase = P_CORRECT;
-    } else {
-        S.beta = rz / S.rz;
-        S.rz = rz;
-        S.phase = P_GO;
+        return;
     }
+    const double beta = g / S.gamma;
+    const double den = d - beta * g / S.alpha;
+    if (!(den > 0) || !isfinite(den)) {
+        S.phase = P_CORRECT;   // breakdown: end this inner solve with the correction so far
+        return;
+    }
+    S.beta = beta;
+    S.alpha = g / den;
+    S.gamma = g;
+    S.phase = P_GO;
 }
 
-// K4: per system, after K3.
-__global__ __launch_bounds__(kBlock) void kb_fin2(const SysDesc *__restrict__ sd, SysState *__restrict__ st,
-                                                  int nslots, KParams prm) {
+// K3: per system, after K1 and K2.
+__global__ __launch_bounds__(kBlock) void kb_fin(const SysDesc *__restrict__ sd, SysState *__restrict__ st,
+                                                 int nslots, KParams prm) {
     __shared__ double sh[kBlock / 64];
     const int s = blockIdx.x;
     if (s >= nslots) return;
     const int ph = st[s].phase;
-    if (ph != P_UPD && ph != P_SETUP) return;
+    if (!works(ph)) return;
     if (sd[s].small) return;   // kb_small's
     const GDesc D = gview(sd[s]);
-    const double rr = sum_parts(D, 0, sh);
-    const double rz = sum_parts(D, 1, sh);
-    if (threadIdx.x == 0) decide_after_update(st[s], rr, rz, prm);
-}
-
-// One tile's share of K5 for a group of kBlock threads: the search
-// direction, or the FP64 correction x += c.
-__device__ __forceinline__ void dir_tile(const GDesc &D, const Tile &T, int ph, float beta, int tid) {
-    const int r0 = T.s0 * 64, r1 = min(D.n, (T.s0 + T.ns) * 64);
-    const int rv = r0 + ((r1 - r0) & ~3);   // 16-byte accesses up to here, as in update_tile
-    if (ph == P_GO) {
-        for (int i = r0 + 4 * tid; i < rv; i += 4 * kBlock) {
-            const bf4 z4 = *reinterpret_cast<gptr<const bf4>>(&D.z[i]);
-            bf4 p4 = *reinterpret_cast<gptr<const bf4>>(&D.p[i]);
-            p4.x = z4.x + beta * p4.x;
-            p4.y = z4.y + beta * p4.y;
-            p4.z = z4.z + beta * p4.z;
-            p4.w = z4.w + beta * p4.w;
-            *reinterpret_cast<gptr<bf4>>(&D.p[i]) = p4;
-        }
-        for (int i = rv + tid; i < r1; i += kBlock) D.p[i] = D.z[i] + beta * D.p[i];
-    } else if (ph == P_START) {
-        for (int i = r0 + 4 * tid; i < rv; i += 4 * kBlock)
-            *reinterpret_cast<gptr<bf4>>(&D.p[i]) = *reinterpret_cast<gptr<const bf4>>(&D.z[i]);
-        for (int i = rv + tid; i < r1; i += kBlock) D.p[i] = D.z[i];
-    } else {
-        for (int i = r0 + 4 * tid; i < rv; i += 4 * kBlock) {
-            const bf4 c4 = *reinterpret_cast<gptr<const bf4>>(&D.c[i]);
-            bd2 xa = *reinterpret_cast<gptr<const bd2>>(&D.x[i]);
-            bd2 xb = *reinterpret_cast<gptr<const bd2>>(&D.x[i + 2]);
-            xa.x += static_cast<double>(c4.x);
-            xa.y += static_cast<double>(c4.y);
-            xb.x += static_cast<double>(c4.z);
-            xb.y += static_cast<double>(c4.w);
-            *reinterpret_cast<gptr<bd2>>(&D.x[i]) = xa;
-            *reinterpret_cast<gptr<bd2>>(&D.x[i + 2]) = xb;
-        }
-        for (int i = rv + tid; i < r1; i += kBlock) D.x[i] += static_cast<double>(D.c[i]);
-    }
-}
-
-// K5: search direction, or the FP64 correction x += c.
-__global__ __launch_bounds__(kBlock) void kb_dir(const SysDesc *__restrict__ sd, const SysState *__restrict__ st,
-                                                 const Tile *__restrict__ tiles, const int *__restrict__ ntiles) {
-    const int nt = *ntiles;
-    for (int t = blockIdx.x; t < nt; t += gridDim.x) {
-        const Tile T = tiles[t];
-        const int ph = st[T.sys].phase;
-        if (ph != P_GO && ph != P_START && ph != P_CORRECT) continue;
-        dir_tile(gview(sd[T.sys]), T, ph, static_cast<float>(st[T.sys].beta), threadIdx.x);
-    }
+    double v[kParts] = {0, 0, 0};
+    for (int q = is_outer(ph) ? 2 : 0; q < kParts; ++q) v[q] = sum_parts(D, q, sh);
+    if (threadIdx.x == 0) decide(st[s], v[0], v[1], v[2], prm);
 }
 
 // Sum of a group of kBlock threads (the 4 waves from `gw`), with block_sum's
-// order: wave sums, then 0 + w0 + w1 + w2 + w3; valid in the group's first
-// thread.  Every thread of the workgroup calls it (it holds a barrier).
+// order: wave sums, then 0 + w0 + w1 + w2 + w3; valid in every thread of the
+// group.  Every thread of the workgroup calls it (it holds a barrier).
 __device__ __forceinline__ double group_sum(double v, double *gsh /* kBlock / 64 */, int gwave, int lane) {
     v = wave_sum(v);
     if (lane == 0) gsh[gwave] = v;
@@ -515,14 +534,14 @@ __device__ __forceinline__ double group_sum(double v, double *gsh /* kBlock / 64
 }
 
 // Systems of at most kSmallTiles tiles: one workgroup per system runs all
-// `iters` iterations of a step, K1 to K5 with barriers in place of launches.
+// `iters` steps of a replay, K1 to K3 with barriers in place of launches.
 // Group g of kBlock threads does what the tile kernels' workgroup does for
-// the system's tile g, and group 0 what the slot kernels do, in the same
+// the system's tile g, and group 0 what the slot kernel does, in the same
 // order of operations, so the results are those of the tile path bit for bit.
 constexpr int kSmallTiles = 4;
 constexpr int kSmallBlock = kSmallTiles * kBlock;
-// iterations of kb_small per step, in chunks: a small system's iterations are
-// short, and the step lasts as long as the tile launches anyway
+// steps of kb_small per replay, in chunks: a small system's steps are short,
+// and the replay lasts as long as the tile launches anyway
 constexpr int kSmallChunks = 4;
 
 __global__ __launch_bounds__(kSmallBlock) void kb_small(const SysDesc *__restrict__ sd, SysState *__restrict__ st,
@@ -531,6 +550,7 @@ __global__ __launch_bounds__(kSmallBlock) void kb_small(const SysDesc *__restric
     __shared__ double gsh[kSmallTiles][kBlock / 64];
     __shared__ int sdict[256];
     __shared__ SysState S;
+    __shared__ double tot[kParts];
     const int g = __builtin_amdgcn_readfirstlane(threadIdx.x / kBlock);
     const int tid = threadIdx.x % kBlock;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
@@ -543,51 +563,37 @@ __global__ __launch_bounds__(kSmallBlock) void kb_small(const SysDesc *__restric
         const bool mine = g < D.ntiles;   // this group's tile exists
         const Tile T{slot, g * kTileSlices, min(kTileSlices, D.nslice - g * kTileSlices), g};
         for (int it = 0; it < iters; ++it) {
-            int ph = S.phase;
-            if (ph == P_DONE || ph == P_IDLE) break;   // uniform
-            if (is_inner_spmv(ph) || is_outer_spmv(ph)) {
-                // K1, then K2 (group 0, the slot kernel's order)
-                const double tot = group_sum(mine ? spmv_tile(D, T, is_inner_spmv(ph), wave, lane, sdict) : 0.0,
-                                             gsh[g], wave, lane);
-                if (tid == 0 && mine) D.part[2 * g] = tot;
+            const SysState Sk = S;
+            const int ph = Sk.phase;
+            if (!works(ph)) break;   // uniform
+            // K1
+            double ru = 0, rr = 0;
+            if (mine) vec_tile(D, T, Sk, tid, ru, rr);
+            if (!is_outer(ph)) {
+                const double tru = group_sum(ru, gsh[g], wave, lane);
                 __syncthreads();
+                const double trr = group_sum(rr, gsh[g], wave, lane);
+                if (tid == 0 && mine) {
+                    D.part[kParts * g] = tru;
+                    D.part[kParts * g + 2] = trr;
+                }
+            }
+            __syncthreads();   // K2 gathers what K1 wrote across the groups
+            // K2
+            const double tsp = group_sum(mine ? spmv_tile(D, T, ph, wave, lane, sdict) : 0.0, gsh[g], wave, lane);
+            if (tid == 0 && mine) D.part[kParts * g + (is_outer(ph) ? 2 : 1)] = tsp;
+            __syncthreads();
+            // K3: the slot kernel's sums, by group 0
+            for (int q = is_outer(ph) ? 2 : 0; q < kParts; ++q) {
                 double v = 0;
                 if (g == 0)
-                    for (int i = tid; i < D.ntiles; i += kBlock) v += D.part[2 * i];
-                const double sum = group_sum(v, gsh[g], wave, lane);
-                if (threadIdx.x == 0) decide_after_spmv(S, sum, prm);
+                    for (int i = tid; i < D.ntiles; i += kBlock) v += D.part[kParts * i + q];
+                const double t = group_sum(v, gsh[g], wave, lane);
+                if (threadIdx.x == 0) tot[q] = t;
                 __syncthreads();
-                ph = S.phase;
             }
-            if (ph == P_UPD || ph == P_SETUP) {
-                // K3, then K4 (group 0)
-                double rr = 0, rz = 0;
-                if (mine) update_tile(D, T, ph, static_cast<float>(S.alpha), tid, rr, rz);
-                const double trr = group_sum(rr, gsh[g], wave, lane);
-                __syncthreads();
-                const double trz = group_sum(rz, gsh[g], wave, lane);
-                if (tid == 0 && mine) {
-                    D.part[2 * g] = trr;
-                    D.part[2 * g + 1] = trz;
-                }
-                __syncthreads();
-                double v0 = 0, v1 = 0;
-                if (g == 0)
-                    for (int i = tid; i < D.ntiles; i += kBlock) {
-                        v0 += D.part[2 * i];
-                        v1 += D.part[2 * i + 1];
-                    }
-                const double srr = group_sum(v0, gsh[g], wave, lane);
-                __syncthreads();
-                const double srz = group_sum(v1, gsh[g], wave, lane);
-                if (threadIdx.x == 0) decide_after_update(S, srr, srz, prm);
-                __syncthreads();
-                ph = S.phase;
-            }
-            if (ph == P_GO || ph == P_START || ph == P_CORRECT) {
-                if (mine) dir_tile(D, T, ph, static_cast<float>(S.beta), tid);
-                __syncthreads();   // the next SpMV gathers p (or x) across the groups
-            }
+            if (threadIdx.x == 0) decide(S, is_outer(ph) ? 0.0 : tot[0], is_outer(ph) ? 0.0 : tot[1], tot[2], prm);
+            __syncthreads();   // S, and the next K1 reads what this K2 wrote
         }
         if (threadIdx.x == 0) st[slot] = S;
         __syncthreads();
@@ -739,10 +745,11 @@ size_t up(size_t b) { return (b + kAlign - 1) / kAlign * kAlign; }
 // words > 0: coded columns (`words` code words, a word pointer per slice and
 // the offset table) instead of E column indices
 struct Layout {
-    size_t ptr, len, col, wptr, dict, v64, v32, invd, b, x, upload, r64, r, c, z, p, q, part, total;
+    size_t ptr, len, col, wptr, dict, v64, v32, invd, b, x, upload, r64, r, w, s, u, c, p, part, total;
     long long words;
     Layout(int n, int nslice, long long E, int ntiles, long long words_ = 0) : words(words_) {
         size_t o = 0;
+        const size_t vf = up(sizeof(float) * static_cast<size_t>(n));
         ptr = o; o += up(sizeof(int) * (nslice + 1));
         len = o; o += up(sizeof(int) * static_cast<size_t>(n));
         col = o; o += up(words ? sizeof(unsigned) * static_cast<size_t>(words) : sizeof(int) * static_cast<size_t>(E));
@@ -750,17 +757,18 @@ struct Layout {
         dict = o; o += words ? up(sizeof(int) * 256) : 0;
         v64 = o; o += up(sizeof(double) * static_cast<size_t>(E));
         v32 = o; o += up(sizeof(float) * static_cast<size_t>(E));
-        invd = o; o += up(sizeof(float) * static_cast<size_t>(n));
+        invd = o; o += vf;
         b = o; o += up(sizeof(double) * static_cast<size_t>(n));
         x = o; o += up(sizeof(double) * static_cast<size_t>(n));
         upload = o;
         r64 = o; o += up(sizeof(double) * static_cast<size_t>(n));
-        r = o; o += up(sizeof(float) * static_cast<size_t>(n));
-        c = o; o += up(sizeof(float) * static_cast<size_t>(n));
-        z = o; o += up(sizeof(float) * static_cast<size_t>(n));
-        p = o; o += up(sizeof(float) * static_cast<size_t>(n));
-        q = o; o += up(sizeof(float) * static_cast<size_t>(n));
-        part = o; o += up(sizeof(double) * 2 * static_cast<size_t>(ntiles));
+        r = o; o += vf;
+        w = o; o += vf;
+        s = o; o += vf;
+        u = o; o += vf;
+        c = o; o += vf;
+        p = o; o += vf;
+        part = o; o += up(sizeof(double) * kParts * static_cast<size_t>(ntiles));
         total = o;
     }
 };
@@ -902,10 +910,11 @@ int activate(hfh_batch *B, int slot, size_t off, const Layout &L, int n, int nsl
     d.x = reinterpret_cast<double *>(base + L.x);
     d.r64 = reinterpret_cast<double *>(base + L.r64);
     d.r = reinterpret_cast<float *>(base + L.r);
+    d.w = reinterpret_cast<float *>(base + L.w);
+    d.s = reinterpret_cast<float *>(base + L.s);
+    d.u = reinterpret_cast<float *>(base + L.u);
     d.c = reinterpret_cast<float *>(base + L.c);
-    d.z = reinterpret_cast<float *>(base + L.z);
     d.p = reinterpret_cast<float *>(base + L.p);
-    d.q = reinterpret_cast<float *>(base + L.q);
     d.part = reinterpret_cast<double *>(base + L.part);
     hfh_batch::Slot &S = B->slots[slot];
     S.id = B->next_id++;
@@ -972,14 +981,11 @@ int enqueue_iterations(hfh_batch *B, int iters) {
     const int nslots = static_cast<int>(B->slots.size());
     const KParams kp{B->prm.rtol, B->prm.atol, B->prm.inner_rtol, B->prm.outer_maxits, B->prm.inner_maxits};
     for (int it = 0; it < iters; ++it) {
+        hipLaunchKernelGGL(kb_vec, dim3(kTileGrid), dim3(kBlock), 0, s, B->d_sys, B->d_st, B->d_tiles, B->d_ntiles);
         hipLaunchKernelGGL(kb_spmv, dim3(kTileGrid), dim3(kBlock), 0, s, B->d_sys, B->d_st, B->d_tiles, B->d_ntiles);
-        hipLaunchKernelGGL(kb_fin1, dim3(nslots), dim3(kBlock), 0, s, B->d_sys, B->d_st, nslots, kp);
-        hipLaunchKernelGGL(kb_update, dim3(kTileGrid), dim3(kBlock), 0, s, B->d_sys, B->d_st, B->d_tiles,
-                           B->d_ntiles);
-        hipLaunchKernelGGL(kb_fin2, dim3(nslots), dim3(kBlock), 0, s, B->d_sys, B->d_st, nslots, kp);
-        hipLaunchKernelGGL(kb_dir, dim3(kTileGrid), dim3(kBlock), 0, s, B->d_sys, B->d_st, B->d_tiles, B->d_ntiles);
+        hipLaunchKernelGGL(kb_fin, dim3(nslots), dim3(kBlock), 0, s, B->d_sys, B->d_st, nslots, kp);
     }
-    return launch_check("hfh_batch iteration");
+    return launch_check("hfh_batch step");
 }
 
 // Marks the systems a state readback shows finished (the id guards against a
