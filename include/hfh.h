@@ -426,9 +426,11 @@ int hfh_dc_solve_ex(const hfh_csr *Ah, const hfh_csr *Al, hfh_prec *Pl, const hf
 /* ---- serving: many independent SPD systems solved together ----
  * No reference counterpart: the engine runs, for every admitted system, the
  * defect correction of MPIR<D,S>::Solve (mpir.cc:69-327) with an FP32 inner
- * Jacobi-PCG (cg.cc:84-248; lpreconditioner.cc:104-166) and FP64 true
- * residual, update and stop test, for all systems in the same batched
- * launches (a hipGraph of `chunk` iterations).  Systems are admitted while
+ * Jacobi-preconditioned CG (lpreconditioner.cc:104-166) in the
+ * Chronopoulos-Gear form of cg.cc:84-248 (the same iterates in exact
+ * arithmetic, one reduction per step) and FP64 true residual, update and
+ * stop test, for all systems in the same batched launches (a hipGraph of
+ * `chunk` steps, three launches each).  Systems are admitted while
  * others run, as slots and the HBM budget allow, and leave when collected.
  * A system's result does not depend on what else is in the batch: its
  * arithmetic and summation order are fixed by its own size.
