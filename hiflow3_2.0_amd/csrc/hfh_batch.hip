@@ -438,16 +438,6 @@ __global__ __launch_bounds__(kBlock) void kb_spmv(const SysDesc *__restrict__ sd
     }
 }
 
-// Deterministic sum of the system's partials q over its tiles (kBlock
-// threads strided over the tiles, then block_sum's tree); valid in thread 0.
-__device__ __forceinline__ double sum_parts(const GDesc &D, int q, double *sh) {
-    double v = 0;
-    for (int i = threadIdx.x; i < D.ntiles; i += kBlock) v += D.part[kParts * i + q];
-    const double tot = block_sum(v, sh);
-    __syncthreads();
-    return tot;
-}
-
 // The decision after K1 and K2 from the sums of their partials (g = r.u,
 // d = w.u, rho = r.r, or the true residual's r64.r64).
 __device__ __forceinline__ void decide(SysState &S, double g, double d, double rho, const KParams &prm) {
@@ -507,18 +497,41 @@ __device__ __forceinline__ void decide(SysState &S, double g, double d, double r
 }
 
 // K3: per system, after K1 and K2.
+// The slot's phase and its descriptor's fields are loaded together, then
+// all its partials in one pass.  Each quantity is summed in a fixed order:
+// threads strided over the tiles in ascending order, the wave trees, then
+// 0 + the four wave totals in order (kb_small's group 0 sums the same way).
 __global__ __launch_bounds__(kBlock) void kb_fin(const SysDesc *__restrict__ sd, SysState *__restrict__ st,
                                                  int nslots, KParams prm) {
-    __shared__ double sh[kBlock / 64];
+    __shared__ double sh[kParts][kBlock / 64];
     const int s = blockIdx.x;
     if (s >= nslots) return;
     const int ph = st[s].phase;
-    if (!works(ph)) return;
-    if (sd[s].small) return;   // kb_small's
-    const GDesc D = gview(sd[s]);
+    const int small = sd[s].small, nt = sd[s].ntiles;
+    const gptr<const double> part = (gptr<const double>)sd[s].part;
+    if (!works(ph) || small) return;   // idle or finished, or kb_small's
+    const int q0 = is_outer(ph) ? 2 : 0;
     double v[kParts] = {0, 0, 0};
-    for (int q = is_outer(ph) ? 2 : 0; q < kParts; ++q) v[q] = sum_parts(D, q, sh);
-    if (threadIdx.x == 0) decide(st[s], v[0], v[1], v[2], prm);
+    for (int i = threadIdx.x; i < nt; i += kBlock)
+#pragma unroll
+        for (int q = 0; q < kParts; ++q)
+            if (q >= q0) v[q] += part[kParts * i + q];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < kParts; ++q) {
+        const double t = wave_sum(v[q]);
+        if (lane == 0) sh[q][w] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double tot[kParts];
+#pragma unroll
+        for (int q = 0; q < kParts; ++q) {
+            tot[q] = 0;
+            for (int i = 0; i < kBlock / 64; ++i) tot[q] += sh[q][i];
+        }
+        decide(st[s], tot[0], tot[1], tot[2], prm);
+    }
 }
 
 // Sum of a group of kBlock threads (the 4 waves from `gw`), with block_sum's

@@ -720,6 +720,30 @@ def test_gmres_restarts(pkg, ctx, oracle):
     assert true_res(row, col, val, X.numpy(), rhs) < 3e-8
 
 
+@pytest.mark.parametrize("maxits", [7, 10, 11])
+def test_gmres_stops_at_maxits_mid_cycle(pkg, ctx, oracle, maxits):
+    # The device runs Arnoldi step j+1 while the host handles step j; a solve
+    # cut by maxits inside a cycle (7, 11) or at its end (10) must still stop
+    # at that step with the iterates of the sequential loop: the history is a
+    # prefix of an uncut solve's, bit for bit, and x matches the oracle's
+    row, col, val, rhs = pkg.problem(pkg.UPWIND_CONVDIFF, 12, p0=1e-2, p1=5.0, seed=8)
+    A = pkg.CsrMatrix(ctx, row, col, val)
+    P = pkg.Preconditioner.jacobi(A)
+    B = pkg.Vector(ctx, rhs)
+    print(f"gmres maxits case {maxits}", flush=True)
+    X = pkg.Vector(ctx, len(rhs))
+    info = pkg.gmres(A, B, X, P, m=5, maxits=maxits, atol=0.0, rtol=1e-14, history=True, check=False)
+    assert info["iters"] == maxits and info["state"] == pkg.FAIL_MAXITS
+    Xl = pkg.Vector(ctx, len(rhs))
+    full = pkg.gmres(A, B, Xl, P, m=5, maxits=3000, atol=0.0, rtol=1e-8, history=True)
+    assert full["iters"] > maxits
+    assert np.array_equal(info["hist"][: maxits + 1], full["hist"][: maxits + 1])
+    OA = oracle.Csr(row, col, val)
+    xo, oi = oracle.gmres(OA, rhs, P=oracle.Prec.jacobi(OA), m=5, ctrl=(maxits, 0.0, 1e-14, 1e6))
+    assert oi["iters"] == maxits
+    assert relnorm(X.numpy(), xo) < 1e-10
+
+
 def test_defect_correction_matches_fp64(pkg, ctx, oracle):
     row, col, val, rhs = fv3(pkg, 20, sigma=1.0, seed=5)
     nc, color, sizes, perm = pkg.multicoloring(row, col)
