@@ -676,6 +676,9 @@ def serving_bytes(shapes, inner):
     return float(np.sum(ns * inner * (8 * widths + 60))), ns
 
 
+FIXED_RATE = 116.0   # arrivals/s of the fixed-rate serving stream
+
+
 def bench_serving(P, args):
     ctx = P.Context(0)
     budget = 200 << 30
@@ -703,6 +706,15 @@ def bench_serving(P, args):
            "gpu_busy_frac": round(stats[1] * 1e-3 / wall, 3),
            "busy_GBps_algorithmic": round(inner_bytes / (stats[1] * 1e-3) / 1e9, 1) if stats[1] else None,
            "replays": stats[0], "ms_per_replay": round(stats[1] / max(stats[0], 1), 3), "closed_loop": closed}
+    # the same stream at a fixed arrival rate, comparable across engine
+    # versions whose capacity (and so whose 0.8x rate) differs
+    wall, lat, ok, _, _, stats = serve_stream(P, ctx, args.serve_requests, FIXED_RATE, args.serve_chunk, 10_000,
+                                              budget, args.serve_slots)
+    out["fixed_rate"] = {"arrival_rate_per_s": FIXED_RATE, "solves_per_s": round(args.serve_requests / wall, 2),
+                         "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 2),
+                         "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 2), "converged": int(ok),
+                         "ms_per_replay": round(stats[1] / max(stats[0], 1), 3)}
+    log(f"[serving] fixed rate {out['fixed_rate']}")
     if not args.no_cpu:
         # the oracle's MPIR with an FP32 Jacobi-CG inner solver (the engine's
         # algorithm) on the first requests of the same stream, within a budget
