@@ -28,7 +28,7 @@
 // code word w of a lane holds slots 4w..4w+3, ceil(L/4) words per lane of a
 // slice of width L, so the values keep the slice width.  A slot table holds
 // each system's descriptor (pointers, sizes) and its solver state.  A tile
-// table lists every active system's tiles (32 slices = 2048 rows each).
+// table lists every active system's tiles (16 slices = 1024 rows each).
 //
 // One step of all systems is three launches:
 //   K1 tiles : inner  p = u + beta p, s = w + beta s, c += alpha p,
@@ -57,13 +57,16 @@
 namespace hfh {
 namespace {
 
-constexpr int kTileSlices = 32;     // 2048 rows per tile
+constexpr int kTileSlices = 16;     // 1024 rows per tile
 constexpr int kTileRows = kTileSlices * 64;
-// Workgroups of the tile kernels: four times the resident ones, so the
+// Workgroups of the tile kernels: eight times the resident ones, so the
 // hardware hands out tiles as workgroups finish (a fixed stride over many
 // tiles leaves the slowest workgroups last: closed-loop capacity 142 -> 159
-// solves/s from 2,048 to 8,192; the workgroups past the last tile end at once).
-constexpr int kTileGrid = 8192;
+// solves/s from 2,048 to 8,192; the workgroups past the last tile end at
+// once).  Tiles of 1,024 rows give a lightly loaded step twice the
+// workgroups of 2,048-row tiles and half the slices per wave (0.59 -> 0.49
+// ms per 16-step replay at 116 arrivals/s), at the same capacity.
+constexpr int kTileGrid = 16384;
 constexpr int kSellU = 8;           // column slots per SELL step
 constexpr size_t kAlign = 256;
 constexpr int kParts = 3;           // partials per tile: r.u, w.u, r.r (outer: 0, 0, r64.r64)

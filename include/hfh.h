@@ -446,7 +446,7 @@ typedef struct {
     int outer_maxits;      /* corrections per system */
     double inner_rtol;     /* inner stop */
     int inner_maxits;      /* iterations per inner solve */
-    /* 1..4: systems of at most this many tiles of 2048 rows run 4 * `chunk`
+    /* 1..4: systems of at most this many tiles of 1024 rows run 4 * `chunk`
      * iterations per step in one persistent workgroup each (kb_small, on a
      * second stream), beside the batched tile launches of the larger ones;
      * 0 (the default): every system in the tile launches.  Both paths give

@@ -192,14 +192,15 @@ def test_batch_rejects_bad_requests(pkg, ctx):
 
 
 def test_small_systems_persistent_equals_tiles(pkg, ctx):
-    """With small_tiles = 4, systems of at most 4 tiles (8,192 rows) run a
+    """With small_tiles = 4, systems of at most 4 tiles (4,096 rows) run a
     step's iterations in one persistent workgroup each (kb_small) beside the
     tile launches of the larger ones.  Group g of the workgroup does what the
     tile path does for tile g, in the same order: the results equal the tile
-    path's bit for bit, in a mix with a large system."""
-    specs = [(9, 7, 1, 31), (40, 40, 1, 32), (20, 20, 20, 33), (90, 90, 1, 34), (64, 64, 2, 35), (130, 120, 1, 36)]
+    path's bit for bit, in a mix with larger systems."""
+    specs = [(9, 7, 1, 31), (40, 40, 1, 32), (16, 16, 12, 33), (64, 32, 2, 35), (20, 20, 20, 34), (130, 120, 1, 36)]
     data = {s: fv(pkg, *s) for s in specs}
-    assert (130 * 120 + 2047) // 2048 > 4 and (64 * 64 * 2 + 2047) // 2048 == 4
+    tiles = [(a * b * c + 1023) // 1024 for a, b, c, _ in specs]
+    assert tiles[:4] == [1, 2, 3, 4] and min(tiles[4:]) > 4
     res = {}
     for small_tiles in (4, 0):
         E = engine(pkg, ctx, small_tiles=small_tiles)
