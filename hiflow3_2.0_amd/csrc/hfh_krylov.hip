@@ -461,14 +461,30 @@ static int pcg_loop(PcgRun<T> &R, int maxits, bool use_graph, bool allow_timing 
         R.exec_x = R.x;
         R.exec_chunk = chunk;
     }
-    int *hflag = reinterpret_cast<int *>(c->host_res);
+    // Chunk k+1 is launched before the host reads chunk k's state (copied
+    // into one of two pinned slots, each with an event), unless chunk k is
+    // expected to end the solve: the residual after it, extrapolated from
+    // the last two chunks read, meets the tolerance.  A chunk launched after
+    // the end finds the state done and every kernel returns at once, so the
+    // iterates are those of the one-chunk-at-a-time loop.  (With timing on,
+    // one chunk at a time.)
+    struct Pipe {
+        hipEvent_t ev[2] = {nullptr, nullptr};
+        ~Pipe() {
+            for (hipEvent_t e : ev)
+                if (e) (void)hipEventDestroy(e);
+        }
+    } pipe;
+    for (hipEvent_t &e : pipe.ev) HFH_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    static_assert(sizeof(KState) <= 128 * sizeof(double), "two state slots in the pinned result buffer");
+    KState *slot[2] = {reinterpret_cast<KState *>(c->host_res), reinterpret_cast<KState *>(c->host_res + 128)};
     std::vector<hipEvent_t> evs;
-    long long launched = 0;
-    for (;;) {
+    auto launch_chunk = [&](int k) -> int {
         if (exec) {
             HFH_HIP(hipGraphLaunch(exec, s));
         } else {
-            for (int k = 0; k < chunk && !rc; ++k) {
+            int r2 = HFH_OK;
+            for (int i = 0; i < chunk && !r2; ++i) {
                 if (timing) {
                     hipEvent_t a, b;
                     HFH_HIP(hipEventCreate(&a));
@@ -476,22 +492,44 @@ static int pcg_loop(PcgRun<T> &R, int maxits, bool use_graph, bool allow_timing 
                     evs.push_back(a);
                     evs.push_back(b);
                     HFH_HIP(hipEventRecord(a, s));
-                    rc = R.spmv_kernel();
+                    r2 = R.spmv_kernel();
                     HFH_HIP(hipEventRecord(b, s));
-                    if (!rc) rc = R.fin_pap();
-                    if (!rc) rc = R.tail();
+                    if (!r2) r2 = R.fin_pap();
+                    if (!r2) r2 = R.tail();
                 } else {
-                    rc = R.iteration();
+                    r2 = R.iteration();
                 }
             }
+            if (r2) return r2;
         }
-        if (rc) break;
-        launched += chunk;
-        HFH_HIP(hipMemcpyAsync(hflag, &R.st->done, sizeof(int), hipMemcpyDeviceToHost, s));
-        HFH_HIP(hipStreamSynchronize(s));
-        if (*hflag) break;
-        if (launched > static_cast<long long>(maxits) + chunk) break;  // the device test stops at maxits
+        HFH_HIP(hipMemcpyAsync(slot[k & 1], R.st, sizeof(KState), hipMemcpyDeviceToHost, s));
+        HFH_HIP(hipEventRecord(pipe.ev[k & 1], s));
+        return HFH_OK;
+    };
+    double ra = -1, rb = -1;   // residuals after the last two chunks read (ra the later)
+    int queued = 0;            // chunks enqueued
+    for (int k = 0;; ++k) {
+        if (queued == k) {
+            if ((rc = launch_chunk(k))) break;
+            queued = k + 1;
+        }
+        if (!timing && static_cast<long long>(k + 1) * chunk <= maxits) {
+            const KState &h = *slot[(k + 1) & 1];   // chunk k-1's copy: tolerances and res0
+            const bool ends = k >= 2 && ra > 0 && rb > 0 &&
+                              (ra * (ra / rb) < h.atol || ra * (ra / rb) < h.rtol * h.res0);
+            if (!ends) {
+                if ((rc = launch_chunk(k + 1))) break;
+                queued = k + 2;
+            }
+        }
+        HFH_HIP(hipEventSynchronize(pipe.ev[k & 1]));
+        const KState h = *slot[k & 1];
+        if (h.done) break;
+        rb = ra;
+        ra = h.res;
+        if (static_cast<long long>(k + 1) * chunk > static_cast<long long>(maxits) + chunk) break;  // the device test stops at maxits
     }
+    if (queued > 0) HFH_HIP(hipStreamSynchronize(s));
     if (!evs.empty()) {
         int it = 0;
         (void)hipStreamSynchronize(s);
