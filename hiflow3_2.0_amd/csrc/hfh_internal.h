@@ -155,6 +155,12 @@ struct hfh_tsweep {
     int *dpos = nullptr;       // [64 nslice] CSR position of each lane's diagonal (-1: padding)
     void *val = nullptr;       // [nent] values
     void *dval = nullptr;      // [64 nslice] diagonal values
+    // Blocked form only (slices are geometric blocks of a structured grid,
+    // in block-level order): a column below 0 names the lane -(col + 1) of
+    // the same block, whose result arrives through a cross-lane shuffle
+    int *lev = nullptr;        // [64 nslice] level of each lane inside its block (-1: padding)
+    int *bnl = nullptr;        // [nslice] levels inside each block
+    int kmax = 0;              // most entries of any lane
 };
 
 struct hfh_prec {
@@ -172,6 +178,7 @@ struct hfh_prec {
     hfh_levels lo, up;              // SGS / ILU0 level schedules
     hfh_sweep fw, bw;               // MC-SGS: lower-colour / upper-colour entries per row
     hfh_tsweep tlo, tup;            // SGS / ILU0: forward / backward sweep rows by level
+    hfh_tsweep blo, bup;            // SGS / ILU0 on a structured grid: the sweeps by geometric block
     unsigned tversion = 0;          // A->version the sweep values were copied at
     // SGS / ILU0 outside the fused drivers: both sweeps captured once as a
     // hipGraph on internal in/out buffers (one launch instead of one per level)
@@ -181,7 +188,7 @@ struct hfh_prec {
     bool graph_failed = false;
     // SGS / ILU0 sweep schedule: sync-free (one launch per sweep) or one
     // launch per level; the sync-free form keeps the forward results in sf_y
-    // and its ticket counters and failure word in sf_ctl[0..2]
+    // and its ticket counters and failure word in sf_ctl
     int sweep_mode = HFH_SWEEP_SYNCFREE;
     int sf_waves = 0;                  // sync-free: waves per CU (0: from the level widths)
     void *sf_y = nullptr;

@@ -364,14 +364,22 @@ __global__ __launch_bounds__(kMergeBlock) void k_tsweep_run(const int *__restric
 
 // ---------------------------------------------------------------------------
 // Sync-free triangular sweeps: one launch per sweep instead of one per level.
-// Waves take slices in level order from a ticket counter; a lane needing the
-// result of an earlier row polls that result itself.  Every result starts as
+// Waves take slices in level order from ticket counters; a lane needing the
+// result of an earlier row polls that result itself.  One counter hands out
+// a ticket about every 11 ns (same-address atomics serialise), which bounds
+// a sweep of 32,768 slices at about 0.4 ms: the tickets are dealt over
+// kSfClasses counters instead, each on its own 128-byte line, and the waves
+// of workgroup g take the tickets t = g mod kSfClasses (mod kSfClasses) in
+// order from counter g mod kSfClasses.  Every result starts as
 // a signalling-NaN sentinel (k_sf_fill, before the forward sweep).
 // Arithmetic never yields a signalling NaN, so a value other than the
 // sentinel is final.  Results are stored and polled with agent-scope relaxed
 // atomics: coherent across the XCDs' L2 caches, with no fence.  A wave waits
-// only on slices with smaller tickets, whose waves are already running or
-// done, so the smallest unfinished slice always progresses.  Every wait is
+// only on slices with smaller tickets, so the smallest unfinished slice
+// always progresses: either a wave holds it, or it is the next ticket of its
+// class, whose waves have finished every earlier (smaller) ticket of the
+// class and take it next.  Every class has waves: the grid is a multiple of
+// kSfClasses workgroups, few enough to be resident together.  Every wait is
 // bounded: after kSfSpins polls, or once any lane has given up, a lane
 // raises the failure word and goes on with the sentinel (the NaN then spreads
 // through the results), so every wave ends and the drivers report the
@@ -389,6 +397,22 @@ template <> struct Sent<float> {
     static constexpr U quiet = 1u << 22;
 };
 constexpr int kSfSpins = 1 << 20;
+constexpr int kSfClasses = 8;
+// ctl: [2] failure word, [3] the most polls any lane needed for one result;
+// ticket counter c of the forward (backward) sweep at int 32 (1 + c)
+// (32 (1 + kSfClasses + c))
+constexpr int kSfCtlInts = 32 * (1 + 2 * kSfClasses);
+
+__device__ __forceinline__ int *sf_ticket(int *ctl, bool backward) {
+    return ctl + 32 * (1 + (backward ? kSfClasses : 0) + static_cast<int>(blockIdx.x % kSfClasses));
+}
+
+// The next ticket of this workgroup's class (wave-uniform).
+__device__ __forceinline__ int sf_next(int *counter, int lane) {
+    int t = 0;
+    if (lane == 0) t = __hip_atomic_fetch_add(counter, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __builtin_amdgcn_readfirstlane(t) * kSfClasses + static_cast<int>(blockIdx.x % kSfClasses);
+}
 
 template <typename T>
 __device__ __forceinline__ typename Sent<T>::U sf_load(const T *p) {
@@ -434,7 +458,7 @@ __device__ __forceinline__ void sf_wait_all(const T *out, const int *c, typename
     most = max(most, spins);
 }
 
-// y and z to the sentinel, the two ticket counters to 0 (ctl[0], ctl[1]).
+// y and z to the sentinel, every ticket counter to 0.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_sf_fill(int n, T *y, T *z, int *ctl, const int *__restrict__ done) {
     if (done && *done) return;
@@ -444,10 +468,7 @@ __global__ __launch_bounds__(kBlock) void k_sf_fill(int n, T *y, T *z, int *ctl,
         reinterpret_cast<U *>(y)[i] = Sent<T>::bits;
         reinterpret_cast<U *>(z)[i] = Sent<T>::bits;
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        ctl[0] = 0;
-        ctl[1] = 0;
-    }
+    if (blockIdx.x == 0 && threadIdx.x < 2 * kSfClasses) ctl[32 * (1 + threadIdx.x)] = 0;
 }
 
 // One slice of a sync-free sweep; MODE as tsweep_slice.  Forward (0, 2):
@@ -491,8 +512,7 @@ __device__ __forceinline__ void sf_slice(int s, int lane, const int *__restrict_
 }
 
 // A whole sweep: each wave takes one slice per ticket until every slice is
-// taken.  ctl: [0] forward ticket, [1] backward ticket,
-// [2] failure word, [3] the most polls any lane needed for one result.
+// taken (ctl: see kSfCtlInts).
 template <typename T, int MODE>
 __global__ __launch_bounds__(kBlock) void k_tsweep_sf(int nslice, const int *__restrict__ ptr,
                                                       const int *__restrict__ srow, const int *__restrict__ slen,
@@ -502,14 +522,90 @@ __global__ __launch_bounds__(kBlock) void k_tsweep_sf(int nslice, const int *__r
                                                       const int *__restrict__ done) {
     if (done && *done) return;
     const int lane = threadIdx.x & 63;
-    int *ticket = ctl + ((MODE == 1 || MODE == 3) ? 1 : 0);
+    int *ticket = sf_ticket(ctl, MODE == 1 || MODE == 3);
     int most = 0;
     for (;;) {
-        int t = 0;
-        if (lane == 0) t = __hip_atomic_fetch_add(ticket, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        t = __builtin_amdgcn_readfirstlane(t);
+        const int t = sf_next(ticket, lane);
         if (t >= nslice) break;
         sf_slice<T, MODE>(t, lane, ptr, srow, slen, scol, sval, dval, in, yin, out, ctl + 2, most);
+    }
+    most = wave_max(most);
+    if (lane == 0 && most > 0) __hip_atomic_fetch_max(ctl + 3, most, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Sync-free sweep over geometric blocks of a structured grid (4x4x4 or
+// 8x8 cells, at most 64 rows, one wave each), taken in block-level order
+// from the ticket counters.  A wave first waits for every result its block
+// needs from other blocks (bounded, as sf_slice), then runs the block's own
+// levels: at each, the rows of that level read the results of their own
+// block from the wave's LDS slots (written at earlier levels by the lanes
+// that computed them; a wave's LDS accesses complete in issue order), finish,
+// and write their own.  One memory round trip per block instead of one per
+// row level: a 128^3 7-point sweep has 94 block levels against 382 row
+// levels.  Each row's arithmetic is sf_slice's (the starting value, the
+// entries in stored order, then the diagonal): results are bit-identical.
+// K bounds the entries of any lane.
+template <typename T, int MODE, int K>
+__global__ __launch_bounds__(kBlock) void k_bsweep_sf(int nblk, const int *__restrict__ ptr,
+                                                      const int *__restrict__ brow, const int *__restrict__ blen,
+                                                      const int *__restrict__ blev, const int *__restrict__ bnl,
+                                                      const int *__restrict__ bcol, const T *__restrict__ bval,
+                                                      const T *__restrict__ dval, const T *__restrict__ in,
+                                                      const T *__restrict__ yin, T *out, int *ctl,
+                                                      const int *__restrict__ done) {
+    if (done && *done) return;
+    typedef typename Sent<T>::U U;
+    __shared__ T xch[kBlock];   // each lane's result, for the later rows of its block
+    const int lane = threadIdx.x & 63;
+    T *wx = xch + (threadIdx.x & ~63);
+    int *ticket = sf_ticket(ctl, MODE == 1 || MODE == 3);
+    int most = 0;
+    for (;;) {
+        const int b = sf_next(ticket, lane);
+        if (b >= nblk) break;
+        const int p = b * 64 + lane;
+        const int base = ptr[b], L = (ptr[b + 1] - base) >> 6, nl = bnl[b];
+        const int i = brow[p], len = blen[p], my = blev[p];   // padding lane: -1, 0, -1
+        int c[K];
+        T v[K];
+        U w[K];
+#pragma unroll
+        for (int u = 0; u < K; ++u) {
+            c[u] = 0;
+            v[u] = T(0);
+            if (u < L) {
+                c[u] = bcol[base + u * 64 + lane];
+                v[u] = bval[base + u * 64 + lane];
+            }
+        }
+        // the row's own operands before the wait, off the critical path
+        T d = T(1), acc0 = T(0);
+        if (i >= 0) {
+            d = dval[p];
+            acc0 = MODE == 0 || MODE == 2 ? in[i] : yin[i];
+        }
+#pragma unroll
+        for (int u = 0; u < K; ++u) w[u] = u < len && c[u] >= 0 ? sf_load(out + c[u]) : U(0);
+        // only results from other blocks can still be the sentinel
+        sf_wait_all<T, K>(out, c, w, len, ctl + 2, most);
+        if (MODE == 1) acc0 = mul_rn(acc0, d);
+        for (int l = 0; l < nl; ++l) {
+            __builtin_amdgcn_wave_barrier();
+            if (l == my) {
+                T x[K];
+#pragma unroll
+                for (int u = 0; u < K; ++u)   // all reads issued before the first use
+                    if (u < L) x[u] = c[u] < 0 ? wx[-1 - c[u]] : __builtin_bit_cast(T, w[u]);
+                T acc = acc0;
+#pragma unroll
+                for (int u = 0; u < K; ++u)
+                    if (u < len) acc = add_rn(acc, -mul_rn(x[u], v[u]));
+                if (MODE != 2) acc = acc / d;
+                wx[lane] = acc;
+                sf_store(out + i, acc);
+            }
+        }
+        __builtin_amdgcn_wave_barrier();   // the block's reads before the next block's writes
     }
     most = wave_max(most);
     if (lane == 0 && most > 0) __hip_atomic_fetch_max(ctl + 3, most, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -701,6 +797,19 @@ static void sweep(const hfh_levels &Lv, const hfh_tsweep &W, const T *in, T *out
     }
 }
 
+// One blocked sweep, its entry bound K from the most entries of any lane.
+template <typename T, int MODE>
+static void bsweep_launch(const hfh_tsweep &W, int grid, const T *in, const T *yin, T *out, int *ctl,
+                          const int *done, hipStream_t s) {
+#define HFH_BSWEEP(K)                                                                                             \
+    hipLaunchKernelGGL((k_bsweep_sf<T, MODE, K>), dim3(grid), dim3(kBlock), 0, s, W.nslice, W.ptr, W.row, W.len, \
+                       W.lev, W.bnl, W.col, (const T *)W.val, (const T *)W.dval, in, yin, out, ctl, done)
+    if (W.kmax <= 4) HFH_BSWEEP(4);
+    else if (W.kmax <= 8) HFH_BSWEEP(8);
+    else HFH_BSWEEP(16);
+#undef HFH_BSWEEP
+}
+
 // Sync-free form of both sweeps: the fill, the forward sweep into P->sf_y,
 // the backward sweep into out.
 template <typename T, int FW, int BW>
@@ -724,23 +833,34 @@ static void sf_sweeps(hfh_prec *P, const T *in, T *out, const int *done) {
             waves = std::min(ncu * 8, std::max(ncu, 3 * ((W.nslice + nlev - 1) / nlev)));
         }
         waves = std::min(waves, W.nslice);
-        return std::max(1, (waves + kBlock / 64 - 1) / (kBlock / 64));
+        const int wg = (waves + kBlock / 64 - 1) / (kBlock / 64);
+        return std::max(1, (wg + kSfClasses - 1) / kSfClasses) * kSfClasses;   // every class has waves
     };
-    const hfh_tsweep &L = P->tlo, &U = P->tup;
-    if (L.nslice > 0)
-        hipLaunchKernelGGL((k_tsweep_sf<T, FW>), dim3(grid(L)), dim3(kBlock), 0, s, L.nslice, L.ptr, L.row,
-                           L.len, L.col, (const T *)L.val, (const T *)L.dval, in, (const T *)nullptr, y, P->sf_ctl,
-                           done);
-    if (U.nslice > 0)
-        hipLaunchKernelGGL((k_tsweep_sf<T, BW>), dim3(grid(U)), dim3(kBlock), 0, s, U.nslice, U.ptr, U.row,
-                           U.len, U.col, (const T *)U.val, (const T *)U.dval, (const T *)nullptr, (const T *)y, out,
-                           P->sf_ctl, done);
+    // geometric blocks where the grid was recognised (HFH_SWEEP_SYNCFREE),
+    // else slices of one level
+    const bool blocked = P->sweep_mode == HFH_SWEEP_SYNCFREE;
+    const hfh_tsweep &L = blocked && P->blo.nslice > 0 ? P->blo : P->tlo;
+    const hfh_tsweep &U = blocked && P->bup.nslice > 0 ? P->bup : P->tup;
+    if (L.nslice > 0) {
+        if (L.lev) bsweep_launch<T, FW>(L, grid(L), in, nullptr, y, P->sf_ctl, done, s);
+        else
+            hipLaunchKernelGGL((k_tsweep_sf<T, FW>), dim3(grid(L)), dim3(kBlock), 0, s, L.nslice, L.ptr, L.row,
+                               L.len, L.col, (const T *)L.val, (const T *)L.dval, in, (const T *)nullptr, y,
+                               P->sf_ctl, done);
+    }
+    if (U.nslice > 0) {
+        if (U.lev) bsweep_launch<T, BW>(U, grid(U), nullptr, y, out, P->sf_ctl, done, s);
+        else
+            hipLaunchKernelGGL((k_tsweep_sf<T, BW>), dim3(grid(U)), dim3(kBlock), 0, s, U.nslice, U.ptr, U.row,
+                               U.len, U.col, (const T *)U.val, (const T *)U.dval, (const T *)nullptr, (const T *)y,
+                               out, P->sf_ctl, done);
+    }
 }
 
 template <typename T>
 static int trsv_sweeps(hfh_prec *P, const T *in, T *out, const int *done) {
     hipStream_t s = P->ctx->stream;
-    if (P->sweep_mode == HFH_SWEEP_SYNCFREE && P->sf_y && in != out) {
+    if (P->sweep_mode != HFH_SWEEP_LEVELS && P->sf_y && in != out) {
         if (P->kind == HFH_PREC_ILU0) sf_sweeps<T, 2, 3>(P, in, out, done);
         else sf_sweeps<T, 0, 1>(P, in, out, done);
         return launch_check("k_tsweep_sf");
@@ -1083,8 +1203,209 @@ static int build_tsweep(const hfh_csr *A, const hfh_levels &Lv, const int *lrows
     return tsweep_fill(A, W);
 }
 
+// Geometry of a natural-order structured grid, read from the offsets i - c
+// of the lower entries of a sample of rows (7-, 9-, 27-point stencils and
+// the like): nx is the middle one of the smallest cluster of frequent
+// offsets above 1, nxy that of the next cluster (0: a 2D grid).  False when
+// the pattern shows no such grid.
+static bool detect_grid(const hfh_csr *A, int &nx, long long &nxy) {
+    const std::vector<int> &row = A->h_row, &col = A->h_col;
+    const int n = A->nrow;
+    if (n < 4096) return false;
+    const int step = std::max(1, n / 65536);
+    std::vector<long long> off;
+    long long rows = 0;
+    for (int i = 0; i < n; i += step, ++rows)
+        for (int j = row[i]; j < row[i + 1]; ++j)
+            if (col[j] < i - 1) off.push_back(static_cast<long long>(i) - col[j]);
+    std::sort(off.begin(), off.end());
+    std::vector<long long> freq;   // offsets in at least 1/8 of the sampled rows
+    for (size_t a = 0; a < off.size();) {
+        size_t e = a;
+        while (e < off.size() && off[e] == off[a]) ++e;
+        if (8LL * static_cast<long long>(e - a) >= rows) freq.push_back(off[a]);
+        a = e;
+    }
+    if (freq.empty()) return false;
+    size_t e1 = 0;
+    while (e1 < freq.size() && freq[e1] < 2 * freq[0]) ++e1;
+    nx = static_cast<int>(freq[(e1 - 1) / 2]);
+    nxy = 0;
+    if (nx < 8) return false;
+    if (e1 < freq.size()) {
+        size_t e2 = e1;
+        while (e2 < freq.size() && freq[e2] <= freq[e1] + 2LL * nx + 2) ++e2;
+        nxy = freq[e1 + (e2 - e1 - 1) / 2];
+        if (nxy % nx != 0 || nxy / nx < 4) return false;
+    }
+    return true;
+}
+
+// Builds the blocked copy of one sweep (k_bsweep_sf) for an nx (x nxy / nx)
+// grid: blocks of 4x4x4 cells (3D) or 8x8 (2D), each row's level inside its
+// block from its dependencies there, the blocks ordered by their level in
+// the graph of dependencies between blocks.  With `shear` the blocks tile
+// the coordinates (x + y + 2z, y + z, z) instead: a 9- or 27-point stencil
+// has a lower neighbour at (x + 1, y - 1), which makes side-by-side blocks
+// wait for each other, while in those coordinates no lower neighbour lies
+// ahead in any direction.  Leaves W empty (the slice form then runs) if the
+// graph between blocks has a cycle or a row more than 16 entries.
+static int build_bsweep(const hfh_csr *A, int nx, long long nxy, bool forward, bool shear, hfh_tsweep &W) {
+    const std::vector<int> &row = A->h_row, &col = A->h_col;
+    const int n = A->nrow;
+    const int bx = nxy ? 4 : 8, by = bx, bz = nxy ? 4 : 1;
+    const long long ny = nxy ? nxy / nx : (n + nx - 1) / nx;
+    const long long nz = nxy ? (n + nxy - 1) / nxy : 1;
+    const long long xr = shear ? nx + ny + 2 * nz : nx, yr = shear ? ny + nz : ny;
+    const long long nbx = (xr + bx - 1) / bx, nby = (yr + by - 1) / by, nbz = (nz + bz - 1) / bz;
+    if (nbx * nby * nbz >= (1LL << 30)) return HFH_OK;
+    const int nraw = static_cast<int>(nbx * nby * nbz);
+    std::vector<int> bid(n), bcnt(nraw + 1, 0);
+    for (int i = 0; i < n; ++i) {
+        long long x = i % nx, t = i / nx, y = t % ny, z = t / ny;
+        if (shear) {
+            x += y + 2 * z;
+            y += z;
+        }
+        bid[i] = static_cast<int>((z / bz * nby + y / by) * nbx + x / bx);
+        ++bcnt[bid[i] + 1];
+    }
+    for (int b = 0; b < nraw; ++b) bcnt[b + 1] += bcnt[b];
+    // rows of each raw block, ascending; a row's lane is its place there
+    std::vector<int> brows(n), lane(n), fill(bcnt.begin(), bcnt.end() - 1);
+    for (int i = 0; i < n; ++i) {
+        lane[i] = fill[bid[i]] - bcnt[bid[i]];
+        brows[fill[bid[i]]++] = i;
+    }
+    auto dep = [&](int c, int i) { return forward ? c < i : c > i; };
+    // level of each row inside its block (dependencies there come first in
+    // row order, ascending forward, descending backward) and the edges
+    // between blocks
+    std::vector<int> ilev(n, 0);
+    std::vector<std::pair<int, int>> edges;   // (block, block it waits for)
+    int kmax = 0;
+    for (int q = 0; q < n; ++q) {
+        const int i = forward ? q : n - 1 - q;
+        int l = 0, cnt = 0;
+        for (int j = row[i]; j < row[i + 1]; ++j) {
+            const int c = col[j];
+            if (!dep(c, i)) continue;
+            ++cnt;
+            if (bid[c] == bid[i]) l = std::max(l, ilev[c] + 1);
+            else edges.emplace_back(bid[i], bid[c]);
+        }
+        ilev[i] = l;
+        kmax = std::max(kmax, cnt);
+    }
+    if (kmax > 16) return HFH_OK;
+    std::sort(edges.begin(), edges.end());
+    edges.erase(std::unique(edges.begin(), edges.end()), edges.end());
+    // block levels (Kahn): a block runs after every block it waits for
+    std::vector<int> indeg(nraw, 0), sptr(nraw + 1, 0), succ(edges.size()), blev(nraw, 0);
+    for (const auto &e : edges) {
+        ++indeg[e.first];
+        ++sptr[e.second + 1];
+    }
+    for (int b = 0; b < nraw; ++b) sptr[b + 1] += sptr[b];
+    {
+        std::vector<int> f(sptr.begin(), sptr.end() - 1);
+        for (const auto &e : edges) succ[f[e.second]++] = e.first;
+    }
+    std::vector<int> queue;
+    queue.reserve(nraw);
+    for (int b = 0; b < nraw; ++b)
+        if (!indeg[b]) queue.push_back(b);
+    for (size_t h = 0; h < queue.size(); ++h) {
+        const int b = queue[h];
+        for (int k = sptr[b]; k < sptr[b + 1]; ++k) {
+            const int t = succ[k];
+            blev[t] = std::max(blev[t], blev[b] + 1);
+            if (--indeg[t] == 0) queue.push_back(t);
+        }
+    }
+    if (static_cast<int>(queue.size()) != nraw) return HFH_OK;   // a cycle
+    // non-empty blocks in (level, block) order
+    int nlev = 0;
+    for (int b = 0; b < nraw; ++b) nlev = std::max(nlev, blev[b] + 1);
+    std::vector<int> lcnt(nlev + 1, 0);
+    for (int b = 0; b < nraw; ++b)
+        if (bcnt[b + 1] > bcnt[b]) ++lcnt[blev[b] + 1];
+    for (int l = 0; l < nlev; ++l) lcnt[l + 1] += lcnt[l];
+    const int nblk = lcnt[nlev];
+    std::vector<int> order(nblk);
+    {
+        std::vector<int> f(lcnt.begin(), lcnt.end() - 1);
+        for (int b = 0; b < nraw; ++b)
+            if (bcnt[b + 1] > bcnt[b]) order[f[blev[b]]++] = b;
+    }
+    W.lslice.assign(lcnt.begin(), lcnt.end());
+    const size_t nl64 = 64ULL * std::max(nblk, 1);
+    std::vector<int> xptr(nblk + 1, 0), xrow(nl64, -1), xlen(nl64, 0), xlev(nl64, -1), xdpos(nl64, -1),
+        xnl(std::max(nblk, 1), 0);
+    long long total = 0;
+    for (int t = 0; t < nblk; ++t) {
+        const int b = order[t];
+        int L = 0, nl = 0;
+        for (int k = bcnt[b]; k < bcnt[b + 1]; ++k) {
+            const int i = brows[k], p = t * 64 + lane[i];
+            int cnt = 0;
+            for (int j = row[i]; j < row[i + 1]; ++j) {
+                if (col[j] == i) xdpos[p] = j;
+                cnt += dep(col[j], i);
+            }
+            xrow[p] = i;
+            xlen[p] = cnt;
+            xlev[p] = ilev[i];
+            L = std::max(L, cnt);
+            nl = std::max(nl, ilev[i] + 1);
+        }
+        xnl[t] = nl;
+        xptr[t] = static_cast<int>(total);
+        total += 64LL * L;
+        HFH_CHECK(total < (1LL << 31), HFH_ERR_UNSUPPORTED, "sweep copy exceeds 2^31 entries");
+    }
+    xptr[nblk] = static_cast<int>(total);
+    std::vector<int> xcol(std::max(total, 1LL), 0), xpos(std::max(total, 1LL), -1);
+    for (int t = 0; t < nblk; ++t)
+        for (int ln = 0; ln < 64; ++ln) {
+            const int p = t * 64 + ln, i = xrow[p];
+            if (i < 0) continue;
+            int k = 0;
+            for (int j = row[i]; j < row[i + 1]; ++j)
+                if (dep(col[j], i)) {
+                    const long long idx = xptr[t] + 64LL * k + ln;
+                    xcol[idx] = bid[col[j]] == bid[i] ? -1 - lane[col[j]] : col[j];
+                    xpos[idx] = j;
+                    ++k;
+                }
+        }
+    W.nslice = nblk;
+    W.nent = total;
+    W.kmax = kmax;
+    int rc;
+    if ((rc = dev_alloc(reinterpret_cast<void **>(&W.ptr), sizeof(int) * xptr.size())) ||
+        (rc = dev_alloc(reinterpret_cast<void **>(&W.row), sizeof(int) * xrow.size())) ||
+        (rc = dev_alloc(reinterpret_cast<void **>(&W.len), sizeof(int) * xlen.size())) ||
+        (rc = dev_alloc(reinterpret_cast<void **>(&W.lev), sizeof(int) * xlev.size())) ||
+        (rc = dev_alloc(reinterpret_cast<void **>(&W.bnl), sizeof(int) * xnl.size())) ||
+        (rc = dev_alloc(reinterpret_cast<void **>(&W.dpos), sizeof(int) * xdpos.size())) ||
+        (rc = dev_alloc(reinterpret_cast<void **>(&W.col), sizeof(int) * xcol.size())) ||
+        (rc = dev_alloc(reinterpret_cast<void **>(&W.pos), sizeof(int) * xpos.size())) ||
+        (rc = dev_alloc(&W.val, A->elem() * xcol.size())) || (rc = dev_alloc(&W.dval, A->elem() * xdpos.size())))
+        return rc;
+    HFH_HIP(hipMemcpy(W.ptr, xptr.data(), sizeof(int) * xptr.size(), hipMemcpyHostToDevice));
+    HFH_HIP(hipMemcpy(W.row, xrow.data(), sizeof(int) * xrow.size(), hipMemcpyHostToDevice));
+    HFH_HIP(hipMemcpy(W.len, xlen.data(), sizeof(int) * xlen.size(), hipMemcpyHostToDevice));
+    HFH_HIP(hipMemcpy(W.lev, xlev.data(), sizeof(int) * xlev.size(), hipMemcpyHostToDevice));
+    HFH_HIP(hipMemcpy(W.bnl, xnl.data(), sizeof(int) * xnl.size(), hipMemcpyHostToDevice));
+    HFH_HIP(hipMemcpy(W.dpos, xdpos.data(), sizeof(int) * xdpos.size(), hipMemcpyHostToDevice));
+    HFH_HIP(hipMemcpy(W.col, xcol.data(), sizeof(int) * xcol.size(), hipMemcpyHostToDevice));
+    HFH_HIP(hipMemcpy(W.pos, xpos.data(), sizeof(int) * xpos.size(), hipMemcpyHostToDevice));
+    return tsweep_fill(A, W);
+}
+
 static void free_tsweep(hfh_tsweep &W) {
-    void *bufs[] = {W.lslice_d, W.ptr, W.row, W.len, W.col, W.pos, W.dpos, W.val, W.dval};
+    void *bufs[] = {W.lslice_d, W.ptr, W.row, W.len, W.col, W.pos, W.dpos, W.val, W.dval, W.lev, W.bnl};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     W = hfh_tsweep();
@@ -1109,7 +1430,7 @@ template <typename T>
 static int trsv_graph(hfh_prec *P, const T *in, T *out) {
     hipStream_t s = P->ctx->stream;
     // three launches already: no graph
-    if (P->sweep_mode == HFH_SWEEP_SYNCFREE && P->sf_y && in != out) return trsv_sweeps<T>(P, in, out, nullptr);
+    if (P->sweep_mode != HFH_SWEEP_LEVELS && P->sf_y && in != out) return trsv_sweeps<T>(P, in, out, nullptr);
     const size_t bytes = sizeof(T) * static_cast<size_t>(P->n);
     if (!P->sweep_exec && !P->graph_failed && graphs_enabled()) {
         int rc = HFH_OK;
@@ -1145,6 +1466,8 @@ int prec_sync(hfh_prec *P) {
     if (hipStreamIsCapturing(P->ctx->stream, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone) return HFH_OK;
     int rc = tsweep_fill(P->A, P->tlo);
     if (!rc) rc = tsweep_fill(P->A, P->tup);
+    if (!rc && P->blo.nslice > 0) rc = tsweep_fill(P->A, P->blo);
+    if (!rc && P->bup.nslice > 0) rc = tsweep_fill(P->A, P->bup);
     if (!rc) P->tversion = P->A->version;
     return rc;
 }
@@ -1278,11 +1601,27 @@ static int create_trsv(const hfh_csr *A, int kind, int nlev, const int *lp, cons
     if (!rc) rc = validate_levels(A, P->up, true, what);
     if (!rc) rc = build_tsweep(A, P->lo, lr, true, P->tlo);
     if (!rc) rc = build_tsweep(A, P->up, ur, false, P->tup);
+    int gnx = 0;
+    long long gnxy = 0;
+    if (!rc && detect_grid(A, gnx, gnxy)) {
+        for (int shear = 0; shear < 2 && !rc && P->blo.nslice == 0; ++shear)
+            rc = build_bsweep(A, gnx, gnxy, true, shear, P->blo);
+        for (int shear = 0; shear < 2 && !rc && P->bup.nslice == 0; ++shear)
+            rc = build_bsweep(A, gnx, gnxy, false, shear, P->bup);
+        // a block level costs about two row levels (one round trip, then
+        // the block's own levels): keep the blocks where they halve the
+        // chain at least
+        auto keep = [](const hfh_tsweep &B, int nlev) {
+            return B.nslice > 0 && 2 * (static_cast<long long>(B.lslice.size()) - 1) <= nlev;
+        };
+        if (!keep(P->blo, P->lo.nlev)) free_tsweep(P->blo);
+        if (!keep(P->bup, P->up.nlev)) free_tsweep(P->bup);
+    }
     // sync-free sweeps: forward results and the ticket / failure words
     if (!rc) rc = dev_alloc(&P->sf_y, A->elem() * static_cast<size_t>(std::max(A->nrow, 1)));
-    if (!rc) rc = dev_alloc(reinterpret_cast<void **>(&P->sf_ctl), sizeof(int) * 4);
+    if (!rc) rc = dev_alloc(reinterpret_cast<void **>(&P->sf_ctl), sizeof(int) * kSfCtlInts);
     if (!rc) {
-        const hipError_t e = hipMemsetAsync(P->sf_ctl, 0, sizeof(int) * 4, A->ctx->stream);
+        const hipError_t e = hipMemsetAsync(P->sf_ctl, 0, sizeof(int) * kSfCtlInts, A->ctx->stream);
         if (e != hipSuccess) rc = hip_fail(e, what);
     }
     if (!rc) {
@@ -1313,10 +1652,22 @@ extern "C" int hfh_prec_apply(hfh_prec *P, const hfh_vec *in, hfh_vec *out) {
     return prec_apply_raw(P, in->d, out->d);
 }
 
+extern "C" int hfh_prec_sweep_blocks(const hfh_prec *P, int *fw_blocks, int *fw_levels, int *bw_blocks,
+                                     int *bw_levels) {
+    HFH_CHECK(P && fw_blocks && fw_levels && bw_blocks && bw_levels, HFH_ERR_INVALID,
+              "hfh_prec_sweep_blocks: null argument");
+    auto lv = [](const hfh_tsweep &B) { return B.nslice > 0 ? static_cast<int>(B.lslice.size()) - 1 : 0; };
+    *fw_blocks = P->blo.nslice;
+    *fw_levels = lv(P->blo);
+    *bw_blocks = P->bup.nslice;
+    *bw_levels = lv(P->bup);
+    return HFH_OK;
+}
+
 extern "C" int hfh_prec_set_sweep(hfh_prec *P, int mode, int waves_per_cu) {
     HFH_CHECK(P, HFH_ERR_INVALID, "hfh_prec_set_sweep: null preconditioner");
-    HFH_CHECK(mode == HFH_SWEEP_SYNCFREE || mode == HFH_SWEEP_LEVELS, HFH_ERR_INVALID,
-              "hfh_prec_set_sweep: mode %d is neither HFH_SWEEP_SYNCFREE nor HFH_SWEEP_LEVELS", mode);
+    HFH_CHECK(mode == HFH_SWEEP_SYNCFREE || mode == HFH_SWEEP_LEVELS || mode == HFH_SWEEP_SLICES, HFH_ERR_INVALID,
+              "hfh_prec_set_sweep: mode %d is not HFH_SWEEP_SYNCFREE, HFH_SWEEP_LEVELS or HFH_SWEEP_SLICES", mode);
     HFH_CHECK(waves_per_cu >= 0 && waves_per_cu <= 32, HFH_ERR_INVALID,
               "hfh_prec_set_sweep: waves_per_cu %d not 0 (default) or in [1, 32]", waves_per_cu);
     P->sweep_mode = mode;
@@ -1366,6 +1717,8 @@ extern "C" int hfh_prec_destroy(hfh_prec *P) {
     free_sweep(P->bw);
     free_tsweep(P->tlo);
     free_tsweep(P->tup);
+    free_tsweep(P->blo);
+    free_tsweep(P->bup);
     if (P->sweep_exec) (void)hipGraphExecDestroy(P->sweep_exec);
     if (P->sweep_graph) (void)hipGraphDestroy(P->sweep_graph);
     if (P->gin) (void)hipFree(P->gin);

@@ -21,7 +21,7 @@ OK, ERR_INVALID, ERR_NOMEM, ERR_HIP, ERR_ZERO_PIVOT, ERR_NOT_CONVERGED, ERR_BUDG
     0, -1, -2, -3, -4, -5, -6, -7
 
 Q1_2D, Q1_3D, FV_DIFFUSION, UPWIND_CONVDIFF = 1, 2, 3, 4
-SWEEP_SYNCFREE, SWEEP_LEVELS = 0, 1
+SWEEP_SYNCFREE, SWEEP_LEVELS, SWEEP_SLICES = 0, 1, 2
 SPMV_AUTO, SPMV_STREAM, SPMV_VECTOR, SPMV_WAVE, SPMV_SELL, SPMV_SELL_DICT, SPMV_SELL_PAIRS, SPMV_SELL_LDS = \
     0, 1, 2, 3, 4, 5, 6, 7
 
@@ -134,6 +134,7 @@ HIP_API = {
     "hfh_prec_kind": (_i, [_vp, _P(_i)]),
     "hfh_prec_set_sweep": (_i, [_vp, _i, _i]),
     "hfh_prec_sweep_status": (_i, [_vp]),
+    "hfh_prec_sweep_blocks": (_i, [_vp, _P(C.c_int), _P(C.c_int), _P(C.c_int), _P(C.c_int)]),
     "hfh_prec_destroy": (_i, [_vp]),
     "hfh_pcg_solve": (_i, [_vp, _vp, _vp, _vp, _P(Control), _P(SolveInfo), _vp]),
     "hfh_pcg_solve_noalloc": (_i, [_vp, _vp, _vp, _vp, _P(Control), _P(SolveInfo), _vp]),
@@ -635,9 +636,18 @@ class Preconditioner:
         _ck(hip().hfh_prec_apply(self.h, x.h, y.h))
 
     def set_sweep(self, mode, waves_per_cu=0):
-        """SGS / ILU(0): SWEEP_SYNCFREE (default) or SWEEP_LEVELS schedule;
-        waves_per_cu of a sync-free sweep (0: default)."""
+        """SGS / ILU(0): SWEEP_SYNCFREE (default: geometric blocks on a
+        structured grid, else level slices), SWEEP_SLICES or SWEEP_LEVELS
+        schedule; waves_per_cu of a sync-free sweep (0: default)."""
         _ck(hip().hfh_prec_set_sweep(self.h, mode, waves_per_cu))
+
+    def sweep_blocks(self):
+        """(forward blocks, forward block levels, backward blocks, backward
+        block levels) of the blocked sync-free sweeps; zeros when the
+        pattern is not a structured grid's."""
+        v = [C.c_int() for _ in range(4)]
+        _ck(hip().hfh_prec_sweep_blocks(self.h, *[C.byref(x) for x in v]))
+        return tuple(x.value for x in v)
 
     def sweep_status(self):
         """Raises if a sync-free sweep gave up waiting since the last check."""

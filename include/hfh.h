@@ -293,18 +293,26 @@ int hfh_prec_destroy(hfh_prec *P);
 
 /* Schedule of the SGS / ILU(0) triangular sweeps (no reference counterpart;
  * the reference sweeps row by row on the CPU).  HFH_SWEEP_SYNCFREE (the
- * default): one launch per sweep; waves take slices of rows in level order
- * from a counter and each row waits for the earlier results it needs
- * (bounded waits).  HFH_SWEEP_LEVELS: one launch per level (runs of small
- * levels merged).  Both give bit-identical results.  waves_per_cu: waves of
- * a sync-free sweep per compute unit (1..32); 0 (the default) sizes it to
- * about three levels' slices, between 1 and 8 per compute unit. */
-enum { HFH_SWEEP_SYNCFREE = 0, HFH_SWEEP_LEVELS = 1 };
+ * default): one launch per sweep; waves take work in dependency order from
+ * a counter and each row waits for the earlier results it needs (bounded
+ * waits).  Where the pattern is that of a natural-order structured grid
+ * (found at create time), a wave's work is a geometric block of up to 64
+ * rows whose own dependencies resolve between lanes, else a slice of 64 rows
+ * of one level.  HFH_SWEEP_SLICES: sync-free by level slices only.
+ * HFH_SWEEP_LEVELS: one launch per level (runs of small levels merged).  All
+ * give bit-identical results.  waves_per_cu: waves of a sync-free sweep per
+ * compute unit (1..32); 0 (the default) sizes it to about three levels'
+ * work, between 1 and 8 per compute unit. */
+enum { HFH_SWEEP_SYNCFREE = 0, HFH_SWEEP_LEVELS = 1, HFH_SWEEP_SLICES = 2 };
 int hfh_prec_set_sweep(hfh_prec *P, int mode, int waves_per_cu);
 /* HFH_ERR_HIP when a sync-free sweep of P gave up waiting since the last
  * call (its results are then NaN); clears the condition.  Synchronises the
  * context's stream.  The drivers check it at the end of each solve. */
 int hfh_prec_sweep_status(hfh_prec *P);
+/* Shape of the blocked sync-free sweeps (no reference counterpart): blocks
+ * and block levels of the forward and backward sweeps; 0 blocks where the
+ * pattern was not recognised as a structured grid (slices then run). */
+int hfh_prec_sweep_blocks(const hfh_prec *P, int *fw_blocks, int *fw_levels, int *bw_blocks, int *bw_levels);
 
 /* ---- device-resident Krylov drivers ---- */
 typedef struct {

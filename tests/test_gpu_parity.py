@@ -543,14 +543,24 @@ def _ragged_lower_heavy(n, seed):
 
 @pytest.mark.parametrize("prec", ["ilu0", "sgs"])
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
-@pytest.mark.parametrize("case", ["upwind40", "ragged"])
+@pytest.mark.parametrize("case", ["upwind40", "upwind37", "q1_2d", "q1_3d", "ragged"])
 def test_sweep_schedules_bitwise(pkg, ctx, oracle, prec, dtype, case):
-    """The sync-free sweeps (the default; waves take slices in level order and
-    poll the results they need) give the level launches' results bit for bit,
-    for any number of polling waves, and match the oracle's sequential loops
-    bit for bit."""
+    """The sync-free sweeps (the default: geometric blocks whose own
+    dependencies resolve between lanes on a structured grid, else slices in
+    level order; waves poll the results they need from other blocks or
+    slices) give the level launches' results bit for bit, for any number of
+    polling waves, and match the oracle's sequential loops bit for bit.
+    7-point (40^3, and 37^3 whose blocks are cut at the far faces), 9-point
+    2D and 27-point 3D grids take the blocked form; the ragged pattern has no
+    grid and takes the slices."""
     if case == "upwind40":
         row, col, val, rhs = pkg.problem(pkg.UPWIND_CONVDIFF, 40, p0=1e-2, p1=5.0, seed=6)
+    elif case == "upwind37":
+        row, col, val, rhs = pkg.problem(pkg.UPWIND_CONVDIFF, 37, p0=1e-2, p1=5.0, seed=7)
+    elif case == "q1_2d":
+        row, col, val, rhs = pkg.problem(pkg.Q1_2D, 120, seed=8)
+    elif case == "q1_3d":
+        row, col, val, rhs = pkg.problem(pkg.Q1_3D, 24, seed=9)
     else:
         row, col, val = _ragged_lower_heavy(6000, 11)
         rhs = np.random.default_rng(12).uniform(-1, 1, len(row) - 1)
@@ -563,10 +573,19 @@ def test_sweep_schedules_bitwise(pkg, ctx, oracle, prec, dtype, case):
         A = pkg.CsrMatrix(ctx, row, col, val.astype(dtype))
         P = pkg.Preconditioner.sgs(A, row, col)
         ref = oracle.sgs(oracle.Csr(row, col, val.astype(dtype)), rhs.astype(dtype))
+    fb, fl, bb, bl = P.sweep_blocks()
+    lo, up = pkg.levels(row, col)[0], pkg.levels(row, col, upper=True)[0]
+    print(f"sweep case {case}: n {len(rhs)} row levels {len(lo) - 1}/{len(up) - 1} "
+          f"blocks {fb}/{bb} block levels {fl}/{bl}", flush=True)
+    if case == "ragged":
+        assert (fb, bb) == (0, 0)
+    else:
+        assert fb > 0 and bb > 0 and 0 < 2 * fl <= len(lo) - 1 and 0 < 2 * bl <= len(up) - 1
     Rv = pkg.Vector(ctx, rhs.astype(dtype), dtype)
     results = {}
-    for mode, waves in ((pkg.SWEEP_LEVELS, 0), (pkg.SWEEP_SYNCFREE, 0), (pkg.SWEEP_SYNCFREE, 4),
-                        (pkg.SWEEP_SYNCFREE, 32)):
+    for mode, waves in ((pkg.SWEEP_LEVELS, 0), (pkg.SWEEP_SYNCFREE, 0), (pkg.SWEEP_SYNCFREE, 1),
+                        (pkg.SWEEP_SYNCFREE, 4), (pkg.SWEEP_SYNCFREE, 32), (pkg.SWEEP_SLICES, 0),
+                        (pkg.SWEEP_SLICES, 4)):
         P.set_sweep(mode, waves)
         Z = pkg.Vector(ctx, len(rhs), dtype)
         for _ in range(2):   # a second apply reuses the counters and buffers
@@ -585,14 +604,15 @@ def test_gmres_ilu0_sweep_schedules_same_iterates(pkg, ctx):
     A = pkg.CsrMatrix(ctx, row, col, val)
     P = pkg.Preconditioner.ilu0(pkg.CsrMatrix(ctx, row, col, lu), row, col)
     out = []
-    for mode in (pkg.SWEEP_LEVELS, pkg.SWEEP_SYNCFREE):
+    for mode in (pkg.SWEEP_LEVELS, pkg.SWEEP_SYNCFREE, pkg.SWEEP_SLICES):
         P.set_sweep(mode)
         X = pkg.Vector(ctx, len(rhs))
         info = pkg.gmres(A, pkg.Vector(ctx, rhs), X, P, m=30, maxits=1000, atol=0.0, rtol=1e-8, history=True)
         out.append((info, X.numpy()))
-    assert out[0][0]["iters"] == out[1][0]["iters"]
-    assert np.array_equal(out[0][0]["hist"], out[1][0]["hist"])
-    assert np.array_equal(out[0][1], out[1][1])
+    for o in out[1:]:
+        assert out[0][0]["iters"] == o[0]["iters"]
+        assert np.array_equal(out[0][0]["hist"], o[0]["hist"])
+        assert np.array_equal(out[0][1], o[1])
 
 
 # ---------------------------------------------------------------- Krylov drivers
