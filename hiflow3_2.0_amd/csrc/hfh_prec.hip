@@ -592,14 +592,20 @@ __global__ __launch_bounds__(kBlock) void k_bsweep_sf(int nblk, const int *__res
         for (int l = 0; l < nl; ++l) {
             __builtin_amdgcn_wave_barrier();
             if (l == my) {
+                // straight-line: every lane reads a slot for every entry (its
+                // own for an entry from another block or a padding one), all
+                // reads issued before the first use, then selects.  Padding
+                // entries (u >= len) hold value 0 and operand +0, and
+                // acc + -(0 * 0) = acc exactly, so they need no guard.
                 T x[K];
 #pragma unroll
-                for (int u = 0; u < K; ++u)   // all reads issued before the first use
-                    if (u < L) x[u] = c[u] < 0 ? wx[-1 - c[u]] : __builtin_bit_cast(T, w[u]);
-                T acc = acc0;
+                for (int u = 0; u < K; ++u) x[u] = wx[c[u] < 0 ? -1 - c[u] : lane];
 #pragma unroll
                 for (int u = 0; u < K; ++u)
-                    if (u < len) acc = add_rn(acc, -mul_rn(x[u], v[u]));
+                    if (c[u] >= 0) x[u] = __builtin_bit_cast(T, w[u]);
+                T acc = acc0;
+#pragma unroll
+                for (int u = 0; u < K; ++u) acc = add_rn(acc, -mul_rn(x[u], v[u]));
                 if (MODE != 2) acc = acc / d;
                 wx[lane] = acc;
                 sf_store(out + i, acc);
@@ -804,7 +810,8 @@ static void bsweep_launch(const hfh_tsweep &W, int grid, const T *in, const T *y
 #define HFH_BSWEEP(K)                                                                                             \
     hipLaunchKernelGGL((k_bsweep_sf<T, MODE, K>), dim3(grid), dim3(kBlock), 0, s, W.nslice, W.ptr, W.row, W.len, \
                        W.lev, W.bnl, W.col, (const T *)W.val, (const T *)W.dval, in, yin, out, ctl, done)
-    if (W.kmax <= 4) HFH_BSWEEP(4);
+    if (W.kmax <= 3) HFH_BSWEEP(3);   // 7-point stencils
+    else if (W.kmax <= 4) HFH_BSWEEP(4);
     else if (W.kmax <= 8) HFH_BSWEEP(8);
     else HFH_BSWEEP(16);
 #undef HFH_BSWEEP
