@@ -437,13 +437,20 @@ def test_mcsgs_apply(pkg, ctx, oracle, omega, kind):
     assert relnorm(Z.numpy(), ref) < 1e-13
 
 
+@pytest.mark.parametrize("case", ["fv12", "upwind20"])
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
-def test_sgs_natural_apply(pkg, ctx, oracle, dtype):
+def test_sgs_natural_apply(pkg, ctx, oracle, dtype, case):
     """Level-scheduled SGS: every product and difference rounded separately in
-    stored order, as the CPU loop: bit-identical."""
-    row, col, val, rhs = fv3(pkg, 12)
+    stored order, as the CPU loop: bit-identical; also after new values, by
+    level slices (12^3, below the blocking size) and by geometric blocks
+    (20^3)."""
+    if case == "fv12":
+        row, col, val, rhs = fv3(pkg, 12)
+    else:
+        row, col, val, rhs = pkg.problem(pkg.UPWIND_CONVDIFF, 20, p0=1e-2, p1=5.0, seed=8)
     A = pkg.CsrMatrix(ctx, row, col, val.astype(dtype))
     P = pkg.Preconditioner.sgs(A, row, col)
+    assert (P.sweep_blocks()[0] > 0) == (case == "upwind20")
     Z = pkg.Vector(ctx, len(rhs), dtype)
     P.apply(pkg.Vector(ctx, rhs.astype(dtype), dtype), Z)
     assert np.array_equal(Z.numpy(), oracle.sgs(oracle.Csr(row, col, val.astype(dtype)), rhs.astype(dtype)))
@@ -464,6 +471,7 @@ def test_ilu0_apply(pkg, ctx, oracle, case):
     lu = pkg.ilu0_factor(row, col, val)
     LU = pkg.CsrMatrix(ctx, row, col, lu)
     P = pkg.Preconditioner.ilu0(LU, row, col)
+    assert (P.sweep_blocks()[0] > 0) == (case == "upwind")   # 16^3: geometric blocks; 14^3: slices
     Z = pkg.Vector(ctx, len(rhs))
     P.apply(pkg.Vector(ctx, rhs), Z)
     assert np.array_equal(Z.numpy(), oracle.ilu_solve(oracle.Csr(row, col, lu), rhs))
