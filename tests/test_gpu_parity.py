@@ -549,9 +549,34 @@ def _ragged_lower_heavy(n, seed):
     return row, col, val
 
 
+def _grid_stencil(nx, ny, nz, lower, seed):
+    """A natural-order nx x ny x nz grid matrix whose rows couple to the
+    neighbours at the offsets `lower` (dx, dy, dz) and their mirrors,
+    nonsymmetric values, strictly diagonally dominant."""
+    rng = np.random.default_rng(seed)
+    offs = sorted({o for d in lower for o in (d, tuple(-v for v in d))} | {(0, 0, 0)},
+                  key=lambda d: (d[2], d[1], d[0]))
+    n = nx * ny * nz
+    idx = np.arange(n)
+    x, y, z = idx % nx, (idx // nx) % ny, idx // (nx * ny)
+    rows, cols = [], []
+    for dx, dy, dz in offs:   # ascending column order within each row
+        ok = (x + dx >= 0) & (x + dx < nx) & (y + dy >= 0) & (y + dy < ny) & (z + dz >= 0) & (z + dz < nz)
+        rows.append(idx[ok])
+        cols.append(idx[ok] + dx + nx * dy + nx * ny * dz)
+    r, c = np.concatenate(rows), np.concatenate(cols)
+    order = np.lexsort((c, r))
+    r, c = r[order], c[order]
+    val = rng.uniform(-1, 1, len(c))
+    val[r == c] = 2.0 * len(offs) + rng.random(n)
+    row = np.zeros(n + 1, np.int32)
+    np.add.at(row, r + 1, 1)
+    return np.cumsum(row).astype(np.int32), c.astype(np.int32), val
+
+
 @pytest.mark.parametrize("prec", ["ilu0", "sgs"])
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
-@pytest.mark.parametrize("case", ["upwind40", "upwind37", "q1_2d", "q1_3d", "ragged"])
+@pytest.mark.parametrize("case", ["upwind40", "upwind37", "q1_2d", "q1_3d", "edge11", "ragged"])
 def test_sweep_schedules_bitwise(pkg, ctx, oracle, prec, dtype, case):
     """The sync-free sweeps (the default: geometric blocks whose own
     dependencies resolve between lanes on a structured grid, else slices in
@@ -560,7 +585,9 @@ def test_sweep_schedules_bitwise(pkg, ctx, oracle, prec, dtype, case):
     polling waves, and match the oracle's sequential loops bit for bit.
     7-point (40^3, and 37^3 whose blocks are cut at the far faces), 9-point
     2D and 27-point 3D grids take the blocked form; the ragged pattern has no
-    grid and takes the slices."""
+    grid and takes the slices.  The 11-point 3D stencil (7-point plus the
+    neighbours at (+-1, -1, 0) and (+-1, 0, -1)) has 5 lower entries per row
+    (the K = 8 kernel) and needs the sheared blocks."""
     if case == "upwind40":
         row, col, val, rhs = pkg.problem(pkg.UPWIND_CONVDIFF, 40, p0=1e-2, p1=5.0, seed=6)
     elif case == "upwind37":
@@ -569,6 +596,10 @@ def test_sweep_schedules_bitwise(pkg, ctx, oracle, prec, dtype, case):
         row, col, val, rhs = pkg.problem(pkg.Q1_2D, 120, seed=8)
     elif case == "q1_3d":
         row, col, val, rhs = pkg.problem(pkg.Q1_3D, 24, seed=9)
+    elif case == "edge11":
+        row, col, val = _grid_stencil(24, 20, 18, [(1, 0, 0), (0, 1, 0), (0, 0, 1), (1, 1, 0), (-1, 1, 0),
+                                                   (1, 0, 1), (-1, 0, 1)], 13)
+        rhs = np.random.default_rng(14).uniform(-1, 1, len(row) - 1)
     else:
         row, col, val = _ragged_lower_heavy(6000, 11)
         rhs = np.random.default_rng(12).uniform(-1, 1, len(row) - 1)
