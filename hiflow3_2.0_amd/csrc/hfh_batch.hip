@@ -65,7 +65,8 @@ constexpr int kTileRows = kTileSlices * 64;
 // solves/s from 2,048 to 8,192; the workgroups past the last tile end at
 // once).  Tiles of 1,024 rows give a lightly loaded step twice the
 // workgroups of 2,048-row tiles and half the slices per wave (0.59 -> 0.49
-// ms per 16-step replay at 116 arrivals/s), at the same capacity.
+// ms per 16-step replay at 116 arrivals/s), at the same capacity; 512-row
+// tiles were slower again (0.52 ms, 0.66 ms on 32,768 workgroups).
 constexpr int kTileGrid = 16384;
 constexpr int kSellU = 8;           // column slots per SELL step
 constexpr size_t kAlign = 256;
