@@ -803,6 +803,38 @@ def test_gmres_stops_at_maxits_mid_cycle(pkg, ctx, oracle, maxits):
     assert relnorm(X.numpy(), xo) < 1e-10
 
 
+@pytest.mark.parametrize("where", ["two_per_thread", "at_limit", "above_limit"])
+def test_gmres_mgs_cooperative_and_multi_launch(pkg, ctx, oracle, where):
+    """The MGS sweep of a column runs as one cooperative launch (w kept in
+    registers: 2 elements per thread at 2 x 1024 x CUs rows, 8 at the limit
+    of 8 x 1024 x CUs) or, above the limit (and at most 1024 x CUs rows), as
+    one launch per step: each follows the oracle's GMRES for 10 iterations on
+    a nonsymmetric tridiagonal system of that size."""
+    per = 1024 * ctx.info()["num_cu"]
+    n = {"two_per_thread": 2 * per, "at_limit": 8 * per, "above_limit": 8 * per + 1000}[where]
+    rng = np.random.default_rng(21)
+    lo, up = -1.0 - 0.3 * rng.random(n), -1.0 + 0.3 * rng.random(n)
+    r = np.repeat(np.arange(n), 3).reshape(n, 3)
+    c = r + np.array([-1, 0, 1])
+    v = np.stack([lo, 4.0 + rng.random(n), up], axis=1)
+    keep = (c >= 0) & (c < n)
+    row = np.zeros(n + 1, np.int32)
+    row[1:] = np.cumsum(keep.sum(axis=1))
+    col, val = c[keep].astype(np.int32), v[keep]
+    rhs = rng.uniform(-1, 1, n)
+    A = pkg.CsrMatrix(ctx, row, col, val)
+    P = pkg.Preconditioner.jacobi(A)
+    print(f"gmres mgs case {where}: n {n}", flush=True)
+    X = pkg.Vector(ctx, n)
+    info = pkg.gmres(A, pkg.Vector(ctx, rhs), X, P, m=8, maxits=10, atol=0.0, rtol=1e-30, history=True,
+                     check=False)
+    OA = oracle.Csr(row, col, val)
+    xo, oi = oracle.gmres(OA, rhs, P=oracle.Prec.jacobi(OA), m=8, ctrl=(10, 0.0, 1e-30, 1e6))
+    assert info["iters"] == oi["iters"] == 10
+    assert np.allclose(info["hist"], oi["hist"], rtol=1e-9)
+    assert relnorm(X.numpy(), xo) < 1e-10
+
+
 def test_defect_correction_matches_fp64(pkg, ctx, oracle):
     row, col, val, rhs = fv3(pkg, 20, sigma=1.0, seed=5)
     nc, color, sizes, perm = pkg.multicoloring(row, col)
