@@ -357,7 +357,9 @@ int hfh_pcg_solve_noalloc(const hfh_csr *A, hfh_prec *P, const hfh_vec *b, hfh_v
 /* GMRES<LAD>::SolveRight (gmres.cc:285-510) / FGMRES<LAD>::SolveRight
  * (fgmres.cc:75-262) with right preconditioning, modified Gram-Schmidt and
  * Givens rotations; the Hessenberg scalars live on the device.  flexible
- * selects FGMRES.  P may be NULL. */
+ * selects FGMRES.  P may be NULL.  For n above 1024 x (compute units) and up
+ * to 8 times that, the MGS sweep of each Arnoldi column runs as one
+ * cooperative launch (HFH_ERR_HIP if one of its bounded waits gave up). */
 int hfh_gmres_solve(const hfh_csr *A, hfh_prec *P, int size_basis, int flexible,
                     const hfh_vec *b, hfh_vec *x, const hfh_control *ctl,
                     hfh_solve_info *info, double *hist);
