@@ -25,6 +25,10 @@ void set_error(const char *fmt, ...);
 constexpr int kBlock = 256;          // threads per workgroup (4 wave64)
 constexpr int kSpmvTile = 1024;      // nnz staged per workgroup by the streaming SpMV
 constexpr int kRedBlocks = 1024;     // max partials per reduction
+// Dictionary-coded SELL copies (SpMV, MC-SGS sweeps): the code of a padding
+// slot (a slot past its row's length), so kernels need no row lengths;
+// dictionaries therefore hold at most 255 offsets
+constexpr unsigned kPadCode = 255;
 constexpr int kNumXcd = 8;           // MI355X: 8 accelerator complex dies
 // Size of the per-device static array that the allocation-free CG
 // (hfh_pcg_solve_noalloc) works in: FP64 systems up to 2,096,000 rows.

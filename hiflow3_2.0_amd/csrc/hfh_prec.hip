@@ -113,25 +113,34 @@ constexpr int kSweepU = 8;   // column slots in flight (two code words when code
 // Columns of the kSweepU slots k.. of row i: stored (scol), or row + the
 // offset coded in the slot's byte of its code word (codes: kSweepU / 4 words
 // per step; the column clamped to [0, n): padding slots and rows past the
-// colour read a valid entry, and their products are not summed).  Slots past
-// the slice's width L are not loaded (a wave-uniform test: L is the slice's),
-// so a slice of 3 slots issues 3 loads, not kSweepU.
+// colour read a valid entry, and their products are not summed).  live[u]:
+// slot k + u is one of the row's entries (stored: k + u < len; coded: its
+// code is not kPadCode, so the row's length is not loaded).  Slots past the
+// slice's width L are not loaded (a wave-uniform test: L is the slice's), so
+// a slice of 3 slots issues 3 loads, not kSweepU.
 template <bool CODED>
-__device__ __forceinline__ void sweep_cols(int (&cj)[kSweepU], int k, int L, int base, int wbase, int lane, int i, int n,
-                                           const int *__restrict__ scol, const unsigned *__restrict__ code,
-                                           const int *dict) {
+__device__ __forceinline__ void sweep_cols(int (&cj)[kSweepU], bool (&live)[kSweepU], int k, int L, int len, int base,
+                                           int wbase, int lane, int i, int n, const int *__restrict__ scol,
+                                           const unsigned *__restrict__ code, const int *dict) {
     if (!CODED) {
 #pragma unroll
-        for (int u = 0; u < kSweepU; ++u)
+        for (int u = 0; u < kSweepU; ++u) {
             cj[u] = k + u < L ? __builtin_nontemporal_load(&scol[base + (k + u) * 64 + lane]) : min(i, n - 1);
+            live[u] = k + u < len;
+        }
         return;
     }
     const int W = (L + 3) >> 2;   // code words of the slice (its width rounded up to 4 slots)
 #pragma unroll
     for (int q = 0; q < kSweepU / 4; ++q) {
-        const unsigned cw = (k >> 2) + q < W ? __builtin_nontemporal_load(&code[wbase + ((k >> 2) + q) * 64 + lane]) : 0u;
+        const unsigned cw =
+            (k >> 2) + q < W ? __builtin_nontemporal_load(&code[wbase + ((k >> 2) + q) * 64 + lane]) : ~0u;
 #pragma unroll
-        for (int t = 0; t < 4; ++t) cj[4 * q + t] = min(i + dict[(cw >> (8 * t)) & 255u], n - 1);
+        for (int t = 0; t < 4; ++t) {
+            const unsigned cd = (cw >> (8 * t)) & 255u;
+            cj[4 * q + t] = min(i + dict[cd], n - 1);
+            live[4 * q + t] = cd != kPadCode;
+        }
     }
 }
 
@@ -147,8 +156,8 @@ __global__ __launch_bounds__(kBlock) void k_mcs_fwd(int c, ColorOffsets co, bool
                                                     const int *__restrict__ done, double *__restrict__ rzp) {
     if (done && *done) return;
     __shared__ int dict[CODED ? 256 : 1];
-    if (CODED) {
-        for (int t = threadIdx.x; t < ndict; t += kBlock) dict[t] = gdict[t];
+    if (CODED) {   // kPadCode: offset 0
+        for (int t = threadIdx.x; t < 256; t += kBlock) dict[t] = t < ndict ? gdict[t] : 0;
         __syncthreads();
     }
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
@@ -159,15 +168,16 @@ __global__ __launch_bounds__(kBlock) void k_mcs_fwd(int c, ColorOffsets co, bool
         const bool valid = i < co.off[c + 1];
         const int base = sptr[s], L = (sptr[s + 1] - base) >> 6;
         const int wbase = CODED ? wptr[s] : 0;
-        const int len = valid ? slen[i] : 0;
+        const int len = !CODED && valid ? slen[i] : 0;   // coded: padding slots carry kPadCode
         const T ri = valid ? in[i] : static_cast<T>(0);
         T chunk = ri * static_cast<T>(-1);
         int blk = 0, blk_end = co.off[1];
         T res = 0;
         for (int k = 0; k < L; k += kSweepU) {
             int cj[kSweepU];
+            bool live[kSweepU];
             T v[kSweepU], xv[kSweepU];
-            sweep_cols<CODED>(cj, k, L, base, wbase, lane, i, n, scol, code, dict);
+            sweep_cols<CODED>(cj, live, k, L, len, base, wbase, lane, i, n, scol, code, dict);
 #pragma unroll
             for (int u = 0; u < kSweepU; ++u) {   // slots past the slice width: not loaded (wave-uniform)
                 v[u] = k + u < L ? __builtin_nontemporal_load(&sval[base + (k + u) * 64 + lane]) : static_cast<T>(0);
@@ -175,7 +185,7 @@ __global__ __launch_bounds__(kBlock) void k_mcs_fwd(int c, ColorOffsets co, bool
             }
 #pragma unroll
             for (int u = 0; u < kSweepU; ++u) {
-                if (k + u >= len) continue;
+                if (!live[u]) continue;
                 while (cj[u] >= blk_end) {   // entries are column sorted: flush finished blocks
                     chunk += res;
                     res = 0;
@@ -213,8 +223,8 @@ __global__ __launch_bounds__(kBlock) void k_mcs_bwd(int c, int ncolors, ColorOff
                                                     const T *__restrict__ rin, double *__restrict__ rzp) {
     if (done && *done) return;
     __shared__ int dict[CODED ? 256 : 1];
-    if (CODED) {
-        for (int t = threadIdx.x; t < ndict; t += kBlock) dict[t] = gdict[t];
+    if (CODED) {   // kPadCode: offset 0
+        for (int t = threadIdx.x; t < 256; t += kBlock) dict[t] = t < ndict ? gdict[t] : 0;
         __syncthreads();
     }
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
@@ -225,14 +235,15 @@ __global__ __launch_bounds__(kBlock) void k_mcs_bwd(int c, int ncolors, ColorOff
         const bool valid = i < co.off[c + 1];
         const int base = sptr[s], L = (sptr[s + 1] - base) >> 6;
         const int wbase = CODED ? wptr[s] : 0;
-        const int len = valid ? slen[i] : 0;
+        const int len = !CODED && valid ? slen[i] : 0;   // coded: padding slots carry kPadCode
         T chunk = valid ? out[i] * dv[i] : static_cast<T>(0);
         int blk = ncolors - 1, blk_lo = co.off[ncolors - 1];
         T res = 0;
         for (int k = 0; k < L; k += kSweepU) {
             int cj[kSweepU];
+            bool live[kSweepU];
             T v[kSweepU], xv[kSweepU];
-            sweep_cols<CODED>(cj, k, L, base, wbase, lane, i, n, scol, code, dict);
+            sweep_cols<CODED>(cj, live, k, L, len, base, wbase, lane, i, n, scol, code, dict);
 #pragma unroll
             for (int u = 0; u < kSweepU; ++u) {   // slots past the slice width: not loaded (wave-uniform)
                 v[u] = k + u < L ? __builtin_nontemporal_load(&sval[base + (k + u) * 64 + lane]) : static_cast<T>(0);
@@ -240,7 +251,7 @@ __global__ __launch_bounds__(kBlock) void k_mcs_bwd(int c, int ncolors, ColorOff
             }
 #pragma unroll
             for (int u = 0; u < kSweepU; ++u) {
-                if (k + u >= len) continue;
+                if (!live[u]) continue;
                 while (cj[u] < blk_lo) {   // next lower colour block
                     chunk += res;
                     res = 0;
@@ -1035,7 +1046,7 @@ static int build_sweep(hfh_prec *P, const hfh_csr *A, bool forward, hfh_sweep &W
                     auto it = std::lower_bound(offs.begin(), offs.end(), o);
                     if (it == offs.end() || *it != o) {
                         offs.insert(it, o);
-                        coded = offs.size() <= 256;
+                        coded = offs.size() <= kPadCode;   // code kPadCode marks padding slots
                     }
                 }
             }
@@ -1056,7 +1067,6 @@ static int build_sweep(hfh_prec *P, const hfh_csr *A, bool forward, hfh_sweep &W
     wptr[nsl] = static_cast<int>(wtotal);
     std::vector<int> pcol(coded ? 1 : std::max<long long>(total, 1)), pos(std::max<long long>(total, 1));
     std::vector<unsigned> words(coded ? std::max<long long>(wtotal, 1) : 1, 0u);
-    const unsigned zero = static_cast<unsigned>(std::lower_bound(offs.begin(), offs.end(), 0LL) - offs.begin());
     for (int c = 0; c < nc; ++c)
         for (int w = 0; w < W.slice0[c + 1] - W.slice0[c]; ++w) {
             const int sl = W.slice0[c] + w;
@@ -1079,7 +1089,7 @@ static int build_sweep(hfh_prec *P, const hfh_csr *A, bool forward, hfh_sweep &W
                         real ? static_cast<unsigned>(std::lower_bound(offs.begin(), offs.end(),
                                                                       static_cast<long long>(col[e[k]]) - i) -
                                                      offs.begin())
-                             : zero;
+                             : kPadCode;
                     words[wbase + (k / 4) * 64 + lane] |= code << (8 * (k % 4));
                 }
             }

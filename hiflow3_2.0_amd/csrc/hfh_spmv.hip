@@ -470,22 +470,24 @@ __global__ __launch_bounds__(kBlock) void k_spmv_sellx(
 // SELL-64 with dictionary-coded columns: column = row + dict[code].  Each
 // slice keeps its width L of value slots; code word w of a lane holds the
 // codes of slots 4w..4w+3 (byte t = slot 4w+t), ceil(L / 4) words per lane,
-// and the slots of the last word past L are neither stored nor loaded.  Row
-// sums as k_spmv_sell.
+// and the slots of the last word past L are neither stored nor loaded.  A
+// row's slots past its length hold kPadCode, so no row length is loaded
+// (its gather reads x[row], in range, and is not summed).  Row sums as
+// k_spmv_sell.
 // JAC 1 / 2: a Jacobi step (normal / damped) -- entries with the code of
 // offset 0 (code0, the diagonal) are skipped and xn_r = jacobi_update(...)
 // is stored instead of the row sum.
 template <typename T, bool ADD, bool DOT, int UW, int JAC>
 __global__ __launch_bounds__(kBlock) void k_spmv_selld(
     int nrow, int ncol, int nslice, const int *__restrict__ sptr, const int *__restrict__ wptr,
-    const int *__restrict__ rlen, const unsigned *__restrict__ codes, const int *__restrict__ gdict, int ndict,
+    const unsigned *__restrict__ codes, const int *__restrict__ gdict, int ndict,
     const T *__restrict__ sval, const T *__restrict__ x, T *__restrict__ y, const T *__restrict__ w, double *part,
     const int *__restrict__ done, unsigned code0, const T *__restrict__ b, const T *__restrict__ invd, T jw,
     T jomw) {
     if (done && *done) return;
     __shared__ int dict[256];
     __shared__ double shd[kBlock / 64];
-    for (int i = threadIdx.x; i < ndict; i += kBlock) dict[i] = gdict[i];
+    for (int i = threadIdx.x; i < 256; i += kBlock) dict[i] = i < ndict ? gdict[i] : 0;   // kPadCode: offset 0
     __syncthreads();
     constexpr int WPB = kBlock / 64;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
@@ -494,7 +496,6 @@ __global__ __launch_bounds__(kBlock) void k_spmv_selld(
         const int vbase = sptr[s], L = (sptr[s + 1] - vbase) >> 6, W = (L + 3) >> 2;   // slots, code words
         const int cbase = wptr[s];
         const int r = s * 64 + lane;
-        const int len = r < nrow ? rlen[r] : 0;
         T acc = 0;
         for (int k = 0; k < W; k += UW) {
             unsigned cw[UW];
@@ -515,9 +516,12 @@ __global__ __launch_bounds__(kBlock) void k_spmv_selld(
 #pragma unroll
             for (int u = 0; u < UW; ++u)
 #pragma unroll
-                for (int t = 0; t < 4; ++t)
-                    if (4 * (k + u) + t < len && (!JAC || ((cw[u] >> (8 * t)) & 255u) != code0))
+                for (int t = 0; t < 4; ++t) {
+                    // words past W repeat the last one (the clamped load above)
+                    const unsigned cd = (cw[u] >> (8 * t)) & 255u;
+                    if (k + u < W && cd != kPadCode && (!JAC || cd != code0))
                         acc = add_rn(acc, mul_rn(v[u][t], xv[u][t]));
+                }
         }
         if (r < nrow) {
             if (JAC) {
@@ -740,14 +744,14 @@ static bool lds_plan(const hfh_csr *A, const std::vector<int> &ptr, int ch, LdsP
     return true;
 }
 
-// The sorted distinct column offsets col - row of A, offset 0 included (the
-// code of every padding slot).  False when there are more than 256.  Rows
+// The sorted distinct column offsets col - row of A, offset 0 included.
+// False when there are more than 255 (code 255 marks padding slots).  Rows
 // whose offsets repeat the previous row's (most rows of a stencil matrix)
 // are skipped; the row ranges run on host threads.
 static bool dict_offsets(const hfh_csr *A, std::vector<int> &dict) {
     const int n = A->nrow;
     const int *row = A->h_row.data(), *col = A->h_col.data();
-    constexpr int kMax = 256;
+    constexpr int kMax = kPadCode;
     const int nt = n < 4096 ? 1 : static_cast<int>(std::min(16u, std::max(1u, std::thread::hardware_concurrency())));
     std::vector<std::vector<long long>> found(nt);
     std::vector<char> over(nt, 0);
@@ -840,7 +844,7 @@ int spmv_configure(hfh_csr *A, int algo, int p1, int p2) {
             // 83 us), else FP64 with slot pairs and FP32 with 4 slots in
             // flight.  Short (stencil) rows share few x entries within a
             // slice; they run fastest with one-byte dictionary codes where
-            // the matrix has at most 256 distinct offsets col - row (7-point
+            // the matrix has at most 255 distinct offsets col - row (7-point
             // FV 192^3: FP64 115 us against 129 us with plain SELL-64, FP32
             // 72 us against 79 us), else with SELL-64, 8 slots in flight
             if (mean >= 16.0) {
@@ -858,7 +862,7 @@ int spmv_configure(hfh_csr *A, int algo, int p1, int p2) {
         }
         p2 = 0;
     }
-    // the dictionary of distinct column offsets (at most 256 codes)
+    // the dictionary of distinct column offsets (at most 255 codes)
     std::vector<int> dict;
     if (algo == HFH_SPMV_SELL_DICT && !dict_offsets(A, dict)) {
         algo = HFH_SPMV_SELL;
@@ -961,8 +965,8 @@ int spmv_configure(hfh_csr *A, int algo, int p1, int p2) {
         wp[nsl] = static_cast<int>(woff);
         // code words: byte t of word w of a lane = code of slot 4w + t
         std::vector<unsigned> words(std::max(woff, 1LL), 0u);
-        // padding slots take the code of offset 0, which the table always
-        // holds: their gather reads x[min(r, ncol - 1)], in range
+        // padding slots take kPadCode (offset 0 in the kernel's table: their
+        // gather reads x[min(r, ncol - 1)], in range); zero: the diagonal's
         const unsigned zero = static_cast<unsigned>(std::lower_bound(dict.begin(), dict.end(), 0) - dict.begin());
         host_parallel(nsl, [&](int s0, int s1) {
             std::vector<unsigned> codes;
@@ -984,7 +988,7 @@ int spmv_configure(hfh_csr *A, int algo, int p1, int p2) {
                     }
                     if (r < n) prev = r;
                     for (int k = 0; k < L4; ++k) {
-                        const unsigned code = k < l ? codes[k] : zero;
+                        const unsigned code = k < l ? codes[k] : kPadCode;
                         words[wp[sl] + (k / 4) * 64 + lane] |= code << (8 * (k % 4));
                     }
                 }
@@ -1095,9 +1099,9 @@ static void dict_launch(const hfh_csr *A, int nwg, const T *x, T *y, const T *w,
     const T *sv = static_cast<const T *>(A->sl_val);
     const unsigned code0 = static_cast<unsigned>(A->sd_code0);
     if (A->spmv_p1 == 2)
-        hipLaunchKernelGGL((k_spmv_selld<T, ADD, DOT, 2, JAC>), g, bl, 0, s, A->nrow, A->ncol, A->nslice, A->sl_ptr, A->sd_wptr, A->sl_len, A->sd_code, A->sd_dict, A->sd_ndict, sv, x, y, w, part, done, code0, b, invd, jw, jomw);
+        hipLaunchKernelGGL((k_spmv_selld<T, ADD, DOT, 2, JAC>), g, bl, 0, s, A->nrow, A->ncol, A->nslice, A->sl_ptr, A->sd_wptr, A->sd_code, A->sd_dict, A->sd_ndict, sv, x, y, w, part, done, code0, b, invd, jw, jomw);
     else
-        hipLaunchKernelGGL((k_spmv_selld<T, ADD, DOT, 1, JAC>), g, bl, 0, s, A->nrow, A->ncol, A->nslice, A->sl_ptr, A->sd_wptr, A->sl_len, A->sd_code, A->sd_dict, A->sd_ndict, sv, x, y, w, part, done, code0, b, invd, jw, jomw);
+        hipLaunchKernelGGL((k_spmv_selld<T, ADD, DOT, 1, JAC>), g, bl, 0, s, A->nrow, A->ncol, A->nslice, A->sl_ptr, A->sd_wptr, A->sd_code, A->sd_dict, A->sd_ndict, sv, x, y, w, part, done, code0, b, invd, jw, jomw);
 }
 
 template <typename T>
