@@ -23,10 +23,10 @@
 // with the matrix in SELL-64 form (slices of 64 rows, column-major, E
 // padded entries; FP64 values for the true residual, FP32 values for the
 // inner solve, one shared column array).  `cols` is either sl_col[E] or,
-// when the system has at most 256 distinct offsets col - row (every FV
-// system: 5 or 7), one-byte codes into the system's offset table dict[256]:
-// code word w of a lane holds slots 4w..4w+3, ceil(L/4) words per lane of a
-// slice of width L, so the values keep the slice width.  A slot table holds
+// for every FV system (5 or 7 distinct offsets col - row), one-byte codes
+// into the system's offset table dict[256]: code word w of a lane holds
+// slots 4w..4w+3, ceil(L/4) words per lane of a slice of width L, so the
+// values keep the slice width; a row's padding slots hold kPadCode.  A slot table holds
 // each system's descriptor (pointers, sizes) and its solver state.  A tile
 // table lists every active system's tiles (16 slices = 1024 rows each).
 //
@@ -213,13 +213,14 @@ __device__ __forceinline__ T sell_row(const GDesc &D, gptr<const T> val, const F
 }
 
 // The same row sum over coded columns: column = min(r + sdict[code], n-1)
-// (padding slots hold the code of offset 0; rows past n read x[n-1] and are
-// not stored).  Word w of a lane (slots 4w..4w+3) sits at
-// sl_wptr[s] + 64 w + lane; the values keep the slice width L.  Two words
-// (8 slots) per step, the slots past L skipped (wave-uniform).
+// (padding slots hold kPadCode, offset 0 in sdict, and are not summed, so
+// no row length is loaded; rows past n read x[n-1] and are not stored).
+// Word w of a lane (slots 4w..4w+3) sits at sl_wptr[s] + 64 w + lane; the
+// values keep the slice width L.  Two words (8 slots) per step, the slots
+// past L skipped (wave-uniform).
 template <typename T, typename F>
-__device__ __forceinline__ T sell_row_coded(const GDesc &D, gptr<const T> val, const F &v, int s, int lane, int len,
-                                            int r, const int *sdict) {
+__device__ __forceinline__ T sell_row_coded(const GDesc &D, gptr<const T> val, const F &v, int s, int lane, int r,
+                                            const int *sdict) {
     const int base = D.sl_ptr[s];
     const int L = (D.sl_ptr[s + 1] - base) >> 6;
     const int wb = D.sl_wptr[s];
@@ -242,7 +243,7 @@ __device__ __forceinline__ T sell_row_coded(const GDesc &D, gptr<const T> val, c
         }
 #pragma unroll
         for (int u = 0; u < 8; ++u)
-            if (4 * w + u < len) acc = badd(acc, bmul(a[u], xv[u]));
+            if (4 * w + u < L && ((cw[u >> 2] >> (8 * (u & 3))) & 255u) != kPadCode) acc = badd(acc, bmul(a[u], xv[u]));
     }
     return acc;
 }
@@ -374,10 +375,10 @@ __device__ __forceinline__ double spmv_tile(const GDesc &D, const Tile &T, int p
     double dsum = 0;
     for (int s = T.s0 + wave; s < T.s0 + T.ns; s += kBlock / 64) {
         const int r = s * 64 + lane;
-        const int len = r < D.n ? D.sl_len[r] : 0;
+        const int len = !coded && r < D.n ? D.sl_len[r] : 0;   // coded: padding slots carry kPadCode
         if (!outer) {
             const VecAt<float> f{gptr<const float>(D.u)};
-            const float acc = coded ? sell_row_coded<float>(D, D.v32, f, s, lane, len, r, sdict)
+            const float acc = coded ? sell_row_coded<float>(D, D.v32, f, s, lane, r, sdict)
                                     : sell_row<float>(D, D.v32, f, s, lane, len);
             if (r < D.n) {
                 D.w[r] = acc;
@@ -385,7 +386,7 @@ __device__ __forceinline__ double spmv_tile(const GDesc &D, const Tile &T, int p
             }
         } else {
             const VecAt<double> f{gptr<const double>(D.x)};
-            const double acc = coded ? sell_row_coded<double>(D, D.v64, f, s, lane, len, r, sdict)
+            const double acc = coded ? sell_row_coded<double>(D, D.v64, f, s, lane, r, sdict)
                                      : sell_row<double>(D, D.v64, f, s, lane, len);
             if (r < D.n) {
                 const double rr = D.b[r] - acc;
@@ -437,7 +438,7 @@ __global__ __launch_bounds__(kBlock) void kb_spmv(const SysDesc *__restrict__ sd
         if (!works(ph)) continue;   // uniform over the workgroup
         const GDesc D = gview(sd[T.sys]);
         if (D.ndict > 0) {   // the system's offset table (the previous tile's readers passed the barrier below)
-            for (int i = threadIdx.x; i < D.ndict; i += kBlock) sdict[i] = D.dict[i];
+            for (int i = threadIdx.x; i < 256; i += kBlock) sdict[i] = i < D.ndict ? D.dict[i] : 0;
             __syncthreads();
         }
         const double tot = block_sum(spmv_tile(D, T, ph, wave, lane, sdict), sh);
@@ -578,7 +579,7 @@ __global__ __launch_bounds__(kSmallBlock) void kb_small(const SysDesc *__restric
     for (int k = blockIdx.x; k < nsmall; k += gridDim.x) {
         const int slot = small[k];
         const GDesc D = gview(sd[slot]);
-        for (int i = threadIdx.x; i < D.ndict; i += kSmallBlock) sdict[i] = D.dict[i];
+        for (int i = threadIdx.x; i < 256; i += kSmallBlock) sdict[i] = i < D.ndict ? D.dict[i] : 0;
         if (threadIdx.x == 0) S = st[slot];
         __syncthreads();
         const bool mine = g < D.ntiles;   // this group's tile exists
@@ -647,7 +648,8 @@ __global__ __launch_bounds__(kBlock) void kb_fv_assemble(int nx, int ny, int nz,
     const bool in3 = nz > 1;
     const long long nxy = static_cast<long long>(nx) * ny;
     // offset table, in the order of a row's entries: -nxy, -nx, -1, 0, 1, nx, nxy
-    // (2D: without +-nxy); code0 = the code of offset 0 (padding)
+    // (2D: without +-nxy); code0 = the code of offset 0 (the diagonal);
+    // padding slots and rows past n: kPadCode
     const int nd = in3 ? 7 : 5, code0 = in3 ? 3 : 2;
     const int WW = (W + 3) / 4;   // code words per lane and slice
     const int r = blockIdx.x * kBlock + threadIdx.x;
@@ -671,7 +673,7 @@ __global__ __launch_bounds__(kBlock) void kb_fv_assemble(int nx, int ny, int nz,
             v64[base + e * 64 + lane] = 0.0;
             v32[base + e * 64 + lane] = 0.0f;
         }
-        for (int e = 0; e < 4 * WW; ++e) code_at(e, code0);
+        for (int e = 0; e < 4 * WW; ++e) code_at(e, kPadCode);
         for (int w = 0; w < WW; ++w) sl_code[wbase + w * 64 + lane] = words[w];
         return;
     }
@@ -714,7 +716,7 @@ __global__ __launch_bounds__(kBlock) void kb_fv_assemble(int nx, int ny, int nz,
         v64[idx] = 0.0;
         v32[idx] = 0.0f;
     }
-    for (int f = e; f < 4 * WW; ++f) code_at(f, code0);
+    for (int f = e; f < 4 * WW; ++f) code_at(f, kPadCode);
     for (int w = 0; w < WW; ++w) sl_code[wbase + w * 64 + lane] = words[w];
     invd[r] = 1.0f / static_cast<float>(diag);
 }
