@@ -803,13 +803,19 @@ def test_gmres_stops_at_maxits_mid_cycle(pkg, ctx, oracle, maxits):
     assert relnorm(X.numpy(), xo) < 1e-10
 
 
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
 @pytest.mark.parametrize("where", ["two_per_thread", "at_limit", "above_limit"])
-def test_gmres_mgs_cooperative_and_multi_launch(pkg, ctx, oracle, where):
+def test_gmres_mgs_cooperative_and_multi_launch(pkg, ctx, oracle, where, dtype):
     """The MGS sweep of a column runs as one cooperative launch (w kept in
     registers: 2 elements per thread at 2 x 1024 x CUs rows, 8 at the limit
     of 8 x 1024 x CUs) or, above the limit (and at most 1024 x CUs rows), as
     one launch per step: each follows the oracle's GMRES for 10 iterations on
-    a nonsymmetric tridiagonal system of that size."""
+    a nonsymmetric tridiagonal system of that size, in FP64 and in FP32.  The
+    FP32 solve is checked against the oracle in FP64 on the same FP32-rounded
+    matrix and right-hand side: the oracle's own FP32 run sums its dot products
+    (half a million terms and more) sequentially and drifts 2.3 % from exact
+    arithmetic by the 8th iteration at 524,288 rows, while the device's tree
+    reductions stay within 1e-4."""
     per = 1024 * ctx.info()["num_cu"]
     n = {"two_per_thread": 2 * per, "at_limit": 8 * per, "above_limit": 8 * per + 1000}[where]
     rng = np.random.default_rng(21)
@@ -820,19 +826,20 @@ def test_gmres_mgs_cooperative_and_multi_launch(pkg, ctx, oracle, where):
     keep = (c >= 0) & (c < n)
     row = np.zeros(n + 1, np.int32)
     row[1:] = np.cumsum(keep.sum(axis=1))
-    col, val = c[keep].astype(np.int32), v[keep]
-    rhs = rng.uniform(-1, 1, n)
+    col, val = c[keep].astype(np.int32), v[keep].astype(dtype)
+    rhs = rng.uniform(-1, 1, n).astype(dtype)
     A = pkg.CsrMatrix(ctx, row, col, val)
     P = pkg.Preconditioner.jacobi(A)
-    print(f"gmres mgs case {where}: n {n}", flush=True)
-    X = pkg.Vector(ctx, n)
-    info = pkg.gmres(A, pkg.Vector(ctx, rhs), X, P, m=8, maxits=10, atol=0.0, rtol=1e-30, history=True,
+    print(f"gmres mgs case {where} {np.dtype(dtype).name}: n {n}", flush=True)
+    X = pkg.Vector(ctx, n, dtype)
+    info = pkg.gmres(A, pkg.Vector(ctx, rhs, dtype), X, P, m=8, maxits=10, atol=0.0, rtol=1e-30, history=True,
                      check=False)
-    OA = oracle.Csr(row, col, val)
-    xo, oi = oracle.gmres(OA, rhs, P=oracle.Prec.jacobi(OA), m=8, ctrl=(10, 0.0, 1e-30, 1e6))
+    OA = oracle.Csr(row, col, val.astype(np.float64))
+    xo, oi = oracle.gmres(OA, rhs.astype(np.float64), P=oracle.Prec.jacobi(OA), m=8, ctrl=(10, 0.0, 1e-30, 1e6))
     assert info["iters"] == oi["iters"] == 10
-    assert np.allclose(info["hist"], oi["hist"], rtol=1e-9)
-    assert relnorm(X.numpy(), xo) < 1e-10
+    f64 = dtype == np.float64
+    assert np.allclose(info["hist"], oi["hist"], rtol=1e-9 if f64 else 1e-4)
+    assert relnorm(X.numpy(), xo) < (1e-10 if f64 else 1e-4)
 
 
 def test_defect_correction_matches_fp64(pkg, ctx, oracle):
