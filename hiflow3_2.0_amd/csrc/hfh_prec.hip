@@ -118,30 +118,90 @@ constexpr int kSweepU = 8;   // column slots in flight (two code words when code
 // code is not kPadCode, so the row's length is not loaded).  Slots past the
 // slice's width L are not loaded (a wave-uniform test: L is the slice's), so
 // a slice of 3 slots issues 3 loads, not kSweepU.
+// The loads (sweep_raw: stored columns, or kSweepU / 4 code words) and their
+// decoding (sweep_decode) are separate so that a wave can issue the loads of
+// its next slice before it needs them.
 template <bool CODED>
-__device__ __forceinline__ void sweep_cols(int (&cj)[kSweepU], bool (&live)[kSweepU], int k, int L, int len, int base,
-                                           int wbase, int lane, int i, int n, const int *__restrict__ scol,
-                                           const unsigned *__restrict__ code, const int *dict) {
+__device__ __forceinline__ void sweep_raw(unsigned (&raw)[kSweepU], int k, int L, int base, int wbase, int lane, int i,
+                                          int n, const int *__restrict__ scol, const unsigned *__restrict__ code) {
     if (!CODED) {
 #pragma unroll
-        for (int u = 0; u < kSweepU; ++u) {
-            cj[u] = k + u < L ? __builtin_nontemporal_load(&scol[base + (k + u) * 64 + lane]) : min(i, n - 1);
-            live[u] = k + u < len;
-        }
+        for (int u = 0; u < kSweepU; ++u)
+            raw[u] = k + u < L ? __builtin_nontemporal_load(&scol[base + (k + u) * 64 + lane]) : min(i, n - 1);
         return;
     }
     const int W = (L + 3) >> 2;   // code words of the slice (its width rounded up to 4 slots)
 #pragma unroll
-    for (int q = 0; q < kSweepU / 4; ++q) {
-        const unsigned cw =
-            (k >> 2) + q < W ? __builtin_nontemporal_load(&code[wbase + ((k >> 2) + q) * 64 + lane]) : ~0u;
+    for (int q = 0; q < kSweepU / 4; ++q)
+        raw[q] = (k >> 2) + q < W ? __builtin_nontemporal_load(&code[wbase + ((k >> 2) + q) * 64 + lane]) : ~0u;
+}
+
+template <bool CODED>
+__device__ __forceinline__ void sweep_decode(const unsigned (&raw)[kSweepU], int (&cj)[kSweepU], bool (&live)[kSweepU],
+                                             int k, int len, int i, int n, const int *dict) {
+    if (!CODED) {
+#pragma unroll
+        for (int u = 0; u < kSweepU; ++u) {
+            cj[u] = static_cast<int>(raw[u]);
+            live[u] = k + u < len;
+        }
+        return;
+    }
+#pragma unroll
+    for (int q = 0; q < kSweepU / 4; ++q)
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-            const unsigned cd = (cw >> (8 * t)) & 255u;
+            const unsigned cd = (raw[q] >> (8 * t)) & 255u;
             cj[4 * q + t] = min(i + dict[cd], n - 1);
             live[4 * q + t] = cd != kPadCode;
         }
-    }
+}
+
+template <bool CODED>
+__device__ __forceinline__ void sweep_cols(int (&cj)[kSweepU], bool (&live)[kSweepU], int k, int L, int len, int base,
+                                           int wbase, int lane, int i, int n, const int *__restrict__ scol,
+                                           const unsigned *__restrict__ code, const int *dict) {
+    unsigned raw[kSweepU];
+    sweep_raw<CODED>(raw, k, L, base, wbase, lane, i, n, scol, code);
+    sweep_decode<CODED>(raw, cj, live, k, len, i, n, dict);
+}
+
+// Workgroups per CU of a multicolour SGS colour step (0: one slice per
+// wave).  FP32 steps are light: 8 workgroups per CU whose waves take several
+// slices each (and store 8 r.z partials per CU instead of one per 4 slices)
+// beat one slice per wave; FP64 keeps one slice per wave (FV 192^3: the FP32
+// defect correction 96 -> 85 ms with the FP32 SpMV's like grid; FP64 CG +
+// MC-SGS 150 ms either way, 158 ms with FP64 steps capped too).
+template <typename T> constexpr int kSweepGridPerCu = 0;
+template <> constexpr int kSweepGridPerCu<float> = 8;
+
+// A slice's header, its first step's loads and its rows' own inputs.  On a
+// capped grid (kSweepGridPerCu > 0) a wave walks several slices and issues
+// these one slice ahead, after the current slice's gathers, so their latency
+// overlaps the gathers.  With one slice per wave (FP64) the colour steps keep
+// their loads in place: the look-ahead loop costs those kernels registers
+// and occupancy (70 -> 92 VGPRs) for no overlap.
+template <typename T>
+struct SweepAhead {
+    int base = 0, L = 0, wbase = 0, len = 0;
+    unsigned raw[kSweepU];
+    T v[kSweepU];
+    T a = 0, b = 0;   // forward: r_i; backward: z_i and d_i
+};
+
+template <typename T, bool CODED>
+__device__ __forceinline__ void sweep_ahead(SweepAhead<T> &na, int s, int i, bool valid, int lane, int n,
+                                            const int *__restrict__ sptr, const int *__restrict__ slen,
+                                            const int *__restrict__ scol, const unsigned *__restrict__ code,
+                                            const int *__restrict__ wptr, const T *__restrict__ sval) {
+    na.base = sptr[s];
+    na.L = (sptr[s + 1] - na.base) >> 6;
+    na.wbase = CODED ? wptr[s] : 0;
+    na.len = !CODED && valid ? slen[i] : 0;   // coded: padding slots carry kPadCode
+    sweep_raw<CODED>(na.raw, 0, na.L, na.base, na.wbase, lane, i, n, scol, code);
+#pragma unroll
+    for (int u = 0; u < kSweepU; ++u)   // slots past the slice width: not loaded (wave-uniform)
+        na.v[u] = u < na.L ? __builtin_nontemporal_load(&sval[na.base + u * 64 + lane]) : static_cast<T>(0);
 }
 
 // RZ: the step also stores its workgroup's partial of in.out over its rows
@@ -162,45 +222,105 @@ __global__ __launch_bounds__(kBlock) void k_mcs_fwd(int c, ColorOffsets co, bool
     }
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     double rz = 0;
-    for (int w = blockIdx.x * (kBlock / 64) + wave; w < nsl; w += gridDim.x * (kBlock / 64)) {
-        const int s = slice0 + w;
-        const int i = co.off[c] + w * 64 + lane;
-        const bool valid = i < co.off[c + 1];
-        const int base = sptr[s], L = (sptr[s + 1] - base) >> 6;
-        const int wbase = CODED ? wptr[s] : 0;
-        const int len = !CODED && valid ? slen[i] : 0;   // coded: padding slots carry kPadCode
-        const T ri = valid ? in[i] : static_cast<T>(0);
-        T chunk = ri * static_cast<T>(-1);
-        int blk = 0, blk_end = co.off[1];
-        T res = 0;
-        for (int k = 0; k < L; k += kSweepU) {
-            int cj[kSweepU];
-            bool live[kSweepU];
-            T v[kSweepU], xv[kSweepU];
-            sweep_cols<CODED>(cj, live, k, L, len, base, wbase, lane, i, n, scol, code, dict);
+    if constexpr (kSweepGridPerCu<T> == 0) {   // one slice per wave: loads in place
+        for (int w = blockIdx.x * (kBlock / 64) + wave; w < nsl; w += gridDim.x * (kBlock / 64)) {
+            const int s = slice0 + w;
+            const int i = co.off[c] + w * 64 + lane;
+            const bool valid = i < co.off[c + 1];
+            const int base = sptr[s], L = (sptr[s + 1] - base) >> 6;
+            const int wbase = CODED ? wptr[s] : 0;
+            const int len = !CODED && valid ? slen[i] : 0;   // coded: padding slots carry kPadCode
+            const T ri = valid ? in[i] : static_cast<T>(0);
+            T chunk = ri * static_cast<T>(-1);
+            int blk = 0, blk_end = co.off[1];
+            T res = 0;
+            for (int k = 0; k < L; k += kSweepU) {
+                int cj[kSweepU];
+                bool live[kSweepU];
+                T v[kSweepU], xv[kSweepU];
+                sweep_cols<CODED>(cj, live, k, L, len, base, wbase, lane, i, n, scol, code, dict);
 #pragma unroll
-            for (int u = 0; u < kSweepU; ++u) {   // slots past the slice width: not loaded (wave-uniform)
-                v[u] = k + u < L ? __builtin_nontemporal_load(&sval[base + (k + u) * 64 + lane]) : static_cast<T>(0);
-                xv[u] = k + u < L ? out[cj[u]] : static_cast<T>(0);
-            }
-#pragma unroll
-            for (int u = 0; u < kSweepU; ++u) {
-                if (!live[u]) continue;
-                while (cj[u] >= blk_end) {   // entries are column sorted: flush finished blocks
-                    chunk += res;
-                    res = 0;
-                    ++blk;
-                    blk_end = co.off[blk + 1];
+                for (int u = 0; u < kSweepU; ++u) {   // slots past the slice width: not loaded (wave-uniform)
+                    v[u] = k + u < L ? __builtin_nontemporal_load(&sval[base + (k + u) * 64 + lane]) : static_cast<T>(0);
+                    xv[u] = k + u < L ? out[cj[u]] : static_cast<T>(0);
                 }
-                res += v[u] * xv[u];
+#pragma unroll
+                for (int u = 0; u < kSweepU; ++u) {
+                    if (!live[u]) continue;
+                    while (cj[u] >= blk_end) {   // entries are column sorted: flush finished blocks
+                        chunk += res;
+                        res = 0;
+                        ++blk;
+                        blk_end = co.off[blk + 1];
+                    }
+                    res += v[u] * xv[u];
+                }
+            }
+            if (valid) {
+                chunk += res;
+                chunk *= idv[i];
+                if (last) chunk = (chunk * dv[i]) * idv[i];
+                out[i] = chunk;
+                if (RZ) rz += static_cast<double>(ri) * static_cast<double>(chunk);
             }
         }
-        if (valid) {
-            chunk += res;
-            chunk *= idv[i];
-            if (last) chunk = (chunk * dv[i]) * idv[i];
-            out[i] = chunk;
-            if (RZ) rz += static_cast<double>(ri) * static_cast<double>(chunk);
+    } else {
+        const int stride = gridDim.x * (kBlock / 64);
+        SweepAhead<T> na;
+        auto ahead = [&](int wn) {
+            const int in_ = co.off[c] + wn * 64 + lane;
+            const bool vn = in_ < co.off[c + 1];
+            sweep_ahead<T, CODED>(na, slice0 + wn, in_, vn, lane, n, sptr, slen, scol, code, wptr, sval);
+            na.a = vn ? in[in_] : static_cast<T>(0);
+        };
+        int w = blockIdx.x * (kBlock / 64) + wave;
+        if (w < nsl) ahead(w);
+        for (; w < nsl; w += stride) {
+            const int i = co.off[c] + w * 64 + lane;
+            const bool valid = i < co.off[c + 1];
+            const int base = na.base, L = na.L, wbase = na.wbase, len = na.len;
+            const T ri = na.a;
+            T chunk = ri * static_cast<T>(-1);
+            int blk = 0, blk_end = co.off[1];
+            T res = 0;
+            for (int k = 0; k < L; k += kSweepU) {
+                int cj[kSweepU];
+                bool live[kSweepU];
+                T v[kSweepU], xv[kSweepU];
+                if (k == 0) {
+                    sweep_decode<CODED>(na.raw, cj, live, 0, len, i, n, dict);
+#pragma unroll
+                    for (int u = 0; u < kSweepU; ++u) v[u] = na.v[u];
+                } else {
+                    sweep_cols<CODED>(cj, live, k, L, len, base, wbase, lane, i, n, scol, code, dict);
+#pragma unroll
+                    for (int u = 0; u < kSweepU; ++u)   // slots past the slice width: not loaded (wave-uniform)
+                        v[u] = k + u < L ? __builtin_nontemporal_load(&sval[base + (k + u) * 64 + lane])
+                                         : static_cast<T>(0);
+                }
+#pragma unroll
+                for (int u = 0; u < kSweepU; ++u) xv[u] = k + u < L ? out[cj[u]] : static_cast<T>(0);
+                if (k + kSweepU >= L && w + stride < nsl) ahead(w + stride);   // after this step's gathers
+#pragma unroll
+                for (int u = 0; u < kSweepU; ++u) {
+                    if (!live[u]) continue;
+                    while (cj[u] >= blk_end) {   // entries are column sorted: flush finished blocks
+                        chunk += res;
+                        res = 0;
+                        ++blk;
+                        blk_end = co.off[blk + 1];
+                    }
+                    res += v[u] * xv[u];
+                }
+            }
+            if (L == 0 && w + stride < nsl) ahead(w + stride);   // an empty slice ran no step
+            if (valid) {
+                chunk += res;
+                chunk *= idv[i];
+                if (last) chunk = (chunk * dv[i]) * idv[i];
+                out[i] = chunk;
+                if (RZ) rz += static_cast<double>(ri) * static_cast<double>(chunk);
+            }
         }
     }
     if (RZ) {
@@ -229,43 +349,102 @@ __global__ __launch_bounds__(kBlock) void k_mcs_bwd(int c, int ncolors, ColorOff
     }
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     double rz = 0;
-    for (int w = blockIdx.x * (kBlock / 64) + wave; w < nsl; w += gridDim.x * (kBlock / 64)) {
-        const int s = slice0 + w;
-        const int i = co.off[c] + w * 64 + lane;
-        const bool valid = i < co.off[c + 1];
-        const int base = sptr[s], L = (sptr[s + 1] - base) >> 6;
-        const int wbase = CODED ? wptr[s] : 0;
-        const int len = !CODED && valid ? slen[i] : 0;   // coded: padding slots carry kPadCode
-        T chunk = valid ? out[i] * dv[i] : static_cast<T>(0);
-        int blk = ncolors - 1, blk_lo = co.off[ncolors - 1];
-        T res = 0;
-        for (int k = 0; k < L; k += kSweepU) {
-            int cj[kSweepU];
-            bool live[kSweepU];
-            T v[kSweepU], xv[kSweepU];
-            sweep_cols<CODED>(cj, live, k, L, len, base, wbase, lane, i, n, scol, code, dict);
+    if constexpr (kSweepGridPerCu<T> == 0) {   // one slice per wave: loads in place
+        for (int w = blockIdx.x * (kBlock / 64) + wave; w < nsl; w += gridDim.x * (kBlock / 64)) {
+            const int s = slice0 + w;
+            const int i = co.off[c] + w * 64 + lane;
+            const bool valid = i < co.off[c + 1];
+            const int base = sptr[s], L = (sptr[s + 1] - base) >> 6;
+            const int wbase = CODED ? wptr[s] : 0;
+            const int len = !CODED && valid ? slen[i] : 0;   // coded: padding slots carry kPadCode
+            T chunk = valid ? out[i] * dv[i] : static_cast<T>(0);
+            int blk = ncolors - 1, blk_lo = co.off[ncolors - 1];
+            T res = 0;
+            for (int k = 0; k < L; k += kSweepU) {
+                int cj[kSweepU];
+                bool live[kSweepU];
+                T v[kSweepU], xv[kSweepU];
+                sweep_cols<CODED>(cj, live, k, L, len, base, wbase, lane, i, n, scol, code, dict);
 #pragma unroll
-            for (int u = 0; u < kSweepU; ++u) {   // slots past the slice width: not loaded (wave-uniform)
-                v[u] = k + u < L ? __builtin_nontemporal_load(&sval[base + (k + u) * 64 + lane]) : static_cast<T>(0);
-                xv[u] = k + u < L ? out[cj[u]] : static_cast<T>(0);
-            }
-#pragma unroll
-            for (int u = 0; u < kSweepU; ++u) {
-                if (!live[u]) continue;
-                while (cj[u] < blk_lo) {   // next lower colour block
-                    chunk += res;
-                    res = 0;
-                    --blk;
-                    blk_lo = co.off[blk];
+                for (int u = 0; u < kSweepU; ++u) {   // slots past the slice width: not loaded (wave-uniform)
+                    v[u] = k + u < L ? __builtin_nontemporal_load(&sval[base + (k + u) * 64 + lane]) : static_cast<T>(0);
+                    xv[u] = k + u < L ? out[cj[u]] : static_cast<T>(0);
                 }
-                res += v[u] * xv[u];
+#pragma unroll
+                for (int u = 0; u < kSweepU; ++u) {
+                    if (!live[u]) continue;
+                    while (cj[u] < blk_lo) {   // next lower colour block
+                        chunk += res;
+                        res = 0;
+                        --blk;
+                        blk_lo = co.off[blk];
+                    }
+                    res += v[u] * xv[u];
+                }
+            }
+            if (valid) {
+                chunk += res;
+                const T zi = chunk * idv[i];
+                out[i] = zi;
+                if (RZ) rz += static_cast<double>(rin[i]) * static_cast<double>(zi);
             }
         }
-        if (valid) {
-            chunk += res;
-            const T zi = chunk * idv[i];
-            out[i] = zi;
-            if (RZ) rz += static_cast<double>(rin[i]) * static_cast<double>(zi);
+    } else {
+        const int stride = gridDim.x * (kBlock / 64);
+        SweepAhead<T> na;
+        auto ahead = [&](int wn) {   // z_i and d_i of the next slice's rows: only this wave writes them
+            const int in_ = co.off[c] + wn * 64 + lane;
+            const bool vn = in_ < co.off[c + 1];
+            sweep_ahead<T, CODED>(na, slice0 + wn, in_, vn, lane, n, sptr, slen, scol, code, wptr, sval);
+            na.a = vn ? out[in_] : static_cast<T>(0);
+            na.b = vn ? dv[in_] : static_cast<T>(0);
+        };
+        int w = blockIdx.x * (kBlock / 64) + wave;
+        if (w < nsl) ahead(w);
+        for (; w < nsl; w += stride) {
+            const int i = co.off[c] + w * 64 + lane;
+            const bool valid = i < co.off[c + 1];
+            const int base = na.base, L = na.L, wbase = na.wbase, len = na.len;
+            T chunk = valid ? na.a * na.b : static_cast<T>(0);
+            int blk = ncolors - 1, blk_lo = co.off[ncolors - 1];
+            T res = 0;
+            for (int k = 0; k < L; k += kSweepU) {
+                int cj[kSweepU];
+                bool live[kSweepU];
+                T v[kSweepU], xv[kSweepU];
+                if (k == 0) {
+                    sweep_decode<CODED>(na.raw, cj, live, 0, len, i, n, dict);
+#pragma unroll
+                    for (int u = 0; u < kSweepU; ++u) v[u] = na.v[u];
+                } else {
+                    sweep_cols<CODED>(cj, live, k, L, len, base, wbase, lane, i, n, scol, code, dict);
+#pragma unroll
+                    for (int u = 0; u < kSweepU; ++u)   // slots past the slice width: not loaded (wave-uniform)
+                        v[u] = k + u < L ? __builtin_nontemporal_load(&sval[base + (k + u) * 64 + lane])
+                                         : static_cast<T>(0);
+                }
+#pragma unroll
+                for (int u = 0; u < kSweepU; ++u) xv[u] = k + u < L ? out[cj[u]] : static_cast<T>(0);
+                if (k + kSweepU >= L && w + stride < nsl) ahead(w + stride);   // after this step's gathers
+#pragma unroll
+                for (int u = 0; u < kSweepU; ++u) {
+                    if (!live[u]) continue;
+                    while (cj[u] < blk_lo) {   // next lower colour block
+                        chunk += res;
+                        res = 0;
+                        --blk;
+                        blk_lo = co.off[blk];
+                    }
+                    res += v[u] * xv[u];
+                }
+            }
+            if (L == 0 && w + stride < nsl) ahead(w + stride);   // an empty slice ran no step
+            if (valid) {
+                chunk += res;
+                const T zi = chunk * idv[i];
+                out[i] = zi;
+                if (RZ) rz += static_cast<double>(rin[i]) * static_cast<double>(zi);
+            }
         }
     }
     if (RZ) {
@@ -892,15 +1071,6 @@ static int trsv_sweeps(hfh_prec *P, const T *in, T *out, const int *done) {
     }
     return launch_check("k_tsweep");
 }
-
-// Workgroups per CU of a multicolour SGS colour step (0: one slice per
-// wave).  FP32 steps are light: 8 workgroups per CU whose waves take several
-// slices each (and store 8 r.z partials per CU instead of one per 4 slices)
-// beat one slice per wave; FP64 keeps one slice per wave (FV 192^3: the FP32
-// defect correction 96 -> 85 ms with the FP32 SpMV's like grid; FP64 CG +
-// MC-SGS 150 ms either way, 158 ms with FP64 steps capped too).
-template <typename T> constexpr int kSweepGridPerCu = 0;
-template <> constexpr int kSweepGridPerCu<float> = 8;
 
 // r.z partials that a multicolour SGS apply with rzp stores: one per
 // workgroup of the steps that finalise z (an upper bound for any colouring).

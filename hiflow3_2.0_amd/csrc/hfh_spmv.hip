@@ -467,6 +467,12 @@ __global__ __launch_bounds__(kBlock) void k_spmv_sellx(
     }
 }
 
+// FP32 rows are light: waves that take several slices each (8 workgroups per
+// CU) beat one slice per wave (FV 192^3: 60.6 us against 70.1 us); FP64
+// keeps one slice per wave (103.7 us against 115.8 us at 8 workgroups per CU).
+template <typename T> constexpr bool kDictCappedGrid = false;
+template <> constexpr bool kDictCappedGrid<float> = true;
+
 // SELL-64 with dictionary-coded columns: column = row + dict[code].  Each
 // slice keeps its width L of value slots; code word w of a lane holds the
 // codes of slots 4w..4w+3 (byte t = slot 4w+t), ceil(L / 4) words per lane,
@@ -492,44 +498,119 @@ __global__ __launch_bounds__(kBlock) void k_spmv_selld(
     constexpr int WPB = kBlock / 64;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     double dsum = 0;
-    for (int s = blockIdx.x * WPB + wave; s < nslice; s += gridDim.x * WPB) {
-        const int vbase = sptr[s], L = (sptr[s + 1] - vbase) >> 6, W = (L + 3) >> 2;   // slots, code words
-        const int cbase = wptr[s];
-        const int r = s * 64 + lane;
-        T acc = 0;
-        for (int k = 0; k < W; k += UW) {
-            unsigned cw[UW];
-            T v[UW][4], xv[UW][4];
+    if constexpr (!kDictCappedGrid<T>) {   // one slice per wave (FP64): loads in place
+        for (int s = blockIdx.x * WPB + wave; s < nslice; s += gridDim.x * WPB) {
+            const int vbase = sptr[s], L = (sptr[s + 1] - vbase) >> 6, W = (L + 3) >> 2;   // slots, code words
+            const int cbase = wptr[s];
+            const int r = s * 64 + lane;
+            T acc = 0;
+            for (int k = 0; k < W; k += UW) {
+                unsigned cw[UW];
+                T v[UW][4], xv[UW][4];
+#pragma unroll
+                for (int u = 0; u < UW; ++u) {
+                    const int ww = min(k + u, W - 1);
+                    cw[u] = __builtin_nontemporal_load(&codes[cbase + ww * 64 + lane]);
+#pragma unroll
+                    for (int t = 0; t < 4; ++t)   // slots past the slice width: not stored (wave-uniform test)
+                        v[u][t] = 4 * ww + t < L ? __builtin_nontemporal_load(&sval[vbase + (4 * ww + t) * 64 + lane])
+                                                 : static_cast<T>(0);
+                }
+#pragma unroll
+                for (int u = 0; u < UW; ++u)
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) xv[u][t] = x[min(r + dict[(cw[u] >> (8 * t)) & 255u], ncol - 1)];
+#pragma unroll
+                for (int u = 0; u < UW; ++u)
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) {
+                        // words past W repeat the last one (the clamped load above)
+                        const unsigned cd = (cw[u] >> (8 * t)) & 255u;
+                        if (k + u < W && cd != kPadCode && (!JAC || cd != code0))
+                            acc = add_rn(acc, mul_rn(v[u][t], xv[u][t]));
+                    }
+            }
+            if (r < nrow) {
+                if (JAC) {
+                    y[r] = jacobi_update<T, JAC == 2>(acc, x[r], b[r], invd[r], jw, jomw);
+                } else {
+                    if (ADD) acc = y[r] + acc;
+                    y[r] = acc;
+                    if (DOT) dsum += static_cast<double>(acc) * static_cast<double>(w[r]);
+                }
+            }
+        }
+    } else {
+        // Loads of step k (words k..k+UW-1) of the slice at value base vb, code
+        // base cb, width L slots / W words; words past W repeat the last one.
+        auto load_step = [&](int vb, int cb, int L, int W, int k, unsigned (&cw)[UW], T (&v)[UW][4]) {
 #pragma unroll
             for (int u = 0; u < UW; ++u) {
                 const int ww = min(k + u, W - 1);
-                cw[u] = __builtin_nontemporal_load(&codes[cbase + ww * 64 + lane]);
+                cw[u] = __builtin_nontemporal_load(&codes[cb + ww * 64 + lane]);
 #pragma unroll
                 for (int t = 0; t < 4; ++t)   // slots past the slice width: not stored (wave-uniform test)
-                    v[u][t] = 4 * ww + t < L ? __builtin_nontemporal_load(&sval[vbase + (4 * ww + t) * 64 + lane])
+                    v[u][t] = 4 * ww + t < L ? __builtin_nontemporal_load(&sval[vb + (4 * ww + t) * 64 + lane])
                                              : static_cast<T>(0);
             }
+        };
+        const int stride = gridDim.x * WPB;
+        int s = blockIdx.x * WPB + wave;
+        // On the capped FP32 grid a wave walks several slices: the header and
+        // first step of its next slice are loaded before the row sums of its
+        // current one, so their latency overlaps the current gathers.
+        int nvb = 0, ncb = 0, nL = 0, nW = 0;
+        unsigned ncw[UW];
+        T nv[UW][4];
+        auto fetch = [&](int sn) {
+            nvb = sptr[sn];
+            nL = (sptr[sn + 1] - nvb) >> 6;
+            nW = (nL + 3) >> 2;
+            ncb = wptr[sn];
+            if (nW > 0) load_step(nvb, ncb, nL, nW, 0, ncw, nv);
+        };
+        if (s < nslice) fetch(s);
+        for (; s < nslice; s += stride) {
+            const int vbase = nvb, L = nL, W = nW, cbase = ncb;   // slots, code words
+            const int r = s * 64 + lane;
+            T acc = 0;
+            for (int k = 0; k < W; k += UW) {
+                unsigned cw[UW];
+                T v[UW][4], xv[UW][4];
+                if (k == 0) {
 #pragma unroll
-            for (int u = 0; u < UW; ++u)
+                    for (int u = 0; u < UW; ++u) {
+                        cw[u] = ncw[u];
 #pragma unroll
-                for (int t = 0; t < 4; ++t) xv[u][t] = x[min(r + dict[(cw[u] >> (8 * t)) & 255u], ncol - 1)];
-#pragma unroll
-            for (int u = 0; u < UW; ++u)
-#pragma unroll
-                for (int t = 0; t < 4; ++t) {
-                    // words past W repeat the last one (the clamped load above)
-                    const unsigned cd = (cw[u] >> (8 * t)) & 255u;
-                    if (k + u < W && cd != kPadCode && (!JAC || cd != code0))
-                        acc = add_rn(acc, mul_rn(v[u][t], xv[u][t]));
+                        for (int t = 0; t < 4; ++t) v[u][t] = nv[u][t];
+                    }
+                } else {
+                    load_step(vbase, cbase, L, W, k, cw, v);
                 }
-        }
-        if (r < nrow) {
-            if (JAC) {
-                y[r] = jacobi_update<T, JAC == 2>(acc, x[r], b[r], invd[r], jw, jomw);
-            } else {
-                if (ADD) acc = y[r] + acc;
-                y[r] = acc;
-                if (DOT) dsum += static_cast<double>(acc) * static_cast<double>(w[r]);
+#pragma unroll
+                for (int u = 0; u < UW; ++u)
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) xv[u][t] = x[min(r + dict[(cw[u] >> (8 * t)) & 255u], ncol - 1)];
+                if (k + UW >= W && s + stride < nslice) fetch(s + stride);   // after this step's gathers
+#pragma unroll
+                for (int u = 0; u < UW; ++u)
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) {
+                        // words past W repeat the last one (the clamped load above)
+                        const unsigned cd = (cw[u] >> (8 * t)) & 255u;
+                        if (k + u < W && cd != kPadCode && (!JAC || cd != code0))
+                            acc = add_rn(acc, mul_rn(v[u][t], xv[u][t]));
+                    }
+            }
+            if (W == 0 && s + stride < nslice) fetch(s + stride);   // an empty slice ran no step
+            if (r < nrow) {
+                if (JAC) {
+                    y[r] = jacobi_update<T, JAC == 2>(acc, x[r], b[r], invd[r], jw, jomw);
+                } else {
+                    if (ADD) acc = y[r] + acc;
+                    y[r] = acc;
+                    if (DOT) dsum += static_cast<double>(acc) * static_cast<double>(w[r]);
+                }
             }
         }
     }
@@ -1019,11 +1100,9 @@ int spmv_configure(hfh_csr *A, int algo, int p1, int p2) {
         A->spmv_algo = HFH_SPMV_SELL_DICT;
         A->spmv_p1 = p1 == 2 ? 2 : 1;
         const long long need = (nsl + kBlock / 64 - 1) / (kBlock / 64);
-        // FP32 rows are light: waves that take several slices each (8
-        // workgroups per CU) beat one slice per wave (FV 192^3: 60.6 us
-        // against 70.1 us); FP64 keeps one slice per wave (103.7 us against
-        // 115.8 us at 8 workgroups per CU)
-        const long long dflt = A->dtype == HFH_F32 ? std::min<long long>(need, static_cast<long long>(cus) * 8) : need;
+        // the kernel's look-ahead follows the same choice (kDictCappedGrid)
+        const bool capped = A->dtype == HFH_F32 ? kDictCappedGrid<float> : kDictCappedGrid<double>;
+        const long long dflt = capped ? std::min<long long>(need, static_cast<long long>(cus) * 8) : need;
         const long long g = p2 > 0 ? std::min<long long>(p2, need) : dflt;
         A->nwg = round8(std::max<long long>(g, 1));
         A->sell_dirty = true;  // values filled from the device CSR by spmv_sell_fill
