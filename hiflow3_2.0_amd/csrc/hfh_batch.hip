@@ -365,6 +365,71 @@ __device__ __forceinline__ void vec_tile(const GDesc &D, const Tile &T, const Sy
         vec_rows<P_GO>(D, r0, r1, rv, a, b, tid, ru, rr);
 }
 
+// w = A32 u over a tile of a dictionary-coded system, with this thread's part
+// of w.u: the rows of sell_row_coded<float> in the same order, but a wave
+// (which walks T.ns / 4 slices of the tile) loads its next slice's header,
+// code words and values right after its current gathers, so their latency
+// overlaps those gathers.
+__device__ __forceinline__ double spmv_tile_coded32(const GDesc &D, const Tile &T, int wave, int lane,
+                                                    const int *sdict) {
+    const VecAt<float> f{gptr<const float>(D.u)};
+    const gptr<const float> val = D.v32;
+    constexpr int kStride = kBlock / 64;
+    const int se = T.s0 + T.ns;
+    int nb = 0, nL = 0, nwb = 0;
+    unsigned ncw[2];
+    float na[8];
+    auto fetch = [&](int sn) {   // word 0..1 and slots 0..7 of slice sn
+        nb = D.sl_ptr[sn];
+        nL = (D.sl_ptr[sn + 1] - nb) >> 6;
+        nwb = D.sl_wptr[sn];
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+            ncw[h] = 4 * h < nL ? __builtin_nontemporal_load(&D.sl_code[nwb + h * 64 + lane]) : 0u;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) na[u] = u < nL ? __builtin_nontemporal_load(&val[nb + u * 64 + lane]) : 0.0f;
+    };
+    double dsum = 0;
+    int s = T.s0 + wave;
+    if (s < se) fetch(s);
+    for (; s < se; s += kStride) {
+        const int base = nb, L = nL, wb = nwb;
+        const int r = s * 64 + lane;
+        float acc = 0;
+        for (int w = 0; 4 * w < L; w += 2) {
+            unsigned cw[2];
+            float a[8], xv[8];
+            if (w == 0) {
+#pragma unroll
+                for (int h = 0; h < 2; ++h) cw[h] = ncw[h];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) a[u] = na[u];
+            } else {
+#pragma unroll
+                for (int h = 0; h < 2; ++h)
+                    cw[h] = 4 * (w + h) < L ? __builtin_nontemporal_load(&D.sl_code[wb + (w + h) * 64 + lane]) : 0u;
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+                    a[u] = 4 * w + u < L ? __builtin_nontemporal_load(&val[base + (4 * w + u) * 64 + lane]) : 0.0f;
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u)   // wave-uniform test
+                xv[u] = 4 * w + u < L ? f(min(r + sdict[(cw[u >> 2] >> (8 * (u & 3))) & 255u], D.n - 1)) : 0.0f;
+            if (4 * (w + 2) >= L && s + kStride < se) fetch(s + kStride);   // after this step's gathers
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (4 * w + u < L && ((cw[u >> 2] >> (8 * (u & 3))) & 255u) != kPadCode)
+                    acc = badd(acc, bmul(a[u], xv[u]));
+        }
+        if (L == 0 && s + kStride < se) fetch(s + kStride);   // an empty slice ran no step
+        if (r < D.n) {
+            D.w[r] = acc;
+            dsum += static_cast<double>(acc) * static_cast<double>(f(r));
+        }
+    }
+    return dsum;
+}
+
 // One tile's share of K2 for a group of kBlock threads (wave 0..3 of the
 // group, wave-uniform): w = A32 u with this thread's part of w.u, or the
 // FP64 true residual r64 = b - A64 x with its part of r64.r64.
@@ -373,6 +438,7 @@ __device__ __forceinline__ double spmv_tile(const GDesc &D, const Tile &T, int p
     const bool coded = D.ndict > 0;
     const bool outer = is_outer(ph);
     double dsum = 0;
+    if (coded && !outer) return spmv_tile_coded32(D, T, wave, lane, sdict);
     for (int s = T.s0 + wave; s < T.s0 + T.ns; s += kBlock / 64) {
         const int r = s * 64 + lane;
         const int len = !coded && r < D.n ? D.sl_len[r] : 0;   // coded: padding slots carry kPadCode
