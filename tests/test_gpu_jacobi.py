@@ -13,8 +13,9 @@ from conftest import relnorm
 
 pytestmark = pytest.mark.gpu
 
-LAYOUTS = [(4, 4), (4, 8), (6, 1), (7, 0), (5, 1), (5, 2), (1, 2048)]
-# SELL, SELL, pairs, LDS-staged, dictionary codes, row blocks (CSR step)
+LAYOUTS = [(4, 4), (4, 8), (6, 1), (7, 0), (5, 1), (5, 2), (5, 1, 1), (5, 2, 1), (1, 2048)]
+# SELL, SELL, pairs, LDS-staged, dictionary codes (the last two on one
+# workgroup, whose waves walk several slices each), row blocks (CSR step)
 
 
 def matrices(pkg, case):

@@ -129,6 +129,34 @@ def test_spmv_dictionary_columns(pkg, ctx, oracle, dtype, words):
     assert np.array_equal(Y.numpy(), oracle.spmv_add(oracle.Csr(row, col, val.astype(dtype), ncol), X.numpy(), y0))
 
 
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("words", [1, 2])
+def test_spmv_dictionary_waves_walk_slices(pkg, ctx, oracle, dtype, words):
+    """Dictionary-coded columns on one workgroup, whose 4 waves walk 6 slices
+    each (FP32: each wave loads its next slice ahead of its current row sums):
+    an all-empty slice, slices up to 20 slots wide (several steps), a partial
+    last slice; y = A x and y += A x bit for bit."""
+    rng = np.random.default_rng(11)
+    n, ncol = 1500, 1500
+    offs = np.sort(rng.choice(np.arange(-200, 200), 20, replace=False))
+    rows, cols = [0], []
+    for r in range(n):
+        k = 0 if 128 <= r < 192 or r % 97 == 0 else int(rng.integers(0, 21))
+        c = np.unique(np.clip(r + rng.choice(offs, k, replace=False), 0, ncol - 1)) if k else np.array([], int)
+        c = c[np.isin(c - r, offs)]
+        cols.extend(c.tolist())
+        rows.append(len(cols))
+    row, col = np.array(rows, np.int32), np.array(cols, np.int32)
+    assert row[192] == row[128]   # slice 2 holds no entry
+    val = rng.uniform(-1, 1, len(col))
+    y, ref, A, X, Y = spmv_case(pkg, ctx, oracle, row, col, val, ncol=ncol, dtype=dtype, algo=(5, words, 1))
+    assert np.array_equal(y, ref)
+    y0 = rng.uniform(-1, 1, n).astype(dtype)
+    Y.upload(y0)
+    A.spmv_add(X, Y)
+    assert np.array_equal(Y.numpy(), oracle.spmv_add(oracle.Csr(row, col, val.astype(dtype), ncol), X.numpy(), y0))
+
+
 def banded_ragged(rng, n, ncol, halfwidth, maxlen):
     """Rows of 0..maxlen random columns within +-halfwidth of the row (every
     97th row empty), clipped to [0, ncol)."""
@@ -435,6 +463,38 @@ def test_mcsgs_apply(pkg, ctx, oracle, omega, kind):
     P.apply(pkg.Vector(ctx, r), Z)
     ref = oracle.mcsgs(oracle.Csr(prow, pcol, pval), sizes, r, omega=omega)
     assert relnorm(Z.numpy(), ref) < 1e-13
+
+
+def test_mcsgs_fp32_waves_walk_slices(pkg, ctx, oracle):
+    """FP32 multicolour SGS colour steps run on 8 workgroups per CU; at FV
+    128^3 (16,384 slices per colour) their waves walk several slices each,
+    loading the next one ahead of the current row sums.  The apply follows
+    the oracle's FP32 apply; 20 CG + MC-SGS steps (the steps that store r.z
+    partials) follow the oracle's FP64 solve of the same FP32-rounded system
+    (the oracle's FP32 CG sums its 2-million-term dot products sequentially,
+    the device in FP64 partials)."""
+    n_cu = ctx.info()["num_cu"]
+    row, col, val, rhs = pkg.problem(pkg.FV_DIFFUSION, 128, p0=1.0, seed=2)
+    nc, color, sizes, perm = pkg.multicoloring(row, col)
+    assert nc == 2 and min(sizes) // 64 > 8 * n_cu * 4   # more slices than waves
+    prow, pcol, pval = pkg.csr_permute(row, col, val, perm)
+    pval, b = pval.astype(np.float32), rhs[perm].astype(np.float32)
+    A = pkg.CsrMatrix(ctx, prow, pcol, pval)
+    P = pkg.Preconditioner.mcsgs(A, sizes)
+    OA = oracle.Csr(prow, pcol, pval)
+    Z = pkg.Vector(ctx, len(b), np.float32)
+    P.apply(pkg.Vector(ctx, b, np.float32), Z)
+    ref = oracle.mcsgs(OA, sizes, b)
+    assert relnorm(Z.numpy(), ref) < 1e-5
+    X = pkg.Vector(ctx, len(b), np.float32)
+    info = pkg.pcg(A, pkg.Vector(ctx, b, np.float32), X, P, maxits=20, atol=0.0, rtol=1e-30, history=True,
+                   check=False)
+    OA64 = oracle.Csr(prow, pcol, pval.astype(np.float64))
+    xo, oi = oracle.cg(OA64, b.astype(np.float64), P=oracle.Prec(oracle.PREC_MCSGS, OA64, sizes=sizes),
+                       ctrl=(20, 0.0, 1e-30, 1e6))
+    assert info["iters"] == oi["iters"] == 20
+    assert np.allclose(info["hist"], oi["hist"], rtol=1e-3)
+    assert relnorm(X.numpy(), xo) < 1e-3
 
 
 @pytest.mark.parametrize("case", ["fv12", "upwind20"])
