@@ -497,6 +497,57 @@ def test_mcsgs_fp32_waves_walk_slices(pkg, ctx, oracle):
     assert relnorm(X.numpy(), xo) < 1e-3
 
 
+def test_mcsgs_fp32_empty_slices_walked(pkg, ctx, oracle):
+    """A chain of 2.2 million rows (two colours, about 17,000 slices each, so
+    the FP32 colour steps' waves walk several slices) with 400 isolated rows:
+    first fit puts them in colour 0 as one run, so colour 0's backward step
+    meets whole slices without entries, after which its waves still load
+    their next slice ahead.  Apply and 12 CG steps as in the FV case."""
+    n, lo, hi = 2_200_000, 1000, 1400
+    link = np.ones(n - 1, bool)                    # link i -- i+1
+    link[lo - 1:hi] = False                        # rows lo..hi-1 touch nothing
+    iso = np.zeros(n, bool)
+    iso[lo:hi] = True
+    up = np.append(link, False)                    # row i has i+1
+    dn = np.insert(link, 0, False)                 # row i has i-1
+    cnt = 1 + up.astype(np.int32) + dn.astype(np.int32)
+    row = np.zeros(n + 1, np.int32)
+    row[1:] = np.cumsum(cnt)
+    col = np.empty(row[-1], np.int32)
+    val = np.empty(row[-1])
+    i = np.arange(n)
+    pos = row[:-1].copy()
+    col[pos[dn]] = i[dn] - 1
+    val[pos[dn]] = -1.0
+    pos[dn] += 1
+    col[pos] = i
+    val[pos] = np.where(iso, 1.0, 2.001)
+    pos += 1
+    col[pos[up]] = i[up] + 1
+    val[pos[up]] = -1.0
+    nc, color, sizes, perm = pkg.multicoloring(row, col)
+    # colours are numbered from 1, as the reference's Multicoloring numbers them
+    assert nc == 2 and np.all(color[lo:hi] == 1) and min(sizes) // 64 > 8 * ctx.info()["num_cu"] * 4
+    prow, pcol, pval = pkg.csr_permute(row, col, val, perm)
+    pval = pval.astype(np.float32)
+    b = np.random.default_rng(4).uniform(-1, 1, n).astype(np.float32)
+    A = pkg.CsrMatrix(ctx, prow, pcol, pval)
+    P = pkg.Preconditioner.mcsgs(A, sizes)
+    OA = oracle.Csr(prow, pcol, pval)
+    Z = pkg.Vector(ctx, n, np.float32)
+    P.apply(pkg.Vector(ctx, b, np.float32), Z)
+    assert relnorm(Z.numpy(), oracle.mcsgs(OA, sizes, b)) < 1e-5
+    X = pkg.Vector(ctx, n, np.float32)
+    info = pkg.pcg(A, pkg.Vector(ctx, b, np.float32), X, P, maxits=12, atol=0.0, rtol=1e-30, history=True,
+                   check=False)
+    OA64 = oracle.Csr(prow, pcol, pval.astype(np.float64))
+    xo, oi = oracle.cg(OA64, b.astype(np.float64), P=oracle.Prec(oracle.PREC_MCSGS, OA64, sizes=sizes),
+                       ctrl=(12, 0.0, 1e-30, 1e6))
+    assert info["iters"] == oi["iters"] == 12
+    assert np.allclose(info["hist"], oi["hist"], rtol=1e-3)
+    assert relnorm(X.numpy(), xo) < 1e-3
+
+
 @pytest.mark.parametrize("case", ["fv12", "upwind20"])
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_sgs_natural_apply(pkg, ctx, oracle, dtype, case):
