@@ -136,6 +136,38 @@ __device__ __forceinline__ void sweep_raw(unsigned (&raw)[kSweepU], int k, int L
         raw[q] = (k >> 2) + q < W ? __builtin_nontemporal_load(&code[wbase + ((k >> 2) + q) * 64 + lane]) : ~0u;
 }
 
+// v, taken as unknown until this point: an empty asm that reads and writes
+// it (and clobbers memory, so the loads before it stay before it) moves the
+// wait for the load that produced v here, after those loads have issued.
+template <typename T>
+__device__ __forceinline__ T settled(T v) {
+    asm volatile("" : "+v"(v) : : "memory");
+    return v;
+}
+
+// sweep_raw without branches: every load reads a slot of the slice (the
+// index clamped to its last slot, L > 0) and the slots past its width are
+// replaced afterwards, so a step's loads issue back to back.
+template <bool CODED>
+__device__ __forceinline__ void sweep_raw_flat(unsigned (&raw)[kSweepU], int k, int L, int base, int wbase, int lane,
+                                               int i, int n, const int *__restrict__ scol,
+                                               const unsigned *__restrict__ code) {
+    if (!CODED) {
+#pragma unroll
+        for (int u = 0; u < kSweepU; ++u) {
+            const unsigned t = __builtin_nontemporal_load(&scol[base + min(k + u, L - 1) * 64 + lane]);
+            raw[u] = k + u < L ? t : static_cast<unsigned>(min(i, n - 1));
+        }
+        return;
+    }
+    const int W = (L + 3) >> 2;
+#pragma unroll
+    for (int q = 0; q < kSweepU / 4; ++q) {
+        const unsigned t = __builtin_nontemporal_load(&code[wbase + min((k >> 2) + q, W - 1) * 64 + lane]);
+        raw[q] = (k >> 2) + q < W ? t : ~0u;
+    }
+}
+
 template <bool CODED>
 __device__ __forceinline__ void sweep_decode(const unsigned (&raw)[kSweepU], int (&cj)[kSweepU], bool (&live)[kSweepU],
                                              int k, int len, int i, int n, const int *dict) {
@@ -230,20 +262,31 @@ __global__ __launch_bounds__(kBlock) void k_mcs_fwd(int c, ColorOffsets co, bool
             const int base = sptr[s], L = (sptr[s + 1] - base) >> 6;
             const int wbase = CODED ? wptr[s] : 0;
             const int len = !CODED && valid ? slen[i] : 0;   // coded: padding slots carry kPadCode
-            const T ri = valid ? in[i] : static_cast<T>(0);
-            T chunk = ri * static_cast<T>(-1);
+            // r_i loaded unmasked (rows past the colour read row n-1; their
+            // results are neither stored nor summed) and -r_i formed once the
+            // first step's loads are issued, so no wait for r_i comes first
+            const T ri = in[min(i, n - 1)];
+            T chunk = 0;
             int blk = 0, blk_end = co.off[1];
             T res = 0;
-            for (int k = 0; k < L; k += kSweepU) {
+            // one step of kSweepU slots; the first is peeled off the loop (a
+            // slice of the stencil systems is one step), so its waits are not
+            // merged with those of a loop back edge
+            auto step = [&](int k) {
                 int cj[kSweepU];
                 bool live[kSweepU];
+                unsigned raw[kSweepU];
                 T v[kSweepU], xv[kSweepU];
-                sweep_cols<CODED>(cj, live, k, L, len, base, wbase, lane, i, n, scol, code, dict);
+                // straight-line loads: slots past the slice width re-read its
+                // last slot (not live, not summed)
+                sweep_raw_flat<CODED>(raw, k, L, base, wbase, lane, i, n, scol, code);
 #pragma unroll
-                for (int u = 0; u < kSweepU; ++u) {   // slots past the slice width: not loaded (wave-uniform)
-                    v[u] = k + u < L ? __builtin_nontemporal_load(&sval[base + (k + u) * 64 + lane]) : static_cast<T>(0);
-                    xv[u] = k + u < L ? out[cj[u]] : static_cast<T>(0);
-                }
+                for (int u = 0; u < kSweepU; ++u)
+                    v[u] = __builtin_nontemporal_load(&sval[base + min(k + u, L - 1) * 64 + lane]);
+                sweep_decode<CODED>(raw, cj, live, k, len, i, n, dict);
+#pragma unroll
+                for (int u = 0; u < kSweepU; ++u) xv[u] = out[cj[u]];
+                if (k == 0) chunk = settled(ri) * static_cast<T>(-1);
 #pragma unroll
                 for (int u = 0; u < kSweepU; ++u) {
                     if (!live[u]) continue;
@@ -255,6 +298,12 @@ __global__ __launch_bounds__(kBlock) void k_mcs_fwd(int c, ColorOffsets co, bool
                     }
                     res += v[u] * xv[u];
                 }
+            };
+            if (L > 0) {
+                step(0);
+                for (int k = kSweepU; k < L; k += kSweepU) step(k);
+            } else {
+                chunk = ri * static_cast<T>(-1);   // a slice without entries runs no step
             }
             if (valid) {
                 chunk += res;
@@ -357,19 +406,29 @@ __global__ __launch_bounds__(kBlock) void k_mcs_bwd(int c, int ncolors, ColorOff
             const int base = sptr[s], L = (sptr[s + 1] - base) >> 6;
             const int wbase = CODED ? wptr[s] : 0;
             const int len = !CODED && valid ? slen[i] : 0;   // coded: padding slots carry kPadCode
-            T chunk = valid ? out[i] * dv[i] : static_cast<T>(0);
+            // z_i and d_i loaded unmasked (rows past the colour read row n-1;
+            // their results are not stored) and z_i d_i formed once the first
+            // step's loads are issued
+            const T zi0 = out[min(i, n - 1)], di = dv[min(i, n - 1)];
+            T chunk = 0;
             int blk = ncolors - 1, blk_lo = co.off[ncolors - 1];
             T res = 0;
-            for (int k = 0; k < L; k += kSweepU) {
+            // the first step peeled off the loop, as in k_mcs_fwd
+            auto step = [&](int k) {
                 int cj[kSweepU];
                 bool live[kSweepU];
+                unsigned raw[kSweepU];
                 T v[kSweepU], xv[kSweepU];
-                sweep_cols<CODED>(cj, live, k, L, len, base, wbase, lane, i, n, scol, code, dict);
+                // straight-line loads: slots past the slice width re-read its
+                // last slot (not live, not summed)
+                sweep_raw_flat<CODED>(raw, k, L, base, wbase, lane, i, n, scol, code);
 #pragma unroll
-                for (int u = 0; u < kSweepU; ++u) {   // slots past the slice width: not loaded (wave-uniform)
-                    v[u] = k + u < L ? __builtin_nontemporal_load(&sval[base + (k + u) * 64 + lane]) : static_cast<T>(0);
-                    xv[u] = k + u < L ? out[cj[u]] : static_cast<T>(0);
-                }
+                for (int u = 0; u < kSweepU; ++u)
+                    v[u] = __builtin_nontemporal_load(&sval[base + min(k + u, L - 1) * 64 + lane]);
+                sweep_decode<CODED>(raw, cj, live, k, len, i, n, dict);
+#pragma unroll
+                for (int u = 0; u < kSweepU; ++u) xv[u] = out[cj[u]];
+                if (k == 0) chunk = settled(zi0) * settled(di);
 #pragma unroll
                 for (int u = 0; u < kSweepU; ++u) {
                     if (!live[u]) continue;
@@ -381,6 +440,12 @@ __global__ __launch_bounds__(kBlock) void k_mcs_bwd(int c, int ncolors, ColorOff
                     }
                     res += v[u] * xv[u];
                 }
+            };
+            if (L > 0) {
+                step(0);
+                for (int k = kSweepU; k < L; k += kSweepU) step(k);
+            } else {
+                chunk = zi0 * di;   // a slice without entries runs no step
             }
             if (valid) {
                 chunk += res;
